@@ -1,0 +1,100 @@
+// Device data layout and kernel launchers of the 3x3-block Krylov solver.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace hecmw33 {
+
+constexpr int kWave = 64;       // CDNA wavefront
+constexpr int kWavesPerBlock = 4;
+constexpr int kBlock = kWave * kWavesPerBlock;
+constexpr int kNumXcd = 8;      // MI355X: 8 XCDs, each with its own L2
+
+// Block rows in SELL-64 form: rows are grouped 64 to a slice (one slice per
+// wavefront, one row per lane).  Slice s holds width(s) = ptr[s+1]-ptr[s] block
+// slots per lane; slot k of lane l keeps its column in col[(ptr[s]+k)*64 + l]
+// and its 3x3 values in 9 planes val[((ptr[s]+k)*9 + e)*64 + l], e = 3*row+col
+// of the block.  A load of one plane by a wavefront is one contiguous
+// 512-byte (FP64) or 256-byte (FP32) segment.
+template <typename T>
+struct Sell {
+  int32_t nslices = 0;
+  int32_t nrows = 0;
+  int64_t nslots = 0;          // sum of widths * 64
+  int32_t* ptr = nullptr;      // [nslices+1]
+  int32_t* len = nullptr;      // [nslices*64] blocks in the lane's row (0 = idle lane)
+  int32_t* col = nullptr;      // [nslots] 0-based node id of the block column
+  T* val = nullptr;            // [9*nslots]
+};
+
+// Device-resident scalars of one Krylov solve; kernels read and update them so
+// the host never waits inside an iteration.  Only the last workgroup of a
+// reduction kernel writes them, after every other workgroup has finished.
+struct KrylovState {
+  double bnrm2, tol;
+  double rho, rho1, alpha, beta, c1, dnrm2, resid;
+  double omega, c2, ts, tt;
+  int32_t iter;      // 1-based iteration being executed
+  int32_t maxit;
+  int32_t done;      // 1: the loop has exited, every later kernel returns at once
+  int32_t error;
+  int32_t n_indef;
+  int32_t recheck;   // recompute the true residual in this iteration
+  int32_t nrec;      // true-residual period (50 CG, 100 BiCGSTAB)
+  int32_t pad;
+  uint32_t ticket[4];
+};
+
+constexpr int kMaxPartials = 65536;  // per reduction component
+
+// ---- setup ---------------------------------------------------------------------
+// Fill SELL values from the caller's D/AL/AU through a slot->source map
+// (src >= 0: block index into [D | AL | AU], -1: padding).
+void launch_fill_values(const Sell<double>& a, const int64_t* src, const double* D, const double* AL,
+                        const double* AU, int64_t nD, int64_t nAL, hipStream_t s);
+void launch_fill_values_f32(const Sell<float>& a, const int64_t* src, const double* D, const double* AL,
+                            const double* AU, int64_t nD, int64_t nAL, hipStream_t s);
+// 3x3 no-pivot LU with sigma on the pivots of D[row_src[t]] into the 9-plane
+// slice layout (hecmw_precond_SSOR_33.F90:164-208, hecmw_precond_DIAG_33.F90:58-91)
+void launch_block_lu(int32_t nslots, const int32_t* row_src, const double* D, double sigma, double* alu,
+                     hipStream_t s);
+void launch_block_lu_f32(int32_t nslots, const int32_t* row_src, const double* D, double sigma, float* alu,
+                         hipStream_t s);
+
+// ---- plain operators -------------------------------------------------------------
+void launch_spmv(const Sell<double>& a, const double* x, double* y, hipStream_t s);
+void launch_spmv_f32(const Sell<float>& a, const float* x, float* y, hipStream_t s);
+void launch_matresid(const Sell<double>& a, const double* x, const double* b, double* r, hipStream_t s);
+void launch_dot_store(int64_t n, const double* x, const double* y, double* out, KrylovState* st, double* part,
+                      hipStream_t s);
+void launch_ssor_fwd(const Sell<double>& L, const int32_t* row_old, const double* alu, int32_t s0, int32_t ns,
+                     const double* rin, double* z, const KrylovState* st, hipStream_t s);
+void launch_ssor_bwd(const Sell<double>& U, const int32_t* row_old, const double* alu, int32_t s0, int32_t ns,
+                     double* z, const KrylovState* st, hipStream_t s);
+void launch_ssor_fwd_f32(const Sell<float>& L, const int32_t* row_old, const float* alu, int32_t s0, int32_t ns,
+                         const float* rin, float* z, hipStream_t s);
+void launch_ssor_bwd_f32(const Sell<float>& U, const int32_t* row_old, const float* alu, int32_t s0, int32_t ns,
+                         float* z, hipStream_t s);
+void launch_diag_apply(int32_t n, const double* alu, const double* r, double* z, const KrylovState* st,
+                       hipStream_t s);
+
+// ---- fused CG iteration pieces (hecmw_solver_CG.f90:164-279) ----------------------
+// z = M^-1 r with rho = r.z and the rho/beta logic
+void cg_precond_ssor_rho(const Sell<double>& L, const Sell<double>& U, const int32_t* row_old, const double* alu,
+                         const int32_t* color_slice, int32_t ncolor, const double* r, double* z, KrylovState* st,
+                         double* part, hipStream_t s);
+void cg_precond_diag_rho(int32_t n, const double* alu, const double* r, double* z, KrylovState* st, double* part,
+                         hipStream_t s);
+void cg_precond_none_rho(int64_t n3, const double* r, double* z, KrylovState* st, double* part, hipStream_t s);
+// p = z | z + beta p
+void cg_xpay(int64_t n3, const double* z, double* p, const KrylovState* st, hipStream_t s);
+// q = A p, c1 = p.q, alpha
+void cg_spmv_dot(const Sell<double>& a, const double* p, double* q, KrylovState* st, double* part, hipStream_t s);
+// x += alpha p, r -= alpha q, |r|^2
+void cg_update(int64_t n3, const double* p, const double* q, double* x, double* r, KrylovState* st, double* part,
+               double* hist, hipStream_t s);
+// conditional r = b - A x with |r|^2, convergence, end of iteration
+void cg_true_resid(const Sell<double>& a, const double* x, const double* b, double* r, KrylovState* st, double* part,
+                   double* hist, hipStream_t s);
+
+}  // namespace hecmw33

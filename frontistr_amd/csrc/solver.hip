@@ -1,0 +1,855 @@
+// Host runtime of libhecmw33: device context, storage build, preconditioner
+// setup and the Krylov drivers behind the C ABI of include/hecmw33.h.
+//
+// Layout of one context in HBM (FP64 path):
+//   A      : full matrix, natural row order, SELL-64, D block first per row
+//            then L then U in the caller's storage order (the reference CPU
+//            SpMV order, hecmw_solver_las_33.F90:281-308)
+//   raw    : the caller's D/AL/AU uploaded as-is (source of every value fill)
+//   L', U' : SSOR factors in multicolour order, SELL-64 slices never crossing
+//            a colour; columns hold old (natural) node ids, as after
+//            hecmw_matrix_reorder_renum_item, so the sweeps gather z in the
+//            natural order like the reference (hecmw_precond_SSOR_33.F90:151-155)
+//   ALU    : 3x3 LU of the diagonal blocks, 9-plane slice layout
+//   vectors: b, x, r, z, p, q (+ BiCGSTAB's) of 3*NP doubles
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/hecmw33.h"
+#include "device.h"
+#include "ordering.h"
+
+namespace hecmw33 {
+
+static thread_local std::string g_last_error;
+
+static void set_error(const std::string& s) { g_last_error = s; }
+
+#define HIPC(expr)                                                                          \
+  do {                                                                                      \
+    hipError_t _e = (expr);                                                                 \
+    if (_e != hipSuccess) {                                                                 \
+      set_error(std::string(#expr) + ": " + hipGetErrorString(_e));                         \
+      return HECMW33_ERROR_DEVICE;                                                          \
+    }                                                                                       \
+  } while (0)
+
+static double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Iarray / Rarray slots (1-based in the reference, hecmw_matrix_misc.F90:107-150)
+enum {
+  I_ITER = 1, I_METHOD = 2, I_PRECOND = 3, I_ITERPREMAX = 5, I_SCALING = 7, I_METHOD2 = 8,
+  I_ITERLOG = 21, I_TIMELOG = 22, I_NCOLOR_IN = 34, I_MAXRECYCLE = 35, I_FLAG_CONVERGED = 81,
+  I_FLAG_DIVERGED = 82, I_NRECYCLE = 96, I_FLAG_NUMFACT = 97, I_FLAG_SYMBFACT = 98,
+  R_RESID = 1, R_SIGMA_DIAG = 2,
+};
+static inline int32_t& IA(int32_t* a, int k) { return a[k - 1]; }
+static inline int32_t IA(const int32_t* a, int k) { return a[k - 1]; }
+static inline double RA(const double* a, int k) { return a[k - 1]; }
+
+// ---------------------------------------------------------------------------
+// device memory helper
+struct DMem {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int32_t ensure(size_t n) {
+    if (n <= bytes && p) return 0;
+    if (p) hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (n == 0) n = 8;
+    HIPC(hipMalloc(&p, n));
+    bytes = n;
+    return 0;
+  }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+struct SellHost {
+  int32_t nslices = 0, nrows = 0;
+  std::vector<int32_t> ptr, len, col;
+  std::vector<int64_t> src;
+};
+
+struct SellDev {
+  DMem ptr, len, col, src, val, val32;
+  Sell<double> view;
+  Sell<float> view32;
+};
+
+// Build SELL-64 storage for rows given lane by lane (lane_rows[s*64+l] = row
+// key or -1).  count(key) = blocks in that row; fill(key, emit) calls
+// emit(k, col, src) for k = 0..count-1 in storage order.
+template <class Count, class Fill>
+static void build_sell(const std::vector<int32_t>& lane_rows, int32_t nrows, Count count, Fill fill, SellHost& h) {
+  const int32_t ns = (int32_t)(lane_rows.size() / kWave);
+  h.nslices = ns;
+  h.nrows = nrows;
+  h.len.assign(lane_rows.size(), 0);
+  h.ptr.assign(ns + 1, 0);
+#pragma omp parallel for schedule(static)
+  for (int64_t t = 0; t < (int64_t)lane_rows.size(); ++t) h.len[t] = lane_rows[t] >= 0 ? count(lane_rows[t]) : 0;
+  for (int32_t s = 0; s < ns; ++s) {
+    int32_t w = 0;
+    for (int l = 0; l < kWave; ++l) w = std::max(w, h.len[(int64_t)s * kWave + l]);
+    h.ptr[s + 1] = h.ptr[s] + w;
+  }
+  const int64_t nslots = (int64_t)h.ptr[ns] * kWave;
+  h.col.assign(nslots, 0);
+  h.src.assign(nslots, -1);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int32_t s = 0; s < ns; ++s)
+    for (int l = 0; l < kWave; ++l) {
+      const int32_t key = lane_rows[(int64_t)s * kWave + l];
+      if (key < 0) continue;
+      const int64_t base = (int64_t)h.ptr[s] * kWave + l;
+      fill(key, [&](int32_t k, int32_t c, int64_t sidx) {
+        h.col[base + (int64_t)k * kWave] = c;
+        h.src[base + (int64_t)k * kWave] = sidx;
+      });
+    }
+}
+
+static int32_t upload_sell(const SellHost& h, SellDev& d, hipStream_t s) {
+  const int64_t nslots = (int64_t)h.ptr[h.nslices] * kWave;
+  if (d.ptr.ensure(sizeof(int32_t) * (h.nslices + 1)) || d.len.ensure(sizeof(int32_t) * h.len.size()) ||
+      d.col.ensure(sizeof(int32_t) * std::max<int64_t>(nslots, 1)) ||
+      d.src.ensure(sizeof(int64_t) * std::max<int64_t>(nslots, 1)) ||
+      d.val.ensure(sizeof(double) * 9 * std::max<int64_t>(nslots, 1)))
+    return HECMW33_ERROR_DEVICE;
+  HIPC(hipMemcpyAsync(d.ptr.p, h.ptr.data(), sizeof(int32_t) * (h.nslices + 1), hipMemcpyHostToDevice, s));
+  HIPC(hipMemcpyAsync(d.len.p, h.len.data(), sizeof(int32_t) * h.len.size(), hipMemcpyHostToDevice, s));
+  if (nslots > 0) {
+    HIPC(hipMemcpyAsync(d.col.p, h.col.data(), sizeof(int32_t) * nslots, hipMemcpyHostToDevice, s));
+    HIPC(hipMemcpyAsync(d.src.p, h.src.data(), sizeof(int64_t) * nslots, hipMemcpyHostToDevice, s));
+  }
+  d.view.nslices = h.nslices;
+  d.view.nrows = h.nrows;
+  d.view.nslots = nslots;
+  d.view.ptr = d.ptr.as<int32_t>();
+  d.view.len = d.len.as<int32_t>();
+  d.view.col = d.col.as<int32_t>();
+  d.view.val = d.val.as<double>();
+  d.view32.nslices = h.nslices;
+  d.view32.nrows = h.nrows;
+  d.view32.nslots = nslots;
+  d.view32.ptr = d.view.ptr;
+  d.view32.len = d.view.len;
+  d.view32.col = d.view.col;
+  d.view32.val = nullptr;
+  HIPC(hipStreamSynchronize(s));  // host vectors may be released by the caller
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+struct Context {
+  hipStream_t stream = nullptr;
+  // structure key
+  int32_t N = -1, NP = -1, NPL = -1, NPU = -1;
+  const void* key[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool have_struct = false;
+  // raw arrays and full matrix
+  DMem dD, dAL, dAU;
+  SellDev A;
+  // host copy of the structure for preconditioner builds
+  std::vector<int32_t> h_indexL, h_indexU, h_itemL, h_itemU;
+  // SSOR
+  bool ssor_init = false;
+  int32_t ncolor = 0;
+  std::vector<int32_t> perm, iperm, colorindex, color_slice;
+  SellDev L, U;
+  DMem row_old, alu_ssor;
+  // DIAG
+  bool diag_init = false;
+  DMem row_diag, alu_diag;
+  // Krylov vectors and state
+  DMem vb, vx, vr, vz, vp, vq, vrt, vs, vt, vv, vpt, vst;
+  DMem st, part, hist;
+  KrylovState* h_st = nullptr;  // pinned, 2 slots for polling
+  int32_t last_iter = 0;
+
+  ~Context() {
+    if (stream) hipStreamSynchronize(stream);
+    for (DMem* m : {&dD, &dAL, &dAU, &row_old, &alu_ssor, &row_diag, &alu_diag, &vb, &vx, &vr, &vz, &vp, &vq, &vrt,
+                    &vs, &vt, &vv, &vpt, &vst, &st, &part, &hist})
+      m->release();
+    for (SellDev* s : {&A, &L, &U})
+      for (DMem* m : {&s->ptr, &s->len, &s->col, &s->src, &s->val, &s->val32}) m->release();
+    if (h_st) hipHostFree(h_st);
+    if (stream) hipStreamDestroy(stream);
+  }
+};
+
+static int32_t ctx_init(Context& c) {
+  if (c.stream) return 0;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
+    set_error("no HIP device visible");
+    return HECMW33_ERROR_DEVICE;
+  }
+  HIPC(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  HIPC(hipHostMalloc((void**)&c.h_st, 2 * sizeof(KrylovState), hipHostMallocDefault));
+  if (c.st.ensure(sizeof(KrylovState)) || c.part.ensure(sizeof(double) * 4 * kMaxPartials))
+    return HECMW33_ERROR_DEVICE;
+  HIPC(hipMemsetAsync(c.st.p, 0, sizeof(KrylovState), c.stream));
+  HIPC(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+static int32_t check_sys(const hecmw33_sys* A) {
+  if (!A || A->N < 1 || A->NP < A->N || A->NPL < 0 || A->NPU < 0 || !A->D || !A->indexL || !A->indexU ||
+      (A->NPL > 0 && (!A->AL || !A->itemL)) || (A->NPU > 0 && (!A->AU || !A->itemU))) {
+    set_error("invalid hecmw33_sys");
+    return HECMW33_ERROR_ARGS;
+  }
+  if (A->indexL[0] != 0 || A->indexU[0] != 0 || A->indexL[A->NP] != A->NPL || A->indexU[A->NP] != A->NPU) {
+    set_error("index arrays inconsistent with NPL/NPU");
+    return HECMW33_ERROR_ARGS;
+  }
+  return 0;
+}
+
+// Structure of A (rebuilt when sizes, array identities or the symbolic flag change).
+static int32_t load_structure(Context& c, const hecmw33_sys* A, bool force) {
+  const bool same = c.have_struct && c.N == A->N && c.NP == A->NP && c.NPL == A->NPL && c.NPU == A->NPU &&
+                    c.key[0] == A->indexL && c.key[1] == A->indexU && c.key[2] == A->itemL &&
+                    c.key[3] == A->itemU;
+  if (same && !force) return 0;
+  const int32_t N = A->N, NP = A->NP;
+  c.h_indexL.assign(A->indexL, A->indexL + NP + 1);
+  c.h_indexU.assign(A->indexU, A->indexU + NP + 1);
+  c.h_itemL.assign(A->itemL, A->itemL + A->NPL);
+  c.h_itemU.assign(A->itemU, A->itemU + A->NPU);
+  for (int64_t q = 0; q < A->NPL; ++q)
+    if (A->itemL[q] < 1 || A->itemL[q] > NP) {
+      set_error("itemL out of range");
+      return HECMW33_ERROR_ARGS;
+    }
+  for (int64_t q = 0; q < A->NPU; ++q)
+    if (A->itemU[q] < 1 || A->itemU[q] > NP) {
+      set_error("itemU out of range");
+      return HECMW33_ERROR_ARGS;
+    }
+  const int32_t ns = (N + kWave - 1) / kWave;
+  std::vector<int32_t> lanes((int64_t)ns * kWave, -1);
+  for (int32_t i = 0; i < N; ++i) lanes[i] = i;
+  const int32_t* iL = A->indexL;
+  const int32_t* iU = A->indexU;
+  const int32_t* tL = A->itemL;
+  const int32_t* tU = A->itemU;
+  const int64_t offL = N, offU = (int64_t)N + A->NPL;
+  SellHost h;
+  build_sell(
+      lanes, N, [&](int32_t i) { return 1 + (iL[i + 1] - iL[i]) + (iU[i + 1] - iU[i]); },
+      [&](int32_t i, auto emit) {
+        int32_t k = 0;
+        emit(k++, i, (int64_t)i);
+        for (int32_t q = iL[i]; q < iL[i + 1]; ++q) emit(k++, tL[q] - 1, offL + q);
+        for (int32_t q = iU[i]; q < iU[i + 1]; ++q) emit(k++, tU[q] - 1, offU + q);
+      },
+      h);
+  if (int32_t e = upload_sell(h, c.A, c.stream)) return e;
+  c.N = N;
+  c.NP = NP;
+  c.NPL = A->NPL;
+  c.NPU = A->NPU;
+  c.key[0] = A->indexL;
+  c.key[1] = A->indexU;
+  c.key[2] = A->itemL;
+  c.key[3] = A->itemU;
+  c.have_struct = true;
+  c.ssor_init = false;
+  c.diag_init = false;
+  const size_t vb = sizeof(double) * 3 * (size_t)NP;
+  for (DMem* m : {&c.vb, &c.vx, &c.vr, &c.vz, &c.vp, &c.vq})
+    if (m->ensure(vb)) return HECMW33_ERROR_DEVICE;
+  return 0;
+}
+
+// Values: upload D/AL/AU and refill the SpMV storage.
+static int32_t load_values(Context& c, const hecmw33_sys* A) {
+  if (c.dD.ensure(sizeof(double) * 9 * (size_t)A->NP) || c.dAL.ensure(sizeof(double) * 9 * (size_t)A->NPL) ||
+      c.dAU.ensure(sizeof(double) * 9 * (size_t)A->NPU))
+    return HECMW33_ERROR_DEVICE;
+  HIPC(hipMemcpyAsync(c.dD.p, A->D, sizeof(double) * 9 * (size_t)A->NP, hipMemcpyHostToDevice, c.stream));
+  if (A->NPL) HIPC(hipMemcpyAsync(c.dAL.p, A->AL, sizeof(double) * 9 * (size_t)A->NPL, hipMemcpyHostToDevice, c.stream));
+  if (A->NPU) HIPC(hipMemcpyAsync(c.dAU.p, A->AU, sizeof(double) * 9 * (size_t)A->NPU, hipMemcpyHostToDevice, c.stream));
+  launch_fill_values(c.A.view, c.A.src.as<int64_t>(), c.dD.as<double>(), c.dAL.as<double>(), c.dAU.as<double>(),
+                     c.N, c.NPL, c.stream);
+  HIPC(hipGetLastError());
+  return 0;
+}
+
+// ---- preconditioners ----------------------------------------------------------
+static int32_t setup_ssor(Context& c, double sigma, int32_t ncolor_in) {
+  const int32_t N = c.N;
+  const int32_t* iL = c.h_indexL.data();
+  const int32_t* iU = c.h_indexU.data();
+  const int32_t* tL = c.h_itemL.data();
+  const int32_t* tU = c.h_itemU.data();
+  std::vector<int32_t> perm_rcm(N), iperm_rcm(N);
+  ordering_rcm(N, iL, tL, iU, tU, perm_rcm.data(), iperm_rcm.data());
+  c.perm.assign(N, 0);
+  c.iperm.assign(N, 0);
+  c.colorindex.assign(N + 1, 0);
+  c.ncolor = ordering_mc(N, iL, tL, iU, tU, perm_rcm.data(), ncolor_in, c.colorindex.data(), c.perm.data(),
+                         c.iperm.data());
+  std::vector<int32_t> iLp, tLp, iUp, tUp;
+  reorder_profile(N, c.perm.data(), c.iperm.data(), iL, tL, iU, tU, iLp, tLp, iUp, tUp);
+  // slices per colour
+  c.color_slice.assign(c.ncolor + 1, 0);
+  for (int32_t k = 0; k < c.ncolor; ++k) {
+    const int32_t cnt = c.colorindex[k + 1] - c.colorindex[k];
+    c.color_slice[k + 1] = c.color_slice[k] + (cnt + kWave - 1) / kWave;
+  }
+  const int32_t ns = c.color_slice[c.ncolor];
+  std::vector<int32_t> lanes((int64_t)ns * kWave, -1), row_old((int64_t)ns * kWave, -1);
+  for (int32_t k = 0; k < c.ncolor; ++k)
+    for (int32_t inew = c.colorindex[k]; inew < c.colorindex[k + 1]; ++inew) {
+      const int64_t t = (int64_t)c.color_slice[k] * kWave + (inew - c.colorindex[k]);
+      lanes[t] = inew;                    // 0-based new position
+      row_old[t] = c.perm[inew] - 1;      // 0-based old id
+    }
+  const int64_t offL = N, offU = (int64_t)N + c.NPL;
+  const std::vector<int32_t>& perm = c.perm;
+  // source block of A(iold, kold) inside the caller's AL/AU (row iold)
+  auto find_src = [&](int32_t iold1, int32_t kold1) -> int64_t {
+    if (kold1 < iold1) {
+      for (int32_t q = iL[iold1 - 1]; q < iL[iold1]; ++q)
+        if (tL[q] == kold1) return offL + q;
+    } else {
+      for (int32_t q = iU[iold1 - 1]; q < iU[iold1]; ++q)
+        if (tU[q] == kold1) return offU + q;
+    }
+    return -1;
+  };
+  SellHost hl, hu;
+  build_sell(
+      lanes, N, [&](int32_t inew) { return iLp[inew + 1] - iLp[inew]; },
+      [&](int32_t inew, auto emit) {
+        const int32_t iold1 = perm[inew];
+        int32_t k = 0;
+        for (int32_t q = iLp[inew]; q < iLp[inew + 1]; ++q) {
+          const int32_t kold1 = perm[tLp[q] - 1];
+          emit(k++, kold1 - 1, find_src(iold1, kold1));
+        }
+      },
+      hl);
+  build_sell(
+      lanes, N, [&](int32_t inew) { return iUp[inew + 1] - iUp[inew]; },
+      [&](int32_t inew, auto emit) {
+        const int32_t iold1 = perm[inew];
+        int32_t k = 0;
+        for (int32_t q = iUp[inew]; q < iUp[inew + 1]; ++q) {
+          const int32_t kold1 = perm[tUp[q] - 1];
+          emit(k++, kold1 - 1, find_src(iold1, kold1));
+        }
+      },
+      hu);
+  if (int32_t e = upload_sell(hl, c.L, c.stream)) return e;
+  if (int32_t e = upload_sell(hu, c.U, c.stream)) return e;
+  if (c.row_old.ensure(sizeof(int32_t) * row_old.size()) || c.alu_ssor.ensure(sizeof(double) * 9 * row_old.size()))
+    return HECMW33_ERROR_DEVICE;
+  HIPC(hipMemcpyAsync(c.row_old.p, row_old.data(), sizeof(int32_t) * row_old.size(), hipMemcpyHostToDevice,
+                      c.stream));
+  launch_fill_values(c.L.view, c.L.src.as<int64_t>(), c.dD.as<double>(), c.dAL.as<double>(), c.dAU.as<double>(), N,
+                     c.NPL, c.stream);
+  launch_fill_values(c.U.view, c.U.src.as<int64_t>(), c.dD.as<double>(), c.dAL.as<double>(), c.dAU.as<double>(), N,
+                     c.NPL, c.stream);
+  launch_block_lu((int32_t)row_old.size(), c.row_old.as<int32_t>(), c.dD.as<double>(), sigma,
+                  c.alu_ssor.as<double>(), c.stream);
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(c.stream));
+  c.ssor_init = true;
+  return 0;
+}
+
+static int32_t setup_diag(Context& c, double sigma) {
+  const int32_t ns = (c.N + kWave - 1) / kWave;
+  std::vector<int32_t> rows((int64_t)ns * kWave, -1);
+  for (int32_t i = 0; i < c.N; ++i) rows[i] = i;
+  if (c.row_diag.ensure(sizeof(int32_t) * rows.size()) || c.alu_diag.ensure(sizeof(double) * 9 * rows.size()))
+    return HECMW33_ERROR_DEVICE;
+  HIPC(hipMemcpyAsync(c.row_diag.p, rows.data(), sizeof(int32_t) * rows.size(), hipMemcpyHostToDevice, c.stream));
+  launch_block_lu((int32_t)rows.size(), c.row_diag.as<int32_t>(), c.dD.as<double>(), sigma, c.alu_diag.as<double>(),
+                  c.stream);
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(c.stream));
+  c.diag_init = true;
+  return 0;
+}
+
+// hecmw_precond_setup (hecmw_precond.f90:30-58) with the SSOR/DIAG
+// reuse rule (hecmw_precond_SSOR_33.F90:74-82, hecmw_precond_DIAG_33.F90:38-41).
+static int32_t precond_setup(Context& c, int32_t* iarray, const double* rarray) {
+  if (IA(iarray, I_ITERPREMAX) <= 0) return 0;
+  const int32_t pc = IA(iarray, I_PRECOND);
+  const double sigma = RA(rarray, R_SIGMA_DIAG);
+  const bool want = IA(iarray, I_FLAG_SYMBFACT) == 1 || IA(iarray, I_FLAG_NUMFACT) == 1;
+  if (pc == 1 || pc == 2) {
+    if (c.ssor_init && !want) return 0;
+    if (int32_t e = setup_ssor(c, sigma, IA(iarray, I_NCOLOR_IN))) return e;
+  } else if (pc == 3) {
+    if (c.diag_init && IA(iarray, I_FLAG_SYMBFACT) == 0 && IA(iarray, I_FLAG_NUMFACT) == 0) return 0;
+    if (int32_t e = setup_diag(c, sigma)) return e;
+  } else {
+    set_error("PRECOND not provided by the GPU library (1, 2: SSOR; 3: DIAG)");
+    return HECMW33_ERROR_INCONS_PC;
+  }
+  IA(iarray, I_FLAG_SYMBFACT) = 0;
+  IA(iarray, I_FLAG_NUMFACT) = 0;
+  return 0;
+}
+
+// ---- CG ---------------------------------------------------------------------
+static void enqueue_cg_iteration(Context& c, int32_t pc, int32_t iterpremax) {
+  const int64_t n3 = 3 * (int64_t)c.N;
+  KrylovState* st = c.st.as<KrylovState>();
+  double* part = c.part.as<double>();
+  double* r = c.vr.as<double>();
+  double* z = c.vz.as<double>();
+  if (iterpremax <= 0) {
+    cg_precond_none_rho(n3, r, z, st, part, c.stream);
+  } else if (pc == 3) {
+    cg_precond_diag_rho(c.N, c.alu_diag.as<double>(), r, z, st, part, c.stream);
+  } else {
+    cg_precond_ssor_rho(c.L.view, c.U.view, c.row_old.as<int32_t>(), c.alu_ssor.as<double>(), c.color_slice.data(),
+                        c.ncolor, r, z, st, part, c.stream);
+  }
+  cg_xpay(n3, z, c.vp.as<double>(), st, c.stream);
+  cg_spmv_dot(c.A.view, c.vp.as<double>(), c.vq.as<double>(), st, part, c.stream);
+  cg_update(n3, c.vp.as<double>(), c.vq.as<double>(), c.vx.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
+  cg_true_resid(c.A.view, c.vx.as<double>(), c.vb.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
+}
+
+// Krylov solve on the context's vectors vb (b) and vx (x0 in, x out).
+static int32_t krylov_resident(Context& c, int32_t* iarray, double* rarray, hecmw33_out* out) {
+  const double t0 = now_s();
+  const int32_t method = IA(iarray, I_METHOD);
+  const int32_t pc = IA(iarray, I_PRECOND);
+  const int32_t iterpremax = IA(iarray, I_ITERPREMAX);
+  const int32_t maxit = IA(iarray, I_ITER);
+  const double tol = RA(rarray, R_RESID);
+  if (method != 1) {
+    set_error("METHOD not provided by the GPU library yet (1: CG)");
+    return HECMW33_ERROR_UNSUPPORTED;
+  }
+  if (iterpremax > 1) {
+    set_error("ITERPREMAX > 1 not provided by the GPU library yet");
+    return HECMW33_ERROR_UNSUPPORTED;
+  }
+  if (int32_t e = precond_setup(c, iarray, rarray)) return e;
+  const int64_t n3 = 3 * (int64_t)c.N;
+  KrylovState* st = c.st.as<KrylovState>();
+  double* part = c.part.as<double>();
+  if (c.hist.ensure(sizeof(double) * std::max(1, maxit))) return HECMW33_ERROR_DEVICE;
+  // r0 = b - A x0; bnrm2 = b.b  (hecmw_solver_CG.f90:131-141)
+  launch_matresid(c.A.view, c.vx.as<double>(), c.vb.as<double>(), c.vr.as<double>(), c.stream);
+  launch_dot_store(n3, c.vb.as<double>(), c.vb.as<double>(), &st->bnrm2, st, part, c.stream);
+  // p starts at zero (WW = 0), including the external entries the SpMV reads
+  HIPC(hipMemsetAsync(c.vp.p, 0, sizeof(double) * 3 * (size_t)c.NP, c.stream));
+  HIPC(hipGetLastError());
+  KrylovState h{};
+  HIPC(hipMemcpyAsync(&c.h_st[0], st, sizeof(KrylovState), hipMemcpyDeviceToHost, c.stream));
+  HIPC(hipStreamSynchronize(c.stream));
+  const double bnrm2 = c.h_st[0].bnrm2;
+  out->method = method;
+  out->ncolor = (iterpremax > 0 && (pc == 1 || pc == 2)) ? c.ncolor : 0;
+  if (bnrm2 == 0.0 || maxit < 1) {
+    // zero-trip DO loop: ITER reads 1 (hecmw_solver_CG.f90:135-141,164)
+    if (bnrm2 == 0.0) HIPC(hipMemsetAsync(c.vx.p, 0, sizeof(double) * 3 * (size_t)c.NP, c.stream));
+    HIPC(hipStreamSynchronize(c.stream));
+    out->iter = 1;
+    out->error = 0;
+    out->resid = bnrm2 == 0.0 ? 0.0 : tol;
+    out->Tset = now_s() - t0;
+    out->Tsol = 0.0;
+    c.last_iter = 0;
+    return 0;
+  }
+  h = c.h_st[0];
+  h.tol = tol;
+  h.rho = h.rho1 = h.alpha = h.beta = h.c1 = h.dnrm2 = h.resid = 0.0;
+  h.omega = h.c2 = h.ts = h.tt = 0.0;
+  h.iter = 1;
+  h.maxit = maxit;
+  h.done = 0;
+  h.error = 0;
+  h.n_indef = 0;
+  h.recheck = 0;
+  h.nrec = 50;
+  for (auto& t : h.ticket) t = 0;
+  c.h_st[1] = h;
+  HIPC(hipMemcpyAsync(st, &c.h_st[1], sizeof(KrylovState), hipMemcpyHostToDevice, c.stream));
+  HIPC(hipStreamSynchronize(c.stream));
+  const double t1 = now_s();
+  out->Tset = t1 - t0;
+  // iterations: enqueue in chunks, poll the previous chunk's state while the
+  // current one runs; kernels after the exit return immediately
+  const int chunk = 8;
+  hipEvent_t ev[2];
+  HIPC(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  HIPC(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  int32_t enq = 0, nchunk = 0;
+  bool finished = false;
+  while (enq < maxit && !finished) {
+    const int k = std::min(chunk, maxit - enq);
+    for (int i = 0; i < k; ++i) enqueue_cg_iteration(c, pc, iterpremax);
+    enq += k;
+    const int slot = nchunk & 1;
+    HIPC(hipMemcpyAsync(&c.h_st[slot], st, sizeof(KrylovState), hipMemcpyDeviceToHost, c.stream));
+    HIPC(hipEventRecord(ev[slot], c.stream));
+    if (nchunk > 0) {
+      HIPC(hipEventSynchronize(ev[slot ^ 1]));
+      if (c.h_st[slot ^ 1].done) finished = true;
+    }
+    ++nchunk;
+  }
+  HIPC(hipGetLastError());
+  HIPC(hipMemcpyAsync(&c.h_st[0], st, sizeof(KrylovState), hipMemcpyDeviceToHost, c.stream));
+  HIPC(hipStreamSynchronize(c.stream));
+  hipEventDestroy(ev[0]);
+  hipEventDestroy(ev[1]);
+  h = c.h_st[0];
+  out->Tsol = now_s() - t1;
+  out->iter = h.iter;
+  out->error = h.error;
+  out->resid = h.resid;
+  c.last_iter = std::min(h.iter, maxit);
+  return 0;
+}
+
+static std::mutex g_mu;
+static Context* g_ctx = nullptr;
+
+static Context* default_ctx() {
+  if (!g_ctx) g_ctx = new Context();
+  return g_ctx;
+}
+
+// upload x0/b (host) into the context, run, copy x back
+static int32_t krylov_host(Context& c, const hecmw33_sys* A, const double* B, double* X, int32_t* iarray,
+                           double* rarray, hecmw33_out* out) {
+  const double t0 = now_s();
+  const size_t vb = sizeof(double) * 3 * (size_t)A->NP;
+  HIPC(hipMemcpyAsync(c.vb.p, B, vb, hipMemcpyHostToDevice, c.stream));
+  HIPC(hipMemcpyAsync(c.vx.p, X, vb, hipMemcpyHostToDevice, c.stream));
+  HIPC(hipStreamSynchronize(c.stream));
+  out->t_upload += now_s() - t0;
+  if (int32_t e = krylov_resident(c, iarray, rarray, out)) return e;
+  HIPC(hipMemcpyAsync(X, c.vx.p, sizeof(double) * 3 * (size_t)A->N, hipMemcpyDeviceToHost, c.stream));
+  HIPC(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+static int32_t prepare(Context& c, const hecmw33_sys* A, bool force_symbolic, hecmw33_out* out) {
+  if (int32_t e = check_sys(A)) return e;
+  if (int32_t e = ctx_init(c)) return e;
+  const double t0 = now_s();
+  if (int32_t e = load_structure(c, A, force_symbolic)) return e;
+  if (int32_t e = load_values(c, A)) return e;
+  HIPC(hipStreamSynchronize(c.stream));
+  if (out) out->t_upload += now_s() - t0;
+  return 0;
+}
+
+static int32_t rel_resid_resident(Context& c, double* resid) {
+  const int64_t n3 = 3 * (int64_t)c.N;
+  KrylovState* st = c.st.as<KrylovState>();
+  double* part = c.part.as<double>();
+  launch_dot_store(n3, c.vb.as<double>(), c.vb.as<double>(), &st->ts, st, part, c.stream);
+  launch_matresid(c.A.view, c.vx.as<double>(), c.vb.as<double>(), c.vr.as<double>(), c.stream);
+  launch_dot_store(n3, c.vr.as<double>(), c.vr.as<double>(), &st->tt, st, part, c.stream);
+  HIPC(hipGetLastError());
+  HIPC(hipMemcpyAsync(&c.h_st[0], st, sizeof(KrylovState), hipMemcpyDeviceToHost, c.stream));
+  HIPC(hipStreamSynchronize(c.stream));
+  double bn = c.h_st[0].ts;
+  if (bn == 0.0) bn = 1.0;
+  *resid = std::sqrt(c.h_st[0].tt / bn);
+  return 0;
+}
+
+}  // namespace hecmw33
+
+using namespace hecmw33;
+
+struct hecmw33_ctx {
+  Context c;
+};
+
+extern "C" {
+
+const char* hecmw33_last_error(void) { return g_last_error.c_str(); }
+
+int32_t hecmw33_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int32_t hecmw33_solve_krylov(const hecmw33_sys* A, const double* B, double* X, int32_t iarray[100],
+                             double rarray[100], hecmw33_out* out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  hecmw33_out tmp{};
+  if (!out) out = &tmp;
+  *out = hecmw33_out{};
+  Context& c = *default_ctx();
+  if (int32_t e = prepare(c, A, IA(iarray, I_FLAG_SYMBFACT) == 1, out)) return e;
+  if (int32_t e = krylov_host(c, A, B, X, iarray, rarray, out)) return e;
+  return out->error;
+}
+
+int32_t hecmw33_solve(const hecmw33_sys* A, const double* B, double* X, int32_t iarray[100], double rarray[100],
+                      hecmw33_out* out) {
+  hecmw33_out tmp{};
+  if (!out) out = &tmp;
+  *out = hecmw33_out{};
+  if (int32_t e = check_sys(A)) return e;
+  const int32_t pc = IA(iarray, I_PRECOND), iterpremax = IA(iarray, I_ITERPREMAX);
+  // hecmw_solve_check_zerodiag (hecmw_solver_Iterative.f90:256-283)
+  for (int32_t i = 0; i < A->N; ++i)
+    for (int j = 0; j < 3; ++j)
+      if (std::fabs(A->D[9 * (int64_t)i + 4 * j]) == 0.0 && pc < 10 && iterpremax > 0) {
+        set_error("zero component in a diagonal block");
+        out->error = HECMW33_ERROR_ZERO_DIAG;
+        return HECMW33_ERROR_ZERO_DIAG;
+      }
+  // hecmw_mat_recycle_precond_setting (hecmw_matrix_misc.F90:858-877)
+  if (IA(iarray, I_FLAG_SYMBFACT) >= 1) {
+    IA(iarray, I_FLAG_NUMFACT) = 1;
+    IA(iarray, I_NRECYCLE) = 0;
+  } else if (IA(iarray, I_FLAG_NUMFACT) > 1) {
+    IA(iarray, I_NRECYCLE) = 0;
+    IA(iarray, I_FLAG_NUMFACT) = 1;
+  } else if (IA(iarray, I_FLAG_NUMFACT) == 1) {
+    if (IA(iarray, I_NRECYCLE) < IA(iarray, I_MAXRECYCLE)) {
+      IA(iarray, I_FLAG_NUMFACT) = 0;
+      IA(iarray, I_NRECYCLE) += 1;
+    } else {
+      IA(iarray, I_NRECYCLE) = 0;
+    }
+  }
+  double sigma = rarray[R_SIGMA_DIAG - 1];
+  const bool auto_sigma = sigma < 0.0;
+  if (auto_sigma) sigma = 1.0;
+  int32_t method = IA(iarray, I_METHOD);
+  const int32_t method2 = IA(iarray, I_METHOD2);
+  std::lock_guard<std::mutex> lk(g_mu);
+  Context& c = *default_ctx();
+  if (int32_t e = prepare(c, A, IA(iarray, I_FLAG_SYMBFACT) == 1, out)) return e;
+  int32_t error = 0;
+  for (;;) {
+    IA(iarray, I_FLAG_CONVERGED) = 0;
+    IA(iarray, I_FLAG_DIVERGED) = 0;
+    if (auto_sigma) rarray[R_SIGMA_DIAG - 1] = std::min(2.0, std::max(1.0, sigma));
+    int32_t ia2[100];
+    std::memcpy(ia2, iarray, sizeof(ia2));
+    IA(ia2, I_METHOD) = method;
+    if (int32_t e = krylov_host(c, A, B, X, ia2, rarray, out)) return e;
+    IA(iarray, I_FLAG_SYMBFACT) = IA(ia2, I_FLAG_SYMBFACT);
+    IA(iarray, I_FLAG_NUMFACT) = IA(ia2, I_FLAG_NUMFACT);
+    error = out->error;
+    if (error == HECMW33_ERROR_DIVERGE_PC || error == HECMW33_ERROR_DIVERGE_MAT || error == HECMW33_ERROR_DIVERGE_NAN) {
+      IA(iarray, I_FLAG_DIVERGED) = 1;
+      if (method == 1 && method2 > 1) {
+        if (auto_sigma) sigma = 1.0;
+        method = method2;
+        continue;
+      }
+    }
+    if (auto_sigma) rarray[R_SIGMA_DIAG - 1] = -1.0;
+    break;
+  }
+  double r2 = 0.0;
+  if (int32_t e = rel_resid_resident(c, &r2)) return e;
+  out->resid_true = r2;
+  if (r2 < RA(rarray, R_RESID)) IA(iarray, I_FLAG_CONVERGED) = 1;
+  return error;
+}
+
+int32_t hecmw33_matvec(const hecmw33_sys* A, const double* x, double* y) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  Context& c = *default_ctx();
+  if (int32_t e = prepare(c, A, false, nullptr)) return e;
+  HIPC(hipMemcpyAsync(c.vx.p, x, sizeof(double) * 3 * (size_t)A->NP, hipMemcpyHostToDevice, c.stream));
+  launch_spmv(c.A.view, c.vx.as<double>(), c.vq.as<double>(), c.stream);
+  HIPC(hipGetLastError());
+  HIPC(hipMemcpyAsync(y, c.vq.p, sizeof(double) * 3 * (size_t)A->N, hipMemcpyDeviceToHost, c.stream));
+  HIPC(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+int32_t hecmw33_precond_apply(const hecmw33_sys* A, const int32_t iarray[100], const double rarray[100],
+                              const double* r, double* z) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  Context& c = *default_ctx();
+  if (int32_t e = prepare(c, A, false, nullptr)) return e;
+  int32_t ia[100];
+  std::memcpy(ia, iarray, sizeof(ia));
+  IA(ia, I_FLAG_SYMBFACT) = 1;
+  IA(ia, I_FLAG_NUMFACT) = 1;
+  if (int32_t e = precond_setup(c, ia, rarray)) return e;
+  const size_t vb = sizeof(double) * 3 * (size_t)A->NP;
+  HIPC(hipMemcpyAsync(c.vr.p, r, vb, hipMemcpyHostToDevice, c.stream));
+  const int32_t pc = IA(ia, I_PRECOND);
+  if (IA(ia, I_ITERPREMAX) <= 0) {
+    HIPC(hipMemcpyAsync(c.vz.p, c.vr.p, vb, hipMemcpyDeviceToDevice, c.stream));
+  } else if (pc == 3) {
+    launch_diag_apply(c.N, c.alu_diag.as<double>(), c.vr.as<double>(), c.vz.as<double>(), nullptr, c.stream);
+  } else {
+    for (int k = 0; k < c.ncolor; ++k)
+      launch_ssor_fwd(c.L.view, c.row_old.as<int32_t>(), c.alu_ssor.as<double>(), c.color_slice[k],
+                      c.color_slice[k + 1] - c.color_slice[k], c.vr.as<double>(), c.vz.as<double>(), nullptr,
+                      c.stream);
+    for (int k = c.ncolor - 1; k >= 0; --k)
+      launch_ssor_bwd(c.U.view, c.row_old.as<int32_t>(), c.alu_ssor.as<double>(), c.color_slice[k],
+                      c.color_slice[k + 1] - c.color_slice[k], c.vz.as<double>(), nullptr, c.stream);
+  }
+  HIPC(hipGetLastError());
+  HIPC(hipMemcpyAsync(z, c.vz.p, sizeof(double) * 3 * (size_t)A->N, hipMemcpyDeviceToHost, c.stream));
+  HIPC(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+int32_t hecmw33_rel_resid_L2(const hecmw33_sys* A, const double* B, const double* X, double* resid) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  Context& c = *default_ctx();
+  if (int32_t e = prepare(c, A, false, nullptr)) return e;
+  const size_t vb = sizeof(double) * 3 * (size_t)A->NP;
+  HIPC(hipMemcpyAsync(c.vb.p, B, vb, hipMemcpyHostToDevice, c.stream));
+  HIPC(hipMemcpyAsync(c.vx.p, X, vb, hipMemcpyHostToDevice, c.stream));
+  return rel_resid_resident(c, resid);
+}
+
+int32_t hecmw33_ordering(int32_t N, const int32_t* indexL, const int32_t* itemL, const int32_t* indexU,
+                         const int32_t* itemU, int32_t ncolor_in, int32_t* perm_rcm, int32_t* perm, int32_t* iperm,
+                         int32_t* ncolor, int32_t* colorindex) {
+  if (N < 1 || !indexL || !indexU || !perm_rcm || !perm || !iperm || !ncolor || !colorindex) return HECMW33_ERROR_ARGS;
+  std::vector<int32_t> iperm_rcm(N);
+  ordering_rcm(N, indexL, itemL, indexU, itemU, perm_rcm, iperm_rcm.data());
+  *ncolor = ordering_mc(N, indexL, itemL, indexU, itemU, perm_rcm, ncolor_in, colorindex, perm, iperm);
+  return 0;
+}
+
+// ---- context API -----------------------------------------------------------
+hecmw33_ctx* hecmw33_ctx_create(void) {
+  hecmw33_ctx* c = new hecmw33_ctx();
+  if (ctx_init(c->c) != 0) {
+    delete c;
+    return nullptr;
+  }
+  return c;
+}
+
+void hecmw33_ctx_destroy(hecmw33_ctx* ctx) { delete ctx; }
+
+int32_t hecmw33_ctx_load(hecmw33_ctx* ctx, const hecmw33_sys* A, const int32_t iarray[100], const double rarray[100]) {
+  if (!ctx) return HECMW33_ERROR_ARGS;
+  if (int32_t e = prepare(ctx->c, A, true, nullptr)) return e;
+  int32_t ia[100];
+  std::memcpy(ia, iarray, sizeof(ia));
+  IA(ia, I_FLAG_SYMBFACT) = 1;
+  IA(ia, I_FLAG_NUMFACT) = 1;
+  return precond_setup(ctx->c, ia, rarray);
+}
+
+int32_t hecmw33_ctx_solve_device(hecmw33_ctx* ctx, const double* B_dev, double* X_dev, const int32_t iarray[100],
+                                 const double rarray[100], hecmw33_out* out) {
+  if (!ctx || !ctx->c.have_struct) return HECMW33_ERROR_ARGS;
+  hecmw33_out tmp{};
+  if (!out) out = &tmp;
+  *out = hecmw33_out{};
+  Context& c = ctx->c;
+  const size_t vb = sizeof(double) * 3 * (size_t)c.NP;
+  HIPC(hipMemcpyAsync(c.vb.p, B_dev, vb, hipMemcpyDeviceToDevice, c.stream));
+  HIPC(hipMemcpyAsync(c.vx.p, X_dev, vb, hipMemcpyDeviceToDevice, c.stream));
+  int32_t ia[100];
+  double ra[100];
+  std::memcpy(ia, iarray, sizeof(ia));
+  std::memcpy(ra, rarray, sizeof(ra));
+  if (int32_t e = krylov_resident(c, ia, ra, out)) return e;
+  HIPC(hipMemcpyAsync(X_dev, c.vx.p, vb, hipMemcpyDeviceToDevice, c.stream));
+  HIPC(hipStreamSynchronize(c.stream));
+  return out->error;
+}
+
+int32_t hecmw33_ctx_matvec_device(hecmw33_ctx* ctx, const double* x_dev, double* y_dev, int32_t count, double* ms) {
+  if (!ctx || !ctx->c.have_struct || count < 1) return HECMW33_ERROR_ARGS;
+  Context& c = ctx->c;
+  hipEvent_t e0, e1;
+  HIPC(hipEventCreate(&e0));
+  HIPC(hipEventCreate(&e1));
+  HIPC(hipEventRecord(e0, c.stream));
+  for (int i = 0; i < count; ++i) launch_spmv(c.A.view, x_dev, y_dev, c.stream);
+  HIPC(hipEventRecord(e1, c.stream));
+  HIPC(hipEventSynchronize(e1));
+  float t = 0.f;
+  HIPC(hipEventElapsedTime(&t, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (ms) *ms = (double)t / count;
+  return 0;
+}
+
+int32_t hecmw33_ctx_matvec_device_f32(hecmw33_ctx* ctx, const float* x_dev, float* y_dev, int32_t count, double* ms) {
+  if (!ctx || !ctx->c.have_struct || count < 1) return HECMW33_ERROR_ARGS;
+  Context& c = ctx->c;
+  if (!c.A.val32.p) {
+    if (c.A.val32.ensure(sizeof(float) * 9 * (size_t)std::max<int64_t>(c.A.view.nslots, 1))) return HECMW33_ERROR_DEVICE;
+    c.A.view32.val = c.A.val32.as<float>();
+    launch_fill_values_f32(c.A.view32, c.A.src.as<int64_t>(), c.dD.as<double>(), c.dAL.as<double>(),
+                           c.dAU.as<double>(), c.N, c.NPL, c.stream);
+    HIPC(hipGetLastError());
+  }
+  hipEvent_t e0, e1;
+  HIPC(hipEventCreate(&e0));
+  HIPC(hipEventCreate(&e1));
+  HIPC(hipEventRecord(e0, c.stream));
+  for (int i = 0; i < count; ++i) launch_spmv_f32(c.A.view32, x_dev, y_dev, c.stream);
+  HIPC(hipEventRecord(e1, c.stream));
+  HIPC(hipEventSynchronize(e1));
+  float t = 0.f;
+  HIPC(hipEventElapsedTime(&t, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (ms) *ms = (double)t / count;
+  return 0;
+}
+
+int32_t hecmw33_ctx_history(hecmw33_ctx* ctx, double* hist, int32_t cap) {
+  if (!ctx || !hist) return HECMW33_ERROR_ARGS;
+  Context& c = ctx->c;
+  const int32_t n = std::min(cap, c.last_iter);
+  if (n > 0) {
+    HIPC(hipMemcpyAsync(hist, c.hist.p, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    HIPC(hipStreamSynchronize(c.stream));
+  }
+  return n;
+}
+
+int32_t hecmw33_ctx_info(hecmw33_ctx* ctx, int64_t* info, int32_t n) {
+  if (!ctx || !info) return HECMW33_ERROR_ARGS;
+  Context& c = ctx->c;
+  const int64_t v[6] = {c.A.view.nslots, (int64_t)c.N, (int64_t)c.NPL, (int64_t)c.NPU, (int64_t)c.ncolor,
+                        c.L.view.nslots + c.U.view.nslots};
+  for (int i = 0; i < n && i < 6; ++i) info[i] = v[i];
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,342 @@
+"""ctypes binding of libhecmw33.so and a numpy mirror of hecmwST_matrix."""
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+lib_path = os.path.join(_HERE, "lib", "libhecmw33.so")
+
+# hecmw1/src/solver/init/hecmw_solve_error.f90:9-16
+ERRORS = {
+    1001: "INCONS_PC",
+    2001: "ZERO_DIAG",
+    2002: "ZERO_RHS",
+    3001: "NOCONV_MAXIT",
+    3002: "DIVERGE_MAT",
+    3003: "DIVERGE_PC",
+    3004: "DIVERGE_NAN",
+}
+
+
+class HecmwError(RuntimeError):
+    """A library failure (negative return code); reference error codes are
+    returned in SolveResult.error instead, as the reference returns them."""
+
+
+class _Sys(C.Structure):
+    _fields_ = [
+        ("N", C.c_int32), ("NP", C.c_int32), ("NPL", C.c_int32), ("NPU", C.c_int32),
+        ("D", C.c_void_p), ("AL", C.c_void_p), ("AU", C.c_void_p),
+        ("indexL", C.c_void_p), ("indexU", C.c_void_p), ("itemL", C.c_void_p), ("itemU", C.c_void_p),
+    ]
+
+
+class _Out(C.Structure):
+    _fields_ = [
+        ("iter", C.c_int32), ("error", C.c_int32), ("method", C.c_int32), ("ncolor", C.c_int32),
+        ("resid", C.c_double), ("resid_true", C.c_double),
+        ("Tset", C.c_double), ("Tsol", C.c_double), ("Tcomm", C.c_double),
+        ("time_Ax", C.c_double), ("time_precond", C.c_double), ("t_upload", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load libhecmw33.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(lib_path):
+        raise HecmwError("libhecmw33.so is not built: run python3 frontistr_amd/build.py")
+    L = C.CDLL(lib_path)
+    P = C.c_void_p
+    I = C.c_int32
+    L.hecmw33_solve.argtypes = [C.POINTER(_Sys), P, P, P, P, C.POINTER(_Out)]
+    L.hecmw33_solve_krylov.argtypes = [C.POINTER(_Sys), P, P, P, P, C.POINTER(_Out)]
+    L.hecmw33_matvec.argtypes = [C.POINTER(_Sys), P, P]
+    L.hecmw33_precond_apply.argtypes = [C.POINTER(_Sys), P, P, P, P]
+    L.hecmw33_rel_resid_L2.argtypes = [C.POINTER(_Sys), P, P, C.POINTER(C.c_double)]
+    L.hecmw33_ordering.argtypes = [I, P, P, P, P, I, P, P, P, C.POINTER(I), P]
+    L.hecmw33_ctx_create.restype = P
+    L.hecmw33_ctx_destroy.argtypes = [P]
+    L.hecmw33_ctx_load.argtypes = [P, C.POINTER(_Sys), P, P]
+    L.hecmw33_ctx_solve_device.argtypes = [P, P, P, P, P, C.POINTER(_Out)]
+    L.hecmw33_ctx_matvec_device.argtypes = [P, P, P, I, C.POINTER(C.c_double)]
+    L.hecmw33_ctx_matvec_device_f32.argtypes = [P, P, P, I, C.POINTER(C.c_double)]
+    L.hecmw33_ctx_history.argtypes = [P, P, I]
+    L.hecmw33_ctx_info.argtypes = [P, P, I]
+    L.hecmw33_gen_hex8_count.argtypes = [I, C.POINTER(I), C.POINTER(I), C.POINTER(I)]
+    L.hecmw33_gen_hex8.argtypes = [I, I, P, C.c_uint64, P, P, P, P, P, P, P, P, P]
+    L.hecmw33_last_error.restype = C.c_char_p
+    for f in ("hecmw33_solve", "hecmw33_solve_krylov", "hecmw33_matvec", "hecmw33_precond_apply",
+              "hecmw33_rel_resid_L2", "hecmw33_ordering", "hecmw33_ctx_load", "hecmw33_ctx_solve_device",
+              "hecmw33_ctx_matvec_device", "hecmw33_ctx_matvec_device_f32", "hecmw33_ctx_history",
+              "hecmw33_ctx_info", "hecmw33_gen_hex8_count", "hecmw33_gen_hex8", "hecmw33_device_count"):
+        getattr(L, f).restype = C.c_int32
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None and a.size else None
+
+
+def _check(rc):
+    if rc < 0:
+        raise HecmwError("libhecmw33 error %d: %s" % (rc, lib().hecmw33_last_error().decode()))
+    return rc
+
+
+class HecMatrix:
+    """The arrays of hecmwST_matrix for NDOF = 3, with hecmw_mat_init defaults.
+
+    index arrays are 0-based cumulative (length NP+1), items 1-based, blocks
+    row-major, B/X xyz interleaved -- exactly the reference layout."""
+
+    # Iarray / Rarray slots, hecmw1/src/solver/matrix/hecmw_matrix_misc.F90:107-150
+    I_ITER, I_METHOD, I_PRECOND, I_NSET, I_ITERPREMAX, I_NREST, I_SCALING, I_METHOD2 = 1, 2, 3, 4, 5, 6, 7, 8
+    I_ITERLOG, I_TIMELOG, I_NCOLOR_IN, I_MAXRECYCLE = 21, 22, 34, 35
+    I_FLAG_CONVERGED, I_FLAG_DIVERGED = 81, 82
+    I_NRECYCLE, I_FLAG_NUMFACT, I_FLAG_SYMBFACT, I_SOLVER_TYPE = 96, 97, 98, 99
+    R_RESID, R_SIGMA_DIAG, R_SIGMA, R_THRESH, R_FILTER = 1, 2, 3, 4, 5
+
+    def __init__(self, N, D, AL, AU, indexL, itemL, indexU, itemU, B=None, X=None, NP=None):
+        self.N = int(N)
+        self.NP = int(NP if NP is not None else N)
+        self.D = np.ascontiguousarray(D, dtype=np.float64)
+        self.AL = np.ascontiguousarray(AL, dtype=np.float64)
+        self.AU = np.ascontiguousarray(AU, dtype=np.float64)
+        self.indexL = np.ascontiguousarray(indexL, dtype=np.int32)
+        self.indexU = np.ascontiguousarray(indexU, dtype=np.int32)
+        self.itemL = np.ascontiguousarray(itemL, dtype=np.int32)
+        self.itemU = np.ascontiguousarray(itemU, dtype=np.int32)
+        self.NPL = int(self.indexL[self.NP])
+        self.NPU = int(self.indexU[self.NP])
+        self.B = np.zeros(3 * self.NP) if B is None else np.ascontiguousarray(B, dtype=np.float64)
+        self.X = np.zeros(3 * self.NP) if X is None else np.ascontiguousarray(X, dtype=np.float64)
+        self.Iarray = np.zeros(100, dtype=np.int32)
+        self.Rarray = np.zeros(100, dtype=np.float64)
+        self.mat_init()
+
+    # hecmw_mat_init (hecmw_matrix_misc.F90:171-212)
+    def mat_init(self):
+        self.Iarray[:] = 0
+        self.Rarray[:] = 0.0
+        self.set_i(self.I_ITER, 100)
+        self.set_i(self.I_METHOD, 1)
+        self.set_i(self.I_PRECOND, 1)
+        self.set_i(self.I_ITERPREMAX, 1)
+        self.set_i(self.I_NREST, 10)
+        self.set_i(self.I_NCOLOR_IN, 10)
+        self.set_i(self.I_MAXRECYCLE, 3)
+        self.set_i(24, -1)
+        self.Rarray[self.R_RESID - 1] = 1.0e-8
+        self.Rarray[self.R_SIGMA_DIAG - 1] = 1.0
+        self.Rarray[self.R_THRESH - 1] = 0.10
+        self.Rarray[self.R_FILTER - 1] = 0.10
+        self.Rarray[11 - 1] = 1.0e4
+        self.set_i(self.I_NRECYCLE, 0)
+        self.set_i(self.I_FLAG_NUMFACT, 1)
+        self.set_i(self.I_FLAG_SYMBFACT, 1)
+        self.set_i(self.I_SOLVER_TYPE, 1)
+
+    def set_i(self, k, v):
+        self.Iarray[k - 1] = v
+
+    def get_i(self, k):
+        return int(self.Iarray[k - 1])
+
+    # named setters mirroring hecmw_mat_set_*
+    def set_iter(self, v): self.set_i(self.I_ITER, v)
+    def set_method(self, v): self.set_i(self.I_METHOD, v)
+    def set_method2(self, v): self.set_i(self.I_METHOD2, v)
+    def set_precond(self, v): self.set_i(self.I_PRECOND, v)
+    def set_iterpremax(self, v): self.set_i(self.I_ITERPREMAX, v)
+    def set_ncolor_in(self, v): self.set_i(self.I_NCOLOR_IN, v)
+    def set_resid(self, v): self.Rarray[self.R_RESID - 1] = v
+
+    def set_sigma_diag(self, v):
+        # hecmw_mat_set_sigma_diag clamps to [1, 2], negative = automatic (-1)
+        self.Rarray[self.R_SIGMA_DIAG - 1] = -1.0 if v < 0 else min(2.0, max(1.0, v))
+
+    def _sys(self):
+        s = _Sys(self.N, self.NP, self.NPL, self.NPU, _ptr(self.D), _ptr(self.AL), _ptr(self.AU),
+                 _ptr(self.indexL), _ptr(self.indexU), _ptr(self.itemL), _ptr(self.itemU))
+        return s
+
+    def copy(self):
+        m = HecMatrix(self.N, self.D.copy(), self.AL.copy(), self.AU.copy(), self.indexL.copy(), self.itemL.copy(),
+                      self.indexU.copy(), self.itemU.copy(), self.B.copy(), self.X.copy(), NP=self.NP)
+        m.Iarray[:] = self.Iarray
+        m.Rarray[:] = self.Rarray
+        return m
+
+
+@dataclass
+class SolveResult:
+    iter: int
+    error: int
+    method: int
+    ncolor: int
+    resid: float
+    resid_true: float
+    Tset: float
+    Tsol: float
+    t_upload: float
+
+    @classmethod
+    def _from(cls, o):
+        return cls(o.iter, o.error, o.method, o.ncolor, o.resid, o.resid_true, o.Tset, o.Tsol, o.t_upload)
+
+
+def hecmw_solve_iterative(mat):
+    """hecmw_solve_iterative for NDOF=3 on the GPU; mat.X is updated and
+    mat.Iarray/Rarray receive the flags the reference writes."""
+    out = _Out()
+    s = mat._sys()
+    rc = lib().hecmw33_solve(C.byref(s), _ptr(mat.B), _ptr(mat.X), _ptr(mat.Iarray), _ptr(mat.Rarray), C.byref(out))
+    _check(rc)
+    if rc == 2001:
+        out.error = 2001
+    return SolveResult._from(out)
+
+
+def hecmw_solve_krylov(mat):
+    """One hecmw_solve_CG / hecmw_solve_BiCGSTAB call (METHOD from Iarray)."""
+    out = _Out()
+    s = mat._sys()
+    rc = lib().hecmw33_solve_krylov(C.byref(s), _ptr(mat.B), _ptr(mat.X), _ptr(mat.Iarray), _ptr(mat.Rarray),
+                                    C.byref(out))
+    _check(rc)
+    return SolveResult._from(out)
+
+
+def hecmw_matvec(mat, x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros(3 * mat.N)
+    s = mat._sys()
+    _check(lib().hecmw33_matvec(C.byref(s), _ptr(x), _ptr(y)))
+    return y
+
+
+def hecmw_precond_apply(mat, r):
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    z = np.zeros(3 * mat.N)
+    s = mat._sys()
+    _check(lib().hecmw33_precond_apply(C.byref(s), _ptr(mat.Iarray), _ptr(mat.Rarray), _ptr(r), _ptr(z)))
+    return z
+
+
+def hecmw_rel_resid_L2(mat):
+    s = mat._sys()
+    v = C.c_double(0.0)
+    _check(lib().hecmw33_rel_resid_L2(C.byref(s), _ptr(mat.B), _ptr(mat.X), C.byref(v)))
+    return v.value
+
+
+def hecmw_ordering(mat, ncolor_in=10):
+    """(perm_rcm, perm, iperm, colorindex) of the SSOR setup, 1-based ids."""
+    N = mat.N
+    pr = np.zeros(N, np.int32)
+    p = np.zeros(N, np.int32)
+    ip = np.zeros(N, np.int32)
+    ci = np.zeros(N + 1, np.int32)
+    nc = C.c_int32(0)
+    _check(lib().hecmw33_ordering(N, _ptr(mat.indexL), _ptr(mat.itemL), _ptr(mat.indexU), _ptr(mat.itemU),
+                                  ncolor_in, _ptr(pr), _ptr(p), _ptr(ip), C.byref(nc), _ptr(ci)))
+    return pr, p, ip, ci[: nc.value + 1].copy()
+
+
+def gen_hex8(n, kind="uniform", E=210000.0, nu=0.3, seed=0, emin=1e3, emax=1e6, numin=0.2, numax=0.4,
+             contrast=1e4, rmin=2, rmax=6, vfrac=0.15, return_E=False):
+    """Structured n^3 HEX8 unit cube assembled like fistr1 (see gen_hex8.cpp).
+
+    kind: "uniform" (E, nu), "lognormal" per-element E log-uniform in
+    [emin, emax] with nu uniform in [numin, numax], or "inclusions"
+    (two-phase spheres of E*contrast, vfrac volume fraction)."""
+    L = lib()
+    N = C.c_int32()
+    NPL = C.c_int32()
+    NPU = C.c_int32()
+    _check(L.hecmw33_gen_hex8_count(n, C.byref(N), C.byref(NPL), C.byref(NPU)))
+    N, NPL, NPU = N.value, NPL.value, NPU.value
+    D = np.empty(9 * N)
+    AL = np.empty(9 * NPL)
+    AU = np.empty(9 * NPU)
+    iL = np.empty(N + 1, np.int32)
+    iU = np.empty(N + 1, np.int32)
+    tL = np.empty(NPL, np.int32)
+    tU = np.empty(NPU, np.int32)
+    B = np.empty(3 * N)
+    Ee = np.empty(n ** 3, np.float32) if return_E else None
+    if kind == "uniform":
+        k, p = 0, [E, nu]
+    elif kind == "lognormal":
+        k, p = 1, [emin, emax, numin, numax]
+    elif kind == "inclusions":
+        k, p = 2, [E, nu, contrast, rmin, rmax, vfrac]
+    else:
+        raise ValueError(kind)
+    pa = np.array(p + [0.0] * (8 - len(p)), dtype=np.float64)
+    rc = L.hecmw33_gen_hex8(n, k, _ptr(pa), seed, _ptr(D), _ptr(AL), _ptr(AU), _ptr(iL), _ptr(tL), _ptr(iU),
+                            _ptr(tU), _ptr(B), _ptr(Ee) if Ee is not None else None)
+    if rc != 0:
+        raise HecmwError("hecmw33_gen_hex8 failed (%d)" % rc)
+    m = HecMatrix(N, D, AL, AU, iL, tL, iU, tU, B=B)
+    if return_E:
+        return m, Ee
+    return m
+
+
+class DeviceContext:
+    """A context that keeps the system resident in HBM (hecmw33_ctx_*)."""
+
+    def __init__(self):
+        self._h = lib().hecmw33_ctx_create()
+        if not self._h:
+            raise HecmwError("hecmw33_ctx_create failed: %s" % lib().hecmw33_last_error().decode())
+
+    def close(self):
+        if self._h:
+            lib().hecmw33_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load(self, mat):
+        s = mat._sys()
+        _check(lib().hecmw33_ctx_load(self._h, C.byref(s), _ptr(mat.Iarray), _ptr(mat.Rarray)))
+
+    def solve_device(self, b_ptr, x_ptr, iarray, rarray):
+        out = _Out()
+        ia = np.ascontiguousarray(iarray, dtype=np.int32)
+        ra = np.ascontiguousarray(rarray, dtype=np.float64)
+        rc = lib().hecmw33_ctx_solve_device(self._h, C.c_void_p(b_ptr), C.c_void_p(x_ptr), _ptr(ia), _ptr(ra),
+                                            C.byref(out))
+        _check(rc)
+        return SolveResult._from(out)
+
+    def matvec_device(self, x_ptr, y_ptr, count=1, fp32=False):
+        ms = C.c_double(0.0)
+        f = lib().hecmw33_ctx_matvec_device_f32 if fp32 else lib().hecmw33_ctx_matvec_device
+        _check(f(self._h, C.c_void_p(x_ptr), C.c_void_p(y_ptr), count, C.byref(ms)))
+        return ms.value
+
+    def history(self, cap):
+        h = np.zeros(cap)
+        n = _check(lib().hecmw33_ctx_history(self._h, _ptr(h), cap))
+        return h[:n]
+
+    def info(self):
+        v = np.zeros(6, np.int64)
+        _check(lib().hecmw33_ctx_info(self._h, _ptr(v), 6))
+        return {"nslots": int(v[0]), "N": int(v[1]), "NPL": int(v[2]), "NPU": int(v[3]), "ncolor": int(v[4]),
+                "ssor_slots": int(v[5])}

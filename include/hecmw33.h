@@ -1,0 +1,145 @@
+/*
+ * hecmw33 — MI355X (gfx950) solver for hecmw1's 3-DOF-per-node linear systems.
+ *
+ * C ABI of libhecmw33.so.  Plain pointers and sizes only; arrays follow the
+ * reference's hecmwST_matrix layout verbatim (hecmw1/src/common/hecmw_util_f.F90:458-470):
+ *   D(9*NP), AL(9*NPL), AU(9*NPU): 3x3 blocks, row-major inside a block
+ *   indexL/indexU(0:NP): cumulative counts (0-based offsets), NP+1 entries
+ *   itemL/itemU: 1-based node ids (itemL < row, itemU > row, itemU may be > N)
+ *   B, X (3*NP): xyz interleaved per node
+ *   iarray[100], rarray[100]: hecMAT%Iarray / hecMAT%Rarray, 1-based slot k is
+ *   element k-1 (index map: hecmw1/src/solver/matrix/hecmw_matrix_misc.F90:107-150)
+ * Host arrays are never freed or resized by the library; B and the matrix are
+ * left bitwise unchanged.
+ *
+ * Return codes are the reference's (hecmw1/src/solver/init/hecmw_solve_error.f90:9-16);
+ * negative codes are library failures (see hecmw33_last_error()).
+ */
+#ifndef HECMW33_H
+#define HECMW33_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  HECMW33_OK = 0,
+  HECMW33_ERROR_INCONS_PC = 1001,
+  HECMW33_ERROR_ZERO_DIAG = 2001,
+  HECMW33_ERROR_ZERO_RHS = 2002,
+  HECMW33_ERROR_NOCONV_MAXIT = 3001,
+  HECMW33_ERROR_DIVERGE_MAT = 3002,
+  HECMW33_ERROR_DIVERGE_PC = 3003,
+  HECMW33_ERROR_DIVERGE_NAN = 3004,
+  HECMW33_ERROR_ARGS = -1,     /* invalid sizes or pointers */
+  HECMW33_ERROR_DEVICE = -2,   /* HIP runtime failure or no GPU */
+  HECMW33_ERROR_UNSUPPORTED = -3
+};
+
+/* The system: pointers into the caller's hecmwST_matrix arrays. */
+typedef struct hecmw33_sys {
+  int32_t N, NP, NPL, NPU;
+  const double* D;
+  const double* AL;
+  const double* AU;
+  const int32_t* indexL;
+  const int32_t* indexU;
+  const int32_t* itemL;
+  const int32_t* itemU;
+} hecmw33_sys;
+
+/* What hecmw_solve_CG / hecmw_solve_BiCGSTAB return through their arguments,
+ * plus the dispatcher's post-solve figures. */
+typedef struct hecmw33_out {
+  int32_t iter;          /* ITER (the Fortran DO variable: MAXIT+1 when not converged) */
+  int32_t error;         /* 0 or a reference error code */
+  int32_t method;        /* METHOD actually used (after a METHOD2 fall-back) */
+  int32_t ncolor;        /* colours of the SSOR ordering, 0 otherwise */
+  double resid;          /* RESID of the Krylov loop */
+  double resid_true;     /* hecmw_rel_resid_L2 after the solve (hecmw33_solve only) */
+  double Tset, Tsol, Tcomm;
+  double time_Ax, time_precond;  /* filled when TIMELOG (Iarray 22) >= 1 */
+  double t_upload;       /* host->device copies of this call (not part of Tset/Tsol) */
+} hecmw33_out;
+
+/* ---- the drop-in entry points --------------------------------------------- */
+
+/* Replaces hecmw_solve_iterative for NDOF == 3
+ * (hecmw1/src/solver/iterative/hecmw_solver_Iterative.f90:13-187): parameter
+ * read-out, zero-diagonal check (returned as 2001 instead of aborting), the
+ * preconditioner recycle setting (writes iarray 96/97/98), the METHOD2
+ * fall-back loop, hecmw_rel_resid_L2 and the converged/diverged flags
+ * (iarray 81/82).  METHOD 1 (CG) and 2 (BiCGSTAB), PRECOND 1/2 (multicolour
+ * SSOR) and 3 (DIAG), ITERPREMAX >= 0.  X: x0 in, solution out (host memory). */
+int32_t hecmw33_solve(const hecmw33_sys* A, const double* B, double* X, int32_t iarray[100], double rarray[100],
+                      hecmw33_out* out);
+
+/* Replaces one call of hecmw_solve_CG (METHOD 1,
+ * hecmw1/src/solver/iterative/hecmw_solver_CG.f90:19-320) or
+ * hecmw_solve_BiCGSTAB (METHOD 2, hecmw_solver_BiCGSTAB.f90:16-308), same
+ * control flow, breakdown codes and residual recomputation cadence.  The
+ * preconditioner is set up as hecmw_precond_setup does (reuse unless
+ * iarray 97/98 ask for a new one; both are cleared afterwards). */
+int32_t hecmw33_solve_krylov(const hecmw33_sys* A, const double* B, double* X, int32_t iarray[100],
+                             double rarray[100], hecmw33_out* out);
+
+/* y = A x for the internal rows (hecmw_matvec_33, hecmw1/src/solver/las/hecmw_solver_las_33.F90:34-59).
+ * Host pointers, x has 3*NP entries, y receives 3*N. */
+int32_t hecmw33_matvec(const hecmw33_sys* A, const double* x, double* y);
+
+/* z = M^-1 r with the preconditioner selected by iarray/rarray
+ * (hecmw_precond_setup + hecmw_precond_apply, hecmw1/src/solver/precond/hecmw_precond.f90:30-172).
+ * A new setup is always made.  Host pointers, 3*NP entries. */
+int32_t hecmw33_precond_apply(const hecmw33_sys* A, const int32_t iarray[100], const double rarray[100],
+                              const double* r, double* z);
+
+/* sqrt(|b - A x|^2 / |b|^2) (hecmw_rel_resid_L2_33, hecmw_solver_las_33.F90:406-435). */
+int32_t hecmw33_rel_resid_L2(const hecmw33_sys* A, const double* B, const double* X, double* resid);
+
+/* RCM + capped greedy multicolouring of the internal rows, as
+ * hecmw_precond_SSOR_33_setup builds it (hecmw_matrix_ordering_CM.f90:65-78 then
+ * hecmw_matrix_ordering_MC.f90:15-72).  Host only.  perm_rcm, perm, iperm:
+ * N entries of 1-based ids; colorindex: N+1 entries, *ncolor colours. */
+int32_t hecmw33_ordering(int32_t N, const int32_t* indexL, const int32_t* itemL, const int32_t* indexU,
+                         const int32_t* itemU, int32_t ncolor_in, int32_t* perm_rcm, int32_t* perm,
+                         int32_t* iperm, int32_t* ncolor, int32_t* colorindex);
+
+/* ---- device-resident use (no reference counterpart: for callers and benchmarks
+ *      that keep the system in HBM between solves) ------------------------------ */
+
+typedef struct hecmw33_ctx hecmw33_ctx;
+
+hecmw33_ctx* hecmw33_ctx_create(void);
+void hecmw33_ctx_destroy(hecmw33_ctx* ctx);
+/* Upload the matrix (structure when new, values always) and set up the
+ * preconditioner that iarray/rarray select. */
+int32_t hecmw33_ctx_load(hecmw33_ctx* ctx, const hecmw33_sys* A, const int32_t iarray[100], const double rarray[100]);
+/* Krylov solve with B, X already in device memory (3*NP doubles each). */
+int32_t hecmw33_ctx_solve_device(hecmw33_ctx* ctx, const double* B_dev, double* X_dev, const int32_t iarray[100],
+                                 const double rarray[100], hecmw33_out* out);
+/* `count` back-to-back products y = A x on device vectors; returns the mean
+ * kernel time in *ms (HIP events on the context's stream). */
+int32_t hecmw33_ctx_matvec_device(hecmw33_ctx* ctx, const double* x_dev, double* y_dev, int32_t count, double* ms);
+int32_t hecmw33_ctx_matvec_device_f32(hecmw33_ctx* ctx, const float* x_dev, float* y_dev, int32_t count, double* ms);
+/* Residual history of the last solve (ITERLOG values), up to cap entries. */
+int32_t hecmw33_ctx_history(hecmw33_ctx* ctx, double* hist, int32_t cap);
+/* Matrix storage figures: nslots (SELL slots incl. padding), bytes of the FP64 SpMV stream. */
+int32_t hecmw33_ctx_info(hecmw33_ctx* ctx, int64_t* info, int32_t n);
+
+/* ---- synthetic systems (fistr1's assembly for a structured HEX8 cube) ------- */
+int32_t hecmw33_gen_hex8_count(int32_t n, int32_t* N, int32_t* NPL, int32_t* NPU);
+int32_t hecmw33_gen_hex8(int32_t n, int32_t mat_kind, const double* params, uint64_t seed, double* D, double* AL,
+                         double* AU, int32_t* indexL, int32_t* itemL, int32_t* indexU, int32_t* itemU, double* B,
+                         float* elem_E);
+
+/* ---- misc -------------------------------------------------------------------- */
+const char* hecmw33_last_error(void);
+int32_t hecmw33_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HECMW33_H */

@@ -1,0 +1,130 @@
+"""Loader for oracle/_ref/libhecmw_ref.so: the reference's own hecmw1 solver
+compiled from its sources by oracle/build_ref.py.
+
+Test infrastructure only (the checker): imported by tests/, by
+__graft_entry__.smoke() and by bench.py's CPU-baseline leg, never by the
+product package.
+
+The reference keeps module-level state between calls (the SSOR factorisation,
+the SpMV thread partition); every function here forces a fresh preconditioner
+setup unless told otherwise.  One caveat of the reference: its SpMV thread
+partition is recomputed only when the row count changes by more than one
+(hecmw_solver_las_33.F90:165-171), so callers should not alternate between
+systems whose N differ by exactly one.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_ref", "libhecmw_ref.so")
+_lib = None
+
+
+def available():
+    return os.path.exists(LIB)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not available():
+            raise RuntimeError("oracle/_ref/libhecmw_ref.so missing: run python3 oracle/build_ref.py")
+        L = C.CDLL(LIB)
+        P = C.c_void_p
+        I = C.c_int32
+        L.ref_solve_krylov.argtypes = [I, I, I, I] + [P] * 9 + [P, P, I, P, P]
+        L.ref_solve_krylov.restype = I
+        L.ref_solve_iterative.argtypes = [I, I, I, I] + [P] * 9 + [P, P, I, P, P]
+        L.ref_solve_iterative.restype = I
+        L.ref_ordering.argtypes = [I, P, P, P, P, I, P, P, C.POINTER(I), P, P, P]
+        L.ref_ordering.restype = None
+        L.ref_matvec.argtypes = [I, I, I, I] + [P] * 9 + [I]
+        L.ref_matvec.restype = None
+        L.ref_precond.argtypes = [I, I, I, I] + [P] * 9 + [P, P, I]
+        L.ref_precond.restype = None
+        L.ref_rel_resid.argtypes = [I, I, I, I] + [P] * 9 + [I]
+        L.ref_rel_resid.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def _mat_args(m):
+    # keep at least one element behind every pointer (empty AL/AU on 1-row systems)
+    AL = m.AL if m.AL.size else np.zeros(9)
+    AU = m.AU if m.AU.size else np.zeros(9)
+    tL = m.itemL if m.itemL.size else np.ones(1, np.int32)
+    tU = m.itemU if m.itemU.size else np.ones(1, np.int32)
+    return [m.N, m.NP, m.NPL, m.NPU, _p(m.D), _p(AL), _p(AU), _p(m.indexL), _p(tL), _p(m.indexU), _p(tU)]
+
+
+def solve_krylov(m, x0=None, nthreads=2, iarray=None, rarray=None):
+    """hecmw_solve_CG / hecmw_solve_BiCGSTAB on a copy of the system.
+    Returns (x, info) with info = dict(iter, resid, error, Tset, Tsol, Tcomm, time_Ax, time_precond, iarray)."""
+    x = (np.zeros(3 * m.NP) if x0 is None else np.array(x0, dtype=np.float64)).copy()
+    b = m.B.copy()
+    ia = np.array(m.Iarray if iarray is None else iarray, dtype=np.int32).copy()
+    ra = np.array(m.Rarray if rarray is None else rarray, dtype=np.float64).copy()
+    ia[96] = 1  # Iarray(97): numeric setup
+    ia[97] = 1  # Iarray(98): symbolic setup
+    iout = np.zeros(100, np.int32)
+    out = np.zeros(7)
+    err = lib().ref_solve_krylov(*_mat_args(m), _p(b), _p(x), _p(ia), _p(ra), nthreads, _p(iout), _p(out))
+    info = dict(iter=int(out[0]), resid=float(out[1]), error=int(err), Tset=float(out[2]), Tsol=float(out[3]),
+                Tcomm=float(out[4]), time_Ax=float(out[5]), time_precond=float(out[6]), iarray=iout)
+    return x, info
+
+
+def solve_iterative(m, x0=None, nthreads=2):
+    """hecmw_solve_iterative (full dispatcher).  Returns (x, iarray_out, rarray_out)."""
+    x = (np.zeros(3 * m.NP) if x0 is None else np.array(x0, dtype=np.float64)).copy()
+    b = m.B.copy()
+    ia = np.array(m.Iarray, dtype=np.int32).copy()
+    ra = np.array(m.Rarray, dtype=np.float64).copy()
+    iout = np.zeros(100, np.int32)
+    rout = np.zeros(100)
+    lib().ref_solve_iterative(*_mat_args(m), _p(b), _p(x), _p(ia), _p(ra), nthreads, _p(iout), _p(rout))
+    return x, iout, rout
+
+
+def ordering(m, ncolor_in=10):
+    """(perm_rcm, iperm_rcm, perm, iperm, colorindex) from the reference routines, 1-based."""
+    N = m.N
+    pr = np.zeros(N, np.int32)
+    ipr = np.zeros(N, np.int32)
+    p = np.zeros(N, np.int32)
+    ip = np.zeros(N, np.int32)
+    ci = np.zeros(N + 1, np.int32)
+    nc = C.c_int32(0)
+    tL = m.itemL if m.itemL.size else np.ones(1, np.int32)
+    tU = m.itemU if m.itemU.size else np.ones(1, np.int32)
+    lib().ref_ordering(N, _p(m.indexL), _p(tL), _p(m.indexU), _p(tU), ncolor_in, _p(pr), _p(ipr), C.byref(nc),
+                       _p(ci), _p(p), _p(ip))
+    return pr, ipr, p, ip, ci[: nc.value + 1].copy()
+
+
+def matvec(m, x, nthreads=1):
+    x = np.array(x, dtype=np.float64)
+    y = np.zeros(3 * m.NP)
+    lib().ref_matvec(*_mat_args(m), _p(x), _p(y), nthreads)
+    return y[: 3 * m.N]
+
+
+def precond(m, r, nthreads=2, iarray=None, rarray=None):
+    r = np.array(r, dtype=np.float64)
+    z = np.zeros(3 * m.NP)
+    ia = np.array(m.Iarray if iarray is None else iarray, dtype=np.int32).copy()
+    ra = np.array(m.Rarray if rarray is None else rarray, dtype=np.float64).copy()
+    lib().ref_precond(*_mat_args(m), _p(r), _p(z), _p(ia), _p(ra), nthreads)
+    return z[: 3 * m.N]
+
+
+def rel_resid(m, x, nthreads=1):
+    x = np.array(x, dtype=np.float64)
+    b = m.B.copy()
+    return lib().ref_rel_resid(*_mat_args(m), _p(b), _p(x), nthreads)
