@@ -24,6 +24,7 @@ from .hecmw import (  # noqa: F401
     lib_path,
     hecmw_solve_iterative,
     hecmw_solve_krylov,
+    last_history,
     hecmw_matvec,
     hecmw_precond_apply,
     hecmw_rel_resid_L2,

@@ -27,9 +27,9 @@ struct Sell {
   T* val = nullptr;            // [9*nslots]
 };
 
-// Device-resident scalars of one Krylov solve; kernels read and update them so
-// the host never waits inside an iteration.  Only the last workgroup of a
-// reduction kernel writes them, after every other workgroup has finished.
+// Device-resident scalars of one Krylov solve; kernels read them and the
+// one-workgroup tail kernels update them, so the host never waits inside an
+// iteration.
 struct KrylovState {
   double bnrm2, tol;
   double rho, rho1, alpha, beta, c1, dnrm2, resid;
@@ -42,7 +42,6 @@ struct KrylovState {
   int32_t recheck;   // recompute the true residual in this iteration
   int32_t nrec;      // true-residual period (50 CG, 100 BiCGSTAB)
   int32_t pad;
-  uint32_t ticket[4];
 };
 
 constexpr int kMaxPartials = 65536;  // per reduction component
@@ -77,6 +76,15 @@ void launch_ssor_bwd_f32(const Sell<float>& U, const int32_t* row_old, const flo
                          float* z, hipStream_t s);
 void launch_diag_apply(int32_t n, const double* alu, const double* r, double* z, const KrylovState* st,
                        hipStream_t s);
+void launch_copy(int64_t n3, const double* r, double* z, const KrylovState* st, hipStream_t s);
+void launch_zero(int64_t n, double* x, const KrylovState* st, hipStream_t s);
+void launch_add(int64_t n, const double* x, double* y, const KrylovState* st, hipStream_t s);
+void launch_matresid_guarded(const Sell<double>& a, const double* x, const double* b, double* r,
+                             const KrylovState* st, hipStream_t s);
+// z = M^-1 r, kind 0: copy, 1/2: multicolour SSOR, 3: DIAG (guarded by st->done)
+void precond_apply(int32_t kind, const Sell<double>& L, const Sell<double>& U, const int32_t* row_old,
+                   const double* alu_ssor, const int32_t* color_slice, int32_t ncolor, const double* alu_diag,
+                   int32_t n, const double* r, double* z, const KrylovState* st, hipStream_t s);
 
 // ---- fused CG iteration pieces (hecmw_solver_CG.f90:164-279) ----------------------
 // z = M^-1 r with rho = r.z and the rho/beta logic
@@ -96,5 +104,19 @@ void cg_update(int64_t n3, const double* p, const double* q, double* x, double* 
 // conditional r = b - A x with |r|^2, convergence, end of iteration
 void cg_true_resid(const Sell<double>& a, const double* x, const double* b, double* r, KrylovState* st, double* part,
                    double* hist, hipStream_t s);
+
+// rho = r.z only (ITERPREMAX > 1 path)
+void cg_rho(int64_t n3, const double* r, const double* z, KrylovState* st, double* part, hipStream_t s);
+
+// ---- fused BiCGSTAB iteration pieces (hecmw_solver_BiCGSTAB.f90:164-275) ------------
+void bi_rho(int64_t n3, const double* r, const double* rt, KrylovState* st, double* part, hipStream_t s);
+void bi_p(int64_t n3, const double* r, const double* v, double* p, const KrylovState* st, hipStream_t s);
+void bi_spmv_c2(const Sell<double>& a, const double* pt, const double* rt, double* v, KrylovState* st, double* part,
+                hipStream_t s);
+void bi_s(int64_t n3, const double* v, const double* r, double* sv, const KrylovState* st, hipStream_t s);
+void bi_spmv_omega(const Sell<double>& a, const double* stv, const double* sv, double* t, KrylovState* st,
+                   double* part, hipStream_t s);
+void bi_update(int64_t n3, const double* pt, const double* stv, const double* t, const double* sv, double* x,
+               double* r, KrylovState* st, double* part, double* hist, hipStream_t s);
 
 }  // namespace hecmw33

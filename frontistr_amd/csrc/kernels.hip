@@ -1,19 +1,26 @@
 // HIP kernels of the 3x3-block Krylov solver for gfx950 (MI355X).
 //
-// Everything here is HBM-bandwidth bound (0.23 flop/byte in FP64), so the
-// design goals are: every wavefront load of matrix data is one contiguous
-// segment (SELL-64 planes), x/z gathers stay inside an XCD's L2 (blocks are
-// remapped so each XCD sweeps a contiguous run of slices), no host round trip
-// inside an iteration (scalars live in KrylovState), and reductions are
-// deterministic (per-block partials summed in a fixed order by the last block).
+// Everything here is HBM-bandwidth bound (0.23 flop/byte in FP64), so:
+//  * matrix data is read as contiguous wavefront segments (SELL-64 planes);
+//  * each lane issues a whole chunk of block loads (values, columns, gathers)
+//    before it accumulates, so one wavefront keeps many HBM requests in flight
+//    (the per-colour SSOR launches have few wavefronts);
+//  * workgroups are remapped so each XCD sweeps a contiguous run of slices and
+//    x/z gathers hit that XCD's L2;
+//  * scalars stay on the device (KrylovState); a reduction kernel writes one
+//    partial per workgroup and a one-workgroup tail kernel sums them in a fixed
+//    order and runs the Krylov scalar logic.  No grid-wide fences or atomics:
+//    on gfx950 an agent-scope fence writes the XCD's L2 back, which made the
+//    in-kernel "last block" pattern cost ~0.2 ms per reduction.
 //
 // FP64 row arithmetic is written with contraction off and in the reference's
 // operation order, so SpMV, SSOR and DIAG results are bitwise those of the
 // reference CPU loops (hecmw_solver_las_33.F90:281-308,
 // hecmw_precond_SSOR_33.F90:302-405, hecmw_precond_DIAG_33.F90:115-140).
 #include <hip/hip_runtime.h>
-#include <cmath>
+
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 
 #include "device.h"
@@ -22,10 +29,10 @@ namespace hecmw33 {
 
 namespace {
 
+constexpr int32_t kErrNoconv = 3001;
 constexpr int32_t kErrDivergeMat = 3002;
 constexpr int32_t kErrDivergePc = 3003;
 constexpr int32_t kErrDivergeNan = 3004;
-constexpr int32_t kErrNoconv = 3001;
 
 __device__ __forceinline__ int xcd_block(int bid, int nblk) {
   // Workgroups are dealt to the 8 XCDs round-robin; give each XCD a contiguous
@@ -41,14 +48,10 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Block-reduce K values, publish one partial per block, and let the last block
-// to arrive sum all partials in a fixed order.  Returns true in every thread
-// of the last block, with tot[] holding the totals.
+// Block-reduce K values and store the block's partial sums (fixed tree).
 template <int K>
-__device__ bool reduce_publish(double (&v)[K], double* part, int pstride, int pbase, int nparts_total,
-                               bool finish, uint32_t* ticket, int nblk, double (&tot)[K]) {
+__device__ __forceinline__ void block_partials(double (&v)[K], double* part, int slot) {
   __shared__ double red[K][kWavesPerBlock];
-  __shared__ int is_last;
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -62,44 +65,39 @@ __device__ bool reduce_publish(double (&v)[K], double* part, int pstride, int pb
       double s = red[k][0];
 #pragma unroll
       for (int w = 1; w < kWavesPerBlock; ++w) s += red[k][w];
-      part[k * pstride + pbase + blockIdx.x] = s;
-    }
-    is_last = 0;
-    if (finish) {
-      __threadfence();
-      const uint32_t t = atomicAdd(ticket, 1u);
-      is_last = (t == (uint32_t)(nblk - 1));
+      part[k * kMaxPartials + slot] = s;
     }
   }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
+}
+
+// Sum nparts partials of K components in a fixed order (one workgroup).
+template <int K>
+__device__ __forceinline__ void sum_partials(const double* part, int nparts, double (&tot)[K]) {
+  __shared__ double red[K][kWavesPerBlock];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     double s = 0.0;
-    for (int i = threadIdx.x; i < nparts_total; i += blockDim.x) {
-      s += __builtin_nontemporal_load(part + k * pstride + i);
-    }
+    for (int i = threadIdx.x; i < nparts; i += kBlock) s += part[k * kMaxPartials + i];
     s = wave_sum(s);
-    __syncthreads();
     if (lane == 0) red[k][wave] = s;
-    __syncthreads();
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
     double t = red[k][0];
 #pragma unroll
     for (int w = 1; w < kWavesPerBlock; ++w) t += red[k][w];
     tot[k] = t;
   }
-  if (threadIdx.x == 0) *ticket = 0u;
-  return true;
 }
 
-// ---- scalar logic run by one thread of the last block ------------------------
+// ---- scalar logic of the Krylov loops ------------------------------------------
 enum Tail : int {
   kTailStore = 0,      // out[0..K) = totals
   kTailCgRho = 1,      // hecmw_solver_CG.f90:179-207
   kTailCgC1 = 2,       // :221-232
   kTailDnrm2 = 3,      // :248-250, 267 (recurrence residual)
-  kTailTrue = 4,       // :242-275 (true residual, convergence, loop end)
   kTailBiRho = 5,      // hecmw_solver_BiCGSTAB.f90:169-176
   kTailBiC2 = 6,       // :200-202
   kTailBiOmega = 7,    // :227-234
@@ -139,17 +137,12 @@ __device__ void run_tail(KrylovState* st, const double (&tot)[K], double* out, d
       st->alpha = st->rho / c1;
     }
   } else if (TAIL == kTailDnrm2) {
+    // iterations that recompute the true residual produce no recurrence partials
+    if ((st->iter % st->nrec) == 0) return;
     st->dnrm2 = tot[0];
     st->resid = sqrt(tot[0] / st->bnrm2);
     if (hist) hist[st->iter - 1] = st->resid;
     if (st->resid <= st->tol) st->recheck = 1;
-  } else if (TAIL == kTailTrue) {
-    st->dnrm2 = tot[0];
-    st->resid = sqrt(tot[0] / st->bnrm2);
-    const bool periodic = (st->iter % st->nrec) == 0;
-    if (periodic && hist) hist[st->iter - 1] = st->resid;
-    if (st->resid <= st->tol) st->done = 1;
-    st->recheck = 0;
   } else if (TAIL == kTailBiRho) {
     st->rho = tot[0];
     if (st->iter > 1) st->beta = (st->rho / st->rho1) * (st->alpha / st->omega);
@@ -163,26 +156,56 @@ __device__ void run_tail(KrylovState* st, const double (&tot)[K], double* out, d
   }
 }
 
-// End of one Krylov iteration (hecmw_solver_CG.f90:275-277): NOCONV at MAXIT
-// (the Fortran DO variable then reads MAXIT+1), else rho1 = rho, iter += 1.
-__device__ void end_iteration(KrylovState* st) {
+__device__ __forceinline__ bool true_resid_active(const KrylovState* st) {
+  return (st->iter % st->nrec) == 0 || st->recheck;
+}
+
+template <int TAIL, int K>
+__global__ __launch_bounds__(kBlock) void k_tail(KrylovState* st, const double* part, int nparts, double* out,
+                                                 double* hist) {
+  if (TAIL != kTailStore && st->done) return;
+  double tot[K];
+  sum_partials<K>(part, nparts, tot);
+  if (threadIdx.x == 0) run_tail<TAIL, K>(st, tot, out, hist);
+}
+
+// True residual bookkeeping and the end of one iteration
+// (hecmw_solver_CG.f90:242-277): convergence after a recomputed residual,
+// NOCONV at MAXIT (the Fortran DO variable then reads MAXIT+1), else
+// rho1 = rho and the next iteration.
+__global__ __launch_bounds__(kBlock) void k_tail_true(KrylovState* st, const double* part, int nparts,
+                                                      double* hist) {
   if (st->done) return;
-  if (st->iter >= st->maxit) {
-    st->error = kErrNoconv;
-    st->done = 1;
-    st->iter = st->maxit + 1;
-  } else {
-    st->rho1 = st->rho;
-    st->iter += 1;
+  if (true_resid_active(st)) {
+    double tot[1];
+    sum_partials<1>(part, nparts, tot);
+    if (threadIdx.x == 0) {
+      st->dnrm2 = tot[0];
+      st->resid = sqrt(tot[0] / st->bnrm2);
+      if ((st->iter % st->nrec) == 0 && hist) hist[st->iter - 1] = st->resid;
+      if (st->resid <= st->tol) st->done = 1;
+      st->recheck = 0;
+    }
+  }
+  if (threadIdx.x == 0 && !st->done) {
+    if (st->iter >= st->maxit) {
+      st->error = kErrNoconv;
+      st->done = 1;
+      st->iter = st->maxit + 1;
+    } else {
+      st->rho1 = st->rho;
+      st->iter += 1;
+    }
   }
 }
 
-// ---- SELL-64 block row product -----------------------------------------------
-// FP64: reference order, D block first then L then U in storage order,
-// y = ((y + a1*x1) + a2*x2) + a3*x3 per row, no fused multiply-add.
-template <typename VT, typename XT>
-__device__ __forceinline__ void sell_row(const Sell<VT>& a, int s, int lane, const XT* __restrict__ x, double (&y)[3],
-                                         double (&xo)[3]) {
+// ---- SELL-64 block row product ----------------------------------------------
+// FP64, reference order: the D block first, then L, then U in storage order;
+// y = ((y + a1*x1) + a2*x2) + a3*x3, no fused multiply-add.  Loads of U
+// blocks are issued together before they are accumulated in order.
+template <int U>
+__device__ __forceinline__ void sell_row(const Sell<double>& a, int s, int lane, const double* __restrict__ x,
+                                         double (&y)[3], double (&xo)[3]) {
 #pragma clang fp contract(off)
   const int base = a.ptr[s];
   const int len = a.len[s * kWave + lane];
@@ -190,36 +213,50 @@ __device__ __forceinline__ void sell_row(const Sell<VT>& a, int s, int lane, con
   xo[0] = xo[1] = xo[2] = 0.0;
   if (len == 0) return;
   const int32_t* __restrict__ colp = a.col + (int64_t)base * kWave + lane;
-  const VT* __restrict__ vp = a.val + (int64_t)base * 9 * kWave + lane;
+  const double* __restrict__ vp = a.val + (int64_t)base * 9 * kWave + lane;
   {
     const int c = colp[0];
+    double v[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) v[e] = vp[e * kWave];
     const double x0 = x[3 * (int64_t)c], x1 = x[3 * (int64_t)c + 1], x2 = x[3 * (int64_t)c + 2];
     xo[0] = x0;
     xo[1] = x1;
     xo[2] = x2;
-    double v[9];
-#pragma unroll
-    for (int e = 0; e < 9; ++e) v[e] = vp[e * kWave];
     y[0] = v[0] * x0 + v[1] * x1 + v[2] * x2;
     y[1] = v[3] * x0 + v[4] * x1 + v[5] * x2;
     y[2] = v[6] * x0 + v[7] * x1 + v[8] * x2;
   }
-#pragma unroll 2
-  for (int k = 1; k < len; ++k) {
-    const int c = colp[(int64_t)k * kWave];
-    const double x0 = x[3 * (int64_t)c], x1 = x[3 * (int64_t)c + 1], x2 = x[3 * (int64_t)c + 2];
-    const VT* __restrict__ vk = vp + (int64_t)k * 9 * kWave;
-    double v[9];
+  for (int k0 = 1; k0 < len; k0 += U) {
+    int c[U];
 #pragma unroll
-    for (int e = 0; e < 9; ++e) v[e] = vk[e * kWave];
-    y[0] = y[0] + v[0] * x0 + v[1] * x1 + v[2] * x2;
-    y[1] = y[1] + v[3] * x0 + v[4] * x1 + v[5] * x2;
-    y[2] = y[2] + v[6] * x0 + v[7] * x1 + v[8] * x2;
+    for (int j = 0; j < U; ++j) c[j] = (k0 + j < len) ? colp[(int64_t)(k0 + j) * kWave] : 0;
+    double v[U][9], xv[U][3];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      if (k0 + j < len) {
+        const double* __restrict__ vk = vp + (int64_t)(k0 + j) * 9 * kWave;
+#pragma unroll
+        for (int e = 0; e < 9; ++e) v[j][e] = vk[e * kWave];
+        xv[j][0] = x[3 * (int64_t)c[j]];
+        xv[j][1] = x[3 * (int64_t)c[j] + 1];
+        xv[j][2] = x[3 * (int64_t)c[j] + 2];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      if (k0 + j < len) {
+        y[0] = y[0] + v[j][0] * xv[j][0] + v[j][1] * xv[j][1] + v[j][2] * xv[j][2];
+        y[1] = y[1] + v[j][3] * xv[j][0] + v[j][4] * xv[j][1] + v[j][5] * xv[j][2];
+        y[2] = y[2] + v[j][6] * xv[j][0] + v[j][7] * xv[j][1] + v[j][8] * xv[j][2];
+      }
+    }
   }
 }
 
 // FP32 values and vectors, FP32 accumulation with FMA (mixed-precision inner
 // solves only).
+template <int U>
 __device__ __forceinline__ void sell_row_f32(const Sell<float>& a, int s, int lane, const float* __restrict__ x,
                                              float (&y)[3]) {
   const int base = a.ptr[s];
@@ -227,21 +264,34 @@ __device__ __forceinline__ void sell_row_f32(const Sell<float>& a, int s, int la
   y[0] = y[1] = y[2] = 0.0f;
   const int32_t* __restrict__ colp = a.col + (int64_t)base * kWave + lane;
   const float* __restrict__ vp = a.val + (int64_t)base * 9 * kWave + lane;
-#pragma unroll 2
-  for (int k = 0; k < len; ++k) {
-    const int c = colp[(int64_t)k * kWave];
-    const float x0 = x[3 * (int64_t)c], x1 = x[3 * (int64_t)c + 1], x2 = x[3 * (int64_t)c + 2];
-    const float* __restrict__ vk = vp + (int64_t)k * 9 * kWave;
-    float v[9];
+  for (int k0 = 0; k0 < len; k0 += U) {
+    int c[U];
 #pragma unroll
-    for (int e = 0; e < 9; ++e) v[e] = vk[e * kWave];
-    y[0] = fmaf(v[2], x2, fmaf(v[1], x1, fmaf(v[0], x0, y[0])));
-    y[1] = fmaf(v[5], x2, fmaf(v[4], x1, fmaf(v[3], x0, y[1])));
-    y[2] = fmaf(v[8], x2, fmaf(v[7], x1, fmaf(v[6], x0, y[2])));
+    for (int j = 0; j < U; ++j) c[j] = (k0 + j < len) ? colp[(int64_t)(k0 + j) * kWave] : 0;
+    float v[U][9], xv[U][3];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      if (k0 + j < len) {
+        const float* __restrict__ vk = vp + (int64_t)(k0 + j) * 9 * kWave;
+#pragma unroll
+        for (int e = 0; e < 9; ++e) v[j][e] = vk[e * kWave];
+        xv[j][0] = x[3 * (int64_t)c[j]];
+        xv[j][1] = x[3 * (int64_t)c[j] + 1];
+        xv[j][2] = x[3 * (int64_t)c[j] + 2];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      if (k0 + j < len) {
+        y[0] = fmaf(v[j][2], xv[j][2], fmaf(v[j][1], xv[j][1], fmaf(v[j][0], xv[j][0], y[0])));
+        y[1] = fmaf(v[j][5], xv[j][2], fmaf(v[j][4], xv[j][1], fmaf(v[j][3], xv[j][0], y[1])));
+        y[2] = fmaf(v[j][8], xv[j][2], fmaf(v[j][7], xv[j][1], fmaf(v[j][6], xv[j][0], y[2])));
+      }
+    }
   }
 }
 
-// Slices handled by this wavefront: contiguous chunk after XCD remapping.
+// Slices handled by this wavefront: a contiguous chunk after XCD remapping.
 struct SliceRange {
   int begin, end;
 };
@@ -261,34 +311,66 @@ int spmv_grid(int nslices) {
   return g < 16384 ? (g < 1 ? 1 : g) : 16384;
 }
 
+int vec_grid(int64_t n) {
+  int64_t g = (n + kBlock - 1) / kBlock;
+  if (g > 4096) g = 4096;
+  return g < 1 ? 1 : (int)g;
+}
+
+constexpr int kSpmvChunk = 4;
+constexpr int kSsorChunk = 8;
+
 // ---- SpMV kernels --------------------------------------------------------------
-enum SpmvMode : int {
-  kSpmvPlain = 0,   // y = A x
-  kSpmvResid = 1,   // y = b - A x
+enum SpmvEpi : int {
+  kEpiStore = 0,   // y = A x
+  kEpiResid = 1,   // y = b - A x
+  kEpiDotX = 2,    // y = A x, partial x.y            (CG: q = A p, p.q)
+  kEpiDotW = 3,    // y = A x, partial w.y            (BiCGSTAB: v = A pt, rt.v)
+  kEpiDot2W = 4,   // y = A x, partials w.y and y.y   (BiCGSTAB: t = A st, t.s, t.t)
+  kEpiTrue = 5,    // y = b - A x, partial y.y, only in residual-recompute iterations
 };
 
-template <int MODE>
+template <int EPI>
 __global__ __launch_bounds__(kBlock) void k_spmv(Sell<double> a, const double* __restrict__ x,
-                                                   const double* __restrict__ b, double* __restrict__ y) {
+                                                   const double* __restrict__ bw, double* __restrict__ y,
+                                                   const KrylovState* st, double* part) {
 #pragma clang fp contract(off)
+  if (st) {
+    if (st->done) return;
+    if (EPI == kEpiTrue && !true_resid_active(st)) return;
+  }
+  constexpr int K = EPI == kEpiDot2W ? 2 : 1;
   const int lane = threadIdx.x & (kWave - 1);
   const SliceRange sr = wave_slices(a.nslices);
+  double d[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) d[k] = 0.0;
   for (int s = sr.begin; s < sr.end; ++s) {
     double acc[3], xo[3];
-    sell_row(a, s, lane, x, acc, xo);
+    sell_row<kSpmvChunk>(a, s, lane, x, acc, xo);
     const int64_t row = (int64_t)s * kWave + lane;
     if (row < a.nrows) {
-      if (MODE == kSpmvResid) {
-        y[3 * row] = b[3 * row] - acc[0];
-        y[3 * row + 1] = b[3 * row + 1] - acc[1];
-        y[3 * row + 2] = b[3 * row + 2] - acc[2];
+      double o[3];
+      if (EPI == kEpiResid || EPI == kEpiTrue) {
+        o[0] = bw[3 * row] - acc[0];
+        o[1] = bw[3 * row + 1] - acc[1];
+        o[2] = bw[3 * row + 2] - acc[2];
       } else {
-        y[3 * row] = acc[0];
-        y[3 * row + 1] = acc[1];
-        y[3 * row + 2] = acc[2];
+        o[0] = acc[0];
+        o[1] = acc[1];
+        o[2] = acc[2];
       }
+      y[3 * row] = o[0];
+      y[3 * row + 1] = o[1];
+      y[3 * row + 2] = o[2];
+      if (EPI == kEpiDotX) d[0] += xo[0] * o[0] + xo[1] * o[1] + xo[2] * o[2];
+      if (EPI == kEpiDotW || EPI == kEpiDot2W)
+        d[0] += bw[3 * row] * o[0] + bw[3 * row + 1] * o[1] + bw[3 * row + 2] * o[2];
+      if (EPI == kEpiDot2W) d[K - 1] += o[0] * o[0] + o[1] * o[1] + o[2] * o[2];
+      if (EPI == kEpiTrue) d[0] += o[0] * o[0] + o[1] * o[1] + o[2] * o[2];
     }
   }
+  if (EPI >= kEpiDotX) block_partials<K>(d, part, blockIdx.x);
 }
 
 __global__ __launch_bounds__(kBlock) void k_spmv_f32(Sell<float> a, const float* __restrict__ x,
@@ -297,7 +379,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_f32(Sell<float> a, const float*
   const SliceRange sr = wave_slices(a.nslices);
   for (int s = sr.begin; s < sr.end; ++s) {
     float acc[3];
-    sell_row_f32(a, s, lane, x, acc);
+    sell_row_f32<kSpmvChunk>(a, s, lane, x, acc);
     const int64_t row = (int64_t)s * kWave + lane;
     if (row < a.nrows) {
       y[3 * row] = acc[0];
@@ -307,79 +389,21 @@ __global__ __launch_bounds__(kBlock) void k_spmv_f32(Sell<float> a, const float*
   }
 }
 
-// CG: q = A p and the partial sums of p.q; tail computes alpha.
-__global__ __launch_bounds__(kBlock) void k_cg_spmv_dot(Sell<double> a, const double* __restrict__ p,
-                                                          double* __restrict__ q, KrylovState* st, double* part) {
-#pragma clang fp contract(off)
-  if (st->done) return;
-  const int lane = threadIdx.x & (kWave - 1);
-  const SliceRange sr = wave_slices(a.nslices);
-  double d[1] = {0.0};
-  for (int s = sr.begin; s < sr.end; ++s) {
-    double acc[3], xo[3];
-    sell_row(a, s, lane, p, acc, xo);
-    const int64_t row = (int64_t)s * kWave + lane;
-    if (row < a.nrows) {
-      q[3 * row] = acc[0];
-      q[3 * row + 1] = acc[1];
-      q[3 * row + 2] = acc[2];
-      d[0] += xo[0] * acc[0] + xo[1] * acc[1] + xo[2] * acc[2];
-    }
-  }
-  double tot[1];
-  if (reduce_publish<1>(d, part, kMaxPartials, 0, gridDim.x, true, &st->ticket[0], gridDim.x, tot) &&
-      threadIdx.x == 0)
-    run_tail<kTailCgC1, 1>(st, tot, nullptr, nullptr);
-}
-
-// True residual r = b - A x with its squared norm; runs only in iterations
-// that recompute the residual (every nrec-th, or a convergence recheck).
-// Ends the iteration.
-__global__ __launch_bounds__(kBlock) void k_true_resid(Sell<double> a, const double* __restrict__ x,
-                                                         const double* __restrict__ b, double* __restrict__ r,
-                                                         KrylovState* st, double* part, double* hist) {
-#pragma clang fp contract(off)
-  if (st->done) return;
-  const bool active = (st->iter % st->nrec) == 0 || st->recheck;
-  if (!active) {
-    // Every block has read the state above before the last one to arrive
-    // advances the iteration.
-    if (threadIdx.x == 0) {
-      __threadfence();
-      const uint32_t t = atomicAdd(&st->ticket[1], 1u);
-      if (t == (uint32_t)(gridDim.x - 1)) {
-        __threadfence();
-        end_iteration(st);
-        st->ticket[1] = 0u;
-      }
-    }
-    return;
-  }
-  const int lane = threadIdx.x & (kWave - 1);
-  const SliceRange sr = wave_slices(a.nslices);
-  double d[1] = {0.0};
-  for (int s = sr.begin; s < sr.end; ++s) {
-    double acc[3], xo[3];
-    sell_row(a, s, lane, x, acc, xo);
-    const int64_t row = (int64_t)s * kWave + lane;
-    if (row < a.nrows) {
-      const double r0 = b[3 * row] - acc[0], r1 = b[3 * row + 1] - acc[1], r2 = b[3 * row + 2] - acc[2];
-      r[3 * row] = r0;
-      r[3 * row + 1] = r1;
-      r[3 * row + 2] = r2;
-      d[0] += r0 * r0 + r1 * r1 + r2 * r2;
-    }
-  }
-  double tot[1];
-  if (reduce_publish<1>(d, part, kMaxPartials, 0, gridDim.x, true, &st->ticket[1], gridDim.x, tot) &&
-      threadIdx.x == 0) {
-    run_tail<kTailTrue, 1>(st, tot, nullptr, hist);
-    end_iteration(st);
-  }
-}
-
 // ---- SSOR sweeps (one launch per colour) ---------------------------------------
-// Forward, colour rows independent: z[i] = LU^-1 (r[i] - sum_{j in L'(i)} L'_ij z[j])
+// LU solve with the factors of hecmw_precond_SSOR_33.F90:183-195 (inverted
+// pivots on the diagonal, multipliers below): the reference's statements.
+template <typename T>
+__device__ __forceinline__ void lu3_solve(const T (&a)[9], T& x0, T& x1, T& x2) {
+#pragma clang fp contract(off)
+  x1 = x1 - a[3] * x0;
+  x2 = x2 - a[6] * x0 - a[7] * x1;
+  x2 = a[8] * x2;
+  x1 = a[4] * (x1 - a[5] * x2);
+  x0 = a[0] * (x0 - a[2] * x2 - a[1] * x1);
+}
+
+// Forward, rows of one colour independent:
+//   z[i] = LU^-1 (r[i] - sum_{j in L'(i), ascending} L'_ij z[j])
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_ssor_fwd(Sell<T> L, const int32_t* __restrict__ row_old,
                                                        const T* __restrict__ alu, int s0, int ns,
@@ -394,49 +418,55 @@ __global__ __launch_bounds__(kBlock) void k_ssor_fwd(Sell<T> L, const int32_t* _
     const int64_t slot = (int64_t)s * kWave + lane;
     const int iold = row_old[slot];
     if (iold < 0) continue;
-    T sw0 = rin[3 * (int64_t)iold], sw1 = rin[3 * (int64_t)iold + 1], sw2 = rin[3 * (int64_t)iold + 2];
     const int len = L.len[slot];
     const int base = L.ptr[s];
     const int32_t* __restrict__ colp = L.col + (int64_t)base * kWave + lane;
     const T* __restrict__ vp = L.val + (int64_t)base * 9 * kWave + lane;
-#pragma unroll 2
-    for (int k = 0; k < len; ++k) {
-      const int c = colp[(int64_t)k * kWave];
-      const T x0 = z[3 * (int64_t)c], x1 = z[3 * (int64_t)c + 1], x2 = z[3 * (int64_t)c + 2];
-      const T* __restrict__ vk = vp + (int64_t)k * 9 * kWave;
-      T v[9];
-#pragma unroll
-      for (int e = 0; e < 9; ++e) v[e] = vk[e * kWave];
-      sw0 = sw0 - v[0] * x0 - v[1] * x1 - v[2] * x2;
-      sw1 = sw1 - v[3] * x0 - v[4] * x1 - v[5] * x2;
-      sw2 = sw2 - v[6] * x0 - v[7] * x1 - v[8] * x2;
-    }
-    const T* __restrict__ f = alu + (int64_t)s * 9 * kWave + lane;
+    T sw0 = rin[3 * (int64_t)iold], sw1 = rin[3 * (int64_t)iold + 1], sw2 = rin[3 * (int64_t)iold + 2];
     T a[9];
+    const T* __restrict__ f = alu + (int64_t)s * 9 * kWave + lane;
 #pragma unroll
     for (int e = 0; e < 9; ++e) a[e] = f[e * kWave];
-    T x0 = sw0, x1 = sw1, x2 = sw2;
-    x1 = x1 - a[3] * x0;
-    x2 = x2 - a[6] * x0 - a[7] * x1;
-    x2 = a[8] * x2;
-    x1 = a[4] * (x1 - a[5] * x2);
-    x0 = a[0] * (x0 - a[2] * x2 - a[1] * x1);
-    z[3 * (int64_t)iold] = x0;
-    z[3 * (int64_t)iold + 1] = x1;
-    z[3 * (int64_t)iold + 2] = x2;
+    for (int k0 = 0; k0 < len; k0 += kSsorChunk) {
+      int c[kSsorChunk];
+#pragma unroll
+      for (int j = 0; j < kSsorChunk; ++j) c[j] = (k0 + j < len) ? colp[(int64_t)(k0 + j) * kWave] : 0;
+      T v[kSsorChunk][9], xv[kSsorChunk][3];
+#pragma unroll
+      for (int j = 0; j < kSsorChunk; ++j) {
+        if (k0 + j < len) {
+          const T* __restrict__ vk = vp + (int64_t)(k0 + j) * 9 * kWave;
+#pragma unroll
+          for (int e = 0; e < 9; ++e) v[j][e] = vk[e * kWave];
+          xv[j][0] = z[3 * (int64_t)c[j]];
+          xv[j][1] = z[3 * (int64_t)c[j] + 1];
+          xv[j][2] = z[3 * (int64_t)c[j] + 2];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < kSsorChunk; ++j) {
+        if (k0 + j < len) {
+          sw0 = sw0 - v[j][0] * xv[j][0] - v[j][1] * xv[j][1] - v[j][2] * xv[j][2];
+          sw1 = sw1 - v[j][3] * xv[j][0] - v[j][4] * xv[j][1] - v[j][5] * xv[j][2];
+          sw2 = sw2 - v[j][6] * xv[j][0] - v[j][7] * xv[j][1] - v[j][8] * xv[j][2];
+        }
+      }
+    }
+    lu3_solve<T>(a, sw0, sw1, sw2);
+    z[3 * (int64_t)iold] = sw0;
+    z[3 * (int64_t)iold + 1] = sw1;
+    z[3 * (int64_t)iold + 2] = sw2;
   }
 }
 
-// Backward, colours in reverse: z[i] -= LU^-1 sum_{j in U'(i), descending} U'_ij z[j].
-// With RHO, also accumulates r.z over this colour's rows (their z is final
-// now) into part[pbase + block]; the launch with `finish` sums every colour's
-// partials and runs the CG rho logic.
+// Backward, colours in reverse:
+//   z[i] -= LU^-1 sum_{j in U'(i), descending} U'_ij z[j]
+// With RHO, r.z over this colour's rows (final now) goes to part[pbase+block].
 template <typename T, bool RHO>
 __global__ __launch_bounds__(kBlock) void k_ssor_bwd(Sell<T> U, const int32_t* __restrict__ row_old,
                                                        const T* __restrict__ alu, int s0, int ns,
                                                        T* __restrict__ z, const double* __restrict__ r,
-                                                       KrylovState* st, double* part, int pbase, int ptotal,
-                                                       int finish) {
+                                                       const KrylovState* st, double* part, int pbase) {
 #pragma clang fp contract(off)
   if (st && st->done) return;
   const int lane = threadIdx.x & (kWave - 1);
@@ -447,34 +477,44 @@ __global__ __launch_bounds__(kBlock) void k_ssor_bwd(Sell<T> U, const int32_t* _
     const int64_t slot = (int64_t)s * kWave + lane;
     const int iold = row_old[slot];
     if (iold < 0) continue;
-    T sw0 = 0, sw1 = 0, sw2 = 0;
     const int len = U.len[slot];
     const int base = U.ptr[s];
     const int32_t* __restrict__ colp = U.col + (int64_t)base * kWave + lane;
     const T* __restrict__ vp = U.val + (int64_t)base * 9 * kWave + lane;
-#pragma unroll 2
-    for (int k = len - 1; k >= 0; --k) {
-      const int c = colp[(int64_t)k * kWave];
-      const T x0 = z[3 * (int64_t)c], x1 = z[3 * (int64_t)c + 1], x2 = z[3 * (int64_t)c + 2];
-      const T* __restrict__ vk = vp + (int64_t)k * 9 * kWave;
-      T v[9];
-#pragma unroll
-      for (int e = 0; e < 9; ++e) v[e] = vk[e * kWave];
-      sw0 = sw0 + v[0] * x0 + v[1] * x1 + v[2] * x2;
-      sw1 = sw1 + v[3] * x0 + v[4] * x1 + v[5] * x2;
-      sw2 = sw2 + v[6] * x0 + v[7] * x1 + v[8] * x2;
-    }
-    const T* __restrict__ f = alu + (int64_t)s * 9 * kWave + lane;
     T a[9];
+    const T* __restrict__ f = alu + (int64_t)s * 9 * kWave + lane;
 #pragma unroll
     for (int e = 0; e < 9; ++e) a[e] = f[e * kWave];
-    T x0 = sw0, x1 = sw1, x2 = sw2;
-    x1 = x1 - a[3] * x0;
-    x2 = x2 - a[6] * x0 - a[7] * x1;
-    x2 = a[8] * x2;
-    x1 = a[4] * (x1 - a[5] * x2);
-    x0 = a[0] * (x0 - a[2] * x2 - a[1] * x1);
-    const T z0 = z[3 * (int64_t)iold] - x0, z1 = z[3 * (int64_t)iold + 1] - x1, z2 = z[3 * (int64_t)iold + 2] - x2;
+    T zo0 = z[3 * (int64_t)iold], zo1 = z[3 * (int64_t)iold + 1], zo2 = z[3 * (int64_t)iold + 2];
+    T sw0 = 0, sw1 = 0, sw2 = 0;
+    // descending k: chunk j covers k = top - j
+    for (int top = len - 1; top >= 0; top -= kSsorChunk) {
+      int c[kSsorChunk];
+#pragma unroll
+      for (int j = 0; j < kSsorChunk; ++j) c[j] = (top - j >= 0) ? colp[(int64_t)(top - j) * kWave] : 0;
+      T v[kSsorChunk][9], xv[kSsorChunk][3];
+#pragma unroll
+      for (int j = 0; j < kSsorChunk; ++j) {
+        if (top - j >= 0) {
+          const T* __restrict__ vk = vp + (int64_t)(top - j) * 9 * kWave;
+#pragma unroll
+          for (int e = 0; e < 9; ++e) v[j][e] = vk[e * kWave];
+          xv[j][0] = z[3 * (int64_t)c[j]];
+          xv[j][1] = z[3 * (int64_t)c[j] + 1];
+          xv[j][2] = z[3 * (int64_t)c[j] + 2];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < kSsorChunk; ++j) {
+        if (top - j >= 0) {
+          sw0 = sw0 + v[j][0] * xv[j][0] + v[j][1] * xv[j][1] + v[j][2] * xv[j][2];
+          sw1 = sw1 + v[j][3] * xv[j][0] + v[j][4] * xv[j][1] + v[j][5] * xv[j][2];
+          sw2 = sw2 + v[j][6] * xv[j][0] + v[j][7] * xv[j][1] + v[j][8] * xv[j][2];
+        }
+      }
+    }
+    lu3_solve<T>(a, sw0, sw1, sw2);
+    const T z0 = zo0 - sw0, z1 = zo1 - sw1, z2 = zo2 - sw2;
     z[3 * (int64_t)iold] = z0;
     z[3 * (int64_t)iold + 1] = z1;
     z[3 * (int64_t)iold + 2] = z2;
@@ -483,19 +523,14 @@ __global__ __launch_bounds__(kBlock) void k_ssor_bwd(Sell<T> U, const int32_t* _
               r[3 * (int64_t)iold + 2] * (double)z2;
     }
   }
-  if (RHO) {
-    double tot[1];
-    if (reduce_publish<1>(d, part, kMaxPartials, pbase, ptotal, finish != 0, &st->ticket[2], gridDim.x, tot) &&
-        threadIdx.x == 0)
-      run_tail<kTailCgRho, 1>(st, tot, nullptr, nullptr);
-  }
+  if (RHO) block_partials<1>(d, part, pbase + blockIdx.x);
 }
 
-// ---- DIAG (block Jacobi) --------------------------------------------------------
+// ---- DIAG (block Jacobi), hecmw_precond_DIAG_33.F90:106-142 --------------------
 template <bool RHO>
 __global__ __launch_bounds__(kBlock) void k_diag(int32_t n, const double* __restrict__ alu,
                                                    const double* __restrict__ r, double* __restrict__ z,
-                                                   KrylovState* st, double* part) {
+                                                   const KrylovState* st, double* part) {
 #pragma clang fp contract(off)
   if (st && st->done) return;
   const int blk = xcd_block(blockIdx.x, gridDim.x);
@@ -508,61 +543,53 @@ __global__ __launch_bounds__(kBlock) void k_diag(int32_t n, const double* __rest
     for (int e = 0; e < 9; ++e) a[e] = f[e * kWave];
     const double r0 = r[3 * i], r1 = r[3 * i + 1], r2 = r[3 * i + 2];
     double x0 = r0, x1 = r1, x2 = r2;
-    x1 = x1 - a[3] * x0;
-    x2 = x2 - a[6] * x0 - a[7] * x1;
-    x2 = a[8] * x2;
-    x1 = a[4] * (x1 - a[5] * x2);
-    x0 = a[0] * (x0 - a[2] * x2 - a[1] * x1);
+    lu3_solve<double>(a, x0, x1, x2);
     z[3 * i] = x0;
     z[3 * i + 1] = x1;
     z[3 * i + 2] = x2;
     if (RHO) d[0] += r0 * x0 + r1 * x1 + r2 * x2;
   }
-  if (RHO) {
-    double tot[1];
-    if (reduce_publish<1>(d, part, kMaxPartials, 0, gridDim.x, true, &st->ticket[2], gridDim.x, tot) &&
-        threadIdx.x == 0)
-      run_tail<kTailCgRho, 1>(st, tot, nullptr, nullptr);
-  }
+  if (RHO) block_partials<1>(d, part, blockIdx.x);
 }
 
 // ---- vector kernels -----------------------------------------------------------
-int vec_grid(int64_t n) {
-  int64_t g = (n + kBlock - 1) / kBlock;
-  if (g > 4096) g = 4096;
-  return g < 1 ? 1 : (int)g;
-}
-
-// z = r (no preconditioning) with rho = r.z
-__global__ __launch_bounds__(kBlock) void k_copy_rho(int64_t n, const double* __restrict__ r, double* __restrict__ z,
-                                                       KrylovState* st, double* part) {
+// z = r with partial r.r (no preconditioning: rho = r.z)
+__global__ __launch_bounds__(kBlock) void k_copy_dot(int64_t n, const double* __restrict__ r, double* __restrict__ z,
+                                                       const KrylovState* st, double* part, int with_dot) {
 #pragma clang fp contract(off)
-  if (st->done) return;
+  if (st && st->done) return;
   double d[1] = {0.0};
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
     const double v = r[i];
     z[i] = v;
     d[0] += v * v;
   }
-  double tot[1];
-  if (reduce_publish<1>(d, part, kMaxPartials, 0, gridDim.x, true, &st->ticket[2], gridDim.x, tot) &&
-      threadIdx.x == 0)
-    run_tail<kTailCgRho, 1>(st, tot, nullptr, nullptr);
+  if (with_dot) block_partials<1>(d, part, blockIdx.x);
 }
 
-// generic dot with a selectable tail
-template <int TAIL>
-__global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
-                                                  KrylovState* st, double* part, double* out, int guard) {
+// x = 0 / y += x over n entries (the ITERPREMAX > 1 accumulation of
+// hecmw_precond.f90:120-165)
+__global__ __launch_bounds__(kBlock) void k_zero(int64_t n, double* __restrict__ x, const KrylovState* st) {
+  if (st && st->done) return;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) x[i] = 0.0;
+}
+__global__ __launch_bounds__(kBlock) void k_add(int64_t n, const double* __restrict__ x, double* __restrict__ y,
+                                                  const KrylovState* st) {
 #pragma clang fp contract(off)
-  if (guard && st->done) return;
+  if (st && st->done) return;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    y[i] = y[i] + x[i];
+}
+
+// partials of x.y
+__global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
+                                                  const KrylovState* st, double* part) {
+#pragma clang fp contract(off)
+  if (st && st->done) return;
   double d[1] = {0.0};
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
     d[0] += x[i] * y[i];
-  double tot[1];
-  if (reduce_publish<1>(d, part, kMaxPartials, 0, gridDim.x, true, &st->ticket[3], gridDim.x, tot) &&
-      threadIdx.x == 0)
-    run_tail<TAIL, 1>(st, tot, out, nullptr);
+  block_partials<1>(d, part, blockIdx.x);
 }
 
 // p = z (first iteration) or p = z + beta p (hecmw_xpay_R: Y = X + a*Y)
@@ -576,12 +603,11 @@ __global__ __launch_bounds__(kBlock) void k_cg_xpay(int64_t n, const double* __r
     p[i] = first ? z[i] : z[i] + beta * p[i];
 }
 
-// x += alpha p; r -= alpha q (hecmw_axpy_R: Y = a*X + Y) and r.r, except in
-// iterations that recompute the true residual (there only x is updated).
+// x += alpha p; r -= alpha q (hecmw_axpy_R: Y = a*X + Y) with partial r.r,
+// except in iterations that recompute the true residual (x only).
 __global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, const double* __restrict__ p,
                                                         const double* __restrict__ q, double* __restrict__ x,
-                                                        double* __restrict__ r, KrylovState* st, double* part,
-                                                        double* hist) {
+                                                        double* __restrict__ r, const KrylovState* st, double* part) {
 #pragma clang fp contract(off)
   if (st->done) return;
   const double alpha = st->alpha;
@@ -595,11 +621,58 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, const double* _
       d[0] += v * v;
     }
   }
-  if (periodic) return;
-  double tot[1];
-  if (reduce_publish<1>(d, part, kMaxPartials, 0, gridDim.x, true, &st->ticket[3], gridDim.x, tot) &&
-      threadIdx.x == 0)
-    run_tail<kTailDnrm2, 1>(st, tot, nullptr, hist);
+  if (!periodic) block_partials<1>(d, part, blockIdx.x);
+}
+
+// BiCGSTAB p (hecmw_solver_BiCGSTAB.f90:176-182): p = r on the first
+// iteration, else p = -omega v + p (axpy) then p = r + beta p (xpay).
+__global__ __launch_bounds__(kBlock) void k_bi_p(int64_t n, const double* __restrict__ r, const double* __restrict__ v,
+                                                   double* __restrict__ p, const KrylovState* st) {
+#pragma clang fp contract(off)
+  if (st->done) return;
+  const bool first = st->iter == 1;
+  const double beta = st->beta, omega = st->omega;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    if (first) {
+      p[i] = r[i];
+    } else {
+      const double t = -omega * v[i] + p[i];
+      p[i] = r[i] + beta * t;
+    }
+  }
+}
+
+// s = -alpha v + r (hecmw_axpyz_R: Z = a*X + Y)
+__global__ __launch_bounds__(kBlock) void k_bi_s(int64_t n, const double* __restrict__ v, const double* __restrict__ r,
+                                                   double* __restrict__ s, const KrylovState* st) {
+#pragma clang fp contract(off)
+  if (st->done) return;
+  const double alpha = st->alpha;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    s[i] = -alpha * v[i] + r[i];
+}
+
+// x = alpha pt + x; x = omega st + x; r = -omega t + s with partial r.r unless
+// the iteration recomputes the true residual (hecmw_solver_BiCGSTAB.f90:241-249).
+__global__ __launch_bounds__(kBlock) void k_bi_update(int64_t n, const double* __restrict__ pt,
+                                                        const double* __restrict__ stv, const double* __restrict__ t,
+                                                        const double* __restrict__ s, double* __restrict__ x,
+                                                        double* __restrict__ r, const KrylovState* st, double* part) {
+#pragma clang fp contract(off)
+  if (st->done) return;
+  const double alpha = st->alpha, omega = st->omega;
+  const bool periodic = (st->iter % st->nrec) == 0;
+  double d[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const double x1 = alpha * pt[i] + x[i];
+    x[i] = omega * stv[i] + x1;
+    if (!periodic) {
+      const double v = -omega * t[i] + s[i];
+      r[i] = v;
+      d[0] += v * v;
+    }
+  }
+  if (!periodic) block_partials<1>(d, part, blockIdx.x);
 }
 
 // ---- numeric setup ---------------------------------------------------------------
@@ -658,17 +731,19 @@ __global__ void k_block_lu(int32_t nslots, const int32_t* __restrict__ row_src, 
 
 // ---- launchers ---------------------------------------------------------------------
 void launch_spmv(const Sell<double>& a, const double* x, double* y, hipStream_t s) {
-  k_spmv<kSpmvPlain><<<spmv_grid(a.nslices), kBlock, 0, s>>>(a, x, nullptr, y);
+  k_spmv<kEpiStore><<<spmv_grid(a.nslices), kBlock, 0, s>>>(a, x, nullptr, y, nullptr, nullptr);
 }
 void launch_spmv_f32(const Sell<float>& a, const float* x, float* y, hipStream_t s) {
   k_spmv_f32<<<spmv_grid(a.nslices), kBlock, 0, s>>>(a, x, y);
 }
 void launch_matresid(const Sell<double>& a, const double* x, const double* b, double* r, hipStream_t s) {
-  k_spmv<kSpmvResid><<<spmv_grid(a.nslices), kBlock, 0, s>>>(a, x, b, r);
+  k_spmv<kEpiResid><<<spmv_grid(a.nslices), kBlock, 0, s>>>(a, x, b, r, nullptr, nullptr);
 }
 void launch_dot_store(int64_t n, const double* x, const double* y, double* out, KrylovState* st, double* part,
                       hipStream_t s) {
-  k_dot<kTailStore><<<vec_grid(n), kBlock, 0, s>>>(n, x, y, st, part, out, 0);
+  const int g = vec_grid(n);
+  k_dot<<<g, kBlock, 0, s>>>(n, x, y, nullptr, part);
+  k_tail<kTailStore, 1><<<1, kBlock, 0, s>>>(st, part, g, out, nullptr);
 }
 void launch_fill_values(const Sell<double>& a, const int64_t* src, const double* D, const double* AL,
                         const double* AU, int64_t nD, int64_t nAL, hipStream_t s) {
@@ -686,7 +761,6 @@ void launch_block_lu_f32(int32_t nslots, const int32_t* row_src, const double* D
                          hipStream_t s) {
   k_block_lu<float><<<2048, 256, 0, s>>>(nslots, row_src, D, sigma, alu);
 }
-
 void launch_ssor_fwd(const Sell<double>& L, const int32_t* row_old, const double* alu, int32_t s0, int32_t ns,
                      const double* rin, double* z, const KrylovState* st, hipStream_t s) {
   if (ns <= 0) return;
@@ -695,8 +769,7 @@ void launch_ssor_fwd(const Sell<double>& L, const int32_t* row_old, const double
 void launch_ssor_bwd(const Sell<double>& U, const int32_t* row_old, const double* alu, int32_t s0, int32_t ns,
                      double* z, const KrylovState* st, hipStream_t s) {
   if (ns <= 0) return;
-  k_ssor_bwd<double, false><<<spmv_grid(ns), kBlock, 0, s>>>(U, row_old, alu, s0, ns, z, nullptr,
-                                                              const_cast<KrylovState*>(st), nullptr, 0, 0, 0);
+  k_ssor_bwd<double, false><<<spmv_grid(ns), kBlock, 0, s>>>(U, row_old, alu, s0, ns, z, nullptr, st, nullptr, 0);
 }
 void launch_ssor_fwd_f32(const Sell<float>& L, const int32_t* row_old, const float* alu, int32_t s0, int32_t ns,
                          const float* rin, float* z, hipStream_t s) {
@@ -706,12 +779,24 @@ void launch_ssor_fwd_f32(const Sell<float>& L, const int32_t* row_old, const flo
 void launch_ssor_bwd_f32(const Sell<float>& U, const int32_t* row_old, const float* alu, int32_t s0, int32_t ns,
                          float* z, hipStream_t s) {
   if (ns <= 0) return;
-  k_ssor_bwd<float, false><<<spmv_grid(ns), kBlock, 0, s>>>(U, row_old, alu, s0, ns, z, nullptr, nullptr, nullptr,
-                                                             0, 0, 0);
+  k_ssor_bwd<float, false><<<spmv_grid(ns), kBlock, 0, s>>>(U, row_old, alu, s0, ns, z, nullptr, nullptr, nullptr, 0);
 }
 void launch_diag_apply(int32_t n, const double* alu, const double* r, double* z, const KrylovState* st,
                        hipStream_t s) {
-  k_diag<false><<<vec_grid(n), kBlock, 0, s>>>(n, alu, r, z, const_cast<KrylovState*>(st), nullptr);
+  k_diag<false><<<vec_grid(n), kBlock, 0, s>>>(n, alu, r, z, st, nullptr);
+}
+void launch_copy(int64_t n3, const double* r, double* z, const KrylovState* st, hipStream_t s) {
+  k_copy_dot<<<vec_grid(n3), kBlock, 0, s>>>(n3, r, z, st, nullptr, 0);
+}
+void launch_zero(int64_t n, double* x, const KrylovState* st, hipStream_t s) {
+  k_zero<<<vec_grid(n), kBlock, 0, s>>>(n, x, st);
+}
+void launch_add(int64_t n, const double* x, double* y, const KrylovState* st, hipStream_t s) {
+  k_add<<<vec_grid(n), kBlock, 0, s>>>(n, x, y, st);
+}
+void launch_matresid_guarded(const Sell<double>& a, const double* x, const double* b, double* r,
+                             const KrylovState* st, hipStream_t s) {
+  k_spmv<kEpiResid><<<spmv_grid(a.nslices), kBlock, 0, s>>>(a, x, b, r, st, nullptr);
 }
 
 // ---- CG iteration pieces ------------------------------------------------------------
@@ -720,43 +805,101 @@ void cg_precond_ssor_rho(const Sell<double>& L, const Sell<double>& U, const int
                          double* part, hipStream_t s) {
   for (int c = 0; c < ncolor; ++c)
     launch_ssor_fwd(L, row_old, alu, color_slice[c], color_slice[c + 1] - color_slice[c], r, z, st, s);
-  // partial slots per colour launch, laid out in launch order; the grid of
-  // each colour is capped so that all colours fit in the partials buffer
+  // every colour's partials in launch order; grids capped to fit the buffer
   const int gcap = std::max(1, kMaxPartials / std::max(1, ncolor));
-  int ptotal = 0;
-  for (int c = ncolor - 1; c >= 0; --c) ptotal += std::min(gcap, spmv_grid(color_slice[c + 1] - color_slice[c]));
   int pbase = 0;
   for (int c = ncolor - 1; c >= 0; --c) {
     const int ns = color_slice[c + 1] - color_slice[c];
+    if (ns <= 0) continue;
     const int g = std::min(gcap, spmv_grid(ns));
-    k_ssor_bwd<double, true><<<g, kBlock, 0, s>>>(U, row_old, alu, color_slice[c], ns, z, r, st, part, pbase, ptotal,
-                                                  c == 0 ? 1 : 0);
+    k_ssor_bwd<double, true><<<g, kBlock, 0, s>>>(U, row_old, alu, color_slice[c], ns, z, r, st, part, pbase);
     pbase += g;
   }
+  k_tail<kTailCgRho, 1><<<1, kBlock, 0, s>>>(st, part, pbase, nullptr, nullptr);
 }
 void cg_precond_diag_rho(int32_t n, const double* alu, const double* r, double* z, KrylovState* st, double* part,
                          hipStream_t s) {
-  k_diag<true><<<vec_grid(n), kBlock, 0, s>>>(n, alu, r, z, st, part);
+  const int g = vec_grid(n);
+  k_diag<true><<<g, kBlock, 0, s>>>(n, alu, r, z, st, part);
+  k_tail<kTailCgRho, 1><<<1, kBlock, 0, s>>>(st, part, g, nullptr, nullptr);
 }
 void cg_precond_none_rho(int64_t n3, const double* r, double* z, KrylovState* st, double* part, hipStream_t s) {
-  k_copy_rho<<<vec_grid(n3), kBlock, 0, s>>>(n3, r, z, st, part);
+  const int g = vec_grid(n3);
+  k_copy_dot<<<g, kBlock, 0, s>>>(n3, r, z, st, part, 1);
+  k_tail<kTailCgRho, 1><<<1, kBlock, 0, s>>>(st, part, g, nullptr, nullptr);
 }
 void cg_rho(int64_t n3, const double* r, const double* z, KrylovState* st, double* part, hipStream_t s) {
-  k_dot<kTailCgRho><<<vec_grid(n3), kBlock, 0, s>>>(n3, r, z, st, part, nullptr, 1);
+  const int g = vec_grid(n3);
+  k_dot<<<g, kBlock, 0, s>>>(n3, r, z, st, part);
+  k_tail<kTailCgRho, 1><<<1, kBlock, 0, s>>>(st, part, g, nullptr, nullptr);
 }
 void cg_xpay(int64_t n3, const double* z, double* p, const KrylovState* st, hipStream_t s) {
   k_cg_xpay<<<vec_grid(n3), kBlock, 0, s>>>(n3, z, p, st);
 }
 void cg_spmv_dot(const Sell<double>& a, const double* p, double* q, KrylovState* st, double* part, hipStream_t s) {
-  k_cg_spmv_dot<<<spmv_grid(a.nslices), kBlock, 0, s>>>(a, p, q, st, part);
+  const int g = spmv_grid(a.nslices);
+  k_spmv<kEpiDotX><<<g, kBlock, 0, s>>>(a, p, nullptr, q, st, part);
+  k_tail<kTailCgC1, 1><<<1, kBlock, 0, s>>>(st, part, g, nullptr, nullptr);
 }
 void cg_update(int64_t n3, const double* p, const double* q, double* x, double* r, KrylovState* st, double* part,
                double* hist, hipStream_t s) {
-  k_cg_update<<<vec_grid(n3), kBlock, 0, s>>>(n3, p, q, x, r, st, part, hist);
+  const int g = vec_grid(n3);
+  k_cg_update<<<g, kBlock, 0, s>>>(n3, p, q, x, r, st, part);
+  k_tail<kTailDnrm2, 1><<<1, kBlock, 0, s>>>(st, part, g, nullptr, hist);
 }
+// only the non-periodic iterations produce the dnrm2 partials; the tail is
+// guarded the same way
 void cg_true_resid(const Sell<double>& a, const double* x, const double* b, double* r, KrylovState* st, double* part,
                    double* hist, hipStream_t s) {
-  k_true_resid<<<spmv_grid(a.nslices), kBlock, 0, s>>>(a, x, b, r, st, part, hist);
+  const int g = spmv_grid(a.nslices);
+  k_spmv<kEpiTrue><<<g, kBlock, 0, s>>>(a, x, b, r, st, part);
+  k_tail_true<<<1, kBlock, 0, s>>>(st, part, g, hist);
+}
+
+// ---- BiCGSTAB iteration pieces (hecmw_solver_BiCGSTAB.f90:164-275) --------------------
+void bi_rho(int64_t n3, const double* r, const double* rt, KrylovState* st, double* part, hipStream_t s) {
+  const int g = vec_grid(n3);
+  k_dot<<<g, kBlock, 0, s>>>(n3, r, rt, st, part);
+  k_tail<kTailBiRho, 1><<<1, kBlock, 0, s>>>(st, part, g, nullptr, nullptr);
+}
+void bi_p(int64_t n3, const double* r, const double* v, double* p, const KrylovState* st, hipStream_t s) {
+  k_bi_p<<<vec_grid(n3), kBlock, 0, s>>>(n3, r, v, p, st);
+}
+void bi_spmv_c2(const Sell<double>& a, const double* pt, const double* rt, double* v, KrylovState* st, double* part,
+                hipStream_t s) {
+  const int g = spmv_grid(a.nslices);
+  k_spmv<kEpiDotW><<<g, kBlock, 0, s>>>(a, pt, rt, v, st, part);
+  k_tail<kTailBiC2, 1><<<1, kBlock, 0, s>>>(st, part, g, nullptr, nullptr);
+}
+void bi_s(int64_t n3, const double* v, const double* r, double* sv, const KrylovState* st, hipStream_t s) {
+  k_bi_s<<<vec_grid(n3), kBlock, 0, s>>>(n3, v, r, sv, st);
+}
+void bi_spmv_omega(const Sell<double>& a, const double* stv, const double* sv, double* t, KrylovState* st,
+                   double* part, hipStream_t s) {
+  const int g = spmv_grid(a.nslices);
+  k_spmv<kEpiDot2W><<<g, kBlock, 0, s>>>(a, stv, sv, t, st, part);
+  k_tail<kTailBiOmega, 2><<<1, kBlock, 0, s>>>(st, part, g, nullptr, nullptr);
+}
+void bi_update(int64_t n3, const double* pt, const double* stv, const double* t, const double* sv, double* x,
+               double* r, KrylovState* st, double* part, double* hist, hipStream_t s) {
+  const int g = vec_grid(n3);
+  k_bi_update<<<g, kBlock, 0, s>>>(n3, pt, stv, t, sv, x, r, st, part);
+  k_tail<kTailDnrm2, 1><<<1, kBlock, 0, s>>>(st, part, g, nullptr, hist);
+}
+// generic preconditioner application z = M^-1 r for BiCGSTAB (no fused dot)
+void precond_apply(int32_t kind, const Sell<double>& L, const Sell<double>& U, const int32_t* row_old,
+                   const double* alu_ssor, const int32_t* color_slice, int32_t ncolor, const double* alu_diag,
+                   int32_t n, const double* r, double* z, const KrylovState* st, hipStream_t s) {
+  if (kind == 0) {
+    launch_copy(3 * (int64_t)n, r, z, st, s);
+  } else if (kind == 3) {
+    launch_diag_apply(n, alu_diag, r, z, st, s);
+  } else {
+    for (int c = 0; c < ncolor; ++c)
+      launch_ssor_fwd(L, row_old, alu_ssor, color_slice[c], color_slice[c + 1] - color_slice[c], r, z, st, s);
+    for (int c = ncolor - 1; c >= 0; --c)
+      launch_ssor_bwd(U, row_old, alu_ssor, color_slice[c], color_slice[c + 1] - color_slice[c], z, st, s);
+  }
 }
 
 }  // namespace hecmw33

@@ -179,18 +179,30 @@ struct Context {
   bool diag_init = false;
   DMem row_diag, alu_diag;
   // Krylov vectors and state
-  DMem vb, vx, vr, vz, vp, vq, vrt, vs, vt, vv, vpt, vst;
+  DMem vb, vx, vr, vz, vp, vq, vrt, vs, vt, vv, vpt, vst, vzp, vzt;
   DMem st, part, hist;
   KrylovState* h_st = nullptr;  // pinned, 2 slots for polling
   int32_t last_iter = 0;
+  // TIMELOG: event pairs around the preconditioner and the SpMV of every
+  // enqueued iteration (4 events per iteration)
+  std::vector<hipEvent_t> events;
+  hipEvent_t event(size_t i) {
+    while (events.size() <= i) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      events.push_back(e);
+    }
+    return events[i];
+  }
 
   ~Context() {
     if (stream) hipStreamSynchronize(stream);
     for (DMem* m : {&dD, &dAL, &dAU, &row_old, &alu_ssor, &row_diag, &alu_diag, &vb, &vx, &vr, &vz, &vp, &vq, &vrt,
-                    &vs, &vt, &vv, &vpt, &vst, &st, &part, &hist})
+                    &vs, &vt, &vv, &vpt, &vst, &vzp, &vzt, &st, &part, &hist})
       m->release();
     for (SellDev* s : {&A, &L, &U})
       for (DMem* m : {&s->ptr, &s->len, &s->col, &s->src, &s->val, &s->val32}) m->release();
+    for (hipEvent_t e : events) hipEventDestroy(e);
     if (h_st) hipHostFree(h_st);
     if (stream) hipStreamDestroy(stream);
   }
@@ -417,24 +429,90 @@ static int32_t precond_setup(Context& c, int32_t* iarray, const double* rarray) 
   return 0;
 }
 
-// ---- CG ---------------------------------------------------------------------
-static void enqueue_cg_iteration(Context& c, int32_t pc, int32_t iterpremax) {
+// ---- Krylov iterations ----------------------------------------------------------
+// Preconditioner kind as the sweeps see it: 0 none, 1 multicolour SSOR, 3 DIAG.
+static int32_t pc_kind(int32_t pc, int32_t iterpremax) {
+  if (iterpremax <= 0) return 0;
+  return pc == 3 ? 3 : 1;
+}
+
+// z = M^-1 r with hecmw_precond_apply's ITERPREMAX loop (hecmw_precond.f90:114-168):
+// ZP = R; Z = 0; repeat: ZP = M^-1 ZP; Z += ZP; ZP = R - A Z (except after the last).
+static void enqueue_precond(Context& c, int32_t kind, int32_t iterpremax, const double* r, double* z) {
+  KrylovState* st = c.st.as<KrylovState>();
+  const int64_t n3 = 3 * (int64_t)c.N;
+  auto apply = [&](const double* in, double* out) {
+    precond_apply(kind, c.L.view, c.U.view, c.row_old.as<int32_t>(), c.alu_ssor.as<double>(), c.color_slice.data(),
+                  c.ncolor, c.alu_diag.as<double>(), c.N, in, out, st, c.stream);
+  };
+  if (kind == 0 || iterpremax <= 1) {
+    apply(r, z);
+    return;
+  }
+  double* zp = c.vzp.as<double>();
+  double* zt = c.vzt.as<double>();
+  launch_zero(3 * (int64_t)c.NP, z, st, c.stream);
+  launch_copy(n3, r, zp, st, c.stream);
+  for (int it = 1; it <= iterpremax; ++it) {
+    apply(zp, zt);
+    launch_add(n3, zt, z, st, c.stream);
+    if (it == iterpremax) break;
+    launch_matresid_guarded(c.A.view, z, r, zp, st, c.stream);
+  }
+}
+
+// One CG iteration (hecmw_solver_CG.f90:164-279).
+static void enqueue_cg_iteration(Context& c, int32_t kind, int32_t iterpremax, hipEvent_t* ev) {
   const int64_t n3 = 3 * (int64_t)c.N;
   KrylovState* st = c.st.as<KrylovState>();
   double* part = c.part.as<double>();
   double* r = c.vr.as<double>();
   double* z = c.vz.as<double>();
-  if (iterpremax <= 0) {
+  if (ev) hipEventRecord(ev[0], c.stream);
+  if (iterpremax > 1 && kind != 0) {
+    enqueue_precond(c, kind, iterpremax, r, z);
+    cg_rho(n3, r, z, st, part, c.stream);
+  } else if (kind == 0) {
     cg_precond_none_rho(n3, r, z, st, part, c.stream);
-  } else if (pc == 3) {
+  } else if (kind == 3) {
     cg_precond_diag_rho(c.N, c.alu_diag.as<double>(), r, z, st, part, c.stream);
   } else {
     cg_precond_ssor_rho(c.L.view, c.U.view, c.row_old.as<int32_t>(), c.alu_ssor.as<double>(), c.color_slice.data(),
                         c.ncolor, r, z, st, part, c.stream);
   }
+  if (ev) hipEventRecord(ev[1], c.stream);
   cg_xpay(n3, z, c.vp.as<double>(), st, c.stream);
+  if (ev) hipEventRecord(ev[2], c.stream);
   cg_spmv_dot(c.A.view, c.vp.as<double>(), c.vq.as<double>(), st, part, c.stream);
+  if (ev) hipEventRecord(ev[3], c.stream);
   cg_update(n3, c.vp.as<double>(), c.vq.as<double>(), c.vx.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
+  cg_true_resid(c.A.view, c.vx.as<double>(), c.vb.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
+}
+
+// One BiCGSTAB iteration (hecmw_solver_BiCGSTAB.f90:164-275).  The reference
+// keeps s~ in R's slot (ST aliases R); a separate vector gives the same values.
+static void enqueue_bicg_iteration(Context& c, int32_t kind, int32_t iterpremax, hipEvent_t* ev) {
+  const int64_t n3 = 3 * (int64_t)c.N;
+  KrylovState* st = c.st.as<KrylovState>();
+  double* part = c.part.as<double>();
+  double *r = c.vr.as<double>(), *rt = c.vrt.as<double>(), *p = c.vp.as<double>(), *pt = c.vpt.as<double>();
+  double *v = c.vv.as<double>(), *sv = c.vs.as<double>(), *stv = c.vst.as<double>(), *t = c.vt.as<double>();
+  bi_rho(n3, r, rt, st, part, c.stream);
+  bi_p(n3, r, v, p, st, c.stream);
+  if (ev) hipEventRecord(ev[0], c.stream);
+  enqueue_precond(c, kind, iterpremax, p, pt);
+  if (ev) hipEventRecord(ev[1], c.stream);
+  if (ev) hipEventRecord(ev[2], c.stream);
+  bi_spmv_c2(c.A.view, pt, rt, v, st, part, c.stream);
+  if (ev) hipEventRecord(ev[3], c.stream);
+  bi_s(n3, v, r, sv, st, c.stream);
+  if (ev) hipEventRecord(ev[4], c.stream);
+  enqueue_precond(c, kind, iterpremax, sv, stv);
+  if (ev) hipEventRecord(ev[5], c.stream);
+  if (ev) hipEventRecord(ev[6], c.stream);
+  bi_spmv_omega(c.A.view, stv, sv, t, st, part, c.stream);
+  if (ev) hipEventRecord(ev[7], c.stream);
+  bi_update(n3, pt, stv, t, sv, c.vx.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
   cg_true_resid(c.A.view, c.vx.as<double>(), c.vb.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
 }
 
@@ -446,34 +524,43 @@ static int32_t krylov_resident(Context& c, int32_t* iarray, double* rarray, hecm
   const int32_t iterpremax = IA(iarray, I_ITERPREMAX);
   const int32_t maxit = IA(iarray, I_ITER);
   const double tol = RA(rarray, R_RESID);
-  if (method != 1) {
-    set_error("METHOD not provided by the GPU library yet (1: CG)");
-    return HECMW33_ERROR_UNSUPPORTED;
-  }
-  if (iterpremax > 1) {
-    set_error("ITERPREMAX > 1 not provided by the GPU library yet");
+  const bool timelog = IA(iarray, I_TIMELOG) >= 1;
+  if (method != 1 && method != 2) {
+    set_error("METHOD not provided by the GPU library (1: CG, 2: BiCGSTAB)");
     return HECMW33_ERROR_UNSUPPORTED;
   }
   if (int32_t e = precond_setup(c, iarray, rarray)) return e;
+  const int32_t kind = pc_kind(pc, iterpremax);
   const int64_t n3 = 3 * (int64_t)c.N;
+  const size_t vbytes = sizeof(double) * 3 * (size_t)c.NP;
   KrylovState* st = c.st.as<KrylovState>();
   double* part = c.part.as<double>();
   if (c.hist.ensure(sizeof(double) * std::max(1, maxit))) return HECMW33_ERROR_DEVICE;
+  if (iterpremax > 1 && (c.vzp.ensure(vbytes) || c.vzt.ensure(vbytes))) return HECMW33_ERROR_DEVICE;
+  if (method == 2) {
+    for (DMem* m : {&c.vrt, &c.vs, &c.vt, &c.vv, &c.vpt, &c.vst})
+      if (m->ensure(vbytes)) return HECMW33_ERROR_DEVICE;
+  }
   // r0 = b - A x0; bnrm2 = b.b  (hecmw_solver_CG.f90:131-141)
   launch_matresid(c.A.view, c.vx.as<double>(), c.vb.as<double>(), c.vr.as<double>(), c.stream);
   launch_dot_store(n3, c.vb.as<double>(), c.vb.as<double>(), &st->bnrm2, st, part, c.stream);
-  // p starts at zero (WW = 0), including the external entries the SpMV reads
-  HIPC(hipMemsetAsync(c.vp.p, 0, sizeof(double) * 3 * (size_t)c.NP, c.stream));
+  // work vectors start at zero (WW = 0), including the external entries the
+  // SpMV reads; nothing writes those afterwards
+  HIPC(hipMemsetAsync(c.vp.p, 0, vbytes, c.stream));
+  if (method == 2) {
+    HIPC(hipMemcpyAsync(c.vrt.p, c.vr.p, vbytes, hipMemcpyDeviceToDevice, c.stream));  // r~ = r0
+    for (DMem* m : {&c.vv, &c.vpt, &c.vst}) HIPC(hipMemsetAsync(m->p, 0, vbytes, c.stream));
+  }
   HIPC(hipGetLastError());
   KrylovState h{};
   HIPC(hipMemcpyAsync(&c.h_st[0], st, sizeof(KrylovState), hipMemcpyDeviceToHost, c.stream));
   HIPC(hipStreamSynchronize(c.stream));
   const double bnrm2 = c.h_st[0].bnrm2;
   out->method = method;
-  out->ncolor = (iterpremax > 0 && (pc == 1 || pc == 2)) ? c.ncolor : 0;
+  out->ncolor = kind == 1 ? c.ncolor : 0;
   if (bnrm2 == 0.0 || maxit < 1) {
     // zero-trip DO loop: ITER reads 1 (hecmw_solver_CG.f90:135-141,164)
-    if (bnrm2 == 0.0) HIPC(hipMemsetAsync(c.vx.p, 0, sizeof(double) * 3 * (size_t)c.NP, c.stream));
+    if (bnrm2 == 0.0) HIPC(hipMemsetAsync(c.vx.p, 0, vbytes, c.stream));
     HIPC(hipStreamSynchronize(c.stream));
     out->iter = 1;
     out->error = 0;
@@ -493,24 +580,35 @@ static int32_t krylov_resident(Context& c, int32_t* iarray, double* rarray, hecm
   h.error = 0;
   h.n_indef = 0;
   h.recheck = 0;
-  h.nrec = 50;
-  for (auto& t : h.ticket) t = 0;
+  h.nrec = method == 1 ? 50 : 100;
   c.h_st[1] = h;
   HIPC(hipMemcpyAsync(st, &c.h_st[1], sizeof(KrylovState), hipMemcpyHostToDevice, c.stream));
   HIPC(hipStreamSynchronize(c.stream));
   const double t1 = now_s();
   out->Tset = t1 - t0;
-  // iterations: enqueue in chunks, poll the previous chunk's state while the
-  // current one runs; kernels after the exit return immediately
+  // iterations: enqueue in chunks and poll the previous chunk's state while
+  // the current one runs; kernels after the exit return at once
   const int chunk = 8;
+  const int nev = method == 1 ? 4 : 8;
   hipEvent_t ev[2];
   HIPC(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
   HIPC(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
   int32_t enq = 0, nchunk = 0;
   bool finished = false;
+  std::vector<hipEvent_t> itev(nev);
   while (enq < maxit && !finished) {
     const int k = std::min(chunk, maxit - enq);
-    for (int i = 0; i < k; ++i) enqueue_cg_iteration(c, pc, iterpremax);
+    for (int i = 0; i < k; ++i) {
+      hipEvent_t* e = nullptr;
+      if (timelog) {
+        for (int j = 0; j < nev; ++j) itev[j] = c.event((size_t)nev * (enq + i) + j);
+        e = itev.data();
+      }
+      if (method == 1)
+        enqueue_cg_iteration(c, kind, iterpremax, e);
+      else
+        enqueue_bicg_iteration(c, kind, iterpremax, e);
+    }
     enq += k;
     const int slot = nchunk & 1;
     HIPC(hipMemcpyAsync(&c.h_st[slot], st, sizeof(KrylovState), hipMemcpyDeviceToHost, c.stream));
@@ -528,6 +626,22 @@ static int32_t krylov_resident(Context& c, int32_t* iarray, double* rarray, hecm
   hipEventDestroy(ev[1]);
   h = c.h_st[0];
   out->Tsol = now_s() - t1;
+  if (timelog) {
+    // kernel time of the iterations that ran (later ones returned at once)
+    const int64_t ran = std::min<int64_t>(std::min(h.iter, maxit), enq);
+    double tax = 0.0, tpc = 0.0;
+    for (int64_t i = 0; i < ran; ++i) {
+      for (int j = 0; j < nev; j += 4) {
+        float a = 0.f, b = 0.f;
+        HIPC(hipEventElapsedTime(&a, c.events[nev * i + j], c.events[nev * i + j + 1]));
+        HIPC(hipEventElapsedTime(&b, c.events[nev * i + j + 2], c.events[nev * i + j + 3]));
+        tpc += a;
+        tax += b;
+      }
+    }
+    out->time_precond = tpc * 1e-3;
+    out->time_Ax = tax * 1e-3;
+  }
   out->iter = h.iter;
   out->error = h.error;
   out->resid = h.resid;
@@ -735,6 +849,18 @@ int32_t hecmw33_rel_resid_L2(const hecmw33_sys* A, const double* B, const double
   HIPC(hipMemcpyAsync(c.vb.p, B, vb, hipMemcpyHostToDevice, c.stream));
   HIPC(hipMemcpyAsync(c.vx.p, X, vb, hipMemcpyHostToDevice, c.stream));
   return rel_resid_resident(c, resid);
+}
+
+int32_t hecmw33_last_history(double* hist, int32_t cap) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_ctx || !hist || cap < 0) return HECMW33_ERROR_ARGS;
+  Context& c = *g_ctx;
+  const int32_t n = std::min(cap, c.last_iter);
+  if (n > 0) {
+    HIPC(hipMemcpyAsync(hist, c.hist.p, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    HIPC(hipStreamSynchronize(c.stream));
+  }
+  return n;
 }
 
 int32_t hecmw33_ordering(int32_t N, const int32_t* indexL, const int32_t* itemL, const int32_t* indexU,

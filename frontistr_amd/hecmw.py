@@ -71,11 +71,13 @@ def lib():
     L.hecmw33_ctx_info.argtypes = [P, P, I]
     L.hecmw33_gen_hex8_count.argtypes = [I, C.POINTER(I), C.POINTER(I), C.POINTER(I)]
     L.hecmw33_gen_hex8.argtypes = [I, I, P, C.c_uint64, P, P, P, P, P, P, P, P, P]
+    L.hecmw33_last_history.argtypes = [P, I]
     L.hecmw33_last_error.restype = C.c_char_p
     for f in ("hecmw33_solve", "hecmw33_solve_krylov", "hecmw33_matvec", "hecmw33_precond_apply",
               "hecmw33_rel_resid_L2", "hecmw33_ordering", "hecmw33_ctx_load", "hecmw33_ctx_solve_device",
               "hecmw33_ctx_matvec_device", "hecmw33_ctx_matvec_device_f32", "hecmw33_ctx_history",
-              "hecmw33_ctx_info", "hecmw33_gen_hex8_count", "hecmw33_gen_hex8", "hecmw33_device_count"):
+              "hecmw33_ctx_info", "hecmw33_gen_hex8_count", "hecmw33_gen_hex8", "hecmw33_device_count",
+              "hecmw33_last_history"):
         getattr(L, f).restype = C.c_int32
     _lib = L
     return L
@@ -186,11 +188,14 @@ class SolveResult:
     resid_true: float
     Tset: float
     Tsol: float
+    time_Ax: float
+    time_precond: float
     t_upload: float
 
     @classmethod
     def _from(cls, o):
-        return cls(o.iter, o.error, o.method, o.ncolor, o.resid, o.resid_true, o.Tset, o.Tsol, o.t_upload)
+        return cls(o.iter, o.error, o.method, o.ncolor, o.resid, o.resid_true, o.Tset, o.Tsol, o.time_Ax,
+                   o.time_precond, o.t_upload)
 
 
 def hecmw_solve_iterative(mat):
@@ -213,6 +218,13 @@ def hecmw_solve_krylov(mat):
                                     C.byref(out))
     _check(rc)
     return SolveResult._from(out)
+
+
+def last_history(cap=100000):
+    """ITERLOG residuals of the last hecmw_solve_iterative / hecmw_solve_krylov call."""
+    h = np.zeros(cap)
+    n = _check(lib().hecmw33_last_history(_ptr(h), cap))
+    return h[:n]
 
 
 def hecmw_matvec(mat, x):

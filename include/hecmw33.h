@@ -98,6 +98,11 @@ int32_t hecmw33_precond_apply(const hecmw33_sys* A, const int32_t iarray[100], c
 /* sqrt(|b - A x|^2 / |b|^2) (hecmw_rel_resid_L2_33, hecmw_solver_las_33.F90:406-435). */
 int32_t hecmw33_rel_resid_L2(const hecmw33_sys* A, const double* B, const double* X, double* resid);
 
+/* Per-iteration relative residuals of the last hecmw33_solve / hecmw33_solve_krylov
+ * call, the values the reference prints with ITERLOG (hecmw_solver_CG.f90:253);
+ * returns the number written (at most cap). */
+int32_t hecmw33_last_history(double* hist, int32_t cap);
+
 /* RCM + capped greedy multicolouring of the internal rows, as
  * hecmw_precond_SSOR_33_setup builds it (hecmw_matrix_ordering_CM.f90:65-78 then
  * hecmw_matrix_ordering_MC.f90:15-72).  Host only.  perm_rcm, perm, iperm:

@@ -46,6 +46,8 @@ def lib():
         L.ref_precond.restype = None
         L.ref_rel_resid.argtypes = [I, I, I, I] + [P] * 9 + [I]
         L.ref_rel_resid.restype = C.c_double
+        L.ref_flush.argtypes = []
+        L.ref_flush.restype = None
         _lib = L
     return _lib
 
@@ -63,20 +65,58 @@ def _mat_args(m):
     return [m.N, m.NP, m.NPL, m.NPU, _p(m.D), _p(AL), _p(AU), _p(m.indexL), _p(tL), _p(m.indexU), _p(tU)]
 
 
-def solve_krylov(m, x0=None, nthreads=2, iarray=None, rarray=None):
+def _capture_stdout(fn):
+    """Run fn() with file descriptor 1 redirected to a temporary file; return
+    (fn's result, captured text).  Used to read the reference's ITERLOG lines."""
+    import sys
+    import tempfile
+    sys.stdout.flush()
+    saved = os.dup(1)
+    with tempfile.TemporaryFile() as tf:
+        os.dup2(tf.fileno(), 1)
+        try:
+            res = fn()
+            lib().ref_flush()
+        finally:
+            os.dup2(saved, 1)
+            os.close(saved)
+        tf.seek(0)
+        text = tf.read().decode(errors="replace")
+    return res, text
+
+
+def solve_krylov(m, x0=None, nthreads=2, iarray=None, rarray=None, history=False):
     """hecmw_solve_CG / hecmw_solve_BiCGSTAB on a copy of the system.
-    Returns (x, info) with info = dict(iter, resid, error, Tset, Tsol, Tcomm, time_Ax, time_precond, iarray)."""
+    Returns (x, info) with info = dict(iter, resid, error, Tset, Tsol, Tcomm, time_Ax, time_precond, iarray
+    [, history]).  history=True turns on ITERLOG and returns the printed
+    per-iteration relative residuals (format (i7,1pe16.6), hecmw_solver_CG.f90:253)."""
     x = (np.zeros(3 * m.NP) if x0 is None else np.array(x0, dtype=np.float64)).copy()
     b = m.B.copy()
     ia = np.array(m.Iarray if iarray is None else iarray, dtype=np.int32).copy()
     ra = np.array(m.Rarray if rarray is None else rarray, dtype=np.float64).copy()
     ia[96] = 1  # Iarray(97): numeric setup
     ia[97] = 1  # Iarray(98): symbolic setup
+    if history:
+        ia[20] = 1  # Iarray(21): ITERLOG
     iout = np.zeros(100, np.int32)
     out = np.zeros(7)
-    err = lib().ref_solve_krylov(*_mat_args(m), _p(b), _p(x), _p(ia), _p(ra), nthreads, _p(iout), _p(out))
+
+    def run():
+        return lib().ref_solve_krylov(*_mat_args(m), _p(b), _p(x), _p(ia), _p(ra), nthreads, _p(iout), _p(out))
+
+    if history:
+        err, text = _capture_stdout(run)
+    else:
+        err, text = run(), ""
     info = dict(iter=int(out[0]), resid=float(out[1]), error=int(err), Tset=float(out[2]), Tsol=float(out[3]),
                 Tcomm=float(out[4]), time_Ax=float(out[5]), time_precond=float(out[6]), iarray=iout)
+    if history:
+        hist = {}
+        for line in text.splitlines():
+            parts = line.split()
+            if len(parts) == 2 and parts[0].isdigit():
+                hist[int(parts[0])] = float(parts[1])
+        info["history"] = np.array([hist[k] for k in sorted(hist)])
     return x, info
 
 

@@ -225,4 +225,9 @@ contains
     res = hecmw_rel_resid_L2(mesh, mat)
   end function ref_rel_resid
 
+  !> Flush Fortran unit 6 (ITERLOG lines) so a caller capturing stdout sees them.
+  subroutine ref_flush() bind(c, name='ref_flush')
+    flush(6)
+  end subroutine ref_flush
+
 end module hecmw_ref_driver
