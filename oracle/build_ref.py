@@ -10,7 +10,7 @@ What is compiled
     directories the serial iterative path never reaches and that need
     libraries absent from this image (clustermkl, solver_direct_parallel: MPI;
     contact/ and main/hecmw_solver.f90: they use clustermkl);
-  * the twelve C files of hecmw1/src/common and precond/ml that the linked
+  * the thirteen C files of hecmw1/src/common and precond/ml that the linked
     Fortran objects call (timer, comm/control init, message table, ML wrapper
     compiled without ML).  Nothing is stubbed: every symbol resolves to
     reference code.

@@ -7,10 +7,13 @@ product package.
 
 The reference keeps module-level state between calls (the SSOR factorisation,
 the SpMV thread partition); every function here forces a fresh preconditioner
-setup unless told otherwise.  One caveat of the reference: its SpMV thread
-partition is recomputed only when the row count changes by more than one
-(hecmw_solver_las_33.F90:165-171), so callers should not alternate between
-systems whose N differ by exactly one.
+setup unless told otherwise.  Caveat of the reference: its SpMV keeps the
+previous thread partition (hecmw_solver_las_33.F90:164-171) whenever the last
+block ends at N-1, which is always the case for systems with fewer rows than
+100 x threads, so a change of the OpenMP thread count between two systems of
+the same N leaves rows uncomputed.  Every call here therefore uses the same
+thread count (NTHREADS) unless a caller deliberately passes another one (the
+CPU baseline, in its own process).
 """
 import ctypes as C
 import os
@@ -20,6 +23,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_ref", "libhecmw_ref.so")
 _lib = None
+NTHREADS = 2  # >= 2 selects the RCM + multicolour SSOR ordering (hecmw_precond_SSOR_33.F90:115-127)
 
 
 def available():
@@ -85,7 +89,7 @@ def _capture_stdout(fn):
     return res, text
 
 
-def solve_krylov(m, x0=None, nthreads=2, iarray=None, rarray=None, history=False):
+def solve_krylov(m, x0=None, nthreads=NTHREADS, iarray=None, rarray=None, history=False):
     """hecmw_solve_CG / hecmw_solve_BiCGSTAB on a copy of the system.
     Returns (x, info) with info = dict(iter, resid, error, Tset, Tsol, Tcomm, time_Ax, time_precond, iarray
     [, history]).  history=True turns on ITERLOG and returns the printed
@@ -120,7 +124,7 @@ def solve_krylov(m, x0=None, nthreads=2, iarray=None, rarray=None, history=False
     return x, info
 
 
-def solve_iterative(m, x0=None, nthreads=2):
+def solve_iterative(m, x0=None, nthreads=NTHREADS):
     """hecmw_solve_iterative (full dispatcher).  Returns (x, iarray_out, rarray_out)."""
     x = (np.zeros(3 * m.NP) if x0 is None else np.array(x0, dtype=np.float64)).copy()
     b = m.B.copy()
@@ -148,14 +152,14 @@ def ordering(m, ncolor_in=10):
     return pr, ipr, p, ip, ci[: nc.value + 1].copy()
 
 
-def matvec(m, x, nthreads=1):
+def matvec(m, x, nthreads=NTHREADS):
     x = np.array(x, dtype=np.float64)
     y = np.zeros(3 * m.NP)
     lib().ref_matvec(*_mat_args(m), _p(x), _p(y), nthreads)
     return y[: 3 * m.N]
 
 
-def precond(m, r, nthreads=2, iarray=None, rarray=None):
+def precond(m, r, nthreads=NTHREADS, iarray=None, rarray=None):
     r = np.array(r, dtype=np.float64)
     z = np.zeros(3 * m.NP)
     ia = np.array(m.Iarray if iarray is None else iarray, dtype=np.int32).copy()
@@ -164,7 +168,7 @@ def precond(m, r, nthreads=2, iarray=None, rarray=None):
     return z[: 3 * m.N]
 
 
-def rel_resid(m, x, nthreads=1):
+def rel_resid(m, x, nthreads=NTHREADS):
     x = np.array(x, dtype=np.float64)
     b = m.B.copy()
     return lib().ref_rel_resid(*_mat_args(m), _p(b), _p(x), nthreads)
