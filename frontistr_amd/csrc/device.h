@@ -10,19 +10,30 @@ constexpr int kWavesPerBlock = 4;
 constexpr int kBlock = kWave * kWavesPerBlock;
 constexpr int kNumXcd = 8;      // MI355X: 8 XCDs, each with its own L2
 
-// Block rows in SELL-64 form: rows are grouped 64 to a slice (one slice per
-// wavefront, one row per lane).  Slice s holds width(s) = ptr[s+1]-ptr[s] block
-// slots per lane; slot k of lane l keeps its column in col[(ptr[s]+k)*64 + l]
-// and its 3x3 values in 9 planes val[((ptr[s]+k)*9 + e)*64 + l], e = 3*row+col
-// of the block.  A load of one plane by a wavefront is one contiguous
-// 512-byte (FP64) or 256-byte (FP32) segment.
+// Block rows in SELL-C form, C = rows per slice: slice s holds
+// width(s) = ptr[s+1]-ptr[s] block slots per row; slot k of row r (0 <= r < C)
+// keeps its column in col[(ptr[s]+k)*C + r] and its 3x3 values in 9 planes
+// val[((ptr[s]+k)*9 + e)*C + r], e = 3*i+j for block entry (i,j).
+//  * The full matrix for the SpMV uses C = 64, one row per lane: in the
+//    natural order consecutive rows have consecutive neighbours, so the x
+//    gathers of a wavefront are already nearly contiguous, and one wavefront
+//    load of a plane is one 512-byte segment.
+//  * The multicolour SSOR factors use C = 16 with four lanes per row (lane 4r+d
+//    computes component d of row r, the fourth lane mirrors d = 2): inside a
+//    colour the rows are scattered nodes, so the three component lanes of a row
+//    gather a neighbour's z from one cache line and share it with DPP quad
+//    broadcasts, and a colour launch has four times as many wavefronts.
+constexpr int kRowsA = 64;
+constexpr int kRowsS = 16;
+
 template <typename T>
 struct Sell {
   int32_t nslices = 0;
   int32_t nrows = 0;
-  int64_t nslots = 0;          // sum of widths * 64
+  int32_t rows = kRowsA;       // rows per slice (C)
+  int64_t nslots = 0;          // sum of widths * C
   int32_t* ptr = nullptr;      // [nslices+1]
-  int32_t* len = nullptr;      // [nslices*64] blocks in the lane's row (0 = idle lane)
+  int32_t* len = nullptr;      // [nslices*C] blocks in the row (0 = idle row)
   int32_t* col = nullptr;      // [nslots] 0-based node id of the block column
   T* val = nullptr;            // [9*nslots]
 };
@@ -53,11 +64,12 @@ void launch_fill_values(const Sell<double>& a, const int64_t* src, const double*
                         const double* AU, int64_t nD, int64_t nAL, hipStream_t s);
 void launch_fill_values_f32(const Sell<float>& a, const int64_t* src, const double* D, const double* AL,
                             const double* AU, int64_t nD, int64_t nAL, hipStream_t s);
-// 3x3 no-pivot LU with sigma on the pivots of D[row_src[t]] into the 9-plane
-// slice layout (hecmw_precond_SSOR_33.F90:164-208, hecmw_precond_DIAG_33.F90:58-91)
-void launch_block_lu(int32_t nslots, const int32_t* row_src, const double* D, double sigma, double* alu,
+// 3x3 no-pivot LU with sigma on the pivots of D[row_src[t]] into 9-plane
+// slices of `rows` rows (hecmw_precond_SSOR_33.F90:164-208,
+// hecmw_precond_DIAG_33.F90:58-91)
+void launch_block_lu(int32_t nslots, const int32_t* row_src, const double* D, double sigma, double* alu, int rows,
                      hipStream_t s);
-void launch_block_lu_f32(int32_t nslots, const int32_t* row_src, const double* D, double sigma, float* alu,
+void launch_block_lu_f32(int32_t nslots, const int32_t* row_src, const double* D, double sigma, float* alu, int rows,
                          hipStream_t s);
 
 // ---- plain operators -------------------------------------------------------------

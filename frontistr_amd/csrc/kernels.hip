@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 
 #include "device.h"
 
@@ -70,15 +71,27 @@ __device__ __forceinline__ void block_partials(double (&v)[K], double* part, int
   }
 }
 
-// Sum nparts partials of K components in a fixed order (one workgroup).
+// Sum nparts partials of K components in a fixed order (one workgroup of
+// kTailThreads): thread t adds partials t, t+T, t+2T, ... with eight loads in
+// flight, then a fixed tree.
+constexpr int kTailThreads = 1024;
 template <int K>
 __device__ __forceinline__ void sum_partials(const double* part, int nparts, double (&tot)[K]) {
-  __shared__ double red[K][kWavesPerBlock];
+  constexpr int kW = kTailThreads / kWave;
+  __shared__ double red[K][kW];
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += kBlock) s += part[k * kMaxPartials + i];
+    const double* p = part + k * kMaxPartials;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i0 = threadIdx.x; i0 < nparts; i0 += 8 * kTailThreads) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = i0 + u * kTailThreads;
+        if (i < nparts) acc[u] += p[i];
+      }
+    }
+    double s = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
     s = wave_sum(s);
     if (lane == 0) red[k][wave] = s;
   }
@@ -87,7 +100,7 @@ __device__ __forceinline__ void sum_partials(const double* part, int nparts, dou
   for (int k = 0; k < K; ++k) {
     double t = red[k][0];
 #pragma unroll
-    for (int w = 1; w < kWavesPerBlock; ++w) t += red[k][w];
+    for (int w = 1; w < kW; ++w) t += red[k][w];
     tot[k] = t;
   }
 }
@@ -161,8 +174,8 @@ __device__ __forceinline__ bool true_resid_active(const KrylovState* st) {
 }
 
 template <int TAIL, int K>
-__global__ __launch_bounds__(kBlock) void k_tail(KrylovState* st, const double* part, int nparts, double* out,
-                                                 double* hist) {
+__global__ __launch_bounds__(kTailThreads) void k_tail(KrylovState* st, const double* part, int nparts, double* out,
+                                                       double* hist) {
   if (TAIL != kTailStore && st->done) return;
   double tot[K];
   sum_partials<K>(part, nparts, tot);
@@ -173,8 +186,8 @@ __global__ __launch_bounds__(kBlock) void k_tail(KrylovState* st, const double* 
 // (hecmw_solver_CG.f90:242-277): convergence after a recomputed residual,
 // NOCONV at MAXIT (the Fortran DO variable then reads MAXIT+1), else
 // rho1 = rho and the next iteration.
-__global__ __launch_bounds__(kBlock) void k_tail_true(KrylovState* st, const double* part, int nparts,
-                                                      double* hist) {
+__global__ __launch_bounds__(kTailThreads) void k_tail_true(KrylovState* st, const double* part, int nparts,
+                                                            double* hist) {
   if (st->done) return;
   if (true_resid_active(st)) {
     double tot[1];
@@ -199,26 +212,48 @@ __global__ __launch_bounds__(kBlock) void k_tail_true(KrylovState* st, const dou
   }
 }
 
-// ---- SELL-64 block row product ----------------------------------------------
-// FP64, reference order: the D block first, then L, then U in storage order;
-// y = ((y + a1*x1) + a2*x2) + a3*x3, no fused multiply-add.  Loads of U
-// blocks are issued together before they are accumulated in order.
+// ---- SELL-16 block row product (four lanes per row) -------------------------
+// Lane 4r+d of a wavefront works on row r of the slice and output component
+// d (the fourth lane of every quad mirrors d = 2).  The three component lanes
+// of a row gather the three entries of a neighbour's x from one cache line
+// and share them with DPP quad broadcasts, so every component's sum runs
+// over the row's blocks in storage order exactly as the reference's loop.
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+
+// DPP quad_perm broadcast of lane J of each group of four lanes
+template <int J>
+__device__ __forceinline__ double quad_bcast(double v) {
+  constexpr int ctrl = J | (J << 2) | (J << 4) | (J << 6);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), ctrl, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), ctrl, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+template <int J>
+__device__ __forceinline__ float quad_bcast(float v) {
+  constexpr int ctrl = J | (J << 2) | (J << 4) | (J << 6);
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctrl, 0xf, 0xf, false));
+}
+
+// SpMV rows, SELL-64, one row per lane (kRowsA).  FP64 in the reference's
+// order: the D block first, then L, then U in storage order;
+// y = ((y + a1*x1) + a2*x2) + a3*x3, no fused multiply-add.  A chunk of U
+// blocks is loaded before it is accumulated in order.  xo = x of the row itself.
 template <int U>
 __device__ __forceinline__ void sell_row(const Sell<double>& a, int s, int lane, const double* __restrict__ x,
                                          double (&y)[3], double (&xo)[3]) {
 #pragma clang fp contract(off)
   const int base = a.ptr[s];
-  const int len = a.len[s * kWave + lane];
+  const int len = a.len[s * kRowsA + lane];
   y[0] = y[1] = y[2] = 0.0;
   xo[0] = xo[1] = xo[2] = 0.0;
   if (len == 0) return;
-  const int32_t* __restrict__ colp = a.col + (int64_t)base * kWave + lane;
-  const double* __restrict__ vp = a.val + (int64_t)base * 9 * kWave + lane;
+  const int32_t* __restrict__ colp = a.col + (int64_t)base * kRowsA + lane;
+  const double* __restrict__ vp = a.val + (int64_t)base * 9 * kRowsA + lane;
   {
     const int c = colp[0];
     double v[9];
 #pragma unroll
-    for (int e = 0; e < 9; ++e) v[e] = vp[e * kWave];
+    for (int e = 0; e < 9; ++e) v[e] = vp[e * kRowsA];
     const double x0 = x[3 * (int64_t)c], x1 = x[3 * (int64_t)c + 1], x2 = x[3 * (int64_t)c + 2];
     xo[0] = x0;
     xo[1] = x1;
@@ -228,28 +263,33 @@ __device__ __forceinline__ void sell_row(const Sell<double>& a, int s, int lane,
     y[2] = v[6] * x0 + v[7] * x1 + v[8] * x2;
   }
   for (int k0 = 1; k0 < len; k0 += U) {
-    int c[U];
+    // branch-free chunk: slots past the row's end re-read the last block and
+    // are masked out, so the chunk's loads are all in flight together
+    int kk[U], c[U];
 #pragma unroll
-    for (int j = 0; j < U; ++j) c[j] = (k0 + j < len) ? colp[(int64_t)(k0 + j) * kWave] : 0;
+    for (int j = 0; j < U; ++j) {
+      kk[j] = (k0 + j < len) ? k0 + j : len - 1;
+      c[j] = colp[(int64_t)kk[j] * kRowsA];
+    }
     double v[U][9], xv[U][3];
 #pragma unroll
     for (int j = 0; j < U; ++j) {
-      if (k0 + j < len) {
-        const double* __restrict__ vk = vp + (int64_t)(k0 + j) * 9 * kWave;
+      const double* __restrict__ vk = vp + (int64_t)kk[j] * 9 * kRowsA;
 #pragma unroll
-        for (int e = 0; e < 9; ++e) v[j][e] = vk[e * kWave];
-        xv[j][0] = x[3 * (int64_t)c[j]];
-        xv[j][1] = x[3 * (int64_t)c[j] + 1];
-        xv[j][2] = x[3 * (int64_t)c[j] + 2];
-      }
+      for (int e = 0; e < 9; ++e) v[j][e] = vk[e * kRowsA];
+      xv[j][0] = x[3 * (int64_t)c[j]];
+      xv[j][1] = x[3 * (int64_t)c[j] + 1];
+      xv[j][2] = x[3 * (int64_t)c[j] + 2];
     }
 #pragma unroll
     for (int j = 0; j < U; ++j) {
-      if (k0 + j < len) {
-        y[0] = y[0] + v[j][0] * xv[j][0] + v[j][1] * xv[j][1] + v[j][2] * xv[j][2];
-        y[1] = y[1] + v[j][3] * xv[j][0] + v[j][4] * xv[j][1] + v[j][5] * xv[j][2];
-        y[2] = y[2] + v[j][6] * xv[j][0] + v[j][7] * xv[j][1] + v[j][8] * xv[j][2];
-      }
+      const bool live = k0 + j < len;
+      const double t0 = y[0] + v[j][0] * xv[j][0] + v[j][1] * xv[j][1] + v[j][2] * xv[j][2];
+      const double t1 = y[1] + v[j][3] * xv[j][0] + v[j][4] * xv[j][1] + v[j][5] * xv[j][2];
+      const double t2 = y[2] + v[j][6] * xv[j][0] + v[j][7] * xv[j][1] + v[j][8] * xv[j][2];
+      y[0] = live ? t0 : y[0];
+      y[1] = live ? t1 : y[1];
+      y[2] = live ? t2 : y[2];
     }
   }
 }
@@ -260,21 +300,21 @@ template <int U>
 __device__ __forceinline__ void sell_row_f32(const Sell<float>& a, int s, int lane, const float* __restrict__ x,
                                              float (&y)[3]) {
   const int base = a.ptr[s];
-  const int len = a.len[s * kWave + lane];
+  const int len = a.len[s * kRowsA + lane];
   y[0] = y[1] = y[2] = 0.0f;
-  const int32_t* __restrict__ colp = a.col + (int64_t)base * kWave + lane;
-  const float* __restrict__ vp = a.val + (int64_t)base * 9 * kWave + lane;
+  const int32_t* __restrict__ colp = a.col + (int64_t)base * kRowsA + lane;
+  const float* __restrict__ vp = a.val + (int64_t)base * 9 * kRowsA + lane;
   for (int k0 = 0; k0 < len; k0 += U) {
     int c[U];
 #pragma unroll
-    for (int j = 0; j < U; ++j) c[j] = (k0 + j < len) ? colp[(int64_t)(k0 + j) * kWave] : 0;
+    for (int j = 0; j < U; ++j) c[j] = (k0 + j < len) ? colp[(int64_t)(k0 + j) * kRowsA] : 0;
     float v[U][9], xv[U][3];
 #pragma unroll
     for (int j = 0; j < U; ++j) {
       if (k0 + j < len) {
-        const float* __restrict__ vk = vp + (int64_t)(k0 + j) * 9 * kWave;
+        const float* __restrict__ vk = vp + (int64_t)(k0 + j) * 9 * kRowsA;
 #pragma unroll
-        for (int e = 0; e < 9; ++e) v[j][e] = vk[e * kWave];
+        for (int e = 0; e < 9; ++e) v[j][e] = vk[e * kRowsA];
         xv[j][0] = x[3 * (int64_t)c[j]];
         xv[j][1] = x[3 * (int64_t)c[j] + 1];
         xv[j][2] = x[3 * (int64_t)c[j] + 2];
@@ -318,7 +358,6 @@ int vec_grid(int64_t n) {
 }
 
 constexpr int kSpmvChunk = 4;
-constexpr int kSsorChunk = 8;
 
 // ---- SpMV kernels --------------------------------------------------------------
 enum SpmvEpi : int {
@@ -340,51 +379,42 @@ __global__ __launch_bounds__(kBlock) void k_spmv(Sell<double> a, const double* _
     if (EPI == kEpiTrue && !true_resid_active(st)) return;
   }
   constexpr int K = EPI == kEpiDot2W ? 2 : 1;
-  const int lane = threadIdx.x & (kWave - 1);
+  const int lane = lane_id();
   const SliceRange sr = wave_slices(a.nslices);
-  double d[K];
+  double acc[K];
 #pragma unroll
-  for (int k = 0; k < K; ++k) d[k] = 0.0;
+  for (int k = 0; k < K; ++k) acc[k] = 0.0;
   for (int s = sr.begin; s < sr.end; ++s) {
-    double acc[3], xo[3];
-    sell_row<kSpmvChunk>(a, s, lane, x, acc, xo);
-    const int64_t row = (int64_t)s * kWave + lane;
+    double yv[3], xo[3];
+    sell_row<kSpmvChunk>(a, s, lane, x, yv, xo);
+    const int64_t row = (int64_t)s * kRowsA + lane;
     if (row < a.nrows) {
       double o[3];
-      if (EPI == kEpiResid || EPI == kEpiTrue) {
-        o[0] = bw[3 * row] - acc[0];
-        o[1] = bw[3 * row + 1] - acc[1];
-        o[2] = bw[3 * row + 2] - acc[2];
-      } else {
-        o[0] = acc[0];
-        o[1] = acc[1];
-        o[2] = acc[2];
-      }
-      y[3 * row] = o[0];
-      y[3 * row + 1] = o[1];
-      y[3 * row + 2] = o[2];
-      if (EPI == kEpiDotX) d[0] += xo[0] * o[0] + xo[1] * o[1] + xo[2] * o[2];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) o[d] = (EPI == kEpiResid || EPI == kEpiTrue) ? bw[3 * row + d] - yv[d] : yv[d];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) y[3 * row + d] = o[d];
+      if (EPI == kEpiDotX) acc[0] += xo[0] * o[0] + xo[1] * o[1] + xo[2] * o[2];
       if (EPI == kEpiDotW || EPI == kEpiDot2W)
-        d[0] += bw[3 * row] * o[0] + bw[3 * row + 1] * o[1] + bw[3 * row + 2] * o[2];
-      if (EPI == kEpiDot2W) d[K - 1] += o[0] * o[0] + o[1] * o[1] + o[2] * o[2];
-      if (EPI == kEpiTrue) d[0] += o[0] * o[0] + o[1] * o[1] + o[2] * o[2];
+        acc[0] += bw[3 * row] * o[0] + bw[3 * row + 1] * o[1] + bw[3 * row + 2] * o[2];
+      if (EPI == kEpiDot2W || EPI == kEpiTrue) acc[K - 1] += o[0] * o[0] + o[1] * o[1] + o[2] * o[2];
     }
   }
-  if (EPI >= kEpiDotX) block_partials<K>(d, part, blockIdx.x);
+  if (EPI >= kEpiDotX) block_partials<K>(acc, part, blockIdx.x);
 }
 
 __global__ __launch_bounds__(kBlock) void k_spmv_f32(Sell<float> a, const float* __restrict__ x,
                                                        float* __restrict__ y) {
-  const int lane = threadIdx.x & (kWave - 1);
+  const int lane = lane_id();
   const SliceRange sr = wave_slices(a.nslices);
   for (int s = sr.begin; s < sr.end; ++s) {
-    float acc[3];
-    sell_row_f32<kSpmvChunk>(a, s, lane, x, acc);
-    const int64_t row = (int64_t)s * kWave + lane;
+    float yv[3];
+    sell_row_f32<kSpmvChunk>(a, s, lane, x, yv);
+    const int64_t row = (int64_t)s * kRowsA + lane;
     if (row < a.nrows) {
-      y[3 * row] = acc[0];
-      y[3 * row + 1] = acc[1];
-      y[3 * row + 2] = acc[2];
+      y[3 * row] = yv[0];
+      y[3 * row + 1] = yv[1];
+      y[3 * row + 2] = yv[2];
     }
   }
 }
@@ -402,128 +432,122 @@ __device__ __forceinline__ void lu3_solve(const T (&a)[9], T& x0, T& x1, T& x2) 
   x0 = a[0] * (x0 - a[2] * x2 - a[1] * x1);
 }
 
+// 9 LU factors of row r of slice s, 16-row planes (all lanes of the quad)
+template <typename T>
+__device__ __forceinline__ void load_alu(const T* __restrict__ alu, int s, int r, T (&a)[9]) {
+  const T* __restrict__ f = alu + (int64_t)s * 9 * kRowsS + r;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) a[e] = f[e * kRowsS];
+}
+
+template <typename T>
+__device__ __forceinline__ T pick3(int d, T a, T b, T c) {
+  return d == 0 ? a : (d == 1 ? b : c);
+}
+
+// Accumulate blocks [k0, k0+CH) of a row (descending when DESC) into the
+// lane's component sum; all CH column indices, gathers and values are
+// requested before the first one is used.  SIGN -1: sw - a*x (forward),
+// +1: sw + a*x (backward), each block as the reference writes it.
+template <typename T, int CH, bool DESC, int SIGN>
+__device__ __forceinline__ void ssor_chunk(const int32_t* __restrict__ colp, const T* __restrict__ vp,
+                                           const T* __restrict__ z, int k0, int len, int dd, T& sw) {
+#pragma clang fp contract(off)
+  // branch-free: slots past the row's end re-read a valid block and are
+  // masked out of the sum, so every load of the chunk can be in flight at once
+  int kk[CH], c[CH];
+#pragma unroll
+  for (int j = 0; j < CH; ++j) {
+    const int k = DESC ? k0 - j : k0 + j;
+    kk[j] = DESC ? (k < 0 ? 0 : k) : (k < len ? k : len - 1);
+    c[j] = colp[(int64_t)kk[j] * kRowsS];
+  }
+  T xd[CH], v[CH][3];
+#pragma unroll
+  for (int j = 0; j < CH; ++j) {
+    const T* __restrict__ vk = vp + (int64_t)kk[j] * 9 * kRowsS;
+    xd[j] = z[3 * (int64_t)c[j] + dd];
+    v[j][0] = vk[0];
+    v[j][1] = vk[kRowsS];
+    v[j][2] = vk[2 * kRowsS];
+  }
+#pragma unroll
+  for (int j = 0; j < CH; ++j) {
+    const int k = DESC ? k0 - j : k0 + j;
+    const bool live = DESC ? k >= 0 : k < len;
+    const T x0 = quad_bcast<0>(xd[j]), x1 = quad_bcast<1>(xd[j]), x2 = quad_bcast<2>(xd[j]);
+    const T t = SIGN < 0 ? sw - v[j][0] * x0 - v[j][1] * x1 - v[j][2] * x2
+                         : sw + v[j][0] * x0 + v[j][1] * x1 + v[j][2] * x2;
+    sw = live ? t : sw;
+  }
+}
+
 // Forward, rows of one colour independent:
 //   z[i] = LU^-1 (r[i] - sum_{j in L'(i), ascending} L'_ij z[j])
-template <typename T>
+template <typename T, int CH>
 __global__ __launch_bounds__(kBlock) void k_ssor_fwd(Sell<T> L, const int32_t* __restrict__ row_old,
                                                        const T* __restrict__ alu, int s0, int ns,
                                                        const T* __restrict__ rin, T* __restrict__ z,
                                                        const KrylovState* st) {
 #pragma clang fp contract(off)
   if (st && st->done) return;
-  const int lane = threadIdx.x & (kWave - 1);
+  const int lane = lane_id(), r = lane >> 2, d = lane & 3, dd = d < 3 ? d : 2;
   const SliceRange sr = wave_slices(ns);
   for (int si = sr.begin; si < sr.end; ++si) {
     const int s = s0 + si;
-    const int64_t slot = (int64_t)s * kWave + lane;
+    const int64_t slot = (int64_t)s * kRowsS + r;
     const int iold = row_old[slot];
-    if (iold < 0) continue;
     const int len = L.len[slot];
     const int base = L.ptr[s];
-    const int32_t* __restrict__ colp = L.col + (int64_t)base * kWave + lane;
-    const T* __restrict__ vp = L.val + (int64_t)base * 9 * kWave + lane;
-    T sw0 = rin[3 * (int64_t)iold], sw1 = rin[3 * (int64_t)iold + 1], sw2 = rin[3 * (int64_t)iold + 2];
     T a[9];
-    const T* __restrict__ f = alu + (int64_t)s * 9 * kWave + lane;
-#pragma unroll
-    for (int e = 0; e < 9; ++e) a[e] = f[e * kWave];
-    for (int k0 = 0; k0 < len; k0 += kSsorChunk) {
-      int c[kSsorChunk];
-#pragma unroll
-      for (int j = 0; j < kSsorChunk; ++j) c[j] = (k0 + j < len) ? colp[(int64_t)(k0 + j) * kWave] : 0;
-      T v[kSsorChunk][9], xv[kSsorChunk][3];
-#pragma unroll
-      for (int j = 0; j < kSsorChunk; ++j) {
-        if (k0 + j < len) {
-          const T* __restrict__ vk = vp + (int64_t)(k0 + j) * 9 * kWave;
-#pragma unroll
-          for (int e = 0; e < 9; ++e) v[j][e] = vk[e * kWave];
-          xv[j][0] = z[3 * (int64_t)c[j]];
-          xv[j][1] = z[3 * (int64_t)c[j] + 1];
-          xv[j][2] = z[3 * (int64_t)c[j] + 2];
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < kSsorChunk; ++j) {
-        if (k0 + j < len) {
-          sw0 = sw0 - v[j][0] * xv[j][0] - v[j][1] * xv[j][1] - v[j][2] * xv[j][2];
-          sw1 = sw1 - v[j][3] * xv[j][0] - v[j][4] * xv[j][1] - v[j][5] * xv[j][2];
-          sw2 = sw2 - v[j][6] * xv[j][0] - v[j][7] * xv[j][1] - v[j][8] * xv[j][2];
-        }
-      }
-    }
-    lu3_solve<T>(a, sw0, sw1, sw2);
-    z[3 * (int64_t)iold] = sw0;
-    z[3 * (int64_t)iold + 1] = sw1;
-    z[3 * (int64_t)iold + 2] = sw2;
+    load_alu<T>(alu, s, r, a);
+    if (iold < 0) continue;
+    const int32_t* __restrict__ colp = L.col + (int64_t)base * kRowsS + r;
+    const T* __restrict__ vp = L.val + ((int64_t)base * 9 + 3 * dd) * kRowsS + r;
+    T sw = rin[3 * (int64_t)iold + dd];
+    for (int k0 = 0; k0 < len; k0 += CH) ssor_chunk<T, CH, false, -1>(colp, vp, z, k0, len, dd, sw);
+    T x0 = quad_bcast<0>(sw), x1 = quad_bcast<1>(sw), x2 = quad_bcast<2>(sw);
+    lu3_solve<T>(a, x0, x1, x2);
+    if (d < 3) z[3 * (int64_t)iold + d] = pick3(d, x0, x1, x2);
   }
 }
 
 // Backward, colours in reverse:
 //   z[i] -= LU^-1 sum_{j in U'(i), descending} U'_ij z[j]
 // With RHO, r.z over this colour's rows (final now) goes to part[pbase+block].
-template <typename T, bool RHO>
+template <typename T, int CH, bool RHO>
 __global__ __launch_bounds__(kBlock) void k_ssor_bwd(Sell<T> U, const int32_t* __restrict__ row_old,
                                                        const T* __restrict__ alu, int s0, int ns,
-                                                       T* __restrict__ z, const double* __restrict__ r,
+                                                       T* __restrict__ z, const double* __restrict__ r_rho,
                                                        const KrylovState* st, double* part, int pbase) {
 #pragma clang fp contract(off)
   if (st && st->done) return;
-  const int lane = threadIdx.x & (kWave - 1);
+  const int lane = lane_id(), r = lane >> 2, d = lane & 3, dd = d < 3 ? d : 2;
   const SliceRange sr = wave_slices(ns);
-  double d[1] = {0.0};
+  double acc[1] = {0.0};
   for (int si = sr.begin; si < sr.end; ++si) {
     const int s = s0 + si;
-    const int64_t slot = (int64_t)s * kWave + lane;
+    const int64_t slot = (int64_t)s * kRowsS + r;
     const int iold = row_old[slot];
-    if (iold < 0) continue;
     const int len = U.len[slot];
     const int base = U.ptr[s];
-    const int32_t* __restrict__ colp = U.col + (int64_t)base * kWave + lane;
-    const T* __restrict__ vp = U.val + (int64_t)base * 9 * kWave + lane;
     T a[9];
-    const T* __restrict__ f = alu + (int64_t)s * 9 * kWave + lane;
-#pragma unroll
-    for (int e = 0; e < 9; ++e) a[e] = f[e * kWave];
-    T zo0 = z[3 * (int64_t)iold], zo1 = z[3 * (int64_t)iold + 1], zo2 = z[3 * (int64_t)iold + 2];
-    T sw0 = 0, sw1 = 0, sw2 = 0;
-    // descending k: chunk j covers k = top - j
-    for (int top = len - 1; top >= 0; top -= kSsorChunk) {
-      int c[kSsorChunk];
-#pragma unroll
-      for (int j = 0; j < kSsorChunk; ++j) c[j] = (top - j >= 0) ? colp[(int64_t)(top - j) * kWave] : 0;
-      T v[kSsorChunk][9], xv[kSsorChunk][3];
-#pragma unroll
-      for (int j = 0; j < kSsorChunk; ++j) {
-        if (top - j >= 0) {
-          const T* __restrict__ vk = vp + (int64_t)(top - j) * 9 * kWave;
-#pragma unroll
-          for (int e = 0; e < 9; ++e) v[j][e] = vk[e * kWave];
-          xv[j][0] = z[3 * (int64_t)c[j]];
-          xv[j][1] = z[3 * (int64_t)c[j] + 1];
-          xv[j][2] = z[3 * (int64_t)c[j] + 2];
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < kSsorChunk; ++j) {
-        if (top - j >= 0) {
-          sw0 = sw0 + v[j][0] * xv[j][0] + v[j][1] * xv[j][1] + v[j][2] * xv[j][2];
-          sw1 = sw1 + v[j][3] * xv[j][0] + v[j][4] * xv[j][1] + v[j][5] * xv[j][2];
-          sw2 = sw2 + v[j][6] * xv[j][0] + v[j][7] * xv[j][1] + v[j][8] * xv[j][2];
-        }
-      }
-    }
-    lu3_solve<T>(a, sw0, sw1, sw2);
-    const T z0 = zo0 - sw0, z1 = zo1 - sw1, z2 = zo2 - sw2;
-    z[3 * (int64_t)iold] = z0;
-    z[3 * (int64_t)iold + 1] = z1;
-    z[3 * (int64_t)iold + 2] = z2;
-    if (RHO) {
-      d[0] += r[3 * (int64_t)iold] * (double)z0 + r[3 * (int64_t)iold + 1] * (double)z1 +
-              r[3 * (int64_t)iold + 2] * (double)z2;
+    load_alu<T>(alu, s, r, a);
+    if (iold < 0) continue;
+    const int32_t* __restrict__ colp = U.col + (int64_t)base * kRowsS + r;
+    const T* __restrict__ vp = U.val + ((int64_t)base * 9 + 3 * dd) * kRowsS + r;
+    const T zo = z[3 * (int64_t)iold + dd];
+    T sw = 0;
+    for (int top = len - 1; top >= 0; top -= CH) ssor_chunk<T, CH, true, 1>(colp, vp, z, top, len, dd, sw);
+    T x0 = quad_bcast<0>(sw), x1 = quad_bcast<1>(sw), x2 = quad_bcast<2>(sw);
+    lu3_solve<T>(a, x0, x1, x2);
+    const T zn = zo - pick3(dd, x0, x1, x2);
+    if (d < 3) {
+      z[3 * (int64_t)iold + d] = zn;
+      if (RHO) acc[0] += r_rho[3 * (int64_t)iold + d] * (double)zn;
     }
   }
-  if (RHO) block_partials<1>(d, part, pbase + blockIdx.x);
+  if (RHO) block_partials<1>(acc, part, pbase + blockIdx.x);
 }
 
 // ---- DIAG (block Jacobi), hecmw_precond_DIAG_33.F90:106-142 --------------------
@@ -681,28 +705,29 @@ __global__ void k_fill_values(Sell<T> a, const int64_t* __restrict__ src, const 
                               const double* __restrict__ AL, const double* __restrict__ AU, int64_t nD, int64_t nAL) {
   const int64_t nslot = a.nslots;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nslot; t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t grp = t / kWave, lane = t % kWave;  // slot = grp*64 + lane
+    const int64_t C = a.rows;
+    const int64_t grp = t / C, r = t % C;  // slot = grp*C + r
     const int64_t b = src[t];
     const double* blk = nullptr;
     if (b >= 0) blk = b < nD ? D + 9 * b : (b < nD + nAL ? AL + 9 * (b - nD) : AU + 9 * (b - nD - nAL));
-    T* out = a.val + grp * 9 * kWave + lane;
+    T* out = a.val + grp * 9 * C + r;
 #pragma unroll
-    for (int e = 0; e < 9; ++e) out[e * kWave] = blk ? (T)blk[e] : (T)0;
+    for (int e = 0; e < 9; ++e) out[e * C] = blk ? (T)blk[e] : (T)0;
   }
 }
 
 template <typename T>
 __global__ void k_block_lu(int32_t nslots, const int32_t* __restrict__ row_src, const double* __restrict__ D,
-                           double sigma, T* __restrict__ alu) {
+                           double sigma, T* __restrict__ alu, int rows) {
 #pragma clang fp contract(off)
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nslots; t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t grp = t / kWave, lane = t % kWave;
+    const int64_t grp = t / rows, r = t % rows;
     const int32_t i = row_src[t];
     double a[3][3];
     if (i >= 0) {
       const double* d = D + 9 * (int64_t)i;
-      for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 3; ++c) a[r][c] = d[3 * r + c];
+      for (int p = 0; p < 3; ++p)
+        for (int q = 0; q < 3; ++q) a[p][q] = d[3 * p + q];
       a[0][0] = a[0][0] * sigma;
       a[1][1] = a[1][1] * sigma;
       a[2][2] = a[2][2] * sigma;
@@ -710,20 +735,20 @@ __global__ void k_block_lu(int32_t nslots, const int32_t* __restrict__ row_src, 
       // multipliers below, updated U above.
       for (int k = 0; k < 3; ++k) {
         a[k][k] = 1.0 / a[k][k];
-        for (int r = k + 1; r < 3; ++r) {
-          a[r][k] = a[r][k] * a[k][k];
+        for (int p = k + 1; p < 3; ++p) {
+          a[p][k] = a[p][k] * a[k][k];
           double pw[3];
-          for (int c = k + 1; c < 3; ++c) pw[c] = a[r][c] - a[r][k] * a[k][c];
-          for (int c = k + 1; c < 3; ++c) a[r][c] = pw[c];
+          for (int q = k + 1; q < 3; ++q) pw[q] = a[p][q] - a[p][k] * a[k][q];
+          for (int q = k + 1; q < 3; ++q) a[p][q] = pw[q];
         }
       }
     } else {
-      for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 3; ++c) a[r][c] = 0.0;
+      for (int p = 0; p < 3; ++p)
+        for (int q = 0; q < 3; ++q) a[p][q] = 0.0;
     }
-    T* out = alu + grp * 9 * kWave + lane;
-    for (int r = 0; r < 3; ++r)
-      for (int c = 0; c < 3; ++c) out[(3 * r + c) * kWave] = (T)a[r][c];
+    T* out = alu + grp * 9 * rows + r;
+    for (int p = 0; p < 3; ++p)
+      for (int q = 0; q < 3; ++q) out[(3 * p + q) * rows] = (T)a[p][q];
   }
 }
 
@@ -743,7 +768,7 @@ void launch_dot_store(int64_t n, const double* x, const double* y, double* out, 
                       hipStream_t s) {
   const int g = vec_grid(n);
   k_dot<<<g, kBlock, 0, s>>>(n, x, y, nullptr, part);
-  k_tail<kTailStore, 1><<<1, kBlock, 0, s>>>(st, part, g, out, nullptr);
+  k_tail<kTailStore, 1><<<1, kTailThreads, 0, s>>>(st, part, g, out, nullptr);
 }
 void launch_fill_values(const Sell<double>& a, const int64_t* src, const double* D, const double* AL,
                         const double* AU, int64_t nD, int64_t nAL, hipStream_t s) {
@@ -753,33 +778,64 @@ void launch_fill_values_f32(const Sell<float>& a, const int64_t* src, const doub
                             const double* AU, int64_t nD, int64_t nAL, hipStream_t s) {
   k_fill_values<float><<<4096, 256, 0, s>>>(a, src, D, AL, AU, nD, nAL);
 }
-void launch_block_lu(int32_t nslots, const int32_t* row_src, const double* D, double sigma, double* alu,
+void launch_block_lu(int32_t nslots, const int32_t* row_src, const double* D, double sigma, double* alu, int rows,
                      hipStream_t s) {
-  k_block_lu<double><<<2048, 256, 0, s>>>(nslots, row_src, D, sigma, alu);
+  k_block_lu<double><<<2048, 256, 0, s>>>(nslots, row_src, D, sigma, alu, rows);
 }
-void launch_block_lu_f32(int32_t nslots, const int32_t* row_src, const double* D, double sigma, float* alu,
+void launch_block_lu_f32(int32_t nslots, const int32_t* row_src, const double* D, double sigma, float* alu, int rows,
                          hipStream_t s) {
-  k_block_lu<float><<<2048, 256, 0, s>>>(nslots, row_src, D, sigma, alu);
+  k_block_lu<float><<<2048, 256, 0, s>>>(nslots, row_src, D, sigma, alu, rows);
 }
+// Blocks whose loads a sweep issues together (HECMW33_SSOR_CHUNK = 4, 8 or
+// 16; 8 measured fastest on the 100^3 cube).
+static int ssor_chunk_sel() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = getenv("HECMW33_SSOR_CHUNK");
+    const int c = e ? atoi(e) : 8;
+    v = (c == 4 || c == 16) ? c : 8;
+  }
+  return v;
+}
+
+template <typename T>
+static void ssor_fwd_any(const Sell<T>& L, const int32_t* row_old, const T* alu, int32_t s0, int32_t ns, const T* rin,
+                         T* z, const KrylovState* st, hipStream_t s) {
+  if (ns <= 0) return;
+  const int g = spmv_grid(ns);
+  switch (ssor_chunk_sel()) {
+    case 4: k_ssor_fwd<T, 4><<<g, kBlock, 0, s>>>(L, row_old, alu, s0, ns, rin, z, st); break;
+    case 16: k_ssor_fwd<T, 16><<<g, kBlock, 0, s>>>(L, row_old, alu, s0, ns, rin, z, st); break;
+    default: k_ssor_fwd<T, 8><<<g, kBlock, 0, s>>>(L, row_old, alu, s0, ns, rin, z, st); break;
+  }
+}
+
+template <typename T, bool RHO>
+static void ssor_bwd_any(const Sell<T>& U, const int32_t* row_old, const T* alu, int32_t s0, int32_t ns, T* z,
+                         const double* r_rho, const KrylovState* st, double* part, int pbase, int g, hipStream_t s) {
+  if (ns <= 0) return;
+  switch (ssor_chunk_sel()) {
+    case 4: k_ssor_bwd<T, 4, RHO><<<g, kBlock, 0, s>>>(U, row_old, alu, s0, ns, z, r_rho, st, part, pbase); break;
+    case 16: k_ssor_bwd<T, 16, RHO><<<g, kBlock, 0, s>>>(U, row_old, alu, s0, ns, z, r_rho, st, part, pbase); break;
+    default: k_ssor_bwd<T, 8, RHO><<<g, kBlock, 0, s>>>(U, row_old, alu, s0, ns, z, r_rho, st, part, pbase); break;
+  }
+}
+
 void launch_ssor_fwd(const Sell<double>& L, const int32_t* row_old, const double* alu, int32_t s0, int32_t ns,
                      const double* rin, double* z, const KrylovState* st, hipStream_t s) {
-  if (ns <= 0) return;
-  k_ssor_fwd<double><<<spmv_grid(ns), kBlock, 0, s>>>(L, row_old, alu, s0, ns, rin, z, st);
+  ssor_fwd_any<double>(L, row_old, alu, s0, ns, rin, z, st, s);
 }
 void launch_ssor_bwd(const Sell<double>& U, const int32_t* row_old, const double* alu, int32_t s0, int32_t ns,
                      double* z, const KrylovState* st, hipStream_t s) {
-  if (ns <= 0) return;
-  k_ssor_bwd<double, false><<<spmv_grid(ns), kBlock, 0, s>>>(U, row_old, alu, s0, ns, z, nullptr, st, nullptr, 0);
+  ssor_bwd_any<double, false>(U, row_old, alu, s0, ns, z, nullptr, st, nullptr, 0, spmv_grid(ns), s);
 }
 void launch_ssor_fwd_f32(const Sell<float>& L, const int32_t* row_old, const float* alu, int32_t s0, int32_t ns,
                          const float* rin, float* z, hipStream_t s) {
-  if (ns <= 0) return;
-  k_ssor_fwd<float><<<spmv_grid(ns), kBlock, 0, s>>>(L, row_old, alu, s0, ns, rin, z, nullptr);
+  ssor_fwd_any<float>(L, row_old, alu, s0, ns, rin, z, nullptr, s);
 }
 void launch_ssor_bwd_f32(const Sell<float>& U, const int32_t* row_old, const float* alu, int32_t s0, int32_t ns,
                          float* z, hipStream_t s) {
-  if (ns <= 0) return;
-  k_ssor_bwd<float, false><<<spmv_grid(ns), kBlock, 0, s>>>(U, row_old, alu, s0, ns, z, nullptr, nullptr, nullptr, 0);
+  ssor_bwd_any<float, false>(U, row_old, alu, s0, ns, z, nullptr, nullptr, nullptr, 0, spmv_grid(ns), s);
 }
 void launch_diag_apply(int32_t n, const double* alu, const double* r, double* z, const KrylovState* st,
                        hipStream_t s) {
@@ -812,26 +868,26 @@ void cg_precond_ssor_rho(const Sell<double>& L, const Sell<double>& U, const int
     const int ns = color_slice[c + 1] - color_slice[c];
     if (ns <= 0) continue;
     const int g = std::min(gcap, spmv_grid(ns));
-    k_ssor_bwd<double, true><<<g, kBlock, 0, s>>>(U, row_old, alu, color_slice[c], ns, z, r, st, part, pbase);
+    ssor_bwd_any<double, true>(U, row_old, alu, color_slice[c], ns, z, r, st, part, pbase, g, s);
     pbase += g;
   }
-  k_tail<kTailCgRho, 1><<<1, kBlock, 0, s>>>(st, part, pbase, nullptr, nullptr);
+  k_tail<kTailCgRho, 1><<<1, kTailThreads, 0, s>>>(st, part, pbase, nullptr, nullptr);
 }
 void cg_precond_diag_rho(int32_t n, const double* alu, const double* r, double* z, KrylovState* st, double* part,
                          hipStream_t s) {
   const int g = vec_grid(n);
   k_diag<true><<<g, kBlock, 0, s>>>(n, alu, r, z, st, part);
-  k_tail<kTailCgRho, 1><<<1, kBlock, 0, s>>>(st, part, g, nullptr, nullptr);
+  k_tail<kTailCgRho, 1><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, nullptr);
 }
 void cg_precond_none_rho(int64_t n3, const double* r, double* z, KrylovState* st, double* part, hipStream_t s) {
   const int g = vec_grid(n3);
   k_copy_dot<<<g, kBlock, 0, s>>>(n3, r, z, st, part, 1);
-  k_tail<kTailCgRho, 1><<<1, kBlock, 0, s>>>(st, part, g, nullptr, nullptr);
+  k_tail<kTailCgRho, 1><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, nullptr);
 }
 void cg_rho(int64_t n3, const double* r, const double* z, KrylovState* st, double* part, hipStream_t s) {
   const int g = vec_grid(n3);
   k_dot<<<g, kBlock, 0, s>>>(n3, r, z, st, part);
-  k_tail<kTailCgRho, 1><<<1, kBlock, 0, s>>>(st, part, g, nullptr, nullptr);
+  k_tail<kTailCgRho, 1><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, nullptr);
 }
 void cg_xpay(int64_t n3, const double* z, double* p, const KrylovState* st, hipStream_t s) {
   k_cg_xpay<<<vec_grid(n3), kBlock, 0, s>>>(n3, z, p, st);
@@ -839,13 +895,13 @@ void cg_xpay(int64_t n3, const double* z, double* p, const KrylovState* st, hipS
 void cg_spmv_dot(const Sell<double>& a, const double* p, double* q, KrylovState* st, double* part, hipStream_t s) {
   const int g = spmv_grid(a.nslices);
   k_spmv<kEpiDotX><<<g, kBlock, 0, s>>>(a, p, nullptr, q, st, part);
-  k_tail<kTailCgC1, 1><<<1, kBlock, 0, s>>>(st, part, g, nullptr, nullptr);
+  k_tail<kTailCgC1, 1><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, nullptr);
 }
 void cg_update(int64_t n3, const double* p, const double* q, double* x, double* r, KrylovState* st, double* part,
                double* hist, hipStream_t s) {
   const int g = vec_grid(n3);
   k_cg_update<<<g, kBlock, 0, s>>>(n3, p, q, x, r, st, part);
-  k_tail<kTailDnrm2, 1><<<1, kBlock, 0, s>>>(st, part, g, nullptr, hist);
+  k_tail<kTailDnrm2, 1><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, hist);
 }
 // only the non-periodic iterations produce the dnrm2 partials; the tail is
 // guarded the same way
@@ -853,14 +909,14 @@ void cg_true_resid(const Sell<double>& a, const double* x, const double* b, doub
                    double* hist, hipStream_t s) {
   const int g = spmv_grid(a.nslices);
   k_spmv<kEpiTrue><<<g, kBlock, 0, s>>>(a, x, b, r, st, part);
-  k_tail_true<<<1, kBlock, 0, s>>>(st, part, g, hist);
+  k_tail_true<<<1, kTailThreads, 0, s>>>(st, part, g, hist);
 }
 
 // ---- BiCGSTAB iteration pieces (hecmw_solver_BiCGSTAB.f90:164-275) --------------------
 void bi_rho(int64_t n3, const double* r, const double* rt, KrylovState* st, double* part, hipStream_t s) {
   const int g = vec_grid(n3);
   k_dot<<<g, kBlock, 0, s>>>(n3, r, rt, st, part);
-  k_tail<kTailBiRho, 1><<<1, kBlock, 0, s>>>(st, part, g, nullptr, nullptr);
+  k_tail<kTailBiRho, 1><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, nullptr);
 }
 void bi_p(int64_t n3, const double* r, const double* v, double* p, const KrylovState* st, hipStream_t s) {
   k_bi_p<<<vec_grid(n3), kBlock, 0, s>>>(n3, r, v, p, st);
@@ -869,7 +925,7 @@ void bi_spmv_c2(const Sell<double>& a, const double* pt, const double* rt, doubl
                 hipStream_t s) {
   const int g = spmv_grid(a.nslices);
   k_spmv<kEpiDotW><<<g, kBlock, 0, s>>>(a, pt, rt, v, st, part);
-  k_tail<kTailBiC2, 1><<<1, kBlock, 0, s>>>(st, part, g, nullptr, nullptr);
+  k_tail<kTailBiC2, 1><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, nullptr);
 }
 void bi_s(int64_t n3, const double* v, const double* r, double* sv, const KrylovState* st, hipStream_t s) {
   k_bi_s<<<vec_grid(n3), kBlock, 0, s>>>(n3, v, r, sv, st);
@@ -878,13 +934,13 @@ void bi_spmv_omega(const Sell<double>& a, const double* stv, const double* sv, d
                    double* part, hipStream_t s) {
   const int g = spmv_grid(a.nslices);
   k_spmv<kEpiDot2W><<<g, kBlock, 0, s>>>(a, stv, sv, t, st, part);
-  k_tail<kTailBiOmega, 2><<<1, kBlock, 0, s>>>(st, part, g, nullptr, nullptr);
+  k_tail<kTailBiOmega, 2><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, nullptr);
 }
 void bi_update(int64_t n3, const double* pt, const double* stv, const double* t, const double* sv, double* x,
                double* r, KrylovState* st, double* part, double* hist, hipStream_t s) {
   const int g = vec_grid(n3);
   k_bi_update<<<g, kBlock, 0, s>>>(n3, pt, stv, t, sv, x, r, st, part);
-  k_tail<kTailDnrm2, 1><<<1, kBlock, 0, s>>>(st, part, g, nullptr, hist);
+  k_tail<kTailDnrm2, 1><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, hist);
 }
 // generic preconditioner application z = M^-1 r for BiCGSTAB (no fused dot)
 void precond_apply(int32_t kind, const Sell<double>& L, const Sell<double>& U, const int32_t* row_old,

@@ -18,6 +18,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -82,7 +83,7 @@ struct DMem {
 };
 
 struct SellHost {
-  int32_t nslices = 0, nrows = 0;
+  int32_t nslices = 0, nrows = 0, rows = kRowsA;
   std::vector<int32_t> ptr, len, col;
   std::vector<int64_t> src;
 };
@@ -93,12 +94,13 @@ struct SellDev {
   Sell<float> view32;
 };
 
-// Build SELL-64 storage for rows given lane by lane (lane_rows[s*64+l] = row
-// key or -1).  count(key) = blocks in that row; fill(key, emit) calls
-// emit(k, col, src) for k = 0..count-1 in storage order.
-template <class Count, class Fill>
+// Build SELL-C storage (C = kRows) for rows given slot by slot
+// (lane_rows[s*C+r] = row key or -1).  count(key) = blocks in that row;
+// fill(key, emit) calls emit(k, col, src) for k = 0..count-1 in storage order.
+template <int kRows, class Count, class Fill>
 static void build_sell(const std::vector<int32_t>& lane_rows, int32_t nrows, Count count, Fill fill, SellHost& h) {
-  const int32_t ns = (int32_t)(lane_rows.size() / kWave);
+  h.rows = kRows;
+  const int32_t ns = (int32_t)(lane_rows.size() / kRows);
   h.nslices = ns;
   h.nrows = nrows;
   h.len.assign(lane_rows.size(), 0);
@@ -107,27 +109,27 @@ static void build_sell(const std::vector<int32_t>& lane_rows, int32_t nrows, Cou
   for (int64_t t = 0; t < (int64_t)lane_rows.size(); ++t) h.len[t] = lane_rows[t] >= 0 ? count(lane_rows[t]) : 0;
   for (int32_t s = 0; s < ns; ++s) {
     int32_t w = 0;
-    for (int l = 0; l < kWave; ++l) w = std::max(w, h.len[(int64_t)s * kWave + l]);
+    for (int l = 0; l < kRows; ++l) w = std::max(w, h.len[(int64_t)s * kRows + l]);
     h.ptr[s + 1] = h.ptr[s] + w;
   }
-  const int64_t nslots = (int64_t)h.ptr[ns] * kWave;
+  const int64_t nslots = (int64_t)h.ptr[ns] * kRows;
   h.col.assign(nslots, 0);
   h.src.assign(nslots, -1);
 #pragma omp parallel for schedule(dynamic, 64)
   for (int32_t s = 0; s < ns; ++s)
-    for (int l = 0; l < kWave; ++l) {
-      const int32_t key = lane_rows[(int64_t)s * kWave + l];
+    for (int l = 0; l < kRows; ++l) {
+      const int32_t key = lane_rows[(int64_t)s * kRows + l];
       if (key < 0) continue;
-      const int64_t base = (int64_t)h.ptr[s] * kWave + l;
+      const int64_t base = (int64_t)h.ptr[s] * kRows + l;
       fill(key, [&](int32_t k, int32_t c, int64_t sidx) {
-        h.col[base + (int64_t)k * kWave] = c;
-        h.src[base + (int64_t)k * kWave] = sidx;
+        h.col[base + (int64_t)k * kRows] = c;
+        h.src[base + (int64_t)k * kRows] = sidx;
       });
     }
 }
 
 static int32_t upload_sell(const SellHost& h, SellDev& d, hipStream_t s) {
-  const int64_t nslots = (int64_t)h.ptr[h.nslices] * kWave;
+  const int64_t nslots = (int64_t)h.ptr[h.nslices] * h.rows;
   if (d.ptr.ensure(sizeof(int32_t) * (h.nslices + 1)) || d.len.ensure(sizeof(int32_t) * h.len.size()) ||
       d.col.ensure(sizeof(int32_t) * std::max<int64_t>(nslots, 1)) ||
       d.src.ensure(sizeof(int64_t) * std::max<int64_t>(nslots, 1)) ||
@@ -141,6 +143,8 @@ static int32_t upload_sell(const SellHost& h, SellDev& d, hipStream_t s) {
   }
   d.view.nslices = h.nslices;
   d.view.nrows = h.nrows;
+  d.view.rows = h.rows;
+  d.view32.rows = h.rows;
   d.view.nslots = nslots;
   d.view.ptr = d.ptr.as<int32_t>();
   d.view.len = d.len.as<int32_t>();
@@ -258,8 +262,8 @@ static int32_t load_structure(Context& c, const hecmw33_sys* A, bool force) {
       set_error("itemU out of range");
       return HECMW33_ERROR_ARGS;
     }
-  const int32_t ns = (N + kWave - 1) / kWave;
-  std::vector<int32_t> lanes((int64_t)ns * kWave, -1);
+  const int32_t ns = (N + kRowsA - 1) / kRowsA;
+  std::vector<int32_t> lanes((int64_t)ns * kRowsA, -1);
   for (int32_t i = 0; i < N; ++i) lanes[i] = i;
   const int32_t* iL = A->indexL;
   const int32_t* iU = A->indexU;
@@ -267,7 +271,7 @@ static int32_t load_structure(Context& c, const hecmw33_sys* A, bool force) {
   const int32_t* tU = A->itemU;
   const int64_t offL = N, offU = (int64_t)N + A->NPL;
   SellHost h;
-  build_sell(
+  build_sell<kRowsA>(
       lanes, N, [&](int32_t i) { return 1 + (iL[i + 1] - iL[i]) + (iU[i + 1] - iU[i]); },
       [&](int32_t i, auto emit) {
         int32_t k = 0;
@@ -328,16 +332,27 @@ static int32_t setup_ssor(Context& c, double sigma, int32_t ncolor_in) {
   c.color_slice.assign(c.ncolor + 1, 0);
   for (int32_t k = 0; k < c.ncolor; ++k) {
     const int32_t cnt = c.colorindex[k + 1] - c.colorindex[k];
-    c.color_slice[k + 1] = c.color_slice[k] + (cnt + kWave - 1) / kWave;
+    c.color_slice[k + 1] = c.color_slice[k] + (cnt + kRowsS - 1) / kRowsS;
   }
   const int32_t ns = c.color_slice[c.ncolor];
-  std::vector<int32_t> lanes((int64_t)ns * kWave, -1), row_old((int64_t)ns * kWave, -1);
-  for (int32_t k = 0; k < c.ncolor; ++k)
-    for (int32_t inew = c.colorindex[k]; inew < c.colorindex[k + 1]; ++inew) {
-      const int64_t t = (int64_t)c.color_slice[k] * kWave + (inew - c.colorindex[k]);
+  std::vector<int32_t> lanes((int64_t)ns * kRowsS, -1), row_old((int64_t)ns * kRowsS, -1);
+  // Rows of one colour are independent, so the lane that sweeps a row is
+  // free; optionally (HECMW33_SSOR_SORT=1) a colour's rows are dealt in
+  // natural node order so neighbouring lanes gather neighbouring nodes.
+  const char* sort_env = getenv("HECMW33_SSOR_SORT");
+  const bool sort_rows = sort_env && atoi(sort_env);
+  std::vector<int32_t> order;
+  for (int32_t k = 0; k < c.ncolor; ++k) {
+    order.resize(c.colorindex[k + 1] - c.colorindex[k]);
+    for (int32_t q = 0; q < (int32_t)order.size(); ++q) order[q] = c.colorindex[k] + q;
+    if (sort_rows) std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return c.perm[a] < c.perm[b]; });
+    for (int32_t q = 0; q < (int32_t)order.size(); ++q) {
+      const int32_t inew = order[q];
+      const int64_t t = (int64_t)c.color_slice[k] * kRowsS + q;
       lanes[t] = inew;                    // 0-based new position
       row_old[t] = c.perm[inew] - 1;      // 0-based old id
     }
+  }
   const int64_t offL = N, offU = (int64_t)N + c.NPL;
   const std::vector<int32_t>& perm = c.perm;
   // source block of A(iold, kold) inside the caller's AL/AU (row iold)
@@ -352,7 +367,7 @@ static int32_t setup_ssor(Context& c, double sigma, int32_t ncolor_in) {
     return -1;
   };
   SellHost hl, hu;
-  build_sell(
+  build_sell<kRowsS>(
       lanes, N, [&](int32_t inew) { return iLp[inew + 1] - iLp[inew]; },
       [&](int32_t inew, auto emit) {
         const int32_t iold1 = perm[inew];
@@ -363,7 +378,7 @@ static int32_t setup_ssor(Context& c, double sigma, int32_t ncolor_in) {
         }
       },
       hl);
-  build_sell(
+  build_sell<kRowsS>(
       lanes, N, [&](int32_t inew) { return iUp[inew + 1] - iUp[inew]; },
       [&](int32_t inew, auto emit) {
         const int32_t iold1 = perm[inew];
@@ -385,7 +400,7 @@ static int32_t setup_ssor(Context& c, double sigma, int32_t ncolor_in) {
   launch_fill_values(c.U.view, c.U.src.as<int64_t>(), c.dD.as<double>(), c.dAL.as<double>(), c.dAU.as<double>(), N,
                      c.NPL, c.stream);
   launch_block_lu((int32_t)row_old.size(), c.row_old.as<int32_t>(), c.dD.as<double>(), sigma,
-                  c.alu_ssor.as<double>(), c.stream);
+                  c.alu_ssor.as<double>(), kRowsS, c.stream);
   HIPC(hipGetLastError());
   HIPC(hipStreamSynchronize(c.stream));
   c.ssor_init = true;
@@ -400,7 +415,7 @@ static int32_t setup_diag(Context& c, double sigma) {
     return HECMW33_ERROR_DEVICE;
   HIPC(hipMemcpyAsync(c.row_diag.p, rows.data(), sizeof(int32_t) * rows.size(), hipMemcpyHostToDevice, c.stream));
   launch_block_lu((int32_t)rows.size(), c.row_diag.as<int32_t>(), c.dD.as<double>(), sigma, c.alu_diag.as<double>(),
-                  c.stream);
+                  kWave, c.stream);
   HIPC(hipGetLastError());
   HIPC(hipStreamSynchronize(c.stream));
   c.diag_init = true;

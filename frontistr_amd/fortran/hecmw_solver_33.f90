@@ -1,0 +1,156 @@
+!-------------------------------------------------------------------------------
+! hecmw_solve_33: GPU (MI355X) drop-in for hecmw1's NDOF=3 iterative solve.
+!
+! Same argument list as hecmw_solve_iterative
+! (hecmw1/src/solver/iterative/hecmw_solver_Iterative.f90:13) and the name the
+! reference's commented-out NDOF dispatch still carries
+! (hecmw1/src/solver/main/hecmw_solver.f90:113).  The hecmwST_matrix arrays are
+! handed to libhecmw33.so through iso_c_binding without copies; the library
+! returns the reference's error codes and writes the same Iarray/Rarray slots
+! the reference writes (recycle counters 96-98, converged/diverged 81/82,
+! SIGMA_DIAG during the automatic retry).  Here the codes are handled as
+! hecmw_solve_iterative handles them: hecmw_solve_error (which stops on
+! 1001/2001) and the ITERLOG/TIMELOG lines.
+!
+! hecmw33_available() tells a caller whether the GPU path can take a system
+! (library loaded, a device visible, METHOD 1/2, PRECOND 1/2/3, no MPC, one
+! domain); otherwise the caller keeps the reference path.
+!-------------------------------------------------------------------------------
+module hecmw_solver_33
+  use iso_c_binding
+  use hecmw_util
+  implicit none
+  private
+  public :: hecmw_solve_33
+  public :: hecmw33_available
+
+  type, bind(c) :: hecmw33_sys
+    integer(c_int32_t) :: N, NP, NPL, NPU
+    type(c_ptr) :: D, AL, AU, indexL, indexU, itemL, itemU
+  end type hecmw33_sys
+
+  type, bind(c) :: hecmw33_out
+    integer(c_int32_t) :: iter, error, method, ncolor
+    real(c_double) :: resid, resid_true, Tset, Tsol, Tcomm, time_Ax, time_precond, t_upload
+  end type hecmw33_out
+
+  interface
+    function c_hecmw33_solve(A, B, X, iarray, rarray, out) bind(c, name='hecmw33_solve') result(rc)
+      import :: hecmw33_sys, hecmw33_out, c_ptr, c_int32_t, c_double
+      type(hecmw33_sys), intent(in) :: A
+      type(c_ptr), value :: B, X
+      integer(c_int32_t), intent(inout) :: iarray(100)
+      real(c_double), intent(inout) :: rarray(100)
+      type(hecmw33_out), intent(out) :: out
+      integer(c_int32_t) :: rc
+    end function c_hecmw33_solve
+
+    function c_hecmw33_device_count() bind(c, name='hecmw33_device_count') result(n)
+      import :: c_int32_t
+      integer(c_int32_t) :: n
+    end function c_hecmw33_device_count
+
+    function c_hecmw33_last_history(hist, cap) bind(c, name='hecmw33_last_history') result(n)
+      import :: c_double, c_int32_t
+      real(c_double), intent(out) :: hist(*)
+      integer(c_int32_t), value :: cap
+      integer(c_int32_t) :: n
+    end function c_hecmw33_last_history
+  end interface
+
+contains
+
+  logical function hecmw33_available(hecMESH, hecMAT)
+    type(hecmwST_local_mesh) :: hecMESH
+    type(hecmwST_matrix) :: hecMAT
+    integer(kind=kint) :: method, precond
+    method = hecMAT%Iarray(2)
+    precond = hecMAT%Iarray(3)
+    hecmw33_available = hecMAT%NDOF == 3 .and. (method == 1 .or. method == 2) &
+      .and. (precond == 1 .or. precond == 2 .or. precond == 3 .or. hecMAT%Iarray(5) <= 0) &
+      .and. hecMESH%n_neighbor_pe == 0 .and. hecMESH%mpc%n_mpc == 0 &
+      .and. hecmw_mat_get_scaling_33(hecMAT) == 0
+    if (hecmw33_available) hecmw33_available = c_hecmw33_device_count() > 0
+  end function hecmw33_available
+
+  integer(kind=kint) function hecmw_mat_get_scaling_33(hecMAT)
+    type(hecmwST_matrix) :: hecMAT
+    hecmw_mat_get_scaling_33 = hecMAT%Iarray(7)
+  end function hecmw_mat_get_scaling_33
+
+  subroutine hecmw_solve_33(hecMESH, hecMAT)
+    use m_hecmw_solve_error
+    type(hecmwST_local_mesh) :: hecMESH
+    type(hecmwST_matrix), target :: hecMAT
+    type(hecmw33_sys) :: A
+    type(hecmw33_out) :: out
+    integer(c_int32_t) :: rc, ia(100), nh, i
+    real(c_double) :: ra(100)
+    real(c_double), allocatable :: hist(:)
+    real(kind=kreal) :: TR
+
+    A%N = hecMAT%N
+    A%NP = hecMAT%NP
+    A%NPL = hecMAT%indexL(hecMAT%NP)
+    A%NPU = hecMAT%indexU(hecMAT%NP)
+    A%D = c_loc(hecMAT%D(1))
+    A%AL = c_null_ptr
+    A%AU = c_null_ptr
+    A%itemL = c_null_ptr
+    A%itemU = c_null_ptr
+    if (A%NPL > 0) then
+      A%AL = c_loc(hecMAT%AL(1))
+      A%itemL = c_loc(hecMAT%itemL(1))
+    endif
+    if (A%NPU > 0) then
+      A%AU = c_loc(hecMAT%AU(1))
+      A%itemU = c_loc(hecMAT%itemU(1))
+    endif
+    A%indexL = c_loc(hecMAT%indexL(0))
+    A%indexU = c_loc(hecMAT%indexU(0))
+    ia(:) = hecMAT%Iarray(:)
+    ra(:) = hecMAT%Rarray(:)
+
+    rc = c_hecmw33_solve(A, c_loc(hecMAT%B(1)), c_loc(hecMAT%X(1)), ia, ra, out)
+    hecMAT%Iarray(:) = ia(:)
+    hecMAT%Rarray(:) = ra(:)
+
+    if (rc < 0) then
+      write(*,'(a,i0)') '#### hecmw_solve_33: GPU library failure ', rc
+      call hecmw_abort(hecmw_comm_get_comm())
+    endif
+    if (rc == HECMW_SOLVER_ERROR_ZERO_DIAG .or. rc == HECMW_SOLVER_ERROR_INCONS_PC) then
+      call hecmw_solve_error(hecMESH, rc)
+    endif
+
+    ! per-iteration residuals, as hecmw_solve_CG prints them (hecmw_solver_CG.f90:253)
+    if (hecMESH%my_rank == 0 .and. hecMAT%Iarray(21) == 1) then
+      allocate(hist(max(1, out%iter)))
+      nh = c_hecmw33_last_history(hist, int(size(hist), c_int32_t))
+      do i = 1, nh
+        write(*,'(i7, 1pe16.6)') i, hist(i)
+      enddo
+      deallocate(hist)
+    endif
+
+    if (out%error /= 0) call hecmw_solve_error(hecMESH, out%error)
+
+    ! post-solve lines of hecmw_solve_iterative (hecmw_solver_Iterative.f90:155-183)
+    if (hecMESH%my_rank == 0 .and. (hecMAT%Iarray(21) == 1 .or. hecMAT%Iarray(22) >= 1)) then
+      write(*,"(a,1pe12.5)") '### Relative residual =', out%resid_true
+    endif
+    if (hecMESH%my_rank == 0 .and. hecMAT%Iarray(22) >= 1) then
+      TR = (out%Tsol - out%Tcomm) / (out%Tsol + 1.d-24) * 100.d0
+      write(*,'(/a)')          '### summary of linear solver'
+      write(*,'(i10,a, 1pe16.6)')      out%iter, ' iterations  ', out%resid
+      write(*,'(a, 1pe16.6 )') '    set-up time      : ', out%Tset
+      write(*,'(a, 1pe16.6 )') '    solver time      : ', out%Tsol
+      write(*,'(a, 1pe16.6 )') '    solver/comm time : ', out%Tcomm
+      write(*,'(a, 1pe16.6 )') '    solver/matvec    : ', out%time_Ax
+      write(*,'(a, 1pe16.6 )') '    solver/precond   : ', out%time_precond
+      if (out%iter > 0) write(*,'(a, 1pe16.6 )') '    solver/1 iter    : ', out%Tsol / out%iter
+      write(*,'(a, 1pe16.6/)') '    work ratio (%)   : ', TR
+    endif
+  end subroutine hecmw_solve_33
+
+end module hecmw_solver_33

@@ -132,7 +132,29 @@ def main():
          "-Wl,--no-undefined", "-Wl,--version-script=" + vers])
     os.replace(so + ".tmp", so)
     print("build_ref: built %s from %d Fortran + %d C reference files" % (so, len(fsrc), len(C_FILES)))
+    build_dropin(mod, obj, lib_a)
     return 0
+
+
+def build_dropin(mod, obj, lib_a):
+    """Test binary: the reference's objects + the hecmw_solve_33 Fortran module
+    + libhecmw33.so, driven by tests/fortran/dropin_driver.f90 (see there)."""
+    root = os.path.dirname(HERE)
+    libdir = os.path.join(root, "frontistr_amd", "lib")
+    if not os.path.exists(os.path.join(libdir, "libhecmw33.so")):
+        print("build_ref: libhecmw33.so not built, skipping the drop-in test binary")
+        return
+    shim = os.path.join(root, "frontistr_amd", "fortran", "hecmw_solver_33.f90")
+    drv = os.path.join(root, "tests", "fortran", "dropin_driver.f90")
+    shim_o = os.path.join(obj, "hecmw_solver_33.o")
+    drv_o = os.path.join(obj, "dropin_driver.o")
+    run(["flang"] + FFLAGS + ["-module-dir", mod, "-I", mod, "-c", shim, "-o", shim_o])
+    run(["flang"] + FFLAGS + ["-module-dir", mod, "-I", mod, "-c", drv, "-o", drv_o])
+    exe = os.path.join(OUT, "dropin_test")
+    run(["flang", "-fopenmp", "-o", exe + ".tmp", drv_o, shim_o, lib_a, "-L", libdir, "-lhecmw33",
+         "-Wl,-rpath,$ORIGIN/../../frontistr_amd/lib", "-Wl,--no-undefined"])
+    os.replace(exe + ".tmp", exe)
+    print("build_ref: built %s" % exe)
 
 
 if __name__ == "__main__":

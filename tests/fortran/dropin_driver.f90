@@ -1,0 +1,105 @@
+!-------------------------------------------------------------------------------
+! Drop-in check (test infrastructure): one hecmwST_matrix, solved once by the
+! reference's own hecmw_solve_iterative (CPU, compiled from the FrontISTR
+! sources) and once by hecmw_solve_33 (GPU, frontistr_amd/fortran), both
+! called from Fortran exactly as fistr1 would call them.
+!
+!   dropin_test <n> <method> <precond>    (OMP_NUM_THREADS >= 2)
+! prints one line:  DROPIN n method precond conv_ref conv_gpu div_ref div_gpu relerr
+!-------------------------------------------------------------------------------
+program dropin_test
+  use iso_c_binding
+  use hecmw_util
+  use hecmw_matrix_misc
+  use hecmw_solver_iterative
+  use hecmw_solver_33
+  implicit none
+
+  interface
+    function gen_count(n, N_, NPL, NPU) bind(c, name='hecmw33_gen_hex8_count') result(rc)
+      import :: c_int32_t
+      integer(c_int32_t), value :: n
+      integer(c_int32_t), intent(out) :: N_, NPL, NPU
+      integer(c_int32_t) :: rc
+    end function gen_count
+    function gen(n, kind, p, seed, D, AL, AU, iL, tL, iU, tU, B, E) bind(c, name='hecmw33_gen_hex8') result(rc)
+      import :: c_int32_t, c_double, c_int64_t, c_ptr
+      integer(c_int32_t), value :: n, kind
+      real(c_double), intent(in) :: p(*)
+      integer(c_int64_t), value :: seed
+      real(c_double), intent(out) :: D(*), AL(*), AU(*), B(*)
+      integer(c_int32_t), intent(out) :: iL(*), tL(*), iU(*), tU(*)
+      type(c_ptr), value :: E
+      integer(c_int32_t) :: rc
+    end function gen
+  end interface
+
+  type(hecmwST_matrix), target :: mref, mgpu
+  type(hecmwST_local_mesh) :: mesh
+  integer(c_int32_t) :: n, N_, NPL, NPU, rc
+  integer :: method, precond
+  character(len=32) :: arg
+  real(c_double) :: p(8)
+  real(kind=kreal) :: relerr
+
+  call get_command_argument(1, arg); read(arg, *) n
+  call get_command_argument(2, arg); read(arg, *) method
+  call get_command_argument(3, arg); read(arg, *) precond
+
+  rc = gen_count(n, N_, NPL, NPU)
+  call setup(mref)
+  call setup(mgpu)
+  p = 0.d0
+  p(1) = 210000.d0
+  p(2) = 0.3d0
+  rc = gen(n, 0_c_int32_t, p, 0_c_int64_t, mref%D, mref%AL, mref%AU, mref%indexL, mref%itemL, &
+           mref%indexU, mref%itemU, mref%B, c_null_ptr)
+  mgpu%D = mref%D
+  mgpu%AL = mref%AL
+  mgpu%AU = mref%AU
+  mgpu%indexL = mref%indexL
+  mgpu%indexU = mref%indexU
+  mgpu%itemL = mref%itemL
+  mgpu%itemU = mref%itemU
+  mgpu%B = mref%B
+
+  mesh%nn_internal = N_
+  mesh%n_node = N_
+  mesh%n_neighbor_pe = 0
+  mesh%my_rank = 0
+  mesh%PETOT = 1
+  mesh%zero = 1
+  mesh%mpc%n_mpc = 0
+
+  call hecmw_solve_iterative(mesh, mref)
+  if (.not. hecmw33_available(mesh, mgpu)) then
+    write(*,'(a)') 'DROPIN unavailable'
+    stop 1
+  endif
+  call hecmw_solve_33(mesh, mgpu)
+  relerr = sqrt(sum((mgpu%X - mref%X)**2) / sum(mref%X**2))
+  write(*,'(a,3(1x,i0),4(1x,i0),1x,es12.4)') 'DROPIN', n, method, precond, &
+    hecmw_mat_get_flag_converged(mref), hecmw_mat_get_flag_converged(mgpu), &
+    hecmw_mat_get_flag_diverged(mref), hecmw_mat_get_flag_diverged(mgpu), relerr
+
+contains
+
+  subroutine setup(m)
+    type(hecmwST_matrix), target :: m
+    call hecmw_mat_init(m)
+    m%N = N_
+    m%NP = N_
+    m%NPL = NPL
+    m%NPU = NPU
+    m%NDOF = 3
+    allocate(m%D(9*N_), m%AL(9*NPL), m%AU(9*NPU), m%B(3*N_), m%X(3*N_))
+    allocate(m%indexL(0:N_), m%indexU(0:N_), m%itemL(NPL), m%itemU(NPU))
+    m%X = 0.d0
+    call hecmw_mat_set_iter(m, 5000)
+    call hecmw_mat_set_method(m, method)
+    call hecmw_mat_set_precond(m, precond)
+    call hecmw_mat_set_iterlog(m, 0)
+    call hecmw_mat_set_timelog(m, 0)
+  end subroutine setup
+
+end program dropin_test
