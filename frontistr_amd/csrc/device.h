@@ -53,6 +53,12 @@ struct KrylovState {
   int32_t recheck;   // recompute the true residual in this iteration
   int32_t nrec;      // true-residual period (50 CG, 100 BiCGSTAB)
   int32_t pad;
+  // flexible CG with FP32 operators (reliable updates)
+  double zq, pq, rp;   // z.q_prev, p.q, r.p of the current iteration
+  double rr_ref;       // |r|^2 at the last FP64 true-residual update
+  double delta2;       // update when |r|^2 <= delta2 * rr_ref
+  int32_t reliable;    // recompute r = b - A x in FP64 in this iteration
+  int32_t nreliable;   // FP64 true-residual updates so far
 };
 
 constexpr int kMaxPartials = 65536;  // per reduction component
@@ -130,5 +136,32 @@ void bi_spmv_omega(const Sell<double>& a, const double* stv, const double* sv, d
                    double* part, hipStream_t s);
 void bi_update(int64_t n3, const double* pt, const double* stv, const double* t, const double* sv, double* x,
                double* r, KrylovState* st, double* part, double* hist, hipStream_t s);
+
+// ---- flexible CG with FP32 operators and FP64 recurrences ------------------------
+// z32 = M^-1 r32 in FP32 (kind 0 copy, 1 SSOR, 3 DIAG)
+void precond_apply_f32(int32_t kind, const Sell<float>& L, const Sell<float>& U, const int32_t* row_old,
+                       const float* alu_ssor, const int32_t* color_slice, int32_t ncolor, const float* alu_diag,
+                       int32_t n, const float* r, float* z, const KrylovState* st, hipStream_t s);
+// z32 = M^-1 r32 fused with the FP64 dots r.z and z.q_prev; beta = -(z.q_prev)/(p.q)_prev
+// (Notay's FCG(1))
+void fcg_precond_dots(int32_t kind, const Sell<float>& L, const Sell<float>& U, const int32_t* row_old,
+                      const float* alu_ssor, const int32_t* color_slice, int32_t ncolor, const float* alu_diag,
+                      int32_t n, const float* r, float* z, const float* q, KrylovState* st, double* part,
+                      hipStream_t s);
+void fcg_p(int64_t n3, const float* z, float* p, const KrylovState* st, hipStream_t s);
+// q = A32 p, pq = p.q, rp = r.p, alpha = rp / pq
+void fcg_spmv(const Sell<float>& a, const float* p, const float* r, float* q, KrylovState* st, double* part,
+              hipStream_t s);
+// x64 += alpha p, r32 -= alpha q, |r|^2, reliable-update decision
+void fcg_update(int64_t n3, const float* p, const float* q, double* x, float* r, KrylovState* st, double* part,
+                double* hist, hipStream_t s);
+// conditional r32 = b - A64 x64 in FP64 with |r|^2, convergence, end of iteration
+void fcg_reliable(const Sell<double>& a, const double* x, const double* b, float* r, KrylovState* st, double* part,
+                  double* hist, hipStream_t s);
+// r32 = b - A64 x64 and |r|^2 into *rr
+void fcg_init_resid(const Sell<double>& a, const double* x, const double* b, float* r, double* rr, KrylovState* st,
+                    double* part, hipStream_t s);
+// r32 = (float) r64 over n entries
+void launch_to_f32(int64_t n, const double* x, float* y, hipStream_t s);
 
 }  // namespace hecmw33

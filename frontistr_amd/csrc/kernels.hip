@@ -114,6 +114,9 @@ enum Tail : int {
   kTailBiRho = 5,      // hecmw_solver_BiCGSTAB.f90:169-176
   kTailBiC2 = 6,       // :200-202
   kTailBiOmega = 7,    // :227-234
+  kTailFcgBeta = 8,    // flexible CG: rho = r.z, beta = -(z.q_prev)/(p.q)_prev
+  kTailFcgAlpha = 9,   // alpha = (r.p)/(p.q)
+  kTailFcgRes = 10,    // recurrence |r|^2, reliable-update decision
 };
 
 template <int TAIL, int K>
@@ -166,6 +169,48 @@ __device__ void run_tail(KrylovState* st, const double (&tot)[K], double* out, d
     st->ts = tot[0];
     st->tt = tot[K > 1 ? 1 : 0];
     st->omega = st->ts / st->tt;
+  } else if (TAIL == kTailFcgBeta) {
+    // Notay's FCG(1): beta keeps p A-orthogonal to the previous direction
+    // whatever the (FP32) preconditioner did; r.z only checks positivity
+    // with CG's rules.
+    const double rho = tot[0];
+    st->rho = rho;
+    st->zq = tot[K > 1 ? 1 : 0];
+    if (rho == 0.0) {
+      st->done = 1;
+    } else if (rho != rho) {
+      st->error = kErrDivergeNan;
+      st->done = 1;
+    } else if (st->iter > 1 && rho * st->rho1 <= 0.0) {
+      st->n_indef += 1;
+      if (st->n_indef >= 3) {
+        st->error = kErrDivergePc;
+        st->done = 1;
+      }
+    }
+    if (!st->done) st->beta = st->iter > 1 ? -st->zq / st->pq : 0.0;
+  } else if (TAIL == kTailFcgAlpha) {
+    const double pq = tot[0];
+    st->pq = pq;
+    st->rp = tot[K > 1 ? 1 : 0];
+    if (pq <= 0.0) {
+      st->error = kErrDivergeMat;
+      st->done = 1;
+    } else if (pq != pq || st->rp != st->rp) {
+      st->error = kErrDivergeNan;
+      st->done = 1;
+    } else {
+      st->alpha = st->rp / pq;
+    }
+  } else if (TAIL == kTailFcgRes) {
+    st->dnrm2 = tot[0];
+    st->resid = sqrt(tot[0] / st->bnrm2);
+    if (hist) hist[st->iter - 1] = st->resid;
+    // replace the FP32 recurrence residual by b - A x in FP64 when it has
+    // dropped by delta^2 in |r|^2, when it claims convergence, and every nrec
+    // iterations
+    st->reliable = st->resid <= st->tol || tot[0] <= st->delta2 * st->rr_ref || (st->iter % st->nrec) == 0 ||
+                   tot[0] != tot[0];
   }
 }
 
@@ -198,6 +243,42 @@ __global__ __launch_bounds__(kTailThreads) void k_tail_true(KrylovState* st, con
       if ((st->iter % st->nrec) == 0 && hist) hist[st->iter - 1] = st->resid;
       if (st->resid <= st->tol) st->done = 1;
       st->recheck = 0;
+    }
+  }
+  if (threadIdx.x == 0 && !st->done) {
+    if (st->iter >= st->maxit) {
+      st->error = kErrNoconv;
+      st->done = 1;
+      st->iter = st->maxit + 1;
+    } else {
+      st->rho1 = st->rho;
+      st->iter += 1;
+    }
+  }
+}
+
+// Flexible CG: after a reliable update (r = b - A x in FP64) the true
+// residual decides convergence and becomes the new reference norm; then the
+// end of the iteration as in k_tail_true.
+__global__ __launch_bounds__(kTailThreads) void k_tail_fcg_rel(KrylovState* st, const double* part, int nparts,
+                                                               double* hist) {
+  if (st->done) return;
+  if (st->reliable) {
+    double tot[1];
+    sum_partials<1>(part, nparts, tot);
+    if (threadIdx.x == 0) {
+      st->dnrm2 = tot[0];
+      st->rr_ref = tot[0];
+      st->resid = sqrt(tot[0] / st->bnrm2);
+      if (hist) hist[st->iter - 1] = st->resid;
+      st->nreliable += 1;
+      st->reliable = 0;
+      if (st->resid <= st->tol) {
+        st->done = 1;
+      } else if (st->resid != st->resid) {
+        st->error = kErrDivergeNan;
+        st->done = 1;
+      }
     }
   }
   if (threadIdx.x == 0 && !st->done) {
@@ -403,10 +484,16 @@ __global__ __launch_bounds__(kBlock) void k_spmv(Sell<double> a, const double* _
   if (EPI >= kEpiDotX) block_partials<K>(acc, part, blockIdx.x);
 }
 
+// FP32 SpMV; with DOTS (flexible CG: q = A p) also the FP64 partials of p.q
+// and r.p, r passed in w.
+template <bool DOTS>
 __global__ __launch_bounds__(kBlock) void k_spmv_f32(Sell<float> a, const float* __restrict__ x,
-                                                       float* __restrict__ y) {
+                                                       const float* __restrict__ w, float* __restrict__ y,
+                                                       const KrylovState* st, double* part) {
+  if (st && st->done) return;
   const int lane = lane_id();
   const SliceRange sr = wave_slices(a.nslices);
+  double acc[2] = {0.0, 0.0};
   for (int s = sr.begin; s < sr.end; ++s) {
     float yv[3];
     sell_row_f32<kSpmvChunk>(a, s, lane, x, yv);
@@ -415,8 +502,42 @@ __global__ __launch_bounds__(kBlock) void k_spmv_f32(Sell<float> a, const float*
       y[3 * row] = yv[0];
       y[3 * row + 1] = yv[1];
       y[3 * row + 2] = yv[2];
+      if (DOTS) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const double pv = x[3 * row + d];
+          acc[0] += pv * (double)yv[d];
+          acc[1] += (double)w[3 * row + d] * pv;
+        }
+      }
     }
   }
+  if (DOTS) block_partials<2>(acc, part, blockIdx.x);
+}
+
+// r32 = (float)(b - A x) with A, x, b in FP64 and the partials of |b - A x|^2;
+// with st, only when the flexible CG asked for a reliable update.
+__global__ __launch_bounds__(kBlock) void k_spmv_rel(Sell<double> a, const double* __restrict__ x,
+                                                       const double* __restrict__ b, float* __restrict__ r,
+                                                       const KrylovState* st, double* part) {
+  if (st && (st->done || !st->reliable)) return;
+  const int lane = lane_id();
+  const SliceRange sr = wave_slices(a.nslices);
+  double acc[1] = {0.0};
+  for (int s = sr.begin; s < sr.end; ++s) {
+    double yv[3], xo[3];
+    sell_row<kSpmvChunk>(a, s, lane, x, yv, xo);
+    const int64_t row = (int64_t)s * kRowsA + lane;
+    if (row < a.nrows) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double v = b[3 * row + d] - yv[d];
+        r[3 * row + d] = (float)v;
+        acc[0] += v * v;
+      }
+    }
+  }
+  block_partials<1>(acc, part, blockIdx.x);
 }
 
 // ---- SSOR sweeps (one launch per colour) ---------------------------------------
@@ -514,17 +635,20 @@ __global__ __launch_bounds__(kBlock) void k_ssor_fwd(Sell<T> L, const int32_t* _
 
 // Backward, colours in reverse:
 //   z[i] -= LU^-1 sum_{j in U'(i), descending} U'_ij z[j]
-// With RHO, r.z over this colour's rows (final now) goes to part[pbase+block].
-template <typename T, int CH, bool RHO>
+// ND fused dots over this colour's rows (final now): w0.z (and w1.z) go to
+// part[k*kMaxPartials + pbase + block], accumulated in FP64.
+template <typename T, int CH, int ND>
 __global__ __launch_bounds__(kBlock) void k_ssor_bwd(Sell<T> U, const int32_t* __restrict__ row_old,
                                                        const T* __restrict__ alu, int s0, int ns,
-                                                       T* __restrict__ z, const double* __restrict__ r_rho,
-                                                       const KrylovState* st, double* part, int pbase) {
+                                                       T* __restrict__ z, const T* __restrict__ w0,
+                                                       const T* __restrict__ w1, const KrylovState* st, double* part,
+                                                       int pbase) {
 #pragma clang fp contract(off)
   if (st && st->done) return;
   const int lane = lane_id(), r = lane >> 2, d = lane & 3, dd = d < 3 ? d : 2;
   const SliceRange sr = wave_slices(ns);
-  double acc[1] = {0.0};
+  constexpr int KD = ND > 0 ? ND : 1;
+  double acc[KD] = {};
   for (int si = sr.begin; si < sr.end; ++si) {
     const int s = s0 + si;
     const int64_t slot = (int64_t)s * kRowsS + r;
@@ -544,36 +668,42 @@ __global__ __launch_bounds__(kBlock) void k_ssor_bwd(Sell<T> U, const int32_t* _
     const T zn = zo - pick3(dd, x0, x1, x2);
     if (d < 3) {
       z[3 * (int64_t)iold + d] = zn;
-      if (RHO) acc[0] += r_rho[3 * (int64_t)iold + d] * (double)zn;
+      if constexpr (ND > 0) acc[0] += (double)w0[3 * (int64_t)iold + d] * (double)zn;
+      if constexpr (ND > 1) acc[KD - 1] += (double)w1[3 * (int64_t)iold + d] * (double)zn;
     }
   }
-  if (RHO) block_partials<1>(acc, part, pbase + blockIdx.x);
+  if constexpr (ND > 0) block_partials<KD>(acc, part, pbase + blockIdx.x);
 }
 
 // ---- DIAG (block Jacobi), hecmw_precond_DIAG_33.F90:106-142 --------------------
-template <bool RHO>
-__global__ __launch_bounds__(kBlock) void k_diag(int32_t n, const double* __restrict__ alu,
-                                                   const double* __restrict__ r, double* __restrict__ z,
+// ND fused dots: r.z (and w1.z), FP64 accumulation.
+template <typename T, int ND>
+__global__ __launch_bounds__(kBlock) void k_diag(int32_t n, const T* __restrict__ alu, const T* __restrict__ r,
+                                                   T* __restrict__ z, const T* __restrict__ w1,
                                                    const KrylovState* st, double* part) {
 #pragma clang fp contract(off)
   if (st && st->done) return;
   const int blk = xcd_block(blockIdx.x, gridDim.x);
-  double d[1] = {0.0};
+  constexpr int KD = ND > 0 ? ND : 1;
+  double d[KD] = {};
   for (int64_t i = (int64_t)blk * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
     const int64_t s = i / kWave, lane = i % kWave;
-    const double* f = alu + s * 9 * kWave + lane;
-    double a[9];
+    const T* f = alu + s * 9 * kWave + lane;
+    T a[9];
 #pragma unroll
     for (int e = 0; e < 9; ++e) a[e] = f[e * kWave];
-    const double r0 = r[3 * i], r1 = r[3 * i + 1], r2 = r[3 * i + 2];
-    double x0 = r0, x1 = r1, x2 = r2;
-    lu3_solve<double>(a, x0, x1, x2);
+    const T r0 = r[3 * i], r1 = r[3 * i + 1], r2 = r[3 * i + 2];
+    T x0 = r0, x1 = r1, x2 = r2;
+    lu3_solve<T>(a, x0, x1, x2);
     z[3 * i] = x0;
     z[3 * i + 1] = x1;
     z[3 * i + 2] = x2;
-    if (RHO) d[0] += r0 * x0 + r1 * x1 + r2 * x2;
+    if constexpr (ND > 0) d[0] += (double)r0 * (double)x0 + (double)r1 * (double)x1 + (double)r2 * (double)x2;
+    if constexpr (ND > 1)
+      d[KD - 1] += (double)w1[3 * i] * (double)x0 + (double)w1[3 * i + 1] * (double)x1 +
+                   (double)w1[3 * i + 2] * (double)x2;
   }
-  if (RHO) block_partials<1>(d, part, blockIdx.x);
+  if constexpr (ND > 0) block_partials<KD>(d, part, blockIdx.x);
 }
 
 // ---- vector kernels -----------------------------------------------------------
@@ -699,6 +829,53 @@ __global__ __launch_bounds__(kBlock) void k_bi_update(int64_t n, const double* _
   if (!periodic) block_partials<1>(d, part, blockIdx.x);
 }
 
+// ---- flexible CG vector kernels (FP32 vectors, FP64 scalars and x) ---------------
+// z = r with the partials of r.r and w1.r (no preconditioning)
+__global__ __launch_bounds__(kBlock) void k_copy_dots_f32(int64_t n, const float* __restrict__ r,
+                                                            float* __restrict__ z, const float* __restrict__ w1,
+                                                            const KrylovState* st, double* part, int ndots) {
+  if (st && st->done) return;
+  double d[2] = {0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const float v = r[i];
+    z[i] = v;
+    d[0] += (double)v * (double)v;
+    if (ndots > 1) d[1] += (double)w1[i] * (double)v;
+  }
+  if (ndots > 0) block_partials<2>(d, part, blockIdx.x);
+}
+
+// p = z (first iteration) or p = z + beta p
+__global__ __launch_bounds__(kBlock) void k_fcg_p(int64_t n, const float* __restrict__ z, float* __restrict__ p,
+                                                    const KrylovState* st) {
+  if (st->done) return;
+  const bool first = st->iter == 1;
+  const double beta = st->beta;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    p[i] = first ? z[i] : (float)((double)z[i] + beta * (double)p[i]);
+}
+
+// x += alpha p (FP64 x); r = r - alpha q rounded to FP32; partials of |r|^2
+__global__ __launch_bounds__(kBlock) void k_fcg_update(int64_t n, const float* __restrict__ p,
+                                                         const float* __restrict__ q, double* __restrict__ x,
+                                                         float* __restrict__ r, const KrylovState* st, double* part) {
+  if (st->done) return;
+  const double alpha = st->alpha;
+  double d[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    x[i] = x[i] + alpha * (double)p[i];
+    const float v = (float)((double)r[i] - alpha * (double)q[i]);
+    r[i] = v;
+    d[0] += (double)v * (double)v;
+  }
+  block_partials<1>(d, part, blockIdx.x);
+}
+
+__global__ __launch_bounds__(kBlock) void k_to_f32(int64_t n, const double* __restrict__ x, float* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    y[i] = (float)x[i];
+}
+
 // ---- numeric setup ---------------------------------------------------------------
 template <typename T>
 __global__ void k_fill_values(Sell<T> a, const int64_t* __restrict__ src, const double* __restrict__ D,
@@ -759,7 +936,7 @@ void launch_spmv(const Sell<double>& a, const double* x, double* y, hipStream_t 
   k_spmv<kEpiStore><<<spmv_grid(a.nslices), kBlock, 0, s>>>(a, x, nullptr, y, nullptr, nullptr);
 }
 void launch_spmv_f32(const Sell<float>& a, const float* x, float* y, hipStream_t s) {
-  k_spmv_f32<<<spmv_grid(a.nslices), kBlock, 0, s>>>(a, x, y);
+  k_spmv_f32<false><<<spmv_grid(a.nslices), kBlock, 0, s>>>(a, x, nullptr, y, nullptr, nullptr);
 }
 void launch_matresid(const Sell<double>& a, const double* x, const double* b, double* r, hipStream_t s) {
   k_spmv<kEpiResid><<<spmv_grid(a.nslices), kBlock, 0, s>>>(a, x, b, r, nullptr, nullptr);
@@ -810,14 +987,15 @@ static void ssor_fwd_any(const Sell<T>& L, const int32_t* row_old, const T* alu,
   }
 }
 
-template <typename T, bool RHO>
+template <typename T, int ND>
 static void ssor_bwd_any(const Sell<T>& U, const int32_t* row_old, const T* alu, int32_t s0, int32_t ns, T* z,
-                         const double* r_rho, const KrylovState* st, double* part, int pbase, int g, hipStream_t s) {
+                         const T* w0, const T* w1, const KrylovState* st, double* part, int pbase, int g,
+                         hipStream_t s) {
   if (ns <= 0) return;
   switch (ssor_chunk_sel()) {
-    case 4: k_ssor_bwd<T, 4, RHO><<<g, kBlock, 0, s>>>(U, row_old, alu, s0, ns, z, r_rho, st, part, pbase); break;
-    case 16: k_ssor_bwd<T, 16, RHO><<<g, kBlock, 0, s>>>(U, row_old, alu, s0, ns, z, r_rho, st, part, pbase); break;
-    default: k_ssor_bwd<T, 8, RHO><<<g, kBlock, 0, s>>>(U, row_old, alu, s0, ns, z, r_rho, st, part, pbase); break;
+    case 4: k_ssor_bwd<T, 4, ND><<<g, kBlock, 0, s>>>(U, row_old, alu, s0, ns, z, w0, w1, st, part, pbase); break;
+    case 16: k_ssor_bwd<T, 16, ND><<<g, kBlock, 0, s>>>(U, row_old, alu, s0, ns, z, w0, w1, st, part, pbase); break;
+    default: k_ssor_bwd<T, 8, ND><<<g, kBlock, 0, s>>>(U, row_old, alu, s0, ns, z, w0, w1, st, part, pbase); break;
   }
 }
 
@@ -827,7 +1005,7 @@ void launch_ssor_fwd(const Sell<double>& L, const int32_t* row_old, const double
 }
 void launch_ssor_bwd(const Sell<double>& U, const int32_t* row_old, const double* alu, int32_t s0, int32_t ns,
                      double* z, const KrylovState* st, hipStream_t s) {
-  ssor_bwd_any<double, false>(U, row_old, alu, s0, ns, z, nullptr, st, nullptr, 0, spmv_grid(ns), s);
+  ssor_bwd_any<double, 0>(U, row_old, alu, s0, ns, z, nullptr, nullptr, st, nullptr, 0, spmv_grid(ns), s);
 }
 void launch_ssor_fwd_f32(const Sell<float>& L, const int32_t* row_old, const float* alu, int32_t s0, int32_t ns,
                          const float* rin, float* z, hipStream_t s) {
@@ -835,11 +1013,11 @@ void launch_ssor_fwd_f32(const Sell<float>& L, const int32_t* row_old, const flo
 }
 void launch_ssor_bwd_f32(const Sell<float>& U, const int32_t* row_old, const float* alu, int32_t s0, int32_t ns,
                          float* z, hipStream_t s) {
-  ssor_bwd_any<float, false>(U, row_old, alu, s0, ns, z, nullptr, nullptr, nullptr, 0, spmv_grid(ns), s);
+  ssor_bwd_any<float, 0>(U, row_old, alu, s0, ns, z, nullptr, nullptr, nullptr, nullptr, 0, spmv_grid(ns), s);
 }
 void launch_diag_apply(int32_t n, const double* alu, const double* r, double* z, const KrylovState* st,
                        hipStream_t s) {
-  k_diag<false><<<vec_grid(n), kBlock, 0, s>>>(n, alu, r, z, st, nullptr);
+  k_diag<double, 0><<<vec_grid(n), kBlock, 0, s>>>(n, alu, r, z, nullptr, st, nullptr);
 }
 void launch_copy(int64_t n3, const double* r, double* z, const KrylovState* st, hipStream_t s) {
   k_copy_dot<<<vec_grid(n3), kBlock, 0, s>>>(n3, r, z, st, nullptr, 0);
@@ -868,7 +1046,7 @@ void cg_precond_ssor_rho(const Sell<double>& L, const Sell<double>& U, const int
     const int ns = color_slice[c + 1] - color_slice[c];
     if (ns <= 0) continue;
     const int g = std::min(gcap, spmv_grid(ns));
-    ssor_bwd_any<double, true>(U, row_old, alu, color_slice[c], ns, z, r, st, part, pbase, g, s);
+    ssor_bwd_any<double, 1>(U, row_old, alu, color_slice[c], ns, z, r, nullptr, st, part, pbase, g, s);
     pbase += g;
   }
   k_tail<kTailCgRho, 1><<<1, kTailThreads, 0, s>>>(st, part, pbase, nullptr, nullptr);
@@ -876,7 +1054,7 @@ void cg_precond_ssor_rho(const Sell<double>& L, const Sell<double>& U, const int
 void cg_precond_diag_rho(int32_t n, const double* alu, const double* r, double* z, KrylovState* st, double* part,
                          hipStream_t s) {
   const int g = vec_grid(n);
-  k_diag<true><<<g, kBlock, 0, s>>>(n, alu, r, z, st, part);
+  k_diag<double, 1><<<g, kBlock, 0, s>>>(n, alu, r, z, nullptr, st, part);
   k_tail<kTailCgRho, 1><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, nullptr);
 }
 void cg_precond_none_rho(int64_t n3, const double* r, double* z, KrylovState* st, double* part, hipStream_t s) {
@@ -956,6 +1134,83 @@ void precond_apply(int32_t kind, const Sell<double>& L, const Sell<double>& U, c
     for (int c = ncolor - 1; c >= 0; --c)
       launch_ssor_bwd(U, row_old, alu_ssor, color_slice[c], color_slice[c + 1] - color_slice[c], z, st, s);
   }
+}
+
+// ---- flexible CG with FP32 operators ------------------------------------------------
+void launch_to_f32(int64_t n, const double* x, float* y, hipStream_t s) {
+  int64_t g = (n + kBlock - 1) / kBlock;
+  g = g < 1 ? 1 : (g > 8192 ? 8192 : g);
+  k_to_f32<<<(int)g, kBlock, 0, s>>>(n, x, y);
+}
+void precond_apply_f32(int32_t kind, const Sell<float>& L, const Sell<float>& U, const int32_t* row_old,
+                       const float* alu_ssor, const int32_t* color_slice, int32_t ncolor, const float* alu_diag,
+                       int32_t n, const float* r, float* z, const KrylovState* st, hipStream_t s) {
+  if (kind == 0) {
+    k_copy_dots_f32<<<vec_grid(3 * (int64_t)n), kBlock, 0, s>>>(3 * (int64_t)n, r, z, nullptr, st, nullptr, 0);
+  } else if (kind == 3) {
+    k_diag<float, 0><<<vec_grid(n), kBlock, 0, s>>>(n, alu_diag, r, z, nullptr, st, nullptr);
+  } else {
+    for (int c = 0; c < ncolor; ++c)
+      ssor_fwd_any<float>(L, row_old, alu_ssor, color_slice[c], color_slice[c + 1] - color_slice[c], r, z, st, s);
+    for (int c = ncolor - 1; c >= 0; --c) {
+      const int ns = color_slice[c + 1] - color_slice[c];
+      ssor_bwd_any<float, 0>(U, row_old, alu_ssor, color_slice[c], ns, z, nullptr, nullptr, st, nullptr, 0,
+                             spmv_grid(ns), s);
+    }
+  }
+}
+void fcg_precond_dots(int32_t kind, const Sell<float>& L, const Sell<float>& U, const int32_t* row_old,
+                      const float* alu_ssor, const int32_t* color_slice, int32_t ncolor, const float* alu_diag,
+                      int32_t n, const float* r, float* z, const float* q, KrylovState* st, double* part,
+                      hipStream_t s) {
+  int nparts = 0;
+  if (kind == 0) {
+    nparts = vec_grid(3 * (int64_t)n);
+    k_copy_dots_f32<<<nparts, kBlock, 0, s>>>(3 * (int64_t)n, r, z, q, st, part, 2);
+  } else if (kind == 3) {
+    nparts = vec_grid(n);
+    k_diag<float, 2><<<nparts, kBlock, 0, s>>>(n, alu_diag, r, z, q, st, part);
+  } else {
+    for (int c = 0; c < ncolor; ++c)
+      ssor_fwd_any<float>(L, row_old, alu_ssor, color_slice[c], color_slice[c + 1] - color_slice[c], r, z, st, s);
+    const int gcap = std::max(1, kMaxPartials / std::max(1, ncolor));
+    for (int c = ncolor - 1; c >= 0; --c) {
+      const int ns = color_slice[c + 1] - color_slice[c];
+      if (ns <= 0) continue;
+      const int g = std::min(gcap, spmv_grid(ns));
+      ssor_bwd_any<float, 2>(U, row_old, alu_ssor, color_slice[c], ns, z, r, q, st, part, nparts, g, s);
+      nparts += g;
+    }
+  }
+  k_tail<kTailFcgBeta, 2><<<1, kTailThreads, 0, s>>>(st, part, nparts, nullptr, nullptr);
+}
+void fcg_p(int64_t n3, const float* z, float* p, const KrylovState* st, hipStream_t s) {
+  k_fcg_p<<<vec_grid(n3), kBlock, 0, s>>>(n3, z, p, st);
+}
+void fcg_spmv(const Sell<float>& a, const float* p, const float* r, float* q, KrylovState* st, double* part,
+              hipStream_t s) {
+  const int g = spmv_grid(a.nslices);
+  k_spmv_f32<true><<<g, kBlock, 0, s>>>(a, p, r, q, st, part);
+  k_tail<kTailFcgAlpha, 2><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, nullptr);
+}
+void fcg_update(int64_t n3, const float* p, const float* q, double* x, float* r, KrylovState* st, double* part,
+                double* hist, hipStream_t s) {
+  const int g = vec_grid(n3);
+  k_fcg_update<<<g, kBlock, 0, s>>>(n3, p, q, x, r, st, part);
+  k_tail<kTailFcgRes, 1><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, hist);
+}
+void fcg_reliable(const Sell<double>& a, const double* x, const double* b, float* r, KrylovState* st, double* part,
+                  double* hist, hipStream_t s) {
+  const int g = spmv_grid(a.nslices);
+  k_spmv_rel<<<g, kBlock, 0, s>>>(a, x, b, r, st, part);
+  k_tail_fcg_rel<<<1, kTailThreads, 0, s>>>(st, part, g, hist);
+}
+// r32 = b - A x (FP64) and |r|^2 stored to *rr (start of a flexible CG solve)
+void fcg_init_resid(const Sell<double>& a, const double* x, const double* b, float* r, double* rr, KrylovState* st,
+                    double* part, hipStream_t s) {
+  const int g = spmv_grid(a.nslices);
+  k_spmv_rel<<<g, kBlock, 0, s>>>(a, x, b, r, nullptr, part);
+  k_tail<kTailStore, 1><<<1, kTailThreads, 0, s>>>(st, part, g, rr, nullptr);
 }
 
 }  // namespace hecmw33

@@ -50,10 +50,15 @@ static double now_s() {
 // Iarray / Rarray slots (1-based in the reference, hecmw_matrix_misc.F90:107-150)
 enum {
   I_ITER = 1, I_METHOD = 2, I_PRECOND = 3, I_ITERPREMAX = 5, I_SCALING = 7, I_METHOD2 = 8,
-  I_ITERLOG = 21, I_TIMELOG = 22, I_NCOLOR_IN = 34, I_MAXRECYCLE = 35, I_FLAG_CONVERGED = 81,
-  I_FLAG_DIVERGED = 82, I_NRECYCLE = 96, I_FLAG_NUMFACT = 97, I_FLAG_SYMBFACT = 98,
-  R_RESID = 1, R_SIGMA_DIAG = 2,
+  I_ITERLOG = 21, I_TIMELOG = 22, I_NCOLOR_IN = 34, I_MAXRECYCLE = 35, I_SOLVER_OPT = 41,
+  I_FLAG_CONVERGED = 81, I_FLAG_DIVERGED = 82, I_NRECYCLE = 96, I_FLAG_NUMFACT = 97, I_FLAG_SYMBFACT = 98,
+  R_RESID = 1, R_SIGMA_DIAG = 2, R_SOLVER_OPT = 41,
 };
+// SOLVER_OPT(1) (Iarray(41), unused by the reference's CG/BiCGSTAB with
+// PRECOND 1-3) = 1 selects the flexible CG with FP32 operators;
+// Rarray(41) in (0,1) overrides its reliable-update factor.
+constexpr int32_t kOptMixedFcg = 1;
+constexpr double kFcgDelta = 1.0e-2;
 static inline int32_t& IA(int32_t* a, int k) { return a[k - 1]; }
 static inline int32_t IA(const int32_t* a, int k) { return a[k - 1]; }
 static inline double RA(const double* a, int k) { return a[k - 1]; }
@@ -182,11 +187,15 @@ struct Context {
   // DIAG
   bool diag_init = false;
   DMem row_diag, alu_diag;
+  // FP32 copies for the flexible CG (rounded from the FP64 storage)
+  bool a32_valid = false, ssor32_valid = false, diag32_valid = false;
+  DMem alu_ssor32, alu_diag32, vr32, vz32, vp32, vq32;
   // Krylov vectors and state
   DMem vb, vx, vr, vz, vp, vq, vrt, vs, vt, vv, vpt, vst, vzp, vzt;
   DMem st, part, hist;
   KrylovState* h_st = nullptr;  // pinned, 2 slots for polling
   int32_t last_iter = 0;
+  int32_t last_nreliable = 0;  // FP64 residual replacements of the last flexible CG solve
   // TIMELOG: event pairs around the preconditioner and the SpMV of every
   // enqueued iteration (4 events per iteration)
   std::vector<hipEvent_t> events;
@@ -201,8 +210,9 @@ struct Context {
 
   ~Context() {
     if (stream) hipStreamSynchronize(stream);
-    for (DMem* m : {&dD, &dAL, &dAU, &row_old, &alu_ssor, &row_diag, &alu_diag, &vb, &vx, &vr, &vz, &vp, &vq, &vrt,
-                    &vs, &vt, &vv, &vpt, &vst, &vzp, &vzt, &st, &part, &hist})
+    for (DMem* m : {&dD, &dAL, &dAU, &row_old, &alu_ssor, &row_diag, &alu_diag, &alu_ssor32, &alu_diag32, &vr32,
+                    &vz32, &vp32, &vq32, &vb, &vx, &vr, &vz, &vp, &vq, &vrt, &vs, &vt, &vv, &vpt, &vst, &vzp, &vzt, &st,
+                    &part, &hist})
       m->release();
     for (SellDev* s : {&A, &L, &U})
       for (DMem* m : {&s->ptr, &s->len, &s->col, &s->src, &s->val, &s->val32}) m->release();
@@ -309,6 +319,7 @@ static int32_t load_values(Context& c, const hecmw33_sys* A) {
   launch_fill_values(c.A.view, c.A.src.as<int64_t>(), c.dD.as<double>(), c.dAL.as<double>(), c.dAU.as<double>(),
                      c.N, c.NPL, c.stream);
   HIPC(hipGetLastError());
+  c.a32_valid = false;
   return 0;
 }
 
@@ -404,6 +415,7 @@ static int32_t setup_ssor(Context& c, double sigma, int32_t ncolor_in) {
   HIPC(hipGetLastError());
   HIPC(hipStreamSynchronize(c.stream));
   c.ssor_init = true;
+  c.ssor32_valid = false;
   return 0;
 }
 
@@ -419,6 +431,41 @@ static int32_t setup_diag(Context& c, double sigma) {
   HIPC(hipGetLastError());
   HIPC(hipStreamSynchronize(c.stream));
   c.diag_init = true;
+  c.diag32_valid = false;
+  return 0;
+}
+
+// FP32 images of A and of the preconditioner in use, rounded from the FP64
+// storage (so a reused preconditioner keeps the values of its setup).
+static int32_t round_sell(SellDev& d, hipStream_t s) {
+  const int64_t n = 9 * std::max<int64_t>(d.view.nslots, 1);
+  if (d.val32.ensure(sizeof(float) * n)) return HECMW33_ERROR_DEVICE;
+  d.view32.val = d.val32.as<float>();
+  launch_to_f32(9 * d.view.nslots, d.view.val, d.view32.val, s);
+  return 0;
+}
+static int32_t setup_f32(Context& c, int32_t kind) {
+  if (!c.a32_valid) {
+    if (int32_t e = round_sell(c.A, c.stream)) return e;
+    c.a32_valid = true;
+  }
+  if (kind == 1 && !c.ssor32_valid) {
+    if (round_sell(c.L, c.stream) || round_sell(c.U, c.stream)) return HECMW33_ERROR_DEVICE;
+    const int64_t nslot = (int64_t)c.color_slice[c.ncolor] * kRowsS;
+    if (c.alu_ssor32.ensure(sizeof(float) * 9 * std::max<int64_t>(nslot, 1))) return HECMW33_ERROR_DEVICE;
+    launch_to_f32(9 * nslot, c.alu_ssor.as<double>(), c.alu_ssor32.as<float>(), c.stream);
+    c.ssor32_valid = true;
+  }
+  if (kind == 3 && !c.diag32_valid) {
+    const int64_t nslot = (int64_t)((c.N + kWave - 1) / kWave) * kWave;
+    if (c.alu_diag32.ensure(sizeof(float) * 9 * nslot)) return HECMW33_ERROR_DEVICE;
+    launch_to_f32(9 * nslot, c.alu_diag.as<double>(), c.alu_diag32.as<float>(), c.stream);
+    c.diag32_valid = true;
+  }
+  const size_t vb = sizeof(float) * 3 * (size_t)c.NP;
+  for (DMem* m : {&c.vr32, &c.vz32, &c.vp32, &c.vq32})
+    if (m->ensure(vb)) return HECMW33_ERROR_DEVICE;
+  HIPC(hipGetLastError());
   return 0;
 }
 
@@ -531,6 +578,26 @@ static void enqueue_bicg_iteration(Context& c, int32_t kind, int32_t iterpremax,
   cg_true_resid(c.A.view, c.vx.as<double>(), c.vb.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
 }
 
+// One flexible CG iteration with FP32 operators: z = M32^-1 r (fused r.z,
+// z.q_prev), p = z + beta p, q = A32 p (fused p.q, r.p), x += alpha p (FP64),
+// r -= alpha q, then, when the recurrence asks for it, r = b - A x in FP64.
+static void enqueue_fcg_iteration(Context& c, int32_t kind, hipEvent_t* ev) {
+  const int64_t n3 = 3 * (int64_t)c.N;
+  KrylovState* st = c.st.as<KrylovState>();
+  double* part = c.part.as<double>();
+  float *r = c.vr32.as<float>(), *z = c.vz32.as<float>(), *p = c.vp32.as<float>(), *q = c.vq32.as<float>();
+  if (ev) hipEventRecord(ev[0], c.stream);
+  fcg_precond_dots(kind, c.L.view32, c.U.view32, c.row_old.as<int32_t>(), c.alu_ssor32.as<float>(),
+                   c.color_slice.data(), c.ncolor, c.alu_diag32.as<float>(), c.N, r, z, q, st, part, c.stream);
+  if (ev) hipEventRecord(ev[1], c.stream);
+  fcg_p(n3, z, p, st, c.stream);
+  if (ev) hipEventRecord(ev[2], c.stream);
+  fcg_spmv(c.A.view32, p, r, q, st, part, c.stream);
+  if (ev) hipEventRecord(ev[3], c.stream);
+  fcg_update(n3, p, q, c.vx.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
+  fcg_reliable(c.A.view, c.vx.as<double>(), c.vb.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
+}
+
 // Krylov solve on the context's vectors vb (b) and vx (x0 in, x out).
 static int32_t krylov_resident(Context& c, int32_t* iarray, double* rarray, hecmw33_out* out) {
   const double t0 = now_s();
@@ -544,8 +611,15 @@ static int32_t krylov_resident(Context& c, int32_t* iarray, double* rarray, hecm
     set_error("METHOD not provided by the GPU library (1: CG, 2: BiCGSTAB)");
     return HECMW33_ERROR_UNSUPPORTED;
   }
+  const bool mixed = IA(iarray, I_SOLVER_OPT) == kOptMixedFcg;
+  if (mixed && (method != 1 || iterpremax > 1)) {
+    set_error("SOLVER_OPT(1)=1 (flexible CG, FP32 operators) needs METHOD=1 and ITERPREMAX<=1");
+    return HECMW33_ERROR_UNSUPPORTED;
+  }
   if (int32_t e = precond_setup(c, iarray, rarray)) return e;
   const int32_t kind = pc_kind(pc, iterpremax);
+  if (mixed)
+    if (int32_t e = setup_f32(c, kind)) return e;
   const int64_t n3 = 3 * (int64_t)c.N;
   const size_t vbytes = sizeof(double) * 3 * (size_t)c.NP;
   KrylovState* st = c.st.as<KrylovState>();
@@ -557,11 +631,18 @@ static int32_t krylov_resident(Context& c, int32_t* iarray, double* rarray, hecm
       if (m->ensure(vbytes)) return HECMW33_ERROR_DEVICE;
   }
   // r0 = b - A x0; bnrm2 = b.b  (hecmw_solver_CG.f90:131-141)
-  launch_matresid(c.A.view, c.vx.as<double>(), c.vb.as<double>(), c.vr.as<double>(), c.stream);
+  if (mixed) {
+    fcg_init_resid(c.A.view, c.vx.as<double>(), c.vb.as<double>(), c.vr32.as<float>(), &st->rr_ref, st, part,
+                   c.stream);
+  } else {
+    launch_matresid(c.A.view, c.vx.as<double>(), c.vb.as<double>(), c.vr.as<double>(), c.stream);
+  }
   launch_dot_store(n3, c.vb.as<double>(), c.vb.as<double>(), &st->bnrm2, st, part, c.stream);
   // work vectors start at zero (WW = 0), including the external entries the
   // SpMV reads; nothing writes those afterwards
   HIPC(hipMemsetAsync(c.vp.p, 0, vbytes, c.stream));
+  if (mixed)
+    for (DMem* m : {&c.vz32, &c.vp32, &c.vq32}) HIPC(hipMemsetAsync(m->p, 0, vbytes / 2, c.stream));
   if (method == 2) {
     HIPC(hipMemcpyAsync(c.vrt.p, c.vr.p, vbytes, hipMemcpyDeviceToDevice, c.stream));  // r~ = r0
     for (DMem* m : {&c.vv, &c.vpt, &c.vst}) HIPC(hipMemsetAsync(m->p, 0, vbytes, c.stream));
@@ -596,6 +677,12 @@ static int32_t krylov_resident(Context& c, int32_t* iarray, double* rarray, hecm
   h.n_indef = 0;
   h.recheck = 0;
   h.nrec = method == 1 ? 50 : 100;
+  h.zq = h.pq = h.rp = 0.0;
+  const double delta = RA(rarray, R_SOLVER_OPT) > 0.0 && RA(rarray, R_SOLVER_OPT) < 1.0 ? RA(rarray, R_SOLVER_OPT)
+                                                                                         : kFcgDelta;
+  h.delta2 = delta * delta;
+  h.reliable = 0;
+  h.nreliable = 0;
   c.h_st[1] = h;
   HIPC(hipMemcpyAsync(st, &c.h_st[1], sizeof(KrylovState), hipMemcpyHostToDevice, c.stream));
   HIPC(hipStreamSynchronize(c.stream));
@@ -619,7 +706,9 @@ static int32_t krylov_resident(Context& c, int32_t* iarray, double* rarray, hecm
         for (int j = 0; j < nev; ++j) itev[j] = c.event((size_t)nev * (enq + i) + j);
         e = itev.data();
       }
-      if (method == 1)
+      if (mixed)
+        enqueue_fcg_iteration(c, kind, e);
+      else if (method == 1)
         enqueue_cg_iteration(c, kind, iterpremax, e);
       else
         enqueue_bicg_iteration(c, kind, iterpremax, e);
@@ -661,6 +750,7 @@ static int32_t krylov_resident(Context& c, int32_t* iarray, double* rarray, hecm
   out->error = h.error;
   out->resid = h.resid;
   c.last_iter = std::min(h.iter, maxit);
+  c.last_nreliable = mixed ? h.nreliable : 0;
   return 0;
 }
 
@@ -987,9 +1077,10 @@ int32_t hecmw33_ctx_history(hecmw33_ctx* ctx, double* hist, int32_t cap) {
 int32_t hecmw33_ctx_info(hecmw33_ctx* ctx, int64_t* info, int32_t n) {
   if (!ctx || !info) return HECMW33_ERROR_ARGS;
   Context& c = ctx->c;
-  const int64_t v[6] = {c.A.view.nslots, (int64_t)c.N, (int64_t)c.NPL, (int64_t)c.NPU, (int64_t)c.ncolor,
-                        c.L.view.nslots + c.U.view.nslots};
-  for (int i = 0; i < n && i < 6; ++i) info[i] = v[i];
+  const int64_t v[7] = {c.A.view.nslots, (int64_t)c.N,     (int64_t)c.NPL,
+                        (int64_t)c.NPU,    (int64_t)c.ncolor, c.L.view.nslots + c.U.view.nslots,
+                        (int64_t)c.last_nreliable};
+  for (int i = 0; i < n && i < 7; ++i) info[i] = v[i];
   return 0;
 }
 

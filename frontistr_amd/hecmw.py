@@ -52,6 +52,14 @@ def lib():
         return _lib
     if not os.path.exists(lib_path):
         raise HecmwError("libhecmw33.so is not built: run python3 frontistr_amd/build.py")
+    # One HIP runtime per process: torch ships its own libamdhip64 (same
+    # soname), and a process that loads ROCm's first and torch's second holds
+    # two runtimes, after which torch sees no device.  Binding torch's first
+    # makes libhecmw33 resolve to it.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(lib_path)
     P = C.c_void_p
     I = C.c_int32
@@ -105,6 +113,8 @@ class HecMatrix:
     I_FLAG_CONVERGED, I_FLAG_DIVERGED = 81, 82
     I_NRECYCLE, I_FLAG_NUMFACT, I_FLAG_SYMBFACT, I_SOLVER_TYPE = 96, 97, 98, 99
     R_RESID, R_SIGMA_DIAG, R_SIGMA, R_THRESH, R_FILTER = 1, 2, 3, 4, 5
+    # SOLVER_OPT block (hecmw_matrix_misc.F90:138-150), unused by CG with PRECOND 1-3
+    I_SOLVER_OPT, R_SOLVER_OPT = 41, 41
 
     def __init__(self, N, D, AL, AU, indexL, itemL, indexU, itemU, B=None, X=None, NP=None):
         self.N = int(N)
@@ -160,6 +170,13 @@ class HecMatrix:
     def set_iterpremax(self, v): self.set_i(self.I_ITERPREMAX, v)
     def set_ncolor_in(self, v): self.set_i(self.I_NCOLOR_IN, v)
     def set_resid(self, v): self.Rarray[self.R_RESID - 1] = v
+
+    def set_mixed(self, on=True, delta=None):
+        """Flexible CG with FP32 operators and FP64 residual replacement
+        (Iarray 41 = 1, METHOD 1 only); delta: reliable-update factor on |r|
+        (Rarray 41, default 1e-2)."""
+        self.set_i(self.I_SOLVER_OPT, 1 if on else 0)
+        self.Rarray[self.R_SOLVER_OPT - 1] = 0.0 if delta is None else float(delta)
 
     def set_sigma_diag(self, v):
         # hecmw_mat_set_sigma_diag clamps to [1, 2], negative = automatic (-1)
@@ -348,7 +365,7 @@ class DeviceContext:
         return h[:n]
 
     def info(self):
-        v = np.zeros(6, np.int64)
-        _check(lib().hecmw33_ctx_info(self._h, _ptr(v), 6))
+        v = np.zeros(7, np.int64)
+        _check(lib().hecmw33_ctx_info(self._h, _ptr(v), 7))
         return {"nslots": int(v[0]), "N": int(v[1]), "NPL": int(v[2]), "NPU": int(v[3]), "ncolor": int(v[4]),
-                "ssor_slots": int(v[5])}
+                "ssor_slots": int(v[5]), "nreliable": int(v[6])}

@@ -85,6 +85,17 @@ int32_t hecmw33_solve(const hecmw33_sys* A, const double* B, double* X, int32_t 
 int32_t hecmw33_solve_krylov(const hecmw33_sys* A, const double* B, double* X, int32_t iarray[100],
                              double rarray[100], hecmw33_out* out);
 
+/* Mixed precision (no counterpart in the reference's CG; selected through the
+ * SOLVER_OPT block that CG/BiCGSTAB with PRECOND 1-3 never read,
+ * hecmw_matrix_misc.F90:138):
+ *   iarray[40] (Iarray 41) = 1 with METHOD 1, ITERPREMAX <= 1: flexible CG
+ *   (Notay's FCG(1)) whose SpMV, SSOR/DIAG sweeps and work vectors are FP32,
+ *   with FP64 inner products, scalars and x; the residual is replaced by
+ *   b - A x computed in FP64 whenever the recurrence residual has dropped by
+ *   rarray[40] (Rarray 41, default 1e-2), every 50 iterations and before any
+ *   convergence is declared, so RESID is always an FP64 true residual.
+ *   Accepted by hecmw33_solve, hecmw33_solve_krylov and hecmw33_ctx_solve_device. */
+
 /* y = A x for the internal rows (hecmw_matvec_33, hecmw1/src/solver/las/hecmw_solver_las_33.F90:34-59).
  * Host pointers, x has 3*NP entries, y receives 3*N. */
 int32_t hecmw33_matvec(const hecmw33_sys* A, const double* x, double* y);
@@ -130,7 +141,8 @@ int32_t hecmw33_ctx_matvec_device(hecmw33_ctx* ctx, const double* x_dev, double*
 int32_t hecmw33_ctx_matvec_device_f32(hecmw33_ctx* ctx, const float* x_dev, float* y_dev, int32_t count, double* ms);
 /* Residual history of the last solve (ITERLOG values), up to cap entries. */
 int32_t hecmw33_ctx_history(hecmw33_ctx* ctx, double* hist, int32_t cap);
-/* Matrix storage figures: nslots (SELL slots incl. padding), bytes of the FP64 SpMV stream. */
+/* Context figures: [0] SELL-64 slots of A (incl. padding), [1] N, [2] NPL, [3] NPU, [4] colours,
+ * [5] SELL-16 slots of L' + U', [6] FP64 residual replacements of the last flexible CG solve. */
 int32_t hecmw33_ctx_info(hecmw33_ctx* ctx, int64_t* info, int32_t n);
 
 /* ---- synthetic systems (fistr1's assembly for a structured HEX8 cube) ------- */
