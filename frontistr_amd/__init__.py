@@ -30,5 +30,6 @@ from .hecmw import (  # noqa: F401
     hecmw_rel_resid_L2,
     hecmw_ordering,
     gen_hex8,
+    gen_tet10,
     DeviceContext,
 )

@@ -28,6 +28,7 @@ SOURCES = [
     ("solver.hip", True),
     ("ordering.cpp", False),
     ("gen_hex8.cpp", False),
+    ("gen_tet10.cpp", False),
 ]
 
 

@@ -161,6 +161,12 @@ void fcg_reliable(const Sell<double>& a, const double* x, const double* b, float
 // r32 = b - A64 x64 and |r|^2 into *rr
 void fcg_init_resid(const Sell<double>& a, const double* x, const double* b, float* r, double* rr, KrylovState* st,
                     double* part, hipStream_t s);
+// ---- diagonal scaling (hecmw_solver_scaling_33.f90) ----------------------------------
+void launch_make_scale(int32_t n, int32_t np, const double* D, double* sc, hipStream_t s);
+void launch_scale_raw(bool fw, int32_t np, const int32_t* indexL, const int32_t* itemL, const int32_t* indexU,
+                      const int32_t* itemU, double* D, double* AL, double* AU, const double* sc, hipStream_t s);
+void launch_scale_vec(bool mul, int64_t n, double* v, const double* sc, hipStream_t s);
+
 // r32 = (float) r64 over n entries
 void launch_to_f32(int64_t n, const double* x, float* y, hipStream_t s);
 

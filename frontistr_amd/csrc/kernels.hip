@@ -876,6 +876,47 @@ __global__ __launch_bounds__(kBlock) void k_to_f32(int64_t n, const double* __re
     y[i] = (float)x[i];
 }
 
+// ---- diagonal scaling, hecmw_solver_scaling_33.f90:20-208 -----------------------
+// s = 1/sqrt(|D_ii|) per DOF of the internal nodes; external nodes get 1 (the
+// reference takes them from a halo exchange, absent on one domain).
+__global__ void k_make_scale(int32_t n, int32_t np, const double* __restrict__ D, double* __restrict__ sc) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) sc[3 * i + d] = i < n ? 1.0 / sqrt(fabs(D[9 * i + 4 * d])) : 1.0;
+}
+
+// FW: a = a*s_p*s_q (left to right); BK: a = a/(s_p*s_q), over the caller's
+// D/AL/AU rows 0..NP-1 as the reference loops them.
+template <bool FW>
+__device__ __forceinline__ void scale_block(double* __restrict__ a, const double* __restrict__ sp,
+                                            const double* __restrict__ sq) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int p = 0; p < 3; ++p)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) a[3 * p + q] = FW ? a[3 * p + q] * sp[p] * sq[q] : a[3 * p + q] / (sp[p] * sq[q]);
+}
+template <bool FW>
+__global__ void k_scale_raw(int32_t np, const int32_t* __restrict__ indexL, const int32_t* __restrict__ itemL,
+                            const int32_t* __restrict__ indexU, const int32_t* __restrict__ itemU,
+                            double* __restrict__ D, double* __restrict__ AL, double* __restrict__ AU,
+                            const double* __restrict__ sc) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x) {
+    const double* sp = sc + 3 * i;
+    scale_block<FW>(D + 9 * i, sp, sp);
+    for (int64_t k = indexL[i]; k < indexL[i + 1]; ++k) scale_block<FW>(AL + 9 * k, sp, sc + 3 * (int64_t)(itemL[k] - 1));
+    for (int64_t k = indexU[i]; k < indexU[i + 1]; ++k) scale_block<FW>(AU + 9 * k, sp, sc + 3 * (int64_t)(itemU[k] - 1));
+  }
+}
+
+// v = v*s (MUL) or v = v/s over n entries
+template <bool MUL>
+__global__ void k_scale_vec(int64_t n, double* __restrict__ v, const double* __restrict__ sc) {
+#pragma clang fp contract(off)
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    v[i] = MUL ? v[i] * sc[i] : v[i] / sc[i];
+}
+
 // ---- numeric setup ---------------------------------------------------------------
 template <typename T>
 __global__ void k_fill_values(Sell<T> a, const int64_t* __restrict__ src, const double* __restrict__ D,
@@ -1134,6 +1175,24 @@ void precond_apply(int32_t kind, const Sell<double>& L, const Sell<double>& U, c
     for (int c = ncolor - 1; c >= 0; --c)
       launch_ssor_bwd(U, row_old, alu_ssor, color_slice[c], color_slice[c + 1] - color_slice[c], z, st, s);
   }
+}
+
+// ---- diagonal scaling ------------------------------------------------------------------
+void launch_make_scale(int32_t n, int32_t np, const double* D, double* sc, hipStream_t s) {
+  k_make_scale<<<vec_grid(np), kBlock, 0, s>>>(n, np, D, sc);
+}
+void launch_scale_raw(bool fw, int32_t np, const int32_t* indexL, const int32_t* itemL, const int32_t* indexU,
+                      const int32_t* itemU, double* D, double* AL, double* AU, const double* sc, hipStream_t s) {
+  if (fw)
+    k_scale_raw<true><<<vec_grid(np), kBlock, 0, s>>>(np, indexL, itemL, indexU, itemU, D, AL, AU, sc);
+  else
+    k_scale_raw<false><<<vec_grid(np), kBlock, 0, s>>>(np, indexL, itemL, indexU, itemU, D, AL, AU, sc);
+}
+void launch_scale_vec(bool mul, int64_t n, double* v, const double* sc, hipStream_t s) {
+  if (mul)
+    k_scale_vec<true><<<vec_grid(n), kBlock, 0, s>>>(n, v, sc);
+  else
+    k_scale_vec<false><<<vec_grid(n), kBlock, 0, s>>>(n, v, sc);
 }
 
 // ---- flexible CG with FP32 operators ------------------------------------------------

@@ -187,6 +187,9 @@ struct Context {
   // DIAG
   bool diag_init = false;
   DMem row_diag, alu_diag;
+  // device copies of the index arrays and the scale vector (SCALING=1 only)
+  bool dindex_valid = false;
+  DMem dIndexL, dIndexU, dItemL, dItemU, vscale;
   // FP32 copies for the flexible CG (rounded from the FP64 storage)
   bool a32_valid = false, ssor32_valid = false, diag32_valid = false;
   DMem alu_ssor32, alu_diag32, vr32, vz32, vp32, vq32;
@@ -210,6 +213,7 @@ struct Context {
 
   ~Context() {
     if (stream) hipStreamSynchronize(stream);
+    for (DMem* m : {&dIndexL, &dIndexU, &dItemL, &dItemU, &vscale}) m->release();
     for (DMem* m : {&dD, &dAL, &dAU, &row_old, &alu_ssor, &row_diag, &alu_diag, &alu_ssor32, &alu_diag32, &vr32,
                     &vz32, &vp32, &vq32, &vb, &vx, &vr, &vz, &vp, &vq, &vrt, &vs, &vt, &vv, &vpt, &vst, &vzp, &vzt, &st,
                     &part, &hist})
@@ -302,6 +306,7 @@ static int32_t load_structure(Context& c, const hecmw33_sys* A, bool force) {
   c.have_struct = true;
   c.ssor_init = false;
   c.diag_init = false;
+  c.dindex_valid = false;
   const size_t vb = sizeof(double) * 3 * (size_t)NP;
   for (DMem* m : {&c.vb, &c.vx, &c.vr, &c.vz, &c.vp, &c.vq})
     if (m->ensure(vb)) return HECMW33_ERROR_DEVICE;
@@ -598,8 +603,67 @@ static void enqueue_fcg_iteration(Context& c, int32_t kind, hipEvent_t* ev) {
   fcg_reliable(c.A.view, c.vx.as<double>(), c.vb.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
 }
 
-// Krylov solve on the context's vectors vb (b) and vx (x0 in, x out).
+// Diagonal scaling of the device copies, hecmw_solver_scaling_fw_33 / _bk_33
+// (hecmw_solver_scaling_33.f90:20-208): forward scales the uploaded D/AL/AU
+// and b in place (x0 is left as given), backward unscales x and restores b
+// and the matrix by division, as the reference restores hecMAT.  The SpMV
+// storage is refilled after each.
+static int32_t scaling_apply(Context& c, bool fw) {
+  const int32_t N = c.N, NP = c.NP;
+  if (!c.dindex_valid) {
+    if (c.dIndexL.ensure(sizeof(int32_t) * (NP + 1)) || c.dIndexU.ensure(sizeof(int32_t) * (NP + 1)) ||
+        c.dItemL.ensure(sizeof(int32_t) * std::max(c.NPL, 1)) || c.dItemU.ensure(sizeof(int32_t) * std::max(c.NPU, 1)))
+      return HECMW33_ERROR_DEVICE;
+    HIPC(hipMemcpyAsync(c.dIndexL.p, c.h_indexL.data(), sizeof(int32_t) * (NP + 1), hipMemcpyHostToDevice, c.stream));
+    HIPC(hipMemcpyAsync(c.dIndexU.p, c.h_indexU.data(), sizeof(int32_t) * (NP + 1), hipMemcpyHostToDevice, c.stream));
+    if (c.NPL)
+      HIPC(hipMemcpyAsync(c.dItemL.p, c.h_itemL.data(), sizeof(int32_t) * c.NPL, hipMemcpyHostToDevice, c.stream));
+    if (c.NPU)
+      HIPC(hipMemcpyAsync(c.dItemU.p, c.h_itemU.data(), sizeof(int32_t) * c.NPU, hipMemcpyHostToDevice, c.stream));
+    c.dindex_valid = true;
+  }
+  if (c.vscale.ensure(sizeof(double) * 3 * (size_t)NP)) return HECMW33_ERROR_DEVICE;
+  double* sc = c.vscale.as<double>();
+  const int64_t n3 = 3 * (int64_t)N;
+  if (fw) {
+    launch_make_scale(N, NP, c.dD.as<double>(), sc, c.stream);
+    launch_scale_vec(true, n3, c.vb.as<double>(), sc, c.stream);
+  } else {
+    launch_scale_vec(true, n3, c.vx.as<double>(), sc, c.stream);
+    launch_scale_vec(false, n3, c.vb.as<double>(), sc, c.stream);
+  }
+  launch_scale_raw(fw, NP, c.dIndexL.as<int32_t>(), c.dItemL.as<int32_t>(), c.dIndexU.as<int32_t>(),
+                   c.dItemU.as<int32_t>(), c.dD.as<double>(), c.dAL.as<double>(), c.dAU.as<double>(), sc, c.stream);
+  launch_fill_values(c.A.view, c.A.src.as<int64_t>(), c.dD.as<double>(), c.dAL.as<double>(), c.dAU.as<double>(),
+                     c.N, c.NPL, c.stream);
+  c.a32_valid = false;
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_out* out);
+
+// Krylov solve on the context's vectors vb (b) and vx (x0 in, x out), with
+// the reference's optional diagonal scaling around it (Iarray 7).
 static int32_t krylov_resident(Context& c, int32_t* iarray, double* rarray, hecmw33_out* out) {
+  const bool scaling = IA(iarray, I_SCALING) != 0;
+  double tsc = 0.0;
+  if (scaling) {
+    const double t0 = now_s();
+    if (int32_t e = scaling_apply(c, true)) return e;
+    tsc = now_s() - t0;
+  }
+  int32_t e = krylov_core(c, iarray, rarray, out);
+  out->Tset += tsc;
+  if (scaling) {
+    const int32_t e2 = scaling_apply(c, false);
+    if (!e) e = e2;
+  }
+  return e;
+}
+
+static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_out* out) {
   const double t0 = now_s();
   const int32_t method = IA(iarray, I_METHOD);
   const int32_t pc = IA(iarray, I_PRECOND);

@@ -14,7 +14,8 @@
 !
 ! hecmw33_available() tells a caller whether the GPU path can take a system
 ! (library loaded, a device visible, METHOD 1/2, PRECOND 1/2/3, no MPC, one
-! domain); otherwise the caller keeps the reference path.
+! domain; SCALING is applied on the device copies); otherwise the caller keeps
+! the reference path.
 !-------------------------------------------------------------------------------
 module hecmw_solver_33
   use iso_c_binding
@@ -68,15 +69,9 @@ contains
     precond = hecMAT%Iarray(3)
     hecmw33_available = hecMAT%NDOF == 3 .and. (method == 1 .or. method == 2) &
       .and. (precond == 1 .or. precond == 2 .or. precond == 3 .or. hecMAT%Iarray(5) <= 0) &
-      .and. hecMESH%n_neighbor_pe == 0 .and. hecMESH%mpc%n_mpc == 0 &
-      .and. hecmw_mat_get_scaling_33(hecMAT) == 0
+      .and. hecMESH%n_neighbor_pe == 0 .and. hecMESH%mpc%n_mpc == 0
     if (hecmw33_available) hecmw33_available = c_hecmw33_device_count() > 0
   end function hecmw33_available
-
-  integer(kind=kint) function hecmw_mat_get_scaling_33(hecMAT)
-    type(hecmwST_matrix) :: hecMAT
-    hecmw_mat_get_scaling_33 = hecMAT%Iarray(7)
-  end function hecmw_mat_get_scaling_33
 
   subroutine hecmw_solve_33(hecMESH, hecMAT)
     use m_hecmw_solve_error

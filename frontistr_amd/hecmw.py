@@ -79,12 +79,15 @@ def lib():
     L.hecmw33_ctx_info.argtypes = [P, P, I]
     L.hecmw33_gen_hex8_count.argtypes = [I, C.POINTER(I), C.POINTER(I), C.POINTER(I)]
     L.hecmw33_gen_hex8.argtypes = [I, I, P, C.c_uint64, P, P, P, P, P, P, P, P, P]
+    L.hecmw33_gen_tet10_count.argtypes = [I, I, I, C.POINTER(I), C.POINTER(I), C.POINTER(I)]
+    L.hecmw33_gen_tet10.argtypes = [I, I, I, P, C.c_double, C.c_double, I, C.c_uint64, P, P, P, P, P, P, P, P]
     L.hecmw33_last_history.argtypes = [P, I]
     L.hecmw33_last_error.restype = C.c_char_p
     for f in ("hecmw33_solve", "hecmw33_solve_krylov", "hecmw33_matvec", "hecmw33_precond_apply",
               "hecmw33_rel_resid_L2", "hecmw33_ordering", "hecmw33_ctx_load", "hecmw33_ctx_solve_device",
               "hecmw33_ctx_matvec_device", "hecmw33_ctx_matvec_device_f32", "hecmw33_ctx_history",
-              "hecmw33_ctx_info", "hecmw33_gen_hex8_count", "hecmw33_gen_hex8", "hecmw33_device_count",
+              "hecmw33_ctx_info", "hecmw33_gen_hex8_count", "hecmw33_gen_hex8",
+              "hecmw33_gen_tet10_count", "hecmw33_gen_tet10", "hecmw33_device_count",
               "hecmw33_last_history"):
         getattr(L, f).restype = C.c_int32
     _lib = L
@@ -319,6 +322,34 @@ def gen_hex8(n, kind="uniform", E=210000.0, nu=0.3, seed=0, emin=1e3, emax=1e6, 
     if return_E:
         return m, Ee
     return m
+
+
+def gen_tet10(nx, ny, nz, box=(1.0, 1.0, 1.0), E=210000.0, nu=0.3, nrhs=1, seed=0):
+    """TET10 (fistr1 type 342) box on an nx x ny x nz cell grid, 6 tetrahedra per
+    cell, bottom face fixed, nrhs columns of N(0,1) nodal loads on the top face
+    (see gen_tet10.cpp).  Returns (HecMatrix with B = column 0, B_all of shape
+    (nrhs, 3N))."""
+    L = lib()
+    N = C.c_int32()
+    NPL = C.c_int32()
+    NPU = C.c_int32()
+    _check(L.hecmw33_gen_tet10_count(nx, ny, nz, C.byref(N), C.byref(NPL), C.byref(NPU)))
+    N, NPL, NPU = N.value, NPL.value, NPU.value
+    D = np.zeros(9 * N)
+    AL = np.zeros(9 * max(NPL, 1))
+    AU = np.zeros(9 * max(NPU, 1))
+    iL = np.zeros(N + 1, np.int32)
+    iU = np.zeros(N + 1, np.int32)
+    tL = np.zeros(max(NPL, 1), np.int32)
+    tU = np.zeros(max(NPU, 1), np.int32)
+    Ball = np.zeros((max(nrhs, 1), 3 * N))
+    bx = np.array(box, dtype=np.float64)
+    rc = L.hecmw33_gen_tet10(nx, ny, nz, _ptr(bx), E, nu, nrhs, seed, _ptr(D), _ptr(AL), _ptr(AU), _ptr(iL),
+                             _ptr(tL), _ptr(iU), _ptr(tU), _ptr(Ball))
+    if rc != 0:
+        raise HecmwError("hecmw33_gen_tet10 failed (%d)" % rc)
+    m = HecMatrix(N, D, AL[:9 * NPL], AU[:9 * NPU], iL, tL[:NPL], iU, tU[:NPU], B=Ball[0].copy())
+    return m, Ball[:nrhs]
 
 
 class DeviceContext:

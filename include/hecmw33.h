@@ -151,6 +151,14 @@ int32_t hecmw33_gen_hex8(int32_t n, int32_t mat_kind, const double* params, uint
                          double* AU, int32_t* indexL, int32_t* itemL, int32_t* indexU, int32_t* itemU, double* B,
                          float* elem_E);
 
+/* TET10 (fistr1 type 342) box [0,box[0]]x[0,box[1]]x[0,box[2]] on an nx x ny x nz cell grid
+ * (6 tetrahedra per cell), uniform (E, nu), bottom face fixed, nrhs columns of N(0,1) nodal
+ * loads on the top face in B (column k at B + 3*N*k). */
+int32_t hecmw33_gen_tet10_count(int32_t nx, int32_t ny, int32_t nz, int32_t* N, int32_t* NPL, int32_t* NPU);
+int32_t hecmw33_gen_tet10(int32_t nx, int32_t ny, int32_t nz, const double* box, double E, double nu, int32_t nrhs,
+                          uint64_t seed, double* D, double* AL, double* AU, int32_t* indexL, int32_t* itemL,
+                          int32_t* indexU, int32_t* itemU, double* B);
+
 /* ---- misc -------------------------------------------------------------------- */
 const char* hecmw33_last_error(void);
 int32_t hecmw33_device_count(void);

@@ -229,3 +229,53 @@ def test_residual_history_matches_reference(method, precond):
         # (late iterates drift by rounding: the dot products sum in another order)
         n = min(len(h), len(hr)) // 2
         np.testing.assert_allclose(h[:n], hr[:n], rtol=1e-3)
+
+
+@pytest.mark.parametrize("method", [1, 2])
+@pytest.mark.parametrize("precond", [1, 3])
+def test_scaling_matches_reference(method, precond):
+    """SCALING=1 (Iarray 7): the reference scales A and b by 1/sqrt(|D_ii|) in
+    place inside hecmw_solve_CG/BiCGSTAB and restores them afterwards.  The
+    residual history, iteration count and solution follow the reference's;
+    x0 enters the scaled system unscaled, as in the reference; the caller's
+    arrays come back bitwise unchanged."""
+    m = fa.gen_hex8(8, kind="lognormal", seed=5)
+    m.set_iter(5000)
+    m.set_method(method)
+    m.set_precond(precond)
+    m.set_i(m.I_SCALING, 1)
+    x0 = np.random.default_rng(3).standard_normal(3 * m.NP) * 1e-3
+    m.X[:] = x0
+    keep = [a.copy() for a in (m.D, m.AL, m.AU, m.B)]
+    res = fa.hecmw_solve_krylov(m)
+    h = fa.last_history()
+    for a, k in zip((m.D, m.AL, m.AU, m.B), keep):
+        assert np.array_equal(a, k)
+    xr, info = ref.solve_krylov(m.copy(), x0=x0, nthreads=2, history=True)
+    assert res.error == info["error"] == 0
+    hr = info["history"]
+    k = min(15, len(h), len(hr))
+    np.testing.assert_allclose(h[:k], hr[:k], rtol=2e-6)
+    tol_it = 2 if method == 1 else max(2, 0.1 * info["iter"])
+    assert abs(res.iter - info["iter"]) <= tol_it, (res.iter, info["iter"])
+    err = np.linalg.norm(m.X - xr) / np.linalg.norm(xr)
+    assert err <= 1e-6, err
+    # the scaled system's 1e-8 is not the unscaled one's; the reference's
+    # own relative residual of both answers agrees
+    assert ref.rel_resid(m, m.X) == pytest.approx(ref.rel_resid(m, xr), rel=0.5, abs=1e-9)
+
+
+def test_scaling_through_solve_iterative_and_mixed():
+    m = fa.gen_hex8(7, kind="lognormal", seed=6)
+    m.set_iter(5000)
+    m.set_i(m.I_SCALING, 1)
+    xr, ia, ra = ref.solve_iterative(m.copy())
+    res = fa.hecmw_solve_iterative(m)
+    assert res.error == 0
+    assert np.linalg.norm(m.X - xr) / np.linalg.norm(xr) <= 1e-6
+    m2 = m.copy()
+    m2.X[:] = 0.0
+    m2.set_mixed(True)
+    res = fa.hecmw_solve_iterative(m2)
+    assert res.error == 0
+    assert np.linalg.norm(m2.X - xr) / np.linalg.norm(xr) <= 1e-6
