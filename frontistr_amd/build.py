@@ -29,6 +29,7 @@ SOURCES = [
     ("ordering.cpp", False),
     ("gen_hex8.cpp", False),
     ("gen_tet10.cpp", False),
+    ("batch.hip", True),
 ]
 
 

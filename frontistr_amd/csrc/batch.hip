@@ -1,0 +1,991 @@
+// Batched multi-RHS solves: many independent 3x3-block systems, each with up
+// to 16 right-hand sides, solved together by DIAG-preconditioned CG.
+//
+// The reference solves one system and one right-hand side per
+// hecmw_solve_CG call (hecmw_solver_CG.f90:19-320).  Here every system of the
+// batch is one diagonal block of a single block-diagonal operator, so one
+// launch of each kernel serves all systems and all right-hand sides:
+//
+//   rows      : the systems' rows concatenated (system s owns rows
+//               [row0[s], row0[s]+N_s)), merged BCSR per row in the
+//               reference's SpMV order (D, then L, then U in storage order)
+//   vectors   : [row][16 rhs][xyz]; a (system, rhs) pair is one CG instance
+//   tiles     : runs of rows inside one system; one workgroup per tile writes
+//               that tile's partial inner products for all 16 right-hand
+//               sides, and one workgroup per system sums its tiles in a fixed
+//               order (deterministic, no atomics)
+//   precision : FP32 matrix, SpMM, DIAG and work vectors; FP64 x, inner
+//               products and CG scalars; the residual of an instance is
+//               replaced by b - A x computed with the FP64 matrix whenever
+//               its recurrence residual has dropped by delta, every 50
+//               iterations and before convergence is declared (flexible CG,
+//               Notay's FCG(1) recurrences, as the single-system mixed path)
+//   launches  : the kernels of kChunk iterations are captured once into a
+//               hipGraph and replayed until every instance is done; tiles of
+//               finished systems return at once
+//   SpMM shape: lanes per row, right-hand sides per lane and blocks in flight
+//               per lane are picked by timing the candidates on the loaded
+//               matrix (autotuner)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hecmw33.h"
+#include "device.h"
+
+namespace hecmw33 {
+namespace {
+
+constexpr int KS = 16;          // right-hand-side stride of every batch vector
+constexpr int kBT = 256;        // threads per workgroup
+constexpr int kTileRows = 128;  // rows per tile (at least; grown to cap the tile count)
+constexpr int kMaxTiles = 65536;
+constexpr int kChunk = 8;       // iterations per captured graph
+
+constexpr int32_t kErrNoconv = 3001, kErrDivergeMat = 3002, kErrDivergePc = 3003, kErrDivergeNan = 3004;
+
+// per-instance state, structure of arrays over nsys*16 instances
+struct BState {
+  double *bnrm2, *rho, *rho1, *zq, *pq, *rp, *alpha, *beta, *rr_ref, *resid;
+  int32_t *iter, *done, *error, *nindef, *rel;
+  int32_t *sys_active, *sys_rel, *active_total;
+  // solve parameters in device memory, so a captured graph serves any call:
+  // prm = {tol, delta^2}, iprm = {maxit, nrec}
+  double* prm;
+  int32_t* iprm;
+};
+
+struct Tiles {
+  const int32_t *row0, *nrows, *sys;  // per tile
+  const int32_t *sys_tile0, *sys_ntile;
+  int32_t ntiles;
+};
+
+// Partials of ND inner products per right-hand side: thread tid holds the
+// values of rhs k = tid % 16 (rows tid / 16, ...); fixed-order sum over the
+// 16 row lanes, one partial per (dot, rhs, tile).
+template <int ND>
+__device__ __forceinline__ void tile_partials_rowlanes(const double (&acc)[ND], double* part, int pitch, int tile) {
+  __shared__ double red[ND][kBT];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) red[d][threadIdx.x] = acc[d];
+  __syncthreads();
+  if (threadIdx.x < ND * KS) {
+    const int d = threadIdx.x / KS, k = threadIdx.x % KS;
+    double s = 0.0;
+    for (int r = 0; r < kBT / KS; ++r) s += red[d][r * KS + k];
+    part[(int64_t)(d * KS + k) * pitch + tile] = s;
+  }
+}
+
+// ---- preconditioner + fused dots: z = D^-1 r, r.z, z.q ----------------------------
+__global__ __launch_bounds__(kBT) void kb_diag(Tiles T, const float* __restrict__ alu, const float* __restrict__ r,
+                                               float* __restrict__ z, const float* __restrict__ q, BState S,
+                                               double* part) {
+#pragma clang fp contract(off)
+  const int t = blockIdx.x;
+  const int s = T.sys[t];
+  if (S.sys_active[s] == 0) return;
+  const int k = threadIdx.x % KS, rl = threadIdx.x / KS;
+  const bool live = !S.done[s * KS + k];
+  double acc[2] = {0.0, 0.0};
+  const int r0 = T.row0[t], nr = T.nrows[t];
+  for (int i = rl; i < nr && live; i += kBT / KS) {
+    const int64_t g = r0 + i;
+    float a[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) a[e] = alu[9 * g + e];
+    const int64_t o = (g * KS + k) * 3;
+    const float r0v = r[o], r1v = r[o + 1], r2v = r[o + 2];
+    float x0 = r0v, x1 = r1v, x2 = r2v;
+    x1 = x1 - a[3] * x0;
+    x2 = x2 - a[6] * x0 - a[7] * x1;
+    x2 = a[8] * x2;
+    x1 = a[4] * (x1 - a[5] * x2);
+    x0 = a[0] * (x0 - a[2] * x2 - a[1] * x1);
+    z[o] = x0;
+    z[o + 1] = x1;
+    z[o + 2] = x2;
+    acc[0] += (double)r0v * x0 + (double)r1v * x1 + (double)r2v * x2;
+    acc[1] += (double)q[o] * x0 + (double)q[o + 1] * x1 + (double)q[o + 2] * x2;
+  }
+  tile_partials_rowlanes<2>(acc, part, T.ntiles, t);
+}
+
+// ---- p = z (first iteration of an instance) or z + beta p --------------------------
+__global__ __launch_bounds__(kBT) void kb_p(Tiles T, const float* __restrict__ z, float* __restrict__ p, BState S) {
+  const int t = blockIdx.x;
+  const int s = T.sys[t];
+  if (S.sys_active[s] == 0) return;
+  const int k = threadIdx.x % KS, rl = threadIdx.x / KS;
+  const int id = s * KS + k;
+  if (S.done[id]) return;
+  const bool first = S.iter[id] == 1;
+  const double beta = S.beta[id];
+  const int r0 = T.row0[t], nr = T.nrows[t];
+  for (int i = rl; i < nr; i += kBT / KS) {
+    const int64_t o = ((int64_t)(r0 + i) * KS + k) * 3;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) p[o + d] = first ? z[o + d] : (float)((double)z[o + d] + beta * (double)p[o + d]);
+  }
+}
+
+// ---- x += alpha p (FP64), r = r - alpha q (FP32), r.r --------------------------------
+__global__ __launch_bounds__(kBT) void kb_update(Tiles T, const float* __restrict__ p, const float* __restrict__ q,
+                                                 double* __restrict__ x, float* __restrict__ r, BState S,
+                                                 double* part) {
+  const int t = blockIdx.x;
+  const int s = T.sys[t];
+  if (S.sys_active[s] == 0) return;
+  const int k = threadIdx.x % KS, rl = threadIdx.x / KS;
+  const int id = s * KS + k;
+  const bool live = !S.done[id];
+  const double alpha = S.alpha[id];
+  double acc[1] = {0.0};
+  const int r0 = T.row0[t], nr = T.nrows[t];
+  for (int i = rl; i < nr && live; i += kBT / KS) {
+    const int64_t o = ((int64_t)(r0 + i) * KS + k) * 3;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      x[o + d] = x[o + d] + alpha * (double)p[o + d];
+      const float v = (float)((double)r[o + d] - alpha * (double)q[o + d]);
+      r[o + d] = v;
+      acc[0] += (double)v * (double)v;
+    }
+  }
+  tile_partials_rowlanes<1>(acc, part, T.ntiles, t);
+}
+
+// ---- SpMM ----------------------------------------------------------------------------
+// A row is handled by LPR lanes: G = 16/RPL right-hand-side groups (RPL rhs
+// per lane) times NPH = LPR/G phases that take every NPH-th block of the row;
+// phases are summed with lane shuffles.  U blocks per lane are in flight.
+enum BEpi : int {
+  kbEpiQ = 0,      // FP32: q = A p, partials p.q, r.p
+  kbEpiRes = 1,    // FP64: r32 = (float)(b - A x) for instances with rel set, partial |b - A x|^2
+  kbEpiStore = 2,  // y = A x (either precision), no partials
+};
+
+template <typename T, int LPR, int RPL, int U, int EPI>
+__global__ __launch_bounds__(kBT) void kb_spmm(Tiles TL, const int32_t* __restrict__ rowptr,
+                                               const int32_t* __restrict__ col, const T* __restrict__ val,
+                                               const T* __restrict__ x, const void* __restrict__ w,
+                                               void* __restrict__ y, BState S, double* part, int guard) {
+#pragma clang fp contract(off)
+  constexpr int G = KS / RPL, NPH = LPR / G, RPW = 64 / LPR;
+  constexpr int ND = EPI == kbEpiQ ? 2 : (EPI == kbEpiRes ? 1 : 0);
+  const int t = blockIdx.x;
+  const int s = TL.sys[t];
+  if (guard == 1 && S.sys_active[s] == 0) return;
+  if (guard == 2 && S.sys_rel[s] == 0) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int slot = lane / LPR, l = lane % LPR, jg = l % G, ph = l / G;
+  const int r0 = TL.row0[t], nr = TL.nrows[t];
+  double acc[RPL][ND > 0 ? ND : 1];
+#pragma unroll
+  for (int j = 0; j < RPL; ++j)
+#pragma unroll
+    for (int d = 0; d < (ND > 0 ? ND : 1); ++d) acc[j][d] = 0.0;
+  const int npass = (nr + 4 * RPW - 1) / (4 * RPW);
+  for (int pass = 0; pass < npass; ++pass) {
+    const int i = pass * 4 * RPW + wave * RPW + slot;
+    const bool valid = i < nr;
+    const int64_t g = r0 + (valid ? i : 0);
+    const int b0 = rowptr[g], b1 = valid ? rowptr[g + 1] : b0;
+    T yv[RPL][3];
+#pragma unroll
+    for (int j = 0; j < RPL; ++j) yv[j][0] = yv[j][1] = yv[j][2] = (T)0;
+    for (int bb = b0 + ph; bb < b1; bb += NPH * U) {
+      int c[U];
+      T a[U][9];
+      T xv[U][RPL][3];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int b = bb + u * NPH;
+        const int bc = b < b1 ? b : b1 - 1;
+        c[u] = col[bc];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) a[u][e] = val[9 * (int64_t)bc + e];
+#pragma unroll
+        for (int j = 0; j < RPL; ++j)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) xv[u][j][d] = x[((int64_t)c[u] * KS + jg * RPL + j) * 3 + d];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const bool live = bb + u * NPH < b1;
+#pragma unroll
+        for (int j = 0; j < RPL; ++j) {
+          T n0, n1, n2;
+          if constexpr (sizeof(T) == 4) {
+            n0 = fmaf(a[u][2], xv[u][j][2], fmaf(a[u][1], xv[u][j][1], fmaf(a[u][0], xv[u][j][0], yv[j][0])));
+            n1 = fmaf(a[u][5], xv[u][j][2], fmaf(a[u][4], xv[u][j][1], fmaf(a[u][3], xv[u][j][0], yv[j][1])));
+            n2 = fmaf(a[u][8], xv[u][j][2], fmaf(a[u][7], xv[u][j][1], fmaf(a[u][6], xv[u][j][0], yv[j][2])));
+          } else {
+            // the reference's order: ((y + a1 x1) + a2 x2) + a3 x3, blocks in storage order
+            n0 = yv[j][0] + a[u][0] * xv[u][j][0] + a[u][1] * xv[u][j][1] + a[u][2] * xv[u][j][2];
+            n1 = yv[j][1] + a[u][3] * xv[u][j][0] + a[u][4] * xv[u][j][1] + a[u][5] * xv[u][j][2];
+            n2 = yv[j][2] + a[u][6] * xv[u][j][0] + a[u][7] * xv[u][j][1] + a[u][8] * xv[u][j][2];
+          }
+          yv[j][0] = live ? n0 : yv[j][0];
+          yv[j][1] = live ? n1 : yv[j][1];
+          yv[j][2] = live ? n2 : yv[j][2];
+        }
+      }
+    }
+    if constexpr (NPH > 1) {
+#pragma unroll
+      for (int off = G; off < LPR; off <<= 1)
+#pragma unroll
+        for (int j = 0; j < RPL; ++j)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) yv[j][d] += __shfl_xor(yv[j][d], off, 64);
+    }
+    if (valid && ph == 0) {
+#pragma unroll
+      for (int j = 0; j < RPL; ++j) {
+        const int k = jg * RPL + j;
+        const int64_t o = (g * KS + k) * 3;
+        if constexpr (EPI == kbEpiQ) {
+          const float* rr = static_cast<const float*>(w);
+          float* q = static_cast<float*>(y);
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            q[o + d] = (float)yv[j][d];
+            const double pv = (double)x[o + d];
+            acc[j][0] += pv * (double)yv[j][d];
+            acc[j][ND - 1] += (double)rr[o + d] * pv;
+          }
+        } else if constexpr (EPI == kbEpiRes) {
+          const double* b = static_cast<const double*>(w);
+          float* r = static_cast<float*>(y);
+          if (S.rel[s * KS + k]) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+              const double v = b[o + d] - (double)yv[j][d];
+              r[o + d] = (float)v;
+              acc[j][0] += v * v;
+            }
+          }
+        } else {
+          T* yo = static_cast<T*>(y);
+#pragma unroll
+          for (int d = 0; d < 3; ++d) yo[o + d] = yv[j][d];
+        }
+      }
+    }
+  }
+  if constexpr (ND > 0) {
+    // thread (wave, slot, jg, ph=0) holds rhs jg*RPL+j; fixed-order sum over
+    // the workgroup's rows for every (dot, rhs)
+    __shared__ double red[ND][kBT][RPL];
+#pragma unroll
+    for (int d = 0; d < ND; ++d)
+#pragma unroll
+      for (int j = 0; j < RPL; ++j) red[d][threadIdx.x][j] = ph == 0 ? acc[j][d] : 0.0;
+    __syncthreads();
+    if (threadIdx.x < ND * KS) {
+      const int d = threadIdx.x / KS, k = threadIdx.x % KS;
+      const int jgk = k / RPL, j = k % RPL;
+      double sum = 0.0;
+      for (int m = 0; m < kBT / LPR; ++m) sum += red[d][m * LPR + jgk][j];
+      part[(int64_t)(d * KS + k) * TL.ntiles + t] = sum;
+    }
+  }
+}
+
+// ---- per-system tails: sum a system's tile partials, then the CG scalar logic ---------
+enum BTail : int { kbInit = 0, kbBeta = 1, kbAlpha = 2, kbRes = 3, kbRel = 4 };
+
+template <int TAIL, int ND>
+__global__ __launch_bounds__(kBT) void kb_tail(Tiles T, BState S, const double* part) {
+  const int s = blockIdx.x;
+  if (TAIL != kbInit && S.sys_active[s] == 0) return;
+  const int k = threadIdx.x % KS, sub = threadIdx.x / KS;
+  const int t0 = T.sys_tile0[s], nt = T.sys_ntile[s];
+  __shared__ double red[ND][kBT];
+  __shared__ int32_t flag[KS];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    double a = 0.0;
+    for (int i = sub; i < nt; i += kBT / KS) a += part[(int64_t)(d * KS + k) * T.ntiles + t0 + i];
+    red[d][threadIdx.x] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x < KS) {
+    double tot[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      double a = 0.0;
+      for (int m = 0; m < kBT / KS; ++m) a += red[d][m * KS + k];
+      tot[d] = a;
+    }
+    const int id = s * KS + k;
+    int32_t fl = 0;
+    if (TAIL == kbInit) {
+      // tot[0] = b.b, tot[1] = |b - A x0|^2
+      S.bnrm2[id] = tot[0];
+      S.rr_ref[id] = tot[ND - 1];
+      S.rho[id] = S.rho1[id] = S.zq[id] = S.pq[id] = S.rp[id] = S.alpha[id] = S.beta[id] = 0.0;
+      S.iter[id] = 1;
+      S.nindef[id] = 0;
+      S.rel[id] = 0;
+      S.error[id] = 0;
+      S.resid[id] = tot[0] == 0.0 ? 0.0 : sqrt(tot[ND - 1] / tot[0]);
+      // b = 0: x = 0 and nothing to do (hecmw_solver_CG.f90:135-141); the
+      // caller zeroes x for such instances
+      S.done[id] = (tot[0] == 0.0 || S.iprm[0] < 1) ? 1 : 0;
+      fl = !S.done[id];
+    } else if (!S.done[id]) {
+      if (TAIL == kbBeta) {
+        const double rho = tot[0];
+        S.rho[id] = rho;
+        S.zq[id] = tot[1];
+        const int it = S.iter[id];
+        if (rho == 0.0) {
+          S.done[id] = 1;
+        } else if (rho != rho) {
+          S.error[id] = kErrDivergeNan;
+          S.done[id] = 1;
+        } else if (it > 1 && rho * S.rho1[id] <= 0.0) {
+          S.nindef[id] += 1;
+          if (S.nindef[id] >= 3) {
+            S.error[id] = kErrDivergePc;
+            S.done[id] = 1;
+          }
+        }
+        if (!S.done[id]) S.beta[id] = it > 1 ? -S.zq[id] / S.pq[id] : 0.0;
+      } else if (TAIL == kbAlpha) {
+        const double pq = tot[0], rp = tot[1];
+        S.pq[id] = pq;
+        S.rp[id] = rp;
+        if (pq <= 0.0) {
+          S.error[id] = kErrDivergeMat;
+          S.done[id] = 1;
+        } else if (pq != pq || rp != rp) {
+          S.error[id] = kErrDivergeNan;
+          S.done[id] = 1;
+        } else {
+          S.alpha[id] = rp / pq;
+        }
+      } else if (TAIL == kbRes) {
+        const double rr = tot[0];
+        S.resid[id] = sqrt(rr / S.bnrm2[id]);
+        S.rel[id] = S.resid[id] <= S.prm[0] || rr <= S.prm[1] * S.rr_ref[id] || (S.iter[id] % S.iprm[1]) == 0 ||
+                    rr != rr;
+        fl = S.rel[id];
+      } else if (TAIL == kbRel) {
+        if (S.rel[id]) {
+          const double rr = tot[0];
+          S.rr_ref[id] = rr;
+          S.resid[id] = sqrt(rr / S.bnrm2[id]);
+          S.rel[id] = 0;
+          if (S.resid[id] <= S.prm[0]) {
+            S.done[id] = 1;
+          } else if (rr != rr) {
+            S.error[id] = kErrDivergeNan;
+            S.done[id] = 1;
+          }
+        }
+        if (!S.done[id]) {
+          if (S.iter[id] >= S.iprm[0]) {
+            S.error[id] = kErrNoconv;
+            S.done[id] = 1;
+            S.iter[id] = S.iprm[0] + 1;
+          } else {
+            S.rho1[id] = S.rho[id];
+            S.iter[id] += 1;
+          }
+        }
+      }
+    }
+    if (TAIL == kbRel || TAIL == kbInit) fl = !S.done[id];
+    flag[k] = fl;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t n = 0;
+    for (int j = 0; j < KS; ++j) n += flag[j];
+    if (TAIL == kbRes) S.sys_rel[s] = n > 0;
+    if (TAIL == kbRel || TAIL == kbInit) S.sys_active[s] = n;
+    if (TAIL == kbInit) S.sys_rel[s] = n > 0;
+  }
+}
+
+__global__ void kb_count(int32_t nsys, BState S) {
+  if (threadIdx.x == 0) {
+    int32_t n = 0;
+    for (int s = 0; s < nsys; ++s) n += S.sys_active[s];
+    *S.active_total = n;
+  }
+}
+
+// b.b partials per tile (initial phase)
+__global__ __launch_bounds__(kBT) void kb_bb(Tiles T, const double* __restrict__ b, double* part) {
+  const int t = blockIdx.x;
+  const int k = threadIdx.x % KS, rl = threadIdx.x / KS;
+  double acc[1] = {0.0};
+  const int r0 = T.row0[t], nr = T.nrows[t];
+  for (int i = rl; i < nr; i += kBT / KS) {
+    const int64_t o = ((int64_t)(r0 + i) * KS + k) * 3;
+    acc[0] += b[o] * b[o] + b[o + 1] * b[o + 1] + b[o + 2] * b[o + 2];
+  }
+  tile_partials_rowlanes<1>(acc, part, T.ntiles, t);
+}
+
+__global__ void kb_fill0(int64_t n, float* __restrict__ v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    v[i] = 0.0f;
+}
+__global__ void kb_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+
+// ---- SpMM variant table ------------------------------------------------------------------
+using SpmmFn = void (*)(int grid, hipStream_t, Tiles, const int32_t*, const int32_t*, const void*, const void*,
+                        const void*, void*, BState, double*, int);
+
+template <typename T, int LPR, int RPL, int U, int EPI>
+void spmm_launch(int grid, hipStream_t st, Tiles tl, const int32_t* rp, const int32_t* col, const void* val,
+                 const void* x, const void* w, void* y, BState S, double* part, int guard) {
+  kb_spmm<T, LPR, RPL, U, EPI><<<grid, kBT, 0, st>>>(tl, rp, col, static_cast<const T*>(val),
+                                                     static_cast<const T*>(x), w, y, S, part, guard);
+}
+
+struct Variant {
+  const char* name;
+  int lpr, rpl, u;
+  SpmmFn q32, st32;
+};
+#define HECMW33_VARIANT(L, R, U)                                                                        \
+  Variant {                                                                                             \
+    "lpr" #L "_rpl" #R "_u" #U, L, R, U, &spmm_launch<float, L, R, U, kbEpiQ>,                          \
+        &spmm_launch<float, L, R, U, kbEpiStore>                                                        \
+  }
+const Variant kVariants[] = {
+    HECMW33_VARIANT(16, 1, 1), HECMW33_VARIANT(16, 1, 2), HECMW33_VARIANT(8, 2, 1), HECMW33_VARIANT(8, 2, 2),
+    HECMW33_VARIANT(4, 4, 1),  HECMW33_VARIANT(4, 4, 2),  HECMW33_VARIANT(32, 1, 1), HECMW33_VARIANT(32, 1, 2),
+    HECMW33_VARIANT(64, 1, 1), HECMW33_VARIANT(16, 2, 1), HECMW33_VARIANT(32, 2, 1), HECMW33_VARIANT(64, 4, 1),
+};
+constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+#undef HECMW33_VARIANT
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// host side
+static thread_local std::string g_batch_error;
+#define BHIPC(expr)                                                               \
+  do {                                                                            \
+    hipError_t _e = (expr);                                                       \
+    if (_e != hipSuccess) {                                                       \
+      g_batch_error = std::string(#expr) + ": " + hipGetErrorString(_e);          \
+      return HECMW33_ERROR_DEVICE;                                                \
+    }                                                                             \
+  } while (0)
+
+struct BMem {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t n) {
+    if (n <= bytes && p) return 0;
+    if (p) hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (n == 0) n = 8;
+    if (hipMalloc(&p, n) != hipSuccess) {
+      g_batch_error = "hipMalloc failed";
+      return HECMW33_ERROR_DEVICE;
+    }
+    bytes = n;
+    return 0;
+  }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+}  // namespace hecmw33
+
+using namespace hecmw33;
+
+struct hecmw33_batch {
+  hipStream_t stream = nullptr;
+  int32_t nsys = 0, nrhs = 0;
+  int64_t nrows = 0, nnzb = 0;
+  std::vector<int32_t> N, NP, row0;     // per system
+  std::vector<int32_t> tile_sys_h, tile_row0_h;
+  int32_t ntiles = 0;
+  BMem rowptr, col, val64, val32, alu32, d64;
+  BMem tile_row0, tile_nrows, tile_sys, sys_tile0, sys_ntile;
+  BMem b, x, r, z, p, q, part, state;
+  BState S{};
+  Tiles T{};
+  int variant = -1;
+  float variant_ms[64] = {};
+  hipGraphExec_t graph = nullptr;
+  int32_t* h_active = nullptr;  // pinned, 2 slots
+  double spmm_ms = 0.0;
+
+  ~hecmw33_batch() {
+    if (stream) hipStreamSynchronize(stream);
+    if (graph) hipGraphExecDestroy(graph);
+    for (BMem* m : {&rowptr, &col, &val64, &val32, &alu32, &d64, &tile_row0, &tile_nrows, &tile_sys, &sys_tile0,
+                    &sys_ntile, &b, &x, &r, &z, &p, &q, &part, &state})
+      m->release();
+    if (h_active) hipHostFree(h_active);
+    if (stream) hipStreamDestroy(stream);
+  }
+};
+
+namespace hecmw33 {
+
+static int32_t batch_state_layout(hecmw33_batch& B) {
+  const int64_t ni = (int64_t)B.nsys * KS;
+  const size_t bytes = sizeof(double) * (10 * ni + 2) + sizeof(int32_t) * (5 * ni + 2 * B.nsys + 3) + 256;
+  if (B.state.ensure(bytes)) return HECMW33_ERROR_DEVICE;
+  char* base = B.state.as<char>();
+  double* d = reinterpret_cast<double*>(base);
+  BState& S = B.S;
+  double** dp[10] = {&S.bnrm2, &S.rho, &S.rho1, &S.zq, &S.pq, &S.rp, &S.alpha, &S.beta, &S.rr_ref, &S.resid};
+  for (int i = 0; i < 10; ++i) *dp[i] = d + i * ni;
+  S.prm = d + 10 * ni;
+  int32_t* ip = reinterpret_cast<int32_t*>(d + 10 * ni + 2);
+  int32_t** ipp[5] = {&S.iter, &S.done, &S.error, &S.nindef, &S.rel};
+  for (int i = 0; i < 5; ++i) *ipp[i] = ip + i * ni;
+  S.sys_active = ip + 5 * ni;
+  S.sys_rel = S.sys_active + B.nsys;
+  S.active_total = S.sys_rel + B.nsys;
+  S.iprm = S.active_total + 1;
+  return 0;
+}
+
+static int grid_of(int64_t n) { return (int)std::min<int64_t>(std::max<int64_t>((n + kBT - 1) / kBT, 1), 8192); }
+
+// one FCG iteration for every live instance
+static void batch_enqueue_iteration(hecmw33_batch& B) {
+  hipStream_t st = B.stream;
+  const int nt = B.ntiles;
+  double* part = B.part.as<double>();
+  float *r = B.r.as<float>(), *z = B.z.as<float>(), *p = B.p.as<float>(), *q = B.q.as<float>();
+  kb_diag<<<nt, kBT, 0, st>>>(B.T, B.alu32.as<float>(), r, z, q, B.S, part);
+  kb_tail<kbBeta, 2><<<B.nsys, kBT, 0, st>>>(B.T, B.S, part);
+  kb_p<<<nt, kBT, 0, st>>>(B.T, z, p, B.S);
+  kVariants[B.variant].q32(nt, st, B.T, B.rowptr.as<int32_t>(), B.col.as<int32_t>(), B.val32.p, p, r, q, B.S, part, 1);
+  kb_tail<kbAlpha, 2><<<B.nsys, kBT, 0, st>>>(B.T, B.S, part);
+  kb_update<<<nt, kBT, 0, st>>>(B.T, p, q, B.x.as<double>(), r, B.S, part);
+  kb_tail<kbRes, 1><<<B.nsys, kBT, 0, st>>>(B.T, B.S, part);
+  spmm_launch<double, 16, 1, 2, kbEpiRes>(nt, st, B.T, B.rowptr.as<int32_t>(), B.col.as<int32_t>(), B.val64.p,
+                                          B.x.p, B.b.p, r, B.S, part, 2);
+  kb_tail<kbRel, 1><<<B.nsys, kBT, 0, st>>>(B.T, B.S, part);
+}
+
+static int32_t batch_autotune(hecmw33_batch& B) {
+  // time every variant's FP32 SpMM with its fused dots on the loaded matrix
+  hipEvent_t e0, e1;
+  BHIPC(hipEventCreate(&e0));
+  BHIPC(hipEventCreate(&e1));
+  float best = 1e30f;
+  B.variant = 0;
+  // all systems live for the timing
+  std::vector<int32_t> ones(B.nsys, 1);
+  BHIPC(hipMemcpyAsync(B.S.sys_active, ones.data(), sizeof(int32_t) * B.nsys, hipMemcpyHostToDevice, B.stream));
+  for (int v = 0; v < kNumVariants; ++v) {
+    for (int rep = 0; rep < 2; ++rep)
+      kVariants[v].q32(B.ntiles, B.stream, B.T, B.rowptr.as<int32_t>(), B.col.as<int32_t>(), B.val32.p, B.p.p,
+                       B.r.p, B.q.p, B.S, B.part.as<double>(), 0);
+    const int reps = 5;
+    BHIPC(hipEventRecord(e0, B.stream));
+    for (int rep = 0; rep < reps; ++rep)
+      kVariants[v].q32(B.ntiles, B.stream, B.T, B.rowptr.as<int32_t>(), B.col.as<int32_t>(), B.val32.p, B.p.p,
+                       B.r.p, B.q.p, B.S, B.part.as<double>(), 0);
+    BHIPC(hipEventRecord(e1, B.stream));
+    BHIPC(hipEventSynchronize(e1));
+    float ms = 0.f;
+    BHIPC(hipEventElapsedTime(&ms, e0, e1));
+    ms /= reps;
+    B.variant_ms[v] = ms;
+    if (ms < best) {
+      best = ms;
+      B.variant = v;
+    }
+  }
+  B.spmm_ms = best;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  BHIPC(hipGetLastError());
+  return 0;
+}
+
+static int32_t batch_load(hecmw33_batch& B, int32_t nsys, const hecmw33_sys* sys, int32_t nrhs, const double* const* Bh,
+                          const double* rarray) {
+  if (nsys < 1 || !sys || nrhs < 1 || nrhs > KS || !Bh) {
+    g_batch_error = "batch: need nsys >= 1, 1 <= nrhs <= 16 and right-hand sides";
+    return HECMW33_ERROR_ARGS;
+  }
+  if (!B.stream) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
+      g_batch_error = "no HIP device visible";
+      return HECMW33_ERROR_DEVICE;
+    }
+    BHIPC(hipStreamCreateWithFlags(&B.stream, hipStreamNonBlocking));
+    BHIPC(hipHostMalloc((void**)&B.h_active, 2 * sizeof(int32_t), hipHostMallocDefault));
+  }
+  if (B.graph) {
+    hipGraphExecDestroy(B.graph);
+    B.graph = nullptr;
+  }
+  B.nsys = nsys;
+  B.nrhs = nrhs;
+  B.N.assign(nsys, 0);
+  B.NP.assign(nsys, 0);
+  B.row0.assign(nsys + 1, 0);
+  int64_t nnz = 0;
+  for (int s = 0; s < nsys; ++s) {
+    const hecmw33_sys& A = sys[s];
+    if (A.N < 1 || A.NP != A.N || !A.D || !A.indexL || !A.indexU || (A.NPL > 0 && (!A.AL || !A.itemL)) ||
+        (A.NPU > 0 && (!A.AU || !A.itemU)) || A.indexL[A.N] != A.NPL || A.indexU[A.N] != A.NPU || !Bh[s]) {
+      g_batch_error = "batch: invalid system " + std::to_string(s) + " (one domain, NP == N)";
+      return HECMW33_ERROR_ARGS;
+    }
+    for (int64_t q = 0; q < A.NPL; ++q)
+      if (A.itemL[q] < 1 || A.itemL[q] > A.N) {
+        g_batch_error = "batch: itemL out of range";
+        return HECMW33_ERROR_ARGS;
+      }
+    for (int64_t q = 0; q < A.NPU; ++q)
+      if (A.itemU[q] < 1 || A.itemU[q] > A.N) {
+        g_batch_error = "batch: itemU out of range";
+        return HECMW33_ERROR_ARGS;
+      }
+    B.N[s] = A.N;
+    B.NP[s] = A.NP;
+    B.row0[s + 1] = B.row0[s] + A.N;
+    nnz += (int64_t)A.N + A.NPL + A.NPU;
+  }
+  B.nrows = B.row0[nsys];
+  B.nnzb = nnz;
+  if (nnz > INT32_MAX) {
+    g_batch_error = "batch: more than 2^31 blocks";
+    return HECMW33_ERROR_ARGS;
+  }
+  // merged rows D, L, U (storage order), global columns; values FP64
+  std::vector<int32_t> rp(B.nrows + 1), cl(nnz);
+  std::vector<double> v64(9 * nnz), d64(9 * (size_t)B.nrows);
+  for (int s = 0; s < nsys; ++s) {
+    const hecmw33_sys& A = sys[s];
+    const int32_t g0 = B.row0[s];
+    for (int32_t i = 0; i < A.N; ++i) rp[g0 + i + 1] = 1 + (A.indexL[i + 1] - A.indexL[i]) + (A.indexU[i + 1] - A.indexU[i]);
+  }
+  rp[0] = 0;
+  for (int64_t g = 0; g < B.nrows; ++g) rp[g + 1] += rp[g];
+  for (int s = 0; s < nsys; ++s) {
+    const hecmw33_sys& A = sys[s];
+    const int32_t g0 = B.row0[s];
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int32_t i = 0; i < A.N; ++i) {
+      int64_t o = rp[g0 + i];
+      cl[o] = g0 + i;
+      std::memcpy(&v64[9 * o], A.D + 9 * (size_t)i, 72);
+      ++o;
+      for (int32_t q = A.indexL[i]; q < A.indexL[i + 1]; ++q, ++o) {
+        cl[o] = g0 + A.itemL[q] - 1;
+        std::memcpy(&v64[9 * o], A.AL + 9 * (size_t)q, 72);
+      }
+      for (int32_t q = A.indexU[i]; q < A.indexU[i + 1]; ++q, ++o) {
+        cl[o] = g0 + A.itemU[q] - 1;
+        std::memcpy(&v64[9 * o], A.AU + 9 * (size_t)q, 72);
+      }
+      std::memcpy(&d64[9 * (size_t)(g0 + i)], A.D + 9 * (size_t)i, 72);
+    }
+  }
+  // tiles: runs of rows inside a system, at least kTileRows, count capped
+  int tr = kTileRows;
+  for (;;) {
+    int64_t nt = 0;
+    for (int s = 0; s < nsys; ++s) nt += (B.N[s] + tr - 1) / tr;
+    if (nt <= kMaxTiles) break;
+    tr *= 2;
+  }
+  std::vector<int32_t> t_row0, t_nrows, t_sys, s_t0(nsys), s_nt(nsys);
+  for (int s = 0; s < nsys; ++s) {
+    s_t0[s] = (int32_t)t_row0.size();
+    for (int32_t i = 0; i < B.N[s]; i += tr) {
+      t_row0.push_back(B.row0[s] + i);
+      t_nrows.push_back(std::min(tr, B.N[s] - i));
+      t_sys.push_back(s);
+    }
+    s_nt[s] = (int32_t)t_row0.size() - s_t0[s];
+  }
+  B.ntiles = (int32_t)t_row0.size();
+  // device buffers
+  const int64_t nv = B.nrows * KS * 3;
+  if (B.rowptr.ensure(sizeof(int32_t) * rp.size()) || B.col.ensure(sizeof(int32_t) * nnz) ||
+      B.val64.ensure(sizeof(double) * 9 * nnz) || B.val32.ensure(sizeof(float) * 9 * nnz) ||
+      B.d64.ensure(sizeof(double) * 9 * B.nrows) || B.alu32.ensure(sizeof(float) * 9 * B.nrows) ||
+      B.tile_row0.ensure(sizeof(int32_t) * B.ntiles) || B.tile_nrows.ensure(sizeof(int32_t) * B.ntiles) ||
+      B.tile_sys.ensure(sizeof(int32_t) * B.ntiles) || B.sys_tile0.ensure(sizeof(int32_t) * nsys) ||
+      B.sys_ntile.ensure(sizeof(int32_t) * nsys) || B.b.ensure(sizeof(double) * nv) ||
+      B.x.ensure(sizeof(double) * nv) || B.r.ensure(sizeof(float) * nv) || B.z.ensure(sizeof(float) * nv) ||
+      B.p.ensure(sizeof(float) * nv) || B.q.ensure(sizeof(float) * nv) ||
+      B.part.ensure(sizeof(double) * 2 * KS * (size_t)B.ntiles) || batch_state_layout(B))
+    return HECMW33_ERROR_DEVICE;
+  hipStream_t st = B.stream;
+  BHIPC(hipMemcpyAsync(B.rowptr.p, rp.data(), sizeof(int32_t) * rp.size(), hipMemcpyHostToDevice, st));
+  BHIPC(hipMemcpyAsync(B.col.p, cl.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice, st));
+  BHIPC(hipMemcpyAsync(B.val64.p, v64.data(), sizeof(double) * 9 * nnz, hipMemcpyHostToDevice, st));
+  BHIPC(hipMemcpyAsync(B.d64.p, d64.data(), sizeof(double) * 9 * B.nrows, hipMemcpyHostToDevice, st));
+  BHIPC(hipMemcpyAsync(B.tile_row0.p, t_row0.data(), sizeof(int32_t) * B.ntiles, hipMemcpyHostToDevice, st));
+  BHIPC(hipMemcpyAsync(B.tile_nrows.p, t_nrows.data(), sizeof(int32_t) * B.ntiles, hipMemcpyHostToDevice, st));
+  BHIPC(hipMemcpyAsync(B.tile_sys.p, t_sys.data(), sizeof(int32_t) * B.ntiles, hipMemcpyHostToDevice, st));
+  BHIPC(hipMemcpyAsync(B.sys_tile0.p, s_t0.data(), sizeof(int32_t) * nsys, hipMemcpyHostToDevice, st));
+  BHIPC(hipMemcpyAsync(B.sys_ntile.p, s_nt.data(), sizeof(int32_t) * nsys, hipMemcpyHostToDevice, st));
+  // right-hand sides into [row][16][3] (unused columns zero)
+  {
+    std::vector<double> bh(nv, 0.0);
+    for (int s = 0; s < nsys; ++s) {
+      const int64_t n3 = 3 * (int64_t)B.NP[s];
+      for (int k = 0; k < nrhs; ++k)
+        for (int32_t i = 0; i < B.N[s]; ++i)
+          for (int d = 0; d < 3; ++d) bh[((int64_t)(B.row0[s] + i) * KS + k) * 3 + d] = Bh[s][n3 * k + 3 * (int64_t)i + d];
+    }
+    BHIPC(hipMemcpyAsync(B.b.p, bh.data(), sizeof(double) * nv, hipMemcpyHostToDevice, st));
+    BHIPC(hipStreamSynchronize(st));
+  }
+  kb_to_f32<<<grid_of(9 * nnz), kBT, 0, st>>>(9 * nnz, B.val64.as<double>(), B.val32.as<float>());
+  // DIAG factors (hecmw_precond_DIAG_33.F90:27-104) in FP64, rounded to FP32
+  {
+    BMem rows, alu64;
+    std::vector<int32_t> id(B.nrows);
+    for (int64_t g = 0; g < B.nrows; ++g) id[g] = (int32_t)g;
+    if (rows.ensure(sizeof(int32_t) * B.nrows) || alu64.ensure(sizeof(double) * 9 * B.nrows))
+      return HECMW33_ERROR_DEVICE;
+    BHIPC(hipMemcpyAsync(rows.p, id.data(), sizeof(int32_t) * B.nrows, hipMemcpyHostToDevice, st));
+    const double sigma = rarray ? rarray[1] : 1.0;
+    launch_block_lu((int32_t)B.nrows, rows.as<int32_t>(), B.d64.as<double>(), sigma > 0.0 ? sigma : 1.0,
+                    alu64.as<double>(), 1, st);
+    kb_to_f32<<<grid_of(9 * B.nrows), kBT, 0, st>>>(9 * B.nrows, alu64.as<double>(), B.alu32.as<float>());
+    BHIPC(hipStreamSynchronize(st));
+    rows.release();
+    alu64.release();
+  }
+  B.T.row0 = B.tile_row0.as<int32_t>();
+  B.T.nrows = B.tile_nrows.as<int32_t>();
+  B.T.sys = B.tile_sys.as<int32_t>();
+  B.T.sys_tile0 = B.sys_tile0.as<int32_t>();
+  B.T.sys_ntile = B.sys_ntile.as<int32_t>();
+  B.T.ntiles = B.ntiles;
+  for (BMem* m : {&B.r, &B.z, &B.p, &B.q}) kb_fill0<<<grid_of(nv), kBT, 0, st>>>(nv, m->as<float>());
+  BHIPC(hipGetLastError());
+  BHIPC(hipStreamSynchronize(st));
+  return batch_autotune(B);
+}
+
+static int32_t batch_solve(hecmw33_batch& B, const int32_t* iarray, const double* rarray, int32_t* iter,
+                           int32_t* error, double* resid, double* times) {
+  if (!B.stream || B.nsys < 1) {
+    g_batch_error = "batch: nothing loaded";
+    return HECMW33_ERROR_ARGS;
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  hipStream_t st = B.stream;
+  const int32_t maxit = iarray ? iarray[0] : 100;
+  const int32_t pc = iarray ? iarray[2] : 3;
+  const int32_t iterpremax = iarray ? iarray[4] : 1;
+  if (pc != 3 || iterpremax != 1) {
+    g_batch_error = "batch: DIAG (PRECOND=3, ITERPREMAX=1) only";
+    return HECMW33_ERROR_UNSUPPORTED;
+  }
+  const double delta = rarray && rarray[40] > 0.0 && rarray[40] < 1.0 ? rarray[40] : 1.0e-2;
+  const double prm[2] = {rarray ? rarray[0] : 1e-8, delta * delta};
+  const int32_t iprm[2] = {maxit, 50};
+  BHIPC(hipMemcpyAsync(B.S.prm, prm, sizeof(prm), hipMemcpyHostToDevice, st));
+  BHIPC(hipMemcpyAsync(B.S.iprm, iprm, sizeof(iprm), hipMemcpyHostToDevice, st));
+  const int64_t nv = B.nrows * KS * 3;
+  double* part = B.part.as<double>();
+  // x0 = 0; r = b - A x0 in FP64 -> r32; b.b; instance state
+  BHIPC(hipMemsetAsync(B.x.p, 0, sizeof(double) * nv, st));
+  for (BMem* m : {&B.z, &B.p, &B.q}) kb_fill0<<<grid_of(nv), kBT, 0, st>>>(nv, m->as<float>());
+  kb_bb<<<B.ntiles, kBT, 0, st>>>(B.T, B.b.as<double>(), part);
+  std::vector<int32_t> ones((size_t)B.nsys * KS, 1), sys1(B.nsys, 1);
+  BHIPC(hipMemcpyAsync(B.S.rel, ones.data(), sizeof(int32_t) * ones.size(), hipMemcpyHostToDevice, st));
+  BHIPC(hipMemcpyAsync(B.S.sys_rel, sys1.data(), sizeof(int32_t) * B.nsys, hipMemcpyHostToDevice, st));
+  // |b - A x0|^2 into the second dot slot
+  spmm_launch<double, 16, 1, 2, kbEpiRes>(B.ntiles, st, B.T, B.rowptr.as<int32_t>(), B.col.as<int32_t>(), B.val64.p,
+                                          B.x.p, B.b.p, B.r.p, B.S, part + (size_t)KS * B.ntiles, 0);
+  kb_tail<kbInit, 2><<<B.nsys, kBT, 0, st>>>(B.T, B.S, part);
+  BHIPC(hipGetLastError());
+  BHIPC(hipStreamSynchronize(st));
+  const auto t1 = std::chrono::steady_clock::now();
+  // capture kChunk iterations once (pointers and sizes are fixed after load)
+  if (!B.graph) {
+    hipGraph_t g;
+    BHIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < kChunk; ++i) batch_enqueue_iteration(B);
+    kb_count<<<1, 64, 0, st>>>(B.nsys, B.S);
+    BHIPC(hipStreamEndCapture(st, &g));
+    BHIPC(hipGraphInstantiate(&B.graph, g, nullptr, nullptr, 0));
+    hipGraphDestroy(g);
+  }
+  hipEvent_t ev[2];
+  BHIPC(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  BHIPC(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  const int64_t nchunks = (maxit + kChunk - 1) / kChunk;
+  int64_t launched = 0;
+  bool finished = maxit < 1;
+  for (int64_t c = 0; c < nchunks && !finished; ++c) {
+    BHIPC(hipGraphLaunch(B.graph, st));
+    const int slot = (int)(c & 1);
+    BHIPC(hipMemcpyAsync(&B.h_active[slot], B.S.active_total, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    BHIPC(hipEventRecord(ev[slot], st));
+    ++launched;
+    if (c > 0) {
+      BHIPC(hipEventSynchronize(ev[slot ^ 1]));
+      if (B.h_active[slot ^ 1] == 0) finished = true;
+    }
+  }
+  BHIPC(hipStreamSynchronize(st));
+  hipEventDestroy(ev[0]);
+  hipEventDestroy(ev[1]);
+  const auto t2 = std::chrono::steady_clock::now();
+  const int64_t ni = (int64_t)B.nsys * KS;
+  std::vector<int32_t> it(ni), er(ni);
+  std::vector<double> rs(ni);
+  BHIPC(hipMemcpy(it.data(), B.S.iter, sizeof(int32_t) * ni, hipMemcpyDeviceToHost));
+  BHIPC(hipMemcpy(er.data(), B.S.error, sizeof(int32_t) * ni, hipMemcpyDeviceToHost));
+  BHIPC(hipMemcpy(rs.data(), B.S.resid, sizeof(double) * ni, hipMemcpyDeviceToHost));
+  for (int s = 0; s < B.nsys; ++s)
+    for (int k = 0; k < B.nrhs; ++k) {
+      const int64_t o = (int64_t)s * B.nrhs + k;
+      if (iter) iter[o] = it[s * KS + k];
+      if (error) error[o] = er[s * KS + k];
+      if (resid) resid[o] = rs[s * KS + k];
+    }
+  if (times) {
+    times[0] = std::chrono::duration<double>(t1 - t0).count();
+    times[1] = std::chrono::duration<double>(t2 - t1).count();
+    times[2] = B.spmm_ms;
+    times[3] = (double)launched * kChunk;
+  }
+  return 0;
+}
+
+}  // namespace hecmw33
+
+extern "C" {
+
+hecmw33_batch* hecmw33_batch_create(void) { return new (std::nothrow) hecmw33_batch(); }
+
+void hecmw33_batch_destroy(hecmw33_batch* b) { delete b; }
+
+const char* hecmw33_batch_last_error(void) { return g_batch_error.c_str(); }
+
+int32_t hecmw33_batch_load(hecmw33_batch* b, int32_t nsys, const hecmw33_sys* sys, int32_t nrhs,
+                           const double* const* B, const int32_t iarray[100], const double rarray[100]) {
+  (void)iarray;
+  if (!b) return HECMW33_ERROR_ARGS;
+  return batch_load(*b, nsys, sys, nrhs, B, rarray);
+}
+
+int32_t hecmw33_batch_solve(hecmw33_batch* b, const int32_t iarray[100], const double rarray[100], int32_t* iter,
+                            int32_t* error, double* resid, double* times) {
+  if (!b) return HECMW33_ERROR_ARGS;
+  return batch_solve(*b, iarray, rarray, iter, error, resid, times);
+}
+
+int32_t hecmw33_batch_get_x(hecmw33_batch* b, int32_t s, double* X) {
+  if (!b || s < 0 || s >= b->nsys || !X) return HECMW33_ERROR_ARGS;
+  const int64_t n = (int64_t)b->N[s] * KS * 3;
+  std::vector<double> h(n);
+  BHIPC(hipMemcpy(h.data(), b->x.as<double>() + (int64_t)b->row0[s] * KS * 3, sizeof(double) * n,
+                  hipMemcpyDeviceToHost));
+  const int64_t n3 = 3 * (int64_t)b->NP[s];
+  for (int k = 0; k < b->nrhs; ++k)
+    for (int32_t i = 0; i < b->N[s]; ++i)
+      for (int d = 0; d < 3; ++d) X[n3 * k + 3 * (int64_t)i + d] = h[((int64_t)i * KS + k) * 3 + d];
+  return 0;
+}
+
+// Y_s = A_s X_s for every system and right-hand side (host arrays, nrhs
+// columns of 3*N_s each).  fp64 != 0: FP64 in the reference's order;
+// otherwise the FP32 SpMM of the tuned variant (values rounded to FP32).
+int32_t hecmw33_batch_spmm(hecmw33_batch* b, int32_t fp64, const double* const* X, double* const* Y) {
+  if (!b || !b->stream || !X || !Y) return HECMW33_ERROR_ARGS;
+  const int64_t nv = b->nrows * KS * 3;
+  std::vector<double> h(nv, 0.0);
+  for (int s = 0; s < b->nsys; ++s) {
+    const int64_t n3 = 3 * (int64_t)b->NP[s];
+    for (int k = 0; k < b->nrhs; ++k)
+      for (int32_t i = 0; i < b->N[s]; ++i)
+        for (int d = 0; d < 3; ++d) h[((int64_t)(b->row0[s] + i) * KS + k) * 3 + d] = X[s][n3 * k + 3 * (int64_t)i + d];
+  }
+  hipStream_t st = b->stream;
+  BMem xin, yout;
+  if (xin.ensure(sizeof(double) * nv) || yout.ensure(sizeof(double) * nv)) return HECMW33_ERROR_DEVICE;
+  BHIPC(hipMemcpy(xin.p, h.data(), sizeof(double) * nv, hipMemcpyHostToDevice));
+  if (fp64) {
+    spmm_launch<double, 16, 1, 2, kbEpiStore>(b->ntiles, st, b->T, b->rowptr.as<int32_t>(), b->col.as<int32_t>(),
+                                              b->val64.p, xin.p, nullptr, yout.p, b->S, nullptr, 0);
+  } else {
+    float* x32 = b->z.as<float>();  // scratch: the solve re-initialises it
+    float* y32 = b->q.as<float>();
+    kb_to_f32<<<grid_of(nv), kBT, 0, st>>>(nv, xin.as<double>(), x32);
+    kVariants[b->variant].st32(b->ntiles, st, b->T, b->rowptr.as<int32_t>(), b->col.as<int32_t>(), b->val32.p, x32,
+                               nullptr, y32, b->S, nullptr, 0);
+    BHIPC(hipStreamSynchronize(st));
+    std::vector<float> hf(nv);
+    BHIPC(hipMemcpy(hf.data(), y32, sizeof(float) * nv, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < nv; ++i) h[i] = hf[i];
+  }
+  BHIPC(hipGetLastError());
+  BHIPC(hipStreamSynchronize(st));
+  if (fp64) BHIPC(hipMemcpy(h.data(), yout.p, sizeof(double) * nv, hipMemcpyDeviceToHost));
+  for (int s = 0; s < b->nsys; ++s) {
+    const int64_t n3 = 3 * (int64_t)b->NP[s];
+    for (int k = 0; k < b->nrhs; ++k)
+      for (int32_t i = 0; i < b->N[s]; ++i)
+        for (int d = 0; d < 3; ++d) Y[s][n3 * k + 3 * (int64_t)i + d] = h[((int64_t)(b->row0[s] + i) * KS + k) * 3 + d];
+  }
+  return 0;
+}
+
+// [0] rows, [1] blocks, [2] tiles, [3] tuned variant, [4..4+n) variant times in us (FP32 SpMM + dots)
+int32_t hecmw33_batch_info(hecmw33_batch* b, int64_t* info, int32_t n) {
+  if (!b || !info) return HECMW33_ERROR_ARGS;
+  std::vector<int64_t> v = {b->nrows, b->nnzb, b->ntiles, b->variant};
+  for (int i = 0; i < kNumVariants; ++i) v.push_back((int64_t)std::llround(b->variant_ms[i] * 1000.0));
+  for (int i = 0; i < n && i < (int)v.size(); ++i) info[i] = v[i];
+  return (int32_t)v.size();
+}
+
+const char* hecmw33_batch_variant_name(int32_t v) {
+  return v >= 0 && v < kNumVariants ? kVariants[v].name : "";
+}
+
+// One-shot: load, solve, copy every solution back.
+int32_t hecmw33_solve_batch(int32_t nsys, const hecmw33_sys* sys, int32_t nrhs, const double* const* B,
+                            double* const* X, const int32_t iarray[100], const double rarray[100], int32_t* iter,
+                            int32_t* error, double* resid) {
+  hecmw33_batch b;
+  if (int32_t e = batch_load(b, nsys, sys, nrhs, B, rarray)) return e;
+  if (int32_t e = batch_solve(b, iarray, rarray, iter, error, resid, nullptr)) return e;
+  for (int s = 0; s < nsys; ++s)
+    if (int32_t e = hecmw33_batch_get_x(&b, s, X[s])) return e;
+  return 0;
+}
+
+}  // extern "C"

@@ -82,13 +82,25 @@ def lib():
     L.hecmw33_gen_tet10_count.argtypes = [I, I, I, C.POINTER(I), C.POINTER(I), C.POINTER(I)]
     L.hecmw33_gen_tet10.argtypes = [I, I, I, P, C.c_double, C.c_double, I, C.c_uint64, P, P, P, P, P, P, P, P]
     L.hecmw33_last_history.argtypes = [P, I]
+    L.hecmw33_batch_create.restype = P
+    L.hecmw33_batch_destroy.argtypes = [P]
+    L.hecmw33_batch_load.argtypes = [P, I, P, I, P, P, P]
+    L.hecmw33_batch_solve.argtypes = [P, P, P, P, P, P, P]
+    L.hecmw33_batch_get_x.argtypes = [P, I, P]
+    L.hecmw33_batch_spmm.argtypes = [P, I, P, P]
+    L.hecmw33_batch_info.argtypes = [P, P, I]
+    L.hecmw33_batch_variant_name.argtypes = [I]
+    L.hecmw33_batch_variant_name.restype = C.c_char_p
+    L.hecmw33_batch_last_error.restype = C.c_char_p
+    L.hecmw33_solve_batch.argtypes = [I, P, I, P, P, P, P, P, P, P]
     L.hecmw33_last_error.restype = C.c_char_p
     for f in ("hecmw33_solve", "hecmw33_solve_krylov", "hecmw33_matvec", "hecmw33_precond_apply",
               "hecmw33_rel_resid_L2", "hecmw33_ordering", "hecmw33_ctx_load", "hecmw33_ctx_solve_device",
               "hecmw33_ctx_matvec_device", "hecmw33_ctx_matvec_device_f32", "hecmw33_ctx_history",
               "hecmw33_ctx_info", "hecmw33_gen_hex8_count", "hecmw33_gen_hex8",
               "hecmw33_gen_tet10_count", "hecmw33_gen_tet10", "hecmw33_device_count",
-              "hecmw33_last_history"):
+              "hecmw33_last_history", "hecmw33_batch_load", "hecmw33_batch_solve", "hecmw33_batch_get_x",
+              "hecmw33_batch_spmm", "hecmw33_batch_info", "hecmw33_solve_batch"):
         getattr(L, f).restype = C.c_int32
     _lib = L
     return L
@@ -400,3 +412,113 @@ class DeviceContext:
         _check(lib().hecmw33_ctx_info(self._h, _ptr(v), 7))
         return {"nslots": int(v[0]), "N": int(v[1]), "NPL": int(v[2]), "NPU": int(v[3]), "ncolor": int(v[4]),
                 "ssor_slots": int(v[5]), "nreliable": int(v[6])}
+
+
+def _bcheck(rc):
+    if rc < 0:
+        raise HecmwError("libhecmw33 batch error %d: %s" % (rc, lib().hecmw33_batch_last_error().decode()))
+    return rc
+
+
+class BatchSolver:
+    """Many systems x up to 16 right-hand sides in one block-diagonal SpMM
+    (hecmw33_batch_*): DIAG-preconditioned flexible CG per (system, rhs),
+    FP32 operators, FP64 x / inner products / residual replacement."""
+
+    def __init__(self, mats, Bs, iarray=None, rarray=None):
+        self._mats = [m for m in mats]
+        self.nrhs = int(Bs[0].shape[0])
+        self._B = [np.ascontiguousarray(b, dtype=np.float64) for b in Bs]
+        self._sys = (_Sys * len(mats))(*[m._sys() for m in mats])
+        bp = (C.c_void_p * len(mats))(*[b.ctypes.data for b in self._B])
+        ia = np.ascontiguousarray(mats[0].Iarray if iarray is None else iarray, dtype=np.int32)
+        ra = np.ascontiguousarray(mats[0].Rarray if rarray is None else rarray, dtype=np.float64)
+        self._h = lib().hecmw33_batch_create()
+        if not self._h:
+            raise HecmwError("hecmw33_batch_create failed")
+        _bcheck(lib().hecmw33_batch_load(self._h, len(mats), C.cast(self._sys, C.c_void_p), self.nrhs,
+                                         C.cast(bp, C.c_void_p), _ptr(ia), _ptr(ra)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().hecmw33_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, iarray, rarray):
+        """Returns (iter, error, resid) arrays of shape (nsys, nrhs) and the
+        times (setup s, solve s, tuned SpMM ms, iterations enqueued)."""
+        ns = len(self._mats)
+        it = np.zeros(ns * self.nrhs, np.int32)
+        er = np.zeros(ns * self.nrhs, np.int32)
+        rs = np.zeros(ns * self.nrhs)
+        tm = np.zeros(4)
+        ia = np.ascontiguousarray(iarray, dtype=np.int32)
+        ra = np.ascontiguousarray(rarray, dtype=np.float64)
+        _bcheck(lib().hecmw33_batch_solve(self._h, _ptr(ia), _ptr(ra), _ptr(it), _ptr(er), _ptr(rs), _ptr(tm)))
+        sh = (ns, self.nrhs)
+        return it.reshape(sh), er.reshape(sh), rs.reshape(sh), tm
+
+    def x(self, s):
+        X = np.zeros((self.nrhs, 3 * self._mats[s].NP))
+        _bcheck(lib().hecmw33_batch_get_x(self._h, s, _ptr(X)))
+        return X
+
+    def spmm(self, Xs, fp64=True):
+        Xs = [np.ascontiguousarray(x, dtype=np.float64) for x in Xs]
+        Ys = [np.zeros_like(x) for x in Xs]
+        xp = (C.c_void_p * len(Xs))(*[x.ctypes.data for x in Xs])
+        yp = (C.c_void_p * len(Ys))(*[y.ctypes.data for y in Ys])
+        _bcheck(lib().hecmw33_batch_spmm(self._h, 1 if fp64 else 0, C.cast(xp, C.c_void_p), C.cast(yp, C.c_void_p)))
+        return Ys
+
+    def info(self):
+        v = np.zeros(64, np.int64)
+        n = _bcheck(lib().hecmw33_batch_info(self._h, _ptr(v), 64))
+        names = [lib().hecmw33_batch_variant_name(i).decode() for i in range(n - 4)]
+        return {"rows": int(v[0]), "blocks": int(v[1]), "tiles": int(v[2]), "variant": names[int(v[3])],
+                "variant_us": dict(zip(names, [int(t) for t in v[4:n]]))}
+
+
+def solve_batch(mats, Bs, iarray, rarray):
+    """hecmw33_solve_batch: load + solve + copy back in one call.
+    Returns (X list of (nrhs, 3N) arrays, iter, error, resid of shape (nsys, nrhs))."""
+    ns, nrhs = len(mats), int(Bs[0].shape[0])
+    Bs = [np.ascontiguousarray(b, dtype=np.float64) for b in Bs]
+    Xs = [np.zeros_like(b) for b in Bs]
+    sysa = (_Sys * ns)(*[m._sys() for m in mats])
+    bp = (C.c_void_p * ns)(*[b.ctypes.data for b in Bs])
+    xp = (C.c_void_p * ns)(*[x.ctypes.data for x in Xs])
+    it = np.zeros(ns * nrhs, np.int32)
+    er = np.zeros(ns * nrhs, np.int32)
+    rs = np.zeros(ns * nrhs)
+    ia = np.ascontiguousarray(iarray, dtype=np.int32)
+    ra = np.ascontiguousarray(rarray, dtype=np.float64)
+    _bcheck(lib().hecmw33_solve_batch(ns, C.cast(sysa, C.c_void_p), nrhs, C.cast(bp, C.c_void_p),
+                                      C.cast(xp, C.c_void_p), _ptr(ia), _ptr(ra), _ptr(it), _ptr(er), _ptr(rs)))
+    sh = (ns, nrhs)
+    return Xs, it.reshape(sh), er.reshape(sh), rs.reshape(sh)
+
+
+def gen_tet10_batch(nsys=48, nrhs=16, dof_min=5000, dof_max=600000, seed=0, E=210000.0, nu=0.3):
+    """The batched workload: nsys TET10 boxes, sizes log-uniform in
+    [dof_min, dof_max] DOF, aspect ratios y/x and z/x uniform in [0.5, 2],
+    nrhs N(0,1) top-face load columns each (seeded).  Returns (mats, Bs, specs)."""
+    rng = np.random.default_rng(seed)
+    mats, Bs, specs = [], [], []
+    for s in range(nsys):
+        dof = float(np.exp(rng.uniform(np.log(dof_min), np.log(dof_max))))
+        ay, az = rng.uniform(0.5, 2.0, size=2)
+        # nodes = (2nx+1)(2ny+1)(2nz+1) ~ 8 nx^3 ay az
+        nx = max(1, int(round((dof / 3.0 / (8.0 * ay * az)) ** (1.0 / 3.0))))
+        ny, nz = max(1, int(round(nx * ay))), max(1, int(round(nx * az)))
+        m, B = gen_tet10(nx, ny, nz, box=(1.0, ny / nx, nz / nx), E=E, nu=nu, nrhs=nrhs, seed=seed * 1000 + s)
+        mats.append(m)
+        Bs.append(B)
+        specs.append((nx, ny, nz, 3 * m.N))
+    return mats, Bs, specs

