@@ -25,6 +25,9 @@ constexpr int kNumXcd = 8;      // MI355X: 8 XCDs, each with its own L2
 //    broadcasts, and a colour launch has four times as many wavefronts.
 constexpr int kRowsA = 64;
 constexpr int kRowsS = 16;
+// colours above this many rows on average sweep with a lane per row (SELL-64):
+// 4-5% faster at 245k rows per colour (160^3 cube), slower at 60k (100^3)
+constexpr int kSsorRows64Min = 150000;
 
 template <typename T>
 struct Sell {
@@ -41,6 +44,21 @@ struct Sell {
 // Device-resident scalars of one Krylov solve; kernels read them and the
 // one-workgroup tail kernels update them, so the host never waits inside an
 // iteration.
+struct KrylovState;
+
+// Multicolour SSOR operator (hecmw_precond_SSOR_33) as the sweeps see it.
+template <typename T>
+struct SsorOp {
+  Sell<T> L, U;                // L'/U' per colour slice; columns are slot ids
+  const int32_t* row_old;      // node (0-based, natural order) of every slot, -1 = padding
+  const T* alu;                // 3x3 LU factors, planes of L.rows rows
+  const int32_t* color_slice;  // host: first slice of every colour, ncolor+1 entries
+  int32_t ncolor;
+  T* zs;                       // sweep vector in slot order, 3 entries per slot
+};
+void ssor_apply(const SsorOp<double>& op, const double* r, double* z, const KrylovState* st, hipStream_t s);
+void ssor_apply_f32(const SsorOp<float>& op, const float* r, float* z, const KrylovState* st, hipStream_t s);
+
 struct KrylovState {
   double bnrm2, tol;
   double rho, rho1, alpha, beta, c1, dnrm2, resid;
@@ -84,14 +102,6 @@ void launch_spmv_f32(const Sell<float>& a, const float* x, float* y, hipStream_t
 void launch_matresid(const Sell<double>& a, const double* x, const double* b, double* r, hipStream_t s);
 void launch_dot_store(int64_t n, const double* x, const double* y, double* out, KrylovState* st, double* part,
                       hipStream_t s);
-void launch_ssor_fwd(const Sell<double>& L, const int32_t* row_old, const double* alu, int32_t s0, int32_t ns,
-                     const double* rin, double* z, const KrylovState* st, hipStream_t s);
-void launch_ssor_bwd(const Sell<double>& U, const int32_t* row_old, const double* alu, int32_t s0, int32_t ns,
-                     double* z, const KrylovState* st, hipStream_t s);
-void launch_ssor_fwd_f32(const Sell<float>& L, const int32_t* row_old, const float* alu, int32_t s0, int32_t ns,
-                         const float* rin, float* z, hipStream_t s);
-void launch_ssor_bwd_f32(const Sell<float>& U, const int32_t* row_old, const float* alu, int32_t s0, int32_t ns,
-                         float* z, hipStream_t s);
 void launch_diag_apply(int32_t n, const double* alu, const double* r, double* z, const KrylovState* st,
                        hipStream_t s);
 void launch_copy(int64_t n3, const double* r, double* z, const KrylovState* st, hipStream_t s);
@@ -100,15 +110,13 @@ void launch_add(int64_t n, const double* x, double* y, const KrylovState* st, hi
 void launch_matresid_guarded(const Sell<double>& a, const double* x, const double* b, double* r,
                              const KrylovState* st, hipStream_t s);
 // z = M^-1 r, kind 0: copy, 1/2: multicolour SSOR, 3: DIAG (guarded by st->done)
-void precond_apply(int32_t kind, const Sell<double>& L, const Sell<double>& U, const int32_t* row_old,
-                   const double* alu_ssor, const int32_t* color_slice, int32_t ncolor, const double* alu_diag,
-                   int32_t n, const double* r, double* z, const KrylovState* st, hipStream_t s);
+void precond_apply(int32_t kind, const SsorOp<double>& op, const double* alu_diag, int32_t n, const double* r,
+                   double* z, const KrylovState* st, hipStream_t s);
 
 // ---- fused CG iteration pieces (hecmw_solver_CG.f90:164-279) ----------------------
 // z = M^-1 r with rho = r.z and the rho/beta logic
-void cg_precond_ssor_rho(const Sell<double>& L, const Sell<double>& U, const int32_t* row_old, const double* alu,
-                         const int32_t* color_slice, int32_t ncolor, const double* r, double* z, KrylovState* st,
-                         double* part, hipStream_t s);
+void cg_precond_ssor_rho(const SsorOp<double>& op, const double* r, double* z, KrylovState* st, double* part,
+                         hipStream_t s);
 void cg_precond_diag_rho(int32_t n, const double* alu, const double* r, double* z, KrylovState* st, double* part,
                          hipStream_t s);
 void cg_precond_none_rho(int64_t n3, const double* r, double* z, KrylovState* st, double* part, hipStream_t s);
@@ -139,15 +147,12 @@ void bi_update(int64_t n3, const double* pt, const double* stv, const double* t,
 
 // ---- flexible CG with FP32 operators and FP64 recurrences ------------------------
 // z32 = M^-1 r32 in FP32 (kind 0 copy, 1 SSOR, 3 DIAG)
-void precond_apply_f32(int32_t kind, const Sell<float>& L, const Sell<float>& U, const int32_t* row_old,
-                       const float* alu_ssor, const int32_t* color_slice, int32_t ncolor, const float* alu_diag,
-                       int32_t n, const float* r, float* z, const KrylovState* st, hipStream_t s);
+void precond_apply_f32(int32_t kind, const SsorOp<float>& op, const float* alu_diag, int32_t n, const float* r,
+                       float* z, const KrylovState* st, hipStream_t s);
 // z32 = M^-1 r32 fused with the FP64 dots r.z and z.q_prev; beta = -(z.q_prev)/(p.q)_prev
 // (Notay's FCG(1))
-void fcg_precond_dots(int32_t kind, const Sell<float>& L, const Sell<float>& U, const int32_t* row_old,
-                      const float* alu_ssor, const int32_t* color_slice, int32_t ncolor, const float* alu_diag,
-                      int32_t n, const float* r, float* z, const float* q, KrylovState* st, double* part,
-                      hipStream_t s);
+void fcg_precond_dots(int32_t kind, const SsorOp<float>& op, const float* alu_diag, int32_t n, const float* r,
+                      float* z, const float* q, KrylovState* st, double* part, hipStream_t s);
 void fcg_p(int64_t n3, const float* z, float* p, const KrylovState* st, hipStream_t s);
 // q = A32 p, pq = p.q, rp = r.p, alpha = rp / pq
 void fcg_spmv(const Sell<float>& a, const float* p, const float* r, float* q, KrylovState* st, double* part,

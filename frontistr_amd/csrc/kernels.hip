@@ -603,12 +603,17 @@ __device__ __forceinline__ void ssor_chunk(const int32_t* __restrict__ colp, con
   }
 }
 
+// The sweeps work on zs, the preconditioned vector in slot order (3 entries
+// per slot; L'/U' columns hold slot ids), so the rows of a slice, which are
+// consecutive in colour order, gather neighbours that are consecutive too.
+// r comes in and z goes out in the caller's (natural) node order.
+//
 // Forward, rows of one colour independent:
-//   z[i] = LU^-1 (r[i] - sum_{j in L'(i), ascending} L'_ij z[j])
+//   zs[i] = LU^-1 (r[old(i)] - sum_{j in L'(i), ascending} L'_ij zs[j])
 template <typename T, int CH>
 __global__ __launch_bounds__(kBlock) void k_ssor_fwd(Sell<T> L, const int32_t* __restrict__ row_old,
                                                        const T* __restrict__ alu, int s0, int ns,
-                                                       const T* __restrict__ rin, T* __restrict__ z,
+                                                       const T* __restrict__ rin, T* __restrict__ zs,
                                                        const KrylovState* st) {
 #pragma clang fp contract(off)
   if (st && st->done) return;
@@ -626,23 +631,23 @@ __global__ __launch_bounds__(kBlock) void k_ssor_fwd(Sell<T> L, const int32_t* _
     const int32_t* __restrict__ colp = L.col + (int64_t)base * kRowsS + r;
     const T* __restrict__ vp = L.val + ((int64_t)base * 9 + 3 * dd) * kRowsS + r;
     T sw = rin[3 * (int64_t)iold + dd];
-    for (int k0 = 0; k0 < len; k0 += CH) ssor_chunk<T, CH, false, -1>(colp, vp, z, k0, len, dd, sw);
+    for (int k0 = 0; k0 < len; k0 += CH) ssor_chunk<T, CH, false, -1>(colp, vp, zs, k0, len, dd, sw);
     T x0 = quad_bcast<0>(sw), x1 = quad_bcast<1>(sw), x2 = quad_bcast<2>(sw);
     lu3_solve<T>(a, x0, x1, x2);
-    if (d < 3) z[3 * (int64_t)iold + d] = pick3(d, x0, x1, x2);
+    if (d < 3) zs[3 * slot + d] = pick3(d, x0, x1, x2);
   }
 }
 
 // Backward, colours in reverse:
-//   z[i] -= LU^-1 sum_{j in U'(i), descending} U'_ij z[j]
+//   zs[i] -= LU^-1 sum_{j in U'(i), descending} U'_ij zs[j];  z[old(i)] = zs[i]
 // ND fused dots over this colour's rows (final now): w0.z (and w1.z) go to
 // part[k*kMaxPartials + pbase + block], accumulated in FP64.
 template <typename T, int CH, int ND>
 __global__ __launch_bounds__(kBlock) void k_ssor_bwd(Sell<T> U, const int32_t* __restrict__ row_old,
                                                        const T* __restrict__ alu, int s0, int ns,
-                                                       T* __restrict__ z, const T* __restrict__ w0,
-                                                       const T* __restrict__ w1, const KrylovState* st, double* part,
-                                                       int pbase) {
+                                                       T* __restrict__ zs, T* __restrict__ zout,
+                                                       const T* __restrict__ w0, const T* __restrict__ w1,
+                                                       const KrylovState* st, double* part, int pbase) {
 #pragma clang fp contract(off)
   if (st && st->done) return;
   const int lane = lane_id(), r = lane >> 2, d = lane & 3, dd = d < 3 ? d : 2;
@@ -660,16 +665,130 @@ __global__ __launch_bounds__(kBlock) void k_ssor_bwd(Sell<T> U, const int32_t* _
     if (iold < 0) continue;
     const int32_t* __restrict__ colp = U.col + (int64_t)base * kRowsS + r;
     const T* __restrict__ vp = U.val + ((int64_t)base * 9 + 3 * dd) * kRowsS + r;
-    const T zo = z[3 * (int64_t)iold + dd];
+    const T zo = zs[3 * slot + dd];
     T sw = 0;
-    for (int top = len - 1; top >= 0; top -= CH) ssor_chunk<T, CH, true, 1>(colp, vp, z, top, len, dd, sw);
+    for (int top = len - 1; top >= 0; top -= CH) ssor_chunk<T, CH, true, 1>(colp, vp, zs, top, len, dd, sw);
     T x0 = quad_bcast<0>(sw), x1 = quad_bcast<1>(sw), x2 = quad_bcast<2>(sw);
     lu3_solve<T>(a, x0, x1, x2);
     const T zn = zo - pick3(dd, x0, x1, x2);
     if (d < 3) {
-      z[3 * (int64_t)iold + d] = zn;
-      if constexpr (ND > 0) acc[0] += (double)w0[3 * (int64_t)iold + d] * (double)zn;
-      if constexpr (ND > 1) acc[KD - 1] += (double)w1[3 * (int64_t)iold + d] * (double)zn;
+      zs[3 * slot + d] = zn;
+      const int64_t o = 3 * (int64_t)iold + d;
+      zout[o] = zn;
+      if constexpr (ND > 0) acc[0] += (double)w0[o] * (double)zn;
+      if constexpr (ND > 1) acc[KD - 1] += (double)w1[o] * (double)zn;
+    }
+  }
+  if constexpr (ND > 0) block_partials<KD>(acc, part, pbase + blockIdx.x);
+}
+
+// ---- SSOR sweeps, SELL-64 layout: one lane per row ------------------------------
+// For large colours (many more slices than resident waves) a lane per row
+// needs a quarter of the waves of the four-lane layout for the same dependent
+// load chains.  Same arithmetic per row and component as ssor_chunk + the
+// quad layout: the reference's statements in the reference's order.
+template <typename T, int CH, bool DESC, int SIGN>
+__device__ __forceinline__ void ssor64_chunk(const int32_t* __restrict__ colp, const T* __restrict__ vp,
+                                             const T* __restrict__ z, int k0, int len, T (&sw)[3]) {
+#pragma clang fp contract(off)
+  int kk[CH], c[CH];
+#pragma unroll
+  for (int j = 0; j < CH; ++j) {
+    const int k = DESC ? k0 - j : k0 + j;
+    kk[j] = DESC ? (k < 0 ? 0 : k) : (k < len ? k : len - 1);
+    c[j] = colp[(int64_t)kk[j] * kRowsA];
+  }
+  T v[CH][9], x[CH][3];
+#pragma unroll
+  for (int j = 0; j < CH; ++j) {
+    const T* __restrict__ vk = vp + (int64_t)kk[j] * 9 * kRowsA;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) v[j][e] = vk[e * kRowsA];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) x[j][d] = z[3 * (int64_t)c[j] + d];
+  }
+#pragma unroll
+  for (int j = 0; j < CH; ++j) {
+    const int k = DESC ? k0 - j : k0 + j;
+    const bool live = DESC ? k >= 0 : k < len;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const T t = SIGN < 0 ? sw[d] - v[j][3 * d] * x[j][0] - v[j][3 * d + 1] * x[j][1] - v[j][3 * d + 2] * x[j][2]
+                           : sw[d] + v[j][3 * d] * x[j][0] + v[j][3 * d + 1] * x[j][1] + v[j][3 * d + 2] * x[j][2];
+      sw[d] = live ? t : sw[d];
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void load_alu64(const T* __restrict__ alu, int s, int lane, T (&a)[9]) {
+  const T* __restrict__ f = alu + (int64_t)s * 9 * kRowsA + lane;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) a[e] = f[e * kRowsA];
+}
+
+template <typename T, int CH>
+__global__ __launch_bounds__(kBlock) void k_ssor64_fwd(Sell<T> L, const int32_t* __restrict__ row_old,
+                                                         const T* __restrict__ alu, int s0, int ns,
+                                                         const T* __restrict__ rin, T* __restrict__ zs,
+                                                         const KrylovState* st) {
+#pragma clang fp contract(off)
+  if (st && st->done) return;
+  const int lane = lane_id();
+  const SliceRange sr = wave_slices(ns);
+  for (int si = sr.begin; si < sr.end; ++si) {
+    const int s = s0 + si;
+    const int64_t slot = (int64_t)s * kRowsA + lane;
+    const int iold = row_old[slot];
+    const int len = L.len[slot];
+    const int base = L.ptr[s];
+    T a[9];
+    load_alu64<T>(alu, s, lane, a);
+    if (iold < 0) continue;
+    const int32_t* __restrict__ colp = L.col + (int64_t)base * kRowsA + lane;
+    const T* __restrict__ vp = L.val + (int64_t)base * 9 * kRowsA + lane;
+    T sw[3] = {rin[3 * (int64_t)iold], rin[3 * (int64_t)iold + 1], rin[3 * (int64_t)iold + 2]};
+    for (int k0 = 0; k0 < len; k0 += CH) ssor64_chunk<T, CH, false, -1>(colp, vp, zs, k0, len, sw);
+    lu3_solve<T>(a, sw[0], sw[1], sw[2]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) zs[3 * slot + d] = sw[d];
+  }
+}
+
+template <typename T, int CH, int ND>
+__global__ __launch_bounds__(kBlock) void k_ssor64_bwd(Sell<T> U, const int32_t* __restrict__ row_old,
+                                                         const T* __restrict__ alu, int s0, int ns,
+                                                         T* __restrict__ zs, T* __restrict__ zout,
+                                                         const T* __restrict__ w0, const T* __restrict__ w1,
+                                                         const KrylovState* st, double* part, int pbase) {
+#pragma clang fp contract(off)
+  if (st && st->done) return;
+  const int lane = lane_id();
+  const SliceRange sr = wave_slices(ns);
+  constexpr int KD = ND > 0 ? ND : 1;
+  double acc[KD] = {};
+  for (int si = sr.begin; si < sr.end; ++si) {
+    const int s = s0 + si;
+    const int64_t slot = (int64_t)s * kRowsA + lane;
+    const int iold = row_old[slot];
+    const int len = U.len[slot];
+    const int base = U.ptr[s];
+    T a[9];
+    load_alu64<T>(alu, s, lane, a);
+    if (iold < 0) continue;
+    const int32_t* __restrict__ colp = U.col + (int64_t)base * kRowsA + lane;
+    const T* __restrict__ vp = U.val + (int64_t)base * 9 * kRowsA + lane;
+    T sw[3] = {0, 0, 0};
+    for (int top = len - 1; top >= 0; top -= CH) ssor64_chunk<T, CH, true, 1>(colp, vp, zs, top, len, sw);
+    lu3_solve<T>(a, sw[0], sw[1], sw[2]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const T zn = zs[3 * slot + d] - sw[d];
+      zs[3 * slot + d] = zn;
+      const int64_t o = 3 * (int64_t)iold + d;
+      zout[o] = zn;
+      if constexpr (ND > 0) acc[0] += (double)w0[o] * (double)zn;
+      if constexpr (ND > 1) acc[KD - 1] += (double)w1[o] * (double)zn;
     }
   }
   if constexpr (ND > 0) block_partials<KD>(acc, part, pbase + blockIdx.x);
@@ -1017,44 +1136,75 @@ static int ssor_chunk_sel() {
 }
 
 template <typename T>
-static void ssor_fwd_any(const Sell<T>& L, const int32_t* row_old, const T* alu, int32_t s0, int32_t ns, const T* rin,
-                         T* z, const KrylovState* st, hipStream_t s) {
+static void ssor_fwd_any(const SsorOp<T>& op, int c, const T* rin, const KrylovState* st, hipStream_t s) {
+  const int32_t s0 = op.color_slice[c], ns = op.color_slice[c + 1] - s0;
   if (ns <= 0) return;
   const int g = spmv_grid(ns);
+  if (op.L.rows == kRowsA) {
+    k_ssor64_fwd<T, 4><<<g, kBlock, 0, s>>>(op.L, op.row_old, op.alu, s0, ns, rin, op.zs, st);
+    return;
+  }
   switch (ssor_chunk_sel()) {
-    case 4: k_ssor_fwd<T, 4><<<g, kBlock, 0, s>>>(L, row_old, alu, s0, ns, rin, z, st); break;
-    case 16: k_ssor_fwd<T, 16><<<g, kBlock, 0, s>>>(L, row_old, alu, s0, ns, rin, z, st); break;
-    default: k_ssor_fwd<T, 8><<<g, kBlock, 0, s>>>(L, row_old, alu, s0, ns, rin, z, st); break;
+    case 4: k_ssor_fwd<T, 4><<<g, kBlock, 0, s>>>(op.L, op.row_old, op.alu, s0, ns, rin, op.zs, st); break;
+    case 16: k_ssor_fwd<T, 16><<<g, kBlock, 0, s>>>(op.L, op.row_old, op.alu, s0, ns, rin, op.zs, st); break;
+    default: k_ssor_fwd<T, 8><<<g, kBlock, 0, s>>>(op.L, op.row_old, op.alu, s0, ns, rin, op.zs, st); break;
   }
 }
 
 template <typename T, int ND>
-static void ssor_bwd_any(const Sell<T>& U, const int32_t* row_old, const T* alu, int32_t s0, int32_t ns, T* z,
-                         const T* w0, const T* w1, const KrylovState* st, double* part, int pbase, int g,
-                         hipStream_t s) {
+static void ssor_bwd_any(const SsorOp<T>& op, int c, T* z, const T* w0, const T* w1, const KrylovState* st,
+                         double* part, int pbase, int g, hipStream_t s) {
+  const int32_t s0 = op.color_slice[c], ns = op.color_slice[c + 1] - s0;
   if (ns <= 0) return;
+  if (op.U.rows == kRowsA) {
+    k_ssor64_bwd<T, 4, ND><<<g, kBlock, 0, s>>>(op.U, op.row_old, op.alu, s0, ns, op.zs, z, w0, w1, st, part, pbase);
+    return;
+  }
   switch (ssor_chunk_sel()) {
-    case 4: k_ssor_bwd<T, 4, ND><<<g, kBlock, 0, s>>>(U, row_old, alu, s0, ns, z, w0, w1, st, part, pbase); break;
-    case 16: k_ssor_bwd<T, 16, ND><<<g, kBlock, 0, s>>>(U, row_old, alu, s0, ns, z, w0, w1, st, part, pbase); break;
-    default: k_ssor_bwd<T, 8, ND><<<g, kBlock, 0, s>>>(U, row_old, alu, s0, ns, z, w0, w1, st, part, pbase); break;
+    case 4:
+      k_ssor_bwd<T, 4, ND><<<g, kBlock, 0, s>>>(op.U, op.row_old, op.alu, s0, ns, op.zs, z, w0, w1, st, part, pbase);
+      break;
+    case 16:
+      k_ssor_bwd<T, 16, ND><<<g, kBlock, 0, s>>>(op.U, op.row_old, op.alu, s0, ns, op.zs, z, w0, w1, st, part, pbase);
+      break;
+    default:
+      k_ssor_bwd<T, 8, ND><<<g, kBlock, 0, s>>>(op.U, op.row_old, op.alu, s0, ns, op.zs, z, w0, w1, st, part, pbase);
+      break;
   }
 }
 
-void launch_ssor_fwd(const Sell<double>& L, const int32_t* row_old, const double* alu, int32_t s0, int32_t ns,
-                     const double* rin, double* z, const KrylovState* st, hipStream_t s) {
-  ssor_fwd_any<double>(L, row_old, alu, s0, ns, rin, z, st, s);
+// z = M^-1 r, every colour forward then backward (z in node order)
+template <typename T>
+static void ssor_apply_any(const SsorOp<T>& op, const T* r, T* z, const KrylovState* st, hipStream_t s) {
+  for (int c = 0; c < op.ncolor; ++c) ssor_fwd_any<T>(op, c, r, st, s);
+  for (int c = op.ncolor - 1; c >= 0; --c) {
+    const int ns = op.color_slice[c + 1] - op.color_slice[c];
+    ssor_bwd_any<T, 0>(op, c, z, nullptr, nullptr, st, nullptr, 0, spmv_grid(ns), s);
+  }
 }
-void launch_ssor_bwd(const Sell<double>& U, const int32_t* row_old, const double* alu, int32_t s0, int32_t ns,
-                     double* z, const KrylovState* st, hipStream_t s) {
-  ssor_bwd_any<double, 0>(U, row_old, alu, s0, ns, z, nullptr, nullptr, st, nullptr, 0, spmv_grid(ns), s);
+
+// fused dots: colours' partials in launch order, grids capped to fit the buffer
+template <typename T, int ND>
+static int ssor_apply_dots(const SsorOp<T>& op, const T* r, T* z, const T* w1, const KrylovState* st, double* part,
+                           hipStream_t s) {
+  for (int c = 0; c < op.ncolor; ++c) ssor_fwd_any<T>(op, c, r, st, s);
+  const int gcap = std::max(1, kMaxPartials / std::max(1, op.ncolor));
+  int pbase = 0;
+  for (int c = op.ncolor - 1; c >= 0; --c) {
+    const int ns = op.color_slice[c + 1] - op.color_slice[c];
+    if (ns <= 0) continue;
+    const int g = std::min(gcap, spmv_grid(ns));
+    ssor_bwd_any<T, ND>(op, c, z, r, w1, st, part, pbase, g, s);
+    pbase += g;
+  }
+  return pbase;
 }
-void launch_ssor_fwd_f32(const Sell<float>& L, const int32_t* row_old, const float* alu, int32_t s0, int32_t ns,
-                         const float* rin, float* z, hipStream_t s) {
-  ssor_fwd_any<float>(L, row_old, alu, s0, ns, rin, z, nullptr, s);
+
+void ssor_apply(const SsorOp<double>& op, const double* r, double* z, const KrylovState* st, hipStream_t s) {
+  ssor_apply_any<double>(op, r, z, st, s);
 }
-void launch_ssor_bwd_f32(const Sell<float>& U, const int32_t* row_old, const float* alu, int32_t s0, int32_t ns,
-                         float* z, hipStream_t s) {
-  ssor_bwd_any<float, 0>(U, row_old, alu, s0, ns, z, nullptr, nullptr, nullptr, nullptr, 0, spmv_grid(ns), s);
+void ssor_apply_f32(const SsorOp<float>& op, const float* r, float* z, const KrylovState* st, hipStream_t s) {
+  ssor_apply_any<float>(op, r, z, st, s);
 }
 void launch_diag_apply(int32_t n, const double* alu, const double* r, double* z, const KrylovState* st,
                        hipStream_t s) {
@@ -1075,22 +1225,10 @@ void launch_matresid_guarded(const Sell<double>& a, const double* x, const doubl
 }
 
 // ---- CG iteration pieces ------------------------------------------------------------
-void cg_precond_ssor_rho(const Sell<double>& L, const Sell<double>& U, const int32_t* row_old, const double* alu,
-                         const int32_t* color_slice, int32_t ncolor, const double* r, double* z, KrylovState* st,
-                         double* part, hipStream_t s) {
-  for (int c = 0; c < ncolor; ++c)
-    launch_ssor_fwd(L, row_old, alu, color_slice[c], color_slice[c + 1] - color_slice[c], r, z, st, s);
-  // every colour's partials in launch order; grids capped to fit the buffer
-  const int gcap = std::max(1, kMaxPartials / std::max(1, ncolor));
-  int pbase = 0;
-  for (int c = ncolor - 1; c >= 0; --c) {
-    const int ns = color_slice[c + 1] - color_slice[c];
-    if (ns <= 0) continue;
-    const int g = std::min(gcap, spmv_grid(ns));
-    ssor_bwd_any<double, 1>(U, row_old, alu, color_slice[c], ns, z, r, nullptr, st, part, pbase, g, s);
-    pbase += g;
-  }
-  k_tail<kTailCgRho, 1><<<1, kTailThreads, 0, s>>>(st, part, pbase, nullptr, nullptr);
+void cg_precond_ssor_rho(const SsorOp<double>& op, const double* r, double* z, KrylovState* st, double* part,
+                         hipStream_t s) {
+  const int nparts = ssor_apply_dots<double, 1>(op, r, z, nullptr, st, part, s);
+  k_tail<kTailCgRho, 1><<<1, kTailThreads, 0, s>>>(st, part, nparts, nullptr, nullptr);
 }
 void cg_precond_diag_rho(int32_t n, const double* alu, const double* r, double* z, KrylovState* st, double* part,
                          hipStream_t s) {
@@ -1162,18 +1300,14 @@ void bi_update(int64_t n3, const double* pt, const double* stv, const double* t,
   k_tail<kTailDnrm2, 1><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, hist);
 }
 // generic preconditioner application z = M^-1 r for BiCGSTAB (no fused dot)
-void precond_apply(int32_t kind, const Sell<double>& L, const Sell<double>& U, const int32_t* row_old,
-                   const double* alu_ssor, const int32_t* color_slice, int32_t ncolor, const double* alu_diag,
-                   int32_t n, const double* r, double* z, const KrylovState* st, hipStream_t s) {
+void precond_apply(int32_t kind, const SsorOp<double>& op, const double* alu_diag, int32_t n, const double* r,
+                   double* z, const KrylovState* st, hipStream_t s) {
   if (kind == 0) {
     launch_copy(3 * (int64_t)n, r, z, st, s);
   } else if (kind == 3) {
     launch_diag_apply(n, alu_diag, r, z, st, s);
   } else {
-    for (int c = 0; c < ncolor; ++c)
-      launch_ssor_fwd(L, row_old, alu_ssor, color_slice[c], color_slice[c + 1] - color_slice[c], r, z, st, s);
-    for (int c = ncolor - 1; c >= 0; --c)
-      launch_ssor_bwd(U, row_old, alu_ssor, color_slice[c], color_slice[c + 1] - color_slice[c], z, st, s);
+    ssor_apply(op, r, z, st, s);
   }
 }
 
@@ -1201,27 +1335,18 @@ void launch_to_f32(int64_t n, const double* x, float* y, hipStream_t s) {
   g = g < 1 ? 1 : (g > 8192 ? 8192 : g);
   k_to_f32<<<(int)g, kBlock, 0, s>>>(n, x, y);
 }
-void precond_apply_f32(int32_t kind, const Sell<float>& L, const Sell<float>& U, const int32_t* row_old,
-                       const float* alu_ssor, const int32_t* color_slice, int32_t ncolor, const float* alu_diag,
-                       int32_t n, const float* r, float* z, const KrylovState* st, hipStream_t s) {
+void precond_apply_f32(int32_t kind, const SsorOp<float>& op, const float* alu_diag, int32_t n, const float* r,
+                       float* z, const KrylovState* st, hipStream_t s) {
   if (kind == 0) {
     k_copy_dots_f32<<<vec_grid(3 * (int64_t)n), kBlock, 0, s>>>(3 * (int64_t)n, r, z, nullptr, st, nullptr, 0);
   } else if (kind == 3) {
     k_diag<float, 0><<<vec_grid(n), kBlock, 0, s>>>(n, alu_diag, r, z, nullptr, st, nullptr);
   } else {
-    for (int c = 0; c < ncolor; ++c)
-      ssor_fwd_any<float>(L, row_old, alu_ssor, color_slice[c], color_slice[c + 1] - color_slice[c], r, z, st, s);
-    for (int c = ncolor - 1; c >= 0; --c) {
-      const int ns = color_slice[c + 1] - color_slice[c];
-      ssor_bwd_any<float, 0>(U, row_old, alu_ssor, color_slice[c], ns, z, nullptr, nullptr, st, nullptr, 0,
-                             spmv_grid(ns), s);
-    }
+    ssor_apply_f32(op, r, z, st, s);
   }
 }
-void fcg_precond_dots(int32_t kind, const Sell<float>& L, const Sell<float>& U, const int32_t* row_old,
-                      const float* alu_ssor, const int32_t* color_slice, int32_t ncolor, const float* alu_diag,
-                      int32_t n, const float* r, float* z, const float* q, KrylovState* st, double* part,
-                      hipStream_t s) {
+void fcg_precond_dots(int32_t kind, const SsorOp<float>& op, const float* alu_diag, int32_t n, const float* r,
+                      float* z, const float* q, KrylovState* st, double* part, hipStream_t s) {
   int nparts = 0;
   if (kind == 0) {
     nparts = vec_grid(3 * (int64_t)n);
@@ -1230,16 +1355,7 @@ void fcg_precond_dots(int32_t kind, const Sell<float>& L, const Sell<float>& U, 
     nparts = vec_grid(n);
     k_diag<float, 2><<<nparts, kBlock, 0, s>>>(n, alu_diag, r, z, q, st, part);
   } else {
-    for (int c = 0; c < ncolor; ++c)
-      ssor_fwd_any<float>(L, row_old, alu_ssor, color_slice[c], color_slice[c + 1] - color_slice[c], r, z, st, s);
-    const int gcap = std::max(1, kMaxPartials / std::max(1, ncolor));
-    for (int c = ncolor - 1; c >= 0; --c) {
-      const int ns = color_slice[c + 1] - color_slice[c];
-      if (ns <= 0) continue;
-      const int g = std::min(gcap, spmv_grid(ns));
-      ssor_bwd_any<float, 2>(U, row_old, alu_ssor, color_slice[c], ns, z, r, q, st, part, nparts, g, s);
-      nparts += g;
-    }
+    nparts = ssor_apply_dots<float, 2>(op, r, z, q, st, part, s);
   }
   k_tail<kTailFcgBeta, 2><<<1, kTailThreads, 0, s>>>(st, part, nparts, nullptr, nullptr);
 }

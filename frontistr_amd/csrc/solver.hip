@@ -6,10 +6,13 @@
 //            then L then U in the caller's storage order (the reference CPU
 //            SpMV order, hecmw_solver_las_33.F90:281-308)
 //   raw    : the caller's D/AL/AU uploaded as-is (source of every value fill)
-//   L', U' : SSOR factors in multicolour order, SELL-64 slices never crossing
-//            a colour; columns hold old (natural) node ids, as after
-//            hecmw_matrix_reorder_renum_item, so the sweeps gather z in the
-//            natural order like the reference (hecmw_precond_SSOR_33.F90:151-155)
+//   L', U' : SSOR factors in multicolour order, SELL-16 (four lanes per row)
+//            or SELL-64 (a lane per row) slices never crossing a colour; the
+//            sweeps keep their vector in slot order and the columns hold slot
+//            ids, so rows that are consecutive in colour order gather
+//            neighbours that are consecutive too (the reference maps items
+//            back to old ids, hecmw_precond_SSOR_33.F90:151-155, and sweeps
+//            ZP in natural order; the arithmetic is the same)
 //   ALU    : 3x3 LU of the diagonal blocks, 9-plane slice layout
 //   vectors: b, x, r, z, p, q (+ BiCGSTAB's) of 3*NP doubles
 #include <hip/hip_runtime.h>
@@ -22,6 +25,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/hecmw33.h"
@@ -180,10 +184,14 @@ struct Context {
   std::vector<int32_t> h_indexL, h_indexU, h_itemL, h_itemU;
   // SSOR
   bool ssor_init = false;
+  bool ssor_sym = false;        // ordering and L'/U' structure built
+  int32_t ssor_ncolor_in = -1;  // NCOLOR_IN they were built with
+  int64_t ssor_rows = 0;        // row slots of the SSOR slices
+  int32_t ssor_R = kRowsS;      // rows per SSOR slice (16 or 64)
   int32_t ncolor = 0;
   std::vector<int32_t> perm, iperm, colorindex, color_slice;
   SellDev L, U;
-  DMem row_old, alu_ssor;
+  DMem row_old, alu_ssor, vzs, vzs32;
   // DIAG
   bool diag_init = false;
   DMem row_diag, alu_diag;
@@ -211,9 +219,18 @@ struct Context {
     return events[i];
   }
 
+  SsorOp<double> ssor_op() {
+    return SsorOp<double>{L.view, U.view, row_old.as<int32_t>(), alu_ssor.as<double>(), color_slice.data(), ncolor,
+                          vzs.as<double>()};
+  }
+  SsorOp<float> ssor_op32() {
+    return SsorOp<float>{L.view32, U.view32, row_old.as<int32_t>(), alu_ssor32.as<float>(), color_slice.data(),
+                         ncolor, vzs32.as<float>()};
+  }
+
   ~Context() {
     if (stream) hipStreamSynchronize(stream);
-    for (DMem* m : {&dIndexL, &dIndexU, &dItemL, &dItemU, &vscale}) m->release();
+    for (DMem* m : {&dIndexL, &dIndexU, &dItemL, &dItemU, &vscale, &vzs, &vzs32}) m->release();
     for (DMem* m : {&dD, &dAL, &dAU, &row_old, &alu_ssor, &row_diag, &alu_diag, &alu_ssor32, &alu_diag32, &vr32,
                     &vz32, &vp32, &vq32, &vb, &vx, &vr, &vz, &vp, &vq, &vrt, &vs, &vt, &vv, &vpt, &vst, &vzp, &vzt, &st,
                     &part, &hist})
@@ -305,6 +322,7 @@ static int32_t load_structure(Context& c, const hecmw33_sys* A, bool force) {
   c.key[3] = A->itemU;
   c.have_struct = true;
   c.ssor_init = false;
+  c.ssor_sym = false;
   c.diag_init = false;
   c.dindex_valid = false;
   const size_t vb = sizeof(double) * 3 * (size_t)NP;
@@ -329,7 +347,9 @@ static int32_t load_values(Context& c, const hecmw33_sys* A) {
 }
 
 // ---- preconditioners ----------------------------------------------------------
-static int32_t setup_ssor(Context& c, double sigma, int32_t ncolor_in) {
+// Symbolic part: ordering, L'/U' structure, lane assignment.  Depends on the
+// structure and NCOLOR_IN only.
+static int32_t setup_ssor_symbolic(Context& c, int32_t ncolor_in) {
   const int32_t N = c.N;
   const int32_t* iL = c.h_indexL.data();
   const int32_t* iU = c.h_indexU.data();
@@ -344,14 +364,21 @@ static int32_t setup_ssor(Context& c, double sigma, int32_t ncolor_in) {
                          c.iperm.data());
   std::vector<int32_t> iLp, tLp, iUp, tUp;
   reorder_profile(N, c.perm.data(), c.iperm.data(), iL, tL, iU, tU, iLp, tLp, iUp, tUp);
+  // Rows per slice: 16 (four lanes per row) or 64 (a lane per row).  The
+  // lane-per-row sweep needs a quarter of the waves, which pays once a colour
+  // holds many more slices than the GPU keeps resident; HECMW33_SSOR_ROWS
+  // (16 or 64) overrides.
+  int R = (int64_t)N / std::max(1, c.ncolor) > kSsorRows64Min ? kRowsA : kRowsS;
+  if (const char* e = getenv("HECMW33_SSOR_ROWS")) R = atoi(e) == 64 ? kRowsA : (atoi(e) == 16 ? kRowsS : R);
+  c.ssor_R = R;
   // slices per colour
   c.color_slice.assign(c.ncolor + 1, 0);
   for (int32_t k = 0; k < c.ncolor; ++k) {
     const int32_t cnt = c.colorindex[k + 1] - c.colorindex[k];
-    c.color_slice[k + 1] = c.color_slice[k] + (cnt + kRowsS - 1) / kRowsS;
+    c.color_slice[k + 1] = c.color_slice[k] + (cnt + R - 1) / R;
   }
   const int32_t ns = c.color_slice[c.ncolor];
-  std::vector<int32_t> lanes((int64_t)ns * kRowsS, -1), row_old((int64_t)ns * kRowsS, -1);
+  std::vector<int32_t> lanes((int64_t)ns * R, -1), row_old((int64_t)ns * R, -1);
   // Rows of one colour are independent, so the lane that sweeps a row is
   // free; optionally (HECMW33_SSOR_SORT=1) a colour's rows are dealt in
   // natural node order so neighbouring lanes gather neighbouring nodes.
@@ -364,11 +391,15 @@ static int32_t setup_ssor(Context& c, double sigma, int32_t ncolor_in) {
     if (sort_rows) std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return c.perm[a] < c.perm[b]; });
     for (int32_t q = 0; q < (int32_t)order.size(); ++q) {
       const int32_t inew = order[q];
-      const int64_t t = (int64_t)c.color_slice[k] * kRowsS + q;
+      const int64_t t = (int64_t)c.color_slice[k] * R + q;
       lanes[t] = inew;                    // 0-based new position
       row_old[t] = c.perm[inew] - 1;      // 0-based old id
     }
   }
+  // slot of every row (new order): the sweeps' columns
+  std::vector<int32_t> slot_of(N, -1);
+  for (int64_t t = 0; t < (int64_t)lanes.size(); ++t)
+    if (lanes[t] >= 0) slot_of[lanes[t]] = (int32_t)t;
   const int64_t offL = N, offU = (int64_t)N + c.NPL;
   const std::vector<int32_t>& perm = c.perm;
   // source block of A(iold, kold) inside the caller's AL/AU (row iold)
@@ -383,40 +414,65 @@ static int32_t setup_ssor(Context& c, double sigma, int32_t ncolor_in) {
     return -1;
   };
   SellHost hl, hu;
-  build_sell<kRowsS>(
-      lanes, N, [&](int32_t inew) { return iLp[inew + 1] - iLp[inew]; },
-      [&](int32_t inew, auto emit) {
-        const int32_t iold1 = perm[inew];
-        int32_t k = 0;
-        for (int32_t q = iLp[inew]; q < iLp[inew + 1]; ++q) {
-          const int32_t kold1 = perm[tLp[q] - 1];
-          emit(k++, kold1 - 1, find_src(iold1, kold1));
-        }
-      },
-      hl);
-  build_sell<kRowsS>(
-      lanes, N, [&](int32_t inew) { return iUp[inew + 1] - iUp[inew]; },
-      [&](int32_t inew, auto emit) {
-        const int32_t iold1 = perm[inew];
-        int32_t k = 0;
-        for (int32_t q = iUp[inew]; q < iUp[inew + 1]; ++q) {
-          const int32_t kold1 = perm[tUp[q] - 1];
-          emit(k++, kold1 - 1, find_src(iold1, kold1));
-        }
-      },
-      hu);
+  auto build_lu = [&](auto rows_tag) {
+    constexpr int RR = decltype(rows_tag)::value;
+    build_sell<RR>(
+        lanes, N, [&](int32_t inew) { return iLp[inew + 1] - iLp[inew]; },
+        [&](int32_t inew, auto emit) {
+          const int32_t iold1 = perm[inew];
+          int32_t k = 0;
+          for (int32_t q = iLp[inew]; q < iLp[inew + 1]; ++q) {
+            const int32_t kold1 = perm[tLp[q] - 1];
+            emit(k++, slot_of[tLp[q] - 1], find_src(iold1, kold1));
+          }
+        },
+        hl);
+    build_sell<RR>(
+        lanes, N, [&](int32_t inew) { return iUp[inew + 1] - iUp[inew]; },
+        [&](int32_t inew, auto emit) {
+          const int32_t iold1 = perm[inew];
+          int32_t k = 0;
+          for (int32_t q = iUp[inew]; q < iUp[inew + 1]; ++q) {
+            const int32_t kold1 = perm[tUp[q] - 1];
+            emit(k++, slot_of[tUp[q] - 1], find_src(iold1, kold1));
+          }
+        },
+        hu);
+  };
+  if (R == kRowsA)
+    build_lu(std::integral_constant<int, kRowsA>{});
+  else
+    build_lu(std::integral_constant<int, kRowsS>{});
   if (int32_t e = upload_sell(hl, c.L, c.stream)) return e;
   if (int32_t e = upload_sell(hu, c.U, c.stream)) return e;
   if (c.row_old.ensure(sizeof(int32_t) * row_old.size()) || c.alu_ssor.ensure(sizeof(double) * 9 * row_old.size()))
     return HECMW33_ERROR_DEVICE;
   HIPC(hipMemcpyAsync(c.row_old.p, row_old.data(), sizeof(int32_t) * row_old.size(), hipMemcpyHostToDevice,
                       c.stream));
+  if (c.vzs.ensure(sizeof(double) * 3 * row_old.size())) return HECMW33_ERROR_DEVICE;
+  HIPC(hipMemsetAsync(c.vzs.p, 0, sizeof(double) * 3 * row_old.size(), c.stream));
+  HIPC(hipStreamSynchronize(c.stream));
+  c.ssor_rows = (int64_t)row_old.size();
+  c.ssor_ncolor_in = ncolor_in;
+  c.ssor_sym = true;
+  return 0;
+}
+
+// hecmw_precond_SSOR_33_setup (hecmw_precond_SSOR_33.F90:57-218).  The
+// reference redoes the ordering also when only a numeric setup is asked for
+// (Iarray 97 = 1, 98 = 0: ":75-79, numerical setup only ... TEMPORARY"); the
+// ordering depends on the structure and NCOLOR_IN alone, so it is kept here
+// and only the values and factors are refreshed, with the same result.
+static int32_t setup_ssor(Context& c, double sigma, int32_t ncolor_in, bool symbolic) {
+  if (symbolic || !c.ssor_sym || c.ssor_ncolor_in != ncolor_in)
+    if (int32_t e = setup_ssor_symbolic(c, ncolor_in)) return e;
+  const int32_t N = c.N;
   launch_fill_values(c.L.view, c.L.src.as<int64_t>(), c.dD.as<double>(), c.dAL.as<double>(), c.dAU.as<double>(), N,
                      c.NPL, c.stream);
   launch_fill_values(c.U.view, c.U.src.as<int64_t>(), c.dD.as<double>(), c.dAL.as<double>(), c.dAU.as<double>(), N,
                      c.NPL, c.stream);
-  launch_block_lu((int32_t)row_old.size(), c.row_old.as<int32_t>(), c.dD.as<double>(), sigma,
-                  c.alu_ssor.as<double>(), kRowsS, c.stream);
+  launch_block_lu((int32_t)c.ssor_rows, c.row_old.as<int32_t>(), c.dD.as<double>(), sigma, c.alu_ssor.as<double>(),
+                  c.ssor_R, c.stream);
   HIPC(hipGetLastError());
   HIPC(hipStreamSynchronize(c.stream));
   c.ssor_init = true;
@@ -456,8 +512,11 @@ static int32_t setup_f32(Context& c, int32_t kind) {
   }
   if (kind == 1 && !c.ssor32_valid) {
     if (round_sell(c.L, c.stream) || round_sell(c.U, c.stream)) return HECMW33_ERROR_DEVICE;
-    const int64_t nslot = (int64_t)c.color_slice[c.ncolor] * kRowsS;
-    if (c.alu_ssor32.ensure(sizeof(float) * 9 * std::max<int64_t>(nslot, 1))) return HECMW33_ERROR_DEVICE;
+    const int64_t nslot = c.ssor_rows;
+    if (c.alu_ssor32.ensure(sizeof(float) * 9 * std::max<int64_t>(nslot, 1)) ||
+        c.vzs32.ensure(sizeof(float) * 3 * std::max<int64_t>(nslot, 1)))
+      return HECMW33_ERROR_DEVICE;
+    HIPC(hipMemsetAsync(c.vzs32.p, 0, sizeof(float) * 3 * std::max<int64_t>(nslot, 1), c.stream));
     launch_to_f32(9 * nslot, c.alu_ssor.as<double>(), c.alu_ssor32.as<float>(), c.stream);
     c.ssor32_valid = true;
   }
@@ -483,7 +542,7 @@ static int32_t precond_setup(Context& c, int32_t* iarray, const double* rarray) 
   const bool want = IA(iarray, I_FLAG_SYMBFACT) == 1 || IA(iarray, I_FLAG_NUMFACT) == 1;
   if (pc == 1 || pc == 2) {
     if (c.ssor_init && !want) return 0;
-    if (int32_t e = setup_ssor(c, sigma, IA(iarray, I_NCOLOR_IN))) return e;
+    if (int32_t e = setup_ssor(c, sigma, IA(iarray, I_NCOLOR_IN), IA(iarray, I_FLAG_SYMBFACT) == 1)) return e;
   } else if (pc == 3) {
     if (c.diag_init && IA(iarray, I_FLAG_SYMBFACT) == 0 && IA(iarray, I_FLAG_NUMFACT) == 0) return 0;
     if (int32_t e = setup_diag(c, sigma)) return e;
@@ -509,8 +568,7 @@ static void enqueue_precond(Context& c, int32_t kind, int32_t iterpremax, const 
   KrylovState* st = c.st.as<KrylovState>();
   const int64_t n3 = 3 * (int64_t)c.N;
   auto apply = [&](const double* in, double* out) {
-    precond_apply(kind, c.L.view, c.U.view, c.row_old.as<int32_t>(), c.alu_ssor.as<double>(), c.color_slice.data(),
-                  c.ncolor, c.alu_diag.as<double>(), c.N, in, out, st, c.stream);
+    precond_apply(kind, c.ssor_op(), c.alu_diag.as<double>(), c.N, in, out, st, c.stream);
   };
   if (kind == 0 || iterpremax <= 1) {
     apply(r, z);
@@ -544,8 +602,7 @@ static void enqueue_cg_iteration(Context& c, int32_t kind, int32_t iterpremax, h
   } else if (kind == 3) {
     cg_precond_diag_rho(c.N, c.alu_diag.as<double>(), r, z, st, part, c.stream);
   } else {
-    cg_precond_ssor_rho(c.L.view, c.U.view, c.row_old.as<int32_t>(), c.alu_ssor.as<double>(), c.color_slice.data(),
-                        c.ncolor, r, z, st, part, c.stream);
+    cg_precond_ssor_rho(c.ssor_op(), r, z, st, part, c.stream);
   }
   if (ev) hipEventRecord(ev[1], c.stream);
   cg_xpay(n3, z, c.vp.as<double>(), st, c.stream);
@@ -592,8 +649,7 @@ static void enqueue_fcg_iteration(Context& c, int32_t kind, hipEvent_t* ev) {
   double* part = c.part.as<double>();
   float *r = c.vr32.as<float>(), *z = c.vz32.as<float>(), *p = c.vp32.as<float>(), *q = c.vq32.as<float>();
   if (ev) hipEventRecord(ev[0], c.stream);
-  fcg_precond_dots(kind, c.L.view32, c.U.view32, c.row_old.as<int32_t>(), c.alu_ssor32.as<float>(),
-                   c.color_slice.data(), c.ncolor, c.alu_diag32.as<float>(), c.N, r, z, q, st, part, c.stream);
+  fcg_precond_dots(kind, c.ssor_op32(), c.alu_diag32.as<float>(), c.N, r, z, q, st, part, c.stream);
   if (ev) hipEventRecord(ev[1], c.stream);
   fcg_p(n3, z, p, st, c.stream);
   if (ev) hipEventRecord(ev[2], c.stream);
@@ -996,13 +1052,7 @@ int32_t hecmw33_precond_apply(const hecmw33_sys* A, const int32_t iarray[100], c
   } else if (pc == 3) {
     launch_diag_apply(c.N, c.alu_diag.as<double>(), c.vr.as<double>(), c.vz.as<double>(), nullptr, c.stream);
   } else {
-    for (int k = 0; k < c.ncolor; ++k)
-      launch_ssor_fwd(c.L.view, c.row_old.as<int32_t>(), c.alu_ssor.as<double>(), c.color_slice[k],
-                      c.color_slice[k + 1] - c.color_slice[k], c.vr.as<double>(), c.vz.as<double>(), nullptr,
-                      c.stream);
-    for (int k = c.ncolor - 1; k >= 0; --k)
-      launch_ssor_bwd(c.U.view, c.row_old.as<int32_t>(), c.alu_ssor.as<double>(), c.color_slice[k],
-                      c.color_slice[k + 1] - c.color_slice[k], c.vz.as<double>(), nullptr, c.stream);
+    ssor_apply(c.ssor_op(), c.vr.as<double>(), c.vz.as<double>(), nullptr, c.stream);
   }
   HIPC(hipGetLastError());
   HIPC(hipMemcpyAsync(z, c.vz.p, sizeof(double) * 3 * (size_t)A->N, hipMemcpyDeviceToHost, c.stream));

@@ -279,3 +279,29 @@ def test_scaling_through_solve_iterative_and_mixed():
     res = fa.hecmw_solve_iterative(m2)
     assert res.error == 0
     assert np.linalg.norm(m2.X - xr) / np.linalg.norm(xr) <= 1e-6
+
+
+@pytest.mark.parametrize("name", ["hex8_10", "random_700", "random_ext"])
+def test_ssor_lane_per_row_layout_bitwise(name, monkeypatch):
+    """The lane-per-row SSOR layout (SELL-64 slices, chosen for large colours)
+    gives the same bits as the four-lane layout and the reference."""
+    monkeypatch.setenv("HECMW33_SSOR_ROWS", "64")
+    m = dict(SYSTEMS)[name].copy()
+    m.set_precond(1)
+    r = np.random.default_rng(2).standard_normal(3 * m.NP)
+    z = fa.hecmw_precond_apply(m, r)
+    assert np.array_equal(z, ref.precond(m, r, nthreads=2))
+    m.set_iter(5000)
+    res = fa.hecmw_solve_krylov(m)
+    h = fa.last_history()
+    _, info = ref.solve_krylov(m, nthreads=2, history=True)
+    assert res.error == info["error"] == 0
+    assert abs(res.iter - info["iter"]) <= 2
+    k = min(15, len(h), len(info["history"]))
+    np.testing.assert_allclose(h[:k], info["history"][:k], rtol=2e-6)
+    # and through the mixed path
+    m2 = m.copy()
+    m2.X[:] = 0.0
+    m2.set_mixed(True)
+    res = fa.hecmw_solve_krylov(m2)
+    assert res.error == 0 and ref.rel_resid(m2, m2.X) <= 1e-8 * 1.0001
