@@ -11,9 +11,12 @@ the symbolic RCM+multicolour ordering is built once at load).
 
 Also reported: DIAG-CG on the same system, the FP64/FP32 SpMV bandwidth, and
 the reference's own CPU solver (oracle/_ref, built from the FrontISTR sources)
-timed on the host on a bounded sample of the same solve.
+timed on the host on a bounded sample of the same solve; the mixed-precision
+comparison on the 160^3 heterogeneous cube ("mixed": FP64 SSOR-CG vs the
+flexible CG with FP32 operators, same GPU); the batched workload ("batch": 48
+TET10 systems x 16 right-hand sides).
 
-    python bench.py [--steps K] [--warmup W] [--n 100] [--gpus 1]
+    python bench.py [--steps K] [--warmup W] [--n 100] [--gpus 1] [--c3-n 160] [--c4-nsys 48]
 """
 import argparse
 import json
@@ -51,6 +54,8 @@ def main():
     ap.add_argument("--cpu-sample-s", type=float, default=20.0, help="budget of the CPU reference sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=100)
+    ap.add_argument("--c3-n", type=int, default=160, help="mixed-precision comparison cube (0: skip)")
+    ap.add_argument("--c4-nsys", type=int, default=48, help="batched TET10 systems (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus %d requested); multi-GPU sharding is "
@@ -153,6 +158,11 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(m, iters, args.cpu_sample_s)
+    del ctx, b, x, y, xr, x32, y32
+    torch.cuda.empty_cache()
+    mixed = bench_mixed(args, dev) if args.c3_n > 0 else None
+    torch.cuda.empty_cache()
+    batch = bench_batch(args) if args.c4_nsys > 0 else None
 
     out = {
         "metric": "time to 1e-8 rel. residual: mixed+learned FCG vs plain CG; BCSR SpMV GB/s",
@@ -186,8 +196,98 @@ def main():
                         "time_Ax": round(r0.time_Ax, 4), "time_precond": round(r0.time_precond, 4),
                         "load_s": round(t_load, 3)},
         "cpu_baseline": cpu,
+        "mixed": mixed,
+        "batch": batch,
     }
     print(json.dumps(out), flush=True)
+
+
+def bench_mixed(args, dev):
+    """Heterogeneous cube (per-element E log-uniform in [1e3, 1e6], nu uniform
+    in [0.2, 0.4], seed 0): FP64 multicolour-SSOR CG against the flexible CG
+    with FP32 SpMV/SSOR/vectors and FP64 residual replacement, both from
+    x0 = 0 to 1e-8 with the system resident and a numeric preconditioner
+    refresh inside the timed solve."""
+    import torch
+    import frontistr_amd as fa
+    n = args.c3_n
+    t0 = time.time()
+    m = fa.gen_hex8(n, kind="lognormal", seed=0)
+    log("C3: generated %d^3 lognormal HEX8, %d DOF in %.1fs" % (n, 3 * m.N, time.time() - t0))
+    m.set_iter(20000)
+    m.set_precond(1)
+    m.set_i(m.I_TIMELOG, 1)
+    ctx = fa.DeviceContext()
+    t0 = time.time()
+    ctx.load(m)
+    t_load = time.time() - t0
+    b = torch.from_numpy(m.B).to(dev)
+    x = torch.zeros(3 * m.NP, dtype=torch.float64, device=dev)
+    res = {}
+    xs = {}
+    for mode in ("fp64", "mixed"):
+        ia = m.Iarray.copy()
+        ia[96] = 1
+        ia[97] = 0
+        ia[40] = 1 if mode == "mixed" else 0
+        x.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = ctx.solve_device(b.data_ptr(), x.data_ptr(), ia, m.Rarray)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        xs[mode] = x.cpu().numpy()
+        m.X[:] = xs[mode]
+        true = fa.hecmw_rel_resid_L2(m)
+        res[mode] = {"seconds": round(wall, 4), "iterations": r.iter, "error": r.error, "Tset": round(r.Tset, 4),
+                     "Tsol": round(r.Tsol, 4), "ms_per_iter": round(1e3 * r.Tsol / max(1, r.iter), 4),
+                     "time_Ax": round(r.time_Ax, 4), "time_precond": round(r.time_precond, 4),
+                     "rel_resid_true": true}
+        if mode == "mixed":
+            res[mode]["fp64_residual_updates"] = ctx.info()["nreliable"]
+        log("C3 %s: %s" % (mode, res[mode]))
+    err = float(np.linalg.norm(xs["mixed"] - xs["fp64"]) / np.linalg.norm(xs["fp64"]))
+    ctx.close()
+    return {"workload": "%d^3 HEX8, lognormal E in [1e3,1e6], nu in [0.2,0.4], seed 0, %d DOF, %d blocks"
+                        % (n, 3 * m.N, m.N + m.NPL + m.NPU),
+            "fp64_ssor_cg": res["fp64"], "mixed_fcg": res["mixed"],
+            "speedup": round(res["fp64"]["seconds"] / res["mixed"]["seconds"], 3),
+            "rel_err_mixed_vs_fp64": err, "load_s": round(t_load, 2),
+            "learned": None}
+
+
+def bench_batch(args):
+    """48 TET10 boxes of 5k-600k DOF (log-uniform, seeded) x 16 right-hand
+    sides, DIAG flexible CG with FP32 SpMM and FP64 reductions, one
+    block-diagonal launch per kernel, graph-captured iterations."""
+    import frontistr_amd as fa
+    t0 = time.time()
+    mats, Bs, specs = fa.gen_tet10_batch(nsys=args.c4_nsys, nrhs=16, seed=0)
+    dof = sum(sp[3] for sp in specs)
+    log("C4: generated %d TET10 systems, %d DOF total in %.1fs" % (len(mats), dof, time.time() - t0))
+    m0 = fa.gen_hex8(1)
+    m0.set_iter(20000)
+    m0.set_precond(3)
+    t0 = time.time()
+    bs = fa.BatchSolver(mats, Bs)
+    t_load = time.time() - t0
+    info = bs.info()
+    bs.solve(m0.Iarray, m0.Rarray)  # warm-up: graph capture
+    t0 = time.perf_counter()
+    it, er, rs, tm = bs.solve(m0.Iarray, m0.Rarray)
+    wall = time.perf_counter() - t0
+    blocks = info["blocks"]
+    spmm_bytes = 9 * 4 * blocks + 4 * blocks + 4 * (info["rows"] + 1) + 16 * 3 * 4 * 2 * info["rows"]
+    spmm_ms = info["variant_us"][info["variant"]] / 1e3
+    log("C4: %.3fs, iterations %d..%d, max resid %.2e, errors %d, SpMM %s %.3f ms" % (
+        wall, it.min(), it.max(), rs.max(), int((er != 0).sum()), info["variant"], spmm_ms))
+    bs.close()
+    return {"workload": "%d TET10 boxes (5k-600k DOF log-uniform, aspect 0.5-2, seed 0) x 16 RHS, %d DOF, %d blocks"
+                        % (len(mats), dof, blocks),
+            "seconds": round(wall, 4), "instances": int(it.size), "converged": int((er == 0).sum()),
+            "iterations_min": int(it.min()), "iterations_max": int(it.max()), "max_rel_resid": float(rs.max()),
+            "spmm_variant": info["variant"], "spmm_ms": spmm_ms,
+            "spmm_GBs": round(spmm_bytes / (spmm_ms * 1e-3) / 1e9, 1), "load_s": round(t_load, 2)}
 
 
 def cpu_baseline(m, gpu_iters, budget_s):

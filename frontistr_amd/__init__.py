@@ -34,5 +34,8 @@ from .hecmw import (  # noqa: F401
     BatchSolver,
     solve_batch,
     gen_tet10_batch,
+    LearnedNet,
+    set_net,
+    harvest_residuals,
     DeviceContext,
 )

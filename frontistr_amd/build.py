@@ -30,6 +30,7 @@ SOURCES = [
     ("gen_hex8.cpp", False),
     ("gen_tet10.cpp", False),
     ("batch.hip", True),
+    ("learned.hip", True),
 ]
 
 

@@ -77,6 +77,8 @@ struct KrylovState {
   double delta2;       // update when |r|^2 <= delta2 * rr_ref
   int32_t reliable;    // recompute r = b - A x in FP64 in this iteration
   int32_t nreliable;   // FP64 true-residual updates so far
+  // learned correction: off for good / used in this iteration / fallbacks to plain SSOR so far
+  int32_t learned_off, learned_now, nfallback, pad2;
 };
 
 constexpr int kMaxPartials = 65536;  // per reduction component
@@ -171,6 +173,31 @@ void launch_make_scale(int32_t n, int32_t np, const double* D, double* sc, hipSt
 void launch_scale_raw(bool fw, int32_t np, const int32_t* indexL, const int32_t* itemL, const int32_t* indexU,
                       const int32_t* itemU, double* D, double* AL, double* AU, const double* sc, hipStream_t s);
 void launch_scale_vec(bool mul, int64_t n, double* v, const double* sc, hipStream_t s);
+
+// learned flexible CG: beta from the four dots of net_apply (r.z, z.q, r.z0, z0.q), falling
+// back to the plain SSOR output z0 when the learned z is not a descent direction
+void fcg_beta_learned(KrylovState* st, const double* part, int nparts, hipStream_t s);
+// p = (learned z or z0) + beta p
+void fcg_p_sel(int64_t n3, const float* z, const float* z0, float* p, const KrylovState* st, hipStream_t s);
+
+// ---- learned correction (learned.hip) -------------------------------------------------
+int net_weight_count();
+void net_apply(int64_t n, const __bf16* Wb, const float* r, const float* z0, float* z, const float* ajac,
+               const float* mat, const float* q, const KrylovState* st, double* part, int* nparts, hipStream_t s);
+void net_features(int64_t n, const float* r, const float* z0, const float* ajac, const float* mat, __bf16* X,
+                  hipStream_t s);
+void net_forward(int64_t n, const __bf16* Wb, const __bf16* X, __bf16* H1, __bf16* H2, const float* z0, float* z,
+                 hipStream_t s);
+int net_bwd_parts(int64_t n);
+void net_backward(int64_t n, const __bf16* Wb, const __bf16* X, const __bf16* H1, const __bf16* H2, const float* dY,
+                  float* part, float* dW, hipStream_t s);
+void net_adam(float* Wm, float* M, float* V, const float* dW, __bf16* Wb, float lr, float b1, float b2, float eps,
+              int step, float scale, hipStream_t s);
+void net_to_bf16(const float* Wm, __bf16* Wb, hipStream_t s);
+void net_energy_dots(int64_t n3, const float* z, const float* q, const float* r, const float* z0, double* part,
+                     double* dots, hipStream_t s);
+void net_energy_grad(int64_t n3, const float* q, const float* r, const double* dots, float* dY, hipStream_t s);
+void net_material(int64_t n, const double* D, float* m, double* part, double* sum, hipStream_t s);
 
 // r32 = (float) r64 over n entries
 void launch_to_f32(int64_t n, const double* x, float* y, hipStream_t s);

@@ -117,6 +117,7 @@ enum Tail : int {
   kTailFcgBeta = 8,    // flexible CG: rho = r.z, beta = -(z.q_prev)/(p.q)_prev
   kTailFcgAlpha = 9,   // alpha = (r.p)/(p.q)
   kTailFcgRes = 10,    // recurrence |r|^2, reliable-update decision
+  kTailFcgBetaL = 11,  // learned: pick z or z0 from r.z, z.q, r.z0, z0.q, then as kTailFcgBeta
 };
 
 template <int TAIL, int K>
@@ -169,13 +170,25 @@ __device__ void run_tail(KrylovState* st, const double (&tot)[K], double* out, d
     st->ts = tot[0];
     st->tt = tot[K > 1 ? 1 : 0];
     st->omega = st->ts / st->tt;
-  } else if (TAIL == kTailFcgBeta) {
+  } else if (TAIL == kTailFcgBeta || TAIL == kTailFcgBetaL) {
     // Notay's FCG(1): beta keeps p A-orthogonal to the previous direction
     // whatever the (FP32) preconditioner did; r.z only checks positivity
     // with CG's rules.
-    const double rho = tot[0];
-    st->rho = rho;
+    double rho = tot[0];
     st->zq = tot[K > 1 ? 1 : 0];
+    if (TAIL == kTailFcgBetaL) {
+      // the learned z must be a descent direction (r.z > 0, finite), else the
+      // plain SSOR output z0 takes its place in this iteration
+      const bool use = !st->learned_off && tot[0] > 0.0 && tot[0] == tot[0] && st->zq == st->zq &&
+                       tot[0] < 1e300 && fabs(st->zq) < 1e300;
+      if (!use && !st->learned_off) st->nfallback += 1;
+      st->learned_now = use;
+      if (!use) {
+        rho = tot[K > 2 ? 2 : 0];
+        st->zq = tot[K > 3 ? 3 : 0];
+      }
+    }
+    st->rho = rho;
     if (rho == 0.0) {
       st->done = 1;
     } else if (rho != rho) {
@@ -267,6 +280,9 @@ __global__ __launch_bounds__(kTailThreads) void k_tail_fcg_rel(KrylovState* st, 
     double tot[1];
     sum_partials<1>(part, nparts, tot);
     if (threadIdx.x == 0) {
+      // the learned correction is dropped for the rest of the solve once the
+      // FP64 residual grows (4x in |r|^2) between two replacements
+      if (!st->learned_off && tot[0] > 4.0 * st->rr_ref) st->learned_off = 1;
       st->dnrm2 = tot[0];
       st->rr_ref = tot[0];
       st->resid = sqrt(tot[0] / st->bnrm2);
@@ -964,6 +980,18 @@ __global__ __launch_bounds__(kBlock) void k_copy_dots_f32(int64_t n, const float
   if (ndots > 0) block_partials<2>(d, part, blockIdx.x);
 }
 
+// p = (z or z0, as the learned tail chose) + beta p
+__global__ __launch_bounds__(kBlock) void k_fcg_p_sel(int64_t n, const float* __restrict__ z,
+                                                        const float* __restrict__ z0, float* __restrict__ p,
+                                                        const KrylovState* st) {
+  if (st->done) return;
+  const bool first = st->iter == 1;
+  const double beta = st->beta;
+  const float* __restrict__ zz = st->learned_now ? z : z0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    p[i] = first ? zz[i] : (float)((double)zz[i] + beta * (double)p[i]);
+}
+
 // p = z (first iteration) or p = z + beta p
 __global__ __launch_bounds__(kBlock) void k_fcg_p(int64_t n, const float* __restrict__ z, float* __restrict__ p,
                                                     const KrylovState* st) {
@@ -1358,6 +1386,12 @@ void fcg_precond_dots(int32_t kind, const SsorOp<float>& op, const float* alu_di
     nparts = ssor_apply_dots<float, 2>(op, r, z, q, st, part, s);
   }
   k_tail<kTailFcgBeta, 2><<<1, kTailThreads, 0, s>>>(st, part, nparts, nullptr, nullptr);
+}
+void fcg_beta_learned(KrylovState* st, const double* part, int nparts, hipStream_t s) {
+  k_tail<kTailFcgBetaL, 4><<<1, kTailThreads, 0, s>>>(st, part, nparts, nullptr, nullptr);
+}
+void fcg_p_sel(int64_t n3, const float* z, const float* z0, float* p, const KrylovState* st, hipStream_t s) {
+  k_fcg_p_sel<<<vec_grid(n3), kBlock, 0, s>>>(n3, z, z0, p, st);
 }
 void fcg_p(int64_t n3, const float* z, float* p, const KrylovState* st, hipStream_t s) {
   k_fcg_p<<<vec_grid(n3), kBlock, 0, s>>>(n3, z, p, st);

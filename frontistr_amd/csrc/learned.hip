@@ -1,0 +1,562 @@
+// Learned correction for the flexible CG: a per-node MLP evaluated and
+// trained with BF16 matrix-core (MFMA) kernels accumulating in FP32.
+//
+// No counterpart in the reference (its preconditioners are SSOR, DIAG and
+// the ones hecmw_precond_setup dispatches to, hecmw_precond.f90:30-58).
+//
+// Network: features x_i (16, node-local, linear in the residual r) ->
+// 64 -> ReLU -> 64 -> ReLU -> 3, no biases, so N(a x) = a N(x) for a > 0 and
+// the preconditioner scales with the residual.  Features of node i:
+//   z0_i (3)          the FP32 multicolour-SSOR output for r
+//   J r_i (3)         the block-Jacobi output (LU of D_ii, as DIAG)
+//   m_i z0_i, m_i J r_i (6)  both gated by m_i = tanh((ln tr D_ii - mean)/2),
+//                     a normalised local stiffness
+//   0 (4)             padding
+// The preconditioned residual is z = z0 + N(x).
+//
+// Layouts (row-major [k][n], BF16): W1 [16][64], W2 [64][64], W3 [64][16]
+// (columns 3..15 zero).  A v_mfma_f32_16x16x32_bf16 tile takes, in lane l,
+// A[i = l%16][k = 8(l/16)+e] and B[k = 8(l/16)+e][j = l%16], e = 0..7, and
+// returns D[i = 4(l/16)+v][j = l%16], v = 0..3.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "device.h"
+
+namespace hecmw33 {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int kNT = 256;  // threads per workgroup (4 waves)
+constexpr int kFt = 16, kHid = 64, kOut = 16;
+
+struct NetW {
+  const __bf16 *W1, *W2, *W3;
+};
+
+__device__ __forceinline__ floatx4 mfma(const bf16x8& a, const bf16x8& b, const floatx4& c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// B operand of W [K][N] (row-major), k-block kb, column tile nt; rows >= K are zero
+__device__ __forceinline__ bf16x8 load_b(const __bf16* W, int K, int N, int kb, int nt, int lane) {
+  bf16x8 v;
+  const int n = nt * 16 + (lane & 15), k0 = kb * 32 + 8 * (lane >> 4);
+#pragma unroll
+  for (int e = 0; e < 8; ++e) v[e] = (k0 + e < K) ? W[(k0 + e) * N + n] : (__bf16)0.0f;
+  return v;
+}
+// B operand of W^T where W is [N][K] row-major: B[k][n] = W[n][k]; k >= K zero
+__device__ __forceinline__ bf16x8 load_bt(const __bf16* W, int K, int kb, int nt, int lane) {
+  bf16x8 v;
+  const int n = nt * 16 + (lane & 15), k0 = kb * 32 + 8 * (lane >> 4);
+#pragma unroll
+  for (int e = 0; e < 8; ++e) v[e] = (k0 + e < K) ? W[n * K + k0 + e] : (__bf16)0.0f;
+  return v;
+}
+
+// node features (12 used, 16 with padding)
+__device__ __forceinline__ void node_features(int64_t i, const float* __restrict__ r, const float* __restrict__ z0,
+                                              const float* __restrict__ ajac, const float* __restrict__ mat,
+                                              float (&f)[16]) {
+#pragma clang fp contract(off)
+  const float* fa = ajac + (i / 64) * 9 * 64 + (i % 64);
+  float a[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) a[e] = fa[e * 64];
+  float x0 = r[3 * i], x1 = r[3 * i + 1], x2 = r[3 * i + 2];
+  x1 = x1 - a[3] * x0;
+  x2 = x2 - a[6] * x0 - a[7] * x1;
+  x2 = a[8] * x2;
+  x1 = a[4] * (x1 - a[5] * x2);
+  x0 = a[0] * (x0 - a[2] * x2 - a[1] * x1);
+  const float m = mat[i];
+  const float z00 = z0[3 * i], z01 = z0[3 * i + 1], z02 = z0[3 * i + 2];
+  f[0] = z00; f[1] = z01; f[2] = z02;
+  f[3] = x0; f[4] = x1; f[5] = x2;
+  f[6] = m * z00; f[7] = m * z01; f[8] = m * z02;
+  f[9] = m * x0; f[10] = m * x1; f[11] = m * x2;
+  f[12] = f[13] = f[14] = f[15] = 0.0f;
+}
+
+// A operand of the 16-node feature tile for this lane: features 8(l/16)+e
+__device__ __forceinline__ bf16x8 feat_operand(const float (&f)[16], int g, bool valid) {
+  bf16x8 a;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int k = 8 * g + e;
+    a[e] = (valid && k < kFt) ? (__bf16)f[k & 15] : (__bf16)0.0f;
+  }
+  return a;
+}
+
+// Forward of one 16-node tile from the layer-1 A operand; hs: this wave's
+// [16][64] BF16 staging.  Returns the output tile (lane: rows 4g+v, col l%16).
+// If H1/H2 are given, the post-ReLU activations of valid nodes are stored.
+__device__ __forceinline__ floatx4 mlp_tile(const bf16x8& a1, const bf16x8 (&w1)[4], const bf16x8 (&w2)[2][4],
+                                            const bf16x8 (&w3)[2], __bf16 (*hs)[kHid], int lane, int64_t node0,
+                                            int64_t n, __bf16* H1, __bf16* H2) {
+  const int g = lane >> 4, j = lane & 15;
+  const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) {
+    const floatx4 c = mfma(a1, w1[nt], zero);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const __bf16 h = (__bf16)fmaxf(c[v], 0.0f);
+      hs[4 * g + v][nt * 16 + j] = h;
+      const int64_t node = node0 + 4 * g + v;
+      if (H1 && node < n) H1[node * kHid + nt * 16 + j] = h;
+    }
+  }
+  __syncthreads();
+  bf16x8 a2[2];
+#pragma unroll
+  for (int kb = 0; kb < 2; ++kb) a2[kb] = *reinterpret_cast<const bf16x8*>(&hs[j][kb * 32 + 8 * g]);
+  floatx4 c2[4];
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) {
+    c2[nt] = mfma(a2[0], w2[0][nt], zero);
+    c2[nt] = mfma(a2[1], w2[1][nt], c2[nt]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const __bf16 h = (__bf16)fmaxf(c2[nt][v], 0.0f);
+      hs[4 * g + v][nt * 16 + j] = h;
+      const int64_t node = node0 + 4 * g + v;
+      if (H2 && node < n) H2[node * kHid + nt * 16 + j] = h;
+    }
+  __syncthreads();
+#pragma unroll
+  for (int kb = 0; kb < 2; ++kb) a2[kb] = *reinterpret_cast<const bf16x8*>(&hs[j][kb * 32 + 8 * g]);
+  floatx4 c3 = mfma(a2[0], w3[0], zero);
+  c3 = mfma(a2[1], w3[1], c3);
+  __syncthreads();
+  return c3;
+}
+
+__device__ __forceinline__ void load_weights(const NetW& W, int lane, bf16x8 (&w1)[4], bf16x8 (&w2)[2][4],
+                                             bf16x8 (&w3)[2]) {
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) w1[nt] = load_b(W.W1, kFt, kHid, 0, nt, lane);
+#pragma unroll
+  for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) w2[kb][nt] = load_b(W.W2, kHid, kHid, kb, nt, lane);
+#pragma unroll
+  for (int kb = 0; kb < 2; ++kb) w3[kb] = load_b(W.W3, kHid, kOut, kb, 0, lane);
+}
+
+// fixed-order block reduction of K doubles, partial to part[k*kMaxPartials + slot]
+template <int K>
+__device__ __forceinline__ void block_sum(double (&v)[K], double* part, int slot) {
+  __shared__ double red[K][kNT / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double w = v[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) w += __shfl_xor(w, off, 64);
+    if (lane == 0) red[k][wave] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double s = red[k][0];
+      for (int w = 1; w < kNT / 64; ++w) s += red[k][w];
+      part[(int64_t)k * kMaxPartials + slot] = s;
+    }
+}
+
+// ---- inference inside the flexible CG -------------------------------------------
+// z = z0 + N(x) (or z = z0 once the learned step is switched off), with the
+// FP64 partials of r.z, z.q_prev, r.z0, z0.q_prev.
+__global__ __launch_bounds__(kNT) void k_net_apply(int64_t n, NetW W, const float* __restrict__ r,
+                                                   const float* __restrict__ z0, float* __restrict__ z,
+                                                   const float* __restrict__ ajac, const float* __restrict__ mat,
+                                                   const float* __restrict__ q, const KrylovState* st, double* part) {
+  if (st && st->done) return;
+  const bool use_net = !(st && st->learned_off);
+  __shared__ __attribute__((aligned(16))) __bf16 hs[kNT / 64][16][kHid];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, j = lane & 15;
+  bf16x8 w1[4], w2[2][4], w3[2];
+  if (use_net) load_weights(W, lane, w1, w2, w3);
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const int64_t ntile = (n + 15) / 16;
+  const int64_t per_pass = (int64_t)gridDim.x * (kNT / 64);
+  const int64_t npass = (ntile + per_pass - 1) / per_pass;
+  for (int64_t ps = 0; ps < npass; ++ps) {
+    const int64_t t = ps * per_pass + (int64_t)blockIdx.x * (kNT / 64) + wave;
+    const int64_t node0 = t * 16;
+    floatx4 c3 = {0.f, 0.f, 0.f, 0.f};
+    if (use_net) {
+      const int64_t mynode = node0 + j;
+      const bool valid = t < ntile && mynode < n;
+      float f[16];
+      if (valid) node_features(mynode, r, z0, ajac, mat, f);
+      else
+#pragma unroll
+        for (int e = 0; e < 16; ++e) f[e] = 0.f;
+      const bf16x8 a1 = feat_operand(f, g, valid);
+      c3 = mlp_tile(a1, w1, w2, w3, hs[wave], lane, node0, n, nullptr, nullptr);
+    }
+    if (j < 3 && t < ntile) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t node = node0 + 4 * g + v;
+        if (node < n) {
+          const int64_t o = 3 * node + j;
+          const float zz0 = z0[o];
+          const float zz = use_net ? zz0 + c3[v] : zz0;
+          z[o] = zz;
+          const double rv = r[o], qv = q[o];
+          acc[0] += rv * zz;
+          acc[1] += (double)zz * qv;
+          acc[2] += rv * zz0;
+          acc[3] += (double)zz0 * qv;
+        }
+      }
+    }
+  }
+  block_sum<4>(acc, part, blockIdx.x);
+}
+
+// ---- training -------------------------------------------------------------------
+__global__ __launch_bounds__(kNT) void k_net_features(int64_t n, const float* __restrict__ r,
+                                                      const float* __restrict__ z0, const float* __restrict__ ajac,
+                                                      const float* __restrict__ mat, __bf16* __restrict__ X) {
+  for (int64_t i = (int64_t)blockIdx.x * kNT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kNT) {
+    float f[16];
+    node_features(i, r, z0, ajac, mat, f);
+#pragma unroll
+    for (int e = 0; e < 16; ++e) X[i * kFt + e] = (__bf16)f[e];
+  }
+}
+
+// forward on stored features; keeps H1, H2; writes z = z0 + N(x) (z0 may be
+// null: z = N(x), the raw network output)
+__global__ __launch_bounds__(kNT) void k_net_fwd(int64_t n, NetW W, const __bf16* __restrict__ X,
+                                                 __bf16* __restrict__ H1, __bf16* __restrict__ H2,
+                                                 const float* __restrict__ z0, float* __restrict__ z) {
+  __shared__ __attribute__((aligned(16))) __bf16 hs[kNT / 64][16][kHid];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, j = lane & 15;
+  bf16x8 w1[4], w2[2][4], w3[2];
+  load_weights(W, lane, w1, w2, w3);
+  const int64_t ntile = (n + 15) / 16;
+  const int64_t per_pass = (int64_t)gridDim.x * (kNT / 64);
+  const int64_t npass = (ntile + per_pass - 1) / per_pass;
+  for (int64_t ps = 0; ps < npass; ++ps) {
+    const int64_t t = ps * per_pass + (int64_t)blockIdx.x * (kNT / 64) + wave;
+    const int64_t node0 = t * 16;
+    const int64_t mynode = node0 + j;
+    const bool valid = t < ntile && mynode < n;
+    bf16x8 a1;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int k = 8 * g + e;
+      a1[e] = (valid && k < kFt) ? X[mynode * kFt + k] : (__bf16)0.0f;
+    }
+    const floatx4 c3 = mlp_tile(a1, w1, w2, w3, hs[wave], lane, node0, n, H1, H2);
+    if (j < 3 && t < ntile)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t node = node0 + 4 * g + v;
+        if (node < n) z[3 * node + j] = (z0 ? z0[3 * node + j] : 0.0f) + c3[v];
+      }
+  }
+}
+
+// Backward: from dY (n x 3, FP32) to the weight gradients.  Each wave sums
+// its 32-node chunks in MFMA accumulators and stores one partial
+// (W1 16x64, W2 64x64, W3 64x16 = 6144 floats) to part[wave].
+constexpr int kWsz = kFt * kHid + kHid * kHid + kHid * kOut;
+__global__ __launch_bounds__(kNT) void k_net_bwd(int64_t n, NetW W, const __bf16* __restrict__ X,
+                                                 const __bf16* __restrict__ H1, const __bf16* __restrict__ H2,
+                                                 const float* __restrict__ dY, float* __restrict__ part) {
+  __shared__ __attribute__((aligned(16))) __bf16 xs[kNT / 64][32][kFt];
+  __shared__ __attribute__((aligned(16))) __bf16 h1s[kNT / 64][32][kHid];
+  __shared__ __attribute__((aligned(16))) __bf16 h2s[kNT / 64][32][kHid];
+  __shared__ __attribute__((aligned(16))) __bf16 dys[kNT / 64][32][kOut];
+  __shared__ __attribute__((aligned(16))) __bf16 d1s[kNT / 64][32][kHid];
+  __shared__ __attribute__((aligned(16))) __bf16 d2s[kNT / 64][32][kHid];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, j = lane & 15;
+  // W3^T (B[k=j][n=h] = W3[h][j], K = 16) and W2^T (B[k=h2][n=h1] = W2[h1][h2])
+  bf16x8 w3t[4], w2t[2][4];
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) w3t[nt] = load_bt(W.W3, kOut, 0, nt, lane);
+#pragma unroll
+  for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) w2t[kb][nt] = load_bt(W.W2, kHid, kb, nt, lane);
+  const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
+  floatx4 g1[4], g2[4][4], g3[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    g1[a] = zero;
+    g3[a] = zero;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) g2[a][b] = zero;
+  }
+  const int64_t nchunk = (n + 31) / 32;
+  const int64_t per_pass = (int64_t)gridDim.x * (kNT / 64);
+  const int64_t npass = (nchunk + per_pass - 1) / per_pass;
+  for (int64_t ps = 0; ps < npass; ++ps) {
+    const int64_t ch = ps * per_pass + (int64_t)blockIdx.x * (kNT / 64) + wave;
+    const int64_t base = ch * 32;
+    // stage X, H1, H2, dY of the chunk (zeros past the end)
+    for (int idx = lane; idx < 32 * kHid; idx += 64) {
+      const int rr = idx / kHid, cc = idx % kHid;
+      const int64_t node = base + rr;
+      const bool ok = ch < nchunk && node < n;
+      h1s[wave][rr][cc] = ok ? H1[node * kHid + cc] : (__bf16)0.0f;
+      h2s[wave][rr][cc] = ok ? H2[node * kHid + cc] : (__bf16)0.0f;
+    }
+    for (int idx = lane; idx < 32 * kFt; idx += 64) {
+      const int rr = idx / kFt, cc = idx % kFt;
+      const int64_t node = base + rr;
+      const bool ok = ch < nchunk && node < n;
+      xs[wave][rr][cc] = ok ? X[node * kFt + cc] : (__bf16)0.0f;
+      dys[wave][rr][cc] = (ok && cc < 3) ? (__bf16)dY[3 * node + cc] : (__bf16)0.0f;
+    }
+    __syncthreads();
+    // dH2 = (dY W3^T) * relu'(H2), per 16-node sub-tile
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      bf16x8 a;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) a[e] = (8 * g + e < kOut) ? dys[wave][s * 16 + j][8 * g + e] : (__bf16)0.0f;
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        const floatx4 c = mfma(a, w3t[nt], zero);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int rr = s * 16 + 4 * g + v, cc = nt * 16 + j;
+          d2s[wave][rr][cc] = (float)h2s[wave][rr][cc] > 0.0f ? (__bf16)c[v] : (__bf16)0.0f;
+        }
+      }
+    }
+    __syncthreads();
+    // dH1 = (dH2 W2^T) * relu'(H1)
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      bf16x8 a[2];
+#pragma unroll
+      for (int kb = 0; kb < 2; ++kb) a[kb] = *reinterpret_cast<const bf16x8*>(&d2s[wave][s * 16 + j][kb * 32 + 8 * g]);
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        floatx4 c = mfma(a[0], w2t[0][nt], zero);
+        c = mfma(a[1], w2t[1][nt], c);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int rr = s * 16 + 4 * g + v, cc = nt * 16 + j;
+          d1s[wave][rr][cc] = (float)h1s[wave][rr][cc] > 0.0f ? (__bf16)c[v] : (__bf16)0.0f;
+        }
+      }
+    }
+    __syncthreads();
+    // weight gradients, the 32 nodes of the chunk as the MFMA K dimension
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      bf16x8 ah2, ah1, bdy;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        ah2[e] = h2s[wave][8 * g + e][mt * 16 + j];
+        ah1[e] = h1s[wave][8 * g + e][mt * 16 + j];
+        bdy[e] = dys[wave][8 * g + e][j];
+      }
+      g3[mt] = mfma(ah2, bdy, g3[mt]);
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        bf16x8 bd2;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) bd2[e] = d2s[wave][8 * g + e][nt * 16 + j];
+        g2[mt][nt] = mfma(ah1, bd2, g2[mt][nt]);
+      }
+    }
+    {
+      bf16x8 ax;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) ax[e] = xs[wave][8 * g + e][j];
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        bf16x8 bd1;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) bd1[e] = d1s[wave][8 * g + e][nt * 16 + j];
+        g1[nt] = mfma(ax, bd1, g1[nt]);
+      }
+    }
+    __syncthreads();
+  }
+  // this wave's partial: D tile (rows 4g+v, col j)
+  float* out = part + (int64_t)(blockIdx.x * (kNT / 64) + wave) * kWsz;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int rr = 4 * g + v;
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) out[rr * kHid + nt * 16 + j] = g1[nt][v];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) out[kFt * kHid + (mt * 16 + rr) * kHid + nt * 16 + j] = g2[mt][nt][v];
+      out[kFt * kHid + kHid * kHid + (mt * 16 + rr) * kOut + j] = g3[mt][v];
+    }
+  }
+}
+
+// dW = sum of the wave partials in a fixed order
+__global__ void k_net_reduce(int nparts, const float* __restrict__ part, float* __restrict__ dW) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= kWsz) return;
+  float s = 0.0f;
+  for (int p = 0; p < nparts; ++p) s += part[(int64_t)p * kWsz + w];
+  dW[w] = s;
+}
+
+// Adam on the FP32 master weights; BF16 copies for the MFMA kernels.
+// Columns 3..15 of W3 stay zero.
+__global__ void k_net_adam(float* __restrict__ Wm, float* __restrict__ M, float* __restrict__ V,
+                           const float* __restrict__ dW, __bf16* __restrict__ Wb, float lr, float b1, float b2,
+                           float eps, float c1, float c2, float scale) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= kWsz) return;
+  const bool frozen = w >= kFt * kHid + kHid * kHid && ((w - kFt * kHid - kHid * kHid) % kOut) >= 3;
+  if (!frozen) {
+    const float gr = dW[w] * scale;
+    const float m = b1 * M[w] + (1.0f - b1) * gr;
+    const float v = b2 * V[w] + (1.0f - b2) * gr * gr;
+    M[w] = m;
+    V[w] = v;
+    Wm[w] -= lr * (m / c1) / (sqrtf(v / c2) + eps);
+  } else {
+    Wm[w] = 0.0f;
+  }
+  Wb[w] = (__bf16)Wm[w];
+}
+
+__global__ void k_net_to_bf16(int n, const float* __restrict__ a, __bf16* __restrict__ b) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w < n) b[w] = (__bf16)a[w];
+}
+
+// energy functional parts: z.Az, z.r, r.z0 (FP64 partials)
+__global__ __launch_bounds__(kNT) void k_energy_dots(int64_t n3, const float* __restrict__ z,
+                                                     const float* __restrict__ q, const float* __restrict__ r,
+                                                     const float* __restrict__ z0, double* part) {
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kNT + threadIdx.x; i < n3; i += (int64_t)gridDim.x * kNT) {
+    acc[0] += (double)z[i] * (double)q[i];
+    acc[1] += (double)z[i] * (double)r[i];
+    acc[2] += (double)r[i] * (double)z0[i];
+  }
+  block_sum<3>(acc, part, blockIdx.x);
+}
+__global__ void k_sum_parts(int nparts, int K, const double* __restrict__ part, double* __restrict__ out) {
+  if (threadIdx.x < K) {
+    double s = 0.0;
+    for (int p = 0; p < nparts; ++p) s += part[(int64_t)threadIdx.x * kMaxPartials + p];
+    out[threadIdx.x] = s;
+  }
+}
+// dL/dz for L = (z.Az - 2 z.r) / (r.z0): 2 (Az - r) / (r.z0)
+__global__ void k_energy_grad(int64_t n3, const float* __restrict__ q, const float* __restrict__ r,
+                              const double* __restrict__ dots, float* __restrict__ dY) {
+  const double s = 2.0 / dots[2];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (int64_t)gridDim.x * blockDim.x)
+    dY[i] = (float)(s * ((double)q[i] - (double)r[i]));
+}
+
+// m_i = tanh((ln tr D_i - mean)/2): pass 1 ln tr D and partial sums, pass 2 normalise
+__global__ __launch_bounds__(kNT) void k_mat_log(int64_t n, const double* __restrict__ D, float* __restrict__ m,
+                                                 double* part) {
+  double acc[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kNT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kNT) {
+    const double tr = fabs(D[9 * i]) + fabs(D[9 * i + 4]) + fabs(D[9 * i + 8]);
+    const double l = log(tr > 0.0 ? tr : 1e-300);
+    m[i] = (float)l;
+    acc[0] += l;
+  }
+  block_sum<1>(acc, part, blockIdx.x);
+}
+__global__ void k_mat_norm(int64_t n, float* __restrict__ m, const double* __restrict__ sum) {
+  const float mean = (float)(sum[0] / (double)n);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    m[i] = tanhf(0.5f * (m[i] - mean));
+}
+
+int grid_for(int64_t n, int per) {
+  int64_t g = (n + per - 1) / per;
+  return (int)(g < 1 ? 1 : (g > 4096 ? 4096 : g));
+}
+
+}  // namespace
+
+// ---- launchers ---------------------------------------------------------------------
+int net_weight_count() { return kWsz; }
+
+void net_apply(int64_t n, const __bf16* Wb, const float* r, const float* z0, float* z, const float* ajac,
+               const float* mat, const float* q, const KrylovState* st, double* part, int* nparts, hipStream_t s) {
+  const NetW W{Wb, Wb + kFt * kHid, Wb + kFt * kHid + kHid * kHid};
+  const int g = grid_for((n + 15) / 16, kNT / 64);
+  k_net_apply<<<g, kNT, 0, s>>>(n, W, r, z0, z, ajac, mat, q, st, part);
+  *nparts = g;
+}
+
+void net_features(int64_t n, const float* r, const float* z0, const float* ajac, const float* mat, __bf16* X,
+                  hipStream_t s) {
+  k_net_features<<<grid_for(n, kNT), kNT, 0, s>>>(n, r, z0, ajac, mat, X);
+}
+
+void net_forward(int64_t n, const __bf16* Wb, const __bf16* X, __bf16* H1, __bf16* H2, const float* z0, float* z,
+                 hipStream_t s) {
+  const NetW W{Wb, Wb + kFt * kHid, Wb + kFt * kHid + kHid * kHid};
+  k_net_fwd<<<grid_for((n + 15) / 16, kNT / 64), kNT, 0, s>>>(n, W, X, H1, H2, z0, z);
+}
+
+// gradients of the weights for upstream dY (n x 3) into dW (6144 floats);
+// part needs net_bwd_parts() * 6144 floats
+int net_bwd_parts(int64_t n) { return grid_for((n + 31) / 32, kNT / 64) * (kNT / 64); }
+void net_backward(int64_t n, const __bf16* Wb, const __bf16* X, const __bf16* H1, const __bf16* H2, const float* dY,
+                  float* part, float* dW, hipStream_t s) {
+  const NetW W{Wb, Wb + kFt * kHid, Wb + kFt * kHid + kHid * kHid};
+  const int g = grid_for((n + 31) / 32, kNT / 64);
+  k_net_bwd<<<g, kNT, 0, s>>>(n, W, X, H1, H2, dY, part);
+  k_net_reduce<<<(kWsz + 255) / 256, 256, 0, s>>>(g * (kNT / 64), part, dW);
+}
+
+void net_adam(float* Wm, float* M, float* V, const float* dW, __bf16* Wb, float lr, float b1, float b2, float eps,
+              int step, float scale, hipStream_t s) {
+  const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+  k_net_adam<<<(kWsz + 255) / 256, 256, 0, s>>>(Wm, M, V, dW, Wb, lr, b1, b2, eps, c1, c2, scale);
+}
+
+void net_to_bf16(const float* Wm, __bf16* Wb, hipStream_t s) {
+  k_net_to_bf16<<<(kWsz + 255) / 256, 256, 0, s>>>(kWsz, Wm, Wb);
+}
+
+// dots[0..2] = z.Az, z.r, r.z0 (q = A z given)
+void net_energy_dots(int64_t n3, const float* z, const float* q, const float* r, const float* z0, double* part,
+                     double* dots, hipStream_t s) {
+  const int g = grid_for(n3, kNT);
+  k_energy_dots<<<g, kNT, 0, s>>>(n3, z, q, r, z0, part);
+  k_sum_parts<<<1, 64, 0, s>>>(g, 3, part, dots);
+}
+void net_energy_grad(int64_t n3, const float* q, const float* r, const double* dots, float* dY, hipStream_t s) {
+  k_energy_grad<<<grid_for(n3, 256), 256, 0, s>>>(n3, q, r, dots, dY);
+}
+
+void net_material(int64_t n, const double* D, float* m, double* part, double* sum, hipStream_t s) {
+  const int g = grid_for(n, kNT);
+  k_mat_log<<<g, kNT, 0, s>>>(n, D, m, part);
+  k_sum_parts<<<1, 64, 0, s>>>(g, 1, part, sum);
+  k_mat_norm<<<grid_for(n, 256), 256, 0, s>>>(n, m, sum);
+}
+
+}  // namespace hecmw33

@@ -32,6 +32,8 @@
 #include "device.h"
 #include "ordering.h"
 
+struct hecmw33_net;
+
 namespace hecmw33 {
 
 static thread_local std::string g_last_error;
@@ -62,6 +64,9 @@ enum {
 // PRECOND 1-3) = 1 selects the flexible CG with FP32 operators;
 // Rarray(41) in (0,1) overrides its reliable-update factor.
 constexpr int32_t kOptMixedFcg = 1;
+// = 2: the same flexible CG with the learned correction of an attached
+// network added to the FP32 SSOR output (hecmw33_ctx_set_net / hecmw33_set_net)
+constexpr int32_t kOptLearnedFcg = 2;
 constexpr double kFcgDelta = 1.0e-2;
 static inline int32_t& IA(int32_t* a, int k) { return a[k - 1]; }
 static inline int32_t IA(const int32_t* a, int k) { return a[k - 1]; }
@@ -201,12 +206,18 @@ struct Context {
   // FP32 copies for the flexible CG (rounded from the FP64 storage)
   bool a32_valid = false, ssor32_valid = false, diag32_valid = false;
   DMem alu_ssor32, alu_diag32, vr32, vz32, vp32, vq32;
+  // learned correction: attached network, plain SSOR output z0, node stiffness feature
+  hecmw33_net* net = nullptr;
+  bool mat_valid = false;
+  DMem vz0_32, vmat, scal;
   // Krylov vectors and state
   DMem vb, vx, vr, vz, vp, vq, vrt, vs, vt, vv, vpt, vst, vzp, vzt;
   DMem st, part, hist;
   KrylovState* h_st = nullptr;  // pinned, 2 slots for polling
   int32_t last_iter = 0;
   int32_t last_nreliable = 0;  // FP64 residual replacements of the last flexible CG solve
+  int32_t last_nfallback = 0;  // iterations of the last learned solve that used the plain SSOR output
+  int32_t last_learned_off = 1;  // the last learned solve switched the correction off
   // TIMELOG: event pairs around the preconditioner and the SpMV of every
   // enqueued iteration (4 events per iteration)
   std::vector<hipEvent_t> events;
@@ -230,7 +241,7 @@ struct Context {
 
   ~Context() {
     if (stream) hipStreamSynchronize(stream);
-    for (DMem* m : {&dIndexL, &dIndexU, &dItemL, &dItemU, &vscale, &vzs, &vzs32}) m->release();
+    for (DMem* m : {&dIndexL, &dIndexU, &dItemL, &dItemU, &vscale, &vzs, &vzs32, &vz0_32, &vmat, &scal}) m->release();
     for (DMem* m : {&dD, &dAL, &dAU, &row_old, &alu_ssor, &row_diag, &alu_diag, &alu_ssor32, &alu_diag32, &vr32,
                     &vz32, &vp32, &vq32, &vb, &vx, &vr, &vz, &vp, &vq, &vrt, &vs, &vt, &vv, &vpt, &vst, &vzp, &vzt, &st,
                     &part, &hist})
@@ -242,6 +253,41 @@ struct Context {
     if (stream) hipStreamDestroy(stream);
   }
 };
+
+}  // namespace hecmw33
+
+// A learned correction: FP32 master weights, BF16 copies for the MFMA kernels,
+// Adam moments, and the training samples (residual, SSOR output, features)
+// harvested on device contexts, which must outlive the samples.
+struct hecmw33_net {
+  struct Sample {
+    hecmw33::Context* c = nullptr;
+    hecmw33::DMem r32, z0, X;
+    int64_t n = 0;
+    double base = 0.0;  // energy functional of the plain SSOR output
+  };
+  hecmw33::DMem Wm, Wb, M, V, dW, wpart, H1, H2, z, q, dY, dots, dpart, xb;
+  hipStream_t stream = nullptr;
+  int32_t step = 0;
+  std::vector<Sample*> samples;
+  void clear_samples() {
+    for (Sample* p : samples) {
+      p->r32.release();
+      p->z0.release();
+      p->X.release();
+      delete p;
+    }
+    samples.clear();
+  }
+  ~hecmw33_net() {
+    if (stream) hipStreamSynchronize(stream);
+    clear_samples();
+    for (hecmw33::DMem* m : {&Wm, &Wb, &M, &V, &dW, &wpart, &H1, &H2, &z, &q, &dY, &dots, &dpart, &xb}) m->release();
+    if (stream) hipStreamDestroy(stream);
+  }
+};
+
+namespace hecmw33 {
 
 static int32_t ctx_init(Context& c) {
   if (c.stream) return 0;
@@ -343,6 +389,7 @@ static int32_t load_values(Context& c, const hecmw33_sys* A) {
                      c.N, c.NPL, c.stream);
   HIPC(hipGetLastError());
   c.a32_valid = false;
+  c.mat_valid = false;
   return 0;
 }
 
@@ -535,6 +582,29 @@ static int32_t setup_f32(Context& c, int32_t kind) {
 
 // hecmw_precond_setup (hecmw_precond.f90:30-58) with the SSOR/DIAG
 // reuse rule (hecmw_precond_SSOR_33.F90:74-82, hecmw_precond_DIAG_33.F90:38-41).
+// Learned flexible CG on top of the SSOR set-up: FP32 copies of A, of the
+// SSOR operator and of the block-Jacobi factors (a feature), the z0 buffer and
+// the node stiffness feature.
+static int32_t learned_setup(Context& c, double sigma) {
+  if (!c.ssor_init) {
+    set_error("the learned correction needs the SSOR preconditioner (PRECOND 1 or 2) set up");
+    return HECMW33_ERROR_UNSUPPORTED;
+  }
+  if (!c.diag_init)
+    if (int32_t e = setup_diag(c, sigma > 0.0 ? sigma : 1.0)) return e;
+  if (int32_t e = setup_f32(c, 1)) return e;
+  if (int32_t e = setup_f32(c, 3)) return e;
+  if (c.vz0_32.ensure(sizeof(float) * 3 * (size_t)c.NP) || c.vmat.ensure(sizeof(float) * (size_t)c.N) ||
+      c.scal.ensure(sizeof(double) * 8))
+    return HECMW33_ERROR_DEVICE;
+  if (!c.mat_valid) {
+    net_material(c.N, c.dD.as<double>(), c.vmat.as<float>(), c.part.as<double>(), c.scal.as<double>(), c.stream);
+    c.mat_valid = true;
+  }
+  HIPC(hipGetLastError());
+  return 0;
+}
+
 static int32_t precond_setup(Context& c, int32_t* iarray, const double* rarray) {
   if (IA(iarray, I_ITERPREMAX) <= 0) return 0;
   const int32_t pc = IA(iarray, I_PRECOND);
@@ -659,6 +729,30 @@ static void enqueue_fcg_iteration(Context& c, int32_t kind, hipEvent_t* ev) {
   fcg_reliable(c.A.view, c.vx.as<double>(), c.vb.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
 }
 
+// One learned flexible CG iteration: z0 = M32^-1 r (FP32 SSOR), z = z0 + N(x)
+// with the four dots, the tail keeps z or falls back to z0, then as the
+// flexible CG.
+static void enqueue_fcg_learned_iteration(Context& c, hipEvent_t* ev) {
+  const int64_t n3 = 3 * (int64_t)c.N;
+  KrylovState* st = c.st.as<KrylovState>();
+  double* part = c.part.as<double>();
+  float *r = c.vr32.as<float>(), *z = c.vz32.as<float>(), *z0 = c.vz0_32.as<float>();
+  float *p = c.vp32.as<float>(), *q = c.vq32.as<float>();
+  if (ev) hipEventRecord(ev[0], c.stream);
+  ssor_apply_f32(c.ssor_op32(), r, z0, st, c.stream);
+  int np = 0;
+  net_apply(c.N, c.net->Wb.as<__bf16>(), r, z0, z, c.alu_diag32.as<float>(), c.vmat.as<float>(), q, st, part, &np,
+            c.stream);
+  fcg_beta_learned(st, part, np, c.stream);
+  if (ev) hipEventRecord(ev[1], c.stream);
+  fcg_p_sel(n3, z, z0, p, st, c.stream);
+  if (ev) hipEventRecord(ev[2], c.stream);
+  fcg_spmv(c.A.view32, p, r, q, st, part, c.stream);
+  if (ev) hipEventRecord(ev[3], c.stream);
+  fcg_update(n3, p, q, c.vx.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
+  fcg_reliable(c.A.view, c.vx.as<double>(), c.vb.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
+}
+
 // Diagonal scaling of the device copies, hecmw_solver_scaling_fw_33 / _bk_33
 // (hecmw_solver_scaling_33.f90:20-208): forward scales the uploaded D/AL/AU
 // and b in place (x0 is left as given), backward unscales x and restores b
@@ -731,15 +825,23 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
     set_error("METHOD not provided by the GPU library (1: CG, 2: BiCGSTAB)");
     return HECMW33_ERROR_UNSUPPORTED;
   }
-  const bool mixed = IA(iarray, I_SOLVER_OPT) == kOptMixedFcg;
+  const int32_t opt = IA(iarray, I_SOLVER_OPT);
+  const bool learned = opt == kOptLearnedFcg;
+  const bool mixed = opt == kOptMixedFcg || learned;
   if (mixed && (method != 1 || iterpremax > 1)) {
-    set_error("SOLVER_OPT(1)=1 (flexible CG, FP32 operators) needs METHOD=1 and ITERPREMAX<=1");
+    set_error("SOLVER_OPT(1)=1/2 (flexible CG, FP32 operators) needs METHOD=1 and ITERPREMAX<=1");
+    return HECMW33_ERROR_UNSUPPORTED;
+  }
+  if (learned && (!c.net || (pc != 1 && pc != 2) || iterpremax < 1)) {
+    set_error("SOLVER_OPT(1)=2 (learned correction) needs an attached network and PRECOND 1 or 2");
     return HECMW33_ERROR_UNSUPPORTED;
   }
   if (int32_t e = precond_setup(c, iarray, rarray)) return e;
   const int32_t kind = pc_kind(pc, iterpremax);
   if (mixed)
     if (int32_t e = setup_f32(c, kind)) return e;
+  if (learned)
+    if (int32_t e = learned_setup(c, RA(rarray, R_SIGMA_DIAG))) return e;
   const int64_t n3 = 3 * (int64_t)c.N;
   const size_t vbytes = sizeof(double) * 3 * (size_t)c.NP;
   KrylovState* st = c.st.as<KrylovState>();
@@ -803,6 +905,9 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
   h.delta2 = delta * delta;
   h.reliable = 0;
   h.nreliable = 0;
+  h.learned_off = learned ? 0 : 1;
+  h.learned_now = 0;
+  h.nfallback = 0;
   c.h_st[1] = h;
   HIPC(hipMemcpyAsync(st, &c.h_st[1], sizeof(KrylovState), hipMemcpyHostToDevice, c.stream));
   HIPC(hipStreamSynchronize(c.stream));
@@ -826,7 +931,9 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
         for (int j = 0; j < nev; ++j) itev[j] = c.event((size_t)nev * (enq + i) + j);
         e = itev.data();
       }
-      if (mixed)
+      if (learned)
+        enqueue_fcg_learned_iteration(c, e);
+      else if (mixed)
         enqueue_fcg_iteration(c, kind, e);
       else if (method == 1)
         enqueue_cg_iteration(c, kind, iterpremax, e);
@@ -871,6 +978,8 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
   out->resid = h.resid;
   c.last_iter = std::min(h.iter, maxit);
   c.last_nreliable = mixed ? h.nreliable : 0;
+  c.last_nfallback = learned ? h.nfallback : 0;
+  c.last_learned_off = learned ? h.learned_off : 1;
   return 0;
 }
 
@@ -1191,10 +1300,273 @@ int32_t hecmw33_ctx_history(hecmw33_ctx* ctx, double* hist, int32_t cap) {
 int32_t hecmw33_ctx_info(hecmw33_ctx* ctx, int64_t* info, int32_t n) {
   if (!ctx || !info) return HECMW33_ERROR_ARGS;
   Context& c = ctx->c;
-  const int64_t v[7] = {c.A.view.nslots, (int64_t)c.N,     (int64_t)c.NPL,
-                        (int64_t)c.NPU,    (int64_t)c.ncolor, c.L.view.nslots + c.U.view.nslots,
-                        (int64_t)c.last_nreliable};
-  for (int i = 0; i < n && i < 7; ++i) info[i] = v[i];
+  const int64_t v[9] = {c.A.view.nslots,
+                        (int64_t)c.N,
+                        (int64_t)c.NPL,
+                        (int64_t)c.NPU,
+                        (int64_t)c.ncolor,
+                        c.L.view.nslots + c.U.view.nslots,
+                        (int64_t)c.last_nreliable,
+                        (int64_t)c.last_nfallback,
+                        (int64_t)c.last_learned_off};
+  for (int i = 0; i < n && i < 9; ++i) info[i] = v[i];
+  return 0;
+}
+
+}  // extern "C"
+
+// ---- learned correction ---------------------------------------------------------------------
+static uint16_t to_bf16_bits(float f) {
+  // round to nearest even (what the device's float -> __bf16 conversion does)
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16 | ((u & 0xffff) ? 0x40 : 0));
+  const uint32_t lsb = (u >> 16) & 1u;
+  u += 0x7fffu + lsb;
+  return (uint16_t)(u >> 16);
+}
+
+static int32_t net_init(hecmw33_net& N) {
+  if (!N.stream) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
+      set_error("no HIP device visible");
+      return HECMW33_ERROR_DEVICE;
+    }
+    HIPC(hipStreamCreateWithFlags(&N.stream, hipStreamNonBlocking));
+  }
+  const int w = net_weight_count();
+  if (N.Wm.ensure(sizeof(float) * w) || N.Wb.ensure(2 * (size_t)w) || N.M.ensure(sizeof(float) * w) ||
+      N.V.ensure(sizeof(float) * w) || N.dW.ensure(sizeof(float) * w) || N.dots.ensure(sizeof(double) * 8) ||
+      N.dpart.ensure(sizeof(double) * 4 * kMaxPartials))
+    return HECMW33_ERROR_DEVICE;
+  return 0;
+}
+
+static int32_t net_upload(hecmw33_net& N, const float* w) {
+  const int nw = net_weight_count();
+  HIPC(hipMemcpyAsync(N.Wm.p, w, sizeof(float) * nw, hipMemcpyHostToDevice, N.stream));
+  HIPC(hipMemsetAsync(N.M.p, 0, sizeof(float) * nw, N.stream));
+  HIPC(hipMemsetAsync(N.V.p, 0, sizeof(float) * nw, N.stream));
+  net_to_bf16(N.Wm.as<float>(), N.Wb.as<__bf16>(), N.stream);
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(N.stream));
+  N.step = 0;
+  return 0;
+}
+
+// working buffers for n nodes
+static int32_t net_work(hecmw33_net& N, int64_t n) {
+  const size_t n3 = 3 * (size_t)n;
+  if (N.H1.ensure(2 * 64 * (size_t)n) || N.H2.ensure(2 * 64 * (size_t)n) || N.z.ensure(sizeof(float) * n3) ||
+      N.q.ensure(sizeof(float) * n3) || N.dY.ensure(sizeof(float) * n3) ||
+      N.wpart.ensure(sizeof(float) * (size_t)net_bwd_parts(n) * net_weight_count()))
+    return HECMW33_ERROR_DEVICE;
+  return 0;
+}
+
+extern "C" {
+
+hecmw33_net* hecmw33_net_create(uint64_t seed) {
+  hecmw33_net* N = new (std::nothrow) hecmw33_net();
+  if (!N) return nullptr;
+  if (net_init(*N)) {
+    delete N;
+    return nullptr;
+  }
+  // He initialisation of W1 (12 live inputs) and W2, small W3 (columns 0..2)
+  const int nw = net_weight_count();
+  std::vector<float> w(nw, 0.0f);
+  uint64_t st = seed * 0x9E3779B97F4A7C15ull + 1;
+  auto uni = [&]() {
+    uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return ((double)(z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  };
+  auto gauss = [&]() { return std::sqrt(-2.0 * std::log(uni())) * std::cos(6.283185307179586 * uni()); };
+  for (int k = 0; k < 12; ++k)
+    for (int h = 0; h < 64; ++h) w[k * 64 + h] = (float)(gauss() * std::sqrt(2.0 / 12.0));
+  for (int k = 0; k < 64; ++k)
+    for (int h = 0; h < 64; ++h) w[16 * 64 + k * 64 + h] = (float)(gauss() * std::sqrt(2.0 / 64.0));
+  for (int k = 0; k < 64; ++k)
+    for (int j = 0; j < 3; ++j) w[16 * 64 + 64 * 64 + k * 16 + j] = (float)(gauss() * 0.01);
+  if (net_upload(*N, w.data())) {
+    delete N;
+    return nullptr;
+  }
+  return N;
+}
+
+void hecmw33_net_destroy(hecmw33_net* net) { delete net; }
+
+int32_t hecmw33_net_weight_count(void) { return net_weight_count(); }
+
+int32_t hecmw33_net_get_weights(hecmw33_net* net, float* w, int32_t n) {
+  if (!net || !w || n < net_weight_count()) return HECMW33_ERROR_ARGS;
+  HIPC(hipMemcpyAsync(w, net->Wm.p, sizeof(float) * net_weight_count(), hipMemcpyDeviceToHost, net->stream));
+  HIPC(hipStreamSynchronize(net->stream));
+  return 0;
+}
+
+int32_t hecmw33_net_set_weights(hecmw33_net* net, const float* w, int32_t n) {
+  if (!net || !w || n < net_weight_count()) return HECMW33_ERROR_ARGS;
+  return net_upload(*net, w);
+}
+
+int32_t hecmw33_net_eval(hecmw33_net* net, int64_t n, const float* X, const float* dY, float* Y, float* dW) {
+  if (!net || n < 1 || !X || !Y) return HECMW33_ERROR_ARGS;
+  hecmw33_net& N = *net;
+  if (int32_t e = net_work(N, n)) return e;
+  std::vector<uint16_t> xb(16 * (size_t)n);
+  for (size_t i = 0; i < xb.size(); ++i) xb[i] = to_bf16_bits(X[i]);
+  if (N.xb.ensure(2 * xb.size())) return HECMW33_ERROR_DEVICE;
+  HIPC(hipMemcpyAsync(N.xb.p, xb.data(), 2 * xb.size(), hipMemcpyHostToDevice, N.stream));
+  net_forward(n, N.Wb.as<__bf16>(), N.xb.as<__bf16>(), N.H1.as<__bf16>(), N.H2.as<__bf16>(), nullptr, N.z.as<float>(),
+              N.stream);
+  if (dY && dW) {
+    HIPC(hipMemcpyAsync(N.dY.p, dY, sizeof(float) * 3 * n, hipMemcpyHostToDevice, N.stream));
+    net_backward(n, N.Wb.as<__bf16>(), N.xb.as<__bf16>(), N.H1.as<__bf16>(), N.H2.as<__bf16>(), N.dY.as<float>(),
+                 N.wpart.as<float>(), N.dW.as<float>(), N.stream);
+    HIPC(hipMemcpyAsync(dW, N.dW.p, sizeof(float) * net_weight_count(), hipMemcpyDeviceToHost, N.stream));
+  }
+  HIPC(hipMemcpyAsync(Y, N.z.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, N.stream));
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(N.stream));
+  return 0;
+}
+
+// One training sample from a device context holding an SSOR set-up: the
+// residual r (device, FP64, 3*NP), its FP32 SSOR output and the features.
+// Returns the sample count.
+int32_t hecmw33_net_add_sample(hecmw33_net* net, hecmw33_ctx* ctx, const double* r_dev, double rarray_sigma) {
+  if (!net || !ctx || !r_dev || !ctx->c.have_struct) return HECMW33_ERROR_ARGS;
+  Context& c = ctx->c;
+  if (int32_t e = learned_setup(c, rarray_sigma)) return e;
+  hecmw33_net::Sample* S = new (std::nothrow) hecmw33_net::Sample();
+  if (!S) return HECMW33_ERROR_DEVICE;
+  S->c = &c;
+  S->n = c.N;
+  const int64_t n3 = 3 * (int64_t)c.N;
+  if (S->r32.ensure(sizeof(float) * 3 * (size_t)c.NP) || S->z0.ensure(sizeof(float) * 3 * (size_t)c.NP) ||
+      S->X.ensure(2 * 16 * (size_t)c.N) || net_work(*net, c.N)) {
+    delete S;
+    return HECMW33_ERROR_DEVICE;
+  }
+  HIPC(hipMemsetAsync(S->r32.p, 0, sizeof(float) * 3 * (size_t)c.NP, c.stream));
+  launch_to_f32(n3, r_dev, S->r32.as<float>(), c.stream);
+  ssor_apply_f32(c.ssor_op32(), S->r32.as<float>(), S->z0.as<float>(), nullptr, c.stream);
+  net_features(c.N, S->r32.as<float>(), S->z0.as<float>(), c.alu_diag32.as<float>(), c.vmat.as<float>(),
+               S->X.as<__bf16>(), c.stream);
+  // energy functional of the plain SSOR output: (z0.A z0 - 2 z0.r) / (r.z0)
+  launch_spmv_f32(c.A.view32, S->z0.as<float>(), net->q.as<float>(), c.stream);
+  net_energy_dots(n3, S->z0.as<float>(), net->q.as<float>(), S->r32.as<float>(), S->z0.as<float>(),
+                  net->dpart.as<double>(), net->dots.as<double>(), c.stream);
+  double d[3];
+  HIPC(hipMemcpyAsync(d, net->dots.p, sizeof(d), hipMemcpyDeviceToHost, c.stream));
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(c.stream));
+  S->base = (d[0] - 2.0 * d[1]) / d[2];
+  net->samples.push_back(S);
+  return (int32_t)net->samples.size();
+}
+
+int32_t hecmw33_net_clear_samples(hecmw33_net* net) {
+  if (!net) return HECMW33_ERROR_ARGS;
+  net->clear_samples();
+  return 0;
+}
+
+// Adam on the energy functional L(z) = (z.Az - 2 z.r) / (r.z0), z = z0 + N(x),
+// one sample per step, samples in a seeded shuffled order every epoch.
+// loss[0] = mean L of the plain SSOR output, loss[1+e] = mean L over epoch e
+// (before each step's update).
+int32_t hecmw33_net_train(hecmw33_net* net, int32_t epochs, double lr, uint64_t seed, double* loss) {
+  if (!net || epochs < 0 || net->samples.empty()) return HECMW33_ERROR_ARGS;
+  hecmw33_net& N = *net;
+  const size_t ns = N.samples.size();
+  double base = 0.0;
+  for (auto* S : N.samples) base += S->base;
+  if (loss) loss[0] = base / (double)ns;
+  std::vector<size_t> order(ns);
+  for (size_t i = 0; i < ns; ++i) order[i] = i;
+  uint64_t st = seed + 0x1234567ull;
+  std::vector<double> dots(3 * ns);
+  int64_t nmax = 0;
+  for (auto* S : N.samples) nmax = std::max(nmax, S->n);
+  if (int32_t e = net_work(N, nmax)) return e;  // no reallocation inside the loop
+  for (int ep = 0; ep < epochs; ++ep) {
+    for (size_t i = ns; i > 1; --i) {  // Fisher-Yates, splitmix64
+      uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      z ^= z >> 31;
+      std::swap(order[i - 1], order[z % i]);
+    }
+    for (size_t k = 0; k < ns; ++k) {
+      hecmw33_net::Sample& S = *N.samples[order[k]];
+      Context& c = *S.c;
+      const int64_t n = S.n, n3 = 3 * n;
+      if (int32_t e = net_work(N, n)) return e;
+      net_forward(n, N.Wb.as<__bf16>(), S.X.as<__bf16>(), N.H1.as<__bf16>(), N.H2.as<__bf16>(), S.z0.as<float>(),
+                  N.z.as<float>(), N.stream);
+      launch_spmv_f32(c.A.view32, N.z.as<float>(), N.q.as<float>(), N.stream);
+      net_energy_dots(n3, N.z.as<float>(), N.q.as<float>(), S.r32.as<float>(), S.z0.as<float>(),
+                      N.dpart.as<double>(), N.dots.as<double>() , N.stream);
+      HIPC(hipMemcpyAsync(&dots[3 * k], N.dots.p, 3 * sizeof(double), hipMemcpyDeviceToHost, N.stream));
+      net_energy_grad(n3, N.q.as<float>(), S.r32.as<float>(), N.dots.as<double>(), N.dY.as<float>(), N.stream);
+      net_backward(n, N.Wb.as<__bf16>(), S.X.as<__bf16>(), N.H1.as<__bf16>(), N.H2.as<__bf16>(), N.dY.as<float>(),
+                   N.wpart.as<float>(), N.dW.as<float>(), N.stream);
+      net_adam(N.Wm.as<float>(), N.M.as<float>(), N.V.as<float>(), N.dW.as<float>(), N.Wb.as<__bf16>(), (float)lr,
+               0.9f, 0.999f, 1e-8f, ++N.step, 1.0f, N.stream);
+    }
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(N.stream));
+    double acc = 0.0;
+    for (size_t k = 0; k < ns; ++k) acc += (dots[3 * k] - 2.0 * dots[3 * k + 1]) / dots[3 * k + 2];
+    if (loss) loss[1 + ep] = acc / (double)ns;
+  }
+  return 0;
+}
+
+// Mean energy functional of the network's correction over the current
+// samples without training (for held-out checks); base (optional): the plain
+// SSOR output's.
+int32_t hecmw33_net_score(hecmw33_net* net, double* mean, double* base) {
+  if (!net || !mean || net->samples.empty()) return HECMW33_ERROR_ARGS;
+  hecmw33_net& N = *net;
+  double acc = 0.0, b = 0.0;
+  for (auto* Sp : N.samples) {
+    hecmw33_net::Sample& S = *Sp;
+    Context& c = *S.c;
+    const int64_t n = S.n, n3 = 3 * n;
+    if (int32_t e = net_work(N, n)) return e;
+    net_forward(n, N.Wb.as<__bf16>(), S.X.as<__bf16>(), N.H1.as<__bf16>(), N.H2.as<__bf16>(), S.z0.as<float>(),
+                N.z.as<float>(), N.stream);
+    launch_spmv_f32(c.A.view32, N.z.as<float>(), N.q.as<float>(), N.stream);
+    net_energy_dots(n3, N.z.as<float>(), N.q.as<float>(), S.r32.as<float>(), S.z0.as<float>(), N.dpart.as<double>(),
+                    N.dots.as<double>(), N.stream);
+    double d[3];
+    HIPC(hipMemcpyAsync(d, N.dots.p, sizeof(d), hipMemcpyDeviceToHost, N.stream));
+    HIPC(hipStreamSynchronize(N.stream));
+    acc += (d[0] - 2.0 * d[1]) / d[2];
+    b += S.base;
+  }
+  *mean = acc / (double)N.samples.size();
+  if (base) *base = b / (double)N.samples.size();
+  return 0;
+}
+
+int32_t hecmw33_ctx_set_net(hecmw33_ctx* ctx, hecmw33_net* net) {
+  if (!ctx) return HECMW33_ERROR_ARGS;
+  ctx->c.net = net;
+  return 0;
+}
+
+int32_t hecmw33_set_net(hecmw33_net* net) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  default_ctx()->net = net;
   return 0;
 }
 

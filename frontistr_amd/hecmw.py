@@ -93,6 +93,18 @@ def lib():
     L.hecmw33_batch_variant_name.restype = C.c_char_p
     L.hecmw33_batch_last_error.restype = C.c_char_p
     L.hecmw33_solve_batch.argtypes = [I, P, I, P, P, P, P, P, P, P]
+    L.hecmw33_net_create.argtypes = [C.c_uint64]
+    L.hecmw33_net_create.restype = P
+    L.hecmw33_net_destroy.argtypes = [P]
+    L.hecmw33_net_get_weights.argtypes = [P, P, I]
+    L.hecmw33_net_set_weights.argtypes = [P, P, I]
+    L.hecmw33_net_eval.argtypes = [P, C.c_int64, P, P, P, P]
+    L.hecmw33_net_add_sample.argtypes = [P, P, P, C.c_double]
+    L.hecmw33_net_clear_samples.argtypes = [P]
+    L.hecmw33_net_train.argtypes = [P, I, C.c_double, C.c_uint64, P]
+    L.hecmw33_net_score.argtypes = [P, P, P]
+    L.hecmw33_ctx_set_net.argtypes = [P, P]
+    L.hecmw33_set_net.argtypes = [P]
     L.hecmw33_last_error.restype = C.c_char_p
     for f in ("hecmw33_solve", "hecmw33_solve_krylov", "hecmw33_matvec", "hecmw33_precond_apply",
               "hecmw33_rel_resid_L2", "hecmw33_ordering", "hecmw33_ctx_load", "hecmw33_ctx_solve_device",
@@ -100,7 +112,10 @@ def lib():
               "hecmw33_ctx_info", "hecmw33_gen_hex8_count", "hecmw33_gen_hex8",
               "hecmw33_gen_tet10_count", "hecmw33_gen_tet10", "hecmw33_device_count",
               "hecmw33_last_history", "hecmw33_batch_load", "hecmw33_batch_solve", "hecmw33_batch_get_x",
-              "hecmw33_batch_spmm", "hecmw33_batch_info", "hecmw33_solve_batch"):
+              "hecmw33_batch_spmm", "hecmw33_batch_info", "hecmw33_solve_batch", "hecmw33_net_weight_count",
+              "hecmw33_net_get_weights", "hecmw33_net_set_weights", "hecmw33_net_eval", "hecmw33_net_add_sample",
+              "hecmw33_net_clear_samples", "hecmw33_net_train", "hecmw33_net_score", "hecmw33_ctx_set_net",
+              "hecmw33_set_net"):
         getattr(L, f).restype = C.c_int32
     _lib = L
     return L
@@ -408,10 +423,15 @@ class DeviceContext:
         return h[:n]
 
     def info(self):
-        v = np.zeros(7, np.int64)
-        _check(lib().hecmw33_ctx_info(self._h, _ptr(v), 7))
+        v = np.zeros(9, np.int64)
+        _check(lib().hecmw33_ctx_info(self._h, _ptr(v), 9))
         return {"nslots": int(v[0]), "N": int(v[1]), "NPL": int(v[2]), "NPU": int(v[3]), "ncolor": int(v[4]),
-                "ssor_slots": int(v[5]), "nreliable": int(v[6])}
+                "ssor_slots": int(v[5]), "nreliable": int(v[6]), "nfallback": int(v[7]), "learned_off": int(v[8])}
+
+    def set_net(self, net):
+        """Attach a LearnedNet for solves with Iarray(41) = 2 (None detaches)."""
+        self._net = net
+        _check(lib().hecmw33_ctx_set_net(self._h, net._h if net is not None else None))
 
 
 def _bcheck(rc):
@@ -522,3 +542,103 @@ def gen_tet10_batch(nsys=48, nrhs=16, dof_min=5000, dof_max=600000, seed=0, E=21
         Bs.append(B)
         specs.append((nx, ny, nz, 3 * m.N))
     return mats, Bs, specs
+
+
+class LearnedNet:
+    """The learned correction (hecmw33_net_*): per-node MLP 16 -> 64 -> 64 -> 3
+    (ReLU, no biases) with BF16 MFMA kernels, trained on the GPU."""
+
+    def __init__(self, seed=0):
+        self._h = lib().hecmw33_net_create(seed)
+        if not self._h:
+            raise HecmwError("hecmw33_net_create failed: %s" % lib().hecmw33_last_error().decode())
+        self.nw = lib().hecmw33_net_weight_count()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().hecmw33_net_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def weights(self):
+        """(W1 [16,64], W2 [64,64], W3 [64,16]) FP32 master weights."""
+        w = np.zeros(self.nw, np.float32)
+        _check(lib().hecmw33_net_get_weights(self._h, _ptr(w), self.nw))
+        return w[:1024].reshape(16, 64), w[1024:5120].reshape(64, 64), w[5120:].reshape(64, 16)
+
+    def set_weights(self, W1, W2, W3):
+        w = np.concatenate([np.asarray(W1, np.float32).ravel(), np.asarray(W2, np.float32).ravel(),
+                            np.asarray(W3, np.float32).ravel()])
+        _check(lib().hecmw33_net_set_weights(self._h, _ptr(w), self.nw))
+
+    def eval(self, X, dY=None):
+        """Y = N(X) (n x 3); with dY also the weight gradients of sum(dY * Y)."""
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        n = X.shape[0]
+        Y = np.zeros((n, 3), np.float32)
+        dW = np.zeros(self.nw, np.float32) if dY is not None else None
+        dYc = np.ascontiguousarray(dY, dtype=np.float32) if dY is not None else None
+        _check(lib().hecmw33_net_eval(self._h, n, _ptr(X), _ptr(dYc), _ptr(Y), _ptr(dW)))
+        if dY is None:
+            return Y
+        return Y, (dW[:1024].reshape(16, 64), dW[1024:5120].reshape(64, 64), dW[5120:].reshape(64, 16))
+
+    def add_sample(self, ctx, r_ptr, sigma_diag=1.0):
+        return _check(lib().hecmw33_net_add_sample(self._h, ctx._h, C.c_void_p(r_ptr), sigma_diag))
+
+    def clear_samples(self):
+        _check(lib().hecmw33_net_clear_samples(self._h))
+
+    def train(self, epochs, lr=1e-3, seed=0):
+        loss = np.zeros(epochs + 1)
+        _check(lib().hecmw33_net_train(self._h, epochs, lr, seed, _ptr(loss)))
+        return loss
+
+    def score(self):
+        m = C.c_double(0.0)
+        b = C.c_double(0.0)
+        _check(lib().hecmw33_net_score(self._h, C.byref(m), C.byref(b)))
+        return m.value, b.value
+
+
+def set_net(net):
+    """Attach a LearnedNet to the default context (hecmw_solve_iterative / _krylov)."""
+    _check(lib().hecmw33_set_net(net._h if net is not None else None))
+
+
+def harvest_residuals(net, ctx, mat, b_dev, iters=(1, 2, 3, 5, 8, 13, 21, 34, 55, 89)):
+    """Training residuals from the solver itself: r_k = b - A x_k after k
+    FP64 SSOR-CG iterations from x0 = 0 (the context keeps its loaded
+    preconditioner), normalised, added to the network's samples.  b_dev: a
+    torch float64 device tensor of 3*NP entries.  Returns the sample count."""
+    import torch
+    ia = mat.Iarray.copy()
+    ia[mat.I_FLAG_NUMFACT - 1] = 0
+    ia[mat.I_FLAG_SYMBFACT - 1] = 0
+    ia[mat.I_SOLVER_OPT - 1] = 0
+    x = torch.zeros_like(b_dev)
+    y = torch.zeros_like(b_dev)
+    n3 = 3 * mat.N
+    count = 0
+    for k in iters:
+        ia[mat.I_ITER - 1] = k
+        x.zero_()
+        torch.cuda.synchronize()
+        res = ctx.solve_device(b_dev.data_ptr(), x.data_ptr(), ia, mat.Rarray)
+        if res.error not in (0, 3001):
+            continue
+        ctx.matvec_device(x.data_ptr(), y.data_ptr())
+        r = b_dev - y
+        r[n3:] = 0.0
+        nr = float(torch.linalg.vector_norm(r))
+        if not np.isfinite(nr) or nr == 0.0:
+            continue
+        r /= nr
+        torch.cuda.synchronize()
+        count = net.add_sample(ctx, r.data_ptr(), float(mat.Rarray[mat.R_SIGMA_DIAG - 1]))
+    return count

@@ -142,7 +142,9 @@ int32_t hecmw33_ctx_matvec_device_f32(hecmw33_ctx* ctx, const float* x_dev, floa
 /* Residual history of the last solve (ITERLOG values), up to cap entries. */
 int32_t hecmw33_ctx_history(hecmw33_ctx* ctx, double* hist, int32_t cap);
 /* Context figures: [0] SELL-64 slots of A (incl. padding), [1] N, [2] NPL, [3] NPU, [4] colours,
- * [5] SELL-16 slots of L' + U', [6] FP64 residual replacements of the last flexible CG solve. */
+ * [5] SSOR slots of L' + U', [6] FP64 residual replacements of the last flexible CG solve,
+ * [7] iterations of the last learned solve that fell back to the SSOR output, [8] 1 if that
+ * solve dropped the learned correction (or was not a learned solve). */
 int32_t hecmw33_ctx_info(hecmw33_ctx* ctx, int64_t* info, int32_t n);
 
 /* ---- batched solves (no counterpart in the reference: hecmw_solve_CG handles one system and
@@ -174,6 +176,34 @@ const char* hecmw33_batch_last_error(void);
 int32_t hecmw33_solve_batch(int32_t nsys, const hecmw33_sys* sys, int32_t nrhs, const double* const* B,
                             double* const* X, const int32_t iarray[100], const double rarray[100], int32_t* iter,
                             int32_t* error, double* resid);
+
+/* ---- learned correction (no counterpart in the reference) -----------------------------------
+ * A per-node MLP (16 node-local features linear in the residual -> 64 -> 64 -> 3, ReLU, no
+ * biases; BF16 MFMA kernels with FP32 accumulation) whose output is added to the FP32 SSOR
+ * output inside the flexible CG: iarray[40] (Iarray 41) = 2 with an attached network
+ * (hecmw33_ctx_set_net for a device context, hecmw33_set_net for hecmw33_solve /
+ * hecmw33_solve_krylov).  Per iteration the plain SSOR output is used instead when the learned
+ * one is not a descent direction; the correction is dropped for the rest of a solve when the
+ * FP64 residual grows between two residual replacements.  Trained on the GPU on the energy
+ * functional (z.Az - 2 z.r)/(r.z0) of residuals harvested on device contexts.
+ * A network must outlive the contexts it is attached to; contexts must outlive its samples. */
+typedef struct hecmw33_net hecmw33_net;
+hecmw33_net* hecmw33_net_create(uint64_t seed);
+void hecmw33_net_destroy(hecmw33_net* net);
+/* FP32 master weights: W1 [16][64], W2 [64][64], W3 [64][16] row-major (W3 columns 3..15 zero) */
+int32_t hecmw33_net_weight_count(void);
+int32_t hecmw33_net_get_weights(hecmw33_net* net, float* w, int32_t n);
+int32_t hecmw33_net_set_weights(hecmw33_net* net, const float* w, int32_t n);
+/* Y = N(X) for n x 16 features (host); with dY (n x 3) also dW = dL/dW for L = sum dY.Y */
+int32_t hecmw33_net_eval(hecmw33_net* net, int64_t n, const float* X, const float* dY, float* Y, float* dW);
+/* residual r (device, FP64, 3*NP) on a context loaded with an SSOR set-up; returns the sample count */
+int32_t hecmw33_net_add_sample(hecmw33_net* net, hecmw33_ctx* ctx, const double* r_dev, double sigma_diag);
+int32_t hecmw33_net_clear_samples(hecmw33_net* net);
+/* loss[0] = mean energy of the plain SSOR output, loss[1..epochs] = mean per epoch */
+int32_t hecmw33_net_train(hecmw33_net* net, int32_t epochs, double lr, uint64_t seed, double* loss);
+int32_t hecmw33_net_score(hecmw33_net* net, double* mean, double* base);
+int32_t hecmw33_ctx_set_net(hecmw33_ctx* ctx, hecmw33_net* net);
+int32_t hecmw33_set_net(hecmw33_net* net);
 
 /* ---- synthetic systems (fistr1's assembly for a structured HEX8 cube) ------- */
 int32_t hecmw33_gen_hex8_count(int32_t n, int32_t* N, int32_t* NPL, int32_t* NPU);

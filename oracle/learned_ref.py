@@ -1,0 +1,78 @@
+"""CPU restatement of the learned correction's network (test infrastructure
+only: tests and the benchmark's checks import it, the product never does).
+
+The network of frontistr_amd/csrc/learned.hip: per node, features x (16) ->
+H1 = relu(x W1) (64) -> H2 = relu(H1 W2) (64) -> y = H2 W3[:, :3]; no biases.
+The GPU kernels round their MFMA operands to BF16 (round to nearest even):
+the features, the weights, the post-ReLU activations, the upstream gradient
+and the back-propagated activation gradients; products accumulate in FP32.
+`forward`/`backward` here apply the same roundings and accumulate in FP64,
+so they differ from the GPU only by the FP32 accumulation order (and the rare
+BF16 rounding flip that order can cause).
+
+`forward64`/`backward64` are the same formulas without rounding, pinned by
+finite differences in tests/test_learned_oracle.py.  There is no reference
+counterpart: FrontISTR has no learned preconditioner.
+"""
+import numpy as np
+
+
+def bf16(a):
+    """Round float32 values to bfloat16 (nearest even), returned as float32."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    lsb = (u >> 16) & 1
+    r = ((u + 0x7FFF + lsb) >> 16) << 16
+    out = r.astype(np.uint32).view(np.float32)
+    nan = np.isnan(a)
+    if nan.any():
+        out = out.copy()
+        out[nan] = np.nan
+    return out
+
+
+def relu(a):
+    return np.maximum(a, 0.0)
+
+
+def forward(X, W1, W2, W3):
+    """Returns (Y [n,3], cache) with the GPU's BF16 rounding points."""
+    Xb, W1b, W2b, W3b = (bf16(X).astype(np.float64), bf16(W1).astype(np.float64), bf16(W2).astype(np.float64),
+                         bf16(W3).astype(np.float64))
+    H1 = bf16(relu(Xb @ W1b)).astype(np.float64)
+    H2 = bf16(relu(H1 @ W2b)).astype(np.float64)
+    Y = H2 @ W3b
+    return Y[:, :3], (Xb, W1b, W2b, W3b, H1, H2)
+
+
+def backward(dY, cache):
+    """Weight gradients of sum(dY * Y) with the GPU's rounding points."""
+    Xb, W1b, W2b, W3b, H1, H2 = cache
+    n = dY.shape[0]
+    dYp = np.zeros((n, W3b.shape[1]))
+    dYp[:, :3] = bf16(dY).astype(np.float64)
+    dH2 = bf16((dYp @ W3b.T) * (H2 > 0)).astype(np.float64)
+    dH1 = bf16((dH2 @ W2b.T) * (H1 > 0)).astype(np.float64)
+    return Xb.T @ dH1, H1.T @ dH2, H2.T @ dYp
+
+
+def forward64(X, W1, W2, W3):
+    H1p = X @ W1
+    H1 = relu(H1p)
+    H2p = H1 @ W2
+    H2 = relu(H2p)
+    return (H2 @ W3)[:, :3], (X, W1, W2, W3, H1p, H1, H2p, H2)
+
+
+def backward64(dY, cache):
+    X, W1, W2, W3, H1p, H1, H2p, H2 = cache
+    dYp = np.zeros((dY.shape[0], W3.shape[1]))
+    dYp[:, :3] = dY
+    dH2 = (dYp @ W3.T) * (H2p > 0)
+    dH1 = (dH2 @ W2.T) * (H1p > 0)
+    return X.T @ dH1, H1.T @ dH2, H2.T @ dYp
+
+
+def energy(z, Az, r, z0):
+    """The training functional (z.Az - 2 z.r) / (r.z0)."""
+    return (z @ Az - 2.0 * (z @ r)) / (r @ z0)
