@@ -56,6 +56,8 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=100)
     ap.add_argument("--c3-n", type=int, default=160, help="mixed-precision comparison cube (0: skip)")
     ap.add_argument("--c4-nsys", type=int, default=48, help="batched TET10 systems (0: skip)")
+    ap.add_argument("--c5-n", type=int, default=120, help="learned-correction cube (0: skip)")
+    ap.add_argument("--c5-variants", type=int, default=32, help="training variants for the learned correction")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus %d requested); multi-GPU sharding is "
@@ -137,6 +139,14 @@ def main():
     ms32 = ctx.matvec_device(x32.data_ptr(), y32.data_ptr(), args.spmv_reps, fp32=True)
     gbs64 = bytes64 / (ms64 * 1e-3) / 1e9
     gbs32 = spmv_bytes(m, 4) / (ms32 * 1e-3) / 1e9
+    # a plain copy of known size (1 GiB read + 1 GiB written): the reference
+    # point that scales the L2 traffic counters of a profiled run
+    cpy_src = torch.empty(1 << 27, dtype=torch.float64, device=dev).uniform_()
+    cpy_dst = torch.empty_like(cpy_src)
+    for _ in range(3):
+        cpy_dst.copy_(cpy_src)
+    torch.cuda.synchronize()
+    del cpy_src, cpy_dst
     log("SpMV FP64 %.3f ms %.0f GB/s; FP32 %.3f ms %.0f GB/s; in-solve SpMV %.3f ms %.0f GB/s" % (
         ms64, gbs64, ms32, gbs32, ax_ms, achieved))
 
@@ -163,6 +173,8 @@ def main():
     mixed = bench_mixed(args, dev) if args.c3_n > 0 else None
     torch.cuda.empty_cache()
     batch = bench_batch(args) if args.c4_nsys > 0 else None
+    torch.cuda.empty_cache()
+    learned = bench_learned(args, dev) if args.c5_n > 0 else None
 
     out = {
         "metric": "time to 1e-8 rel. residual: mixed+learned FCG vs plain CG; BCSR SpMV GB/s",
@@ -198,6 +210,7 @@ def main():
         "cpu_baseline": cpu,
         "mixed": mixed,
         "batch": batch,
+        "learned": learned,
     }
     print(json.dumps(out), flush=True)
 
@@ -254,6 +267,97 @@ def bench_mixed(args, dev):
             "speedup": round(res["fp64"]["seconds"] / res["mixed"]["seconds"], 3),
             "rel_err_mixed_vs_fp64": err, "load_s": round(t_load, 2),
             "learned": None}
+
+
+def _inclusions(fa, n, seed):
+    m = fa.gen_hex8(n, kind="inclusions", seed=seed, E=1000.0, nu=0.3, contrast=1e4, rmin=2, rmax=6, vfrac=0.15)
+    m.set_iter(30000)
+    m.set_precond(1)
+    m.set_i(m.I_TIMELOG, 1)
+    return m
+
+
+def bench_learned(args, dev):
+    """Two-phase inclusion cube (E contrast 1e4, 15% volume, radii 2-6
+    elements).  The network is trained on residuals the solver harvests from
+    FP64 SSOR-CG runs on seeded variants (smaller cubes with the same
+    inclusion statistics, 7 load cases x 9 iteration counts each), then the
+    learned flexible CG, the plain flexible CG (FP32 operators) and the FP64
+    SSOR-CG solve the target cube from x0 = 0 to 1e-8."""
+    import torch
+    import frontistr_amd as fa
+    t0 = time.time()
+    net = fa.LearnedNet(seed=0)
+    keep = []
+    rng = np.random.default_rng(123)
+    nv = args.c5_variants
+    for v in range(nv):
+        m = _inclusions(fa, 24, 1000 + v)
+        ctx = fa.DeviceContext()
+        ctx.load(m)
+        top = np.nonzero(m.B)[0]
+        for lc in range(7):
+            # the pressure load, then random nodal loads on the loaded face
+            bvec = m.B.copy()
+            if lc > 0:
+                bvec[:] = 0.0
+                bvec[top] = rng.standard_normal(top.size) * np.abs(m.B[top]).mean()
+            b = torch.from_numpy(bvec).to(dev)
+            count = fa.harvest_residuals(net, ctx, m, b)
+        keep.append(ctx)
+    t_harvest = time.time() - t0
+    t0 = time.time()
+    loss = net.train(epochs=4, lr=2e-3, seed=0)
+    t_train = time.time() - t0
+    log("C5: %d residuals from %d variants in %.1fs; training %.1fs, energy %s" % (
+        count, nv, t_harvest, t_train, np.round(loss, 4)))
+    t0 = time.time()
+    m = _inclusions(fa, args.c5_n, 7)
+    ctx = fa.DeviceContext()
+    ctx.load(m)
+    net.clear_samples()
+    b = torch.from_numpy(m.B).to(dev)
+    ctx.set_net(net)
+    x = torch.zeros(3 * m.NP, dtype=torch.float64, device=dev)
+    res, xs = {}, {}
+    for mode, opt in (("fp64_ssor_cg", 0), ("mixed_fcg", 1), ("learned_fcg", 2)):
+        ia = m.Iarray.copy()
+        ia[96] = 1
+        ia[97] = 0
+        ia[40] = opt
+        x.zero_()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        r = ctx.solve_device(b.data_ptr(), x.data_ptr(), ia, m.Rarray)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t1
+        xs[mode] = x.cpu().numpy()
+        m.X[:] = xs[mode]
+        inf = ctx.info()
+        res[mode] = {"seconds": round(wall, 4), "iterations": r.iter, "error": r.error,
+                     "ms_per_iter": round(1e3 * r.Tsol / max(1, r.iter), 4),
+                     "rel_resid_true": fa.hecmw_rel_resid_L2(m)}
+        if opt == 2:
+            res[mode]["fallback_iterations"] = inf["nfallback"]
+            res[mode]["correction_dropped"] = bool(inf["learned_off"])
+        log("C5 %s: %s" % (mode, res[mode]))
+    ctx.set_net(None)
+    ref_x = xs["fp64_ssor_cg"]
+    out = {"workload": "%d^3 HEX8, two-phase inclusions (E 1e3 / 1e7, 15%% volume, radii 2-6 elements, seed 7), "
+                       "%d DOF; network trained on %d residuals harvested from SSOR-CG on %d seeded 24^3 variants"
+                       % (args.c5_n, 3 * m.N, count, nv),
+           "train": {"harvest_s": round(t_harvest, 2), "train_s": round(t_train, 2),
+                     "energy_plain_ssor": loss[0], "energy_per_epoch": [float(v) for v in loss[1:]]},
+           "speedup_learned_vs_fp64": round(res["fp64_ssor_cg"]["seconds"] / res["learned_fcg"]["seconds"], 3),
+           "speedup_mixed_vs_fp64": round(res["fp64_ssor_cg"]["seconds"] / res["mixed_fcg"]["seconds"], 3)}
+    for mode in res:
+        res[mode]["rel_err_vs_fp64"] = float(np.linalg.norm(xs[mode] - ref_x) / np.linalg.norm(ref_x))
+        out[mode] = res[mode]
+    ctx.close()
+    for c in keep:
+        c.close()
+    net.close()
+    return out
 
 
 def bench_batch(args):

@@ -167,6 +167,21 @@ __global__ __launch_bounds__(kBT) void kb_update(Tiles T, const float* __restric
 }
 
 // ---- SpMM ----------------------------------------------------------------------------
+// Tile of this workgroup for the SpMM: runs of kSpmmRun consecutive tiles are
+// dealt round-robin to the 8 XCDs (workgroup b runs on XCD b % 8), so one XCD
+// sweeps contiguous rows whose x neighbourhoods stay in its L2, while the
+// tiles of any one system still spread over every XCD.  Tiles past the last
+// whole round keep the identity map.
+constexpr int kSpmmRun = 64;
+__device__ __forceinline__ int spmm_tile() {
+  const int b = blockIdx.x, nb = gridDim.x;
+  const int round = kNumXcd * kSpmmRun;
+  const int whole = (nb / round) * round;
+  if (b >= whole) return b;
+  const int xcd = b % kNumXcd, l = b / kNumXcd;
+  return ((l / kSpmmRun) * kNumXcd + xcd) * kSpmmRun + (l % kSpmmRun);
+}
+
 // A row is handled by LPR lanes: G = 16/RPL right-hand-side groups (RPL rhs
 // per lane) times NPH = LPR/G phases that take every NPH-th block of the row;
 // phases are summed with lane shuffles.  U blocks per lane are in flight.
@@ -184,7 +199,7 @@ __global__ __launch_bounds__(kBT) void kb_spmm(Tiles TL, const int32_t* __restri
 #pragma clang fp contract(off)
   constexpr int G = KS / RPL, NPH = LPR / G, RPW = 64 / LPR;
   constexpr int ND = EPI == kbEpiQ ? 2 : (EPI == kbEpiRes ? 1 : 0);
-  const int t = blockIdx.x;
+  const int t = spmm_tile();
   const int s = TL.sys[t];
   if (guard == 1 && S.sys_active[s] == 0) return;
   if (guard == 2 && S.sys_rel[s] == 0) return;

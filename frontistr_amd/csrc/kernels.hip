@@ -335,7 +335,12 @@ __device__ __forceinline__ float quad_bcast(float v) {
 // order: the D block first, then L, then U in storage order;
 // y = ((y + a1*x1) + a2*x2) + a3*x3, no fused multiply-add.  A chunk of U
 // blocks is loaded before it is accumulated in order.  xo = x of the row itself.
-template <int U>
+template <typename T>
+__device__ __forceinline__ T ld_stream(const T* p, bool nt) {
+  return nt ? __builtin_nontemporal_load(p) : *p;
+}
+
+template <int U, bool NT = false>
 __device__ __forceinline__ void sell_row(const Sell<double>& a, int s, int lane, const double* __restrict__ x,
                                          double (&y)[3], double (&xo)[3]) {
 #pragma clang fp contract(off)
@@ -350,7 +355,7 @@ __device__ __forceinline__ void sell_row(const Sell<double>& a, int s, int lane,
     const int c = colp[0];
     double v[9];
 #pragma unroll
-    for (int e = 0; e < 9; ++e) v[e] = vp[e * kRowsA];
+    for (int e = 0; e < 9; ++e) v[e] = ld_stream(vp + e * kRowsA, NT);
     const double x0 = x[3 * (int64_t)c], x1 = x[3 * (int64_t)c + 1], x2 = x[3 * (int64_t)c + 2];
     xo[0] = x0;
     xo[1] = x1;
@@ -373,7 +378,7 @@ __device__ __forceinline__ void sell_row(const Sell<double>& a, int s, int lane,
     for (int j = 0; j < U; ++j) {
       const double* __restrict__ vk = vp + (int64_t)kk[j] * 9 * kRowsA;
 #pragma unroll
-      for (int e = 0; e < 9; ++e) v[j][e] = vk[e * kRowsA];
+      for (int e = 0; e < 9; ++e) v[j][e] = ld_stream(vk + e * kRowsA, NT);
       xv[j][0] = x[3 * (int64_t)c[j]];
       xv[j][1] = x[3 * (int64_t)c[j] + 1];
       xv[j][2] = x[3 * (int64_t)c[j] + 2];
@@ -466,7 +471,7 @@ enum SpmvEpi : int {
   kEpiTrue = 5,    // y = b - A x, partial y.y, only in residual-recompute iterations
 };
 
-template <int EPI>
+template <int EPI, int U = kSpmvChunk, bool NT = false>
 __global__ __launch_bounds__(kBlock) void k_spmv(Sell<double> a, const double* __restrict__ x,
                                                    const double* __restrict__ bw, double* __restrict__ y,
                                                    const KrylovState* st, double* part) {
@@ -483,7 +488,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(Sell<double> a, const double* _
   for (int k = 0; k < K; ++k) acc[k] = 0.0;
   for (int s = sr.begin; s < sr.end; ++s) {
     double yv[3], xo[3];
-    sell_row<kSpmvChunk>(a, s, lane, x, yv, xo);
+    sell_row<U, NT>(a, s, lane, x, yv, xo);
     const int64_t row = (int64_t)s * kRowsA + lane;
     if (row < a.nrows) {
       double o[3];
@@ -1120,14 +1125,36 @@ __global__ void k_block_lu(int32_t nslots, const int32_t* __restrict__ row_src, 
 }  // namespace
 
 // ---- launchers ---------------------------------------------------------------------
+// FP64 SpMV shape: blocks in flight per lane (HECMW33_SPMV_CHUNK = 4 or 8) and
+// non-temporal value loads (HECMW33_SPMV_NT = 1); same results either way.
+static int spmv_variant() {
+  static int v = -1;
+  if (v < 0) {
+    const char* c = getenv("HECMW33_SPMV_CHUNK");
+    const char* n = getenv("HECMW33_SPMV_NT");
+    v = ((c && atoi(c) == 8) ? 1 : 0) | ((n && atoi(n) == 1) ? 2 : 0);
+  }
+  return v;
+}
+template <int EPI>
+static void spmv_launch(int g, hipStream_t s, const Sell<double>& a, const double* x, const double* bw, double* y,
+                        const KrylovState* st, double* part) {
+  switch (spmv_variant()) {
+    case 1: k_spmv<EPI, 8, false><<<g, kBlock, 0, s>>>(a, x, bw, y, st, part); break;
+    case 2: k_spmv<EPI, kSpmvChunk, true><<<g, kBlock, 0, s>>>(a, x, bw, y, st, part); break;
+    case 3: k_spmv<EPI, 8, true><<<g, kBlock, 0, s>>>(a, x, bw, y, st, part); break;
+    default: k_spmv<EPI><<<g, kBlock, 0, s>>>(a, x, bw, y, st, part); break;
+  }
+}
+
 void launch_spmv(const Sell<double>& a, const double* x, double* y, hipStream_t s) {
-  k_spmv<kEpiStore><<<spmv_grid(a.nslices), kBlock, 0, s>>>(a, x, nullptr, y, nullptr, nullptr);
+  spmv_launch<kEpiStore>(spmv_grid(a.nslices), s, a, x, nullptr, y, nullptr, nullptr);
 }
 void launch_spmv_f32(const Sell<float>& a, const float* x, float* y, hipStream_t s) {
   k_spmv_f32<false><<<spmv_grid(a.nslices), kBlock, 0, s>>>(a, x, nullptr, y, nullptr, nullptr);
 }
 void launch_matresid(const Sell<double>& a, const double* x, const double* b, double* r, hipStream_t s) {
-  k_spmv<kEpiResid><<<spmv_grid(a.nslices), kBlock, 0, s>>>(a, x, b, r, nullptr, nullptr);
+  spmv_launch<kEpiResid>(spmv_grid(a.nslices), s, a, x, b, r, nullptr, nullptr);
 }
 void launch_dot_store(int64_t n, const double* x, const double* y, double* out, KrylovState* st, double* part,
                       hipStream_t s) {
@@ -1249,7 +1276,7 @@ void launch_add(int64_t n, const double* x, double* y, const KrylovState* st, hi
 }
 void launch_matresid_guarded(const Sell<double>& a, const double* x, const double* b, double* r,
                              const KrylovState* st, hipStream_t s) {
-  k_spmv<kEpiResid><<<spmv_grid(a.nslices), kBlock, 0, s>>>(a, x, b, r, st, nullptr);
+  spmv_launch<kEpiResid>(spmv_grid(a.nslices), s, a, x, b, r, st, nullptr);
 }
 
 // ---- CG iteration pieces ------------------------------------------------------------
@@ -1279,7 +1306,7 @@ void cg_xpay(int64_t n3, const double* z, double* p, const KrylovState* st, hipS
 }
 void cg_spmv_dot(const Sell<double>& a, const double* p, double* q, KrylovState* st, double* part, hipStream_t s) {
   const int g = spmv_grid(a.nslices);
-  k_spmv<kEpiDotX><<<g, kBlock, 0, s>>>(a, p, nullptr, q, st, part);
+  spmv_launch<kEpiDotX>(g, s, a, p, nullptr, q, st, part);
   k_tail<kTailCgC1, 1><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, nullptr);
 }
 void cg_update(int64_t n3, const double* p, const double* q, double* x, double* r, KrylovState* st, double* part,
@@ -1293,7 +1320,7 @@ void cg_update(int64_t n3, const double* p, const double* q, double* x, double* 
 void cg_true_resid(const Sell<double>& a, const double* x, const double* b, double* r, KrylovState* st, double* part,
                    double* hist, hipStream_t s) {
   const int g = spmv_grid(a.nslices);
-  k_spmv<kEpiTrue><<<g, kBlock, 0, s>>>(a, x, b, r, st, part);
+  spmv_launch<kEpiTrue>(g, s, a, x, b, r, st, part);
   k_tail_true<<<1, kTailThreads, 0, s>>>(st, part, g, hist);
 }
 
@@ -1309,7 +1336,7 @@ void bi_p(int64_t n3, const double* r, const double* v, double* p, const KrylovS
 void bi_spmv_c2(const Sell<double>& a, const double* pt, const double* rt, double* v, KrylovState* st, double* part,
                 hipStream_t s) {
   const int g = spmv_grid(a.nslices);
-  k_spmv<kEpiDotW><<<g, kBlock, 0, s>>>(a, pt, rt, v, st, part);
+  spmv_launch<kEpiDotW>(g, s, a, pt, rt, v, st, part);
   k_tail<kTailBiC2, 1><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, nullptr);
 }
 void bi_s(int64_t n3, const double* v, const double* r, double* sv, const KrylovState* st, hipStream_t s) {
@@ -1318,7 +1345,7 @@ void bi_s(int64_t n3, const double* v, const double* r, double* sv, const Krylov
 void bi_spmv_omega(const Sell<double>& a, const double* stv, const double* sv, double* t, KrylovState* st,
                    double* part, hipStream_t s) {
   const int g = spmv_grid(a.nslices);
-  k_spmv<kEpiDot2W><<<g, kBlock, 0, s>>>(a, stv, sv, t, st, part);
+  spmv_launch<kEpiDot2W>(g, s, a, stv, sv, t, st, part);
   k_tail<kTailBiOmega, 2><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, nullptr);
 }
 void bi_update(int64_t n3, const double* pt, const double* stv, const double* t, const double* sv, double* x,
