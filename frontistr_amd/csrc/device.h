@@ -79,6 +79,7 @@ struct KrylovState {
   int32_t nreliable;   // FP64 true-residual updates so far
   // learned correction: off for good / used in this iteration / fallbacks to plain SSOR so far
   int32_t learned_off, learned_now, nfallback, pad2;
+  double edrop;  // sum of alpha (r.p): the decrease of |e|_A^2 over the iterations so far
 };
 
 constexpr int kMaxPartials = 65536;  // per reduction component
@@ -159,6 +160,9 @@ void fcg_p(int64_t n3, const float* z, float* p, const KrylovState* st, hipStrea
 // q = A32 p, pq = p.q, rp = r.p, alpha = rp / pq
 void fcg_spmv(const Sell<float>& a, const float* p, const float* r, float* q, KrylovState* st, double* part,
               hipStream_t s);
+// the same with the FP64 matrix values (FP32 vectors, FP64 accumulation)
+void fcg_spmv_mixed(const Sell<double>& a, const float* p, const float* r, float* q, KrylovState* st, double* part,
+                    hipStream_t s);
 // x64 += alpha p, r32 -= alpha q, |r|^2, reliable-update decision
 void fcg_update(int64_t n3, const float* p, const float* q, double* x, float* r, KrylovState* st, double* part,
                 double* hist, hipStream_t s);
@@ -182,10 +186,11 @@ void fcg_p_sel(int64_t n3, const float* z, const float* z0, float* p, const Kryl
 
 // ---- learned correction (learned.hip) -------------------------------------------------
 int net_weight_count();
-void net_apply(int64_t n, const __bf16* Wb, const float* r, const float* z0, float* z, const float* ajac,
-               const float* mat, const float* q, const KrylovState* st, double* part, int* nparts, hipStream_t s);
-void net_features(int64_t n, const float* r, const float* z0, const float* ajac, const float* mat, __bf16* X,
-                  hipStream_t s);
+void net_static_features(const Sell<double>& A, int64_t n, const float* mat, __bf16* X, hipStream_t s);
+void learned_combine(int64_t n, const float* coef, const float* r, const float* z0, const float* ajac, const float* q,
+                     float* z, const KrylovState* st, double* part, int* nparts, hipStream_t s);
+void learned_combine_train(int64_t n, const float* Y, const float* z0, const float* jr, float* z, hipStream_t s);
+void learned_coef_grad(int64_t n, const float* Y, const float* gz, const float* jr, float* dY, hipStream_t s);
 void net_forward(int64_t n, const __bf16* Wb, const __bf16* X, __bf16* H1, __bf16* H2, const float* z0, float* z,
                  hipStream_t s);
 int net_bwd_parts(int64_t n);

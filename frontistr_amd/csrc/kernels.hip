@@ -214,6 +214,7 @@ __device__ void run_tail(KrylovState* st, const double (&tot)[K], double* out, d
       st->done = 1;
     } else {
       st->alpha = st->rp / pq;
+      st->edrop += st->alpha * st->rp;
     }
   } else if (TAIL == kTailFcgRes) {
     st->dnrm2 = tot[0];
@@ -335,12 +336,7 @@ __device__ __forceinline__ float quad_bcast(float v) {
 // order: the D block first, then L, then U in storage order;
 // y = ((y + a1*x1) + a2*x2) + a3*x3, no fused multiply-add.  A chunk of U
 // blocks is loaded before it is accumulated in order.  xo = x of the row itself.
-template <typename T>
-__device__ __forceinline__ T ld_stream(const T* p, bool nt) {
-  return nt ? __builtin_nontemporal_load(p) : *p;
-}
-
-template <int U, bool NT = false>
+template <int U>
 __device__ __forceinline__ void sell_row(const Sell<double>& a, int s, int lane, const double* __restrict__ x,
                                          double (&y)[3], double (&xo)[3]) {
 #pragma clang fp contract(off)
@@ -355,7 +351,7 @@ __device__ __forceinline__ void sell_row(const Sell<double>& a, int s, int lane,
     const int c = colp[0];
     double v[9];
 #pragma unroll
-    for (int e = 0; e < 9; ++e) v[e] = ld_stream(vp + e * kRowsA, NT);
+    for (int e = 0; e < 9; ++e) v[e] = vp[e * kRowsA];
     const double x0 = x[3 * (int64_t)c], x1 = x[3 * (int64_t)c + 1], x2 = x[3 * (int64_t)c + 2];
     xo[0] = x0;
     xo[1] = x1;
@@ -378,7 +374,7 @@ __device__ __forceinline__ void sell_row(const Sell<double>& a, int s, int lane,
     for (int j = 0; j < U; ++j) {
       const double* __restrict__ vk = vp + (int64_t)kk[j] * 9 * kRowsA;
 #pragma unroll
-      for (int e = 0; e < 9; ++e) v[j][e] = ld_stream(vk + e * kRowsA, NT);
+      for (int e = 0; e < 9; ++e) v[j][e] = vk[e * kRowsA];
       xv[j][0] = x[3 * (int64_t)c[j]];
       xv[j][1] = x[3 * (int64_t)c[j] + 1];
       xv[j][2] = x[3 * (int64_t)c[j] + 2];
@@ -471,7 +467,7 @@ enum SpmvEpi : int {
   kEpiTrue = 5,    // y = b - A x, partial y.y, only in residual-recompute iterations
 };
 
-template <int EPI, int U = kSpmvChunk, bool NT = false>
+template <int EPI>
 __global__ __launch_bounds__(kBlock) void k_spmv(Sell<double> a, const double* __restrict__ x,
                                                    const double* __restrict__ bw, double* __restrict__ y,
                                                    const KrylovState* st, double* part) {
@@ -488,7 +484,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(Sell<double> a, const double* _
   for (int k = 0; k < K; ++k) acc[k] = 0.0;
   for (int s = sr.begin; s < sr.end; ++s) {
     double yv[3], xo[3];
-    sell_row<U, NT>(a, s, lane, x, yv, xo);
+    sell_row<kSpmvChunk>(a, s, lane, x, yv, xo);
     const int64_t row = (int64_t)s * kRowsA + lane;
     if (row < a.nrows) {
       double o[3];
@@ -534,6 +530,62 @@ __global__ __launch_bounds__(kBlock) void k_spmv_f32(Sell<float> a, const float*
     }
   }
   if (DOTS) block_partials<2>(acc, part, blockIdx.x);
+}
+
+// FP64 values, FP32 vectors, FP64 accumulation: q = A p for the flexible CG
+// when the FP32-rounded matrix is too inexact (high material contrast), with
+// the FP64 partials of p.q and r.p.
+__global__ __launch_bounds__(kBlock) void k_spmv_mixed(Sell<double> a, const float* __restrict__ x,
+                                                         const float* __restrict__ w, float* __restrict__ y,
+                                                         const KrylovState* st, double* part) {
+  if (st && st->done) return;
+  const int lane = lane_id();
+  const SliceRange sr = wave_slices(a.nslices);
+  double acc[2] = {0.0, 0.0};
+  for (int s = sr.begin; s < sr.end; ++s) {
+    const int base = a.ptr[s];
+    const int len = a.len[s * kRowsA + lane];
+    const int32_t* __restrict__ colp = a.col + (int64_t)base * kRowsA + lane;
+    const double* __restrict__ vp = a.val + (int64_t)base * 9 * kRowsA + lane;
+    double yv[3] = {0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < len; k0 += kSpmvChunk) {
+      int kk[kSpmvChunk], c[kSpmvChunk];
+#pragma unroll
+      for (int j = 0; j < kSpmvChunk; ++j) {
+        kk[j] = (k0 + j < len) ? k0 + j : len - 1;
+        c[j] = colp[(int64_t)kk[j] * kRowsA];
+      }
+      double v[kSpmvChunk][9], xv[kSpmvChunk][3];
+#pragma unroll
+      for (int j = 0; j < kSpmvChunk; ++j) {
+        const double* __restrict__ vk = vp + (int64_t)kk[j] * 9 * kRowsA;
+#pragma unroll
+        for (int e = 0; e < 9; ++e) v[j][e] = vk[e * kRowsA];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) xv[j][d] = (double)x[3 * (int64_t)c[j] + d];
+      }
+#pragma unroll
+      for (int j = 0; j < kSpmvChunk; ++j) {
+        const bool live = k0 + j < len;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const double t = fma(v[j][3 * d + 2], xv[j][2], fma(v[j][3 * d + 1], xv[j][1], fma(v[j][3 * d], xv[j][0], yv[d])));
+          yv[d] = live ? t : yv[d];
+        }
+      }
+    }
+    const int64_t row = (int64_t)s * kRowsA + lane;
+    if (row < a.nrows) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        y[3 * row + d] = (float)yv[d];
+        const double pv = x[3 * row + d];
+        acc[0] += pv * yv[d];
+        acc[1] += (double)w[3 * row + d] * pv;
+      }
+    }
+  }
+  block_partials<2>(acc, part, blockIdx.x);
 }
 
 // r32 = (float)(b - A x) with A, x, b in FP64 and the partials of |b - A x|^2;
@@ -1125,26 +1177,10 @@ __global__ void k_block_lu(int32_t nslots, const int32_t* __restrict__ row_src, 
 }  // namespace
 
 // ---- launchers ---------------------------------------------------------------------
-// FP64 SpMV shape: blocks in flight per lane (HECMW33_SPMV_CHUNK = 4 or 8) and
-// non-temporal value loads (HECMW33_SPMV_NT = 1); same results either way.
-static int spmv_variant() {
-  static int v = -1;
-  if (v < 0) {
-    const char* c = getenv("HECMW33_SPMV_CHUNK");
-    const char* n = getenv("HECMW33_SPMV_NT");
-    v = ((c && atoi(c) == 8) ? 1 : 0) | ((n && atoi(n) == 1) ? 2 : 0);
-  }
-  return v;
-}
 template <int EPI>
 static void spmv_launch(int g, hipStream_t s, const Sell<double>& a, const double* x, const double* bw, double* y,
                         const KrylovState* st, double* part) {
-  switch (spmv_variant()) {
-    case 1: k_spmv<EPI, 8, false><<<g, kBlock, 0, s>>>(a, x, bw, y, st, part); break;
-    case 2: k_spmv<EPI, kSpmvChunk, true><<<g, kBlock, 0, s>>>(a, x, bw, y, st, part); break;
-    case 3: k_spmv<EPI, 8, true><<<g, kBlock, 0, s>>>(a, x, bw, y, st, part); break;
-    default: k_spmv<EPI><<<g, kBlock, 0, s>>>(a, x, bw, y, st, part); break;
-  }
+  k_spmv<EPI><<<g, kBlock, 0, s>>>(a, x, bw, y, st, part);
 }
 
 void launch_spmv(const Sell<double>& a, const double* x, double* y, hipStream_t s) {
@@ -1427,6 +1463,12 @@ void fcg_spmv(const Sell<float>& a, const float* p, const float* r, float* q, Kr
               hipStream_t s) {
   const int g = spmv_grid(a.nslices);
   k_spmv_f32<true><<<g, kBlock, 0, s>>>(a, p, r, q, st, part);
+  k_tail<kTailFcgAlpha, 2><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, nullptr);
+}
+void fcg_spmv_mixed(const Sell<double>& a, const float* p, const float* r, float* q, KrylovState* st, double* part,
+                    hipStream_t s) {
+  const int g = spmv_grid(a.nslices);
+  k_spmv_mixed<<<g, kBlock, 0, s>>>(a, p, r, q, st, part);
   k_tail<kTailFcgAlpha, 2><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, nullptr);
 }
 void fcg_update(int64_t n3, const float* p, const float* q, double* x, float* r, KrylovState* st, double* part,

@@ -4,20 +4,26 @@
 // No counterpart in the reference (its preconditioners are SSOR, DIAG and
 // the ones hecmw_precond_setup dispatches to, hecmw_precond.f90:30-58).
 //
-// Network: features x_i (16, node-local, linear in the residual r) ->
-// 64 -> ReLU -> 64 -> ReLU -> 3, no biases, so N(a x) = a N(x) for a > 0 and
-// the preconditioner scales with the residual.  Features of node i:
-//   z0_i (3)          the FP32 multicolour-SSOR output for r
-//   J r_i (3)         the block-Jacobi output (LU of D_ii, as DIAG)
-//   m_i z0_i, m_i J r_i (6)  both gated by m_i = tanh((ln tr D_ii - mean)/2),
-//                     a normalised local stiffness
-//   0 (4)             padding
-// The preconditioned residual is z = z0 + N(x).
+// The preconditioned residual is z = M r + B J r, i.e. z_i = z0_i + b_i (J r)_i:
+//   z0 = M r, the FP32 multicolour-SSOR output (M symmetric positive definite),
+//   J r = the block-Jacobi output (LU of D_ii, as DIAG),
+//   b_i = softplus(y_i - 4) > 0 from the network's output y_i = N(g_i)
+// for static node features g_i.  M + B J is symmetric positive definite for
+// every b >= 0 (J is block diagonal and B a scalar per block), and the
+// features do not depend on r, so the flexible CG runs as a preconditioned CG
+// on a fixed SPD operator.  (A correction that depends on r, or that scales
+// z0 node by node, is not symmetric: the flexible CG then lost its short
+// recurrence and needed several times the iterations.)  Features g_i (16):
+// m_i = tanh((ln tr D_ii - mean)/2), the mean, maximum and minimum of m over
+// the node's stencil neighbours, m_i^2, the local contrast max - min, 1 (a
+// bias), the stencil size / 27, zeros.
 //
-// Layouts (row-major [k][n], BF16): W1 [16][64], W2 [64][64], W3 [64][16]
-// (columns 3..15 zero).  A v_mfma_f32_16x16x32_bf16 tile takes, in lane l,
-// A[i = l%16][k = 8(l/16)+e] and B[k = 8(l/16)+e][j = l%16], e = 0..7, and
-// returns D[i = 4(l/16)+v][j = l%16], v = 0..3.
+// Network: 16 -> 64 -> ReLU -> 64 -> ReLU -> 3 (output 1 gives b; 0 and 2 are
+// unused), no biases beyond the constant feature.  Layouts (row-major [k][n],
+// BF16): W1 [16][64], W2 [64][64], W3 [64][16] (columns 3..15 zero).  A
+// v_mfma_f32_16x16x32_bf16 tile takes, in lane l, A[i = l%16][k = 8(l/16)+e]
+// and B[k = 8(l/16)+e][j = l%16], e = 0..7, and returns D[i = 4(l/16)+v][j =
+// l%16], v = 0..3.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -57,41 +63,6 @@ __device__ __forceinline__ bf16x8 load_bt(const __bf16* W, int K, int kb, int nt
 #pragma unroll
   for (int e = 0; e < 8; ++e) v[e] = (k0 + e < K) ? W[n * K + k0 + e] : (__bf16)0.0f;
   return v;
-}
-
-// node features (12 used, 16 with padding)
-__device__ __forceinline__ void node_features(int64_t i, const float* __restrict__ r, const float* __restrict__ z0,
-                                              const float* __restrict__ ajac, const float* __restrict__ mat,
-                                              float (&f)[16]) {
-#pragma clang fp contract(off)
-  const float* fa = ajac + (i / 64) * 9 * 64 + (i % 64);
-  float a[9];
-#pragma unroll
-  for (int e = 0; e < 9; ++e) a[e] = fa[e * 64];
-  float x0 = r[3 * i], x1 = r[3 * i + 1], x2 = r[3 * i + 2];
-  x1 = x1 - a[3] * x0;
-  x2 = x2 - a[6] * x0 - a[7] * x1;
-  x2 = a[8] * x2;
-  x1 = a[4] * (x1 - a[5] * x2);
-  x0 = a[0] * (x0 - a[2] * x2 - a[1] * x1);
-  const float m = mat[i];
-  const float z00 = z0[3 * i], z01 = z0[3 * i + 1], z02 = z0[3 * i + 2];
-  f[0] = z00; f[1] = z01; f[2] = z02;
-  f[3] = x0; f[4] = x1; f[5] = x2;
-  f[6] = m * z00; f[7] = m * z01; f[8] = m * z02;
-  f[9] = m * x0; f[10] = m * x1; f[11] = m * x2;
-  f[12] = f[13] = f[14] = f[15] = 0.0f;
-}
-
-// A operand of the 16-node feature tile for this lane: features 8(l/16)+e
-__device__ __forceinline__ bf16x8 feat_operand(const float (&f)[16], int g, bool valid) {
-  bf16x8 a;
-#pragma unroll
-  for (int e = 0; e < 8; ++e) {
-    const int k = 8 * g + e;
-    a[e] = (valid && k < kFt) ? (__bf16)f[k & 15] : (__bf16)0.0f;
-  }
-  return a;
 }
 
 // Forward of one 16-node tile from the layer-1 A operand; hs: this wave's
@@ -176,71 +147,110 @@ __device__ __forceinline__ void block_sum(double (&v)[K], double* part, int slot
     }
 }
 
-// ---- inference inside the flexible CG -------------------------------------------
-// z = z0 + N(x) (or z = z0 once the learned step is switched off), with the
-// FP64 partials of r.z, z.q_prev, r.z0, z0.q_prev.
-__global__ __launch_bounds__(kNT) void k_net_apply(int64_t n, NetW W, const float* __restrict__ r,
-                                                   const float* __restrict__ z0, float* __restrict__ z,
-                                                   const float* __restrict__ ajac, const float* __restrict__ mat,
-                                                   const float* __restrict__ q, const KrylovState* st, double* part) {
-  if (st && st->done) return;
-  const bool use_net = !(st && st->learned_off);
-  __shared__ __attribute__((aligned(16))) __bf16 hs[kNT / 64][16][kHid];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, j = lane & 15;
-  bf16x8 w1[4], w2[2][4], w3[2];
-  if (use_net) load_weights(W, lane, w1, w2, w3);
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  const int64_t ntile = (n + 15) / 16;
-  const int64_t per_pass = (int64_t)gridDim.x * (kNT / 64);
-  const int64_t npass = (ntile + per_pass - 1) / per_pass;
-  for (int64_t ps = 0; ps < npass; ++ps) {
-    const int64_t t = ps * per_pass + (int64_t)blockIdx.x * (kNT / 64) + wave;
-    const int64_t node0 = t * 16;
-    floatx4 c3 = {0.f, 0.f, 0.f, 0.f};
-    if (use_net) {
-      const int64_t mynode = node0 + j;
-      const bool valid = t < ntile && mynode < n;
-      float f[16];
-      if (valid) node_features(mynode, r, z0, ajac, mat, f);
-      else
-#pragma unroll
-        for (int e = 0; e < 16; ++e) f[e] = 0.f;
-      const bf16x8 a1 = feat_operand(f, g, valid);
-      c3 = mlp_tile(a1, w1, w2, w3, hs[wave], lane, node0, n, nullptr, nullptr);
-    }
-    if (j < 3 && t < ntile) {
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int64_t node = node0 + 4 * g + v;
-        if (node < n) {
-          const int64_t o = 3 * node + j;
-          const float zz0 = z0[o];
-          const float zz = use_net ? zz0 + c3[v] : zz0;
-          z[o] = zz;
-          const double rv = r[o], qv = q[o];
-          acc[0] += rv * zz;
-          acc[1] += (double)zz * qv;
-          acc[2] += rv * zz0;
-          acc[3] += (double)zz0 * qv;
-        }
-      }
-    }
-  }
-  block_sum<4>(acc, part, blockIdx.x);
-}
-
-// ---- training -------------------------------------------------------------------
-__global__ __launch_bounds__(kNT) void k_net_features(int64_t n, const float* __restrict__ r,
-                                                      const float* __restrict__ z0, const float* __restrict__ ajac,
-                                                      const float* __restrict__ mat, __bf16* __restrict__ X) {
+// ---- static features and the linear combination -------------------------------------
+// g_i from the node stiffness m and A's stencil (the SELL-64 structure of A:
+// block 0 of a row is its diagonal block)
+__global__ __launch_bounds__(kNT) void k_static_features(Sell<double> A, int64_t n, const float* __restrict__ mat,
+                                                         __bf16* __restrict__ X) {
   for (int64_t i = (int64_t)blockIdx.x * kNT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kNT) {
-    float f[16];
-    node_features(i, r, z0, ajac, mat, f);
+    const int64_t s = i / kRowsA, lane = i % kRowsA;
+    const int base = A.ptr[s];
+    const int len = A.len[s * kRowsA + lane];
+    const float mi = mat[i];
+    float sum = 0.0f, mx = mi, mn = mi;
+    for (int k = 1; k < len; ++k) {
+      const float mj = mat[A.col[((int64_t)base + k) * kRowsA + lane]];
+      sum += mj;
+      mx = fmaxf(mx, mj);
+      mn = fminf(mn, mj);
+    }
+    const float mean = len > 1 ? sum / (float)(len - 1) : mi;
+    const float f[16] = {mi, mean, mx, mn, mi * mi, mx - mn, 1.0f, (float)len / 27.0f, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
     for (int e = 0; e < 16; ++e) X[i * kFt + e] = (__bf16)f[e];
   }
 }
 
+// J r for node i with the DIAG factors (64-row planes)
+__device__ __forceinline__ void jacobi3(int64_t i, const float* __restrict__ ajac, const float* __restrict__ r,
+                                        float (&x)[3]) {
+#pragma clang fp contract(off)
+  const float* fa = ajac + (i / 64) * 9 * 64 + (i % 64);
+  float a[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) a[e] = fa[e * 64];
+  float x0 = r[3 * i], x1 = r[3 * i + 1], x2 = r[3 * i + 2];
+  x1 = x1 - a[3] * x0;
+  x2 = x2 - a[6] * x0 - a[7] * x1;
+  x2 = a[8] * x2;
+  x1 = a[4] * (x1 - a[5] * x2);
+  x0 = a[0] * (x0 - a[2] * x2 - a[1] * x1);
+  x[0] = x0;
+  x[1] = x1;
+  x[2] = x2;
+}
+
+__device__ __forceinline__ float coef_b(float y) {
+  const float t = y - 4.0f;
+  return t > 20.0f ? t : log1pf(expf(t));
+}
+__device__ __forceinline__ float coef_db(float y) { return 1.0f / (1.0f + expf(4.0f - y)); }
+
+// Inside the flexible CG: z = z0 + b J r (z = z0 once the learned step is
+// switched off), with the FP64 partials of r.z, z.q_prev, r.z0, z0.q_prev.
+__global__ __launch_bounds__(kNT) void k_learned_combine(int64_t n, const float* __restrict__ coef,
+                                                         const float* __restrict__ r, const float* __restrict__ z0,
+                                                         const float* __restrict__ ajac,
+                                                         const float* __restrict__ q, float* __restrict__ z,
+                                                         const KrylovState* st, double* part) {
+  if (st && st->done) return;
+  const bool use_net = !(st && st->learned_off);
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kNT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kNT) {
+    float jr[3] = {0.f, 0.f, 0.f};
+    float cb = 0.f;
+    if (use_net) {
+      jacobi3(i, ajac, r, jr);
+      cb = coef_b(coef[3 * i + 1]);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int64_t o = 3 * i + d;
+      const float zz0 = z0[o];
+      const float zz = use_net ? zz0 + cb * jr[d] : zz0;
+      z[o] = zz;
+      const double rv = r[o], qv = q[o];
+      acc[0] += rv * zz;
+      acc[1] += (double)zz * qv;
+      acc[2] += rv * zz0;
+      acc[3] += (double)zz0 * qv;
+    }
+  }
+  block_sum<4>(acc, part, blockIdx.x);
+}
+
+// training: z = z0 + b J r from the network output Y, and dL/dY from dL/dz
+__global__ void k_combine_train(int64_t n, const float* __restrict__ Y, const float* __restrict__ z0,
+                                const float* __restrict__ jr, float* __restrict__ z) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float b = coef_b(Y[3 * i + 1]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) z[3 * i + d] = z0[3 * i + d] + b * jr[3 * i + d];
+  }
+}
+__global__ void k_coef_grad(int64_t n, const float* __restrict__ Y, const float* __restrict__ gz,
+                            const float* __restrict__ jr, float* __restrict__ dY) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    float gb = 0.f;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) gb += gz[3 * i + d] * jr[3 * i + d];
+    dY[3 * i] = 0.0f;
+    dY[3 * i + 1] = gb * coef_db(Y[3 * i + 1]);
+    dY[3 * i + 2] = 0.0f;
+  }
+}
+
+// ---- training -------------------------------------------------------------------
 // forward on stored features; keeps H1, H2; writes z = z0 + N(x) (z0 may be
 // null: z = N(x), the raw network output)
 __global__ __launch_bounds__(kNT) void k_net_fwd(int64_t n, NetW W, const __bf16* __restrict__ X,
@@ -501,17 +511,23 @@ int grid_for(int64_t n, int per) {
 // ---- launchers ---------------------------------------------------------------------
 int net_weight_count() { return kWsz; }
 
-void net_apply(int64_t n, const __bf16* Wb, const float* r, const float* z0, float* z, const float* ajac,
-               const float* mat, const float* q, const KrylovState* st, double* part, int* nparts, hipStream_t s) {
-  const NetW W{Wb, Wb + kFt * kHid, Wb + kFt * kHid + kHid * kHid};
-  const int g = grid_for((n + 15) / 16, kNT / 64);
-  k_net_apply<<<g, kNT, 0, s>>>(n, W, r, z0, z, ajac, mat, q, st, part);
+void net_static_features(const Sell<double>& A, int64_t n, const float* mat, __bf16* X, hipStream_t s) {
+  k_static_features<<<grid_for(n, kNT), kNT, 0, s>>>(A, n, mat, X);
+}
+
+void learned_combine(int64_t n, const float* coef, const float* r, const float* z0, const float* ajac, const float* q,
+                     float* z, const KrylovState* st, double* part, int* nparts, hipStream_t s) {
+  const int g = grid_for(n, kNT);
+  k_learned_combine<<<g, kNT, 0, s>>>(n, coef, r, z0, ajac, q, z, st, part);
   *nparts = g;
 }
 
-void net_features(int64_t n, const float* r, const float* z0, const float* ajac, const float* mat, __bf16* X,
-                  hipStream_t s) {
-  k_net_features<<<grid_for(n, kNT), kNT, 0, s>>>(n, r, z0, ajac, mat, X);
+void learned_combine_train(int64_t n, const float* Y, const float* z0, const float* jr, float* z, hipStream_t s) {
+  k_combine_train<<<grid_for(n, 256), 256, 0, s>>>(n, Y, z0, jr, z);
+}
+
+void learned_coef_grad(int64_t n, const float* Y, const float* gz, const float* jr, float* dY, hipStream_t s) {
+  k_coef_grad<<<grid_for(n, 256), 256, 0, s>>>(n, Y, gz, jr, dY);
 }
 
 void net_forward(int64_t n, const __bf16* Wb, const __bf16* X, __bf16* H1, __bf16* H2, const float* z0, float* z,

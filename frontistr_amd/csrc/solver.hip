@@ -68,6 +68,15 @@ constexpr int32_t kOptMixedFcg = 1;
 // network added to the FP32 SSOR output (hecmw33_ctx_set_net / hecmw33_set_net)
 constexpr int32_t kOptLearnedFcg = 2;
 constexpr double kFcgDelta = 1.0e-2;
+// iterations of each probe run that decides whether a learned solve keeps
+// its correction
+constexpr int kLearnProbe = 25;
+// SOLVER_OPT(2) (Iarray 42) for the flexible CG's q = A p: 1 = FP32-rounded
+// values, 2 = FP64 values with FP32 vectors (FP64 accumulation), 0 = choose
+// from the stiffness contrast of the diagonal blocks (FP64 values above
+// kContrastFp64Spmv)
+constexpr int I_SOLVER_OPT2 = 42;
+constexpr double kContrastFp64Spmv = 1.0e4;
 static inline int32_t& IA(int32_t* a, int k) { return a[k - 1]; }
 static inline int32_t IA(const int32_t* a, int k) { return a[k - 1]; }
 static inline double RA(const double* a, int k) { return a[k - 1]; }
@@ -209,7 +218,7 @@ struct Context {
   // learned correction: attached network, plain SSOR output z0, node stiffness feature
   hecmw33_net* net = nullptr;
   bool mat_valid = false;
-  DMem vz0_32, vmat, scal;
+  DMem vz0_32, vmat, scal, vfeat, vcoef;  // vfeat: BF16 static node features, vcoef: network outputs per node
   // Krylov vectors and state
   DMem vb, vx, vr, vz, vp, vq, vrt, vs, vt, vv, vpt, vst, vzp, vzt;
   DMem st, part, hist;
@@ -218,6 +227,10 @@ struct Context {
   int32_t last_nreliable = 0;  // FP64 residual replacements of the last flexible CG solve
   int32_t last_nfallback = 0;  // iterations of the last learned solve that used the plain SSOR output
   int32_t last_learned_off = 1;  // the last learned solve switched the correction off
+  bool last_spmv64 = false;      // the last flexible CG used the FP64 matrix values in q = A p
+  double last_probe_ratio = 0.0; // error reduction with / without the learned correction in the last probe
+  DMem vx0;                      // x0 kept across the learned probe runs
+  double contrast = -1.0;        // max/min trace of the diagonal blocks (< 0: not computed for these values)
   // TIMELOG: event pairs around the preconditioner and the SpMV of every
   // enqueued iteration (4 events per iteration)
   std::vector<hipEvent_t> events;
@@ -241,7 +254,9 @@ struct Context {
 
   ~Context() {
     if (stream) hipStreamSynchronize(stream);
-    for (DMem* m : {&dIndexL, &dIndexU, &dItemL, &dItemU, &vscale, &vzs, &vzs32, &vz0_32, &vmat, &scal}) m->release();
+    for (DMem* m : {&dIndexL, &dIndexU, &dItemL, &dItemU, &vscale, &vzs, &vzs32, &vz0_32, &vmat, &scal, &vfeat, &vcoef,
+                    &vx0})
+      m->release();
     for (DMem* m : {&dD, &dAL, &dAU, &row_old, &alu_ssor, &row_diag, &alu_diag, &alu_ssor32, &alu_diag32, &vr32,
                     &vz32, &vp32, &vq32, &vb, &vx, &vr, &vz, &vp, &vq, &vrt, &vs, &vt, &vv, &vpt, &vst, &vzp, &vzt, &st,
                     &part, &hist})
@@ -257,16 +272,16 @@ struct Context {
 }  // namespace hecmw33
 
 // A learned correction: FP32 master weights, BF16 copies for the MFMA kernels,
-// Adam moments, and the training samples (residual, SSOR output, features)
-// harvested on device contexts, which must outlive the samples.
+// Adam moments, and the training samples (residual, SSOR output, block-Jacobi
+// output) harvested on device contexts, which must outlive the samples.
 struct hecmw33_net {
   struct Sample {
     hecmw33::Context* c = nullptr;
-    hecmw33::DMem r32, z0, X;
+    hecmw33::DMem r32, z0, jr;
     int64_t n = 0;
     double base = 0.0;  // energy functional of the plain SSOR output
   };
-  hecmw33::DMem Wm, Wb, M, V, dW, wpart, H1, H2, z, q, dY, dots, dpart, xb;
+  hecmw33::DMem Wm, Wb, M, V, dW, wpart, H1, H2, y, z, q, dY, dYc, dots, dpart, xb;
   hipStream_t stream = nullptr;
   int32_t step = 0;
   std::vector<Sample*> samples;
@@ -274,7 +289,7 @@ struct hecmw33_net {
     for (Sample* p : samples) {
       p->r32.release();
       p->z0.release();
-      p->X.release();
+      p->jr.release();
       delete p;
     }
     samples.clear();
@@ -282,7 +297,8 @@ struct hecmw33_net {
   ~hecmw33_net() {
     if (stream) hipStreamSynchronize(stream);
     clear_samples();
-    for (hecmw33::DMem* m : {&Wm, &Wb, &M, &V, &dW, &wpart, &H1, &H2, &z, &q, &dY, &dots, &dpart, &xb}) m->release();
+    for (hecmw33::DMem* m : {&Wm, &Wb, &M, &V, &dW, &wpart, &H1, &H2, &y, &z, &q, &dY, &dYc, &dots, &dpart, &xb})
+      m->release();
     if (stream) hipStreamDestroy(stream);
   }
 };
@@ -390,6 +406,36 @@ static int32_t load_values(Context& c, const hecmw33_sys* A) {
   HIPC(hipGetLastError());
   c.a32_valid = false;
   c.mat_valid = false;
+  c.contrast = -1.0;
+  return 0;
+}
+
+// Stiffness contrast of the system: max / min over nodes of |tr D_ii|,
+// leaving out the unit rows of fixed degrees of freedom (trace 3 after
+// hecmw_mat_ass_bc: all three fixed) -- computed on the host copy of D once
+// per value upload.
+static int32_t diag_contrast(Context& c, double* out) {
+  if (c.contrast >= 0.0) {
+    *out = c.contrast;
+    return 0;
+  }
+  std::vector<double> d(9 * (size_t)c.N);
+  HIPC(hipMemcpyAsync(d.data(), c.dD.p, sizeof(double) * d.size(), hipMemcpyDeviceToHost, c.stream));
+  HIPC(hipStreamSynchronize(c.stream));
+  double mn = 1e308, mx = 0.0;
+  for (int32_t i = 0; i < c.N; ++i) {
+    const double* b = &d[9 * (size_t)i];
+    const bool unit = b[0] == 1.0 && b[4] == 1.0 && b[8] == 1.0 && b[1] == 0.0 && b[2] == 0.0 && b[3] == 0.0 &&
+                      b[5] == 0.0 && b[6] == 0.0 && b[7] == 0.0;
+    if (unit) continue;
+    const double t = std::fabs(b[0]) + std::fabs(b[4]) + std::fabs(b[8]);
+    if (t > 0.0) {
+      mn = std::min(mn, t);
+      mx = std::max(mx, t);
+    }
+  }
+  c.contrast = mx > 0.0 ? mx / mn : 1.0;
+  *out = c.contrast;
   return 0;
 }
 
@@ -595,10 +641,13 @@ static int32_t learned_setup(Context& c, double sigma) {
   if (int32_t e = setup_f32(c, 1)) return e;
   if (int32_t e = setup_f32(c, 3)) return e;
   if (c.vz0_32.ensure(sizeof(float) * 3 * (size_t)c.NP) || c.vmat.ensure(sizeof(float) * (size_t)c.N) ||
-      c.scal.ensure(sizeof(double) * 8))
+      c.scal.ensure(sizeof(double) * 8) || c.vfeat.ensure(2 * 16 * (size_t)c.N) ||
+      c.vcoef.ensure(sizeof(float) * 3 * (size_t)c.N))
     return HECMW33_ERROR_DEVICE;
   if (!c.mat_valid) {
+    // node stiffness and the static features built on it (refreshed with the values)
     net_material(c.N, c.dD.as<double>(), c.vmat.as<float>(), c.part.as<double>(), c.scal.as<double>(), c.stream);
+    net_static_features(c.A.view, c.N, c.vmat.as<float>(), c.vfeat.as<__bf16>(), c.stream);
     c.mat_valid = true;
   }
   HIPC(hipGetLastError());
@@ -713,7 +762,7 @@ static void enqueue_bicg_iteration(Context& c, int32_t kind, int32_t iterpremax,
 // One flexible CG iteration with FP32 operators: z = M32^-1 r (fused r.z,
 // z.q_prev), p = z + beta p, q = A32 p (fused p.q, r.p), x += alpha p (FP64),
 // r -= alpha q, then, when the recurrence asks for it, r = b - A x in FP64.
-static void enqueue_fcg_iteration(Context& c, int32_t kind, hipEvent_t* ev) {
+static void enqueue_fcg_iteration(Context& c, int32_t kind, bool spmv64, hipEvent_t* ev) {
   const int64_t n3 = 3 * (int64_t)c.N;
   KrylovState* st = c.st.as<KrylovState>();
   double* part = c.part.as<double>();
@@ -723,16 +772,19 @@ static void enqueue_fcg_iteration(Context& c, int32_t kind, hipEvent_t* ev) {
   if (ev) hipEventRecord(ev[1], c.stream);
   fcg_p(n3, z, p, st, c.stream);
   if (ev) hipEventRecord(ev[2], c.stream);
-  fcg_spmv(c.A.view32, p, r, q, st, part, c.stream);
+  if (spmv64)
+    fcg_spmv_mixed(c.A.view, p, r, q, st, part, c.stream);
+  else
+    fcg_spmv(c.A.view32, p, r, q, st, part, c.stream);
   if (ev) hipEventRecord(ev[3], c.stream);
   fcg_update(n3, p, q, c.vx.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
   fcg_reliable(c.A.view, c.vx.as<double>(), c.vb.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
 }
 
-// One learned flexible CG iteration: z0 = M32^-1 r (FP32 SSOR), z = z0 + N(x)
+// One learned flexible CG iteration: z0 = M32^-1 r (FP32 SSOR), z = z0 + b J r
 // with the four dots, the tail keeps z or falls back to z0, then as the
 // flexible CG.
-static void enqueue_fcg_learned_iteration(Context& c, hipEvent_t* ev) {
+static void enqueue_fcg_learned_iteration(Context& c, bool spmv64, hipEvent_t* ev) {
   const int64_t n3 = 3 * (int64_t)c.N;
   KrylovState* st = c.st.as<KrylovState>();
   double* part = c.part.as<double>();
@@ -741,13 +793,15 @@ static void enqueue_fcg_learned_iteration(Context& c, hipEvent_t* ev) {
   if (ev) hipEventRecord(ev[0], c.stream);
   ssor_apply_f32(c.ssor_op32(), r, z0, st, c.stream);
   int np = 0;
-  net_apply(c.N, c.net->Wb.as<__bf16>(), r, z0, z, c.alu_diag32.as<float>(), c.vmat.as<float>(), q, st, part, &np,
-            c.stream);
+  learned_combine(c.N, c.vcoef.as<float>(), r, z0, c.alu_diag32.as<float>(), q, z, st, part, &np, c.stream);
   fcg_beta_learned(st, part, np, c.stream);
   if (ev) hipEventRecord(ev[1], c.stream);
   fcg_p_sel(n3, z, z0, p, st, c.stream);
   if (ev) hipEventRecord(ev[2], c.stream);
-  fcg_spmv(c.A.view32, p, r, q, st, part, c.stream);
+  if (spmv64)
+    fcg_spmv_mixed(c.A.view, p, r, q, st, part, c.stream);
+  else
+    fcg_spmv(c.A.view32, p, r, q, st, part, c.stream);
   if (ev) hipEventRecord(ev[3], c.stream);
   fcg_update(n3, p, q, c.vx.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
   fcg_reliable(c.A.view, c.vx.as<double>(), c.vb.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
@@ -840,8 +894,24 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
   const int32_t kind = pc_kind(pc, iterpremax);
   if (mixed)
     if (int32_t e = setup_f32(c, kind)) return e;
-  if (learned)
+  bool spmv64 = false;
+  if (mixed) {
+    const int32_t o2 = IA(iarray, I_SOLVER_OPT2);
+    if (o2 == 2) {
+      spmv64 = true;
+    } else if (o2 != 1) {
+      if (int32_t e = diag_contrast(c, &c.contrast)) return e;
+      spmv64 = c.contrast > kContrastFp64Spmv;
+    }
+  }
+  c.last_spmv64 = spmv64;
+  if (learned) {
     if (int32_t e = learned_setup(c, RA(rarray, R_SIGMA_DIAG))) return e;
+    // the network's coefficients for this system, fixed for the solve
+    net_forward(c.N, c.net->Wb.as<__bf16>(), c.vfeat.as<__bf16>(), nullptr, nullptr, nullptr, c.vcoef.as<float>(),
+                c.stream);
+    HIPC(hipGetLastError());
+  }
   const int64_t n3 = 3 * (int64_t)c.N;
   const size_t vbytes = sizeof(double) * 3 * (size_t)c.NP;
   KrylovState* st = c.st.as<KrylovState>();
@@ -908,9 +978,54 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
   h.learned_off = learned ? 0 : 1;
   h.learned_now = 0;
   h.nfallback = 0;
+  h.edrop = 0.0;
   c.h_st[1] = h;
   HIPC(hipMemcpyAsync(st, &c.h_st[1], sizeof(KrylovState), hipMemcpyHostToDevice, c.stream));
   HIPC(hipStreamSynchronize(c.stream));
+  if (learned) {
+    // "fails to reduce": kLearnProbe iterations from x0 with and without the
+    // correction; in (flexible) CG |e|_A^2 drops by alpha (r.p) per step, so
+    // the larger sum is the larger error reduction.  The solve then runs from
+    // x0 with the winner; plain SSOR stays the fallback.
+    if (c.vx0.ensure(vbytes)) return HECMW33_ERROR_DEVICE;
+    HIPC(hipMemcpyAsync(c.vx0.p, c.vx.p, vbytes, hipMemcpyDeviceToDevice, c.stream));
+    double drop[2] = {0.0, 0.0};
+    for (int pass = 0; pass < 2; ++pass) {
+      HIPC(hipMemcpyAsync(c.vx.p, c.vx0.p, vbytes, hipMemcpyDeviceToDevice, c.stream));
+      fcg_init_resid(c.A.view, c.vx.as<double>(), c.vb.as<double>(), c.vr32.as<float>(), &st->rr_ref, st, part,
+                     c.stream);
+      for (DMem* m : {&c.vz32, &c.vp32, &c.vq32}) HIPC(hipMemsetAsync(m->p, 0, vbytes / 2, c.stream));
+      HIPC(hipMemcpyAsync(&c.h_st[0], st, sizeof(KrylovState), hipMemcpyDeviceToHost, c.stream));
+      HIPC(hipStreamSynchronize(c.stream));
+      KrylovState hp = h;
+      hp.rr_ref = c.h_st[0].rr_ref;
+      hp.tol = 0.0;
+      hp.maxit = kLearnProbe + 1;
+      hp.learned_off = pass == 0 ? 0 : 1;
+      hp.edrop = 0.0;
+      c.h_st[1] = hp;
+      HIPC(hipMemcpyAsync(st, &c.h_st[1], sizeof(KrylovState), hipMemcpyHostToDevice, c.stream));
+      for (int i = 0; i < kLearnProbe; ++i) enqueue_fcg_learned_iteration(c, spmv64, nullptr);
+      HIPC(hipGetLastError());
+      HIPC(hipMemcpyAsync(&c.h_st[0], st, sizeof(KrylovState), hipMemcpyDeviceToHost, c.stream));
+      HIPC(hipStreamSynchronize(c.stream));
+      drop[pass] = c.h_st[0].error ? -1.0 : c.h_st[0].edrop;
+    }
+    h.learned_off = drop[0] > drop[1] ? 0 : 1;
+    c.last_probe_ratio = drop[1] > 0.0 ? drop[0] / drop[1] : 0.0;
+    // back to x0 and the initial residual for the solve itself
+    HIPC(hipMemcpyAsync(c.vx.p, c.vx0.p, vbytes, hipMemcpyDeviceToDevice, c.stream));
+    fcg_init_resid(c.A.view, c.vx.as<double>(), c.vb.as<double>(), c.vr32.as<float>(), &st->rr_ref, st, part,
+                   c.stream);
+    for (DMem* m : {&c.vz32, &c.vp32, &c.vq32}) HIPC(hipMemsetAsync(m->p, 0, vbytes / 2, c.stream));
+    HIPC(hipMemcpyAsync(&c.h_st[0], st, sizeof(KrylovState), hipMemcpyDeviceToHost, c.stream));
+    HIPC(hipStreamSynchronize(c.stream));
+    h.rr_ref = c.h_st[0].rr_ref;
+    h.edrop = 0.0;
+    c.h_st[1] = h;
+    HIPC(hipMemcpyAsync(st, &c.h_st[1], sizeof(KrylovState), hipMemcpyHostToDevice, c.stream));
+    HIPC(hipStreamSynchronize(c.stream));
+  }
   const double t1 = now_s();
   out->Tset = t1 - t0;
   // iterations: enqueue in chunks and poll the previous chunk's state while
@@ -932,9 +1047,9 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
         e = itev.data();
       }
       if (learned)
-        enqueue_fcg_learned_iteration(c, e);
+        enqueue_fcg_learned_iteration(c, spmv64, e);
       else if (mixed)
-        enqueue_fcg_iteration(c, kind, e);
+        enqueue_fcg_iteration(c, kind, spmv64, e);
       else if (method == 1)
         enqueue_cg_iteration(c, kind, iterpremax, e);
       else
@@ -1300,7 +1415,7 @@ int32_t hecmw33_ctx_history(hecmw33_ctx* ctx, double* hist, int32_t cap) {
 int32_t hecmw33_ctx_info(hecmw33_ctx* ctx, int64_t* info, int32_t n) {
   if (!ctx || !info) return HECMW33_ERROR_ARGS;
   Context& c = ctx->c;
-  const int64_t v[9] = {c.A.view.nslots,
+  const int64_t v[11] = {c.A.view.nslots,
                         (int64_t)c.N,
                         (int64_t)c.NPL,
                         (int64_t)c.NPU,
@@ -1308,8 +1423,10 @@ int32_t hecmw33_ctx_info(hecmw33_ctx* ctx, int64_t* info, int32_t n) {
                         c.L.view.nslots + c.U.view.nslots,
                         (int64_t)c.last_nreliable,
                         (int64_t)c.last_nfallback,
-                        (int64_t)c.last_learned_off};
-  for (int i = 0; i < n && i < 9; ++i) info[i] = v[i];
+                        (int64_t)c.last_learned_off,
+                        (int64_t)c.last_spmv64,
+                        (int64_t)std::llround(c.last_probe_ratio * 1e6)};
+  for (int i = 0; i < n && i < 11; ++i) info[i] = v[i];
   return 0;
 }
 
@@ -1359,6 +1476,7 @@ static int32_t net_upload(hecmw33_net& N, const float* w) {
 static int32_t net_work(hecmw33_net& N, int64_t n) {
   const size_t n3 = 3 * (size_t)n;
   if (N.H1.ensure(2 * 64 * (size_t)n) || N.H2.ensure(2 * 64 * (size_t)n) || N.z.ensure(sizeof(float) * n3) ||
+      N.y.ensure(sizeof(float) * n3) || N.dYc.ensure(sizeof(float) * n3) ||
       N.q.ensure(sizeof(float) * n3) || N.dY.ensure(sizeof(float) * n3) ||
       N.wpart.ensure(sizeof(float) * (size_t)net_bwd_parts(n) * net_weight_count()))
     return HECMW33_ERROR_DEVICE;
@@ -1438,27 +1556,27 @@ int32_t hecmw33_net_eval(hecmw33_net* net, int64_t n, const float* X, const floa
 }
 
 // One training sample from a device context holding an SSOR set-up: the
-// residual r (device, FP64, 3*NP), its FP32 SSOR output and the features.
+// residual r (device, FP64, 3*NP), its FP32 SSOR and block-Jacobi outputs.
 // Returns the sample count.
-int32_t hecmw33_net_add_sample(hecmw33_net* net, hecmw33_ctx* ctx, const double* r_dev, double rarray_sigma) {
+int32_t hecmw33_net_add_sample(hecmw33_net* net, hecmw33_ctx* ctx, const double* r_dev, double sigma_diag) {
   if (!net || !ctx || !r_dev || !ctx->c.have_struct) return HECMW33_ERROR_ARGS;
   Context& c = ctx->c;
-  if (int32_t e = learned_setup(c, rarray_sigma)) return e;
+  if (int32_t e = learned_setup(c, sigma_diag)) return e;
   hecmw33_net::Sample* S = new (std::nothrow) hecmw33_net::Sample();
   if (!S) return HECMW33_ERROR_DEVICE;
   S->c = &c;
   S->n = c.N;
   const int64_t n3 = 3 * (int64_t)c.N;
   if (S->r32.ensure(sizeof(float) * 3 * (size_t)c.NP) || S->z0.ensure(sizeof(float) * 3 * (size_t)c.NP) ||
-      S->X.ensure(2 * 16 * (size_t)c.N) || net_work(*net, c.N)) {
+      S->jr.ensure(sizeof(float) * 3 * (size_t)c.NP) || net_work(*net, c.N)) {
     delete S;
     return HECMW33_ERROR_DEVICE;
   }
   HIPC(hipMemsetAsync(S->r32.p, 0, sizeof(float) * 3 * (size_t)c.NP, c.stream));
   launch_to_f32(n3, r_dev, S->r32.as<float>(), c.stream);
   ssor_apply_f32(c.ssor_op32(), S->r32.as<float>(), S->z0.as<float>(), nullptr, c.stream);
-  net_features(c.N, S->r32.as<float>(), S->z0.as<float>(), c.alu_diag32.as<float>(), c.vmat.as<float>(),
-               S->X.as<__bf16>(), c.stream);
+  precond_apply_f32(3, c.ssor_op32(), c.alu_diag32.as<float>(), c.N, S->r32.as<float>(), S->jr.as<float>(), nullptr,
+                    c.stream);
   // energy functional of the plain SSOR output: (z0.A z0 - 2 z0.r) / (r.z0)
   launch_spmv_f32(c.A.view32, S->z0.as<float>(), net->q.as<float>(), c.stream);
   net_energy_dots(n3, S->z0.as<float>(), net->q.as<float>(), S->r32.as<float>(), S->z0.as<float>(),
@@ -1478,7 +1596,8 @@ int32_t hecmw33_net_clear_samples(hecmw33_net* net) {
   return 0;
 }
 
-// Adam on the energy functional L(z) = (z.Az - 2 z.r) / (r.z0), z = z0 + N(x),
+// Adam on the energy functional L(z) = (z.Az - 2 z.r) / (r.z0),
+// z = z0 + b J r with b = softplus(N(g) - 4) on the context's static features,
 // one sample per step, samples in a seeded shuffled order every epoch.
 // loss[0] = mean L of the plain SSOR output, loss[1+e] = mean L over epoch e
 // (before each step's update).
@@ -1509,15 +1628,17 @@ int32_t hecmw33_net_train(hecmw33_net* net, int32_t epochs, double lr, uint64_t 
       Context& c = *S.c;
       const int64_t n = S.n, n3 = 3 * n;
       if (int32_t e = net_work(N, n)) return e;
-      net_forward(n, N.Wb.as<__bf16>(), S.X.as<__bf16>(), N.H1.as<__bf16>(), N.H2.as<__bf16>(), S.z0.as<float>(),
-                  N.z.as<float>(), N.stream);
+      net_forward(n, N.Wb.as<__bf16>(), c.vfeat.as<__bf16>(), N.H1.as<__bf16>(), N.H2.as<__bf16>(), nullptr,
+                  N.y.as<float>(), N.stream);
+      learned_combine_train(n, N.y.as<float>(), S.z0.as<float>(), S.jr.as<float>(), N.z.as<float>(), N.stream);
       launch_spmv_f32(c.A.view32, N.z.as<float>(), N.q.as<float>(), N.stream);
       net_energy_dots(n3, N.z.as<float>(), N.q.as<float>(), S.r32.as<float>(), S.z0.as<float>(),
-                      N.dpart.as<double>(), N.dots.as<double>() , N.stream);
+                      N.dpart.as<double>(), N.dots.as<double>(), N.stream);
       HIPC(hipMemcpyAsync(&dots[3 * k], N.dots.p, 3 * sizeof(double), hipMemcpyDeviceToHost, N.stream));
       net_energy_grad(n3, N.q.as<float>(), S.r32.as<float>(), N.dots.as<double>(), N.dY.as<float>(), N.stream);
-      net_backward(n, N.Wb.as<__bf16>(), S.X.as<__bf16>(), N.H1.as<__bf16>(), N.H2.as<__bf16>(), N.dY.as<float>(),
-                   N.wpart.as<float>(), N.dW.as<float>(), N.stream);
+      learned_coef_grad(n, N.y.as<float>(), N.dY.as<float>(), S.jr.as<float>(), N.dYc.as<float>(), N.stream);
+      net_backward(n, N.Wb.as<__bf16>(), c.vfeat.as<__bf16>(), N.H1.as<__bf16>(), N.H2.as<__bf16>(),
+                   N.dYc.as<float>(), N.wpart.as<float>(), N.dW.as<float>(), N.stream);
       net_adam(N.Wm.as<float>(), N.M.as<float>(), N.V.as<float>(), N.dW.as<float>(), N.Wb.as<__bf16>(), (float)lr,
                0.9f, 0.999f, 1e-8f, ++N.step, 1.0f, N.stream);
     }
@@ -1542,8 +1663,8 @@ int32_t hecmw33_net_score(hecmw33_net* net, double* mean, double* base) {
     Context& c = *S.c;
     const int64_t n = S.n, n3 = 3 * n;
     if (int32_t e = net_work(N, n)) return e;
-    net_forward(n, N.Wb.as<__bf16>(), S.X.as<__bf16>(), N.H1.as<__bf16>(), N.H2.as<__bf16>(), S.z0.as<float>(),
-                N.z.as<float>(), N.stream);
+    net_forward(n, N.Wb.as<__bf16>(), c.vfeat.as<__bf16>(), nullptr, nullptr, nullptr, N.y.as<float>(), N.stream);
+    learned_combine_train(n, N.y.as<float>(), S.z0.as<float>(), S.jr.as<float>(), N.z.as<float>(), N.stream);
     launch_spmv_f32(c.A.view32, N.z.as<float>(), N.q.as<float>(), N.stream);
     net_energy_dots(n3, N.z.as<float>(), N.q.as<float>(), S.r32.as<float>(), S.z0.as<float>(), N.dpart.as<double>(),
                     N.dots.as<double>(), N.stream);

@@ -201,11 +201,13 @@ class HecMatrix:
     def set_ncolor_in(self, v): self.set_i(self.I_NCOLOR_IN, v)
     def set_resid(self, v): self.Rarray[self.R_RESID - 1] = v
 
-    def set_mixed(self, on=True, delta=None):
+    def set_mixed(self, on=True, delta=None, spmv=0):
         """Flexible CG with FP32 operators and FP64 residual replacement
         (Iarray 41 = 1, METHOD 1 only); delta: reliable-update factor on |r|
-        (Rarray 41, default 1e-2)."""
+        (Rarray 41, default 1e-2); spmv (Iarray 42): 1 FP32 values in q = A p,
+        2 FP64 values with FP32 vectors, 0 chosen from the stiffness contrast."""
         self.set_i(self.I_SOLVER_OPT, 1 if on else 0)
+        self.set_i(self.I_SOLVER_OPT + 1, spmv)
         self.Rarray[self.R_SOLVER_OPT - 1] = 0.0 if delta is None else float(delta)
 
     def set_sigma_diag(self, v):
@@ -423,10 +425,11 @@ class DeviceContext:
         return h[:n]
 
     def info(self):
-        v = np.zeros(9, np.int64)
-        _check(lib().hecmw33_ctx_info(self._h, _ptr(v), 9))
+        v = np.zeros(11, np.int64)
+        _check(lib().hecmw33_ctx_info(self._h, _ptr(v), 11))
         return {"nslots": int(v[0]), "N": int(v[1]), "NPL": int(v[2]), "NPU": int(v[3]), "ncolor": int(v[4]),
-                "ssor_slots": int(v[5]), "nreliable": int(v[6]), "nfallback": int(v[7]), "learned_off": int(v[8])}
+                "ssor_slots": int(v[5]), "nreliable": int(v[6]), "nfallback": int(v[7]), "learned_off": int(v[8]),
+                "spmv64": int(v[9]), "probe_ratio": v[10] / 1e6}
 
     def set_net(self, net):
         """Attach a LearnedNet for solves with Iarray(41) = 2 (None detaches)."""
@@ -545,8 +548,9 @@ def gen_tet10_batch(nsys=48, nrhs=16, dof_min=5000, dof_max=600000, seed=0, E=21
 
 
 class LearnedNet:
-    """The learned correction (hecmw33_net_*): per-node MLP 16 -> 64 -> 64 -> 3
-    (ReLU, no biases) with BF16 MFMA kernels, trained on the GPU."""
+    """The learned correction (hecmw33_net_*): a per-node MLP 16 -> 64 -> 64 -> 3
+    (ReLU; BF16 MFMA kernels) mapping static node features to the coefficient
+    b = softplus(y - 4) of z = z0 + b J r inside the flexible CG; trained on the GPU."""
 
     def __init__(self, seed=0):
         self._h = lib().hecmw33_net_create(seed)

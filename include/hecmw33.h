@@ -89,11 +89,15 @@ int32_t hecmw33_solve_krylov(const hecmw33_sys* A, const double* B, double* X, i
  * SOLVER_OPT block that CG/BiCGSTAB with PRECOND 1-3 never read,
  * hecmw_matrix_misc.F90:138):
  *   iarray[40] (Iarray 41) = 1 with METHOD 1, ITERPREMAX <= 1: flexible CG
- *   (Notay's FCG(1)) whose SpMV, SSOR/DIAG sweeps and work vectors are FP32,
+ *   (Notay's FCG(1)) whose SSOR/DIAG sweeps and work vectors are FP32,
  *   with FP64 inner products, scalars and x; the residual is replaced by
  *   b - A x computed in FP64 whenever the recurrence residual has dropped by
  *   rarray[40] (Rarray 41, default 1e-2), every 50 iterations and before any
  *   convergence is declared, so RESID is always an FP64 true residual.
+ *   iarray[41] (Iarray 42) picks the matrix values of its q = A p: 1 FP32,
+ *   2 FP64 (FP32 vectors, FP64 accumulation), 0 (default) FP64 when the
+ *   stiffness contrast max/min |tr D_ii| exceeds 1e4 (an FP32-rounded operator
+ *   loses the search directions' conjugacy on high-contrast systems), else FP32.
  *   Accepted by hecmw33_solve, hecmw33_solve_krylov and hecmw33_ctx_solve_device. */
 
 /* y = A x for the internal rows (hecmw_matvec_33, hecmw1/src/solver/las/hecmw_solver_las_33.F90:34-59).
@@ -144,7 +148,9 @@ int32_t hecmw33_ctx_history(hecmw33_ctx* ctx, double* hist, int32_t cap);
 /* Context figures: [0] SELL-64 slots of A (incl. padding), [1] N, [2] NPL, [3] NPU, [4] colours,
  * [5] SSOR slots of L' + U', [6] FP64 residual replacements of the last flexible CG solve,
  * [7] iterations of the last learned solve that fell back to the SSOR output, [8] 1 if that
- * solve dropped the learned correction (or was not a learned solve). */
+ * solve dropped the learned correction (or was not a learned solve), [9] 1 if the last flexible
+ * CG used the FP64 matrix values in q = A p, [10] the last learned probe's error reduction with
+ * / without the correction, in millionths. */
 int32_t hecmw33_ctx_info(hecmw33_ctx* ctx, int64_t* info, int32_t n);
 
 /* ---- batched solves (no counterpart in the reference: hecmw_solve_CG handles one system and
@@ -178,14 +184,18 @@ int32_t hecmw33_solve_batch(int32_t nsys, const hecmw33_sys* sys, int32_t nrhs, 
                             int32_t* error, double* resid);
 
 /* ---- learned correction (no counterpart in the reference) -----------------------------------
- * A per-node MLP (16 node-local features linear in the residual -> 64 -> 64 -> 3, ReLU, no
- * biases; BF16 MFMA kernels with FP32 accumulation) whose output is added to the FP32 SSOR
- * output inside the flexible CG: iarray[40] (Iarray 41) = 2 with an attached network
- * (hecmw33_ctx_set_net for a device context, hecmw33_set_net for hecmw33_solve /
- * hecmw33_solve_krylov).  Per iteration the plain SSOR output is used instead when the learned
- * one is not a descent direction; the correction is dropped for the rest of a solve when the
- * FP64 residual grows between two residual replacements.  Trained on the GPU on the energy
- * functional (z.Az - 2 z.r)/(r.z0) of residuals harvested on device contexts.
+ * Inside the flexible CG (iarray[40] = Iarray 41 = 2, PRECOND 1/2, with an attached network:
+ * hecmw33_ctx_set_net for a device context, hecmw33_set_net for hecmw33_solve /
+ * hecmw33_solve_krylov) the preconditioned residual is z_i = z0_i + b_i (J r)_i: z0 the FP32
+ * SSOR output, J r the block-Jacobi one, b_i = softplus(y_i - 4) > 0 with y_i from a per-node MLP
+ * (16 static node features -> 64 -> 64 -> 3, ReLU; BF16 MFMA kernels accumulating in FP32)
+ * evaluated once per solve, so the preconditioner is linear in r, symmetric and positive
+ * definite.  Plain SSOR stays the fallback: each solve first runs 25 iterations from x0 with and
+ * without the correction and keeps it only if its A-norm error reduction (sum of alpha r.p) is
+ * the larger; during the solve the plain SSOR output replaces z when z is not a descent
+ * direction, and the correction is dropped when the FP64 residual grows 4x between two residual
+ * replacements.  Trained on the GPU on the energy functional (z.Az - 2 z.r)/(r.z0) of residuals
+ * harvested on device contexts.
  * A network must outlive the contexts it is attached to; contexts must outlive its samples. */
 typedef struct hecmw33_net hecmw33_net;
 hecmw33_net* hecmw33_net_create(uint64_t seed);

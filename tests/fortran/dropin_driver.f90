@@ -37,7 +37,7 @@ program dropin_test
   type(hecmwST_matrix), target :: mref, mgpu
   type(hecmwST_local_mesh) :: mesh
   integer(c_int32_t) :: n, N_, NPL, NPU, rc
-  integer :: method, precond
+  integer :: method, precond, scaling
   character(len=32) :: arg
   real(c_double) :: p(8)
   real(kind=kreal) :: relerr
@@ -45,6 +45,10 @@ program dropin_test
   call get_command_argument(1, arg); read(arg, *) n
   call get_command_argument(2, arg); read(arg, *) method
   call get_command_argument(3, arg); read(arg, *) precond
+  scaling = 0
+  if (command_argument_count() >= 4) then
+    call get_command_argument(4, arg); read(arg, *) scaling
+  endif
 
   rc = gen_count(n, N_, NPL, NPU)
   call setup(mref)
@@ -100,6 +104,7 @@ contains
     call hecmw_mat_set_precond(m, precond)
     call hecmw_mat_set_iterlog(m, 0)
     call hecmw_mat_set_timelog(m, 0)
+    call hecmw_mat_set_scaling(m, scaling)
   end subroutine setup
 
 end program dropin_test
