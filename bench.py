@@ -30,6 +30,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the in-solve SpMV (k_spmv<2>) on the 100^3 cube, from
+# two rocprofv3 passes (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000, the factors a
+# 1 GiB copy of the same run gives); profiles/spmv_hex8_100_traffic.md
+SPMV_TRAFFIC_100 = 2196170607
 
 
 def spmv_bytes(m, w):
@@ -198,7 +202,8 @@ def main():
         "roofline": {
             "bound": "hbm", "kernel": "k_cg_spmv_dot (q = A p, p.q)",
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": SPMV_TRAFFIC_100 if args.n == 100 else None,
             "bytes_per_launch": bytes64, "avg_launch_ms": round(ax_ms, 4),
         },
         "spmv": {"fp64_ms": round(ms64, 4), "fp64_GBs": round(gbs64, 1), "fp32_ms": round(ms32, 4),
