@@ -15,9 +15,9 @@
 //               sides, and one workgroup per system sums its tiles in a fixed
 //               order (deterministic, no atomics)
 //   precision : FP32 matrix, SpMM, DIAG and work vectors; FP64 x, inner
-//               products and CG scalars; the residual of an instance is
-//               replaced by b - A x computed with the FP64 matrix whenever
-//               its recurrence residual has dropped by delta, every 50
+//               products and CG scalars; the residuals of a system's
+//               instances are replaced by b - A x computed with the FP64
+//               matrix whenever one of them has dropped by delta, every 50
 //               iterations and before convergence is declared (flexible CG,
 //               Notay's FCG(1) recurrences, as the single-system mixed path)
 //   launches  : the kernels of kChunk iterations are captured once into a
@@ -53,6 +53,10 @@ constexpr int32_t kErrNoconv = 3001, kErrDivergeMat = 3002, kErrDivergePc = 3003
 // per-instance state, structure of arrays over nsys*16 instances
 struct BState {
   double *bnrm2, *rho, *rho1, *zq, *pq, *rp, *alpha, *beta, *rr_ref, *resid;
+  // gap: true / recurrence residual seen at the last replacement; the FP32
+  // recurrence runs ahead of the FP64 residual near convergence, so it claims
+  // convergence when resid * gap <= tol
+  double* gap;
   int32_t *iter, *done, *error, *nindef, *rel;
   int32_t *sys_active, *sys_rel, *active_total;
   // solve parameters in device memory, so a captured graph serves any call:
@@ -352,6 +356,7 @@ __global__ __launch_bounds__(kBT) void kb_tail(Tiles T, BState S, const double* 
       S.bnrm2[id] = tot[0];
       S.rr_ref[id] = tot[ND - 1];
       S.rho[id] = S.rho1[id] = S.zq[id] = S.pq[id] = S.rp[id] = S.alpha[id] = S.beta[id] = 0.0;
+      S.gap[id] = 1.0;
       S.iter[id] = 1;
       S.nindef[id] = 0;
       S.rel[id] = 0;
@@ -396,14 +401,16 @@ __global__ __launch_bounds__(kBT) void kb_tail(Tiles T, BState S, const double* 
       } else if (TAIL == kbRes) {
         const double rr = tot[0];
         S.resid[id] = sqrt(rr / S.bnrm2[id]);
-        S.rel[id] = S.resid[id] <= S.prm[0] || rr <= S.prm[1] * S.rr_ref[id] || (S.iter[id] % S.iprm[1]) == 0 ||
-                    rr != rr;
+        S.rel[id] = S.resid[id] * S.gap[id] <= S.prm[0] || rr <= S.prm[1] * S.rr_ref[id] ||
+                    (S.iter[id] % S.iprm[1]) == 0 || rr != rr;
         fl = S.rel[id];
       } else if (TAIL == kbRel) {
         if (S.rel[id]) {
           const double rr = tot[0];
+          const double rec = S.resid[id];
           S.rr_ref[id] = rr;
           S.resid[id] = sqrt(rr / S.bnrm2[id]);
+          if (rec > 0.0) S.gap[id] = fmin(1e3, fmax(1.0, S.resid[id] / rec));
           S.rel[id] = 0;
           if (S.resid[id] <= S.prm[0]) {
             S.done[id] = 1;
@@ -431,7 +438,16 @@ __global__ __launch_bounds__(kBT) void kb_tail(Tiles T, BState S, const double* 
   if (threadIdx.x == 0) {
     int32_t n = 0;
     for (int j = 0; j < KS; ++j) n += flag[j];
+    // The FP64 residual costs the same for one right-hand side as for all of
+    // a system's: when one asks for it, every live one is refreshed, so their
+    // reference norms stay aligned and the next replacement comes later.
+    if (TAIL == kbRes && n > 0)
+      for (int j = 0; j < KS; ++j)
+        if (!S.done[s * KS + j]) S.rel[s * KS + j] = 1;
     if (TAIL == kbRes) S.sys_rel[s] = n > 0;
+    // a finished system's tails return early from now on: clear its request
+    // here, or the FP64 residual SpMM would keep computing its tiles
+    if (TAIL == kbRel) S.sys_rel[s] = 0;
     if (TAIL == kbRel || TAIL == kbInit) S.sys_active[s] = n;
     if (TAIL == kbInit) S.sys_rel[s] = n > 0;
   }
@@ -574,15 +590,15 @@ namespace hecmw33 {
 
 static int32_t batch_state_layout(hecmw33_batch& B) {
   const int64_t ni = (int64_t)B.nsys * KS;
-  const size_t bytes = sizeof(double) * (10 * ni + 2) + sizeof(int32_t) * (5 * ni + 2 * B.nsys + 3) + 256;
+  const size_t bytes = sizeof(double) * (11 * ni + 2) + sizeof(int32_t) * (5 * ni + 2 * B.nsys + 3) + 256;
   if (B.state.ensure(bytes)) return HECMW33_ERROR_DEVICE;
   char* base = B.state.as<char>();
   double* d = reinterpret_cast<double*>(base);
   BState& S = B.S;
-  double** dp[10] = {&S.bnrm2, &S.rho, &S.rho1, &S.zq, &S.pq, &S.rp, &S.alpha, &S.beta, &S.rr_ref, &S.resid};
-  for (int i = 0; i < 10; ++i) *dp[i] = d + i * ni;
-  S.prm = d + 10 * ni;
-  int32_t* ip = reinterpret_cast<int32_t*>(d + 10 * ni + 2);
+  double** dp[11] = {&S.bnrm2, &S.rho, &S.rho1, &S.zq, &S.pq, &S.rp, &S.alpha, &S.beta, &S.rr_ref, &S.resid, &S.gap};
+  for (int i = 0; i < 11; ++i) *dp[i] = d + i * ni;
+  S.prm = d + 11 * ni;
+  int32_t* ip = reinterpret_cast<int32_t*>(d + 11 * ni + 2);
   int32_t** ipp[5] = {&S.iter, &S.done, &S.error, &S.nindef, &S.rel};
   for (int i = 0; i < 5; ++i) *ipp[i] = ip + i * ni;
   S.sys_active = ip + 5 * ni;

@@ -80,6 +80,9 @@ struct KrylovState {
   // learned correction: off for good / used in this iteration / fallbacks to plain SSOR so far
   int32_t learned_off, learned_now, nfallback, pad2;
   double edrop;  // sum of alpha (r.p): the decrease of |e|_A^2 over the iterations so far
+  // true / recurrence residual seen at the last replacement: near convergence
+  // the FP32 recurrence runs ahead, so it claims convergence at resid * gap <= tol
+  double gap;
 };
 
 constexpr int kMaxPartials = 65536;  // per reduction component

@@ -223,8 +223,8 @@ __device__ void run_tail(KrylovState* st, const double (&tot)[K], double* out, d
     // replace the FP32 recurrence residual by b - A x in FP64 when it has
     // dropped by delta^2 in |r|^2, when it claims convergence, and every nrec
     // iterations
-    st->reliable = st->resid <= st->tol || tot[0] <= st->delta2 * st->rr_ref || (st->iter % st->nrec) == 0 ||
-                   tot[0] != tot[0];
+    st->reliable = st->resid * st->gap <= st->tol || tot[0] <= st->delta2 * st->rr_ref ||
+                   (st->iter % st->nrec) == 0 || tot[0] != tot[0];
   }
 }
 
@@ -284,9 +284,11 @@ __global__ __launch_bounds__(kTailThreads) void k_tail_fcg_rel(KrylovState* st, 
       // the learned correction is dropped for the rest of the solve once the
       // FP64 residual grows (4x in |r|^2) between two replacements
       if (!st->learned_off && tot[0] > 4.0 * st->rr_ref) st->learned_off = 1;
+      const double rec = st->resid;
       st->dnrm2 = tot[0];
       st->rr_ref = tot[0];
       st->resid = sqrt(tot[0] / st->bnrm2);
+      if (rec > 0.0) st->gap = fmin(1e3, fmax(1.0, st->resid / rec));
       if (hist) hist[st->iter - 1] = st->resid;
       st->nreliable += 1;
       st->reliable = 0;

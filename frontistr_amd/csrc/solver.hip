@@ -980,6 +980,7 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
   h.learned_now = 0;
   h.nfallback = 0;
   h.edrop = 0.0;
+  h.gap = 1.0;
   c.h_st[1] = h;
   HIPC(hipMemcpyAsync(st, &c.h_st[1], sizeof(KrylovState), hipMemcpyHostToDevice, c.stream));
   HIPC(hipStreamSynchronize(c.stream));
@@ -1004,6 +1005,7 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
       hp.maxit = kLearnProbe + 1;
       hp.learned_off = pass == 0 ? 0 : 1;
       hp.edrop = 0.0;
+      hp.gap = 1.0;
       c.h_st[1] = hp;
       HIPC(hipMemcpyAsync(st, &c.h_st[1], sizeof(KrylovState), hipMemcpyHostToDevice, c.stream));
       for (int i = 0; i < kLearnProbe; ++i) enqueue_fcg_learned_iteration(c, spmv64, nullptr);
