@@ -31,6 +31,8 @@ from .hecmw import (  # noqa: F401
     hecmw_ordering,
     gen_hex8,
     gen_tet10,
+    hex8_coords,
+    set_coords,
     BatchSolver,
     solve_batch,
     gen_tet10_batch,

@@ -31,6 +31,7 @@ SOURCES = [
     ("gen_tet10.cpp", False),
     ("batch.hip", True),
     ("learned.hip", True),
+    ("coarse.hip", True),
 ]
 
 
@@ -55,7 +56,8 @@ def build(jobs=4, verbose=False):
     with cf.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(lambda s: compile_one(s[0], s[1], objdir), SOURCES))
     out = os.path.join(LIBDIR, "libhecmw33.so")
-    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fopenmp", "-o", out + ".tmp"] + objs
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fopenmp", "-o", out + ".tmp"] + objs + [
+        "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocsolver", "-lrocblas"]
     p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if p.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (" ".join(cmd), p.stdout))

@@ -184,6 +184,28 @@ void launch_scale_vec(bool mul, int64_t n, double* v, const double* sc, hipStrea
 // learned flexible CG: beta from the four dots of net_apply (r.z, z.q, r.z0, z0.q), falling
 // back to the plain SSOR output z0 when the learned z is not a descent direction
 void fcg_beta_learned(KrylovState* st, const double* part, int nparts, hipStream_t s);
+// the flexible CG's beta tail over nparts partials of (r.z, z.q_prev)
+void fcg_beta(KrylovState* st, const double* part, int nparts, hipStream_t s);
+
+// Two-level correction (coarse.hip): piecewise rigid-body modes on geometric
+// aggregates, z += Z E^-1 Z^T r.
+struct CoarseOp {
+  int32_t n_agg = 0, nc = 0, ldw = 0;  // aggregates, coarse unknowns (6 per aggregate), row stride of W
+  const int32_t* agg = nullptr;        // [NP] aggregate of each node, -1 for external nodes
+  const int32_t* aptr = nullptr;       // [n_agg+1] into anodes
+  const int32_t* anodes = nullptr;     // nodes of each aggregate, ascending
+  const float* dv = nullptr;           // [3N] scaled offsets from the aggregate centroid
+  float* W = nullptr;                  // E^-1, FP32, nc x ldw row-major
+  float* cv = nullptr;                 // [ldw] Z^T r (entries nc..ldw-1 stay 0)
+  float* yv = nullptr;                 // [nc] E^-1 Z^T r
+};
+void coarse_assemble(const Sell<double>& A, const CoarseOp& op, const int32_t* nbr, const uint32_t* mask,
+                     double* Eblk, double* E, int32_t lda, hipStream_t s);
+void coarse_inverse_f32(const CoarseOp& op, const double* Ei, int32_t lda, hipStream_t s);
+// z += Z E^-1 Z^T r; with dots also the beta partials (r.z, q.z) of the new z.
+// Returns the number of partials.
+int coarse_apply(const CoarseOp& op, int32_t n, const float* r, float* z, const float* q, const KrylovState* st,
+                 double* part, bool dots, hipStream_t s);
 // p = (learned z or z0) + beta p
 void fcg_p_sel(int64_t n3, const float* z, const float* z0, float* p, const KrylovState* st, hipStream_t s);
 

@@ -1452,6 +1452,9 @@ void fcg_precond_dots(int32_t kind, const SsorOp<float>& op, const float* alu_di
   }
   k_tail<kTailFcgBeta, 2><<<1, kTailThreads, 0, s>>>(st, part, nparts, nullptr, nullptr);
 }
+void fcg_beta(KrylovState* st, const double* part, int nparts, hipStream_t s) {
+  k_tail<kTailFcgBeta, 2><<<1, kTailThreads, 0, s>>>(st, part, nparts, nullptr, nullptr);
+}
 void fcg_beta_learned(KrylovState* st, const double* part, int nparts, hipStream_t s) {
   k_tail<kTailFcgBetaL, 4><<<1, kTailThreads, 0, s>>>(st, part, nparts, nullptr, nullptr);
 }

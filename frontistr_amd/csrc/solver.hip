@@ -16,6 +16,8 @@
 //   ALU    : 3x3 LU of the diagonal blocks, 9-plane slice layout
 //   vectors: b, x, r, z, p, q (+ BiCGSTAB's) of 3*NP doubles
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <chrono>
@@ -77,6 +79,15 @@ constexpr int kLearnProbe = 25;
 // kContrastFp64Spmv)
 constexpr int I_SOLVER_OPT2 = 42;
 constexpr double kContrastFp64Spmv = 1.0e4;
+// SOLVER_OPT(3) (Iarray 43) for the flexible CG: > 0 adds the two-level
+// correction z += Z E^-1 Z^T r (rigid-body modes on geometric aggregates of
+// the nodes, which hecmw33_set_coords supplies); 1 = kCoarseAggDefault
+// aggregates, larger values the aggregate count to aim at
+constexpr int I_SOLVER_OPT3 = 43;
+constexpr int32_t kCoarseAggDefault = 1000;
+// the dense coarse inverse is n_c^2 entries (n_c = 6 per aggregate): at most
+// 2048 aggregates (n_c = 12288, 0.6 GB in FP32)
+constexpr int32_t kCoarseAggMax = 2048;
 static inline int32_t& IA(int32_t* a, int k) { return a[k - 1]; }
 static inline int32_t IA(const int32_t* a, int k) { return a[k - 1]; }
 static inline double RA(const double* a, int k) { return a[k - 1]; }
@@ -230,6 +241,20 @@ struct Context {
   bool last_spmv64 = false;      // the last flexible CG used the FP64 matrix values in q = A p
   double last_probe_ratio = 0.0; // error reduction with / without the learned correction in the last probe
   DMem vx0;                      // x0 kept across the learned probe runs
+  // two-level correction: node coordinates (hecMESH%node), aggregation, coarse inverse
+  std::vector<double> h_coord;   // 3*NP
+  int64_t coord_gen = 0;         // bumped whenever coordinates are set
+  bool coarse_sym = false;       // aggregation built for (structure, coordinates, target)
+  bool coarse_num = false;       // E^-1 built for the current values
+  bool coarse_ok = false;        // E was positive definite
+  int32_t coarse_target = 0;
+  int64_t coarse_coord_gen = -1;
+  bool coarse_use = false;       // this solve applies it
+  int32_t last_coarse = 0;       // aggregates used by the last solve (-1: E not positive definite)
+  CoarseOp coarse;
+  std::vector<int32_t> coarse_cell;  // cell coordinates of each aggregate (3 per aggregate)
+  DMem cg_agg, cg_aptr, cg_anodes, cg_nbr, cg_mask, cg_dv, cg_Eblk, cg_E, cg_W, cg_cv, cg_yv, cg_info;
+  rocblas_handle blas = nullptr;
   double contrast = -1.0;        // max/min trace of the diagonal blocks (< 0: not computed for these values)
   // TIMELOG: event pairs around the preconditioner and the SpMV of every
   // enqueued iteration (4 events per iteration)
@@ -257,6 +282,10 @@ struct Context {
     for (DMem* m : {&dIndexL, &dIndexU, &dItemL, &dItemU, &vscale, &vzs, &vzs32, &vz0_32, &vmat, &scal, &vfeat, &vcoef,
                     &vx0})
       m->release();
+    for (DMem* m : {&cg_agg, &cg_aptr, &cg_anodes, &cg_nbr, &cg_mask, &cg_dv, &cg_Eblk, &cg_E, &cg_W, &cg_cv, &cg_yv,
+                    &cg_info})
+      m->release();
+    if (blas) rocblas_destroy_handle(blas);
     for (DMem* m : {&dD, &dAL, &dAU, &row_old, &alu_ssor, &row_diag, &alu_diag, &alu_ssor32, &alu_diag32, &vr32,
                     &vz32, &vp32, &vq32, &vb, &vx, &vr, &vz, &vp, &vq, &vrt, &vs, &vt, &vv, &vpt, &vst, &vzp, &vzt, &st,
                     &part, &hist})
@@ -385,6 +414,7 @@ static int32_t load_structure(Context& c, const hecmw33_sys* A, bool force) {
   c.have_struct = true;
   c.ssor_init = false;
   c.ssor_sym = false;
+  c.coarse_sym = false;
   c.diag_init = false;
   c.dindex_valid = false;
   const size_t vb = sizeof(double) * 3 * (size_t)NP;
@@ -406,6 +436,7 @@ static int32_t load_values(Context& c, const hecmw33_sys* A) {
   HIPC(hipGetLastError());
   c.a32_valid = false;
   c.mat_valid = false;
+  c.coarse_num = false;
   c.contrast = -1.0;
   return 0;
 }
@@ -670,6 +701,8 @@ static int32_t precond_setup(Context& c, int32_t* iarray, const double* rarray) 
     set_error("PRECOND not provided by the GPU library (1, 2: SSOR; 3: DIAG)");
     return HECMW33_ERROR_INCONS_PC;
   }
+  // the two-level correction's coarse inverse is part of the numeric set-up
+  c.coarse_num = false;
   IA(iarray, I_FLAG_SYMBFACT) = 0;
   IA(iarray, I_FLAG_NUMFACT) = 0;
   return 0;
@@ -763,13 +796,238 @@ static void enqueue_bicg_iteration(Context& c, int32_t kind, int32_t iterpremax,
 // One flexible CG iteration with FP32 operators: z = M32^-1 r (fused r.z,
 // z.q_prev), p = z + beta p, q = A32 p (fused p.q, r.p), x += alpha p (FP64),
 // r -= alpha q, then, when the recurrence asks for it, r = b - A x in FP64.
+// ---- two-level correction: aggregation (symbolic) and coarse inverse (numeric) ----------
+// Aggregates are the non-empty cells of a uniform grid over the nodes' bounding
+// box, about `target` of them; the cell size grows until every matrix block
+// couples nodes in the same or adjacent cells, so E_ab is non-zero only for
+// the 27 neighbour cells of a.
+static int32_t coarse_symbolic(Context& c, int32_t target) {
+  const int32_t N = c.N, NP = c.NP;
+  const double* X = c.h_coord.data();
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int32_t i = 0; i < N; ++i)
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = std::min(lo[d], X[3 * (int64_t)i + d]);
+      hi[d] = std::max(hi[d], X[3 * (int64_t)i + d]);
+    }
+  double ext[3], vol = 1.0;
+  int ndim = 0;
+  for (int d = 0; d < 3; ++d) {
+    ext[d] = hi[d] - lo[d];
+    if (!(ext[d] > 0.0)) ext[d] = 0.0;
+  }
+  const double emax = std::max(ext[0], std::max(ext[1], ext[2]));
+  for (int d = 0; d < 3; ++d)
+    if (ext[d] > 1e-9 * emax) {
+      vol *= ext[d];
+      ++ndim;
+    } else {
+      ext[d] = 0.0;
+    }
+  if (ndim == 0) {
+    set_error("coarse correction: all nodes at one point");
+    return HECMW33_ERROR_ARGS;
+  }
+  const int32_t* iL = c.h_indexL.data();
+  const int32_t* iU = c.h_indexU.data();
+  const int32_t* tL = c.h_itemL.data();
+  const int32_t* tU = c.h_itemU.data();
+  std::vector<int32_t> cell(3 * (size_t)N);
+  int nc[3] = {1, 1, 1};
+  double H = std::pow(vol / std::max(1, target), 1.0 / ndim);
+  bool ok = false;
+  for (int attempt = 0; attempt < 40 && !ok; ++attempt, H *= 1.15) {
+    int64_t ncells = 1;
+    for (int d = 0; d < 3; ++d) {
+      nc[d] = ext[d] > 0.0 ? std::max(1, (int)std::ceil(ext[d] / H - 1e-9)) : 1;
+      ncells *= nc[d];
+    }
+    if (ncells > 64 * (int64_t)kCoarseAggMax) continue;
+#pragma omp parallel for schedule(static)
+    for (int32_t i = 0; i < N; ++i)
+      for (int d = 0; d < 3; ++d)
+        cell[3 * (size_t)i + d] =
+            ext[d] > 0.0 ? std::min(nc[d] - 1, (int)((X[3 * (int64_t)i + d] - lo[d]) / ext[d] * nc[d])) : 0;
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(max : bad)
+    for (int32_t i = 0; i < N; ++i) {
+      auto chk = [&](int32_t j) {
+        if (j >= N) return;
+        for (int d = 0; d < 3; ++d)
+          if (std::abs(cell[3 * (size_t)i + d] - cell[3 * (size_t)j + d]) > 1) bad = 1;
+      };
+      for (int32_t q = iL[i]; q < iL[i + 1]; ++q) chk(tL[q] - 1);
+      for (int32_t q = iU[i]; q < iU[i + 1]; ++q) chk(tU[q] - 1);
+    }
+    if (bad) continue;
+    // aggregates: non-empty cells in cell order
+    std::vector<int32_t> cell_agg((size_t)ncells, -1);
+    for (int32_t i = 0; i < N; ++i) {
+      const int64_t id = cell[3 * (size_t)i] + (int64_t)nc[0] * (cell[3 * (size_t)i + 1] + (int64_t)nc[1] * cell[3 * (size_t)i + 2]);
+      cell_agg[id] = 0;
+    }
+    int32_t na = 0;
+    for (auto& v : cell_agg)
+      if (v == 0) v = na++;
+    if (na > kCoarseAggMax) continue;
+    ok = true;
+    std::vector<int32_t> agg(NP, -1), cnt(na + 1, 0);
+    for (int32_t i = 0; i < N; ++i) {
+      const int64_t id = cell[3 * (size_t)i] + (int64_t)nc[0] * (cell[3 * (size_t)i + 1] + (int64_t)nc[1] * cell[3 * (size_t)i + 2]);
+      agg[i] = cell_agg[id];
+      ++cnt[agg[i] + 1];
+    }
+    for (int32_t a = 0; a < na; ++a) cnt[a + 1] += cnt[a];
+    std::vector<int32_t> anodes(N), fillp(cnt.begin(), cnt.end() - 1);
+    for (int32_t i = 0; i < N; ++i) anodes[fillp[agg[i]]++] = i;
+    // centroids, cells and neighbour table
+    std::vector<double> cen(3 * (size_t)na, 0.0);
+    c.coarse_cell.assign(3 * (size_t)na, 0);
+#pragma omp parallel for schedule(static)
+    for (int32_t a = 0; a < na; ++a) {
+      double sx = 0, sy = 0, sz = 0;
+      for (int32_t t = cnt[a]; t < cnt[a + 1]; ++t) {
+        const int64_t i = anodes[t];
+        sx += X[3 * i];
+        sy += X[3 * i + 1];
+        sz += X[3 * i + 2];
+      }
+      const double k = 1.0 / (cnt[a + 1] - cnt[a]);
+      cen[3 * (size_t)a] = sx * k;
+      cen[3 * (size_t)a + 1] = sy * k;
+      cen[3 * (size_t)a + 2] = sz * k;
+      const int64_t i0 = anodes[cnt[a]];
+      for (int d = 0; d < 3; ++d) c.coarse_cell[3 * (size_t)a + d] = cell[3 * i0 + d];
+    }
+    std::vector<int32_t> nbr(27 * (size_t)na, -1);
+    for (int32_t a = 0; a < na; ++a)
+      for (int s = 0; s < 27; ++s) {
+        const int o[3] = {s % 3 - 1, (s / 3) % 3 - 1, s / 9 - 1};
+        int cc[3];
+        bool in = true;
+        for (int d = 0; d < 3; ++d) {
+          cc[d] = c.coarse_cell[3 * (size_t)a + d] + o[d];
+          in = in && cc[d] >= 0 && cc[d] < nc[d];
+        }
+        if (in) nbr[27 * (size_t)a + s] = cell_agg[cc[0] + (int64_t)nc[0] * (cc[1] + (int64_t)nc[1] * cc[2])];
+      }
+    // offsets scaled by half the cell size; row masks of the neighbour cells touched
+    const double sc = 2.0 / H;
+    std::vector<float> dv(3 * (size_t)N);
+    std::vector<uint32_t> mask(N, 0);
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (int32_t i = 0; i < N; ++i) {
+      const int32_t a = agg[i];
+      for (int d = 0; d < 3; ++d) dv[3 * (size_t)i + d] = (float)((X[3 * (int64_t)i + d] - cen[3 * (size_t)a + d]) * sc);
+      uint32_t m = 1u << 13;
+      auto add = [&](int32_t j) {
+        if (j >= N) return;
+        int s = 0;
+        for (int d = 2; d >= 0; --d) s = 3 * s + (cell[3 * (size_t)j + d] - cell[3 * (size_t)i + d] + 1);
+        m |= 1u << s;
+      };
+      for (int32_t q = iL[i]; q < iL[i + 1]; ++q) add(tL[q] - 1);
+      for (int32_t q = iU[i]; q < iU[i + 1]; ++q) add(tU[q] - 1);
+      mask[i] = m;
+    }
+    CoarseOp& op = c.coarse;
+    op.n_agg = na;
+    op.nc = 6 * na;
+    op.ldw = (op.nc + 63) / 64 * 64;
+    if (c.cg_agg.ensure(sizeof(int32_t) * NP) || c.cg_aptr.ensure(sizeof(int32_t) * (na + 1)) ||
+        c.cg_anodes.ensure(sizeof(int32_t) * N) || c.cg_nbr.ensure(sizeof(int32_t) * 27 * (size_t)na) ||
+        c.cg_mask.ensure(sizeof(uint32_t) * N) || c.cg_dv.ensure(sizeof(float) * 3 * (size_t)N) ||
+        c.cg_Eblk.ensure(sizeof(double) * 27 * 36 * (size_t)na) ||
+        c.cg_E.ensure(sizeof(double) * (size_t)op.nc * op.nc) || c.cg_W.ensure(sizeof(float) * (size_t)op.nc * op.ldw) ||
+        c.cg_cv.ensure(sizeof(float) * op.ldw) || c.cg_yv.ensure(sizeof(float) * op.ldw) ||
+        c.cg_info.ensure(sizeof(int32_t) * 2))
+      return HECMW33_ERROR_DEVICE;
+    HIPC(hipMemcpyAsync(c.cg_agg.p, agg.data(), sizeof(int32_t) * NP, hipMemcpyHostToDevice, c.stream));
+    HIPC(hipMemcpyAsync(c.cg_aptr.p, cnt.data(), sizeof(int32_t) * (na + 1), hipMemcpyHostToDevice, c.stream));
+    HIPC(hipMemcpyAsync(c.cg_anodes.p, anodes.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, c.stream));
+    HIPC(hipMemcpyAsync(c.cg_nbr.p, nbr.data(), sizeof(int32_t) * 27 * (size_t)na, hipMemcpyHostToDevice, c.stream));
+    HIPC(hipMemcpyAsync(c.cg_mask.p, mask.data(), sizeof(uint32_t) * N, hipMemcpyHostToDevice, c.stream));
+    HIPC(hipMemcpyAsync(c.cg_dv.p, dv.data(), sizeof(float) * 3 * (size_t)N, hipMemcpyHostToDevice, c.stream));
+    HIPC(hipMemsetAsync(c.cg_cv.p, 0, sizeof(float) * op.ldw, c.stream));
+    op.agg = c.cg_agg.as<int32_t>();
+    op.aptr = c.cg_aptr.as<int32_t>();
+    op.anodes = c.cg_anodes.as<int32_t>();
+    op.dv = c.cg_dv.as<float>();
+    op.W = c.cg_W.as<float>();
+    op.cv = c.cg_cv.as<float>();
+    op.yv = c.cg_yv.as<float>();
+    HIPC(hipStreamSynchronize(c.stream));  // the host vectors above go out of scope
+  }
+  if (!ok) {
+    set_error("coarse correction: no aggregation with neighbour-cell coupling found");
+    return HECMW33_ERROR_ARGS;
+  }
+  c.coarse_sym = true;
+  c.coarse_num = false;
+  c.coarse_target = target;
+  c.coarse_coord_gen = c.coord_gen;
+  return 0;
+}
+
+// E = Z^T A Z on the device, Cholesky factor and inverse by rocSOLVER (FP64),
+// then the FP32 copy the iterations read
+static int32_t coarse_numeric(Context& c) {
+  CoarseOp& op = c.coarse;
+  const int32_t nc = op.nc;
+  HIPC(hipMemsetAsync(c.cg_E.p, 0, sizeof(double) * (size_t)nc * nc, c.stream));
+  coarse_assemble(c.A.view, op, c.cg_nbr.as<int32_t>(), c.cg_mask.as<uint32_t>(), c.cg_Eblk.as<double>(),
+                  c.cg_E.as<double>(), nc, c.stream);
+  HIPC(hipGetLastError());
+  if (!c.blas && rocblas_create_handle(&c.blas) != rocblas_status_success) {
+    c.blas = nullptr;
+    set_error("rocblas_create_handle failed");
+    return HECMW33_ERROR_DEVICE;
+  }
+  if (rocblas_set_stream(c.blas, c.stream) != rocblas_status_success) {
+    set_error("rocblas_set_stream failed");
+    return HECMW33_ERROR_DEVICE;
+  }
+  int32_t* info = c.cg_info.as<int32_t>();
+  if (rocsolver_dpotrf(c.blas, rocblas_fill_lower, nc, c.cg_E.as<double>(), nc, info) != rocblas_status_success ||
+      rocsolver_dpotri(c.blas, rocblas_fill_lower, nc, c.cg_E.as<double>(), nc, info + 1) != rocblas_status_success) {
+    set_error("rocsolver potrf/potri failed");
+    return HECMW33_ERROR_DEVICE;
+  }
+  int32_t hinfo[2] = {0, 0};
+  HIPC(hipMemcpyAsync(hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, c.stream));
+  HIPC(hipStreamSynchronize(c.stream));
+  c.coarse_ok = hinfo[0] == 0 && hinfo[1] == 0;
+  if (c.coarse_ok) coarse_inverse_f32(op, c.cg_E.as<double>(), nc, c.stream);
+  HIPC(hipGetLastError());
+  c.coarse_num = true;
+  return 0;
+}
+
+static int32_t coarse_setup(Context& c, int32_t target) {
+  if ((int64_t)c.h_coord.size() < 3 * (int64_t)c.NP) {
+    set_error("SOLVER_OPT(3) (two-level correction) needs the node coordinates (hecmw33_set_coords)");
+    return HECMW33_ERROR_ARGS;
+  }
+  target = std::max(1, std::min(target, kCoarseAggMax));
+  if (!c.coarse_sym || c.coarse_target != target || c.coarse_coord_gen != c.coord_gen)
+    if (int32_t e = coarse_symbolic(c, target)) return e;
+  if (!c.coarse_num)
+    if (int32_t e = coarse_numeric(c)) return e;
+  return 0;
+}
+
 static void enqueue_fcg_iteration(Context& c, int32_t kind, bool spmv64, hipEvent_t* ev) {
   const int64_t n3 = 3 * (int64_t)c.N;
   KrylovState* st = c.st.as<KrylovState>();
   double* part = c.part.as<double>();
   float *r = c.vr32.as<float>(), *z = c.vz32.as<float>(), *p = c.vp32.as<float>(), *q = c.vq32.as<float>();
   if (ev) hipEventRecord(ev[0], c.stream);
-  fcg_precond_dots(kind, c.ssor_op32(), c.alu_diag32.as<float>(), c.N, r, z, q, st, part, c.stream);
+  if (c.coarse_use) {
+    precond_apply_f32(kind, c.ssor_op32(), c.alu_diag32.as<float>(), c.N, r, z, st, c.stream);
+    fcg_beta(st, part, coarse_apply(c.coarse, c.N, r, z, q, st, part, true, c.stream), c.stream);
+  } else {
+    fcg_precond_dots(kind, c.ssor_op32(), c.alu_diag32.as<float>(), c.N, r, z, q, st, part, c.stream);
+  }
   if (ev) hipEventRecord(ev[1], c.stream);
   fcg_p(n3, z, p, st, c.stream);
   if (ev) hipEventRecord(ev[2], c.stream);
@@ -793,6 +1051,7 @@ static void enqueue_fcg_learned_iteration(Context& c, bool spmv64, hipEvent_t* e
   float *p = c.vp32.as<float>(), *q = c.vq32.as<float>();
   if (ev) hipEventRecord(ev[0], c.stream);
   ssor_apply_f32(c.ssor_op32(), r, z0, st, c.stream);
+  if (c.coarse_use) coarse_apply(c.coarse, c.N, r, z0, nullptr, st, nullptr, false, c.stream);
   int np = 0;
   learned_combine(c.N, c.vcoef.as<float>(), r, z0, c.alu_diag32.as<float>(), q, z, st, part, &np, c.stream);
   fcg_beta_learned(st, part, np, c.stream);
@@ -842,6 +1101,7 @@ static int32_t scaling_apply(Context& c, bool fw) {
   launch_fill_values(c.A.view, c.A.src.as<int64_t>(), c.dD.as<double>(), c.dAL.as<double>(), c.dAU.as<double>(),
                      c.N, c.NPL, c.stream);
   c.a32_valid = false;
+  c.coarse_num = false;
   HIPC(hipGetLastError());
   HIPC(hipStreamSynchronize(c.stream));
   return 0;
@@ -906,6 +1166,14 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
     }
   }
   c.last_spmv64 = spmv64;
+  c.coarse_use = false;
+  c.last_coarse = 0;
+  if (mixed && IA(iarray, I_SOLVER_OPT3) > 0) {
+    const int32_t o3 = IA(iarray, I_SOLVER_OPT3);
+    if (int32_t e = coarse_setup(c, o3 == 1 ? kCoarseAggDefault : o3)) return e;
+    c.coarse_use = c.coarse_ok;
+    c.last_coarse = c.coarse_ok ? c.coarse.n_agg : -1;
+  }
   if (learned) {
     if (int32_t e = learned_setup(c, RA(rarray, R_SIGMA_DIAG))) return e;
     // the network's coefficients for this system, fixed for the solve
@@ -1418,7 +1686,7 @@ int32_t hecmw33_ctx_history(hecmw33_ctx* ctx, double* hist, int32_t cap) {
 int32_t hecmw33_ctx_info(hecmw33_ctx* ctx, int64_t* info, int32_t n) {
   if (!ctx || !info) return HECMW33_ERROR_ARGS;
   Context& c = ctx->c;
-  const int64_t v[11] = {c.A.view.nslots,
+  const int64_t v[12] = {c.A.view.nslots,
                         (int64_t)c.N,
                         (int64_t)c.NPL,
                         (int64_t)c.NPU,
@@ -1428,8 +1696,9 @@ int32_t hecmw33_ctx_info(hecmw33_ctx* ctx, int64_t* info, int32_t n) {
                         (int64_t)c.last_nfallback,
                         (int64_t)c.last_learned_off,
                         (int64_t)c.last_spmv64,
-                        (int64_t)std::llround(c.last_probe_ratio * 1e6)};
-  for (int i = 0; i < n && i < 11; ++i) info[i] = v[i];
+                        (int64_t)std::llround(c.last_probe_ratio * 1e6),
+                        (int64_t)c.last_coarse};
+  for (int i = 0; i < n && i < 12; ++i) info[i] = v[i];
   return 0;
 }
 
@@ -1680,6 +1949,23 @@ int32_t hecmw33_net_score(hecmw33_net* net, double* mean, double* base) {
   *mean = acc / (double)N.samples.size();
   if (base) *base = b / (double)N.samples.size();
   return 0;
+}
+
+static int32_t set_coords(Context& c, int32_t NP, const double* xyz) {
+  if (NP < 0 || (NP > 0 && !xyz)) return HECMW33_ERROR_ARGS;
+  c.h_coord.assign(xyz, xyz + 3 * (size_t)NP);
+  ++c.coord_gen;
+  return 0;
+}
+
+int32_t hecmw33_ctx_set_coords(hecmw33_ctx* ctx, int32_t NP, const double* xyz) {
+  if (!ctx) return HECMW33_ERROR_ARGS;
+  return set_coords(ctx->c, NP, xyz);
+}
+
+int32_t hecmw33_set_coords(int32_t NP, const double* xyz) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return set_coords(*default_ctx(), NP, xyz);
 }
 
 int32_t hecmw33_ctx_set_net(hecmw33_ctx* ctx, hecmw33_net* net) {

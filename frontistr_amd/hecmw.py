@@ -105,6 +105,8 @@ def lib():
     L.hecmw33_net_score.argtypes = [P, P, P]
     L.hecmw33_ctx_set_net.argtypes = [P, P]
     L.hecmw33_set_net.argtypes = [P]
+    L.hecmw33_set_coords.argtypes = [I, P]
+    L.hecmw33_ctx_set_coords.argtypes = [P, I, P]
     L.hecmw33_last_error.restype = C.c_char_p
     for f in ("hecmw33_solve", "hecmw33_solve_krylov", "hecmw33_matvec", "hecmw33_precond_apply",
               "hecmw33_rel_resid_L2", "hecmw33_ordering", "hecmw33_ctx_load", "hecmw33_ctx_solve_device",
@@ -115,7 +117,7 @@ def lib():
               "hecmw33_batch_spmm", "hecmw33_batch_info", "hecmw33_solve_batch", "hecmw33_net_weight_count",
               "hecmw33_net_get_weights", "hecmw33_net_set_weights", "hecmw33_net_eval", "hecmw33_net_add_sample",
               "hecmw33_net_clear_samples", "hecmw33_net_train", "hecmw33_net_score", "hecmw33_ctx_set_net",
-              "hecmw33_set_net"):
+              "hecmw33_set_net", "hecmw33_set_coords", "hecmw33_ctx_set_coords"):
         getattr(L, f).restype = C.c_int32
     _lib = L
     return L
@@ -209,6 +211,12 @@ class HecMatrix:
         self.set_i(self.I_SOLVER_OPT, 1 if on else 0)
         self.set_i(self.I_SOLVER_OPT + 1, spmv)
         self.Rarray[self.R_SOLVER_OPT - 1] = 0.0 if delta is None else float(delta)
+
+    def set_coarse(self, n=1):
+        """Two-level correction of the flexible CG (Iarray 43): 0 off, 1 about
+        1000 aggregates, larger values that many (needs node coordinates:
+        set_coords / DeviceContext.set_coords)."""
+        self.set_i(43, int(n))
 
     def set_sigma_diag(self, v):
         # hecmw_mat_set_sigma_diag clamps to [1, 2], negative = automatic (-1)
@@ -353,6 +361,23 @@ def gen_hex8(n, kind="uniform", E=210000.0, nu=0.3, seed=0, emin=1e3, emax=1e6, 
     return m
 
 
+def hex8_coords(n):
+    """Node coordinates of gen_hex8(n) (node i + (n+1)(j + (n+1)k) at (i, j, k)/n), 3*(n+1)^3."""
+    nn = n + 1
+    idx = np.arange(nn ** 3)
+    return (np.stack([idx % nn, (idx // nn) % nn, idx // (nn * nn)], 1) / float(n)).ravel()
+
+
+def set_coords(xyz):
+    """Node coordinates for hecmw_solve_iterative / hecmw_solve_krylov's two-level correction."""
+    global _coords
+    _coords = np.ascontiguousarray(xyz, dtype=np.float64).ravel()
+    _check(lib().hecmw33_set_coords(_coords.size // 3, _ptr(_coords)))
+
+
+_coords = None
+
+
 def gen_tet10(nx, ny, nz, box=(1.0, 1.0, 1.0), E=210000.0, nu=0.3, nrhs=1, seed=0):
     """TET10 (fistr1 type 342) box on an nx x ny x nz cell grid, 6 tetrahedra per
     cell, bottom face fixed, nrhs columns of N(0,1) nodal loads on the top face
@@ -425,11 +450,16 @@ class DeviceContext:
         return h[:n]
 
     def info(self):
-        v = np.zeros(11, np.int64)
-        _check(lib().hecmw33_ctx_info(self._h, _ptr(v), 11))
+        v = np.zeros(12, np.int64)
+        _check(lib().hecmw33_ctx_info(self._h, _ptr(v), 12))
         return {"nslots": int(v[0]), "N": int(v[1]), "NPL": int(v[2]), "NPU": int(v[3]), "ncolor": int(v[4]),
                 "ssor_slots": int(v[5]), "nreliable": int(v[6]), "nfallback": int(v[7]), "learned_off": int(v[8]),
-                "spmv64": int(v[9]), "probe_ratio": v[10] / 1e6}
+                "spmv64": int(v[9]), "probe_ratio": v[10] / 1e6, "coarse": int(v[11])}
+
+    def set_coords(self, xyz):
+        """Node coordinates (3*NP, xyz interleaved) for the two-level correction."""
+        self._coords = np.ascontiguousarray(xyz, dtype=np.float64).ravel()
+        _check(lib().hecmw33_ctx_set_coords(self._h, self._coords.size // 3, _ptr(self._coords)))
 
     def set_net(self, net):
         """Attach a LearnedNet for solves with Iarray(41) = 2 (None detaches)."""

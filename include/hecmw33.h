@@ -98,7 +98,23 @@ int32_t hecmw33_solve_krylov(const hecmw33_sys* A, const double* B, double* X, i
  *   2 FP64 (FP32 vectors, FP64 accumulation), 0 (default) FP64 when the
  *   stiffness contrast max/min |tr D_ii| exceeds 1e4 (an FP32-rounded operator
  *   loses the search directions' conjugacy on high-contrast systems), else FP32.
+ *   iarray[42] (Iarray 43) > 0 adds a two-level correction to the flexible CG's
+ *   preconditioner, z = M^-1 r + Z E^-1 Z^T r: Z holds the six rigid-body modes
+ *   (three translations, three rotations about the centroid) of each aggregate,
+ *   the non-empty cells of a uniform grid over the node coordinates given by
+ *   hecmw33_set_coords / hecmw33_ctx_set_coords; E = Z^T A Z is inverted densely
+ *   (FP64 Cholesky, FP32 copy for the iterations).  1 aims at 1000 aggregates,
+ *   larger values at that many (at most 2048).  Cells grow until each matrix
+ *   block couples nodes of the same or adjacent cells.  If E is not positive
+ *   definite the solve runs without it (hecmw33_ctx_info [11] = -1).
  *   Accepted by hecmw33_solve, hecmw33_solve_krylov and hecmw33_ctx_solve_device. */
+
+/* Node coordinates for the two-level correction: xyz interleaved, 3*NP doubles
+ * (hecMESH%node, hecmw1/src/common/hecmw_util_f.F90:264; the reference's ML
+ * preconditioner reads the same array for its rigid-body modes,
+ * hecmw1/src/solver/precond/ml/hecmw_ML_helper.f90:35-37).  Copied; a
+ * new call replaces them and rebuilds the aggregation at the next solve. */
+int32_t hecmw33_set_coords(int32_t NP, const double* xyz);
 
 /* y = A x for the internal rows (hecmw_matvec_33, hecmw1/src/solver/las/hecmw_solver_las_33.F90:34-59).
  * Host pointers, x has 3*NP entries, y receives 3*N. */
@@ -150,8 +166,11 @@ int32_t hecmw33_ctx_history(hecmw33_ctx* ctx, double* hist, int32_t cap);
  * [7] iterations of the last learned solve that fell back to the SSOR output, [8] 1 if that
  * solve dropped the learned correction (or was not a learned solve), [9] 1 if the last flexible
  * CG used the FP64 matrix values in q = A p, [10] the last learned probe's error reduction with
- * / without the correction, in millionths. */
+ * / without the correction, in millionths, [11] aggregates of the last flexible CG's two-level
+ * correction (0: none, -1: E was not positive definite and the solve ran without it). */
 int32_t hecmw33_ctx_info(hecmw33_ctx* ctx, int64_t* info, int32_t n);
+/* the two-level correction's node coordinates for a device context (as hecmw33_set_coords) */
+int32_t hecmw33_ctx_set_coords(hecmw33_ctx* ctx, int32_t NP, const double* xyz);
 
 /* ---- batched solves (no counterpart in the reference: hecmw_solve_CG handles one system and
  * one right-hand side per call; these solve many systems with up to 16 right-hand sides each
