@@ -32,6 +32,7 @@ from .hecmw import (  # noqa: F401
     gen_hex8,
     gen_tet10,
     hex8_coords,
+    tet10_coords,
     set_coords,
     BatchSolver,
     solve_batch,

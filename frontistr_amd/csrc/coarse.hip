@@ -126,6 +126,13 @@ __global__ void k_coarse_scatter(int32_t n_agg, const int32_t* __restrict__ nbr,
   E[(int64_t)(6 * b + q) * lda + 6 * a + p] = Eblk[t];
 }
 
+// unit pivot for a coarse unknown whose column of Z is zero (E(k,k) exactly 0):
+// its restriction is exactly 0, so its value stays 0 and E stays invertible
+__global__ void k_coarse_unit_pivots(int32_t nc, double* __restrict__ E, int32_t lda) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nc && E[(int64_t)k * lda + k] == 0.0) E[(int64_t)k * lda + k] = 1.0;
+}
+
 // W = E^-1 in FP32, full symmetric from the lower triangle the inverse leaves
 __global__ void k_coarse_inv32(int32_t nc, const double* __restrict__ Ei, int32_t lda, float* __restrict__ W,
                                int32_t ldw) {
@@ -228,6 +235,7 @@ void coarse_assemble(const Sell<double>& A, const CoarseOp& op, const int32_t* n
   k_coarse_E<<<(unsigned)npair, kCoarseBlock, 0, s>>>(A, op.agg, op.aptr, op.anodes, nbr, mask, op.dv, Eblk);
   const int64_t ne = npair * 36;
   k_coarse_scatter<<<(unsigned)((ne + 255) / 256), 256, 0, s>>>(op.n_agg, nbr, Eblk, E, lda);
+  k_coarse_unit_pivots<<<(op.nc + 255) / 256, 256, 0, s>>>(op.nc, E, lda);
 }
 
 void coarse_inverse_f32(const CoarseOp& op, const double* Ei, int32_t lda, hipStream_t s) {

@@ -88,6 +88,8 @@ constexpr int32_t kCoarseAggDefault = 1000;
 // the dense coarse inverse is n_c^2 entries (n_c = 6 per aggregate): at most
 // 2048 aggregates (n_c = 12288, 0.6 GB in FP32)
 constexpr int32_t kCoarseAggMax = 2048;
+// and at least this many nodes per aggregate on average
+constexpr int32_t kCoarseMinNodes = 32;
 static inline int32_t& IA(int32_t* a, int k) { return a[k - 1]; }
 static inline int32_t IA(const int32_t* a, int k) { return a[k - 1]; }
 static inline double RA(const double* a, int k) { return a[k - 1]; }
@@ -911,6 +913,27 @@ static int32_t coarse_symbolic(Context& c, int32_t target) {
         }
         if (in) nbr[27 * (size_t)a + s] = cell_agg[cc[0] + (int64_t)nc[0] * (cc[1] + (int64_t)nc[1] * cc[2])];
       }
+    // an aggregate whose nodes are collinear (or a single node) has a rotation
+    // that moves none of them: its offsets are set to 0, so all three rotation
+    // columns of Z vanish there, and E gets unit pivots for them
+    std::vector<char> degen(na, 0);
+#pragma omp parallel for schedule(static)
+    for (int32_t a = 0; a < na; ++a) {
+      double M[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      for (int32_t t = cnt[a]; t < cnt[a + 1]; ++t) {
+        const int64_t i = anodes[t];
+        double r[3];
+        for (int d = 0; d < 3; ++d) r[d] = X[3 * i + d] - cen[3 * (size_t)a + d];
+        for (int p = 0; p < 3; ++p)
+          for (int q = 0; q < 3; ++q) M[p][q] += r[p] * r[q];
+      }
+      const double tr = M[0][0] + M[1][1] + M[2][2];
+      double f2 = 0.0;
+      for (int p = 0; p < 3; ++p)
+        for (int q = 0; q < 3; ++q) f2 += M[p][q] * M[p][q];
+      // (tr^2 - |M|_F^2) / 2 = l1 l2 + l1 l3 + l2 l3: ~0 when rank(M) <= 1
+      degen[a] = !(0.5 * (tr * tr - f2) > 1e-8 * tr * tr);
+    }
     // offsets scaled by half the cell size; row masks of the neighbour cells touched
     const double sc = 2.0 / H;
     std::vector<float> dv(3 * (size_t)N);
@@ -918,7 +941,8 @@ static int32_t coarse_symbolic(Context& c, int32_t target) {
 #pragma omp parallel for schedule(dynamic, 4096)
     for (int32_t i = 0; i < N; ++i) {
       const int32_t a = agg[i];
-      for (int d = 0; d < 3; ++d) dv[3 * (size_t)i + d] = (float)((X[3 * (int64_t)i + d] - cen[3 * (size_t)a + d]) * sc);
+      for (int d = 0; d < 3; ++d)
+        dv[3 * (size_t)i + d] = degen[a] ? 0.f : (float)((X[3 * (int64_t)i + d] - cen[3 * (size_t)a + d]) * sc);
       uint32_t m = 1u << 13;
       auto add = [&](int32_t j) {
         if (j >= N) return;
@@ -1008,7 +1032,7 @@ static int32_t coarse_setup(Context& c, int32_t target) {
     set_error("SOLVER_OPT(3) (two-level correction) needs the node coordinates (hecmw33_set_coords)");
     return HECMW33_ERROR_ARGS;
   }
-  target = std::max(1, std::min(target, kCoarseAggMax));
+  target = std::max(1, std::min(std::min(target, kCoarseAggMax), c.N / kCoarseMinNodes));
   if (!c.coarse_sym || c.coarse_target != target || c.coarse_coord_gen != c.coord_gen)
     if (int32_t e = coarse_symbolic(c, target)) return e;
   if (!c.coarse_num)
@@ -1966,6 +1990,30 @@ int32_t hecmw33_ctx_set_coords(hecmw33_ctx* ctx, int32_t NP, const double* xyz) 
 int32_t hecmw33_set_coords(int32_t NP, const double* xyz) {
   std::lock_guard<std::mutex> lk(g_mu);
   return set_coords(*default_ctx(), NP, xyz);
+}
+
+int32_t hecmw33_ctx_coarse_get(hecmw33_ctx* ctx, int32_t* n_agg, int32_t* agg, float* dv, int32_t* nbr,
+                               double* Eblk) {
+  if (!ctx || !n_agg) return HECMW33_ERROR_ARGS;
+  Context& c = ctx->c;
+  if (!c.coarse_sym) {
+    *n_agg = 0;
+    return 0;
+  }
+  const int32_t na = c.coarse.n_agg;
+  *n_agg = na;
+  HIPC(hipStreamSynchronize(c.stream));
+  if (agg) HIPC(hipMemcpy(agg, c.cg_agg.p, sizeof(int32_t) * c.N, hipMemcpyDeviceToHost));
+  if (dv) HIPC(hipMemcpy(dv, c.cg_dv.p, sizeof(float) * 3 * (size_t)c.N, hipMemcpyDeviceToHost));
+  if (nbr) HIPC(hipMemcpy(nbr, c.cg_nbr.p, sizeof(int32_t) * 27 * (size_t)na, hipMemcpyDeviceToHost));
+  if (Eblk) {
+    if (!c.coarse_num) {
+      set_error("coarse matrix not assembled for the current values");
+      return HECMW33_ERROR_ARGS;
+    }
+    HIPC(hipMemcpy(Eblk, c.cg_Eblk.p, sizeof(double) * 27 * 36 * (size_t)na, hipMemcpyDeviceToHost));
+  }
+  return 0;
 }
 
 int32_t hecmw33_ctx_set_net(hecmw33_ctx* ctx, hecmw33_net* net) {

@@ -107,6 +107,7 @@ def lib():
     L.hecmw33_set_net.argtypes = [P]
     L.hecmw33_set_coords.argtypes = [I, P]
     L.hecmw33_ctx_set_coords.argtypes = [P, I, P]
+    L.hecmw33_ctx_coarse_get.argtypes = [P, P, P, P, P, P]
     L.hecmw33_last_error.restype = C.c_char_p
     for f in ("hecmw33_solve", "hecmw33_solve_krylov", "hecmw33_matvec", "hecmw33_precond_apply",
               "hecmw33_rel_resid_L2", "hecmw33_ordering", "hecmw33_ctx_load", "hecmw33_ctx_solve_device",
@@ -117,7 +118,7 @@ def lib():
               "hecmw33_batch_spmm", "hecmw33_batch_info", "hecmw33_solve_batch", "hecmw33_net_weight_count",
               "hecmw33_net_get_weights", "hecmw33_net_set_weights", "hecmw33_net_eval", "hecmw33_net_add_sample",
               "hecmw33_net_clear_samples", "hecmw33_net_train", "hecmw33_net_score", "hecmw33_ctx_set_net",
-              "hecmw33_set_net", "hecmw33_set_coords", "hecmw33_ctx_set_coords"):
+              "hecmw33_set_net", "hecmw33_set_coords", "hecmw33_ctx_set_coords", "hecmw33_ctx_coarse_get"):
         getattr(L, f).restype = C.c_int32
     _lib = L
     return L
@@ -368,6 +369,15 @@ def hex8_coords(n):
     return (np.stack([idx % nn, (idx // nn) % nn, idx // (nn * nn)], 1) / float(n)).ravel()
 
 
+def tet10_coords(nx, ny, nz, box=(1.0, 1.0, 1.0)):
+    """Node coordinates of gen_tet10(nx, ny, nz, box): the half-cell lattice,
+    node x + X(y + Y z) at (x/(2nx) Lx, y/(2ny) Ly, z/(2nz) Lz)."""
+    X, Y, Z = 2 * nx + 1, 2 * ny + 1, 2 * nz + 1
+    idx = np.arange(X * Y * Z)
+    return np.stack([(idx % X) * (box[0] / (2 * nx)), ((idx // X) % Y) * (box[1] / (2 * ny)),
+                     (idx // (X * Y)) * (box[2] / (2 * nz))], 1).ravel()
+
+
 def set_coords(xyz):
     """Node coordinates for hecmw_solve_iterative / hecmw_solve_krylov's two-level correction."""
     global _coords
@@ -455,6 +465,23 @@ class DeviceContext:
         return {"nslots": int(v[0]), "N": int(v[1]), "NPL": int(v[2]), "NPU": int(v[3]), "ncolor": int(v[4]),
                 "ssor_slots": int(v[5]), "nreliable": int(v[6]), "nfallback": int(v[7]), "learned_off": int(v[8]),
                 "spmv64": int(v[9]), "probe_ratio": v[10] / 1e6, "coarse": int(v[11])}
+
+    def coarse(self):
+        """The last two-level set-up: dict(n_agg, agg [N], dv [N,3], nbr [n_agg,27], Eblk [n_agg,27,6,6])."""
+        na = C.c_int32(0)
+        _check(lib().hecmw33_ctx_coarse_get(self._h, C.byref(na), None, None, None, None))
+        na = na.value
+        if na == 0:
+            return None
+        N = self.info()["N"]
+        agg = np.zeros(N, np.int32)
+        dv = np.zeros(3 * N, np.float32)
+        nbr = np.zeros(27 * na, np.int32)
+        E = np.zeros(27 * 36 * na)
+        n2 = C.c_int32(0)
+        _check(lib().hecmw33_ctx_coarse_get(self._h, C.byref(n2), _ptr(agg), _ptr(dv), _ptr(nbr), _ptr(E)))
+        return {"n_agg": na, "agg": agg, "dv": dv.reshape(N, 3), "nbr": nbr.reshape(na, 27),
+                "Eblk": E.reshape(na, 27, 6, 6)}
 
     def set_coords(self, xyz):
         """Node coordinates (3*NP, xyz interleaved) for the two-level correction."""

@@ -171,6 +171,12 @@ int32_t hecmw33_ctx_history(hecmw33_ctx* ctx, double* hist, int32_t cap);
 int32_t hecmw33_ctx_info(hecmw33_ctx* ctx, int64_t* info, int32_t n);
 /* the two-level correction's node coordinates for a device context (as hecmw33_set_coords) */
 int32_t hecmw33_ctx_set_coords(hecmw33_ctx* ctx, int32_t NP, const double* xyz);
+/* Inspection of the last two-level set-up: *n_agg aggregates (0: none built); any non-null
+ * array receives agg (N: aggregate of each node), dv (3N: the scaled offsets from the
+ * aggregate centroid that define Z), nbr (27*n_agg: aggregate of neighbour cell
+ * s = 9(dz+1)+3(dy+1)+(dx+1), or -1) and Eblk (27*36*n_agg: E_(a,nbr[27a+s]) row-major 6x6). */
+int32_t hecmw33_ctx_coarse_get(hecmw33_ctx* ctx, int32_t* n_agg, int32_t* agg, float* dv, int32_t* nbr,
+                               double* Eblk);
 
 /* ---- batched solves (no counterpart in the reference: hecmw_solve_CG handles one system and
  * one right-hand side per call; these solve many systems with up to 16 right-hand sides each
