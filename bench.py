@@ -11,10 +11,13 @@ the symbolic RCM+multicolour ordering is built once at load).
 
 Also reported: DIAG-CG on the same system, the FP64/FP32 SpMV bandwidth, and
 the reference's own CPU solver (oracle/_ref, built from the FrontISTR sources)
-timed on the host on a bounded sample of the same solve; the mixed-precision
-comparison on the 160^3 heterogeneous cube ("mixed": FP64 SSOR-CG vs the
-flexible CG with FP32 operators, same GPU); the batched workload ("batch": 48
-TET10 systems x 16 right-hand sides).
+timed on the host on a bounded sample of the same solve; the 160^3
+heterogeneous cube ("mixed": FP64 SSOR-CG vs the flexible CG with FP32
+operators, alone, with the two-level rigid-body-mode correction, and with the
+learned correction on top, same GPU); the batched workload ("batch": 48 TET10
+systems x 16 right-hand sides); the 120^3 two-phase inclusion cube
+("learned": the same modes plus the learned correction alone, with the
+network trained on the GPU on solver-harvested residuals first).
 
     python bench.py [--steps K] [--warmup W] [--n 100] [--gpus 1] [--c3-n 160] [--c4-nsys 48]
 """
@@ -174,11 +177,15 @@ def main():
         cpu = cpu_baseline(m, iters, args.cpu_sample_s)
     del ctx, b, x, y, xr, x32, y32
     torch.cuda.empty_cache()
-    mixed = bench_mixed(args, dev) if args.c3_n > 0 else None
+    net, train = train_net(args, dev) if (args.c5_n > 0 or args.c3_n > 0) and args.c5_variants > 0 else (None, None)
+    torch.cuda.empty_cache()
+    mixed = bench_mixed(args, dev, net) if args.c3_n > 0 else None
     torch.cuda.empty_cache()
     batch = bench_batch(args) if args.c4_nsys > 0 else None
     torch.cuda.empty_cache()
-    learned = bench_learned(args, dev) if args.c5_n > 0 else None
+    learned = bench_learned(args, dev, net, train) if args.c5_n > 0 else None
+    if net is not None:
+        net.close()
 
     out = {
         "metric": "time to 1e-8 rel. residual: mixed+learned FCG vs plain CG; BCSR SpMV GB/s",
@@ -220,12 +227,62 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def bench_mixed(args, dev):
+# flexible CG variants: (name, Iarray 41, Iarray 43)
+FCG_MODES = (("fp64_ssor_cg", 0, 0), ("mixed_fcg", 1, 0), ("mixed_coarse_fcg", 1, 1), ("learned_coarse_fcg", 2, 1))
+
+
+def _run_modes(tag, fa, ctx, m, b, x, net, modes):
+    """Each mode from x0 = 0 to 1e-8 with the system resident and a numeric
+    preconditioner refresh (SSOR factors and, with Iarray 43, the coarse
+    inverse) inside the timed solve."""
+    import torch
+    res, xs = {}, {}
+    for mode, opt, coarse in modes:
+        if opt == 2 and net is None:
+            continue
+        ia = m.Iarray.copy()
+        ia[96] = 1
+        ia[97] = 0
+        ia[40] = opt
+        ia[42] = coarse
+        x.zero_()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        r = ctx.solve_device(b.data_ptr(), x.data_ptr(), ia, m.Rarray)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t1
+        xs[mode] = x.cpu().numpy()
+        m.X[:] = xs[mode]
+        inf = ctx.info()
+        res[mode] = {"seconds": round(wall, 4), "iterations": r.iter, "error": r.error, "Tset": round(r.Tset, 4),
+                     "Tsol": round(r.Tsol, 4), "ms_per_iter": round(1e3 * r.Tsol / max(1, r.iter), 4),
+                     "time_Ax": round(r.time_Ax, 4), "time_precond": round(r.time_precond, 4),
+                     "rel_resid_true": fa.hecmw_rel_resid_L2(m)}
+        if opt > 0:
+            res[mode]["fp64_residual_updates"] = inf["nreliable"]
+            res[mode]["spmv_fp64_values"] = bool(inf["spmv64"])
+        if coarse:
+            res[mode]["coarse_aggregates"] = inf["coarse"]
+        if opt == 2:
+            res[mode]["fallback_iterations"] = inf["nfallback"]
+            res[mode]["correction_dropped"] = bool(inf["learned_off"])
+            res[mode]["probe_ratio"] = inf["probe_ratio"]
+        log("%s %s: %s" % (tag, mode, res[mode]))
+    ref_x = xs["fp64_ssor_cg"]
+    base = res["fp64_ssor_cg"]["seconds"]
+    for mode in res:
+        res[mode]["rel_err_vs_fp64"] = float(np.linalg.norm(xs[mode] - ref_x) / np.linalg.norm(ref_x))
+        if mode != "fp64_ssor_cg":
+            res[mode]["speedup_vs_fp64"] = round(base / res[mode]["seconds"], 3)
+    return res
+
+
+def bench_mixed(args, dev, net):
     """Heterogeneous cube (per-element E log-uniform in [1e3, 1e6], nu uniform
     in [0.2, 0.4], seed 0): FP64 multicolour-SSOR CG against the flexible CG
-    with FP32 SpMV/SSOR/vectors and FP64 residual replacement, both from
-    x0 = 0 to 1e-8 with the system resident and a numeric preconditioner
-    refresh inside the timed solve."""
+    with FP32 SpMV/SSOR/vectors and FP64 residual replacement, alone, with the
+    two-level (rigid-body-mode coarse space) correction, and with the learned
+    correction on top."""
     import torch
     import frontistr_amd as fa
     n = args.c3_n
@@ -235,43 +292,22 @@ def bench_mixed(args, dev):
     m.set_iter(20000)
     m.set_precond(1)
     m.set_i(m.I_TIMELOG, 1)
+    m.set_coarse(1)  # the aggregation is symbolic set-up, built at load like the colouring
     ctx = fa.DeviceContext()
+    ctx.set_coords(fa.hex8_coords(n))
+    ctx.set_net(net)
     t0 = time.time()
     ctx.load(m)
     t_load = time.time() - t0
     b = torch.from_numpy(m.B).to(dev)
     x = torch.zeros(3 * m.NP, dtype=torch.float64, device=dev)
-    res = {}
-    xs = {}
-    for mode in ("fp64", "mixed"):
-        ia = m.Iarray.copy()
-        ia[96] = 1
-        ia[97] = 0
-        ia[40] = 1 if mode == "mixed" else 0
-        x.zero_()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        r = ctx.solve_device(b.data_ptr(), x.data_ptr(), ia, m.Rarray)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        xs[mode] = x.cpu().numpy()
-        m.X[:] = xs[mode]
-        true = fa.hecmw_rel_resid_L2(m)
-        res[mode] = {"seconds": round(wall, 4), "iterations": r.iter, "error": r.error, "Tset": round(r.Tset, 4),
-                     "Tsol": round(r.Tsol, 4), "ms_per_iter": round(1e3 * r.Tsol / max(1, r.iter), 4),
-                     "time_Ax": round(r.time_Ax, 4), "time_precond": round(r.time_precond, 4),
-                     "rel_resid_true": true}
-        if mode == "mixed":
-            res[mode]["fp64_residual_updates"] = ctx.info()["nreliable"]
-        log("C3 %s: %s" % (mode, res[mode]))
-    err = float(np.linalg.norm(xs["mixed"] - xs["fp64"]) / np.linalg.norm(xs["fp64"]))
+    res = _run_modes("C3", fa, ctx, m, b, x, net, FCG_MODES)
+    ctx.set_net(None)
     ctx.close()
-    return {"workload": "%d^3 HEX8, lognormal E in [1e3,1e6], nu in [0.2,0.4], seed 0, %d DOF, %d blocks"
-                        % (n, 3 * m.N, m.N + m.NPL + m.NPU),
-            "fp64_ssor_cg": res["fp64"], "mixed_fcg": res["mixed"],
-            "speedup": round(res["fp64"]["seconds"] / res["mixed"]["seconds"], 3),
-            "rel_err_mixed_vs_fp64": err, "load_s": round(t_load, 2),
-            "learned": None}
+    out = {"workload": "%d^3 HEX8, lognormal E in [1e3,1e6], nu in [0.2,0.4], seed 0, %d DOF, %d blocks"
+                       % (n, 3 * m.N, m.N + m.NPL + m.NPU), "load_s": round(t_load, 2)}
+    out.update(res)
+    return out
 
 
 def _inclusions(fa, n, seed):
@@ -282,13 +318,11 @@ def _inclusions(fa, n, seed):
     return m
 
 
-def bench_learned(args, dev):
-    """Two-phase inclusion cube (E contrast 1e4, 15% volume, radii 2-6
-    elements).  The network is trained on residuals the solver harvests from
-    FP64 SSOR-CG runs on seeded variants (smaller cubes with the same
-    inclusion statistics, 7 load cases x 9 iteration counts each), then the
-    learned flexible CG, the plain flexible CG (FP32 operators) and the FP64
-    SSOR-CG solve the target cube from x0 = 0 to 1e-8."""
+def train_net(args, dev):
+    """The learned correction's network, trained on residuals the solver
+    harvests from FP64 SSOR-CG runs on seeded variants of the two-phase cube
+    (24^3, the inclusion statistics of the learned-correction workload, 7 load
+    cases x 10 iteration counts each), 4 epochs on the GPU."""
     import torch
     import frontistr_amd as fa
     t0 = time.time()
@@ -296,6 +330,7 @@ def bench_learned(args, dev):
     keep = []
     rng = np.random.default_rng(123)
     nv = args.c5_variants
+    count = 0
     for v in range(nv):
         m = _inclusions(fa, 24, 1000 + v)
         ctx = fa.DeviceContext()
@@ -314,54 +349,38 @@ def bench_learned(args, dev):
     t0 = time.time()
     loss = net.train(epochs=4, lr=2e-3, seed=0)
     t_train = time.time() - t0
-    log("C5: %d residuals from %d variants in %.1fs; training %.1fs, energy %s" % (
+    log("trained: %d residuals from %d variants in %.1fs; training %.1fs, energy %s" % (
         count, nv, t_harvest, t_train, np.round(loss, 4)))
-    t0 = time.time()
-    m = _inclusions(fa, args.c5_n, 7)
-    ctx = fa.DeviceContext()
-    ctx.load(m)
     net.clear_samples()
+    for c in keep:
+        c.close()
+    return net, {"samples": count, "variants": nv, "harvest_s": round(t_harvest, 2), "train_s": round(t_train, 2),
+                 "energy_plain_ssor": loss[0], "energy_per_epoch": [float(v) for v in loss[1:]]}
+
+
+def bench_learned(args, dev, net, train):
+    """Two-phase inclusion cube (E contrast 1e4, 15% volume, radii 2-6
+    elements): FP64 SSOR-CG, the flexible CG with FP32 operators, with the
+    learned correction, and both with the two-level correction."""
+    import torch
+    import frontistr_amd as fa
+    m = _inclusions(fa, args.c5_n, 7)
+    m.set_coarse(1)
+    ctx = fa.DeviceContext()
+    ctx.set_coords(fa.hex8_coords(args.c5_n))
+    ctx.load(m)
     b = torch.from_numpy(m.B).to(dev)
     ctx.set_net(net)
     x = torch.zeros(3 * m.NP, dtype=torch.float64, device=dev)
-    res, xs = {}, {}
-    for mode, opt in (("fp64_ssor_cg", 0), ("mixed_fcg", 1), ("learned_fcg", 2)):
-        ia = m.Iarray.copy()
-        ia[96] = 1
-        ia[97] = 0
-        ia[40] = opt
-        x.zero_()
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        r = ctx.solve_device(b.data_ptr(), x.data_ptr(), ia, m.Rarray)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t1
-        xs[mode] = x.cpu().numpy()
-        m.X[:] = xs[mode]
-        inf = ctx.info()
-        res[mode] = {"seconds": round(wall, 4), "iterations": r.iter, "error": r.error,
-                     "ms_per_iter": round(1e3 * r.Tsol / max(1, r.iter), 4),
-                     "rel_resid_true": fa.hecmw_rel_resid_L2(m)}
-        if opt == 2:
-            res[mode]["fallback_iterations"] = inf["nfallback"]
-            res[mode]["correction_dropped"] = bool(inf["learned_off"])
-        log("C5 %s: %s" % (mode, res[mode]))
+    modes = FCG_MODES[:2] + (("learned_fcg", 2, 0),) + FCG_MODES[2:]
+    res = _run_modes("C5", fa, ctx, m, b, x, net, modes)
     ctx.set_net(None)
-    ref_x = xs["fp64_ssor_cg"]
-    out = {"workload": "%d^3 HEX8, two-phase inclusions (E 1e3 / 1e7, 15%% volume, radii 2-6 elements, seed 7), "
-                       "%d DOF; network trained on %d residuals harvested from SSOR-CG on %d seeded 24^3 variants"
-                       % (args.c5_n, 3 * m.N, count, nv),
-           "train": {"harvest_s": round(t_harvest, 2), "train_s": round(t_train, 2),
-                     "energy_plain_ssor": loss[0], "energy_per_epoch": [float(v) for v in loss[1:]]},
-           "speedup_learned_vs_fp64": round(res["fp64_ssor_cg"]["seconds"] / res["learned_fcg"]["seconds"], 3),
-           "speedup_mixed_vs_fp64": round(res["fp64_ssor_cg"]["seconds"] / res["mixed_fcg"]["seconds"], 3)}
-    for mode in res:
-        res[mode]["rel_err_vs_fp64"] = float(np.linalg.norm(xs[mode] - ref_x) / np.linalg.norm(ref_x))
-        out[mode] = res[mode]
     ctx.close()
-    for c in keep:
-        c.close()
-    net.close()
+    out = {"workload": "%d^3 HEX8, two-phase inclusions (E 1e3 / 1e7, 15%% volume, radii 2-6 elements, seed 7), "
+                       "%d DOF; network trained on %s residuals harvested from SSOR-CG on %s seeded 24^3 variants"
+                       % (args.c5_n, 3 * m.N, train["samples"] if train else 0, train["variants"] if train else 0),
+           "train": train}
+    out.update(res)
     return out
 
 

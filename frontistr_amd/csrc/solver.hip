@@ -1027,12 +1027,17 @@ static int32_t coarse_numeric(Context& c) {
   return 0;
 }
 
+static int32_t coarse_aim(const Context& c, int32_t opt3) {
+  const int32_t t = opt3 == 1 ? kCoarseAggDefault : opt3;
+  return std::max(1, std::min(std::min(t, kCoarseAggMax), c.N / kCoarseMinNodes));
+}
+
 static int32_t coarse_setup(Context& c, int32_t target) {
   if ((int64_t)c.h_coord.size() < 3 * (int64_t)c.NP) {
     set_error("SOLVER_OPT(3) (two-level correction) needs the node coordinates (hecmw33_set_coords)");
     return HECMW33_ERROR_ARGS;
   }
-  target = std::max(1, std::min(std::min(target, kCoarseAggMax), c.N / kCoarseMinNodes));
+  target = coarse_aim(c, target);
   if (!c.coarse_sym || c.coarse_target != target || c.coarse_coord_gen != c.coord_gen)
     if (int32_t e = coarse_symbolic(c, target)) return e;
   if (!c.coarse_num)
@@ -1194,7 +1199,7 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
   c.last_coarse = 0;
   if (mixed && IA(iarray, I_SOLVER_OPT3) > 0) {
     const int32_t o3 = IA(iarray, I_SOLVER_OPT3);
-    if (int32_t e = coarse_setup(c, o3 == 1 ? kCoarseAggDefault : o3)) return e;
+    if (int32_t e = coarse_setup(c, o3)) return e;
     c.coarse_use = c.coarse_ok;
     c.last_coarse = c.coarse_ok ? c.coarse.n_agg : -1;
   }
@@ -1630,7 +1635,14 @@ int32_t hecmw33_ctx_load(hecmw33_ctx* ctx, const hecmw33_sys* A, const int32_t i
   std::memcpy(ia, iarray, sizeof(ia));
   IA(ia, I_FLAG_SYMBFACT) = 1;
   IA(ia, I_FLAG_NUMFACT) = 1;
-  return precond_setup(ctx->c, ia, rarray);
+  if (int32_t e = precond_setup(ctx->c, ia, rarray)) return e;
+  // the two-level correction's aggregation is symbolic set-up too
+  const int32_t o3 = IA(iarray, I_SOLVER_OPT3);
+  if (o3 > 0 && (int64_t)ctx->c.h_coord.size() >= 3 * (int64_t)ctx->c.NP) {
+    Context& c = ctx->c;
+    if (int32_t e = coarse_symbolic(c, coarse_aim(c, o3))) return e;
+  }
+  return 0;
 }
 
 int32_t hecmw33_ctx_solve_device(hecmw33_ctx* ctx, const double* B_dev, double* X_dev, const int32_t iarray[100],
