@@ -65,6 +65,8 @@ def main():
     ap.add_argument("--c4-nsys", type=int, default=48, help="batched TET10 systems (0: skip)")
     ap.add_argument("--c5-n", type=int, default=120, help="learned-correction cube (0: skip)")
     ap.add_argument("--c5-variants", type=int, default=32, help="training variants for the learned correction")
+    ap.add_argument("--only", choices=("c3", "c4", "c5"), default=None,
+                    help="run one workload section alone (for profiling); prints that section's JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus %d requested); multi-GPU sharding is "
@@ -78,6 +80,19 @@ def main():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
     dev = torch.device("cuda:0")
+
+    if args.only:
+        net, train = train_net(args, dev) if args.only != "c4" and args.c5_variants > 0 else (None, None)
+        if args.only == "c3":
+            sec = {"mixed": bench_mixed(args, dev, net)}
+        elif args.only == "c4":
+            sec = {"batch": bench_batch(args)}
+        else:
+            sec = {"learned": bench_learned(args, dev, net, train)}
+        if net is not None:
+            net.close()
+        print(json.dumps(dict(only=args.only, **sec)), flush=True)
+        return
 
     t0 = time.time()
     m = fa.gen_hex8(args.n)

@@ -1636,12 +1636,12 @@ int32_t hecmw33_ctx_load(hecmw33_ctx* ctx, const hecmw33_sys* A, const int32_t i
   IA(ia, I_FLAG_SYMBFACT) = 1;
   IA(ia, I_FLAG_NUMFACT) = 1;
   if (int32_t e = precond_setup(ctx->c, ia, rarray)) return e;
-  // the two-level correction's aggregation is symbolic set-up too
+  // and the two-level correction's set-up when the load asks for it: the
+  // aggregation (symbolic) and the first coarse inverse, which also brings
+  // up the dense-solver library once
   const int32_t o3 = IA(iarray, I_SOLVER_OPT3);
-  if (o3 > 0 && (int64_t)ctx->c.h_coord.size() >= 3 * (int64_t)ctx->c.NP) {
-    Context& c = ctx->c;
-    if (int32_t e = coarse_symbolic(c, coarse_aim(c, o3))) return e;
-  }
+  if (o3 > 0 && (int64_t)ctx->c.h_coord.size() >= 3 * (int64_t)ctx->c.NP)
+    if (int32_t e = coarse_setup(ctx->c, o3)) return e;
   return 0;
 }
 

@@ -51,6 +51,13 @@ module hecmw_solver_33
       integer(c_int32_t) :: n
     end function c_hecmw33_device_count
 
+    function c_hecmw33_set_coords(NP, xyz) bind(c, name='hecmw33_set_coords') result(rc)
+      import :: c_ptr, c_int32_t
+      integer(c_int32_t), value :: NP
+      type(c_ptr), value :: xyz
+      integer(c_int32_t) :: rc
+    end function c_hecmw33_set_coords
+
     function c_hecmw33_last_history(hist, cap) bind(c, name='hecmw33_last_history') result(n)
       import :: c_double, c_int32_t
       real(c_double), intent(out) :: hist(*)
@@ -105,6 +112,17 @@ contains
     A%indexU = c_loc(hecMAT%indexU(0))
     ia(:) = hecMAT%Iarray(:)
     ra(:) = hecMAT%Rarray(:)
+
+    ! the two-level correction of the flexible CG (SOLVER_OPT(3) = Iarray(43) > 0)
+    ! builds its aggregates from the node coordinates (Fortran does not short-circuit
+    ! .and., so the pointer is inspected only when the option is on)
+    if (ia(43) > 0) then
+      if (associated(hecMESH%node)) then
+        if (size(hecMESH%node) >= 3 * hecMAT%NP) then
+          rc = c_hecmw33_set_coords(int(hecMAT%NP, c_int32_t), c_loc(hecMESH%node(1)))
+        endif
+      endif
+    endif
 
     rc = c_hecmw33_solve(A, c_loc(hecMAT%B(1)), c_loc(hecMAT%X(1)), ia, ra, out)
     hecMAT%Iarray(:) = ia(:)

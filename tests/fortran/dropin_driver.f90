@@ -4,7 +4,9 @@
 ! sources) and once by hecmw_solve_33 (GPU, frontistr_amd/fortran), both
 ! called from Fortran exactly as fistr1 would call them.
 !
-!   dropin_test <n> <method> <precond>    (OMP_NUM_THREADS >= 2)
+!   dropin_test <n> <method> <precond> [scaling] [fcg]    (OMP_NUM_THREADS >= 2)
+! fcg = 1: the GPU side runs the flexible CG with FP32 operators and the
+! two-level correction (Iarray 41 = 1, 43 = 1), with hecMESH%node set
 ! prints one line:  DROPIN n method precond conv_ref conv_gpu div_ref div_gpu relerr
 !-------------------------------------------------------------------------------
 program dropin_test
@@ -37,7 +39,7 @@ program dropin_test
   type(hecmwST_matrix), target :: mref, mgpu
   type(hecmwST_local_mesh) :: mesh
   integer(c_int32_t) :: n, N_, NPL, NPU, rc
-  integer :: method, precond, scaling
+  integer :: method, precond, scaling, fcg, i, j, k, nn
   character(len=32) :: arg
   real(c_double) :: p(8)
   real(kind=kreal) :: relerr
@@ -46,8 +48,12 @@ program dropin_test
   call get_command_argument(2, arg); read(arg, *) method
   call get_command_argument(3, arg); read(arg, *) precond
   scaling = 0
+  fcg = 0
   if (command_argument_count() >= 4) then
     call get_command_argument(4, arg); read(arg, *) scaling
+  endif
+  if (command_argument_count() >= 5) then
+    call get_command_argument(5, arg); read(arg, *) fcg
   endif
 
   rc = gen_count(n, N_, NPL, NPU)
@@ -74,6 +80,23 @@ program dropin_test
   mesh%PETOT = 1
   mesh%zero = 1
   mesh%mpc%n_mpc = 0
+  nullify(mesh%node)
+  if (fcg == 1) then
+    ! node i + (n+1)(j + (n+1)k) of the generated cube sits at (i, j, k)/n
+    nn = n + 1
+    allocate(mesh%node(3*N_))
+    do k = 0, n
+      do j = 0, n
+        do i = 0, n
+          mesh%node(3*(i + nn*(j + nn*k)) + 1) = dble(i) / n
+          mesh%node(3*(i + nn*(j + nn*k)) + 2) = dble(j) / n
+          mesh%node(3*(i + nn*(j + nn*k)) + 3) = dble(k) / n
+        enddo
+      enddo
+    enddo
+    mgpu%Iarray(41) = 1
+    mgpu%Iarray(43) = 1
+  endif
 
   call hecmw_solve_iterative(mesh, mref)
   if (.not. hecmw33_available(mesh, mgpu)) then
