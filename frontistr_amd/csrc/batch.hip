@@ -330,15 +330,27 @@ template <int TAIL, int ND>
 __global__ __launch_bounds__(kBT) void kb_tail(Tiles T, BState S, const double* part) {
   const int s = blockIdx.x;
   if (TAIL != kbInit && S.sys_active[s] == 0) return;
-  const int k = threadIdx.x % KS, sub = threadIdx.x / KS;
+  const int k = threadIdx.x % KS;
   const int t0 = T.sys_tile0[s], nt = T.sys_ntile[s];
   __shared__ double red[ND][kBT];
   __shared__ int32_t flag[KS];
+  {
+    // 16 threads per right-hand side sweep the system's tile partials, which
+    // are contiguous per (dot, rhs), with 8 independent loads in flight; a
+    // fixed order throughout
+    constexpr int kSub = kBT / KS;
+    const int kl = threadIdx.x / kSub, sub = threadIdx.x % kSub;
 #pragma unroll
-  for (int d = 0; d < ND; ++d) {
-    double a = 0.0;
-    for (int i = sub; i < nt; i += kBT / KS) a += part[(int64_t)(d * KS + k) * T.ntiles + t0 + i];
-    red[d][threadIdx.x] = a;
+    for (int d = 0; d < ND; ++d) {
+      const double* __restrict__ pp = part + (int64_t)(d * KS + kl) * T.ntiles + t0;
+      double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      int i = sub;
+      for (; i + 7 * kSub < nt; i += 8 * kSub)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] += pp[i + u * kSub];
+      for (; i < nt; i += kSub) a[0] += pp[i];
+      red[d][threadIdx.x] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+    }
   }
   __syncthreads();
   if (threadIdx.x < KS) {
@@ -346,7 +358,7 @@ __global__ __launch_bounds__(kBT) void kb_tail(Tiles T, BState S, const double* 
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
       double a = 0.0;
-      for (int m = 0; m < kBT / KS; ++m) a += red[d][m * KS + k];
+      for (int m = 0; m < kBT / KS; ++m) a += red[d][k * (kBT / KS) + m];
       tot[d] = a;
     }
     const int id = s * KS + k;
