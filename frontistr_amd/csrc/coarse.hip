@@ -14,8 +14,8 @@
 //  * aptr/anodes: the nodes of each aggregate, ascending;
 //  * dv[3i..3i+2]: (x_i - centroid of agg(i)) / scale, FP32 -- the same numbers
 //    define Z in the assembly, the restriction and the prolongation;
-//  * nbr[27a + s]: the aggregate in neighbour cell s = 9(dz+1) + 3(dy+1) + (dx+1)
-//    of aggregate a's cell, or -1; mask[i] bit s: row i has a block in that cell;
+//  * pairs p = (a, b) of aggregates coupled by a matrix block, and per pair the
+//    nodes of a whose rows touch b (pn_ptr/pn);
 //  * W: E^-1 in FP32, row-major n_c x ldw (symmetric).
 //
 // Z_i (3x6) = [ I | R(d_i) ] with R's columns the rotations about x, y, z:
@@ -59,26 +59,24 @@ __device__ __forceinline__ void block_reduce(double (&v)[K], double* out) {
   }
 }
 
-// E block of the aggregate pair (a, b = nbr[27a + s]):
-//   E_ab = sum over nodes i of a, blocks j of row i with agg(j) = b, of Z_i^T A_ij Z_j.
+// E block of the aggregate pair p = (a, b):
+//   E_ab = sum over the nodes i of a that touch b, blocks j of row i with
+//   agg(j) = b, of Z_i^T A_ij Z_j.
 // One workgroup per pair; each thread sums its nodes' blocks in registers, then
 // a fixed-order reduction, so E is the same on every run.
 __global__ __launch_bounds__(kCoarseBlock) void k_coarse_E(Sell<double> A, const int32_t* __restrict__ agg,
-                                                            const int32_t* __restrict__ aptr,
-                                                            const int32_t* __restrict__ anodes,
-                                                            const int32_t* __restrict__ nbr,
-                                                            const uint32_t* __restrict__ mask,
+                                                            const int32_t* __restrict__ pb,
+                                                            const int32_t* __restrict__ pn_ptr,
+                                                            const int32_t* __restrict__ pn,
                                                             const float* __restrict__ dv, double* __restrict__ Eblk) {
-  const int a = blockIdx.x / 27, s = blockIdx.x % 27;
-  const int b = nbr[a * 27 + s];
-  if (b < 0) return;
+  const int p = blockIdx.x;
+  const int b = pb[p];
   double acc[36];
 #pragma unroll
   for (int k = 0; k < 36; ++k) acc[k] = 0.0;
-  const int t1 = aptr[a + 1];
-  for (int t = aptr[a] + threadIdx.x; t < t1; t += kCoarseBlock) {
-    const int i = anodes[t];
-    if (!((mask[i] >> s) & 1u)) continue;
+  const int t1 = pn_ptr[p + 1];
+  for (int t = pn_ptr[p] + threadIdx.x; t < t1; t += kCoarseBlock) {
+    const int i = pn[t];
     const double ix = dv[3 * (int64_t)i], iy = dv[3 * (int64_t)i + 1], iz = dv[3 * (int64_t)i + 2];
     const int sl = i / A.rows, l = i % A.rows;
     const int64_t base = (int64_t)A.ptr[sl] * A.rows;
@@ -111,19 +109,18 @@ __global__ __launch_bounds__(kCoarseBlock) void k_coarse_E(Sell<double> A, const
       }
     }
   }
-  block_reduce<36>(acc, Eblk + (int64_t)blockIdx.x * 36);
+  block_reduce<36>(acc, Eblk + (int64_t)p * 36);
 }
 
-// dense E, column-major (the solver library's layout): E(6a+p, 6b+q) = Eblk[a][s][p][q]
-__global__ void k_coarse_scatter(int32_t n_agg, const int32_t* __restrict__ nbr, const double* __restrict__ Eblk,
-                                 double* __restrict__ E, int32_t lda) {
+// dense E, column-major (the solver library's layout): E(6a+r, 6b+q) = Eblk[p][r][q]
+__global__ void k_coarse_scatter(int32_t npair, const int32_t* __restrict__ pa, const int32_t* __restrict__ pb,
+                                 const double* __restrict__ Eblk, double* __restrict__ E, int32_t lda) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)n_agg * 27 * 36) return;
-  const int pair = (int)(t / 36), e = (int)(t % 36);
-  const int a = pair / 27, b = nbr[pair];
-  if (b < 0) return;
-  const int p = e / 6, q = e % 6;
-  E[(int64_t)(6 * b + q) * lda + 6 * a + p] = Eblk[t];
+  if (t >= (int64_t)npair * 36) return;
+  const int p = (int)(t / 36), e = (int)(t % 36);
+  const int a = pa[p], b = pb[p];
+  const int r = e / 6, q = e % 6;
+  E[(int64_t)(6 * b + q) * lda + 6 * a + r] = Eblk[t];
 }
 
 // unit pivot for a coarse unknown whose column of Z is zero (E(k,k) exactly 0):
@@ -229,12 +226,12 @@ __global__ __launch_bounds__(kCoarseBlock) void k_coarse_prolong(int32_t n, cons
 
 }  // namespace
 
-void coarse_assemble(const Sell<double>& A, const CoarseOp& op, const int32_t* nbr, const uint32_t* mask,
-                     double* Eblk, double* E, int32_t lda, hipStream_t s) {
-  const int64_t npair = (int64_t)op.n_agg * 27;
-  k_coarse_E<<<(unsigned)npair, kCoarseBlock, 0, s>>>(A, op.agg, op.aptr, op.anodes, nbr, mask, op.dv, Eblk);
-  const int64_t ne = npair * 36;
-  k_coarse_scatter<<<(unsigned)((ne + 255) / 256), 256, 0, s>>>(op.n_agg, nbr, Eblk, E, lda);
+void coarse_assemble(const Sell<double>& A, const CoarseOp& op, int32_t npair, const int32_t* pa, const int32_t* pb,
+                     const int32_t* pn_ptr, const int32_t* pn, double* Eblk, double* E, int32_t lda, hipStream_t s) {
+  if (npair <= 0) return;
+  k_coarse_E<<<(unsigned)npair, kCoarseBlock, 0, s>>>(A, op.agg, pb, pn_ptr, pn, op.dv, Eblk);
+  const int64_t ne = (int64_t)npair * 36;
+  k_coarse_scatter<<<(unsigned)((ne + 255) / 256), 256, 0, s>>>(npair, pa, pb, Eblk, E, lda);
   k_coarse_unit_pivots<<<(op.nc + 255) / 256, 256, 0, s>>>(op.nc, E, lda);
 }
 

@@ -199,8 +199,10 @@ struct CoarseOp {
   float* cv = nullptr;                 // [ldw] Z^T r (entries nc..ldw-1 stay 0)
   float* yv = nullptr;                 // [nc] E^-1 Z^T r
 };
-void coarse_assemble(const Sell<double>& A, const CoarseOp& op, const int32_t* nbr, const uint32_t* mask,
-                     double* Eblk, double* E, int32_t lda, hipStream_t s);
+// E = Z^T A Z: the 6x6 block of every coupled aggregate pair (pa[p], pb[p]), summed
+// over the pair's nodes pn[pn_ptr[p]..pn_ptr[p+1]), then scattered into the dense E
+void coarse_assemble(const Sell<double>& A, const CoarseOp& op, int32_t npair, const int32_t* pa, const int32_t* pb,
+                     const int32_t* pn_ptr, const int32_t* pn, double* Eblk, double* E, int32_t lda, hipStream_t s);
 void coarse_inverse_f32(const CoarseOp& op, const double* Ei, int32_t lda, hipStream_t s);
 // z += Z E^-1 Z^T r; with dots also the beta partials (r.z, q.z) of the new z.
 // Returns the number of partials.

@@ -25,6 +25,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <type_traits>
@@ -254,8 +255,8 @@ struct Context {
   bool coarse_use = false;       // this solve applies it
   int32_t last_coarse = 0;       // aggregates used by the last solve (-1: E not positive definite)
   CoarseOp coarse;
-  std::vector<int32_t> coarse_cell;  // cell coordinates of each aggregate (3 per aggregate)
-  DMem cg_agg, cg_aptr, cg_anodes, cg_nbr, cg_mask, cg_dv, cg_Eblk, cg_E, cg_W, cg_cv, cg_yv, cg_info;
+  int32_t coarse_npair = 0;       // coupled aggregate pairs (blocks of E)
+  DMem cg_agg, cg_aptr, cg_anodes, cg_dv, cg_pa, cg_pb, cg_pnptr, cg_pn, cg_Eblk, cg_E, cg_W, cg_cv, cg_yv, cg_info;
   rocblas_handle blas = nullptr;
   double contrast = -1.0;        // max/min trace of the diagonal blocks (< 0: not computed for these values)
   // TIMELOG: event pairs around the preconditioner and the SpMV of every
@@ -284,8 +285,8 @@ struct Context {
     for (DMem* m : {&dIndexL, &dIndexU, &dItemL, &dItemU, &vscale, &vzs, &vzs32, &vz0_32, &vmat, &scal, &vfeat, &vcoef,
                     &vx0})
       m->release();
-    for (DMem* m : {&cg_agg, &cg_aptr, &cg_anodes, &cg_nbr, &cg_mask, &cg_dv, &cg_Eblk, &cg_E, &cg_W, &cg_cv, &cg_yv,
-                    &cg_info})
+    for (DMem* m : {&cg_agg, &cg_aptr, &cg_anodes, &cg_dv, &cg_pa, &cg_pb, &cg_pnptr, &cg_pn, &cg_Eblk, &cg_E, &cg_W,
+                    &cg_cv, &cg_yv, &cg_info})
       m->release();
     if (blas) rocblas_destroy_handle(blas);
     for (DMem* m : {&dD, &dAL, &dAU, &row_old, &alu_ssor, &row_diag, &alu_diag, &alu_ssor32, &alu_diag32, &vr32,
@@ -800,9 +801,11 @@ static void enqueue_bicg_iteration(Context& c, int32_t kind, int32_t iterpremax,
 // r -= alpha q, then, when the recurrence asks for it, r = b - A x in FP64.
 // ---- two-level correction: aggregation (symbolic) and coarse inverse (numeric) ----------
 // Aggregates are the non-empty cells of a uniform grid over the nodes' bounding
-// box, about `target` of them; the cell size grows until every matrix block
-// couples nodes in the same or adjacent cells, so E_ab is non-zero only for
-// the 27 neighbour cells of a.
+// box, sized for about `target` of them (grown while there are more than
+// kCoarseAggMax).  E_ab is assembled for every pair of aggregates coupled by a
+// matrix block; the pair list and, per pair, the nodes of a whose rows touch b
+// come from the matrix graph, so any aggregation works.
+
 static int32_t coarse_symbolic(Context& c, int32_t target) {
   const int32_t N = c.N, NP = c.NP;
   const double* X = c.h_coord.data();
@@ -834,158 +837,177 @@ static int32_t coarse_symbolic(Context& c, int32_t target) {
   const int32_t* iU = c.h_indexU.data();
   const int32_t* tL = c.h_itemL.data();
   const int32_t* tU = c.h_itemU.data();
-  std::vector<int32_t> cell(3 * (size_t)N);
-  int nc[3] = {1, 1, 1};
+
+  // aggregates: the non-empty cells, numbered in order of first appearance over the nodes
+  std::vector<int32_t> agg(NP, -1);
+  int32_t na = 0;
   double H = std::pow(vol / std::max(1, target), 1.0 / ndim);
-  bool ok = false;
-  for (int attempt = 0; attempt < 40 && !ok; ++attempt, H *= 1.15) {
-    int64_t ncells = 1;
-    for (int d = 0; d < 3; ++d) {
-      nc[d] = ext[d] > 0.0 ? std::max(1, (int)std::ceil(ext[d] / H - 1e-9)) : 1;
-      ncells *= nc[d];
-    }
+  for (int attempt = 0; attempt < 60; ++attempt, H *= 1.15) {
+    int nc[3];
+    for (int d = 0; d < 3; ++d) nc[d] = ext[d] > 0.0 ? std::max(1, (int)std::ceil(ext[d] / H - 1e-9)) : 1;
+    const int64_t ncells = (int64_t)nc[0] * nc[1] * nc[2];
     if (ncells > 64 * (int64_t)kCoarseAggMax) continue;
-#pragma omp parallel for schedule(static)
-    for (int32_t i = 0; i < N; ++i)
-      for (int d = 0; d < 3; ++d)
-        cell[3 * (size_t)i + d] =
-            ext[d] > 0.0 ? std::min(nc[d] - 1, (int)((X[3 * (int64_t)i + d] - lo[d]) / ext[d] * nc[d])) : 0;
-    int bad = 0;
-#pragma omp parallel for schedule(dynamic, 4096) reduction(max : bad)
-    for (int32_t i = 0; i < N; ++i) {
-      auto chk = [&](int32_t j) {
-        if (j >= N) return;
-        for (int d = 0; d < 3; ++d)
-          if (std::abs(cell[3 * (size_t)i + d] - cell[3 * (size_t)j + d]) > 1) bad = 1;
-      };
-      for (int32_t q = iL[i]; q < iL[i + 1]; ++q) chk(tL[q] - 1);
-      for (int32_t q = iU[i]; q < iU[i + 1]; ++q) chk(tU[q] - 1);
-    }
-    if (bad) continue;
-    // aggregates: non-empty cells in cell order
     std::vector<int32_t> cell_agg((size_t)ncells, -1);
-    for (int32_t i = 0; i < N; ++i) {
-      const int64_t id = cell[3 * (size_t)i] + (int64_t)nc[0] * (cell[3 * (size_t)i + 1] + (int64_t)nc[1] * cell[3 * (size_t)i + 2]);
-      cell_agg[id] = 0;
-    }
-    int32_t na = 0;
-    for (auto& v : cell_agg)
-      if (v == 0) v = na++;
-    if (na > kCoarseAggMax) continue;
-    ok = true;
-    std::vector<int32_t> agg(NP, -1), cnt(na + 1, 0);
-    for (int32_t i = 0; i < N; ++i) {
-      const int64_t id = cell[3 * (size_t)i] + (int64_t)nc[0] * (cell[3 * (size_t)i + 1] + (int64_t)nc[1] * cell[3 * (size_t)i + 2]);
-      agg[i] = cell_agg[id];
-      ++cnt[agg[i] + 1];
-    }
-    for (int32_t a = 0; a < na; ++a) cnt[a + 1] += cnt[a];
-    std::vector<int32_t> anodes(N), fillp(cnt.begin(), cnt.end() - 1);
-    for (int32_t i = 0; i < N; ++i) anodes[fillp[agg[i]]++] = i;
-    // centroids, cells and neighbour table
-    std::vector<double> cen(3 * (size_t)na, 0.0);
-    c.coarse_cell.assign(3 * (size_t)na, 0);
-#pragma omp parallel for schedule(static)
-    for (int32_t a = 0; a < na; ++a) {
-      double sx = 0, sy = 0, sz = 0;
-      for (int32_t t = cnt[a]; t < cnt[a + 1]; ++t) {
-        const int64_t i = anodes[t];
-        sx += X[3 * i];
-        sy += X[3 * i + 1];
-        sz += X[3 * i + 2];
-      }
-      const double k = 1.0 / (cnt[a + 1] - cnt[a]);
-      cen[3 * (size_t)a] = sx * k;
-      cen[3 * (size_t)a + 1] = sy * k;
-      cen[3 * (size_t)a + 2] = sz * k;
-      const int64_t i0 = anodes[cnt[a]];
-      for (int d = 0; d < 3; ++d) c.coarse_cell[3 * (size_t)a + d] = cell[3 * i0 + d];
-    }
-    std::vector<int32_t> nbr(27 * (size_t)na, -1);
-    for (int32_t a = 0; a < na; ++a)
-      for (int s = 0; s < 27; ++s) {
-        const int o[3] = {s % 3 - 1, (s / 3) % 3 - 1, s / 9 - 1};
-        int cc[3];
-        bool in = true;
-        for (int d = 0; d < 3; ++d) {
-          cc[d] = c.coarse_cell[3 * (size_t)a + d] + o[d];
-          in = in && cc[d] >= 0 && cc[d] < nc[d];
-        }
-        if (in) nbr[27 * (size_t)a + s] = cell_agg[cc[0] + (int64_t)nc[0] * (cc[1] + (int64_t)nc[1] * cc[2])];
-      }
-    // an aggregate whose nodes are collinear (or a single node) has a rotation
-    // that moves none of them: its offsets are set to 0, so all three rotation
-    // columns of Z vanish there, and E gets unit pivots for them
-    std::vector<char> degen(na, 0);
-#pragma omp parallel for schedule(static)
-    for (int32_t a = 0; a < na; ++a) {
-      double M[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-      for (int32_t t = cnt[a]; t < cnt[a + 1]; ++t) {
-        const int64_t i = anodes[t];
-        double r[3];
-        for (int d = 0; d < 3; ++d) r[d] = X[3 * i + d] - cen[3 * (size_t)a + d];
-        for (int p = 0; p < 3; ++p)
-          for (int q = 0; q < 3; ++q) M[p][q] += r[p] * r[q];
-      }
-      const double tr = M[0][0] + M[1][1] + M[2][2];
-      double f2 = 0.0;
-      for (int p = 0; p < 3; ++p)
-        for (int q = 0; q < 3; ++q) f2 += M[p][q] * M[p][q];
-      // (tr^2 - |M|_F^2) / 2 = l1 l2 + l1 l3 + l2 l3: ~0 when rank(M) <= 1
-      degen[a] = !(0.5 * (tr * tr - f2) > 1e-8 * tr * tr);
-    }
-    // offsets scaled by half the cell size; row masks of the neighbour cells touched
-    const double sc = 2.0 / H;
-    std::vector<float> dv(3 * (size_t)N);
-    std::vector<uint32_t> mask(N, 0);
-#pragma omp parallel for schedule(dynamic, 4096)
-    for (int32_t i = 0; i < N; ++i) {
-      const int32_t a = agg[i];
+    na = 0;
+    bool over = false;
+    for (int32_t i = 0; i < N && !over; ++i) {
+      int cd[3];
       for (int d = 0; d < 3; ++d)
-        dv[3 * (size_t)i + d] = degen[a] ? 0.f : (float)((X[3 * (int64_t)i + d] - cen[3 * (size_t)a + d]) * sc);
-      uint32_t m = 1u << 13;
-      auto add = [&](int32_t j) {
-        if (j >= N) return;
-        int s = 0;
-        for (int d = 2; d >= 0; --d) s = 3 * s + (cell[3 * (size_t)j + d] - cell[3 * (size_t)i + d] + 1);
-        m |= 1u << s;
-      };
-      for (int32_t q = iL[i]; q < iL[i + 1]; ++q) add(tL[q] - 1);
-      for (int32_t q = iU[i]; q < iU[i + 1]; ++q) add(tU[q] - 1);
-      mask[i] = m;
+        cd[d] = ext[d] > 0.0 ? std::min(nc[d] - 1, (int)((X[3 * (int64_t)i + d] - lo[d]) / ext[d] * nc[d])) : 0;
+      int32_t& slot = cell_agg[cd[0] + (int64_t)nc[0] * (cd[1] + (int64_t)nc[1] * cd[2])];
+      if (slot < 0) slot = na++;
+      agg[i] = slot;
+      over = na > kCoarseAggMax;
     }
-    CoarseOp& op = c.coarse;
-    op.n_agg = na;
-    op.nc = 6 * na;
-    op.ldw = (op.nc + 63) / 64 * 64;
-    if (c.cg_agg.ensure(sizeof(int32_t) * NP) || c.cg_aptr.ensure(sizeof(int32_t) * (na + 1)) ||
-        c.cg_anodes.ensure(sizeof(int32_t) * N) || c.cg_nbr.ensure(sizeof(int32_t) * 27 * (size_t)na) ||
-        c.cg_mask.ensure(sizeof(uint32_t) * N) || c.cg_dv.ensure(sizeof(float) * 3 * (size_t)N) ||
-        c.cg_Eblk.ensure(sizeof(double) * 27 * 36 * (size_t)na) ||
-        c.cg_E.ensure(sizeof(double) * (size_t)op.nc * op.nc) || c.cg_W.ensure(sizeof(float) * (size_t)op.nc * op.ldw) ||
-        c.cg_cv.ensure(sizeof(float) * op.ldw) || c.cg_yv.ensure(sizeof(float) * op.ldw) ||
-        c.cg_info.ensure(sizeof(int32_t) * 2))
-      return HECMW33_ERROR_DEVICE;
-    HIPC(hipMemcpyAsync(c.cg_agg.p, agg.data(), sizeof(int32_t) * NP, hipMemcpyHostToDevice, c.stream));
-    HIPC(hipMemcpyAsync(c.cg_aptr.p, cnt.data(), sizeof(int32_t) * (na + 1), hipMemcpyHostToDevice, c.stream));
-    HIPC(hipMemcpyAsync(c.cg_anodes.p, anodes.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, c.stream));
-    HIPC(hipMemcpyAsync(c.cg_nbr.p, nbr.data(), sizeof(int32_t) * 27 * (size_t)na, hipMemcpyHostToDevice, c.stream));
-    HIPC(hipMemcpyAsync(c.cg_mask.p, mask.data(), sizeof(uint32_t) * N, hipMemcpyHostToDevice, c.stream));
-    HIPC(hipMemcpyAsync(c.cg_dv.p, dv.data(), sizeof(float) * 3 * (size_t)N, hipMemcpyHostToDevice, c.stream));
-    HIPC(hipMemsetAsync(c.cg_cv.p, 0, sizeof(float) * op.ldw, c.stream));
-    op.agg = c.cg_agg.as<int32_t>();
-    op.aptr = c.cg_aptr.as<int32_t>();
-    op.anodes = c.cg_anodes.as<int32_t>();
-    op.dv = c.cg_dv.as<float>();
-    op.W = c.cg_W.as<float>();
-    op.cv = c.cg_cv.as<float>();
-    op.yv = c.cg_yv.as<float>();
-    HIPC(hipStreamSynchronize(c.stream));  // the host vectors above go out of scope
+    if (!over) break;
+    na = 0;
   }
-  if (!ok) {
-    set_error("coarse correction: no aggregation with neighbour-cell coupling found");
+  if (na == 0) {
+    set_error("coarse correction: no aggregation within the size limit");
     return HECMW33_ERROR_ARGS;
   }
+  std::vector<int32_t> cnt(na + 1, 0);
+  for (int32_t i = 0; i < N; ++i) ++cnt[agg[i] + 1];
+  for (int32_t a = 0; a < na; ++a) cnt[a + 1] += cnt[a];
+  std::vector<int32_t> anodes(N), fillp(cnt.begin(), cnt.end() - 1);
+  for (int32_t i = 0; i < N; ++i) anodes[fillp[agg[i]]++] = i;
+
+  // centroids; an aggregate whose nodes are collinear (or a single node) has a
+  // rotation that moves none of them: its offsets are set to 0, so its three
+  // rotation columns of Z vanish, and E gets unit pivots for them
+  std::vector<double> cen(3 * (size_t)na, 0.0);
+  std::vector<char> degen(na, 0);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int32_t a = 0; a < na; ++a) {
+    double sx = 0, sy = 0, sz = 0;
+    for (int32_t t = cnt[a]; t < cnt[a + 1]; ++t) {
+      const int64_t i = anodes[t];
+      sx += X[3 * i];
+      sy += X[3 * i + 1];
+      sz += X[3 * i + 2];
+    }
+    const double k = 1.0 / (cnt[a + 1] - cnt[a]);
+    cen[3 * (size_t)a] = sx * k;
+    cen[3 * (size_t)a + 1] = sy * k;
+    cen[3 * (size_t)a + 2] = sz * k;
+    double M[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int32_t t = cnt[a]; t < cnt[a + 1]; ++t) {
+      const int64_t i = anodes[t];
+      double r[3];
+      for (int d = 0; d < 3; ++d) r[d] = X[3 * i + d] - cen[3 * (size_t)a + d];
+      for (int p = 0; p < 3; ++p)
+        for (int q = 0; q < 3; ++q) M[p][q] += r[p] * r[q];
+    }
+    const double trm = M[0][0] + M[1][1] + M[2][2];
+    double f2 = 0.0;
+    for (int p = 0; p < 3; ++p)
+      for (int q = 0; q < 3; ++q) f2 += M[p][q] * M[p][q];
+    // (tr^2 - |M|_F^2) / 2 = l1 l2 + l1 l3 + l2 l3: ~0 when rank(M) <= 1
+    degen[a] = !(0.5 * (trm * trm - f2) > 1e-8 * trm * trm);
+  }
+  const double sc = 2.0 / H;
+  std::vector<float> dv(3 * (size_t)N);
+#pragma omp parallel for schedule(static)
+  for (int32_t i = 0; i < N; ++i) {
+    const int32_t a = agg[i];
+    for (int d = 0; d < 3; ++d)
+      dv[3 * (size_t)i + d] = degen[a] ? 0.f : (float)((X[3 * (int64_t)i + d] - cen[3 * (size_t)a + d]) * sc);
+  }
+
+  // per node, the distinct aggregates its row touches (its own first)
+  std::vector<int32_t> ncnt(N + 1, 0);
+  auto row_aggs = [&](int32_t i, int32_t* out) {
+    int n = 0;
+    out[n++] = agg[i];
+    auto add = [&](int32_t j) {
+      if (j >= N) return;
+      const int32_t b = agg[j];
+      for (int k = 0; k < n; ++k)
+        if (out[k] == b) return;
+      out[n++] = b;
+    };
+    for (int32_t q = iL[i]; q < iL[i + 1]; ++q) add(tL[q] - 1);
+    for (int32_t q = iU[i]; q < iU[i + 1]; ++q) add(tU[q] - 1);
+    return n;
+  };
+  int32_t maxrow = 1;
+  for (int32_t i = 0; i < N; ++i) maxrow = std::max(maxrow, 1 + (iL[i + 1] - iL[i]) + (iU[i + 1] - iU[i]));
+#pragma omp parallel
+  {
+    std::vector<int32_t> buf(maxrow);
+#pragma omp for schedule(dynamic, 4096)
+    for (int32_t i = 0; i < N; ++i) ncnt[i + 1] = row_aggs(i, buf.data());
+  }
+  for (int32_t i = 0; i < N; ++i) ncnt[i + 1] += ncnt[i];
+  std::vector<int32_t> nagg(ncnt[N]);
+#pragma omp parallel
+  {
+    std::vector<int32_t> buf(maxrow);
+#pragma omp for schedule(dynamic, 4096)
+    for (int32_t i = 0; i < N; ++i) {
+      const int n = row_aggs(i, buf.data());
+      std::copy(buf.data(), buf.data() + n, nagg.data() + ncnt[i]);
+    }
+  }
+  // aggregate pairs (a, b), b ascending per a, and the nodes of a touching b
+  std::vector<std::vector<int32_t>> adj(na);
+  for (int32_t i = 0; i < N; ++i)
+    for (int32_t k = ncnt[i]; k < ncnt[i + 1]; ++k) adj[agg[i]].push_back(nagg[k]);
+  std::vector<int32_t> pptr(na + 1, 0);
+  for (int32_t a = 0; a < na; ++a) {
+    std::sort(adj[a].begin(), adj[a].end());
+    adj[a].erase(std::unique(adj[a].begin(), adj[a].end()), adj[a].end());
+    pptr[a + 1] = pptr[a] + (int32_t)adj[a].size();
+  }
+  const int32_t npair = pptr[na];
+  std::vector<int32_t> pa(npair), pb(npair), pn_ptr(npair + 1, 0);
+  for (int32_t a = 0; a < na; ++a)
+    for (size_t k = 0; k < adj[a].size(); ++k) {
+      pa[pptr[a] + k] = a;
+      pb[pptr[a] + k] = adj[a][k];
+    }
+  auto pair_of = [&](int32_t a, int32_t b) {
+    return pptr[a] + (int32_t)(std::lower_bound(adj[a].begin(), adj[a].end(), b) - adj[a].begin());
+  };
+  for (int32_t i = 0; i < N; ++i)
+    for (int32_t k = ncnt[i]; k < ncnt[i + 1]; ++k) ++pn_ptr[pair_of(agg[i], nagg[k]) + 1];
+  for (int32_t p = 0; p < npair; ++p) pn_ptr[p + 1] += pn_ptr[p];
+  std::vector<int32_t> pn(pn_ptr[npair]), pfill(pn_ptr.begin(), pn_ptr.end() - 1);
+  for (int32_t i = 0; i < N; ++i)
+    for (int32_t k = ncnt[i]; k < ncnt[i + 1]; ++k) pn[pfill[pair_of(agg[i], nagg[k])]++] = i;
+
+  CoarseOp& op = c.coarse;
+  op.n_agg = na;
+  op.nc = 6 * na;
+  op.ldw = (op.nc + 63) / 64 * 64;
+  c.coarse_npair = npair;
+  const int64_t npn = pn_ptr[npair];
+  if (c.cg_agg.ensure(sizeof(int32_t) * NP) || c.cg_aptr.ensure(sizeof(int32_t) * (na + 1)) ||
+      c.cg_anodes.ensure(sizeof(int32_t) * N) || c.cg_dv.ensure(sizeof(float) * 3 * (size_t)N) ||
+      c.cg_pa.ensure(sizeof(int32_t) * npair) || c.cg_pb.ensure(sizeof(int32_t) * npair) ||
+      c.cg_pnptr.ensure(sizeof(int32_t) * (npair + 1)) || c.cg_pn.ensure(sizeof(int32_t) * std::max<int64_t>(npn, 1)) ||
+      c.cg_Eblk.ensure(sizeof(double) * 36 * (size_t)npair) ||
+      c.cg_E.ensure(sizeof(double) * (size_t)op.nc * op.nc) || c.cg_W.ensure(sizeof(float) * (size_t)op.nc * op.ldw) ||
+      c.cg_cv.ensure(sizeof(float) * op.ldw) || c.cg_yv.ensure(sizeof(float) * op.ldw) ||
+      c.cg_info.ensure(sizeof(int32_t) * 2))
+    return HECMW33_ERROR_DEVICE;
+  HIPC(hipMemcpyAsync(c.cg_agg.p, agg.data(), sizeof(int32_t) * NP, hipMemcpyHostToDevice, c.stream));
+  HIPC(hipMemcpyAsync(c.cg_aptr.p, cnt.data(), sizeof(int32_t) * (na + 1), hipMemcpyHostToDevice, c.stream));
+  HIPC(hipMemcpyAsync(c.cg_anodes.p, anodes.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, c.stream));
+  HIPC(hipMemcpyAsync(c.cg_dv.p, dv.data(), sizeof(float) * 3 * (size_t)N, hipMemcpyHostToDevice, c.stream));
+  HIPC(hipMemcpyAsync(c.cg_pa.p, pa.data(), sizeof(int32_t) * npair, hipMemcpyHostToDevice, c.stream));
+  HIPC(hipMemcpyAsync(c.cg_pb.p, pb.data(), sizeof(int32_t) * npair, hipMemcpyHostToDevice, c.stream));
+  HIPC(hipMemcpyAsync(c.cg_pnptr.p, pn_ptr.data(), sizeof(int32_t) * (npair + 1), hipMemcpyHostToDevice, c.stream));
+  if (npn) HIPC(hipMemcpyAsync(c.cg_pn.p, pn.data(), sizeof(int32_t) * npn, hipMemcpyHostToDevice, c.stream));
+  HIPC(hipMemsetAsync(c.cg_cv.p, 0, sizeof(float) * op.ldw, c.stream));
+  op.agg = c.cg_agg.as<int32_t>();
+  op.aptr = c.cg_aptr.as<int32_t>();
+  op.anodes = c.cg_anodes.as<int32_t>();
+  op.dv = c.cg_dv.as<float>();
+  op.W = c.cg_W.as<float>();
+  op.cv = c.cg_cv.as<float>();
+  op.yv = c.cg_yv.as<float>();
+  HIPC(hipStreamSynchronize(c.stream));  // the host vectors above go out of scope
   c.coarse_sym = true;
   c.coarse_num = false;
   c.coarse_target = target;
@@ -999,8 +1021,9 @@ static int32_t coarse_numeric(Context& c) {
   CoarseOp& op = c.coarse;
   const int32_t nc = op.nc;
   HIPC(hipMemsetAsync(c.cg_E.p, 0, sizeof(double) * (size_t)nc * nc, c.stream));
-  coarse_assemble(c.A.view, op, c.cg_nbr.as<int32_t>(), c.cg_mask.as<uint32_t>(), c.cg_Eblk.as<double>(),
-                  c.cg_E.as<double>(), nc, c.stream);
+  coarse_assemble(c.A.view, op, c.coarse_npair, c.cg_pa.as<int32_t>(), c.cg_pb.as<int32_t>(),
+                  c.cg_pnptr.as<int32_t>(), c.cg_pn.as<int32_t>(), c.cg_Eblk.as<double>(), c.cg_E.as<double>(), nc,
+                  c.stream);
   HIPC(hipGetLastError());
   if (!c.blas && rocblas_create_handle(&c.blas) != rocblas_status_success) {
     c.blas = nullptr;
@@ -1722,7 +1745,7 @@ int32_t hecmw33_ctx_history(hecmw33_ctx* ctx, double* hist, int32_t cap) {
 int32_t hecmw33_ctx_info(hecmw33_ctx* ctx, int64_t* info, int32_t n) {
   if (!ctx || !info) return HECMW33_ERROR_ARGS;
   Context& c = ctx->c;
-  const int64_t v[12] = {c.A.view.nslots,
+  const int64_t v[13] = {c.A.view.nslots,
                         (int64_t)c.N,
                         (int64_t)c.NPL,
                         (int64_t)c.NPU,
@@ -1733,8 +1756,9 @@ int32_t hecmw33_ctx_info(hecmw33_ctx* ctx, int64_t* info, int32_t n) {
                         (int64_t)c.last_learned_off,
                         (int64_t)c.last_spmv64,
                         (int64_t)std::llround(c.last_probe_ratio * 1e6),
-                        (int64_t)c.last_coarse};
-  for (int i = 0; i < n && i < 12; ++i) info[i] = v[i];
+                        (int64_t)c.last_coarse,
+                        (int64_t)c.coarse_npair};
+  for (int i = 0; i < n && i < 13; ++i) info[i] = v[i];
   return 0;
 }
 
@@ -2004,26 +2028,29 @@ int32_t hecmw33_set_coords(int32_t NP, const double* xyz) {
   return set_coords(*default_ctx(), NP, xyz);
 }
 
-int32_t hecmw33_ctx_coarse_get(hecmw33_ctx* ctx, int32_t* n_agg, int32_t* agg, float* dv, int32_t* nbr,
-                               double* Eblk) {
-  if (!ctx || !n_agg) return HECMW33_ERROR_ARGS;
+int32_t hecmw33_ctx_coarse_get(hecmw33_ctx* ctx, int32_t* n_agg, int32_t* n_pair, int32_t* agg, float* dv,
+                               int32_t* pair_a, int32_t* pair_b, double* Eblk) {
+  if (!ctx || !n_agg || !n_pair) return HECMW33_ERROR_ARGS;
   Context& c = ctx->c;
   if (!c.coarse_sym) {
     *n_agg = 0;
+    *n_pair = 0;
     return 0;
   }
-  const int32_t na = c.coarse.n_agg;
+  const int32_t na = c.coarse.n_agg, np = c.coarse_npair;
   *n_agg = na;
+  *n_pair = np;
   HIPC(hipStreamSynchronize(c.stream));
   if (agg) HIPC(hipMemcpy(agg, c.cg_agg.p, sizeof(int32_t) * c.N, hipMemcpyDeviceToHost));
   if (dv) HIPC(hipMemcpy(dv, c.cg_dv.p, sizeof(float) * 3 * (size_t)c.N, hipMemcpyDeviceToHost));
-  if (nbr) HIPC(hipMemcpy(nbr, c.cg_nbr.p, sizeof(int32_t) * 27 * (size_t)na, hipMemcpyDeviceToHost));
+  if (pair_a) HIPC(hipMemcpy(pair_a, c.cg_pa.p, sizeof(int32_t) * np, hipMemcpyDeviceToHost));
+  if (pair_b) HIPC(hipMemcpy(pair_b, c.cg_pb.p, sizeof(int32_t) * np, hipMemcpyDeviceToHost));
   if (Eblk) {
     if (!c.coarse_num) {
       set_error("coarse matrix not assembled for the current values");
       return HECMW33_ERROR_ARGS;
     }
-    HIPC(hipMemcpy(Eblk, c.cg_Eblk.p, sizeof(double) * 27 * 36 * (size_t)na, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(Eblk, c.cg_Eblk.p, sizeof(double) * 36 * (size_t)np, hipMemcpyDeviceToHost));
   }
   return 0;
 }

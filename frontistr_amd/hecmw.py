@@ -107,7 +107,7 @@ def lib():
     L.hecmw33_set_net.argtypes = [P]
     L.hecmw33_set_coords.argtypes = [I, P]
     L.hecmw33_ctx_set_coords.argtypes = [P, I, P]
-    L.hecmw33_ctx_coarse_get.argtypes = [P, P, P, P, P, P]
+    L.hecmw33_ctx_coarse_get.argtypes = [P, P, P, P, P, P, P, P]
     L.hecmw33_last_error.restype = C.c_char_p
     for f in ("hecmw33_solve", "hecmw33_solve_krylov", "hecmw33_matvec", "hecmw33_precond_apply",
               "hecmw33_rel_resid_L2", "hecmw33_ordering", "hecmw33_ctx_load", "hecmw33_ctx_solve_device",
@@ -460,28 +460,31 @@ class DeviceContext:
         return h[:n]
 
     def info(self):
-        v = np.zeros(12, np.int64)
-        _check(lib().hecmw33_ctx_info(self._h, _ptr(v), 12))
+        v = np.zeros(13, np.int64)
+        _check(lib().hecmw33_ctx_info(self._h, _ptr(v), 13))
         return {"nslots": int(v[0]), "N": int(v[1]), "NPL": int(v[2]), "NPU": int(v[3]), "ncolor": int(v[4]),
                 "ssor_slots": int(v[5]), "nreliable": int(v[6]), "nfallback": int(v[7]), "learned_off": int(v[8]),
-                "spmv64": int(v[9]), "probe_ratio": v[10] / 1e6, "coarse": int(v[11])}
+                "spmv64": int(v[9]), "probe_ratio": v[10] / 1e6, "coarse": int(v[11]), "coarse_pairs": int(v[12])}
 
     def coarse(self):
-        """The last two-level set-up: dict(n_agg, agg [N], dv [N,3], nbr [n_agg,27], Eblk [n_agg,27,6,6])."""
-        na = C.c_int32(0)
-        _check(lib().hecmw33_ctx_coarse_get(self._h, C.byref(na), None, None, None, None))
-        na = na.value
+        """The last two-level set-up: dict(n_agg, n_pair, agg [N], dv [N,3], pair_a, pair_b [n_pair],
+        Eblk [n_pair,6,6])."""
+        na, npair = C.c_int32(0), C.c_int32(0)
+        _check(lib().hecmw33_ctx_coarse_get(self._h, C.byref(na), C.byref(npair), None, None, None, None, None))
+        na, npair = na.value, npair.value
         if na == 0:
             return None
         N = self.info()["N"]
         agg = np.zeros(N, np.int32)
         dv = np.zeros(3 * N, np.float32)
-        nbr = np.zeros(27 * na, np.int32)
-        E = np.zeros(27 * 36 * na)
-        n2 = C.c_int32(0)
-        _check(lib().hecmw33_ctx_coarse_get(self._h, C.byref(n2), _ptr(agg), _ptr(dv), _ptr(nbr), _ptr(E)))
-        return {"n_agg": na, "agg": agg, "dv": dv.reshape(N, 3), "nbr": nbr.reshape(na, 27),
-                "Eblk": E.reshape(na, 27, 6, 6)}
+        pa = np.zeros(npair, np.int32)
+        pb = np.zeros(npair, np.int32)
+        E = np.zeros(36 * npair)
+        n2, p2 = C.c_int32(0), C.c_int32(0)
+        _check(lib().hecmw33_ctx_coarse_get(self._h, C.byref(n2), C.byref(p2), _ptr(agg), _ptr(dv), _ptr(pa),
+                                            _ptr(pb), _ptr(E)))
+        return {"n_agg": na, "n_pair": npair, "agg": agg, "dv": dv.reshape(N, 3), "pair_a": pa, "pair_b": pb,
+                "Eblk": E.reshape(npair, 6, 6)}
 
     def set_coords(self, xyz):
         """Node coordinates (3*NP, xyz interleaved) for the two-level correction."""

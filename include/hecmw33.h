@@ -100,13 +100,12 @@ int32_t hecmw33_solve_krylov(const hecmw33_sys* A, const double* B, double* X, i
  *   loses the search directions' conjugacy on high-contrast systems), else FP32.
  *   iarray[42] (Iarray 43) > 0 adds a two-level correction to the flexible CG's
  *   preconditioner, z = M^-1 r + Z E^-1 Z^T r: Z holds the six rigid-body modes
- *   (three translations, three rotations about the centroid) of each aggregate,
- *   the non-empty cells of a uniform grid over the node coordinates given by
- *   hecmw33_set_coords / hecmw33_ctx_set_coords; E = Z^T A Z is inverted densely
- *   (FP64 Cholesky, FP32 copy for the iterations).  1 aims at 1000 aggregates,
- *   larger values at that many (at most 2048).  Cells grow until each matrix
- *   block couples nodes of the same or adjacent cells.  If E is not positive
- *   definite the solve runs without it (hecmw33_ctx_info [11] = -1).
+ *   (three translations, three rotations about the centroid) of each aggregate;
+ *   E = Z^T A Z is inverted densely (FP64 Cholesky, FP32 copy for the
+ *   iterations).  Aggregates: the nodes of each cell of a uniform grid over the
+ *   node coordinates given by hecmw33_set_coords / hecmw33_ctx_set_coords.  1 aims
+ *   at 1000 aggregates, larger values at that many (at most 2048).  If E is not positive definite the solve runs
+ *   without it (hecmw33_ctx_info [11] = -1).
  *   Accepted by hecmw33_solve, hecmw33_solve_krylov and hecmw33_ctx_solve_device. */
 
 /* Node coordinates for the two-level correction: xyz interleaved, 3*NP doubles
@@ -167,16 +166,17 @@ int32_t hecmw33_ctx_history(hecmw33_ctx* ctx, double* hist, int32_t cap);
  * solve dropped the learned correction (or was not a learned solve), [9] 1 if the last flexible
  * CG used the FP64 matrix values in q = A p, [10] the last learned probe's error reduction with
  * / without the correction, in millionths, [11] aggregates of the last flexible CG's two-level
- * correction (0: none, -1: E was not positive definite and the solve ran without it). */
+ * correction (0: none, -1: E was not positive definite and the solve ran without it), [12] coupled
+ * aggregate pairs (blocks of E). */
 int32_t hecmw33_ctx_info(hecmw33_ctx* ctx, int64_t* info, int32_t n);
 /* the two-level correction's node coordinates for a device context (as hecmw33_set_coords) */
 int32_t hecmw33_ctx_set_coords(hecmw33_ctx* ctx, int32_t NP, const double* xyz);
-/* Inspection of the last two-level set-up: *n_agg aggregates (0: none built); any non-null
- * array receives agg (N: aggregate of each node), dv (3N: the scaled offsets from the
- * aggregate centroid that define Z), nbr (27*n_agg: aggregate of neighbour cell
- * s = 9(dz+1)+3(dy+1)+(dx+1), or -1) and Eblk (27*36*n_agg: E_(a,nbr[27a+s]) row-major 6x6). */
-int32_t hecmw33_ctx_coarse_get(hecmw33_ctx* ctx, int32_t* n_agg, int32_t* agg, float* dv, int32_t* nbr,
-                               double* Eblk);
+/* Inspection of the last two-level set-up: *n_agg aggregates (0: none built) and *n_pair coupled
+ * aggregate pairs; any non-null array receives agg (N: aggregate of each node), dv (3N: the
+ * scaled offsets from the aggregate centroid that define Z), pair_a / pair_b (n_pair: the pairs
+ * (a, b) coupled by a matrix block, b ascending per a) and Eblk (36*n_pair: E_(a,b) row-major 6x6). */
+int32_t hecmw33_ctx_coarse_get(hecmw33_ctx* ctx, int32_t* n_agg, int32_t* n_pair, int32_t* agg, float* dv,
+                               int32_t* pair_a, int32_t* pair_b, double* Eblk);
 
 /* ---- batched solves (no counterpart in the reference: hecmw_solve_CG handles one system and
  * one right-hand side per call; these solve many systems with up to 16 right-hand sides each

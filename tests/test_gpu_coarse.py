@@ -3,11 +3,11 @@ has only DIAG and SSOR, so this has no counterpart there; it is pinned by
 
 * the coarse matrix: E = Z^T A Z recomputed with dense numpy products from the
   library's own aggregation and offsets (a wrong rotation column, sign or
-  aggregate pair fails), to FP64 summation-order rounding, and every non-zero
-  block of the dense product inside the library's neighbour table;
+  aggregate pair fails), to FP64 summation-order rounding;
 * the aggregation invariants: every internal node in one aggregate, the
   offsets an affine image of the coordinates (one scale, zero centroid per
-  aggregate), every matrix block coupling the same or neighbouring cells;
+  aggregate), and the pair list equal to the set of aggregate pairs that a
+  matrix block couples;
 * end to end: FP64 true residual at RESID and the solution within 1e-6 of the
   reference's own SSOR-CG solution, on HEX8 and unstructured TET10 meshes,
   with far fewer iterations than without the correction.
@@ -51,21 +51,26 @@ def _z(agg, dv, na):
     return Z
 
 
-@pytest.mark.parametrize("case", ["hex8", "tet10"])
+@pytest.mark.parametrize("case", ["hex8", "tet10", "hex8_inclusions"])
 def test_coarse_matrix_is_ZtAZ(case):
     if case == "hex8":
         m = fa.gen_hex8(7, kind="lognormal", seed=2)
         xyz = fa.hex8_coords(7)
         target = 27
-    else:
+    elif case == "tet10":
         m, _ = fa.gen_tet10(3, 2, 4, box=(1.5, 0.8, 2.0), seed=1)
         xyz = fa.tet10_coords(3, 2, 4, box=(1.5, 0.8, 2.0))
+        target = 8
+    else:
+        m = fa.gen_hex8(10, kind="inclusions", seed=3, E=1000.0, contrast=1e4, rmin=2, rmax=3, vfrac=0.15)
+        xyz = fa.hex8_coords(10)
         target = 8
     ctx, r, _ = _ctx_solve(m, xyz, target)
     assert r.error == 0
     cs = ctx.coarse()
+    inf = ctx.info()
     ctx.close()
-    na, agg, dv, nbr, Eb = cs["n_agg"], cs["agg"], cs["dv"], cs["nbr"], cs["Eblk"]
+    na, agg, dv, pa, pb, Eb = cs["n_agg"], cs["agg"], cs["dv"], cs["pair_a"], cs["pair_b"], cs["Eblk"]
     assert 1 < na <= 2048 and agg.min() == 0 and agg.max() == na - 1
     assert len(np.unique(agg)) == na
     # offsets: one scale s with dv_i = s (x_i - c_agg(i)), zero mean per aggregate
@@ -75,26 +80,24 @@ def test_coarse_matrix_is_ZtAZ(case):
     big = (np.abs(rel) > 1e-3 * np.abs(rel).max()) & (dv != 0)  # degenerate aggregates have dv = 0
     s = dv[big] / rel[big]
     assert np.allclose(s, s.mean(), rtol=1e-5), (s.min(), s.max())
-    # neighbour table: the aggregates of the 27 cells around a's own
-    assert np.all(nbr[np.arange(na), 13] == np.arange(na))
+    # the pair list is exactly the set of aggregate pairs coupled by a matrix block
+    coupled = set()
+    for i in range(m.N):
+        coupled.add((agg[i], agg[i]))
+        for q in range(m.indexL[i], m.indexL[i + 1]):
+            coupled.add((agg[i], agg[m.itemL[q] - 1]))
+        for q in range(m.indexU[i], m.indexU[i + 1]):
+            coupled.add((agg[i], agg[m.itemU[q] - 1]))
+    assert coupled == set(zip(pa.tolist(), pb.tolist()))
     A = dense(m)
     Z = _z(agg, dv, na)
     E = Z.T @ A @ Z
     scale = np.abs(E).max()
     Elib = np.zeros_like(E)
-    for a in range(na):
-        for sl in range(27):
-            b = nbr[a, sl]
-            if b >= 0:
-                Elib[6 * a:6 * a + 6, 6 * b:6 * b + 6] = Eb[a, sl]
+    for p in range(len(pa)):
+        Elib[6 * pa[p]:6 * pa[p] + 6, 6 * pb[p]:6 * pb[p] + 6] = Eb[p]
     assert np.abs(Elib - E).max() <= 1e-11 * scale, np.abs(Elib - E).max() / scale
-    # every block coupling lies between a node's cell and a neighbour cell
-    for i in range(m.N):
-        nb = set(nbr[agg[i]][nbr[agg[i]] >= 0])
-        for q in list(range(m.indexL[i], m.indexL[i + 1])):
-            assert agg[m.itemL[q] - 1] in nb
-        for q in list(range(m.indexU[i], m.indexU[i + 1])):
-            assert agg[m.itemU[q] - 1] in nb
+    assert inf["coarse_pairs"] == len(pa)
 
 
 @pytest.mark.parametrize("case", ["hex8_lognormal", "hex8_inclusions", "tet10_box"])
