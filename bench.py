@@ -425,10 +425,38 @@ def bench_batch(args):
     log("C4: %.3fs, iterations %d..%d, max resid %.2e, errors %d, SpMM %s %.3f ms" % (
         wall, it.min(), it.max(), rs.max(), int((er != 0).sum()), info["variant"], spmm_ms))
     bs.close()
+    # the same instances one at a time through the single-system path (first
+    # right-hand side of every system, x 16): FP64 DIAG-CG (the reference's
+    # method) and the single-system mixed DIAG flexible CG
+    import torch
+    seq = {}
+    for name, opt in (("fp64_diag_cg", 0), ("mixed_diag_fcg", 1)):
+        total, iters = 0.0, 0
+        for s_, mat in enumerate(mats):
+            mm = mat.copy()
+            mm.set_iter(20000)
+            mm.set_precond(3)
+            mm.set_i(41, opt)
+            ctx = fa.DeviceContext()
+            ctx.load(mm)
+            b = torch.from_numpy(np.ascontiguousarray(Bs[s_][0])).to("cuda:0")
+            xv = torch.zeros_like(b)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            r = ctx.solve_device(b.data_ptr(), xv.data_ptr(), mm.Iarray, mm.Rarray)
+            torch.cuda.synchronize()
+            total += time.perf_counter() - t1
+            iters += r.iter
+            ctx.close()
+        seq[name] = {"seconds_first_rhs": round(total, 3), "iterations_first_rhs": iters,
+                     "seconds_all_rhs_est": round(16 * total, 2)}
+        log("C4 one at a time, %s: %.2fs for the 48 first right-hand sides (x16: %.1fs)" % (name, total, 16 * total))
     return {"workload": "%d TET10 boxes (5k-600k DOF log-uniform, aspect 0.5-2, seed 0) x 16 RHS, %d DOF, %d blocks"
                         % (len(mats), dof, blocks),
             "seconds": round(wall, 4), "instances": int(it.size), "converged": int((er == 0).sum()),
             "iterations_min": int(it.min()), "iterations_max": int(it.max()), "max_rel_resid": float(rs.max()),
+            "one_at_a_time": seq,
+            "speedup_vs_one_at_a_time_fp64": round(seq["fp64_diag_cg"]["seconds_all_rhs_est"] / wall, 2),
             "spmm_variant": info["variant"], "spmm_ms": spmm_ms,
             "spmm_GBs": round(spmm_bytes / (spmm_ms * 1e-3) / 1e9, 1), "load_s": round(t_load, 2)}
 
