@@ -222,7 +222,7 @@ def main():
             "ncolor": r0.ncolor, "iterations": iters, "rel_resid_true": true_resid,
         },
         "roofline": {
-            "bound": "hbm", "kernel": "k_cg_spmv_dot (q = A p, p.q)",
+            "bound": "hbm", "kernel": "k_spmv<kEpiDotX> (q = A p with p.q)",
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": SPMV_TRAFFIC_100 if args.n == 100 else None,

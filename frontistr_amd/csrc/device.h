@@ -129,7 +129,10 @@ void cg_precond_none_rho(int64_t n3, const double* r, double* z, KrylovState* st
 // p = z | z + beta p
 void cg_xpay(int64_t n3, const double* z, double* p, const KrylovState* st, hipStream_t s);
 // q = A p, c1 = p.q, alpha
-void cg_spmv_dot(const Sell<double>& a, const double* p, double* q, KrylovState* st, double* part, hipStream_t s);
+// the SpMV launchers with a fused dot record `after` (if given) between the SpMV
+// kernel and its scalar tail, so TIMELOG's time_Ax is the SpMV kernel alone
+void cg_spmv_dot(const Sell<double>& a, const double* p, double* q, KrylovState* st, double* part, hipStream_t s,
+                 hipEvent_t after = nullptr);
 // x += alpha p, r -= alpha q, |r|^2
 void cg_update(int64_t n3, const double* p, const double* q, double* x, double* r, KrylovState* st, double* part,
                double* hist, hipStream_t s);
@@ -144,10 +147,10 @@ void cg_rho(int64_t n3, const double* r, const double* z, KrylovState* st, doubl
 void bi_rho(int64_t n3, const double* r, const double* rt, KrylovState* st, double* part, hipStream_t s);
 void bi_p(int64_t n3, const double* r, const double* v, double* p, const KrylovState* st, hipStream_t s);
 void bi_spmv_c2(const Sell<double>& a, const double* pt, const double* rt, double* v, KrylovState* st, double* part,
-                hipStream_t s);
+                hipStream_t s, hipEvent_t after = nullptr);
 void bi_s(int64_t n3, const double* v, const double* r, double* sv, const KrylovState* st, hipStream_t s);
 void bi_spmv_omega(const Sell<double>& a, const double* stv, const double* sv, double* t, KrylovState* st,
-                   double* part, hipStream_t s);
+                   double* part, hipStream_t s, hipEvent_t after = nullptr);
 void bi_update(int64_t n3, const double* pt, const double* stv, const double* t, const double* sv, double* x,
                double* r, KrylovState* st, double* part, double* hist, hipStream_t s);
 
@@ -162,10 +165,10 @@ void fcg_precond_dots(int32_t kind, const SsorOp<float>& op, const float* alu_di
 void fcg_p(int64_t n3, const float* z, float* p, const KrylovState* st, hipStream_t s);
 // q = A32 p, pq = p.q, rp = r.p, alpha = rp / pq
 void fcg_spmv(const Sell<float>& a, const float* p, const float* r, float* q, KrylovState* st, double* part,
-              hipStream_t s);
+              hipStream_t s, hipEvent_t after = nullptr);
 // the same with the FP64 matrix values (FP32 vectors, FP64 accumulation)
 void fcg_spmv_mixed(const Sell<double>& a, const float* p, const float* r, float* q, KrylovState* st, double* part,
-                    hipStream_t s);
+                    hipStream_t s, hipEvent_t after = nullptr);
 // x64 += alpha p, r32 -= alpha q, |r|^2, reliable-update decision
 void fcg_update(int64_t n3, const float* p, const float* q, double* x, float* r, KrylovState* st, double* part,
                 double* hist, hipStream_t s);

@@ -1342,9 +1342,11 @@ void cg_rho(int64_t n3, const double* r, const double* z, KrylovState* st, doubl
 void cg_xpay(int64_t n3, const double* z, double* p, const KrylovState* st, hipStream_t s) {
   k_cg_xpay<<<vec_grid(n3), kBlock, 0, s>>>(n3, z, p, st);
 }
-void cg_spmv_dot(const Sell<double>& a, const double* p, double* q, KrylovState* st, double* part, hipStream_t s) {
+void cg_spmv_dot(const Sell<double>& a, const double* p, double* q, KrylovState* st, double* part, hipStream_t s,
+                 hipEvent_t after) {
   const int g = spmv_grid(a.nslices);
   spmv_launch<kEpiDotX>(g, s, a, p, nullptr, q, st, part);
+  if (after) hipEventRecord(after, s);
   k_tail<kTailCgC1, 1><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, nullptr);
 }
 void cg_update(int64_t n3, const double* p, const double* q, double* x, double* r, KrylovState* st, double* part,
@@ -1372,18 +1374,20 @@ void bi_p(int64_t n3, const double* r, const double* v, double* p, const KrylovS
   k_bi_p<<<vec_grid(n3), kBlock, 0, s>>>(n3, r, v, p, st);
 }
 void bi_spmv_c2(const Sell<double>& a, const double* pt, const double* rt, double* v, KrylovState* st, double* part,
-                hipStream_t s) {
+                hipStream_t s, hipEvent_t after) {
   const int g = spmv_grid(a.nslices);
   spmv_launch<kEpiDotW>(g, s, a, pt, rt, v, st, part);
+  if (after) hipEventRecord(after, s);
   k_tail<kTailBiC2, 1><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, nullptr);
 }
 void bi_s(int64_t n3, const double* v, const double* r, double* sv, const KrylovState* st, hipStream_t s) {
   k_bi_s<<<vec_grid(n3), kBlock, 0, s>>>(n3, v, r, sv, st);
 }
 void bi_spmv_omega(const Sell<double>& a, const double* stv, const double* sv, double* t, KrylovState* st,
-                   double* part, hipStream_t s) {
+                   double* part, hipStream_t s, hipEvent_t after) {
   const int g = spmv_grid(a.nslices);
   spmv_launch<kEpiDot2W>(g, s, a, stv, sv, t, st, part);
+  if (after) hipEventRecord(after, s);
   k_tail<kTailBiOmega, 2><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, nullptr);
 }
 void bi_update(int64_t n3, const double* pt, const double* stv, const double* t, const double* sv, double* x,
@@ -1465,15 +1469,17 @@ void fcg_p(int64_t n3, const float* z, float* p, const KrylovState* st, hipStrea
   k_fcg_p<<<vec_grid(n3), kBlock, 0, s>>>(n3, z, p, st);
 }
 void fcg_spmv(const Sell<float>& a, const float* p, const float* r, float* q, KrylovState* st, double* part,
-              hipStream_t s) {
+              hipStream_t s, hipEvent_t after) {
   const int g = spmv_grid(a.nslices);
   k_spmv_f32<true><<<g, kBlock, 0, s>>>(a, p, r, q, st, part);
+  if (after) hipEventRecord(after, s);
   k_tail<kTailFcgAlpha, 2><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, nullptr);
 }
 void fcg_spmv_mixed(const Sell<double>& a, const float* p, const float* r, float* q, KrylovState* st, double* part,
-                    hipStream_t s) {
+                    hipStream_t s, hipEvent_t after) {
   const int g = spmv_grid(a.nslices);
   k_spmv_mixed<<<g, kBlock, 0, s>>>(a, p, r, q, st, part);
+  if (after) hipEventRecord(after, s);
   k_tail<kTailFcgAlpha, 2><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, nullptr);
 }
 void fcg_update(int64_t n3, const float* p, const float* q, double* x, float* r, KrylovState* st, double* part,

@@ -763,8 +763,7 @@ static void enqueue_cg_iteration(Context& c, int32_t kind, int32_t iterpremax, h
   if (ev) hipEventRecord(ev[1], c.stream);
   cg_xpay(n3, z, c.vp.as<double>(), st, c.stream);
   if (ev) hipEventRecord(ev[2], c.stream);
-  cg_spmv_dot(c.A.view, c.vp.as<double>(), c.vq.as<double>(), st, part, c.stream);
-  if (ev) hipEventRecord(ev[3], c.stream);
+  cg_spmv_dot(c.A.view, c.vp.as<double>(), c.vq.as<double>(), st, part, c.stream, ev ? ev[3] : nullptr);
   cg_update(n3, c.vp.as<double>(), c.vq.as<double>(), c.vx.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
   cg_true_resid(c.A.view, c.vx.as<double>(), c.vb.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
 }
@@ -783,15 +782,13 @@ static void enqueue_bicg_iteration(Context& c, int32_t kind, int32_t iterpremax,
   enqueue_precond(c, kind, iterpremax, p, pt);
   if (ev) hipEventRecord(ev[1], c.stream);
   if (ev) hipEventRecord(ev[2], c.stream);
-  bi_spmv_c2(c.A.view, pt, rt, v, st, part, c.stream);
-  if (ev) hipEventRecord(ev[3], c.stream);
+  bi_spmv_c2(c.A.view, pt, rt, v, st, part, c.stream, ev ? ev[3] : nullptr);
   bi_s(n3, v, r, sv, st, c.stream);
   if (ev) hipEventRecord(ev[4], c.stream);
   enqueue_precond(c, kind, iterpremax, sv, stv);
   if (ev) hipEventRecord(ev[5], c.stream);
   if (ev) hipEventRecord(ev[6], c.stream);
-  bi_spmv_omega(c.A.view, stv, sv, t, st, part, c.stream);
-  if (ev) hipEventRecord(ev[7], c.stream);
+  bi_spmv_omega(c.A.view, stv, sv, t, st, part, c.stream, ev ? ev[7] : nullptr);
   bi_update(n3, pt, stv, t, sv, c.vx.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
   cg_true_resid(c.A.view, c.vx.as<double>(), c.vb.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
 }
@@ -1084,10 +1081,9 @@ static void enqueue_fcg_iteration(Context& c, int32_t kind, bool spmv64, hipEven
   fcg_p(n3, z, p, st, c.stream);
   if (ev) hipEventRecord(ev[2], c.stream);
   if (spmv64)
-    fcg_spmv_mixed(c.A.view, p, r, q, st, part, c.stream);
+    fcg_spmv_mixed(c.A.view, p, r, q, st, part, c.stream, ev ? ev[3] : nullptr);
   else
-    fcg_spmv(c.A.view32, p, r, q, st, part, c.stream);
-  if (ev) hipEventRecord(ev[3], c.stream);
+    fcg_spmv(c.A.view32, p, r, q, st, part, c.stream, ev ? ev[3] : nullptr);
   fcg_update(n3, p, q, c.vx.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
   fcg_reliable(c.A.view, c.vx.as<double>(), c.vb.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
 }
@@ -1111,10 +1107,9 @@ static void enqueue_fcg_learned_iteration(Context& c, bool spmv64, hipEvent_t* e
   fcg_p_sel(n3, z, z0, p, st, c.stream);
   if (ev) hipEventRecord(ev[2], c.stream);
   if (spmv64)
-    fcg_spmv_mixed(c.A.view, p, r, q, st, part, c.stream);
+    fcg_spmv_mixed(c.A.view, p, r, q, st, part, c.stream, ev ? ev[3] : nullptr);
   else
-    fcg_spmv(c.A.view32, p, r, q, st, part, c.stream);
-  if (ev) hipEventRecord(ev[3], c.stream);
+    fcg_spmv(c.A.view32, p, r, q, st, part, c.stream, ev ? ev[3] : nullptr);
   fcg_update(n3, p, q, c.vx.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
   fcg_reliable(c.A.view, c.vx.as<double>(), c.vb.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
 }
