@@ -8,6 +8,8 @@ libhecmw33.so (include/hecmw33.h):
   ``Iarray``/``Rarray`` initialised as ``hecmw_mat_init`` does
   (hecmw1/src/solver/matrix/hecmw_matrix_misc.F90:171-212) and the same
   named accessors.
+* :func:`read_dump` / :func:`write_dump` read and write hecmw_mat_dump's
+  MM/CSR/BSR files (hecmw1/src/solver/matrix/hecmw_matrix_dump.f90).
 * :func:`hecmw_solve_iterative` / :func:`hecmw_solve_krylov` /
   :func:`hecmw_matvec` / :func:`hecmw_precond_apply` /
   :func:`hecmw_rel_resid_L2` call the GPU library.
@@ -42,3 +44,4 @@ from .hecmw import (  # noqa: F401
     harvest_residuals,
     DeviceContext,
 )
+from .dump import read_dump, write_dump  # noqa: F401

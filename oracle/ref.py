@@ -50,6 +50,8 @@ def lib():
         L.ref_precond.restype = None
         L.ref_rel_resid.argtypes = [I, I, I, I] + [P] * 9 + [I]
         L.ref_rel_resid.restype = C.c_double
+        L.ref_mat_dump.argtypes = [I, I, I, I] + [P] * 9 + [I]
+        L.ref_mat_dump.restype = None
         L.ref_flush.argtypes = []
         L.ref_flush.restype = None
         _lib = L
@@ -172,3 +174,25 @@ def rel_resid(m, x, nthreads=NTHREADS):
     x = np.array(x, dtype=np.float64)
     b = m.B.copy()
     return lib().ref_rel_resid(*_mat_args(m), _p(b), _p(x), nthreads)
+
+
+def mat_dump(m, kind, directory):
+    """The reference's hecmw_mat_dump (DUMPTYPE kind: "mm", "csr" or "bsr") and
+    hecmw_mat_dump_solution of m, run in `directory` (the Fortran writes
+    dump_matrix_<call>_0.* into the current directory).  Returns the path of
+    the matrix file; .rhs and .sol lie beside it."""
+    import glob
+    code = {"mm": 1, "csr": 2, "bsr": 3}[kind]
+    b = m.B.copy()
+    x = m.X.copy()
+    here = os.getcwd()
+    os.chdir(directory)
+    try:
+        before = set(glob.glob("dump_matrix_*." + kind))
+        lib().ref_mat_dump(*_mat_args(m), _p(b), _p(x), code)
+        new = sorted(set(glob.glob("dump_matrix_*." + kind)) - before)
+    finally:
+        os.chdir(here)
+    if len(new) != 1:
+        raise RuntimeError("reference dump wrote %s" % new)
+    return os.path.join(directory, new[0])

@@ -19,6 +19,8 @@
 !   ref_precond       -> hecmw_precond_setup + hecmw_precond_apply
 !                        (hecmw_precond.f90:30,86)
 !   ref_rel_resid     -> hecmw_rel_resid_L2 (hecmw_solver_las.f90:124)
+!   ref_mat_dump      -> hecmw_mat_dump + hecmw_mat_dump_solution
+!                        (hecmw_matrix_dump.f90:31,345)
 !-------------------------------------------------------------------------------
 module hecmw_ref_driver
   use iso_c_binding
@@ -224,6 +226,25 @@ contains
     call bind_matrix(n, np, npl, npu, d, al, au, indexl, iteml, indexu, itemu, b, x, ia, ra)
     res = hecmw_rel_resid_L2(mesh, mat)
   end function ref_rel_resid
+
+  !> hecmw_mat_dump (type 1 MM, 2 CSR, 3 BSR, with the right-hand side) and
+  !> hecmw_mat_dump_solution: the files dump_matrix_<call>_0.{mm,csr,bsr,rhs,sol}
+  !> land in the current directory (hecmw1/src/solver/matrix/hecmw_matrix_dump.f90).
+  subroutine ref_mat_dump(n, np, npl, npu, d, al, au, indexl, iteml, indexu, itemu, &
+      b, x, dump_type) bind(c, name='ref_mat_dump')
+    use hecmw_matrix_misc
+    use hecmw_matrix_dump
+    integer(c_int), value :: n, np, npl, npu, dump_type
+    type(c_ptr), value :: d, al, au, indexl, iteml, indexu, itemu, b, x
+    integer(c_int) :: ia(100)
+    real(c_double) :: ra(100)
+    ia = 0; ra = 0.d0
+    call bind_matrix(n, np, npl, npu, d, al, au, indexl, iteml, indexu, itemu, b, x, ia, ra)
+    call hecmw_mat_set_dump(mat, dump_type)
+    call hecmw_mat_set_dump_exit(mat, 0)
+    call hecmw_mat_dump(mat, mesh)
+    call hecmw_mat_dump_solution(mat)
+  end subroutine ref_mat_dump
 
   !> Flush Fortran unit 6 (ITERLOG lines) so a caller capturing stdout sees them.
   subroutine ref_flush() bind(c, name='ref_flush')
