@@ -28,6 +28,8 @@ from .hecmw import (  # noqa: F401
     hecmw_solve_krylov,
     last_history,
     hecmw_matvec,
+    hecmw_matvec_mpc,
+    set_mpc,
     hecmw_precond_apply,
     hecmw_rel_resid_L2,
     hecmw_ordering,

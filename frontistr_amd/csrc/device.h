@@ -103,9 +103,31 @@ void launch_block_lu_f32(int32_t nslots, const int32_t* row_src, const double* D
                          hipStream_t s);
 
 // ---- plain operators -------------------------------------------------------------
+// Multipoint constraints of the MPC-CG SpMV y = T^T A T x (hecmw_TtmatTvec_33,
+// hecmw_solver_las_33.F90:442-586; hecMESH%mpc): T is the identity except each
+// slave dof's row, which is minus the constraint's master coefficients, so
+// (T x)_slave = -sum_j val_j x_master_j and T^T zeroes the slave entries and
+// subtracts val_j (A T x)_slave from each master.  Constraints with a dof above 3
+// are skipped, as the reference skips them.
+struct MpcOp {
+  int32_t n3 = 0;                     // 3 N internal dofs
+  const int32_t* slave_row = nullptr; // [3N] constraint whose slave this dof is, -1 if none
+  const int32_t* mptr = nullptr;      // [nmpc+1] master terms of each constraint, in order
+  const int32_t* mdof = nullptr;      //   master dof (0-based)
+  const double* mval = nullptr;       //   coefficient
+  const int32_t* tptr = nullptr;      // [3N+1] per dof, the constraints it is a master of,
+  const int32_t* tslave = nullptr;    //   in constraint order: their slave dof
+  const double* tval = nullptr;       //   and its coefficient there
+  double* w1 = nullptr;               // T x   (3 NP)
+  double* w2 = nullptr;               // A T x (3 NP)
+};
+
+// with mpc (n3 > 0) the SpMV launchers below multiply by T^T A T instead of A
 void launch_spmv(const Sell<double>& a, const double* x, double* y, hipStream_t s);
+void launch_spmv_mpc(const Sell<double>& a, const MpcOp& mpc, const double* x, double* y, hipStream_t s);
 void launch_spmv_f32(const Sell<float>& a, const float* x, float* y, hipStream_t s);
-void launch_matresid(const Sell<double>& a, const double* x, const double* b, double* r, hipStream_t s);
+void launch_matresid(const Sell<double>& a, const double* x, const double* b, double* r, hipStream_t s,
+                     const MpcOp* mpc = nullptr);
 void launch_dot_store(int64_t n, const double* x, const double* y, double* out, KrylovState* st, double* part,
                       hipStream_t s);
 void launch_diag_apply(int32_t n, const double* alu, const double* r, double* z, const KrylovState* st,
@@ -114,7 +136,7 @@ void launch_copy(int64_t n3, const double* r, double* z, const KrylovState* st, 
 void launch_zero(int64_t n, double* x, const KrylovState* st, hipStream_t s);
 void launch_add(int64_t n, const double* x, double* y, const KrylovState* st, hipStream_t s);
 void launch_matresid_guarded(const Sell<double>& a, const double* x, const double* b, double* r,
-                             const KrylovState* st, hipStream_t s);
+                             const KrylovState* st, hipStream_t s, const MpcOp* mpc = nullptr);
 // z = M^-1 r, kind 0: copy, 1/2: multicolour SSOR, 3: DIAG (guarded by st->done)
 void precond_apply(int32_t kind, const SsorOp<double>& op, const double* alu_diag, int32_t n, const double* r,
                    double* z, const KrylovState* st, hipStream_t s);
@@ -132,13 +154,13 @@ void cg_xpay(int64_t n3, const double* z, double* p, const KrylovState* st, hipS
 // the SpMV launchers with a fused dot record `after` (if given) between the SpMV
 // kernel and its scalar tail, so TIMELOG's time_Ax is the SpMV kernel alone
 void cg_spmv_dot(const Sell<double>& a, const double* p, double* q, KrylovState* st, double* part, hipStream_t s,
-                 hipEvent_t after = nullptr);
+                 hipEvent_t after = nullptr, const MpcOp* mpc = nullptr);
 // x += alpha p, r -= alpha q, |r|^2
 void cg_update(int64_t n3, const double* p, const double* q, double* x, double* r, KrylovState* st, double* part,
                double* hist, hipStream_t s);
 // conditional r = b - A x with |r|^2, convergence, end of iteration
 void cg_true_resid(const Sell<double>& a, const double* x, const double* b, double* r, KrylovState* st, double* part,
-                   double* hist, hipStream_t s);
+                   double* hist, hipStream_t s, const MpcOp* mpc = nullptr);
 
 // rho = r.z only (ITERPREMAX > 1 path)
 void cg_rho(int64_t n3, const double* r, const double* z, KrylovState* st, double* part, hipStream_t s);
@@ -147,10 +169,10 @@ void cg_rho(int64_t n3, const double* r, const double* z, KrylovState* st, doubl
 void bi_rho(int64_t n3, const double* r, const double* rt, KrylovState* st, double* part, hipStream_t s);
 void bi_p(int64_t n3, const double* r, const double* v, double* p, const KrylovState* st, hipStream_t s);
 void bi_spmv_c2(const Sell<double>& a, const double* pt, const double* rt, double* v, KrylovState* st, double* part,
-                hipStream_t s, hipEvent_t after = nullptr);
+                hipStream_t s, hipEvent_t after = nullptr, const MpcOp* mpc = nullptr);
 void bi_s(int64_t n3, const double* v, const double* r, double* sv, const KrylovState* st, hipStream_t s);
 void bi_spmv_omega(const Sell<double>& a, const double* stv, const double* sv, double* t, KrylovState* st,
-                   double* part, hipStream_t s, hipEvent_t after = nullptr);
+                   double* part, hipStream_t s, hipEvent_t after = nullptr, const MpcOp* mpc = nullptr);
 void bi_update(int64_t n3, const double* pt, const double* stv, const double* t, const double* sv, double* x,
                double* r, KrylovState* st, double* part, double* hist, hipStream_t s);
 

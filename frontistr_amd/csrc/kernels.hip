@@ -467,6 +467,7 @@ enum SpmvEpi : int {
   kEpiDotW = 3,    // y = A x, partial w.y            (BiCGSTAB: v = A pt, rt.v)
   kEpiDot2W = 4,   // y = A x, partials w.y and y.y   (BiCGSTAB: t = A st, t.s, t.t)
   kEpiTrue = 5,    // y = b - A x, partial y.y, only in residual-recompute iterations
+  kEpiStoreTrue = 6,  // y = A x, only in residual-recompute iterations (MPC: A T x of a true residual)
 };
 
 template <int EPI>
@@ -476,7 +477,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(Sell<double> a, const double* _
 #pragma clang fp contract(off)
   if (st) {
     if (st->done) return;
-    if (EPI == kEpiTrue && !true_resid_active(st)) return;
+    if ((EPI == kEpiTrue || EPI == kEpiStoreTrue) && !true_resid_active(st)) return;
   }
   constexpr int K = EPI == kEpiDot2W ? 2 : 1;
   const int lane = lane_id();
@@ -500,7 +501,65 @@ __global__ __launch_bounds__(kBlock) void k_spmv(Sell<double> a, const double* _
       if (EPI == kEpiDot2W || EPI == kEpiTrue) acc[K - 1] += o[0] * o[0] + o[1] * o[1] + o[2] * o[2];
     }
   }
-  if (EPI >= kEpiDotX) block_partials<K>(acc, part, blockIdx.x);
+  if (EPI >= kEpiDotX && EPI <= kEpiTrue) block_partials<K>(acc, part, blockIdx.x);
+}
+
+// ---- MPC-CG: y = T^T A T x (hecmw_TtmatTvec_33, hecmw_solver_las_33.F90:442-586) ------------
+// T x for the internal dofs (hecmw_Tvec_33): x, except a slave dof, which is
+// 0 - val_2 x_2 - val_3 x_3 - ... over its constraint's master terms in order.
+__global__ __launch_bounds__(kBlock) void k_mpc_tvec(MpcOp m, const double* __restrict__ x, double* __restrict__ y,
+                                                       const KrylovState* st, int guard) {
+#pragma clang fp contract(off)
+  if (st) {
+    if (st->done) return;
+    if (guard && !true_resid_active(st)) return;
+  }
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < m.n3; i += (int64_t)gridDim.x * kBlock) {
+    const int c = m.slave_row[i];
+    double v = x[i];
+    if (c >= 0) {
+      v = 0.0;
+      for (int t = m.mptr[c]; t < m.mptr[c + 1]; ++t) v = v - m.mval[t] * x[m.mdof[t]];
+    }
+    y[i] = v;
+  }
+}
+
+// T^T w for the internal rows (hecmw_Ttvec_33), w = A T x in m.w2: a slave entry
+// is 0; a master dof subtracts val * w_slave for every constraint it is a
+// master of, in constraint order (the reference's single-thread order), then
+// the SpMV epilogue of EPI on the node's three entries (x: the product's input).
+template <int EPI>
+__global__ __launch_bounds__(kBlock) void k_mpc_ttvec(MpcOp m, const double* __restrict__ x,
+                                                        const double* __restrict__ bw, double* __restrict__ y,
+                                                        const KrylovState* st, double* part) {
+#pragma clang fp contract(off)
+  if (st) {
+    if (st->done) return;
+    if (EPI == kEpiTrue && !true_resid_active(st)) return;
+  }
+  constexpr int K = EPI == kEpiDot2W ? 2 : 1;
+  double acc[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) acc[k] = 0.0;
+  const int64_t n = m.n3 / 3;
+  for (int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x; row < n; row += (int64_t)gridDim.x * kBlock) {
+    double o[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int64_t k = 3 * row + d;
+      double v = m.slave_row[k] >= 0 ? 0.0 : m.w2[k];
+      for (int t = m.tptr[k]; t < m.tptr[k + 1]; ++t) v = v - m.tval[t] * m.w2[m.tslave[t]];
+      o[d] = (EPI == kEpiResid || EPI == kEpiTrue) ? bw[k] - v : v;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) y[3 * row + d] = o[d];
+    if (EPI == kEpiDotX) acc[0] += x[3 * row] * o[0] + x[3 * row + 1] * o[1] + x[3 * row + 2] * o[2];
+    if (EPI == kEpiDotW || EPI == kEpiDot2W)
+      acc[0] += bw[3 * row] * o[0] + bw[3 * row + 1] * o[1] + bw[3 * row + 2] * o[2];
+    if (EPI == kEpiDot2W || EPI == kEpiTrue) acc[K - 1] += o[0] * o[0] + o[1] * o[1] + o[2] * o[2];
+  }
+  if (EPI >= kEpiDotX && EPI <= kEpiTrue) block_partials<K>(acc, part, blockIdx.x);
 }
 
 // FP32 SpMV; with DOTS (flexible CG: q = A p) also the FP64 partials of p.q
@@ -1185,14 +1244,38 @@ static void spmv_launch(int g, hipStream_t s, const Sell<double>& a, const doubl
   k_spmv<EPI><<<g, kBlock, 0, s>>>(a, x, bw, y, st, part);
 }
 
+// y = A x (or T^T A T x with MPC) with the epilogue EPI; returns the number of
+// partials written
+template <int EPI>
+static int spmv_any(hipStream_t s, const Sell<double>& a, const MpcOp* m, const double* x, const double* bw,
+                    double* y, const KrylovState* st, double* part) {
+  if (!m || m->n3 == 0) {
+    const int g = spmv_grid(a.nslices);
+    spmv_launch<EPI>(g, s, a, x, bw, y, st, part);
+    return g;
+  }
+  k_mpc_tvec<<<vec_grid(m->n3), kBlock, 0, s>>>(*m, x, m->w1, st, EPI == kEpiTrue ? 1 : 0);
+  if (EPI == kEpiTrue)
+    spmv_launch<kEpiStoreTrue>(spmv_grid(a.nslices), s, a, m->w1, nullptr, m->w2, st, nullptr);
+  else
+    spmv_launch<kEpiStore>(spmv_grid(a.nslices), s, a, m->w1, nullptr, m->w2, st, nullptr);
+  const int g = vec_grid(m->n3 / 3);
+  k_mpc_ttvec<EPI><<<g, kBlock, 0, s>>>(*m, x, bw, y, st, part);
+  return g;
+}
+
 void launch_spmv(const Sell<double>& a, const double* x, double* y, hipStream_t s) {
   spmv_launch<kEpiStore>(spmv_grid(a.nslices), s, a, x, nullptr, y, nullptr, nullptr);
+}
+void launch_spmv_mpc(const Sell<double>& a, const MpcOp& mpc, const double* x, double* y, hipStream_t s) {
+  spmv_any<kEpiStore>(s, a, &mpc, x, nullptr, y, nullptr, nullptr);
 }
 void launch_spmv_f32(const Sell<float>& a, const float* x, float* y, hipStream_t s) {
   k_spmv_f32<false><<<spmv_grid(a.nslices), kBlock, 0, s>>>(a, x, nullptr, y, nullptr, nullptr);
 }
-void launch_matresid(const Sell<double>& a, const double* x, const double* b, double* r, hipStream_t s) {
-  spmv_launch<kEpiResid>(spmv_grid(a.nslices), s, a, x, b, r, nullptr, nullptr);
+void launch_matresid(const Sell<double>& a, const double* x, const double* b, double* r, hipStream_t s,
+                     const MpcOp* mpc) {
+  spmv_any<kEpiResid>(s, a, mpc, x, b, r, nullptr, nullptr);
 }
 void launch_dot_store(int64_t n, const double* x, const double* y, double* out, KrylovState* st, double* part,
                       hipStream_t s) {
@@ -1313,8 +1396,8 @@ void launch_add(int64_t n, const double* x, double* y, const KrylovState* st, hi
   k_add<<<vec_grid(n), kBlock, 0, s>>>(n, x, y, st);
 }
 void launch_matresid_guarded(const Sell<double>& a, const double* x, const double* b, double* r,
-                             const KrylovState* st, hipStream_t s) {
-  spmv_launch<kEpiResid>(spmv_grid(a.nslices), s, a, x, b, r, st, nullptr);
+                             const KrylovState* st, hipStream_t s, const MpcOp* mpc) {
+  spmv_any<kEpiResid>(s, a, mpc, x, b, r, st, nullptr);
 }
 
 // ---- CG iteration pieces ------------------------------------------------------------
@@ -1343,9 +1426,8 @@ void cg_xpay(int64_t n3, const double* z, double* p, const KrylovState* st, hipS
   k_cg_xpay<<<vec_grid(n3), kBlock, 0, s>>>(n3, z, p, st);
 }
 void cg_spmv_dot(const Sell<double>& a, const double* p, double* q, KrylovState* st, double* part, hipStream_t s,
-                 hipEvent_t after) {
-  const int g = spmv_grid(a.nslices);
-  spmv_launch<kEpiDotX>(g, s, a, p, nullptr, q, st, part);
+                 hipEvent_t after, const MpcOp* mpc) {
+  const int g = spmv_any<kEpiDotX>(s, a, mpc, p, nullptr, q, st, part);
   if (after) hipEventRecord(after, s);
   k_tail<kTailCgC1, 1><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, nullptr);
 }
@@ -1358,9 +1440,8 @@ void cg_update(int64_t n3, const double* p, const double* q, double* x, double* 
 // only the non-periodic iterations produce the dnrm2 partials; the tail is
 // guarded the same way
 void cg_true_resid(const Sell<double>& a, const double* x, const double* b, double* r, KrylovState* st, double* part,
-                   double* hist, hipStream_t s) {
-  const int g = spmv_grid(a.nslices);
-  spmv_launch<kEpiTrue>(g, s, a, x, b, r, st, part);
+                   double* hist, hipStream_t s, const MpcOp* mpc) {
+  const int g = spmv_any<kEpiTrue>(s, a, mpc, x, b, r, st, part);
   k_tail_true<<<1, kTailThreads, 0, s>>>(st, part, g, hist);
 }
 
@@ -1374,9 +1455,8 @@ void bi_p(int64_t n3, const double* r, const double* v, double* p, const KrylovS
   k_bi_p<<<vec_grid(n3), kBlock, 0, s>>>(n3, r, v, p, st);
 }
 void bi_spmv_c2(const Sell<double>& a, const double* pt, const double* rt, double* v, KrylovState* st, double* part,
-                hipStream_t s, hipEvent_t after) {
-  const int g = spmv_grid(a.nslices);
-  spmv_launch<kEpiDotW>(g, s, a, pt, rt, v, st, part);
+                hipStream_t s, hipEvent_t after, const MpcOp* mpc) {
+  const int g = spmv_any<kEpiDotW>(s, a, mpc, pt, rt, v, st, part);
   if (after) hipEventRecord(after, s);
   k_tail<kTailBiC2, 1><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, nullptr);
 }
@@ -1384,9 +1464,8 @@ void bi_s(int64_t n3, const double* v, const double* r, double* sv, const Krylov
   k_bi_s<<<vec_grid(n3), kBlock, 0, s>>>(n3, v, r, sv, st);
 }
 void bi_spmv_omega(const Sell<double>& a, const double* stv, const double* sv, double* t, KrylovState* st,
-                   double* part, hipStream_t s, hipEvent_t after) {
-  const int g = spmv_grid(a.nslices);
-  spmv_launch<kEpiDot2W>(g, s, a, stv, sv, t, st, part);
+                   double* part, hipStream_t s, hipEvent_t after, const MpcOp* mpc) {
+  const int g = spmv_any<kEpiDot2W>(s, a, mpc, stv, sv, t, st, part);
   if (after) hipEventRecord(after, s);
   k_tail<kTailBiOmega, 2><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, nullptr);
 }

@@ -60,7 +60,8 @@ static double now_s() {
 enum {
   I_ITER = 1, I_METHOD = 2, I_PRECOND = 3, I_ITERPREMAX = 5, I_SCALING = 7, I_METHOD2 = 8,
   I_ITERLOG = 21, I_TIMELOG = 22, I_NCOLOR_IN = 34, I_MAXRECYCLE = 35, I_SOLVER_OPT = 41,
-  I_FLAG_CONVERGED = 81, I_FLAG_DIVERGED = 82, I_NRECYCLE = 96, I_FLAG_NUMFACT = 97, I_FLAG_SYMBFACT = 98,
+  I_MPC_METHOD = 13, I_FLAG_CONVERGED = 81, I_FLAG_DIVERGED = 82, I_FLAG_MPCMATVEC = 83, I_NRECYCLE = 96,
+  I_FLAG_NUMFACT = 97, I_FLAG_SYMBFACT = 98,
   R_RESID = 1, R_SIGMA_DIAG = 2, R_SOLVER_OPT = 41,
 };
 // SOLVER_OPT(1) (Iarray(41), unused by the reference's CG/BiCGSTAB with
@@ -258,6 +259,16 @@ struct Context {
   int32_t coarse_npair = 0;       // coupled aggregate pairs (blocks of E)
   DMem cg_agg, cg_aptr, cg_anodes, cg_dv, cg_pa, cg_pb, cg_pnptr, cg_pn, cg_Eblk, cg_E, cg_W, cg_cv, cg_yv, cg_info;
   rocblas_handle blas = nullptr;
+  // multipoint constraints (hecMESH%mpc) for the MPC-CG product T^T A T
+  // (Iarray 83 set, as hecmw_solve_iterative sets it for MPC_METHOD = 2)
+  int32_t mpc_n = 0, mpc_N = 0;  // constraints given, for N internal nodes
+  std::vector<int32_t> h_mpc_index, h_mpc_item, h_mpc_dof;
+  std::vector<double> h_mpc_val;
+  bool mpc_built = false;  // device structure built for the constraints and N
+  bool mpc_use = false;    // the products of this solve are T^T A T
+  MpcOp mpc;
+  DMem mpc_slave, mpc_mptr, mpc_mdof, mpc_mval, mpc_tptr, mpc_tslave, mpc_tval, mpc_w1, mpc_w2;
+  const MpcOp* mpc_op() const { return mpc_use ? &mpc : nullptr; }
   double contrast = -1.0;        // max/min trace of the diagonal blocks (< 0: not computed for these values)
   // TIMELOG: event pairs around the preconditioner and the SpMV of every
   // enqueued iteration (4 events per iteration)
@@ -289,6 +300,8 @@ struct Context {
                     &cg_cv, &cg_yv, &cg_info})
       m->release();
     if (blas) rocblas_destroy_handle(blas);
+    for (DMem* m : {&mpc_slave, &mpc_mptr, &mpc_mdof, &mpc_mval, &mpc_tptr, &mpc_tslave, &mpc_tval, &mpc_w1, &mpc_w2})
+      m->release();
     for (DMem* m : {&dD, &dAL, &dAU, &row_old, &alu_ssor, &row_diag, &alu_diag, &alu_ssor32, &alu_diag32, &vr32,
                     &vz32, &vp32, &vq32, &vb, &vx, &vr, &vz, &vp, &vq, &vrt, &vs, &vt, &vv, &vpt, &vst, &vzp, &vzt, &st,
                     &part, &hist})
@@ -418,6 +431,7 @@ static int32_t load_structure(Context& c, const hecmw33_sys* A, bool force) {
   c.ssor_init = false;
   c.ssor_sym = false;
   c.coarse_sym = false;
+  c.mpc_built = false;
   c.diag_init = false;
   c.dindex_valid = false;
   const size_t vb = sizeof(double) * 3 * (size_t)NP;
@@ -738,7 +752,7 @@ static void enqueue_precond(Context& c, int32_t kind, int32_t iterpremax, const 
     apply(zp, zt);
     launch_add(n3, zt, z, st, c.stream);
     if (it == iterpremax) break;
-    launch_matresid_guarded(c.A.view, z, r, zp, st, c.stream);
+    launch_matresid_guarded(c.A.view, z, r, zp, st, c.stream, c.mpc_op());
   }
 }
 
@@ -763,9 +777,11 @@ static void enqueue_cg_iteration(Context& c, int32_t kind, int32_t iterpremax, h
   if (ev) hipEventRecord(ev[1], c.stream);
   cg_xpay(n3, z, c.vp.as<double>(), st, c.stream);
   if (ev) hipEventRecord(ev[2], c.stream);
-  cg_spmv_dot(c.A.view, c.vp.as<double>(), c.vq.as<double>(), st, part, c.stream, ev ? ev[3] : nullptr);
+  cg_spmv_dot(c.A.view, c.vp.as<double>(), c.vq.as<double>(), st, part, c.stream, ev ? ev[3] : nullptr,
+              c.mpc_op());
   cg_update(n3, c.vp.as<double>(), c.vq.as<double>(), c.vx.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
-  cg_true_resid(c.A.view, c.vx.as<double>(), c.vb.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
+  cg_true_resid(c.A.view, c.vx.as<double>(), c.vb.as<double>(), r, st, part, c.hist.as<double>(), c.stream,
+                c.mpc_op());
 }
 
 // One BiCGSTAB iteration (hecmw_solver_BiCGSTAB.f90:164-275).  The reference
@@ -782,15 +798,16 @@ static void enqueue_bicg_iteration(Context& c, int32_t kind, int32_t iterpremax,
   enqueue_precond(c, kind, iterpremax, p, pt);
   if (ev) hipEventRecord(ev[1], c.stream);
   if (ev) hipEventRecord(ev[2], c.stream);
-  bi_spmv_c2(c.A.view, pt, rt, v, st, part, c.stream, ev ? ev[3] : nullptr);
+  bi_spmv_c2(c.A.view, pt, rt, v, st, part, c.stream, ev ? ev[3] : nullptr, c.mpc_op());
   bi_s(n3, v, r, sv, st, c.stream);
   if (ev) hipEventRecord(ev[4], c.stream);
   enqueue_precond(c, kind, iterpremax, sv, stv);
   if (ev) hipEventRecord(ev[5], c.stream);
   if (ev) hipEventRecord(ev[6], c.stream);
-  bi_spmv_omega(c.A.view, stv, sv, t, st, part, c.stream, ev ? ev[7] : nullptr);
+  bi_spmv_omega(c.A.view, stv, sv, t, st, part, c.stream, ev ? ev[7] : nullptr, c.mpc_op());
   bi_update(n3, pt, stv, t, sv, c.vx.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
-  cg_true_resid(c.A.view, c.vx.as<double>(), c.vb.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
+  cg_true_resid(c.A.view, c.vx.as<double>(), c.vb.as<double>(), r, st, part, c.hist.as<double>(), c.stream,
+                c.mpc_op());
 }
 
 // One flexible CG iteration with FP32 operators: z = M32^-1 r (fused r.z,
@@ -1175,6 +1192,138 @@ static int32_t krylov_resident(Context& c, int32_t* iarray, double* rarray, hecm
   return e;
 }
 
+// ---- multipoint constraints: the MPC-CG product T^T A T ---------------------------------
+// Validates hecMESH%mpc-style arrays for N internal nodes (1-based items and
+// dofs, mpc_index(0:n) cumulative): constraints with a dof above 3 are skipped,
+// as hecmw_Tvec_33 / hecmw_Ttvec_33 skip them; items must be internal nodes; a
+// slave dof may be the slave of one constraint only and no constraint's master
+// (the reference's parallel loops assume that too).
+static int32_t mpc_check(int32_t N, int32_t n, const int32_t* index, const int32_t* item, const int32_t* dof) {
+  if (n < 0 || N < 1 || (n > 0 && (!index || !item || !dof))) return HECMW33_ERROR_ARGS;
+  if (n == 0) return 0;
+  if (index[0] != 0) {
+    set_error("mpc_index(0) must be 0");
+    return HECMW33_ERROR_ARGS;
+  }
+  std::vector<char> role(3 * (size_t)N, 0);  // 1 slave, 2 master
+  for (int32_t i = 0; i < n; ++i) {
+    if (index[i + 1] <= index[i]) {
+      set_error("a multipoint constraint without terms");
+      return HECMW33_ERROR_ARGS;
+    }
+    bool skip = false;
+    for (int32_t j = index[i]; j < index[i + 1]; ++j) {
+      if (dof[j] > 3) skip = true;
+      if (dof[j] < 1 || item[j] < 1 || item[j] > N) {
+        set_error("a multipoint constraint on an external node or a dof out of range is not supported");
+        return HECMW33_ERROR_UNSUPPORTED;
+      }
+    }
+    if (skip) continue;
+    for (int32_t j = index[i]; j < index[i + 1]; ++j) {
+      const int64_t k = 3 * (int64_t)(item[j] - 1) + dof[j] - 1;
+      const char want = j == index[i] ? 1 : 2;
+      if (role[k] == 1 || (role[k] == 2 && want == 1)) {
+        set_error("chained multipoint constraints (a slave dof that is another constraint's slave or master)");
+        return HECMW33_ERROR_UNSUPPORTED;
+      }
+      role[k] = want;
+    }
+  }
+  return 0;
+}
+
+static int32_t mpc_build(Context& c) {
+  if (c.mpc_built) return 0;
+  if (c.mpc_N != c.N || c.NP != c.N) {
+    set_error("the multipoint constraints were given for another system (or NP > N)");
+    return HECMW33_ERROR_UNSUPPORTED;
+  }
+  const int32_t N = c.N, n = c.mpc_n;
+  const int32_t* index = c.h_mpc_index.data();
+  const int32_t* item = c.h_mpc_item.data();
+  const int32_t* dof = c.h_mpc_dof.data();
+  const double* val = c.h_mpc_val.data();
+  const int64_t n3 = 3 * (int64_t)N;
+  std::vector<int32_t> slave_row(n3, -1), mptr(1, 0), mdof, tcnt(n3 + 1, 0);
+  std::vector<double> mval;
+  std::vector<int32_t> kept;
+  for (int32_t i = 0; i < n; ++i) {
+    bool skip = false;
+    for (int32_t j = index[i]; j < index[i + 1]; ++j) skip = skip || dof[j] > 3;
+    if (skip) continue;
+    const int32_t r = (int32_t)kept.size();
+    kept.push_back(i);
+    slave_row[3 * (int64_t)(item[index[i]] - 1) + dof[index[i]] - 1] = r;
+    for (int32_t j = index[i] + 1; j < index[i + 1]; ++j) {
+      const int32_t k = 3 * (item[j] - 1) + dof[j] - 1;
+      mdof.push_back(k);
+      mval.push_back(val[j]);
+      ++tcnt[k + 1];
+    }
+    mptr.push_back((int32_t)mdof.size());
+  }
+  for (int64_t k = 0; k < n3; ++k) tcnt[k + 1] += tcnt[k];
+  std::vector<int32_t> tslave(std::max<int32_t>(tcnt[n3], 1)), fill(tcnt.begin(), tcnt.end() - 1);
+  std::vector<double> tval(std::max<int32_t>(tcnt[n3], 1));
+  for (size_t r = 0; r < kept.size(); ++r) {
+    const int32_t i = kept[r];
+    const int32_t ks = 3 * (item[index[i]] - 1) + dof[index[i]] - 1;
+    for (int32_t j = index[i] + 1; j < index[i + 1]; ++j) {
+      const int32_t k = 3 * (item[j] - 1) + dof[j] - 1;
+      tslave[fill[k]] = ks;
+      tval[fill[k]++] = val[j];
+    }
+  }
+  const size_t nk = kept.size(), nm = std::max<size_t>(mdof.size(), 1), nt = tslave.size();
+  if (c.mpc_slave.ensure(sizeof(int32_t) * n3) || c.mpc_mptr.ensure(sizeof(int32_t) * (nk + 1)) ||
+      c.mpc_mdof.ensure(sizeof(int32_t) * nm) || c.mpc_mval.ensure(sizeof(double) * nm) ||
+      c.mpc_tptr.ensure(sizeof(int32_t) * (n3 + 1)) || c.mpc_tslave.ensure(sizeof(int32_t) * nt) ||
+      c.mpc_tval.ensure(sizeof(double) * nt) || c.mpc_w1.ensure(sizeof(double) * 3 * (size_t)c.NP) ||
+      c.mpc_w2.ensure(sizeof(double) * 3 * (size_t)c.NP))
+    return HECMW33_ERROR_DEVICE;
+  HIPC(hipMemcpyAsync(c.mpc_slave.p, slave_row.data(), sizeof(int32_t) * n3, hipMemcpyHostToDevice, c.stream));
+  HIPC(hipMemcpyAsync(c.mpc_mptr.p, mptr.data(), sizeof(int32_t) * (nk + 1), hipMemcpyHostToDevice, c.stream));
+  if (!mdof.empty()) {
+    HIPC(hipMemcpyAsync(c.mpc_mdof.p, mdof.data(), sizeof(int32_t) * mdof.size(), hipMemcpyHostToDevice, c.stream));
+    HIPC(hipMemcpyAsync(c.mpc_mval.p, mval.data(), sizeof(double) * mval.size(), hipMemcpyHostToDevice, c.stream));
+  }
+  HIPC(hipMemcpyAsync(c.mpc_tptr.p, tcnt.data(), sizeof(int32_t) * (n3 + 1), hipMemcpyHostToDevice, c.stream));
+  HIPC(hipMemcpyAsync(c.mpc_tslave.p, tslave.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice, c.stream));
+  HIPC(hipMemcpyAsync(c.mpc_tval.p, tval.data(), sizeof(double) * nt, hipMemcpyHostToDevice, c.stream));
+  HIPC(hipStreamSynchronize(c.stream));
+  MpcOp& m = c.mpc;
+  m.n3 = (int32_t)n3;
+  m.slave_row = c.mpc_slave.as<int32_t>();
+  m.mptr = c.mpc_mptr.as<int32_t>();
+  m.mdof = c.mpc_mdof.as<int32_t>();
+  m.mval = c.mpc_mval.as<double>();
+  m.tptr = c.mpc_tptr.as<int32_t>();
+  m.tslave = c.mpc_tslave.as<int32_t>();
+  m.tval = c.mpc_tval.as<double>();
+  m.w1 = c.mpc_w1.as<double>();
+  m.w2 = c.mpc_w2.as<double>();
+  c.mpc_built = true;
+  return 0;
+}
+
+static int32_t set_mpc(Context& c, int32_t N, int32_t n, const int32_t* index, const int32_t* item,
+                       const int32_t* dof, const double* val) {
+  if (n > 0 && !val) return HECMW33_ERROR_ARGS;
+  if (int32_t e = mpc_check(N, n, index, item, dof)) return e;
+  c.mpc_n = n;
+  c.mpc_N = N;
+  c.mpc_built = false;
+  c.mpc_use = false;
+  if (n == 0) return 0;
+  const int32_t nt = index[n];
+  c.h_mpc_index.assign(index, index + n + 1);
+  c.h_mpc_item.assign(item, item + nt);
+  c.h_mpc_dof.assign(dof, dof + nt);
+  c.h_mpc_val.assign(val, val + nt);
+  return 0;
+}
+
 static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_out* out) {
   const double t0 = now_s();
   const int32_t method = IA(iarray, I_METHOD);
@@ -1190,6 +1339,17 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
   const int32_t opt = IA(iarray, I_SOLVER_OPT);
   const bool learned = opt == kOptLearnedFcg;
   const bool mixed = opt == kOptMixedFcg || learned;
+  // MPC-CG: every product is T^T A T while Iarray 83 is set (hecmw_matvec_33,
+  // hecmw_solver_las_33.F90:50-53)
+  c.mpc_use = false;
+  if (c.mpc_n > 0 && IA(iarray, I_FLAG_MPCMATVEC) != 0) {
+    if (mixed) {
+      set_error("the MPC-CG product is provided for the FP64 CG/BiCGSTAB only");
+      return HECMW33_ERROR_UNSUPPORTED;
+    }
+    if (int32_t e = mpc_build(c)) return e;
+    c.mpc_use = true;
+  }
   if (mixed && (method != 1 || iterpremax > 1)) {
     set_error("SOLVER_OPT(1)=1/2 (flexible CG, FP32 operators) needs METHOD=1 and ITERPREMAX<=1");
     return HECMW33_ERROR_UNSUPPORTED;
@@ -1243,7 +1403,7 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
     fcg_init_resid(c.A.view, c.vx.as<double>(), c.vb.as<double>(), c.vr32.as<float>(), &st->rr_ref, st, part,
                    c.stream);
   } else {
-    launch_matresid(c.A.view, c.vx.as<double>(), c.vb.as<double>(), c.vr.as<double>(), c.stream);
+    launch_matresid(c.A.view, c.vx.as<double>(), c.vb.as<double>(), c.vr.as<double>(), c.stream, c.mpc_op());
   }
   launch_dot_store(n3, c.vb.as<double>(), c.vb.as<double>(), &st->bnrm2, st, part, c.stream);
   // work vectors start at zero (WW = 0), including the external entries the
@@ -1455,7 +1615,7 @@ static int32_t rel_resid_resident(Context& c, double* resid) {
   KrylovState* st = c.st.as<KrylovState>();
   double* part = c.part.as<double>();
   launch_dot_store(n3, c.vb.as<double>(), c.vb.as<double>(), &st->ts, st, part, c.stream);
-  launch_matresid(c.A.view, c.vx.as<double>(), c.vb.as<double>(), c.vr.as<double>(), c.stream);
+  launch_matresid(c.A.view, c.vx.as<double>(), c.vb.as<double>(), c.vr.as<double>(), c.stream, c.mpc_op());
   launch_dot_store(n3, c.vr.as<double>(), c.vr.as<double>(), &st->tt, st, part, c.stream);
   HIPC(hipGetLastError());
   HIPC(hipMemcpyAsync(&c.h_st[0], st, sizeof(KrylovState), hipMemcpyDeviceToHost, c.stream));
@@ -1542,6 +1702,8 @@ int32_t hecmw33_solve(const hecmw33_sys* A, const double* B, double* X, int32_t 
     int32_t ia2[100];
     std::memcpy(ia2, iarray, sizeof(ia2));
     IA(ia2, I_METHOD) = method;
+    // MPC-CG (hecmw_solver_Iterative.f90:79-84): T^T A T products for MPC_METHOD 2
+    if (c.mpc_n > 0 && IA(iarray, I_MPC_METHOD) == 2) IA(ia2, I_FLAG_MPCMATVEC) = 1;
     if (int32_t e = krylov_host(c, A, B, X, ia2, rarray, out)) return e;
     IA(iarray, I_FLAG_SYMBFACT) = IA(ia2, I_FLAG_SYMBFACT);
     IA(iarray, I_FLAG_NUMFACT) = IA(ia2, I_FLAG_NUMFACT);
@@ -1557,8 +1719,12 @@ int32_t hecmw33_solve(const hecmw33_sys* A, const double* B, double* X, int32_t 
     if (auto_sigma) rarray[R_SIGMA_DIAG - 1] = -1.0;
     break;
   }
+  // hecmw_rel_resid_L2 still with the MPC product (the reference clears the flag after it)
+  c.mpc_use = c.mpc_n > 0 && IA(iarray, I_MPC_METHOD) == 2 && c.mpc_built;
   double r2 = 0.0;
-  if (int32_t e = rel_resid_resident(c, &r2)) return e;
+  const int32_t er = rel_resid_resident(c, &r2);
+  c.mpc_use = false;
+  if (er) return er;
   out->resid_true = r2;
   if (r2 < RA(rarray, R_RESID)) IA(iarray, I_FLAG_CONVERGED) = 1;
   return error;
@@ -1570,6 +1736,29 @@ int32_t hecmw33_matvec(const hecmw33_sys* A, const double* x, double* y) {
   if (int32_t e = prepare(c, A, false, nullptr)) return e;
   HIPC(hipMemcpyAsync(c.vx.p, x, sizeof(double) * 3 * (size_t)A->NP, hipMemcpyHostToDevice, c.stream));
   launch_spmv(c.A.view, c.vx.as<double>(), c.vq.as<double>(), c.stream);
+  HIPC(hipGetLastError());
+  HIPC(hipMemcpyAsync(y, c.vq.p, sizeof(double) * 3 * (size_t)A->N, hipMemcpyDeviceToHost, c.stream));
+  HIPC(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+int32_t hecmw33_set_mpc(int32_t N, int32_t n_mpc, const int32_t* mpc_index, const int32_t* mpc_item,
+                        const int32_t* mpc_dof, const double* mpc_val) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return set_mpc(*default_ctx(), N, n_mpc, mpc_index, mpc_item, mpc_dof, mpc_val);
+}
+
+int32_t hecmw33_matvec_mpc(const hecmw33_sys* A, const double* x, double* y) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  Context& c = *default_ctx();
+  if (int32_t e = prepare(c, A, false, nullptr)) return e;
+  if (c.mpc_n == 0) {
+    set_error("no multipoint constraints set (hecmw33_set_mpc)");
+    return HECMW33_ERROR_ARGS;
+  }
+  if (int32_t e = mpc_build(c)) return e;
+  HIPC(hipMemcpyAsync(c.vx.p, x, sizeof(double) * 3 * (size_t)A->NP, hipMemcpyHostToDevice, c.stream));
+  launch_spmv_mpc(c.A.view, c.mpc, c.vx.as<double>(), c.vq.as<double>(), c.stream);
   HIPC(hipGetLastError());
   HIPC(hipMemcpyAsync(y, c.vq.p, sizeof(double) * 3 * (size_t)A->N, hipMemcpyDeviceToHost, c.stream));
   HIPC(hipStreamSynchronize(c.stream));
@@ -1603,8 +1792,10 @@ int32_t hecmw33_precond_apply(const hecmw33_sys* A, const int32_t iarray[100], c
 }
 
 int32_t hecmw33_rel_resid_L2(const hecmw33_sys* A, const double* B, const double* X, double* resid) {
+  // no MPC flag here: the plain product, as hecmw_rel_resid_L2 with Iarray 83 = 0
   std::lock_guard<std::mutex> lk(g_mu);
   Context& c = *default_ctx();
+  c.mpc_use = false;
   if (int32_t e = prepare(c, A, false, nullptr)) return e;
   const size_t vb = sizeof(double) * 3 * (size_t)A->NP;
   HIPC(hipMemcpyAsync(c.vb.p, B, vb, hipMemcpyHostToDevice, c.stream));
@@ -2011,6 +2202,12 @@ static int32_t set_coords(Context& c, int32_t NP, const double* xyz) {
   c.h_coord.assign(xyz, xyz + 3 * (size_t)NP);
   ++c.coord_gen;
   return 0;
+}
+
+int32_t hecmw33_ctx_set_mpc(hecmw33_ctx* ctx, int32_t N, int32_t n_mpc, const int32_t* mpc_index,
+                            const int32_t* mpc_item, const int32_t* mpc_dof, const double* mpc_val) {
+  if (!ctx) return HECMW33_ERROR_ARGS;
+  return set_mpc(ctx->c, N, n_mpc, mpc_index, mpc_item, mpc_dof, mpc_val);
 }
 
 int32_t hecmw33_ctx_set_coords(hecmw33_ctx* ctx, int32_t NP, const double* xyz) {

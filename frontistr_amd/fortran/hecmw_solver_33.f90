@@ -13,9 +13,9 @@
 ! 1001/2001) and the ITERLOG/TIMELOG lines.
 !
 ! hecmw33_available() tells a caller whether the GPU path can take a system
-! (library loaded, a device visible, METHOD 1/2, PRECOND 1/2/3, no MPC, one
-! domain; SCALING is applied on the device copies); otherwise the caller keeps
-! the reference path.
+! (library loaded, a device visible, METHOD 1/2, PRECOND 1/2/3, one domain,
+! multipoint constraints the MPC-CG product supports; SCALING is applied on
+! the device copies); otherwise the caller keeps the reference path.
 !-------------------------------------------------------------------------------
 module hecmw_solver_33
   use iso_c_binding
@@ -51,6 +51,13 @@ module hecmw_solver_33
       integer(c_int32_t) :: n
     end function c_hecmw33_device_count
 
+    function c_hecmw33_set_mpc(N, n_mpc, idx, item, dof, val) bind(c, name='hecmw33_set_mpc') result(rc)
+      import :: c_ptr, c_int32_t
+      integer(c_int32_t), value :: N, n_mpc
+      type(c_ptr), value :: idx, item, dof, val
+      integer(c_int32_t) :: rc
+    end function c_hecmw33_set_mpc
+
     function c_hecmw33_set_coords(NP, xyz) bind(c, name='hecmw33_set_coords') result(rc)
       import :: c_ptr, c_int32_t
       integer(c_int32_t), value :: NP
@@ -76,9 +83,26 @@ contains
     precond = hecMAT%Iarray(3)
     hecmw33_available = hecMAT%NDOF == 3 .and. (method == 1 .or. method == 2) &
       .and. (precond == 1 .or. precond == 2 .or. precond == 3 .or. hecMAT%Iarray(5) <= 0) &
-      .and. hecMESH%n_neighbor_pe == 0 .and. hecMESH%mpc%n_mpc == 0
+      .and. hecMESH%n_neighbor_pe == 0
     if (hecmw33_available) hecmw33_available = c_hecmw33_device_count() > 0
+    ! multipoint constraints: the library takes them for the MPC-CG product
+    ! unless they are chained or touch external nodes
+    if (hecmw33_available) hecmw33_available = pass_mpc(hecMESH, hecMAT) == 0
   end function hecmw33_available
+
+  ! hecMESH%mpc to the library (n_mpc = 0 clears what an earlier call left)
+  integer(c_int32_t) function pass_mpc(hecMESH, hecMAT)
+    type(hecmwST_local_mesh), target :: hecMESH
+    type(hecmwST_matrix) :: hecMAT
+    if (hecMESH%mpc%n_mpc > 0) then
+      pass_mpc = c_hecmw33_set_mpc(int(hecMAT%N, c_int32_t), int(hecMESH%mpc%n_mpc, c_int32_t), &
+        c_loc(hecMESH%mpc%mpc_index(0)), c_loc(hecMESH%mpc%mpc_item(1)), c_loc(hecMESH%mpc%mpc_dof(1)), &
+        c_loc(hecMESH%mpc%mpc_val(1)))
+    else
+      pass_mpc = c_hecmw33_set_mpc(int(hecMAT%N, c_int32_t), 0_c_int32_t, c_null_ptr, c_null_ptr, &
+        c_null_ptr, c_null_ptr)
+    endif
+  end function pass_mpc
 
   subroutine hecmw_solve_33(hecMESH, hecMAT)
     use m_hecmw_solve_error
@@ -112,6 +136,11 @@ contains
     A%indexU = c_loc(hecMAT%indexU(0))
     ia(:) = hecMAT%Iarray(:)
     ra(:) = hecMAT%Rarray(:)
+    rc = pass_mpc(hecMESH, hecMAT)
+    if (rc < 0) then
+      write(*,'(a,i0)') '#### hecmw_solve_33: multipoint constraints not supported by the GPU library ', rc
+      call hecmw_abort(hecmw_comm_get_comm())
+    endif
 
     ! the two-level correction of the flexible CG (SOLVER_OPT(3) = Iarray(43) > 0)
     ! builds its aggregates from the node coordinates (Fortran does not short-circuit

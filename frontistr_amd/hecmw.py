@@ -106,6 +106,9 @@ def lib():
     L.hecmw33_ctx_set_net.argtypes = [P, P]
     L.hecmw33_set_net.argtypes = [P]
     L.hecmw33_set_coords.argtypes = [I, P]
+    L.hecmw33_set_mpc.argtypes = [I, I, P, P, P, P]
+    L.hecmw33_ctx_set_mpc.argtypes = [P, I, I, P, P, P, P]
+    L.hecmw33_matvec_mpc.argtypes = [C.POINTER(_Sys), P, P]
     L.hecmw33_ctx_set_coords.argtypes = [P, I, P]
     L.hecmw33_ctx_coarse_get.argtypes = [P, P, P, P, P, P, P, P]
     L.hecmw33_last_error.restype = C.c_char_p
@@ -118,7 +121,8 @@ def lib():
               "hecmw33_batch_spmm", "hecmw33_batch_info", "hecmw33_solve_batch", "hecmw33_net_weight_count",
               "hecmw33_net_get_weights", "hecmw33_net_set_weights", "hecmw33_net_eval", "hecmw33_net_add_sample",
               "hecmw33_net_clear_samples", "hecmw33_net_train", "hecmw33_net_score", "hecmw33_ctx_set_net",
-              "hecmw33_set_net", "hecmw33_set_coords", "hecmw33_ctx_set_coords", "hecmw33_ctx_coarse_get"):
+              "hecmw33_set_net", "hecmw33_set_coords", "hecmw33_ctx_set_coords", "hecmw33_ctx_coarse_get", "hecmw33_set_mpc",
+              "hecmw33_ctx_set_mpc", "hecmw33_matvec_mpc"):
         getattr(L, f).restype = C.c_int32
     _lib = L
     return L
@@ -290,6 +294,39 @@ def hecmw_matvec(mat, x):
     y = np.zeros(3 * mat.N)
     s = mat._sys()
     _check(lib().hecmw33_matvec(C.byref(s), _ptr(x), _ptr(y)))
+    return y
+
+
+def _mpc_arrays(mpc):
+    idx, item, dof, val = mpc
+    return (np.ascontiguousarray(idx, np.int32), np.ascontiguousarray(item, np.int32),
+            np.ascontiguousarray(dof, np.int32), np.ascontiguousarray(val, np.float64))
+
+
+_mpc_keep = None
+
+
+def set_mpc(N, mpc):
+    """hecMESH%mpc for hecmw_solve_iterative / hecmw_solve_krylov's MPC-CG
+    (T^T A T products when MPC_METHOD, Iarray 13, is 2): mpc = (index[0:n+1],
+    item, dof, val) with 1-based items and dofs, the first term of each
+    constraint its slave; None clears."""
+    global _mpc_keep
+    if mpc is None:
+        _mpc_keep = None
+        _check(lib().hecmw33_set_mpc(N, 0, None, None, None, None))
+        return
+    a = _mpc_arrays(mpc)
+    _check(lib().hecmw33_set_mpc(N, len(a[0]) - 1, *[_ptr(x) for x in a]))
+    _mpc_keep = a
+
+
+def hecmw_matvec_mpc(mat, x):
+    """y = T^T A T x (hecmw_TtmatTvec_33) with the constraints of set_mpc."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros(3 * mat.N)
+    s = mat._sys()
+    _check(lib().hecmw33_matvec_mpc(C.byref(s), _ptr(x), _ptr(y)))
     return y
 
 
@@ -485,6 +522,15 @@ class DeviceContext:
                                             _ptr(pb), _ptr(E)))
         return {"n_agg": na, "n_pair": npair, "agg": agg, "dv": dv.reshape(N, 3), "pair_a": pa, "pair_b": pb,
                 "Eblk": E.reshape(npair, 6, 6)}
+
+    def set_mpc(self, N, mpc):
+        """Multipoint constraints (as frontistr_amd.set_mpc) for this context's solves."""
+        if mpc is None:
+            _check(lib().hecmw33_ctx_set_mpc(self._h, N, 0, None, None, None, None))
+            self._mpc = None
+            return
+        self._mpc = _mpc_arrays(mpc)
+        _check(lib().hecmw33_ctx_set_mpc(self._h, N, len(self._mpc[0]) - 1, *[_ptr(x) for x in self._mpc]))
 
     def set_coords(self, xyz):
         """Node coordinates (3*NP, xyz interleaved) for the two-level correction."""

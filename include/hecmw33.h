@@ -119,6 +119,22 @@ int32_t hecmw33_set_coords(int32_t NP, const double* xyz);
  * Host pointers, x has 3*NP entries, y receives 3*N. */
 int32_t hecmw33_matvec(const hecmw33_sys* A, const double* x, double* y);
 
+/* Multipoint constraints of the MPC-CG method (hecMESH%mpc, hecmw1/src/common/hecmw_util_f.F90:97-104):
+ * n_mpc constraints over N internal nodes, mpc_index[0..n_mpc] cumulative (0-based), and per term
+ * mpc_item (1-based node), mpc_dof (1..3) and mpc_val; the first term of a constraint is its slave.
+ * Copied; n_mpc = 0 clears.  hecmw33_solve then multiplies by T^T A T when MPC_METHOD (iarray[12])
+ * is 2, as hecmw_solve_iterative does (hecmw_solver_Iterative.f90:79-84,163-166), and
+ * hecmw33_solve_krylov whenever iarray[82] (the MPC matvec flag) is set.  Constraints with a dof
+ * above 3 are skipped as the reference skips them; chained constraints (a slave that is another
+ * constraint's slave or master) and constraints on external nodes return HECMW33_ERROR_UNSUPPORTED. */
+int32_t hecmw33_set_mpc(int32_t N, int32_t n_mpc, const int32_t* mpc_index, const int32_t* mpc_item,
+                        const int32_t* mpc_dof, const double* mpc_val);
+
+/* y = T^T A T x with the constraints of hecmw33_set_mpc (hecmw_TtmatTvec_33,
+ * hecmw1/src/solver/las/hecmw_solver_las_33.F90:572-586; hecmw_Tvec_33 / hecmw_Ttvec_33 :442-565).
+ * Host pointers, x has 3*NP entries, y receives 3*N. */
+int32_t hecmw33_matvec_mpc(const hecmw33_sys* A, const double* x, double* y);
+
 /* z = M^-1 r with the preconditioner selected by iarray/rarray
  * (hecmw_precond_setup + hecmw_precond_apply, hecmw1/src/solver/precond/hecmw_precond.f90:30-172).
  * A new setup is always made.  Host pointers, 3*NP entries. */
@@ -169,6 +185,9 @@ int32_t hecmw33_ctx_history(hecmw33_ctx* ctx, double* hist, int32_t cap);
  * correction (0: none, -1: E was not positive definite and the solve ran without it), [12] coupled
  * aggregate pairs (blocks of E). */
 int32_t hecmw33_ctx_info(hecmw33_ctx* ctx, int64_t* info, int32_t n);
+/* the multipoint constraints of a device context (as hecmw33_set_mpc) */
+int32_t hecmw33_ctx_set_mpc(hecmw33_ctx* ctx, int32_t N, int32_t n_mpc, const int32_t* mpc_index,
+                            const int32_t* mpc_item, const int32_t* mpc_dof, const double* mpc_val);
 /* the two-level correction's node coordinates for a device context (as hecmw33_set_coords) */
 int32_t hecmw33_ctx_set_coords(hecmw33_ctx* ctx, int32_t NP, const double* xyz);
 /* Inspection of the last two-level set-up: *n_agg aggregates (0: none built) and *n_pair coupled

@@ -50,6 +50,10 @@ def lib():
         L.ref_precond.restype = None
         L.ref_rel_resid.argtypes = [I, I, I, I] + [P] * 9 + [I]
         L.ref_rel_resid.restype = C.c_double
+        L.ref_matvec_mpc.argtypes = [I, I, I, I] + [P] * 9 + [I]
+        L.ref_matvec_mpc.restype = None
+        L.ref_set_mpc.argtypes = [I, P, P, P, P]
+        L.ref_set_mpc.restype = None
         L.ref_mat_dump.argtypes = [I, I, I, I] + [P] * 9 + [I]
         L.ref_mat_dump.restype = None
         L.ref_flush.argtypes = []
@@ -196,3 +200,29 @@ def mat_dump(m, kind, directory):
     if len(new) != 1:
         raise RuntimeError("reference dump wrote %s" % new)
     return os.path.join(directory, new[0])
+
+
+_mpc_keep = None
+
+
+def set_mpc(mpc):
+    """Bind multipoint constraints for the reference calls that follow:
+    mpc = (index[0:n+1], item, dof, val) as in hecMESH%mpc (1-based items and
+    dofs), or None to clear."""
+    global _mpc_keep
+    if mpc is None:
+        _mpc_keep = None
+        lib().ref_set_mpc(0, None, None, None, None)
+        return
+    idx, item, dof, val = (np.ascontiguousarray(mpc[0], np.int32), np.ascontiguousarray(mpc[1], np.int32),
+                           np.ascontiguousarray(mpc[2], np.int32), np.ascontiguousarray(mpc[3], np.float64))
+    _mpc_keep = (idx, item, dof, val)
+    lib().ref_set_mpc(len(idx) - 1, _p(idx), _p(item), _p(dof), _p(val))
+
+
+def matvec_mpc(m, x, nthreads=NTHREADS):
+    """y = T^T A T x (hecmw_TtmatTvec_33) with the constraints of set_mpc."""
+    xin = np.array(x, dtype=np.float64)
+    y = np.zeros(3 * m.NP)
+    lib().ref_matvec_mpc(*_mat_args(m), _p(xin), _p(y), nthreads)
+    return y

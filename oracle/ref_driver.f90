@@ -19,6 +19,8 @@
 !   ref_precond       -> hecmw_precond_setup + hecmw_precond_apply
 !                        (hecmw_precond.f90:30,86)
 !   ref_rel_resid     -> hecmw_rel_resid_L2 (hecmw_solver_las.f90:124)
+!   ref_matvec_mpc    -> hecmw_matvec with the MPC flag (hecmw_TtmatTvec_33)
+!   ref_set_mpc       -> binds hecMESH%mpc for the calls that follow
 !   ref_mat_dump      -> hecmw_mat_dump + hecmw_mat_dump_solution
 !                        (hecmw_matrix_dump.f90:31,345)
 !-------------------------------------------------------------------------------
@@ -30,6 +32,8 @@ module hecmw_ref_driver
 
   type(hecmwST_local_mesh), save, target :: mesh
   type(hecmwST_matrix), save, target :: mat
+  ! multipoint constraints bound by ref_set_mpc (0: none), kept for every later call
+  integer(kind=kint), save :: nmpc = 0
 
 contains
 
@@ -64,12 +68,35 @@ contains
 
     mesh%nn_internal = n
     mesh%n_node = np
+    mesh%n_dof = 3
     mesh%n_neighbor_pe = 0
     mesh%my_rank = 0
     mesh%PETOT = 1
     mesh%zero = 0
-    mesh%mpc%n_mpc = 0
+    mesh%mpc%n_mpc = nmpc
   end subroutine bind_matrix
+
+  !> hecMESH%mpc for the following calls (hecmw_util_f.F90:97-104): n_mpc
+  !> constraints, mpc_index(0:n_mpc), mpc_item/mpc_dof/mpc_val per term (the
+  !> caller's memory; n_mpc = 0 clears).  With MPC_METHOD = 2 (Iarray 13) the
+  !> reference's solvers then multiply by T^T A T (hecmw_TtmatTvec_33).
+  subroutine ref_set_mpc(n_mpc, index, item, dof, val) bind(c, name='ref_set_mpc')
+    integer(c_int), value :: n_mpc
+    type(c_ptr), value :: index, item, dof, val
+    integer(kind=kint), pointer :: tmp(:)
+    integer(kind=kint) :: nt
+    nmpc = n_mpc
+    if (n_mpc <= 0) then
+      nmpc = 0
+      return
+    endif
+    call c_f_pointer(index, tmp, [n_mpc + 1])
+    mesh%mpc%mpc_index(0:n_mpc) => tmp
+    nt = tmp(n_mpc + 1)
+    call c_f_pointer(item, mesh%mpc%mpc_item, [max(nt, 1)])
+    call c_f_pointer(dof, mesh%mpc%mpc_dof, [max(nt, 1)])
+    call c_f_pointer(val, mesh%mpc%mpc_val, [max(nt, 1)])
+  end subroutine ref_set_mpc
 
   subroutine set_threads(nthreads)
     !$ use omp_lib
@@ -185,6 +212,26 @@ contains
     call c_f_pointer(y, yv, [3*np])
     call hecmw_matvec(mesh, mat, xv, yv)
   end subroutine ref_matvec
+
+  !> y = T^T A T x through hecmw_matvec with the MPC flag set (Iarray 83,
+  !> hecmw_TtmatTvec_33, hecmw_solver_las_33.F90:572-586) and the constraints of ref_set_mpc.
+  subroutine ref_matvec_mpc(n, np, npl, npu, d, al, au, indexl, iteml, indexu, itemu, &
+      xin, y, nthreads) bind(c, name='ref_matvec_mpc')
+    use hecmw_solver_las
+    use hecmw_matrix_misc
+    integer(c_int), value :: n, np, npl, npu, nthreads
+    type(c_ptr), value :: d, al, au, indexl, iteml, indexu, itemu, xin, y
+    real(kind=kreal), pointer :: xv(:), yv(:)
+    integer(c_int) :: ia(100)
+    real(c_double) :: ra(100)
+    ia = 0; ra = 0.d0
+    call set_threads(nthreads)
+    call bind_matrix(n, np, npl, npu, d, al, au, indexl, iteml, indexu, itemu, xin, y, ia, ra)
+    call hecmw_mat_set_flag_mpcmatvec(mat, 1)
+    call c_f_pointer(xin, xv, [3*np])
+    call c_f_pointer(y, yv, [3*np])
+    call hecmw_matvec(mesh, mat, xv, yv)
+  end subroutine ref_matvec_mpc
 
   !> z = M^-1 r with the preconditioner selected by iarray (PRECOND, ITERPREMAX,
   !> NCOLOR_IN, SIGMA_DIAG); a fresh setup is forced.

@@ -4,7 +4,10 @@
 ! sources) and once by hecmw_solve_33 (GPU, frontistr_amd/fortran), both
 ! called from Fortran exactly as fistr1 would call them.
 !
-!   dropin_test <n> <method> <precond> [scaling] [fcg]    (OMP_NUM_THREADS >= 2)
+!   dropin_test <n> <method> <precond> [scaling] [fcg] [mpc]    (OMP_NUM_THREADS >= 2)
+! mpc = 1: the z displacement of every top-face node (but one corner) is tied
+! to the top-face centre node's (hecMESH%mpc, MPC_METHOD = 2), B replaced by
+! T^T B as fistr1 hands it over; both sides then solve with T^T A T products
 ! fcg = 1: the GPU side runs the flexible CG with FP32 operators and the
 ! two-level correction (Iarray 41 = 1, 43 = 1), with hecMESH%node set
 ! prints one line:  DROPIN n method precond conv_ref conv_gpu div_ref div_gpu relerr
@@ -39,7 +42,7 @@ program dropin_test
   type(hecmwST_matrix), target :: mref, mgpu
   type(hecmwST_local_mesh) :: mesh
   integer(c_int32_t) :: n, N_, NPL, NPU, rc
-  integer :: method, precond, scaling, fcg, i, j, k, nn
+  integer :: method, precond, scaling, fcg, mpc, i, j, k, nn, c, s, t, ks, km
   character(len=32) :: arg
   real(c_double) :: p(8)
   real(kind=kreal) :: relerr
@@ -54,6 +57,10 @@ program dropin_test
   endif
   if (command_argument_count() >= 5) then
     call get_command_argument(5, arg); read(arg, *) fcg
+  endif
+  mpc = 0
+  if (command_argument_count() >= 6) then
+    call get_command_argument(6, arg); read(arg, *) mpc
   endif
 
   rc = gen_count(n, N_, NPL, NPU)
@@ -75,6 +82,7 @@ program dropin_test
 
   mesh%nn_internal = N_
   mesh%n_node = N_
+  mesh%n_dof = 3
   mesh%n_neighbor_pe = 0
   mesh%my_rank = 0
   mesh%PETOT = 1
@@ -96,6 +104,37 @@ program dropin_test
     enddo
     mgpu%Iarray(41) = 1
     mgpu%Iarray(43) = 1
+  endif
+  mesh%mpc%n_mpc = 0
+  if (mpc == 1) then
+    nn = n + 1
+    c = n / 2
+    t = 0
+    allocate(mesh%mpc%mpc_index(0:nn*nn), mesh%mpc%mpc_item(2*nn*nn), mesh%mpc%mpc_dof(2*nn*nn), &
+             mesh%mpc%mpc_val(2*nn*nn))
+    mesh%mpc%mpc_index(0) = 0
+    km = 1 + c + nn * (c + nn * n)
+    do j = 0, n
+      do i = 0, n
+        s = 1 + i + nn * (j + nn * n)
+        if (s == km .or. (i == 0 .and. j == 0)) cycle
+        mesh%mpc%mpc_item(2*t+1) = s
+        mesh%mpc%mpc_item(2*t+2) = km
+        mesh%mpc%mpc_dof(2*t+1:2*t+2) = 3
+        mesh%mpc%mpc_val(2*t+1) = 1.d0
+        mesh%mpc%mpc_val(2*t+2) = -1.d0
+        t = t + 1
+        mesh%mpc%mpc_index(t) = 2 * t
+        ! B <- T^T B: the slave's load moves to the master
+        ks = 3 * (s - 1) + 3
+        mref%B(3 * (km - 1) + 3) = mref%B(3 * (km - 1) + 3) + mref%B(ks)
+        mref%B(ks) = 0.d0
+      enddo
+    enddo
+    mesh%mpc%n_mpc = t
+    mgpu%B = mref%B
+    call hecmw_mat_set_mpc_method(mref, 2)
+    call hecmw_mat_set_mpc_method(mgpu, 2)
   endif
 
   call hecmw_solve_iterative(mesh, mref)
