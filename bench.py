@@ -69,8 +69,8 @@ def main():
                     help="run one workload section alone (for profiling); prints that section's JSON")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (--gpus %d requested); multi-GPU sharding is "
-              "described in DESIGN.md and not implemented" % args.gpus, file=sys.stderr)
+        print("bench.py: one GPU only (--gpus %d requested); sharding over GPUs (one rank per GPU, a halo "
+              "exchange per product over RCCL) is not implemented" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     import torch
