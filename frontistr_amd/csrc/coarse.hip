@@ -224,7 +224,217 @@ __global__ __launch_bounds__(kCoarseBlock) void k_coarse_prolong(int32_t n, cons
   }
 }
 
+// ---- symmetric two-level step: z1 = Z E^-1 Z^T r, z2 = z1 + M^-1 (r - A z1),
+//      z = z2 + Z E^-1 Z^T (r - A z2), with A Z kept per node and neighbour
+//      aggregate (W), so A z1 and the A Z part of A z need no SpMV ----------------
+// W[e] (3x6, FP32) for the e-th aggregate b touched by row i (node-major,
+// ncnt/nagg): sum of A_ij Z_j over the row's blocks with agg(j) = b.
+__global__ __launch_bounds__(kCoarseBlock) void k_coarse_W(Sell<double> A, int32_t n, const int32_t* __restrict__ agg,
+                                                            const int32_t* __restrict__ ncnt,
+                                                            const int32_t* __restrict__ nagg,
+                                                            const float* __restrict__ dv, float* __restrict__ W) {
+  const int64_t i = (int64_t)blockIdx.x * kCoarseBlock + threadIdx.x;
+  if (i >= n) return;
+  const int sl = (int)(i / A.rows), l = (int)(i % A.rows);
+  const int64_t base = (int64_t)A.ptr[sl] * A.rows;
+  const int len = A.len[i];
+  for (int e = ncnt[i]; e < ncnt[i + 1]; ++e) {
+    const int b = nagg[e];
+    double m[18];
+#pragma unroll
+    for (int k = 0; k < 18; ++k) m[k] = 0.0;
+    for (int k = 0; k < len; ++k) {
+      const int j = A.col[base + (int64_t)k * A.rows + l];
+      if (agg[j] != b) continue;
+      const double* v = A.val + (base + (int64_t)k * A.rows) * 9 + l;
+      const double jx = dv[3 * (int64_t)j], jy = dv[3 * (int64_t)j + 1], jz = dv[3 * (int64_t)j + 2];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const double a0 = v[(3 * r) * A.rows], a1 = v[(3 * r + 1) * A.rows], a2 = v[(3 * r + 2) * A.rows];
+        m[6 * r] += a0;
+        m[6 * r + 1] += a1;
+        m[6 * r + 2] += a2;
+        m[6 * r + 3] += -a1 * jz + a2 * jy;
+        m[6 * r + 4] += a0 * jz - a2 * jx;
+        m[6 * r + 5] += -a0 * jy + a1 * jx;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 18; ++k) W[18 * (int64_t)e + k] = (float)m[k];
+  }
+}
+
+__device__ __forceinline__ void zy(const float* __restrict__ dv, const float* __restrict__ ya, int64_t i, float (&o)[3]) {
+  const float dx = dv[3 * i], dy = dv[3 * i + 1], dz = dv[3 * i + 2];
+  o[0] = ya[0] + (ya[4] * dz - ya[5] * dy);
+  o[1] = ya[1] + (ya[5] * dx - ya[3] * dz);
+  o[2] = ya[2] + (ya[3] * dy - ya[4] * dx);
+}
+
+__device__ __forceinline__ void wy(const int32_t* __restrict__ ncnt, const int32_t* __restrict__ nagg,
+                                   const float* __restrict__ W, const float* __restrict__ y, int64_t i,
+                                   float (&o)[3]) {
+  o[0] = o[1] = o[2] = 0.f;
+  for (int e = ncnt[i]; e < ncnt[i + 1]; ++e) {
+    const float* w = W + 18 * (int64_t)e;
+    const float* yb = y + 6 * nagg[e];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int q = 0; q < 6; ++q) o[r] += w[6 * r + q] * yb[q];
+  }
+}
+
+// z = Z y, r1 = r - (A Z) y
+__global__ __launch_bounds__(kCoarseBlock) void k_coarse_pre(int32_t n, const int32_t* __restrict__ agg,
+                                                              const float* __restrict__ dv,
+                                                              const int32_t* __restrict__ ncnt,
+                                                              const int32_t* __restrict__ nagg,
+                                                              const float* __restrict__ W, const float* __restrict__ y,
+                                                              const float* __restrict__ r, float* __restrict__ z,
+                                                              float* __restrict__ r1, const KrylovState* st) {
+  if (st->done) return;
+  for (int64_t i = (int64_t)blockIdx.x * kCoarseBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kCoarseBlock) {
+    float zi[3], ai[3];
+    zy(dv, y + 6 * agg[i], i, zi);
+    wy(ncnt, nagg, W, y, i, ai);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      z[3 * i + d] = zi[d];
+      r1[3 * i + d] = r[3 * i + d] - ai[d];
+    }
+  }
+}
+
+// z = z + t (the SSOR output added to the coarse part)
+__global__ __launch_bounds__(kCoarseBlock) void k_coarse_add(int64_t n3, const float* __restrict__ t,
+                                                              float* __restrict__ z, const KrylovState* st) {
+  if (st->done) return;
+  for (int64_t i = (int64_t)blockIdx.x * kCoarseBlock + threadIdx.x; i < n3; i += (int64_t)gridDim.x * kCoarseBlock)
+    z[i] = z[i] + t[i];
+}
+
+// c = Z^T (r - w) per aggregate
+__global__ __launch_bounds__(kCoarseBlock) void k_coarse_restrict2(const int32_t* __restrict__ aptr,
+                                                                    const int32_t* __restrict__ anodes,
+                                                                    const float* __restrict__ dv,
+                                                                    const float* __restrict__ r,
+                                                                    const float* __restrict__ w,
+                                                                    float* __restrict__ c, const KrylovState* st) {
+  if (st->done) return;
+  const int a = blockIdx.x;
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const int t1 = aptr[a + 1];
+  for (int t = aptr[a] + threadIdx.x; t < t1; t += kCoarseBlock) {
+    const int64_t i = anodes[t];
+    const double rx = (double)r[3 * i] - (double)w[3 * i], ry = (double)r[3 * i + 1] - (double)w[3 * i + 1],
+                 rz = (double)r[3 * i + 2] - (double)w[3 * i + 2];
+    const double dx = dv[3 * i], dy = dv[3 * i + 1], dz = dv[3 * i + 2];
+    acc[0] += rx;
+    acc[1] += ry;
+    acc[2] += rz;
+    acc[3] += -dz * ry + dy * rz;
+    acc[4] += dz * rx - dx * rz;
+    acc[5] += -dy * rx + dx * ry;
+  }
+  __shared__ double tot[6];
+  block_reduce<6>(acc, tot);
+  if (threadIdx.x < 6) c[6 * a + threadIdx.x] = (float)tot[threadIdx.x];
+}
+
+// z = z + Z y, az = w + (A Z) y, partials r.z and q_prev.z (the flexible CG's beta)
+__global__ __launch_bounds__(kCoarseBlock) void k_coarse_post(int32_t n, const int32_t* __restrict__ agg,
+                                                               const float* __restrict__ dv,
+                                                               const int32_t* __restrict__ ncnt,
+                                                               const int32_t* __restrict__ nagg,
+                                                               const float* __restrict__ W,
+                                                               const float* __restrict__ y, float* __restrict__ z,
+                                                               const float* __restrict__ w, float* __restrict__ az,
+                                                               const float* __restrict__ r,
+                                                               const float* __restrict__ q, const KrylovState* st,
+                                                               double* part) {
+  if (st->done) return;
+  double d[2] = {0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kCoarseBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kCoarseBlock) {
+    float zi[3], ai[3];
+    zy(dv, y + 6 * agg[i], i, zi);
+    wy(ncnt, nagg, W, y, i, ai);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float zn = z[3 * i + k] + zi[k];
+      z[3 * i + k] = zn;
+      az[3 * i + k] = w[3 * i + k] + ai[k];
+      d[0] += (double)zn * (double)r[3 * i + k];
+      d[1] += (double)zn * (double)q[3 * i + k];
+    }
+  }
+  __shared__ double tot[2];
+  block_reduce<2>(d, tot);
+  if (threadIdx.x < 2) part[threadIdx.x * kMaxPartials + blockIdx.x] = tot[threadIdx.x];
+}
+
+// p = z + beta p, q = az + beta q (first iteration: z, az), partials p.q and r.p
+__global__ __launch_bounds__(kCoarseBlock) void k_fcg_pq(int64_t n3, const float* __restrict__ z,
+                                                          const float* __restrict__ az, float* __restrict__ p,
+                                                          float* __restrict__ q, const float* __restrict__ r,
+                                                          const KrylovState* st, double* part) {
+  if (st->done) return;
+  const bool first = st->iter == 1;
+  const double beta = st->beta;
+  double d[2] = {0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kCoarseBlock + threadIdx.x; i < n3; i += (int64_t)gridDim.x * kCoarseBlock) {
+    const float pn = first ? z[i] : (float)((double)z[i] + beta * (double)p[i]);
+    const float qn = first ? az[i] : (float)((double)az[i] + beta * (double)q[i]);
+    p[i] = pn;
+    q[i] = qn;
+    d[0] += (double)pn * (double)qn;
+    d[1] += (double)r[i] * (double)pn;
+  }
+  __shared__ double tot[2];
+  block_reduce<2>(d, tot);
+  if (threadIdx.x < 2) part[threadIdx.x * kMaxPartials + blockIdx.x] = tot[threadIdx.x];
+}
+
 }  // namespace
+
+void coarse_build_W(const Sell<double>& A, const CoarseOp& op, int32_t n, const int32_t* ncnt, const int32_t* nagg,
+                    float* W, hipStream_t s) {
+  if (n > 0) k_coarse_W<<<(n + kCoarseBlock - 1) / kCoarseBlock, kCoarseBlock, 0, s>>>(A, n, op.agg, ncnt, nagg, op.dv, W);
+}
+
+static int coarse_vec_grid(int64_t n) {
+  return (int)std::max<int64_t>(1, std::min<int64_t>((n + kCoarseBlock - 1) / kCoarseBlock, 4096));
+}
+
+void coarse_sym_pre(const CoarseOp& op, int32_t n, const int32_t* ncnt, const int32_t* nagg, const float* W,
+                    const float* r, float* z, float* r1, const KrylovState* st, hipStream_t s) {
+  k_coarse_restrict<<<op.n_agg, kCoarseBlock, 0, s>>>(op.aptr, op.anodes, op.dv, r, op.cv, st);
+  k_coarse_gemv<<<(op.nc + kCoarseWaves - 1) / kCoarseWaves, kCoarseBlock, 0, s>>>(op.nc, op.ldw, op.W, op.cv, op.yv,
+                                                                                   st);
+  k_coarse_pre<<<coarse_vec_grid(n), kCoarseBlock, 0, s>>>(n, op.agg, op.dv, ncnt, nagg, W, op.yv, r, z, r1, st);
+}
+
+void coarse_sym_add(int64_t n3, const float* t, float* z, const KrylovState* st, hipStream_t s) {
+  k_coarse_add<<<coarse_vec_grid(n3), kCoarseBlock, 0, s>>>(n3, t, z, st);
+}
+
+int coarse_sym_post(const CoarseOp& op, int32_t n, const int32_t* ncnt, const int32_t* nagg, const float* W,
+                    const float* r, const float* w, float* z, float* az, const float* q, const KrylovState* st,
+                    double* part, hipStream_t s) {
+  k_coarse_restrict2<<<op.n_agg, kCoarseBlock, 0, s>>>(op.aptr, op.anodes, op.dv, r, w, op.cv, st);
+  k_coarse_gemv<<<(op.nc + kCoarseWaves - 1) / kCoarseWaves, kCoarseBlock, 0, s>>>(op.nc, op.ldw, op.W, op.cv, op.yv,
+                                                                                   st);
+  const int g = coarse_vec_grid(n);
+  k_coarse_post<<<g, kCoarseBlock, 0, s>>>(n, op.agg, op.dv, ncnt, nagg, W, op.yv, z, w, az, r, q, st, part);
+  return g;
+}
+
+int coarse_fcg_pq(int64_t n3, const float* z, const float* az, float* p, float* q, const float* r,
+                  const KrylovState* st, double* part, hipStream_t s) {
+  const int g = coarse_vec_grid(n3);
+  k_fcg_pq<<<g, kCoarseBlock, 0, s>>>(n3, z, az, p, q, r, st, part);
+  return g;
+}
 
 void coarse_assemble(const Sell<double>& A, const CoarseOp& op, int32_t npair, const int32_t* pa, const int32_t* pb,
                      const int32_t* pn_ptr, const int32_t* pn, double* Eblk, double* E, int32_t lda, hipStream_t s) {

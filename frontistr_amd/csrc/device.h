@@ -209,8 +209,13 @@ void launch_scale_vec(bool mul, int64_t n, double* v, const double* sc, hipStrea
 // learned flexible CG: beta from the four dots of net_apply (r.z, z.q, r.z0, z0.q), falling
 // back to the plain SSOR output z0 when the learned z is not a descent direction
 void fcg_beta_learned(KrylovState* st, const double* part, int nparts, hipStream_t s);
-// the flexible CG's beta tail over nparts partials of (r.z, z.q_prev)
+// the flexible CG's beta tail over nparts partials of (r.z, z.q_prev), and its
+// alpha tail over (p.q, r.p)
 void fcg_beta(KrylovState* st, const double* part, int nparts, hipStream_t s);
+void fcg_alpha(KrylovState* st, const double* part, int nparts, hipStream_t s);
+// y = A x for the flexible CG without dots: FP32 values or FP64 values (FP32 vectors)
+void fcg_spmv_store(const Sell<float>& a32, const Sell<double>& a64, bool fp64_values, const float* x, float* y,
+                    const KrylovState* st, hipStream_t s);
 
 // Two-level correction (coarse.hip): piecewise rigid-body modes on geometric
 // aggregates, z += Z E^-1 Z^T r.
@@ -229,6 +234,20 @@ struct CoarseOp {
 void coarse_assemble(const Sell<double>& A, const CoarseOp& op, int32_t npair, const int32_t* pa, const int32_t* pb,
                      const int32_t* pn_ptr, const int32_t* pn, double* Eblk, double* E, int32_t lda, hipStream_t s);
 void coarse_inverse_f32(const CoarseOp& op, const double* Ei, int32_t lda, hipStream_t s);
+// The symmetric two-level step (W = A Z per node and touched aggregate, node-major ncnt/nagg):
+// build W; pre: z = Z E^-1 Z^T r, r1 = r - A z; add: z += t (the SSOR output of r1);
+// post: with w = A z, z += Z E^-1 Z^T (r - w), az = w + A Z E^-1 Z^T (r - w) = A z, partials
+// (r.z, q_prev.z); pq: p = z + beta p, q = az + beta q, partials (p.q, r.p).
+void coarse_build_W(const Sell<double>& A, const CoarseOp& op, int32_t n, const int32_t* ncnt, const int32_t* nagg,
+                    float* W, hipStream_t s);
+void coarse_sym_pre(const CoarseOp& op, int32_t n, const int32_t* ncnt, const int32_t* nagg, const float* W,
+                    const float* r, float* z, float* r1, const KrylovState* st, hipStream_t s);
+void coarse_sym_add(int64_t n3, const float* t, float* z, const KrylovState* st, hipStream_t s);
+int coarse_sym_post(const CoarseOp& op, int32_t n, const int32_t* ncnt, const int32_t* nagg, const float* W,
+                    const float* r, const float* w, float* z, float* az, const float* q, const KrylovState* st,
+                    double* part, hipStream_t s);
+int coarse_fcg_pq(int64_t n3, const float* z, const float* az, float* p, float* q, const float* r,
+                  const KrylovState* st, double* part, hipStream_t s);
 // z += Z E^-1 Z^T r; with dots also the beta partials (r.z, q.z) of the new z.
 // Returns the number of partials.
 int coarse_apply(const CoarseOp& op, int32_t n, const float* r, float* z, const float* q, const KrylovState* st,

@@ -596,6 +596,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_f32(Sell<float> a, const float*
 // FP64 values, FP32 vectors, FP64 accumulation: q = A p for the flexible CG
 // when the FP32-rounded matrix is too inexact (high material contrast), with
 // the FP64 partials of p.q and r.p.
+template <bool DOTS>
 __global__ __launch_bounds__(kBlock) void k_spmv_mixed(Sell<double> a, const float* __restrict__ x,
                                                          const float* __restrict__ w, float* __restrict__ y,
                                                          const KrylovState* st, double* part) {
@@ -640,13 +641,15 @@ __global__ __launch_bounds__(kBlock) void k_spmv_mixed(Sell<double> a, const flo
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         y[3 * row + d] = (float)yv[d];
-        const double pv = x[3 * row + d];
-        acc[0] += pv * yv[d];
-        acc[1] += (double)w[3 * row + d] * pv;
+        if (DOTS) {
+          const double pv = x[3 * row + d];
+          acc[0] += pv * yv[d];
+          acc[1] += (double)w[3 * row + d] * pv;
+        }
       }
     }
   }
-  block_partials<2>(acc, part, blockIdx.x);
+  if (DOTS) block_partials<2>(acc, part, blockIdx.x);
 }
 
 // r32 = (float)(b - A x) with A, x, b in FP64 and the partials of |b - A x|^2;
@@ -1535,6 +1538,17 @@ void fcg_precond_dots(int32_t kind, const SsorOp<float>& op, const float* alu_di
   }
   k_tail<kTailFcgBeta, 2><<<1, kTailThreads, 0, s>>>(st, part, nparts, nullptr, nullptr);
 }
+void fcg_alpha(KrylovState* st, const double* part, int nparts, hipStream_t s) {
+  k_tail<kTailFcgAlpha, 2><<<1, kTailThreads, 0, s>>>(st, part, nparts, nullptr, nullptr);
+}
+void fcg_spmv_store(const Sell<float>& a32, const Sell<double>& a64, bool fp64_values, const float* x, float* y,
+                    const KrylovState* st, hipStream_t s) {
+  const int g = spmv_grid(a64.nslices);
+  if (fp64_values)
+    k_spmv_mixed<false><<<g, kBlock, 0, s>>>(a64, x, nullptr, y, st, nullptr);
+  else
+    k_spmv_f32<false><<<g, kBlock, 0, s>>>(a32, x, nullptr, y, st, nullptr);
+}
 void fcg_beta(KrylovState* st, const double* part, int nparts, hipStream_t s) {
   k_tail<kTailFcgBeta, 2><<<1, kTailThreads, 0, s>>>(st, part, nparts, nullptr, nullptr);
 }
@@ -1557,7 +1571,7 @@ void fcg_spmv(const Sell<float>& a, const float* p, const float* r, float* q, Kr
 void fcg_spmv_mixed(const Sell<double>& a, const float* p, const float* r, float* q, KrylovState* st, double* part,
                     hipStream_t s, hipEvent_t after) {
   const int g = spmv_grid(a.nslices);
-  k_spmv_mixed<<<g, kBlock, 0, s>>>(a, p, r, q, st, part);
+  k_spmv_mixed<true><<<g, kBlock, 0, s>>>(a, p, r, q, st, part);
   if (after) hipEventRecord(after, s);
   k_tail<kTailFcgAlpha, 2><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, nullptr);
 }

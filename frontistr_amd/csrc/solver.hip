@@ -87,6 +87,13 @@ constexpr double kContrastFp64Spmv = 1.0e4;
 // aggregates, larger values the aggregate count to aim at
 constexpr int I_SOLVER_OPT3 = 43;
 constexpr int32_t kCoarseAggDefault = 1000;
+// SOLVER_OPT(4) (Iarray 44) picks the form of the two-level correction: 0
+// (default) the symmetric step z1 = Q r, z2 = z1 + M^-1 (r - A z1),
+// z = z2 + Q (r - A z2) with Q = Z E^-1 Z^T (A z1 and the coarse part of A z
+// from A Z kept per node; q = A p carried as A z + beta q, so an iteration
+// still has one SpMV); 1 the additive z = M^-1 r + Q r.  The learned solve
+// always adds the correction additively.
+constexpr int I_SOLVER_OPT4 = 44;
 // the dense coarse inverse is n_c^2 entries (n_c = 6 per aggregate): at most
 // 2048 aggregates (n_c = 12288, 0.6 GB in FP32)
 constexpr int32_t kCoarseAggMax = 2048;
@@ -257,7 +264,11 @@ struct Context {
   int32_t last_coarse = 0;       // aggregates used by the last solve (-1: E not positive definite)
   CoarseOp coarse;
   int32_t coarse_npair = 0;       // coupled aggregate pairs (blocks of E)
+  bool coarse_sym_use = false;    // this solve takes the symmetric two-level step
+  bool coarse_w_valid = false;    // A Z (per node and touched aggregate) built for the current values
+  int64_t coarse_nw = 0;          // node-aggregate entries of A Z
   DMem cg_agg, cg_aptr, cg_anodes, cg_dv, cg_pa, cg_pb, cg_pnptr, cg_pn, cg_Eblk, cg_E, cg_W, cg_cv, cg_yv, cg_info;
+  DMem cg_ncnt, cg_nagg, cg_AZ, vr1_32, vt32, vw32, vaz32;
   rocblas_handle blas = nullptr;
   // multipoint constraints (hecMESH%mpc) for the MPC-CG product T^T A T
   // (Iarray 83 set, as hecmw_solve_iterative sets it for MPC_METHOD = 2)
@@ -297,7 +308,7 @@ struct Context {
                     &vx0})
       m->release();
     for (DMem* m : {&cg_agg, &cg_aptr, &cg_anodes, &cg_dv, &cg_pa, &cg_pb, &cg_pnptr, &cg_pn, &cg_Eblk, &cg_E, &cg_W,
-                    &cg_cv, &cg_yv, &cg_info})
+                    &cg_cv, &cg_yv, &cg_info, &cg_ncnt, &cg_nagg, &cg_AZ, &vr1_32, &vt32, &vw32, &vaz32})
       m->release();
     if (blas) rocblas_destroy_handle(blas);
     for (DMem* m : {&mpc_slave, &mpc_mptr, &mpc_mdof, &mpc_mval, &mpc_tptr, &mpc_tslave, &mpc_tval, &mpc_w1, &mpc_w2})
@@ -995,6 +1006,12 @@ static int32_t coarse_symbolic(Context& c, int32_t target) {
   op.nc = 6 * na;
   op.ldw = (op.nc + 63) / 64 * 64;
   c.coarse_npair = npair;
+  c.coarse_nw = ncnt[N];
+  if (c.cg_ncnt.ensure(sizeof(int32_t) * (N + 1)) || c.cg_nagg.ensure(sizeof(int32_t) * std::max<int64_t>(ncnt[N], 1)))
+    return HECMW33_ERROR_DEVICE;
+  HIPC(hipMemcpyAsync(c.cg_ncnt.p, ncnt.data(), sizeof(int32_t) * (N + 1), hipMemcpyHostToDevice, c.stream));
+  HIPC(hipMemcpyAsync(c.cg_nagg.p, nagg.data(), sizeof(int32_t) * ncnt[N], hipMemcpyHostToDevice, c.stream));
+  c.coarse_w_valid = false;
   const int64_t npn = pn_ptr[npair];
   if (c.cg_agg.ensure(sizeof(int32_t) * NP) || c.cg_aptr.ensure(sizeof(int32_t) * (na + 1)) ||
       c.cg_anodes.ensure(sizeof(int32_t) * N) || c.cg_dv.ensure(sizeof(float) * 3 * (size_t)N) ||
@@ -1058,6 +1075,7 @@ static int32_t coarse_numeric(Context& c) {
   HIPC(hipMemcpyAsync(hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, c.stream));
   HIPC(hipStreamSynchronize(c.stream));
   c.coarse_ok = hinfo[0] == 0 && hinfo[1] == 0;
+  c.coarse_w_valid = false;
   if (c.coarse_ok) coarse_inverse_f32(op, c.cg_E.as<double>(), nc, c.stream);
   HIPC(hipGetLastError());
   c.coarse_num = true;
@@ -1101,6 +1119,34 @@ static void enqueue_fcg_iteration(Context& c, int32_t kind, bool spmv64, hipEven
     fcg_spmv_mixed(c.A.view, p, r, q, st, part, c.stream, ev ? ev[3] : nullptr);
   else
     fcg_spmv(c.A.view32, p, r, q, st, part, c.stream, ev ? ev[3] : nullptr);
+  fcg_update(n3, p, q, c.vx.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
+  fcg_reliable(c.A.view, c.vx.as<double>(), c.vb.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
+}
+
+// One flexible CG iteration with the symmetric two-level step (FP32 vectors):
+//   z1 = Q r, r1 = r - A z1 (from A Z), z2 = z1 + M32^-1 r1, w = A z2 (the one SpMV),
+//   z = z2 + Q (r - w), A z = w + A Z E^-1 Z^T (r - w);
+// then beta from (r.z, z.q_prev), p = z + beta p and q = A z + beta q (so q = A p
+// without a second SpMV), alpha from (p.q, r.p), and the update as the flexible CG.
+static void enqueue_fcg_sym_iteration(Context& c, int32_t kind, bool spmv64, hipEvent_t* ev) {
+  const int64_t n3 = 3 * (int64_t)c.N;
+  KrylovState* st = c.st.as<KrylovState>();
+  double* part = c.part.as<double>();
+  float *r = c.vr32.as<float>(), *z = c.vz32.as<float>(), *p = c.vp32.as<float>(), *q = c.vq32.as<float>();
+  float *r1 = c.vr1_32.as<float>(), *t = c.vt32.as<float>(), *w = c.vw32.as<float>(), *az = c.vaz32.as<float>();
+  const int32_t* ncnt = c.cg_ncnt.as<int32_t>();
+  const int32_t* nagg = c.cg_nagg.as<int32_t>();
+  const float* AZ = c.cg_AZ.as<float>();
+  if (ev) hipEventRecord(ev[0], c.stream);
+  coarse_sym_pre(c.coarse, c.N, ncnt, nagg, AZ, r, z, r1, st, c.stream);
+  precond_apply_f32(kind, c.ssor_op32(), c.alu_diag32.as<float>(), c.N, r1, t, st, c.stream);
+  coarse_sym_add(n3, t, z, st, c.stream);
+  if (ev) hipEventRecord(ev[1], c.stream);
+  if (ev) hipEventRecord(ev[2], c.stream);
+  fcg_spmv_store(c.A.view32, c.A.view, spmv64, z, w, st, c.stream);
+  if (ev) hipEventRecord(ev[3], c.stream);
+  fcg_beta(st, part, coarse_sym_post(c.coarse, c.N, ncnt, nagg, AZ, r, w, z, az, q, st, part, c.stream), c.stream);
+  fcg_alpha(st, part, coarse_fcg_pq(n3, z, az, p, q, r, st, part, c.stream), c.stream);
   fcg_update(n3, p, q, c.vx.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
   fcg_reliable(c.A.view, c.vx.as<double>(), c.vb.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
 }
@@ -1375,11 +1421,25 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
   c.last_spmv64 = spmv64;
   c.coarse_use = false;
   c.last_coarse = 0;
+  c.coarse_sym_use = false;
   if (mixed && IA(iarray, I_SOLVER_OPT3) > 0) {
     const int32_t o3 = IA(iarray, I_SOLVER_OPT3);
     if (int32_t e = coarse_setup(c, o3)) return e;
     c.coarse_use = c.coarse_ok;
     c.last_coarse = c.coarse_ok ? c.coarse.n_agg : -1;
+    if (c.coarse_use && IA(iarray, I_SOLVER_OPT4) != 1 && !learned) {
+      const size_t vb32 = sizeof(float) * 3 * (size_t)c.NP;
+      for (DMem* m : {&c.vr1_32, &c.vt32, &c.vw32, &c.vaz32})
+        if (m->ensure(vb32)) return HECMW33_ERROR_DEVICE;
+      if (!c.coarse_w_valid) {
+        if (c.cg_AZ.ensure(sizeof(float) * 18 * (size_t)std::max<int64_t>(c.coarse_nw, 1))) return HECMW33_ERROR_DEVICE;
+        coarse_build_W(c.A.view, c.coarse, c.N, c.cg_ncnt.as<int32_t>(), c.cg_nagg.as<int32_t>(),
+                       c.cg_AZ.as<float>(), c.stream);
+        HIPC(hipGetLastError());
+        c.coarse_w_valid = true;
+      }
+      c.coarse_sym_use = true;
+    }
   }
   if (learned) {
     if (int32_t e = learned_setup(c, RA(rarray, R_SIGMA_DIAG))) return e;
@@ -1526,6 +1586,8 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
       }
       if (learned)
         enqueue_fcg_learned_iteration(c, spmv64, e);
+      else if (c.coarse_sym_use)
+        enqueue_fcg_sym_iteration(c, kind, spmv64, e);
       else if (mixed)
         enqueue_fcg_iteration(c, kind, spmv64, e);
       else if (method == 1)

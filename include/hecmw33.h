@@ -104,8 +104,13 @@ int32_t hecmw33_solve_krylov(const hecmw33_sys* A, const double* B, double* X, i
  *   E = Z^T A Z is inverted densely (FP64 Cholesky, FP32 copy for the
  *   iterations).  Aggregates: the nodes of each cell of a uniform grid over the
  *   node coordinates given by hecmw33_set_coords / hecmw33_ctx_set_coords.  1 aims
- *   at 1000 aggregates, larger values at that many (at most 2048).  If E is not positive definite the solve runs
- *   without it (hecmw33_ctx_info [11] = -1).
+ *   at 1000 aggregates, larger values at that many (at most 2048).  If E is not
+ *   positive definite the solve runs without it (hecmw33_ctx_info [11] = -1).
+ *   iarray[43] (Iarray 44) picks its form: 0 (default) the symmetric step
+ *   z1 = Q r, z2 = z1 + M^-1 (r - A z1), z = z2 + Q (r - A z2) with Q = Z E^-1 Z^T
+ *   (A Z is kept per node and neighbouring aggregate, and q = A p is carried as
+ *   A z + beta q, so an iteration still has one SpMV); 1 the additive
+ *   z = M^-1 r + Q r, which the learned solve (Iarray 41 = 2) always uses.
  *   Accepted by hecmw33_solve, hecmw33_solve_krylov and hecmw33_ctx_solve_device. */
 
 /* Node coordinates for the two-level correction: xyz interleaved, 3*NP doubles
