@@ -242,8 +242,11 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-# flexible CG variants: (name, Iarray 41, Iarray 43)
-FCG_MODES = (("fp64_ssor_cg", 0, 0), ("mixed_fcg", 1, 0), ("mixed_coarse_fcg", 1, 1), ("learned_coarse_fcg", 2, 1))
+# flexible CG variants: (name, Iarray 41, Iarray 43, Iarray 44); the two-level
+# correction symmetric (coarse, SSOR, coarse; 44 = 0) or additive (44 = 1); the
+# learned solve adds it additively
+FCG_MODES = (("fp64_ssor_cg", 0, 0, 0), ("mixed_fcg", 1, 0, 0), ("mixed_coarse_fcg", 1, 1, 0),
+             ("mixed_coarse_additive_fcg", 1, 1, 1), ("learned_coarse_fcg", 2, 1, 1))
 
 
 def _run_modes(tag, fa, ctx, m, b, x, net, modes):
@@ -252,7 +255,7 @@ def _run_modes(tag, fa, ctx, m, b, x, net, modes):
     inverse) inside the timed solve."""
     import torch
     res, xs = {}, {}
-    for mode, opt, coarse in modes:
+    for mode, opt, coarse, additive in modes:
         if opt == 2 and net is None:
             continue
         ia = m.Iarray.copy()
@@ -260,6 +263,7 @@ def _run_modes(tag, fa, ctx, m, b, x, net, modes):
         ia[97] = 0
         ia[40] = opt
         ia[42] = coarse
+        ia[43] = additive
         x.zero_()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
@@ -278,6 +282,7 @@ def _run_modes(tag, fa, ctx, m, b, x, net, modes):
             res[mode]["spmv_fp64_values"] = bool(inf["spmv64"])
         if coarse:
             res[mode]["coarse_aggregates"] = inf["coarse"]
+            res[mode]["coarse_form"] = "additive" if additive or opt == 2 else "symmetric"
         if opt == 2:
             res[mode]["fallback_iterations"] = inf["nfallback"]
             res[mode]["correction_dropped"] = bool(inf["learned_off"])
@@ -387,7 +392,7 @@ def bench_learned(args, dev, net, train):
     b = torch.from_numpy(m.B).to(dev)
     ctx.set_net(net)
     x = torch.zeros(3 * m.NP, dtype=torch.float64, device=dev)
-    modes = FCG_MODES[:2] + (("learned_fcg", 2, 0),) + FCG_MODES[2:]
+    modes = FCG_MODES[:2] + (("learned_fcg", 2, 0, 0),) + FCG_MODES[2:]
     res = _run_modes("C5", fa, ctx, m, b, x, net, modes)
     ctx.set_net(None)
     ctx.close()
