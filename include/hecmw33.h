@@ -95,9 +95,11 @@ int32_t hecmw33_solve_krylov(const hecmw33_sys* A, const double* B, double* X, i
  *   rarray[40] (Rarray 41, default 1e-2), every 50 iterations and before any
  *   convergence is declared, so RESID is always an FP64 true residual.
  *   iarray[41] (Iarray 42) picks the matrix values of its q = A p: 1 FP32,
- *   2 FP64 (FP32 vectors, FP64 accumulation), 0 (default) FP64 when the
- *   stiffness contrast max/min |tr D_ii| exceeds 1e4 (an FP32-rounded operator
- *   loses the search directions' conjugacy on high-contrast systems), else FP32.
+ *   2 FP64 (FP32 vectors, FP64 accumulation), 0 (default) FP32 with the
+ *   two-level correction below, otherwise FP64 when the stiffness contrast
+ *   max/min |tr D_ii| exceeds 1e4 (over the long solves of high-contrast
+ *   systems an FP32-rounded operator loses the search directions'
+ *   conjugacy), else FP32.
  *   iarray[42] (Iarray 43) > 0 adds a two-level correction to the flexible CG's
  *   preconditioner, z = M^-1 r + Z E^-1 Z^T r: Z holds the six rigid-body modes
  *   (three translations, three rotations about the centroid) of each aggregate;
