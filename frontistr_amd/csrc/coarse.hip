@@ -85,12 +85,13 @@ __global__ __launch_bounds__(kCoarseBlock) void k_coarse_E(Sell<double> A, const
       const int64_t slot = base + (int64_t)k * A.rows + l;
       const int j = A.col[slot];
       if (agg[j] != b) continue;
-      const double* v = A.val + (base + (int64_t)k * A.rows) * 9 + l;
+      double v[9];
+      aval_load(A.val, (int64_t)A.ptr[sl] + k, l, v);
       double m[3][6];
       const double jx = dv[3 * (int64_t)j], jy = dv[3 * (int64_t)j + 1], jz = dv[3 * (int64_t)j + 2];
 #pragma unroll
       for (int r = 0; r < 3; ++r) {
-        const double a0 = v[(3 * r) * A.rows], a1 = v[(3 * r + 1) * A.rows], a2 = v[(3 * r + 2) * A.rows];
+        const double a0 = v[3 * r], a1 = v[3 * r + 1], a2 = v[3 * r + 2];
         m[r][0] = a0;
         m[r][1] = a1;
         m[r][2] = a2;
@@ -246,11 +247,12 @@ __global__ __launch_bounds__(kCoarseBlock) void k_coarse_W(Sell<double> A, int32
     for (int k = 0; k < len; ++k) {
       const int j = A.col[base + (int64_t)k * A.rows + l];
       if (agg[j] != b) continue;
-      const double* v = A.val + (base + (int64_t)k * A.rows) * 9 + l;
+      double v[9];
+      aval_load(A.val, (int64_t)A.ptr[sl] + k, l, v);
       const double jx = dv[3 * (int64_t)j], jy = dv[3 * (int64_t)j + 1], jz = dv[3 * (int64_t)j + 2];
 #pragma unroll
       for (int r = 0; r < 3; ++r) {
-        const double a0 = v[(3 * r) * A.rows], a1 = v[(3 * r + 1) * A.rows], a2 = v[(3 * r + 2) * A.rows];
+        const double a0 = v[3 * r], a1 = v[3 * r + 1], a2 = v[3 * r + 2];
         m[6 * r] += a0;
         m[6 * r + 1] += a1;
         m[6 * r + 2] += a2;

@@ -25,6 +25,38 @@ constexpr int kNumXcd = 8;      // MI355X: 8 XCDs, each with its own L2
 //    broadcasts, and a colour launch has four times as many wavefronts.
 constexpr int kRowsA = 64;
 constexpr int kRowsS = 16;
+
+// The SpMV matrix A (C = 64) pairs its planes instead: in slot group
+// g = ptr[s]+k, entries e = 2p and 2p+1 of lane r's block are adjacent at
+// val[g*576 + 128p + 2r + (e&1)] (p = 0..3), entry 8 at val[g*576 + 512 + r],
+// so a lane fetches a block with four 16-byte loads (8-byte in FP32) and one
+// 8-byte load, and each wavefront load is still one contiguous segment.
+__host__ __device__ inline int64_t aval_index(int64_t g, int e, int r) {
+  return g * (9 * kRowsA) + (e < 8 ? (int64_t)(e >> 1) * (2 * kRowsA) + 2 * r + (e & 1) : 8 * kRowsA + r);
+}
+template <typename T>
+struct Pair2;
+template <>
+struct Pair2<double> {
+  using type = double2;
+};
+template <>
+struct Pair2<float> {
+  using type = float2;
+};
+// the 9 entries of lane r's block in slot group g of A
+template <typename T>
+__device__ __forceinline__ void aval_load(const T* __restrict__ val, int64_t g, int r, T (&v)[9]) {
+  using V2 = typename Pair2<T>::type;
+  const T* __restrict__ b = val + g * (9 * kRowsA);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const V2 t = *reinterpret_cast<const V2*>(b + q * (2 * kRowsA) + 2 * r);
+    v[2 * q] = t.x;
+    v[2 * q + 1] = t.y;
+  }
+  v[8] = b[8 * kRowsA + r];
+}
 // colours above this many rows on average sweep with a lane per row (SELL-64):
 // 4-5% faster at 245k rows per colour (160^3 cube), slower at 60k (100^3)
 constexpr int kSsorRows64Min = 150000;
@@ -90,10 +122,14 @@ constexpr int kMaxPartials = 65536;  // per reduction component
 // ---- setup ---------------------------------------------------------------------
 // Fill SELL values from the caller's D/AL/AU through a slot->source map
 // (src >= 0: block index into [D | AL | AU], -1: padding).
+// launch_fill_values: 9 planes (the SSOR factors); the _a variants: A's paired
+// planes (aval_index).
 void launch_fill_values(const Sell<double>& a, const int64_t* src, const double* D, const double* AL,
                         const double* AU, int64_t nD, int64_t nAL, hipStream_t s);
-void launch_fill_values_f32(const Sell<float>& a, const int64_t* src, const double* D, const double* AL,
-                            const double* AU, int64_t nD, int64_t nAL, hipStream_t s);
+void launch_fill_values_a(const Sell<double>& a, const int64_t* src, const double* D, const double* AL,
+                          const double* AU, int64_t nD, int64_t nAL, hipStream_t s);
+void launch_fill_values_a_f32(const Sell<float>& a, const int64_t* src, const double* D, const double* AL,
+                              const double* AU, int64_t nD, int64_t nAL, hipStream_t s);
 // 3x3 no-pivot LU with sigma on the pivots of D[row_src[t]] into 9-plane
 // slices of `rows` rows (hecmw_precond_SSOR_33.F90:164-208,
 // hecmw_precond_DIAG_33.F90:58-91)

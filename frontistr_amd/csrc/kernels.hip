@@ -348,12 +348,10 @@ __device__ __forceinline__ void sell_row(const Sell<double>& a, int s, int lane,
   xo[0] = xo[1] = xo[2] = 0.0;
   if (len == 0) return;
   const int32_t* __restrict__ colp = a.col + (int64_t)base * kRowsA + lane;
-  const double* __restrict__ vp = a.val + (int64_t)base * 9 * kRowsA + lane;
   {
     const int c = colp[0];
     double v[9];
-#pragma unroll
-    for (int e = 0; e < 9; ++e) v[e] = vp[e * kRowsA];
+    aval_load(a.val, base, lane, v);
     const double x0 = x[3 * (int64_t)c], x1 = x[3 * (int64_t)c + 1], x2 = x[3 * (int64_t)c + 2];
     xo[0] = x0;
     xo[1] = x1;
@@ -374,9 +372,7 @@ __device__ __forceinline__ void sell_row(const Sell<double>& a, int s, int lane,
     double v[U][9], xv[U][3];
 #pragma unroll
     for (int j = 0; j < U; ++j) {
-      const double* __restrict__ vk = vp + (int64_t)kk[j] * 9 * kRowsA;
-#pragma unroll
-      for (int e = 0; e < 9; ++e) v[j][e] = vk[e * kRowsA];
+      aval_load(a.val, (int64_t)base + kk[j], lane, v[j]);
       xv[j][0] = x[3 * (int64_t)c[j]];
       xv[j][1] = x[3 * (int64_t)c[j] + 1];
       xv[j][2] = x[3 * (int64_t)c[j] + 2];
@@ -403,30 +399,31 @@ __device__ __forceinline__ void sell_row_f32(const Sell<float>& a, int s, int la
   const int len = a.len[s * kRowsA + lane];
   y[0] = y[1] = y[2] = 0.0f;
   const int32_t* __restrict__ colp = a.col + (int64_t)base * kRowsA + lane;
-  const float* __restrict__ vp = a.val + (int64_t)base * 9 * kRowsA + lane;
   for (int k0 = 0; k0 < len; k0 += U) {
-    int c[U];
+    // branch-free chunk, as in sell_row
+    int kk[U], c[U];
 #pragma unroll
-    for (int j = 0; j < U; ++j) c[j] = (k0 + j < len) ? colp[(int64_t)(k0 + j) * kRowsA] : 0;
+    for (int j = 0; j < U; ++j) {
+      kk[j] = (k0 + j < len) ? k0 + j : len - 1;
+      c[j] = colp[(int64_t)kk[j] * kRowsA];
+    }
     float v[U][9], xv[U][3];
 #pragma unroll
     for (int j = 0; j < U; ++j) {
-      if (k0 + j < len) {
-        const float* __restrict__ vk = vp + (int64_t)(k0 + j) * 9 * kRowsA;
-#pragma unroll
-        for (int e = 0; e < 9; ++e) v[j][e] = vk[e * kRowsA];
-        xv[j][0] = x[3 * (int64_t)c[j]];
-        xv[j][1] = x[3 * (int64_t)c[j] + 1];
-        xv[j][2] = x[3 * (int64_t)c[j] + 2];
-      }
+      aval_load(a.val, (int64_t)base + kk[j], lane, v[j]);
+      xv[j][0] = x[3 * (int64_t)c[j]];
+      xv[j][1] = x[3 * (int64_t)c[j] + 1];
+      xv[j][2] = x[3 * (int64_t)c[j] + 2];
     }
 #pragma unroll
     for (int j = 0; j < U; ++j) {
-      if (k0 + j < len) {
-        y[0] = fmaf(v[j][2], xv[j][2], fmaf(v[j][1], xv[j][1], fmaf(v[j][0], xv[j][0], y[0])));
-        y[1] = fmaf(v[j][5], xv[j][2], fmaf(v[j][4], xv[j][1], fmaf(v[j][3], xv[j][0], y[1])));
-        y[2] = fmaf(v[j][8], xv[j][2], fmaf(v[j][7], xv[j][1], fmaf(v[j][6], xv[j][0], y[2])));
-      }
+      const bool live = k0 + j < len;
+      const float t0 = fmaf(v[j][2], xv[j][2], fmaf(v[j][1], xv[j][1], fmaf(v[j][0], xv[j][0], y[0])));
+      const float t1 = fmaf(v[j][5], xv[j][2], fmaf(v[j][4], xv[j][1], fmaf(v[j][3], xv[j][0], y[1])));
+      const float t2 = fmaf(v[j][8], xv[j][2], fmaf(v[j][7], xv[j][1], fmaf(v[j][6], xv[j][0], y[2])));
+      y[0] = live ? t0 : y[0];
+      y[1] = live ? t1 : y[1];
+      y[2] = live ? t2 : y[2];
     }
   }
 }
@@ -608,7 +605,6 @@ __global__ __launch_bounds__(kBlock) void k_spmv_mixed(Sell<double> a, const flo
     const int base = a.ptr[s];
     const int len = a.len[s * kRowsA + lane];
     const int32_t* __restrict__ colp = a.col + (int64_t)base * kRowsA + lane;
-    const double* __restrict__ vp = a.val + (int64_t)base * 9 * kRowsA + lane;
     double yv[3] = {0.0, 0.0, 0.0};
     for (int k0 = 0; k0 < len; k0 += kSpmvChunk) {
       int kk[kSpmvChunk], c[kSpmvChunk];
@@ -620,9 +616,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_mixed(Sell<double> a, const flo
       double v[kSpmvChunk][9], xv[kSpmvChunk][3];
 #pragma unroll
       for (int j = 0; j < kSpmvChunk; ++j) {
-        const double* __restrict__ vk = vp + (int64_t)kk[j] * 9 * kRowsA;
-#pragma unroll
-        for (int e = 0; e < 9; ++e) v[j][e] = vk[e * kRowsA];
+        aval_load(a.val, (int64_t)base + kk[j], lane, v[j]);
 #pragma unroll
         for (int d = 0; d < 3; ++d) xv[j][d] = (double)x[3 * (int64_t)c[j] + d];
       }
@@ -1186,7 +1180,7 @@ __global__ void k_scale_vec(int64_t n, double* __restrict__ v, const double* __r
 }
 
 // ---- numeric setup ---------------------------------------------------------------
-template <typename T>
+template <typename T, bool PAIRED>
 __global__ void k_fill_values(Sell<T> a, const int64_t* __restrict__ src, const double* __restrict__ D,
                               const double* __restrict__ AL, const double* __restrict__ AU, int64_t nD, int64_t nAL) {
   const int64_t nslot = a.nslots;
@@ -1196,9 +1190,14 @@ __global__ void k_fill_values(Sell<T> a, const int64_t* __restrict__ src, const 
     const int64_t b = src[t];
     const double* blk = nullptr;
     if (b >= 0) blk = b < nD ? D + 9 * b : (b < nD + nAL ? AL + 9 * (b - nD) : AU + 9 * (b - nD - nAL));
-    T* out = a.val + grp * 9 * C + r;
+    if (PAIRED) {
 #pragma unroll
-    for (int e = 0; e < 9; ++e) out[e * C] = blk ? (T)blk[e] : (T)0;
+      for (int e = 0; e < 9; ++e) a.val[aval_index(grp, e, (int)r)] = blk ? (T)blk[e] : (T)0;
+    } else {
+      T* out = a.val + grp * 9 * C + r;
+#pragma unroll
+      for (int e = 0; e < 9; ++e) out[e * C] = blk ? (T)blk[e] : (T)0;
+    }
   }
 }
 
@@ -1288,11 +1287,15 @@ void launch_dot_store(int64_t n, const double* x, const double* y, double* out, 
 }
 void launch_fill_values(const Sell<double>& a, const int64_t* src, const double* D, const double* AL,
                         const double* AU, int64_t nD, int64_t nAL, hipStream_t s) {
-  k_fill_values<double><<<4096, 256, 0, s>>>(a, src, D, AL, AU, nD, nAL);
+  k_fill_values<double, false><<<4096, 256, 0, s>>>(a, src, D, AL, AU, nD, nAL);
 }
-void launch_fill_values_f32(const Sell<float>& a, const int64_t* src, const double* D, const double* AL,
-                            const double* AU, int64_t nD, int64_t nAL, hipStream_t s) {
-  k_fill_values<float><<<4096, 256, 0, s>>>(a, src, D, AL, AU, nD, nAL);
+void launch_fill_values_a(const Sell<double>& a, const int64_t* src, const double* D, const double* AL,
+                          const double* AU, int64_t nD, int64_t nAL, hipStream_t s) {
+  k_fill_values<double, true><<<4096, 256, 0, s>>>(a, src, D, AL, AU, nD, nAL);
+}
+void launch_fill_values_a_f32(const Sell<float>& a, const int64_t* src, const double* D, const double* AL,
+                              const double* AU, int64_t nD, int64_t nAL, hipStream_t s) {
+  k_fill_values<float, true><<<4096, 256, 0, s>>>(a, src, D, AL, AU, nD, nAL);
 }
 void launch_block_lu(int32_t nslots, const int32_t* row_src, const double* D, double sigma, double* alu, int rows,
                      hipStream_t s) {

@@ -459,7 +459,7 @@ static int32_t load_values(Context& c, const hecmw33_sys* A) {
   HIPC(hipMemcpyAsync(c.dD.p, A->D, sizeof(double) * 9 * (size_t)A->NP, hipMemcpyHostToDevice, c.stream));
   if (A->NPL) HIPC(hipMemcpyAsync(c.dAL.p, A->AL, sizeof(double) * 9 * (size_t)A->NPL, hipMemcpyHostToDevice, c.stream));
   if (A->NPU) HIPC(hipMemcpyAsync(c.dAU.p, A->AU, sizeof(double) * 9 * (size_t)A->NPU, hipMemcpyHostToDevice, c.stream));
-  launch_fill_values(c.A.view, c.A.src.as<int64_t>(), c.dD.as<double>(), c.dAL.as<double>(), c.dAU.as<double>(),
+  launch_fill_values_a(c.A.view, c.A.src.as<int64_t>(), c.dD.as<double>(), c.dAL.as<double>(), c.dAU.as<double>(),
                      c.N, c.NPL, c.stream);
   HIPC(hipGetLastError());
   c.a32_valid = false;
@@ -1208,7 +1208,7 @@ static int32_t scaling_apply(Context& c, bool fw) {
   }
   launch_scale_raw(fw, NP, c.dIndexL.as<int32_t>(), c.dItemL.as<int32_t>(), c.dIndexU.as<int32_t>(),
                    c.dItemU.as<int32_t>(), c.dD.as<double>(), c.dAL.as<double>(), c.dAU.as<double>(), sc, c.stream);
-  launch_fill_values(c.A.view, c.A.src.as<int64_t>(), c.dD.as<double>(), c.dAL.as<double>(), c.dAU.as<double>(),
+  launch_fill_values_a(c.A.view, c.A.src.as<int64_t>(), c.dD.as<double>(), c.dAL.as<double>(), c.dAU.as<double>(),
                      c.N, c.NPL, c.stream);
   c.a32_valid = false;
   c.coarse_num = false;
@@ -1964,7 +1964,7 @@ int32_t hecmw33_ctx_matvec_device_f32(hecmw33_ctx* ctx, const float* x_dev, floa
   if (!c.A.val32.p) {
     if (c.A.val32.ensure(sizeof(float) * 9 * (size_t)std::max<int64_t>(c.A.view.nslots, 1))) return HECMW33_ERROR_DEVICE;
     c.A.view32.val = c.A.val32.as<float>();
-    launch_fill_values_f32(c.A.view32, c.A.src.as<int64_t>(), c.dD.as<double>(), c.dAL.as<double>(),
+    launch_fill_values_a_f32(c.A.view32, c.A.src.as<int64_t>(), c.dD.as<double>(), c.dAL.as<double>(),
                            c.dAU.as<double>(), c.N, c.NPL, c.stream);
     HIPC(hipGetLastError());
   }
