@@ -29,6 +29,7 @@ from .hecmw import (  # noqa: F401
     last_history,
     hecmw_matvec,
     hecmw_matvec_mpc,
+    stream_read,
     set_mpc,
     hecmw_precond_apply,
     hecmw_rel_resid_L2,

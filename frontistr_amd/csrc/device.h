@@ -122,6 +122,9 @@ constexpr int kMaxPartials = 65536;  // per reduction component
 // ---- setup ---------------------------------------------------------------------
 // Fill SELL values from the caller's D/AL/AU through a slot->source map
 // (src >= 0: block index into [D | AL | AU], -1: padding).
+// Plain streaming read of n doubles (n even, a 16-byte aligned), grid
+// workgroups, one partial each in part.
+void launch_stream_read(const double* a, int64_t n, double* part, int grid, hipStream_t s);
 // launch_fill_values: 9 planes (the SSOR factors); the _a variants: A's paired
 // planes (aval_index).
 void launch_fill_values(const Sell<double>& a, const int64_t* src, const double* D, const double* AL,

@@ -1191,8 +1191,16 @@ __global__ void k_fill_values(Sell<T> a, const int64_t* __restrict__ src, const 
     const double* blk = nullptr;
     if (b >= 0) blk = b < nD ? D + 9 * b : (b < nD + nAL ? AL + 9 * (b - nD) : AU + 9 * (b - nD - nAL));
     if (PAIRED) {
+      using V2 = typename Pair2<T>::type;
+      T* out = a.val + grp * (9 * kRowsA);
 #pragma unroll
-      for (int e = 0; e < 9; ++e) a.val[aval_index(grp, e, (int)r)] = blk ? (T)blk[e] : (T)0;
+      for (int q = 0; q < 4; ++q) {
+        V2 t;
+        t.x = blk ? (T)blk[2 * q] : (T)0;
+        t.y = blk ? (T)blk[2 * q + 1] : (T)0;
+        *reinterpret_cast<V2*>(out + q * (2 * kRowsA) + 2 * r) = t;
+      }
+      out[8 * kRowsA + r] = blk ? (T)blk[8] : (T)0;
     } else {
       T* out = a.val + grp * 9 * C + r;
 #pragma unroll
@@ -1237,6 +1245,19 @@ __global__ void k_block_lu(int32_t nslots, const int32_t* __restrict__ row_src, 
   }
 }
 
+// ---- HBM read ceiling -----------------------------------------------------------------
+// Sum of n2 double2 values with a grid-stride loop, one partial per workgroup:
+// the plain streaming read that bounds what any kernel here can get from HBM.
+__global__ __launch_bounds__(kBlock) void k_stream_read(const double2* __restrict__ a, int64_t n2,
+                                                         double* __restrict__ part) {
+  double acc[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n2; i += (int64_t)gridDim.x * kBlock) {
+    const double2 v = a[i];
+    acc[0] += v.x + v.y;
+  }
+  block_partials<1>(acc, part, blockIdx.x);
+}
+
 }  // namespace
 
 // ---- launchers ---------------------------------------------------------------------
@@ -1266,6 +1287,9 @@ static int spmv_any(hipStream_t s, const Sell<double>& a, const MpcOp* m, const 
   return g;
 }
 
+void launch_stream_read(const double* a, int64_t n, double* part, int grid, hipStream_t s) {
+  k_stream_read<<<grid, kBlock, 0, s>>>(reinterpret_cast<const double2*>(a), n / 2, part);
+}
 void launch_spmv(const Sell<double>& a, const double* x, double* y, hipStream_t s) {
   spmv_launch<kEpiStore>(spmv_grid(a.nslices), s, a, x, nullptr, y, nullptr, nullptr);
 }

@@ -1940,6 +1940,30 @@ int32_t hecmw33_ctx_solve_device(hecmw33_ctx* ctx, const double* B_dev, double* 
   return out->error;
 }
 
+int32_t hecmw33_stream_read(const double* buf_dev, int64_t n, double* part_dev, int32_t reps, double* gbs) {
+  if (!buf_dev || !part_dev || !gbs || n < 2 || (n & 1) || ((uintptr_t)buf_dev & 15) || reps < 1)
+    return HECMW33_ERROR_ARGS;
+  hipEvent_t e0, e1;
+  HIPC(hipEventCreate(&e0));
+  HIPC(hipEventCreate(&e1));
+  double best = 0.0;
+  for (const int grid : {1024, 2048, 4096, 8192, 65536}) {
+    launch_stream_read(buf_dev, n, part_dev, grid, nullptr);
+    HIPC(hipEventRecord(e0, nullptr));
+    for (int i = 0; i < reps; ++i) launch_stream_read(buf_dev, n, part_dev, grid, nullptr);
+    HIPC(hipEventRecord(e1, nullptr));
+    HIPC(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIPC(hipEventElapsedTime(&t, e0, e1));
+    if (t > 0.f) best = std::max(best, 8.0 * (double)n * reps / ((double)t * 1e6));
+  }
+  HIPC(hipGetLastError());
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  *gbs = best;
+  return 0;
+}
+
 int32_t hecmw33_ctx_matvec_device(hecmw33_ctx* ctx, const double* x_dev, double* y_dev, int32_t count, double* ms) {
   if (!ctx || !ctx->c.have_struct || count < 1) return HECMW33_ERROR_ARGS;
   Context& c = ctx->c;

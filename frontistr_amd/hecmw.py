@@ -75,6 +75,7 @@ def lib():
     L.hecmw33_ctx_solve_device.argtypes = [P, P, P, P, P, C.POINTER(_Out)]
     L.hecmw33_ctx_matvec_device.argtypes = [P, P, P, I, C.POINTER(C.c_double)]
     L.hecmw33_ctx_matvec_device_f32.argtypes = [P, P, P, I, C.POINTER(C.c_double)]
+    L.hecmw33_stream_read.argtypes = [P, C.c_int64, P, I, C.POINTER(C.c_double)]
     L.hecmw33_ctx_history.argtypes = [P, P, I]
     L.hecmw33_ctx_info.argtypes = [P, P, I]
     L.hecmw33_gen_hex8_count.argtypes = [I, C.POINTER(I), C.POINTER(I), C.POINTER(I)]
@@ -114,7 +115,7 @@ def lib():
     L.hecmw33_last_error.restype = C.c_char_p
     for f in ("hecmw33_solve", "hecmw33_solve_krylov", "hecmw33_matvec", "hecmw33_precond_apply",
               "hecmw33_rel_resid_L2", "hecmw33_ordering", "hecmw33_ctx_load", "hecmw33_ctx_solve_device",
-              "hecmw33_ctx_matvec_device", "hecmw33_ctx_matvec_device_f32", "hecmw33_ctx_history",
+              "hecmw33_ctx_matvec_device", "hecmw33_ctx_matvec_device_f32", "hecmw33_stream_read", "hecmw33_ctx_history",
               "hecmw33_ctx_info", "hecmw33_gen_hex8_count", "hecmw33_gen_hex8",
               "hecmw33_gen_tet10_count", "hecmw33_gen_tet10", "hecmw33_device_count",
               "hecmw33_last_history", "hecmw33_batch_load", "hecmw33_batch_solve", "hecmw33_batch_get_x",
@@ -289,6 +290,14 @@ def last_history(cap=100000):
     h = np.zeros(cap)
     n = _check(lib().hecmw33_last_history(_ptr(h), cap))
     return h[:n]
+
+
+def stream_read(buf_ptr, n, part_ptr, reps=10):
+    """HBM read bandwidth (GB/s) of a plain streaming kernel over n doubles of
+    device memory at buf_ptr; part_ptr: device scratch of 65536 doubles."""
+    g = C.c_double(0.0)
+    _check(lib().hecmw33_stream_read(buf_ptr, n, part_ptr, reps, C.byref(g)))
+    return g.value
 
 
 def hecmw_matvec(mat, x):

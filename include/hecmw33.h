@@ -181,6 +181,12 @@ int32_t hecmw33_ctx_solve_device(hecmw33_ctx* ctx, const double* B_dev, double* 
  * kernel time in *ms (HIP events on the context's stream). */
 int32_t hecmw33_ctx_matvec_device(hecmw33_ctx* ctx, const double* x_dev, double* y_dev, int32_t count, double* ms);
 int32_t hecmw33_ctx_matvec_device_f32(hecmw33_ctx* ctx, const float* x_dev, float* y_dev, int32_t count, double* ms);
+/* No counterpart in the reference (a measurement aid for the benchmark): the
+ * HBM read bandwidth a plain streaming kernel reaches on this GPU, in GB/s,
+ * best of a few launch shapes, each timed over `reps` reads of the n doubles
+ * at buf_dev (device memory, n even, 16-byte aligned).  part_dev: device
+ * scratch of at least 65536 doubles, overwritten. */
+int32_t hecmw33_stream_read(const double* buf_dev, int64_t n, double* part_dev, int32_t reps, double* gbs);
 /* Residual history of the last solve (ITERLOG values), up to cap entries. */
 int32_t hecmw33_ctx_history(hecmw33_ctx* ctx, double* hist, int32_t cap);
 /* Context figures: [0] SELL-64 slots of A (incl. padding), [1] N, [2] NPL, [3] NPU, [4] colours,

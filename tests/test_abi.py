@@ -34,6 +34,11 @@ def test_library_refuses_bad_arguments_without_gpu():
     m.N = 0  # invalid: no rows
     with pytest.raises(fa.HecmwError):
         fa.hecmw_matvec(m, np.zeros(3 * m.NP))
+    # the read-ceiling probe checks its buffer before any launch: odd length,
+    # misaligned or null pointers
+    for buf, n, part in ((16, 3, 16), (8, 4, 16), (0, 4, 16), (16, 4, 0)):
+        with pytest.raises(fa.HecmwError):
+            fa.stream_read(buf, n, part, 1)
 
 
 def test_product_does_not_reference_oracle():
