@@ -93,30 +93,44 @@ __device__ __forceinline__ void tile_partials_rowlanes(const double (&acc)[ND], 
 }
 
 // ---- preconditioner + fused dots: z = D^-1 r, r.z, z.q ----------------------------
+// z of one (row, rhs) from its three residual entries (the reference's DIAG
+// statements, hecmw_precond_DIAG_33.F90)
+__device__ __forceinline__ void kb_jacobi(const float* __restrict__ alu, int64_t g, float r0v, float r1v, float r2v,
+                                          float& x0, float& x1, float& x2) {
+#pragma clang fp contract(off)
+  float a[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) a[e] = alu[9 * g + e];
+  x0 = r0v;
+  x1 = r1v;
+  x2 = r2v;
+  x1 = x1 - a[3] * x0;
+  x2 = x2 - a[6] * x0 - a[7] * x1;
+  x2 = a[8] * x2;
+  x1 = a[4] * (x1 - a[5] * x2);
+  x0 = a[0] * (x0 - a[2] * x2 - a[1] * x1);
+}
+
+// guard 0: every active system; 2: only systems whose residuals were just
+// replaced (their z is recomputed from the new r)
 __global__ __launch_bounds__(kBT) void kb_diag(Tiles T, const float* __restrict__ alu, const float* __restrict__ r,
                                                float* __restrict__ z, const float* __restrict__ q, BState S,
-                                               double* part) {
+                                               double* part, int guard) {
 #pragma clang fp contract(off)
   const int t = blockIdx.x;
   const int s = T.sys[t];
   if (S.sys_active[s] == 0) return;
+  if (guard == 2 && S.sys_rel[s] == 0) return;
   const int k = threadIdx.x % KS, rl = threadIdx.x / KS;
   const bool live = !S.done[s * KS + k];
   double acc[2] = {0.0, 0.0};
   const int r0 = T.row0[t], nr = T.nrows[t];
   for (int i = rl; i < nr && live; i += kBT / KS) {
     const int64_t g = r0 + i;
-    float a[9];
-#pragma unroll
-    for (int e = 0; e < 9; ++e) a[e] = alu[9 * g + e];
     const int64_t o = (g * KS + k) * 3;
     const float r0v = r[o], r1v = r[o + 1], r2v = r[o + 2];
-    float x0 = r0v, x1 = r1v, x2 = r2v;
-    x1 = x1 - a[3] * x0;
-    x2 = x2 - a[6] * x0 - a[7] * x1;
-    x2 = a[8] * x2;
-    x1 = a[4] * (x1 - a[5] * x2);
-    x0 = a[0] * (x0 - a[2] * x2 - a[1] * x1);
+    float x0, x1, x2;
+    kb_jacobi(alu, g, r0v, r1v, r2v, x0, x1, x2);
     z[o] = x0;
     z[o + 1] = x1;
     z[o + 2] = x2;
@@ -144,10 +158,15 @@ __global__ __launch_bounds__(kBT) void kb_p(Tiles T, const float* __restrict__ z
   }
 }
 
-// ---- x += alpha p (FP64), r = r - alpha q (FP32), r.r --------------------------------
-__global__ __launch_bounds__(kBT) void kb_update(Tiles T, const float* __restrict__ p, const float* __restrict__ q,
-                                                 double* __restrict__ x, float* __restrict__ r, BState S,
-                                                 double* part) {
+// ---- x += alpha p (FP64), r = r - alpha q (FP32), r.r; then the next
+//      iteration's preconditioner on the new r while it is in registers:
+//      z = D^-1 r with r.z and z.q (q of this iteration, the next beta's
+//      q_prev), the values and partials kb_diag would give
+__global__ __launch_bounds__(kBT) void kb_update_diag(Tiles T, const float* __restrict__ alu,
+                                                      const float* __restrict__ p, const float* __restrict__ q,
+                                                      double* __restrict__ x, float* __restrict__ r,
+                                                      float* __restrict__ z, BState S, double* part_rr,
+                                                      double* part_rz) {
   const int t = blockIdx.x;
   const int s = T.sys[t];
   if (S.sys_active[s] == 0) return;
@@ -155,19 +174,33 @@ __global__ __launch_bounds__(kBT) void kb_update(Tiles T, const float* __restric
   const int id = s * KS + k;
   const bool live = !S.done[id];
   const double alpha = S.alpha[id];
-  double acc[1] = {0.0};
+  double acc[1] = {0.0}, acc2[2] = {0.0, 0.0};
   const int r0 = T.row0[t], nr = T.nrows[t];
   for (int i = rl; i < nr && live; i += kBT / KS) {
-    const int64_t o = ((int64_t)(r0 + i) * KS + k) * 3;
+    const int64_t g = r0 + i;
+    const int64_t o = (g * KS + k) * 3;
+    float v[3], qv[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
+      qv[d] = q[o + d];
       x[o + d] = x[o + d] + alpha * (double)p[o + d];
-      const float v = (float)((double)r[o + d] - alpha * (double)q[o + d]);
-      r[o + d] = v;
-      acc[0] += (double)v * (double)v;
+      v[d] = (float)((double)r[o + d] - alpha * (double)qv[d]);
+      r[o + d] = v[d];
+      acc[0] += (double)v[d] * (double)v[d];
+    }
+    float x0, x1, x2;
+    kb_jacobi(alu, g, v[0], v[1], v[2], x0, x1, x2);
+    z[o] = x0;
+    z[o + 1] = x1;
+    z[o + 2] = x2;
+    {
+#pragma clang fp contract(off)
+      acc2[0] += (double)v[0] * x0 + (double)v[1] * x1 + (double)v[2] * x2;
+      acc2[1] += (double)qv[0] * x0 + (double)qv[1] * x1 + (double)qv[2] * x2;
     }
   }
-  tile_partials_rowlanes<1>(acc, part, T.ntiles, t);
+  tile_partials_rowlanes<1>(acc, part_rr, T.ntiles, t);
+  tile_partials_rowlanes<2>(acc2, part_rz, T.ntiles, t);
 }
 
 // ---- SpMM ----------------------------------------------------------------------------
@@ -628,15 +661,20 @@ static void batch_enqueue_iteration(hecmw33_batch& B) {
   const int nt = B.ntiles;
   double* part = B.part.as<double>();
   float *r = B.r.as<float>(), *z = B.z.as<float>(), *p = B.p.as<float>(), *q = B.q.as<float>();
-  kb_diag<<<nt, kBT, 0, st>>>(B.T, B.alu32.as<float>(), r, z, q, B.S, part);
-  kb_tail<kbBeta, 2><<<B.nsys, kBT, 0, st>>>(B.T, B.S, part);
+  // z = D^-1 r and its dots (r.z, z.q) come from the previous iteration's
+  // kb_update_diag (or kb_diag after a residual replacement, or before the
+  // first iteration), in the second half of the partials
+  double* part_rz = part + (size_t)2 * KS * B.ntiles;
+  kb_tail<kbBeta, 2><<<B.nsys, kBT, 0, st>>>(B.T, B.S, part_rz);
   kb_p<<<nt, kBT, 0, st>>>(B.T, z, p, B.S);
   kVariants[B.variant].q32(nt, st, B.T, B.rowptr.as<int32_t>(), B.col.as<int32_t>(), B.val32.p, p, r, q, B.S, part, 1);
   kb_tail<kbAlpha, 2><<<B.nsys, kBT, 0, st>>>(B.T, B.S, part);
-  kb_update<<<nt, kBT, 0, st>>>(B.T, p, q, B.x.as<double>(), r, B.S, part);
+  kb_update_diag<<<nt, kBT, 0, st>>>(B.T, B.alu32.as<float>(), p, q, B.x.as<double>(), r, z, B.S, part, part_rz);
   kb_tail<kbRes, 1><<<B.nsys, kBT, 0, st>>>(B.T, B.S, part);
   spmm_launch<double, 16, 1, 2, kbEpiRes>(nt, st, B.T, B.rowptr.as<int32_t>(), B.col.as<int32_t>(), B.val64.p,
                                           B.x.p, B.b.p, r, B.S, part, 2);
+  // systems whose residuals were just replaced: z from the new r
+  kb_diag<<<nt, kBT, 0, st>>>(B.T, B.alu32.as<float>(), r, z, q, B.S, part_rz, 2);
   kb_tail<kbRel, 1><<<B.nsys, kBT, 0, st>>>(B.T, B.S, part);
 }
 
@@ -789,7 +827,7 @@ static int32_t batch_load(hecmw33_batch& B, int32_t nsys, const hecmw33_sys* sys
       B.sys_ntile.ensure(sizeof(int32_t) * nsys) || B.b.ensure(sizeof(double) * nv) ||
       B.x.ensure(sizeof(double) * nv) || B.r.ensure(sizeof(float) * nv) || B.z.ensure(sizeof(float) * nv) ||
       B.p.ensure(sizeof(float) * nv) || B.q.ensure(sizeof(float) * nv) ||
-      B.part.ensure(sizeof(double) * 2 * KS * (size_t)B.ntiles) || batch_state_layout(B))
+      B.part.ensure(sizeof(double) * 4 * KS * (size_t)B.ntiles) || batch_state_layout(B))
     return HECMW33_ERROR_DEVICE;
   hipStream_t st = B.stream;
   BHIPC(hipMemcpyAsync(B.rowptr.p, rp.data(), sizeof(int32_t) * rp.size(), hipMemcpyHostToDevice, st));
@@ -875,6 +913,9 @@ static int32_t batch_solve(hecmw33_batch& B, const int32_t* iarray, const double
   spmm_launch<double, 16, 1, 2, kbEpiRes>(B.ntiles, st, B.T, B.rowptr.as<int32_t>(), B.col.as<int32_t>(), B.val64.p,
                                           B.x.p, B.b.p, B.r.p, B.S, part + (size_t)KS * B.ntiles, 0);
   kb_tail<kbInit, 2><<<B.nsys, kBT, 0, st>>>(B.T, B.S, part);
+  // the first iteration's z = D^-1 r0 and its dots
+  kb_diag<<<B.ntiles, kBT, 0, st>>>(B.T, B.alu32.as<float>(), B.r.as<float>(), B.z.as<float>(), B.q.as<float>(), B.S,
+                                    part + (size_t)2 * KS * B.ntiles, 0);
   BHIPC(hipGetLastError());
   BHIPC(hipStreamSynchronize(st));
   const auto t1 = std::chrono::steady_clock::now();
