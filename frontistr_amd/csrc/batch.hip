@@ -576,7 +576,7 @@ struct BMem {
   size_t bytes = 0;
   int ensure(size_t n) {
     if (n <= bytes && p) return 0;
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
     if (n == 0) n = 8;
@@ -588,7 +588,7 @@ struct BMem {
     return 0;
   }
   void release() {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
   }
@@ -621,13 +621,13 @@ struct hecmw33_batch {
   double spmm_ms = 0.0;
 
   ~hecmw33_batch() {
-    if (stream) hipStreamSynchronize(stream);
-    if (graph) hipGraphExecDestroy(graph);
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (graph) (void)hipGraphExecDestroy(graph);
     for (BMem* m : {&rowptr, &col, &val64, &val32, &alu32, &d64, &tile_row0, &tile_nrows, &tile_sys, &sys_tile0,
                     &sys_ntile, &b, &x, &r, &z, &p, &q, &part, &state})
       m->release();
-    if (h_active) hipHostFree(h_active);
-    if (stream) hipStreamDestroy(stream);
+    if (h_active) (void)hipHostFree(h_active);
+    if (stream) (void)hipStreamDestroy(stream);
   }
 };
 
@@ -709,8 +709,8 @@ static int32_t batch_autotune(hecmw33_batch& B) {
     }
   }
   B.spmm_ms = best;
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   BHIPC(hipGetLastError());
   return 0;
 }
@@ -731,7 +731,7 @@ static int32_t batch_load(hecmw33_batch& B, int32_t nsys, const hecmw33_sys* sys
     BHIPC(hipHostMalloc((void**)&B.h_active, 2 * sizeof(int32_t), hipHostMallocDefault));
   }
   if (B.graph) {
-    hipGraphExecDestroy(B.graph);
+    (void)hipGraphExecDestroy(B.graph);
     B.graph = nullptr;
   }
   B.nsys = nsys;
@@ -927,7 +927,7 @@ static int32_t batch_solve(hecmw33_batch& B, const int32_t* iarray, const double
     kb_count<<<1, 64, 0, st>>>(B.nsys, B.S);
     BHIPC(hipStreamEndCapture(st, &g));
     BHIPC(hipGraphInstantiate(&B.graph, g, nullptr, nullptr, 0));
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
   }
   hipEvent_t ev[2];
   BHIPC(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
@@ -947,8 +947,8 @@ static int32_t batch_solve(hecmw33_batch& B, const int32_t* iarray, const double
     }
   }
   BHIPC(hipStreamSynchronize(st));
-  hipEventDestroy(ev[0]);
-  hipEventDestroy(ev[1]);
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
   const auto t2 = std::chrono::steady_clock::now();
   const int64_t ni = (int64_t)B.nsys * KS;
   std::vector<int32_t> it(ni), er(ni);

@@ -1458,7 +1458,7 @@ void cg_xpay(int64_t n3, const double* z, double* p, const KrylovState* st, hipS
 void cg_spmv_dot(const Sell<double>& a, const double* p, double* q, KrylovState* st, double* part, hipStream_t s,
                  hipEvent_t after, const MpcOp* mpc) {
   const int g = spmv_any<kEpiDotX>(s, a, mpc, p, nullptr, q, st, part);
-  if (after) hipEventRecord(after, s);
+  if (after) (void)hipEventRecord(after, s);
   k_tail<kTailCgC1, 1><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, nullptr);
 }
 void cg_update(int64_t n3, const double* p, const double* q, double* x, double* r, KrylovState* st, double* part,
@@ -1487,7 +1487,7 @@ void bi_p(int64_t n3, const double* r, const double* v, double* p, const KrylovS
 void bi_spmv_c2(const Sell<double>& a, const double* pt, const double* rt, double* v, KrylovState* st, double* part,
                 hipStream_t s, hipEvent_t after, const MpcOp* mpc) {
   const int g = spmv_any<kEpiDotW>(s, a, mpc, pt, rt, v, st, part);
-  if (after) hipEventRecord(after, s);
+  if (after) (void)hipEventRecord(after, s);
   k_tail<kTailBiC2, 1><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, nullptr);
 }
 void bi_s(int64_t n3, const double* v, const double* r, double* sv, const KrylovState* st, hipStream_t s) {
@@ -1496,7 +1496,7 @@ void bi_s(int64_t n3, const double* v, const double* r, double* sv, const Krylov
 void bi_spmv_omega(const Sell<double>& a, const double* stv, const double* sv, double* t, KrylovState* st,
                    double* part, hipStream_t s, hipEvent_t after, const MpcOp* mpc) {
   const int g = spmv_any<kEpiDot2W>(s, a, mpc, stv, sv, t, st, part);
-  if (after) hipEventRecord(after, s);
+  if (after) (void)hipEventRecord(after, s);
   k_tail<kTailBiOmega, 2><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, nullptr);
 }
 void bi_update(int64_t n3, const double* pt, const double* stv, const double* t, const double* sv, double* x,
@@ -1592,14 +1592,14 @@ void fcg_spmv(const Sell<float>& a, const float* p, const float* r, float* q, Kr
               hipStream_t s, hipEvent_t after) {
   const int g = spmv_grid(a.nslices);
   k_spmv_f32<true><<<g, kBlock, 0, s>>>(a, p, r, q, st, part);
-  if (after) hipEventRecord(after, s);
+  if (after) (void)hipEventRecord(after, s);
   k_tail<kTailFcgAlpha, 2><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, nullptr);
 }
 void fcg_spmv_mixed(const Sell<double>& a, const float* p, const float* r, float* q, KrylovState* st, double* part,
                     hipStream_t s, hipEvent_t after) {
   const int g = spmv_grid(a.nslices);
   k_spmv_mixed<true><<<g, kBlock, 0, s>>>(a, p, r, q, st, part);
-  if (after) hipEventRecord(after, s);
+  if (after) (void)hipEventRecord(after, s);
   k_tail<kTailFcgAlpha, 2><<<1, kTailThreads, 0, s>>>(st, part, g, nullptr, nullptr);
 }
 void fcg_update(int64_t n3, const float* p, const float* q, double* x, float* r, KrylovState* st, double* part,

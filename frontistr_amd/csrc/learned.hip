@@ -469,7 +469,7 @@ __global__ __launch_bounds__(kNT) void k_energy_dots(int64_t n3, const float* __
   block_sum<3>(acc, part, blockIdx.x);
 }
 __global__ void k_sum_parts(int nparts, int K, const double* __restrict__ part, double* __restrict__ out) {
-  if (threadIdx.x < K) {
+  if ((int)threadIdx.x < K) {
     double s = 0.0;
     for (int p = 0; p < nparts; ++p) s += part[(int64_t)threadIdx.x * kMaxPartials + p];
     out[threadIdx.x] = s;
