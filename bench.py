@@ -432,8 +432,9 @@ def bench_batch(args):
     blocks = info["blocks"]
     spmm_bytes = 9 * 4 * blocks + 4 * blocks + 4 * (info["rows"] + 1) + 16 * 3 * 4 * 2 * info["rows"]
     spmm_ms = info["variant_us"][info["variant"]] / 1e3
-    log("C4: %.3fs, iterations %d..%d, max resid %.2e, errors %d, SpMM %s %.3f ms" % (
-        wall, it.min(), it.max(), rs.max(), int((er != 0).sum()), info["variant"], spmm_ms))
+    log("C4: %.3fs, iterations %d..%d, max resid %.2e, errors %d, SpMM %s %.3f ms, FP64 residual SpMM %s %.3f ms" % (
+        wall, it.min(), it.max(), rs.max(), int((er != 0).sum()), info["variant"], spmm_ms, info["variant64"],
+        info["variant64_us"][info["variant64"]] / 1e3))
     bs.close()
     # the same instances one at a time through the single-system path (first
     # right-hand side of every system, x 16): FP64 DIAG-CG (the reference's
@@ -468,6 +469,7 @@ def bench_batch(args):
             "one_at_a_time": seq,
             "speedup_vs_one_at_a_time_fp64": round(seq["fp64_diag_cg"]["seconds_all_rhs_est"] / wall, 2),
             "spmm_variant": info["variant"], "spmm_ms": spmm_ms,
+            "spmm64_variant": info["variant64"], "spmm64_res_ms": info["variant64_us"][info["variant64"]] / 1e3,
             "spmm_GBs": round(spmm_bytes / (spmm_ms * 1e-3) / 1e9, 1), "load_s": round(t_load, 2)}
 
 

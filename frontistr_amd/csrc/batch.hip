@@ -553,9 +553,30 @@ const Variant kVariants[] = {
     HECMW33_VARIANT(16, 1, 1), HECMW33_VARIANT(16, 1, 2), HECMW33_VARIANT(8, 2, 1), HECMW33_VARIANT(8, 2, 2),
     HECMW33_VARIANT(4, 4, 1),  HECMW33_VARIANT(4, 4, 2),  HECMW33_VARIANT(32, 1, 1), HECMW33_VARIANT(32, 1, 2),
     HECMW33_VARIANT(64, 1, 1), HECMW33_VARIANT(16, 2, 1), HECMW33_VARIANT(32, 2, 1), HECMW33_VARIANT(64, 4, 1),
+    HECMW33_VARIANT(16, 4, 1), HECMW33_VARIANT(32, 4, 1), HECMW33_VARIANT(8, 4, 1),  HECMW33_VARIANT(16, 8, 1),
+    HECMW33_VARIANT(32, 8, 1), HECMW33_VARIANT(8, 8, 1),  HECMW33_VARIANT(4, 8, 1),
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 #undef HECMW33_VARIANT
+
+// FP64 SpMM shapes (residual replacement, products on request): one phase per
+// row (LPR = 16 / RPL), so every (row, rhs) sums its blocks in storage order
+// and the result is bitwise the reference's SpMV whichever shape runs.
+struct Variant64 {
+  const char* name;
+  SpmmFn res, st;
+};
+#define HECMW33_VARIANT64(L, R, U)                                                                   \
+  Variant64 {                                                                                        \
+    "lpr" #L "_rpl" #R "_u" #U, &spmm_launch<double, L, R, U, kbEpiRes>,                             \
+        &spmm_launch<double, L, R, U, kbEpiStore>                                                    \
+  }
+const Variant64 kVariants64[] = {
+    HECMW33_VARIANT64(16, 1, 2), HECMW33_VARIANT64(16, 1, 1), HECMW33_VARIANT64(8, 2, 1), HECMW33_VARIANT64(8, 2, 2),
+    HECMW33_VARIANT64(4, 4, 1),  HECMW33_VARIANT64(4, 4, 2),  HECMW33_VARIANT64(2, 8, 1),
+};
+constexpr int kNumVariants64 = sizeof(kVariants64) / sizeof(kVariants64[0]);
+#undef HECMW33_VARIANT64
 
 }  // namespace
 
@@ -616,6 +637,8 @@ struct hecmw33_batch {
   Tiles T{};
   int variant = -1;
   float variant_ms[64] = {};
+  int variant64 = 0;
+  float variant64_ms[16] = {};
   hipGraphExec_t graph = nullptr;
   int32_t* h_active = nullptr;  // pinned, 2 slots
   double spmm_ms = 0.0;
@@ -671,8 +694,8 @@ static void batch_enqueue_iteration(hecmw33_batch& B) {
   kb_tail<kbAlpha, 2><<<B.nsys, kBT, 0, st>>>(B.T, B.S, part);
   kb_update_diag<<<nt, kBT, 0, st>>>(B.T, B.alu32.as<float>(), p, q, B.x.as<double>(), r, z, B.S, part, part_rz);
   kb_tail<kbRes, 1><<<B.nsys, kBT, 0, st>>>(B.T, B.S, part);
-  spmm_launch<double, 16, 1, 2, kbEpiRes>(nt, st, B.T, B.rowptr.as<int32_t>(), B.col.as<int32_t>(), B.val64.p,
-                                          B.x.p, B.b.p, r, B.S, part, 2);
+  kVariants64[B.variant64].res(nt, st, B.T, B.rowptr.as<int32_t>(), B.col.as<int32_t>(), B.val64.p, B.x.p, B.b.p, r,
+                               B.S, part, 2);
   // systems whose residuals were just replaced: z from the new r
   kb_diag<<<nt, kBT, 0, st>>>(B.T, B.alu32.as<float>(), r, z, q, B.S, part_rz, 2);
   kb_tail<kbRel, 1><<<B.nsys, kBT, 0, st>>>(B.T, B.S, part);
@@ -709,6 +732,33 @@ static int32_t batch_autotune(hecmw33_batch& B) {
     }
   }
   B.spmm_ms = best;
+  // the FP64 shapes on the residual replacement of every instance (all
+  // flagged), from x = 0; the solve starts from 0 and rewrites r
+  const int64_t nv = B.nrows * KS * 3;
+  BHIPC(hipMemsetAsync(B.x.p, 0, sizeof(double) * nv, B.stream));
+  std::vector<int32_t> rel((size_t)B.nsys * KS, 1);
+  BHIPC(hipMemcpyAsync(B.S.rel, rel.data(), sizeof(int32_t) * rel.size(), hipMemcpyHostToDevice, B.stream));
+  best = 1e30f;
+  B.variant64 = 0;
+  for (int v = 0; v < kNumVariants64; ++v) {
+    kVariants64[v].res(B.ntiles, B.stream, B.T, B.rowptr.as<int32_t>(), B.col.as<int32_t>(), B.val64.p, B.x.p,
+                       B.b.p, B.r.p, B.S, B.part.as<double>(), 0);
+    const int reps = 3;
+    BHIPC(hipEventRecord(e0, B.stream));
+    for (int rep = 0; rep < reps; ++rep)
+      kVariants64[v].res(B.ntiles, B.stream, B.T, B.rowptr.as<int32_t>(), B.col.as<int32_t>(), B.val64.p, B.x.p,
+                         B.b.p, B.r.p, B.S, B.part.as<double>(), 0);
+    BHIPC(hipEventRecord(e1, B.stream));
+    BHIPC(hipEventSynchronize(e1));
+    float ms = 0.f;
+    BHIPC(hipEventElapsedTime(&ms, e0, e1));
+    ms /= reps;
+    B.variant64_ms[v] = ms;
+    if (ms < best) {
+      best = ms;
+      B.variant64 = v;
+    }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   BHIPC(hipGetLastError());
@@ -910,8 +960,8 @@ static int32_t batch_solve(hecmw33_batch& B, const int32_t* iarray, const double
   BHIPC(hipMemcpyAsync(B.S.rel, ones.data(), sizeof(int32_t) * ones.size(), hipMemcpyHostToDevice, st));
   BHIPC(hipMemcpyAsync(B.S.sys_rel, sys1.data(), sizeof(int32_t) * B.nsys, hipMemcpyHostToDevice, st));
   // |b - A x0|^2 into the second dot slot
-  spmm_launch<double, 16, 1, 2, kbEpiRes>(B.ntiles, st, B.T, B.rowptr.as<int32_t>(), B.col.as<int32_t>(), B.val64.p,
-                                          B.x.p, B.b.p, B.r.p, B.S, part + (size_t)KS * B.ntiles, 0);
+  kVariants64[B.variant64].res(B.ntiles, st, B.T, B.rowptr.as<int32_t>(), B.col.as<int32_t>(), B.val64.p, B.x.p,
+                               B.b.p, B.r.p, B.S, part + (size_t)KS * B.ntiles, 0);
   kb_tail<kbInit, 2><<<B.nsys, kBT, 0, st>>>(B.T, B.S, part);
   // the first iteration's z = D^-1 r0 and its dots
   kb_diag<<<B.ntiles, kBT, 0, st>>>(B.T, B.alu32.as<float>(), B.r.as<float>(), B.z.as<float>(), B.q.as<float>(), B.S,
@@ -1012,7 +1062,7 @@ int32_t hecmw33_batch_get_x(hecmw33_batch* b, int32_t s, double* X) {
 // columns of 3*N_s each).  fp64 != 0: FP64 in the reference's order;
 // otherwise the FP32 SpMM of the tuned variant (values rounded to FP32).
 int32_t hecmw33_batch_spmm(hecmw33_batch* b, int32_t fp64, const double* const* X, double* const* Y) {
-  if (!b || !b->stream || !X || !Y) return HECMW33_ERROR_ARGS;
+  if (!b || !b->stream || !X || !Y || fp64 < 0 || fp64 >= 2 + kNumVariants64) return HECMW33_ERROR_ARGS;
   const int64_t nv = b->nrows * KS * 3;
   std::vector<double> h(nv, 0.0);
   for (int s = 0; s < b->nsys; ++s) {
@@ -1026,8 +1076,9 @@ int32_t hecmw33_batch_spmm(hecmw33_batch* b, int32_t fp64, const double* const* 
   if (xin.ensure(sizeof(double) * nv) || yout.ensure(sizeof(double) * nv)) return HECMW33_ERROR_DEVICE;
   BHIPC(hipMemcpy(xin.p, h.data(), sizeof(double) * nv, hipMemcpyHostToDevice));
   if (fp64) {
-    spmm_launch<double, 16, 1, 2, kbEpiStore>(b->ntiles, st, b->T, b->rowptr.as<int32_t>(), b->col.as<int32_t>(),
-                                              b->val64.p, xin.p, nullptr, yout.p, b->S, nullptr, 0);
+    const int v = fp64 >= 2 ? fp64 - 2 : b->variant64;
+    kVariants64[v].st(b->ntiles, st, b->T, b->rowptr.as<int32_t>(), b->col.as<int32_t>(), b->val64.p, xin.p, nullptr,
+                      yout.p, b->S, nullptr, 0);
   } else {
     float* x32 = b->z.as<float>();  // scratch: the solve re-initialises it
     float* y32 = b->q.as<float>();
@@ -1054,14 +1105,21 @@ int32_t hecmw33_batch_spmm(hecmw33_batch* b, int32_t fp64, const double* const* 
 // [0] rows, [1] blocks, [2] tiles, [3] tuned variant, [4..4+n) variant times in us (FP32 SpMM + dots)
 int32_t hecmw33_batch_info(hecmw33_batch* b, int64_t* info, int32_t n) {
   if (!b || !info) return HECMW33_ERROR_ARGS;
-  std::vector<int64_t> v = {b->nrows, b->nnzb, b->ntiles, b->variant};
+  std::vector<int64_t> v = {b->nrows, b->nnzb, b->ntiles, b->variant, kNumVariants};
   for (int i = 0; i < kNumVariants; ++i) v.push_back((int64_t)std::llround(b->variant_ms[i] * 1000.0));
+  v.push_back(b->variant64);
+  v.push_back(kNumVariants64);
+  for (int i = 0; i < kNumVariants64; ++i) v.push_back((int64_t)std::llround(b->variant64_ms[i] * 1000.0));
   for (int i = 0; i < n && i < (int)v.size(); ++i) info[i] = v[i];
   return (int32_t)v.size();
 }
 
 const char* hecmw33_batch_variant_name(int32_t v) {
   return v >= 0 && v < kNumVariants ? kVariants[v].name : "";
+}
+
+const char* hecmw33_batch_variant64_name(int32_t v) {
+  return v >= 0 && v < kNumVariants64 ? kVariants64[v].name : "";
 }
 
 // One-shot: load, solve, copy every solution back.

@@ -92,6 +92,8 @@ def lib():
     L.hecmw33_batch_info.argtypes = [P, P, I]
     L.hecmw33_batch_variant_name.argtypes = [I]
     L.hecmw33_batch_variant_name.restype = C.c_char_p
+    L.hecmw33_batch_variant64_name.argtypes = [I]
+    L.hecmw33_batch_variant64_name.restype = C.c_char_p
     L.hecmw33_batch_last_error.restype = C.c_char_p
     L.hecmw33_solve_batch.argtypes = [I, P, I, P, P, P, P, P, P, P]
     L.hecmw33_net_create.argtypes = [C.c_uint64]
@@ -609,20 +611,29 @@ class BatchSolver:
         _bcheck(lib().hecmw33_batch_get_x(self._h, s, _ptr(X)))
         return X
 
-    def spmm(self, Xs, fp64=True):
+    def spmm(self, Xs, fp64=True, shape64=None):
+        """Y = A X per system; fp64 with shape64=None runs the tuned FP64 shape,
+        shape64=v FP64 shape v (all sum in the reference's order)."""
         Xs = [np.ascontiguousarray(x, dtype=np.float64) for x in Xs]
         Ys = [np.zeros_like(x) for x in Xs]
         xp = (C.c_void_p * len(Xs))(*[x.ctypes.data for x in Xs])
         yp = (C.c_void_p * len(Ys))(*[y.ctypes.data for y in Ys])
-        _bcheck(lib().hecmw33_batch_spmm(self._h, 1 if fp64 else 0, C.cast(xp, C.c_void_p), C.cast(yp, C.c_void_p)))
+        mode = (1 if shape64 is None else 2 + int(shape64)) if fp64 else 0
+        _bcheck(lib().hecmw33_batch_spmm(self._h, mode, C.cast(xp, C.c_void_p), C.cast(yp, C.c_void_p)))
         return Ys
 
     def info(self):
-        v = np.zeros(64, np.int64)
-        n = _bcheck(lib().hecmw33_batch_info(self._h, _ptr(v), 64))
-        names = [lib().hecmw33_batch_variant_name(i).decode() for i in range(n - 4)]
+        v = np.zeros(128, np.int64)
+        _bcheck(lib().hecmw33_batch_info(self._h, _ptr(v), 128))
+        n32 = int(v[4])
+        names = [lib().hecmw33_batch_variant_name(i).decode() for i in range(n32)]
+        o = 5 + n32
+        n64 = int(v[o + 1])
+        names64 = [lib().hecmw33_batch_variant64_name(i).decode() for i in range(n64)]
         return {"rows": int(v[0]), "blocks": int(v[1]), "tiles": int(v[2]), "variant": names[int(v[3])],
-                "variant_us": dict(zip(names, [int(t) for t in v[4:n]]))}
+                "variant_us": dict(zip(names, [int(t) for t in v[5:5 + n32]])),
+                "variant64": names64[int(v[o])],
+                "variant64_us": dict(zip(names64, [int(t) for t in v[o + 2:o + 2 + n64]]))}
 
 
 def solve_batch(mats, Bs, iarray, rarray):

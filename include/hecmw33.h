@@ -229,11 +229,17 @@ int32_t hecmw33_batch_load(hecmw33_batch* b, int32_t nsys, const hecmw33_sys* sy
 int32_t hecmw33_batch_solve(hecmw33_batch* b, const int32_t iarray[100], const double rarray[100], int32_t* iter,
                             int32_t* error, double* resid, double* times);
 int32_t hecmw33_batch_get_x(hecmw33_batch* b, int32_t s, double* X);
-/* Y_s = A_s X_s, host arrays; fp64 != 0: FP64 in hecmw_matvec_33's order, else the tuned FP32 SpMM */
+/* Y_s = A_s X_s, host arrays.  fp64 = 0: the tuned FP32 SpMM; 1: the tuned FP64
+ * shape; 2 + v: FP64 shape v.  Every FP64 shape sums each row's blocks in
+ * hecmw_matvec_33's order. */
 int32_t hecmw33_batch_spmm(hecmw33_batch* b, int32_t fp64, const double* const* X, double* const* Y);
-/* [0] rows, [1] blocks, [2] tiles, [3] tuned variant, [4..] per-variant FP32 SpMM time (us) */
+/* [0] rows, [1] blocks, [2] tiles, [3] tuned FP32 variant, [4] n32 FP32 variants,
+ * [5 .. 5+n32) their SpMM times (us, with the fused dots), [5+n32] tuned FP64
+ * shape, [6+n32] n64 FP64 shapes, [7+n32 .. 7+n32+n64) their residual-SpMM
+ * times (us).  Returns the number of entries. */
 int32_t hecmw33_batch_info(hecmw33_batch* b, int64_t* info, int32_t n);
 const char* hecmw33_batch_variant_name(int32_t v);
+const char* hecmw33_batch_variant64_name(int32_t v);
 const char* hecmw33_batch_last_error(void);
 /* one shot: load + solve + copy back */
 int32_t hecmw33_solve_batch(int32_t nsys, const hecmw33_sys* sys, int32_t nrhs, const double* const* B,

@@ -58,6 +58,13 @@ def test_spmm_fp64_bitwise_and_fp32_close():
     us = info["variant_us"]
     assert len(us) >= 8 and all(v >= 0 for v in us.values())
     assert us[info["variant"]] == min(us.values())
+    us64 = info["variant64_us"]
+    assert len(us64) >= 2 and us64[info["variant64"]] == min(us64.values())
+    # every FP64 shape, tuned or not, is bitwise the reference's SpMV
+    for v in range(len(us64)):
+        for m, X, y64 in zip(mats, Xs, bs.spmm(Xs, fp64=True, shape64=v)):
+            for k in range(3):
+                assert np.array_equal(y64[k], ref.matvec(m, X[k])), (v, k)
     bs.close()
 
 
