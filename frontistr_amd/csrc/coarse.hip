@@ -228,41 +228,112 @@ __global__ __launch_bounds__(kCoarseBlock) void k_coarse_prolong(int32_t n, cons
 // ---- symmetric two-level step: z1 = Z E^-1 Z^T r, z2 = z1 + M^-1 (r - A z1),
 //      z = z2 + Z E^-1 Z^T (r - A z2), with A Z kept per node and neighbour
 //      aggregate (W), so A z1 and the A Z part of A z need no SpMV ----------------
-// W[e] (3x6, FP32) for the e-th aggregate b touched by row i (node-major,
-// ncnt/nagg): sum of A_ij Z_j over the row's blocks with agg(j) = b.
-__global__ __launch_bounds__(kCoarseBlock) void k_coarse_W(Sell<double> A, int32_t n, const int32_t* __restrict__ agg,
-                                                            const int32_t* __restrict__ ncnt,
-                                                            const int32_t* __restrict__ nagg,
-                                                            const float* __restrict__ dv, float* __restrict__ W) {
-  const int64_t i = (int64_t)blockIdx.x * kCoarseBlock + threadIdx.x;
-  if (i >= n) return;
+// A Z of row i restricted to aggregate b: sum over the row's blocks j with
+// agg(j) = b of A_ij Z_j (3x6, FP64), and the row's scale
+// sum_j max|A_ij| (1 + max|d_j|), which bounds every component.
+__device__ __forceinline__ void row_AZ(const Sell<double>& A, int64_t i, int b, const int32_t* __restrict__ agg,
+                                       const float* __restrict__ dv, double (&m)[18], double& scale) {
   const int sl = (int)(i / A.rows), l = (int)(i % A.rows);
   const int64_t base = (int64_t)A.ptr[sl] * A.rows;
   const int len = A.len[i];
-  for (int e = ncnt[i]; e < ncnt[i + 1]; ++e) {
-    const int b = nagg[e];
-    double m[18];
 #pragma unroll
-    for (int k = 0; k < 18; ++k) m[k] = 0.0;
-    for (int k = 0; k < len; ++k) {
-      const int j = A.col[base + (int64_t)k * A.rows + l];
-      if (agg[j] != b) continue;
-      double v[9];
-      aval_load(A.val, (int64_t)A.ptr[sl] + k, l, v);
-      const double jx = dv[3 * (int64_t)j], jy = dv[3 * (int64_t)j + 1], jz = dv[3 * (int64_t)j + 2];
+  for (int k = 0; k < 18; ++k) m[k] = 0.0;
+  scale = 0.0;
+  for (int k = 0; k < len; ++k) {
+    const int j = A.col[base + (int64_t)k * A.rows + l];
+    if (agg[j] != b) continue;
+    double v[9];
+    aval_load(A.val, (int64_t)A.ptr[sl] + k, l, v);
+    const double jx = dv[3 * (int64_t)j], jy = dv[3 * (int64_t)j + 1], jz = dv[3 * (int64_t)j + 2];
+    double amax = 0.0;
 #pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        const double a0 = v[3 * r], a1 = v[3 * r + 1], a2 = v[3 * r + 2];
-        m[6 * r] += a0;
-        m[6 * r + 1] += a1;
-        m[6 * r + 2] += a2;
-        m[6 * r + 3] += -a1 * jz + a2 * jy;
-        m[6 * r + 4] += a0 * jz - a2 * jx;
-        m[6 * r + 5] += -a0 * jy + a1 * jx;
-      }
+    for (int e = 0; e < 9; ++e) amax = fmax(amax, fabs(v[e]));
+    scale += amax * (1.0 + fmax(fabs(jx), fmax(fabs(jy), fabs(jz))));
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const double a0 = v[3 * r], a1 = v[3 * r + 1], a2 = v[3 * r + 2];
+      m[6 * r] += a0;
+      m[6 * r + 1] += a1;
+      m[6 * r + 2] += a2;
+      m[6 * r + 3] += -a1 * jz + a2 * jy;
+      m[6 * r + 4] += a0 * jz - a2 * jx;
+      m[6 * r + 5] += -a0 * jy + a1 * jx;
     }
+  }
+}
+
+// A Z per node and touched aggregate, compacted for the iterations: node i's
+// entries are slots sc[i].x .. sc[i].x + sc[i].y - 1 of aggW (the aggregate)
+// and W (its 3x6 FP32 block).
+//  * A row whose blocks all lie in its own aggregate applies A to a whole
+//    rigid motion: zero in exact arithmetic unless a boundary condition
+//    changed the row.  What is left is the FP32 rounding of the offsets
+//    (about 1e-9 of the row's scale), the level at which the FP32 SpMV of
+//    the same iteration already works; such an entry is dropped when every
+//    component is below 1e-6 of the scale, so the iterations do not stream
+//    it.  Rows that touch other aggregates keep every entry.
+//  * Workgroup g compacts its 256 nodes inside their own symbolic slots
+//    (from ncnt[256 g] on), so no grid-wide scan is needed; kept[g] counts its
+//    entries.
+__global__ __launch_bounds__(kCoarseBlock) void k_coarse_W(Sell<double> A, int32_t n, const int32_t* __restrict__ agg,
+                                                            const int32_t* __restrict__ ncnt,
+                                                            const int32_t* __restrict__ nagg,
+                                                            const float* __restrict__ dv, int2* __restrict__ sc,
+                                                            int32_t* __restrict__ aggW, float* __restrict__ W,
+                                                            int32_t* __restrict__ kept) {
+  const int64_t i = (int64_t)blockIdx.x * kCoarseBlock + threadIdx.x;
+  const bool valid = i < n;
+  const int e0 = valid ? ncnt[i] : 0;
+  const int ne = valid ? ncnt[i + 1] - e0 : 0;
+  int cnt = ne;
+  float w1[18];
+  if (ne == 1) {
+    double m[18], scale;
+    row_AZ(A, i, nagg[e0], agg, dv, m, scale);
+    double mx = 0.0;
 #pragma unroll
-    for (int k = 0; k < 18; ++k) W[18 * (int64_t)e + k] = (float)m[k];
+    for (int k = 0; k < 18; ++k) {
+      mx = fmax(mx, fabs(m[k]));
+      w1[k] = (float)m[k];
+    }
+    cnt = mx > 1e-6 * scale ? 1 : 0;
+  }
+  // exclusive scan of cnt over the workgroup: per wave by shuffles, then the
+  // wave totals in order
+  __shared__ int wtot[kCoarseWaves];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  int inc = cnt;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const int t = __shfl_up(inc, off, kWave);
+    if (lane >= off) inc += t;
+  }
+  if (lane == kWave - 1) wtot[wave] = inc;
+  __syncthreads();
+  int before = 0, total = 0;
+#pragma unroll
+  for (int w = 0; w < kCoarseWaves; ++w) {
+    before += w < wave ? wtot[w] : 0;
+    total += wtot[w];
+  }
+  const int slot = ncnt[(int64_t)blockIdx.x * kCoarseBlock] + before + inc - cnt;
+  if (threadIdx.x == 0) kept[blockIdx.x] = total;
+  if (!valid) return;
+  sc[i] = make_int2(slot, cnt);
+  if (ne == 1) {
+    if (cnt) {
+      aggW[slot] = nagg[e0];
+#pragma unroll
+      for (int k = 0; k < 18; ++k) W[18 * (int64_t)slot + k] = w1[k];
+    }
+    return;
+  }
+  for (int t = 0; t < ne; ++t) {
+    double m[18], scale;
+    row_AZ(A, i, nagg[e0 + t], agg, dv, m, scale);
+    aggW[slot + t] = nagg[e0 + t];
+#pragma unroll
+    for (int k = 0; k < 18; ++k) W[18 * (int64_t)(slot + t) + k] = (float)m[k];
   }
 }
 
@@ -273,13 +344,14 @@ __device__ __forceinline__ void zy(const float* __restrict__ dv, const float* __
   o[2] = ya[2] + (ya[3] * dy - ya[4] * dx);
 }
 
-__device__ __forceinline__ void wy(const int32_t* __restrict__ ncnt, const int32_t* __restrict__ nagg,
+__device__ __forceinline__ void wy(const int2* __restrict__ sc, const int32_t* __restrict__ aggW,
                                    const float* __restrict__ W, const float* __restrict__ y, int64_t i,
                                    float (&o)[3]) {
   o[0] = o[1] = o[2] = 0.f;
-  for (int e = ncnt[i]; e < ncnt[i + 1]; ++e) {
+  const int2 r = sc[i];
+  for (int e = r.x; e < r.x + r.y; ++e) {
     const float* w = W + 18 * (int64_t)e;
-    const float* yb = y + 6 * nagg[e];
+    const float* yb = y + 6 * aggW[e];
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -290,8 +362,8 @@ __device__ __forceinline__ void wy(const int32_t* __restrict__ ncnt, const int32
 // z = Z y, r1 = r - (A Z) y
 __global__ __launch_bounds__(kCoarseBlock) void k_coarse_pre(int32_t n, const int32_t* __restrict__ agg,
                                                               const float* __restrict__ dv,
-                                                              const int32_t* __restrict__ ncnt,
-                                                              const int32_t* __restrict__ nagg,
+                                                              const int2* __restrict__ sc,
+                                                              const int32_t* __restrict__ aggW,
                                                               const float* __restrict__ W, const float* __restrict__ y,
                                                               const float* __restrict__ r, float* __restrict__ z,
                                                               float* __restrict__ r1, const KrylovState* st) {
@@ -299,7 +371,7 @@ __global__ __launch_bounds__(kCoarseBlock) void k_coarse_pre(int32_t n, const in
   for (int64_t i = (int64_t)blockIdx.x * kCoarseBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kCoarseBlock) {
     float zi[3], ai[3];
     zy(dv, y + 6 * agg[i], i, zi);
-    wy(ncnt, nagg, W, y, i, ai);
+    wy(sc, aggW, W, y, i, ai);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       z[3 * i + d] = zi[d];
@@ -347,8 +419,8 @@ __global__ __launch_bounds__(kCoarseBlock) void k_coarse_restrict2(const int32_t
 // z = z + Z y, az = w + (A Z) y, partials r.z and q_prev.z (the flexible CG's beta)
 __global__ __launch_bounds__(kCoarseBlock) void k_coarse_post(int32_t n, const int32_t* __restrict__ agg,
                                                                const float* __restrict__ dv,
-                                                               const int32_t* __restrict__ ncnt,
-                                                               const int32_t* __restrict__ nagg,
+                                                               const int2* __restrict__ sc,
+                                                               const int32_t* __restrict__ aggW,
                                                                const float* __restrict__ W,
                                                                const float* __restrict__ y, float* __restrict__ z,
                                                                const float* __restrict__ w, float* __restrict__ az,
@@ -360,7 +432,7 @@ __global__ __launch_bounds__(kCoarseBlock) void k_coarse_post(int32_t n, const i
   for (int64_t i = (int64_t)blockIdx.x * kCoarseBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kCoarseBlock) {
     float zi[3], ai[3];
     zy(dv, y + 6 * agg[i], i, zi);
-    wy(ncnt, nagg, W, y, i, ai);
+    wy(sc, aggW, W, y, i, ai);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const float zn = z[3 * i + k] + zi[k];
@@ -400,34 +472,36 @@ __global__ __launch_bounds__(kCoarseBlock) void k_fcg_pq(int64_t n3, const float
 }  // namespace
 
 void coarse_build_W(const Sell<double>& A, const CoarseOp& op, int32_t n, const int32_t* ncnt, const int32_t* nagg,
-                    float* W, hipStream_t s) {
-  if (n > 0) k_coarse_W<<<(n + kCoarseBlock - 1) / kCoarseBlock, kCoarseBlock, 0, s>>>(A, n, op.agg, ncnt, nagg, op.dv, W);
+                    int2* sc, int32_t* aggW, float* W, int32_t* kept, hipStream_t s) {
+  if (n > 0)
+    k_coarse_W<<<(n + kCoarseBlock - 1) / kCoarseBlock, kCoarseBlock, 0, s>>>(A, n, op.agg, ncnt, nagg, op.dv, sc, aggW,
+                                                                              W, kept);
 }
 
 static int coarse_vec_grid(int64_t n) {
   return (int)std::max<int64_t>(1, std::min<int64_t>((n + kCoarseBlock - 1) / kCoarseBlock, 4096));
 }
 
-void coarse_sym_pre(const CoarseOp& op, int32_t n, const int32_t* ncnt, const int32_t* nagg, const float* W,
+void coarse_sym_pre(const CoarseOp& op, int32_t n, const int2* sc, const int32_t* aggW, const float* W,
                     const float* r, float* z, float* r1, const KrylovState* st, hipStream_t s) {
   k_coarse_restrict<<<op.n_agg, kCoarseBlock, 0, s>>>(op.aptr, op.anodes, op.dv, r, op.cv, st);
   k_coarse_gemv<<<(op.nc + kCoarseWaves - 1) / kCoarseWaves, kCoarseBlock, 0, s>>>(op.nc, op.ldw, op.W, op.cv, op.yv,
                                                                                    st);
-  k_coarse_pre<<<coarse_vec_grid(n), kCoarseBlock, 0, s>>>(n, op.agg, op.dv, ncnt, nagg, W, op.yv, r, z, r1, st);
+  k_coarse_pre<<<coarse_vec_grid(n), kCoarseBlock, 0, s>>>(n, op.agg, op.dv, sc, aggW, W, op.yv, r, z, r1, st);
 }
 
 void coarse_sym_add(int64_t n3, const float* t, float* z, const KrylovState* st, hipStream_t s) {
   k_coarse_add<<<coarse_vec_grid(n3), kCoarseBlock, 0, s>>>(n3, t, z, st);
 }
 
-int coarse_sym_post(const CoarseOp& op, int32_t n, const int32_t* ncnt, const int32_t* nagg, const float* W,
+int coarse_sym_post(const CoarseOp& op, int32_t n, const int2* sc, const int32_t* aggW, const float* W,
                     const float* r, const float* w, float* z, float* az, const float* q, const KrylovState* st,
                     double* part, hipStream_t s) {
   k_coarse_restrict2<<<op.n_agg, kCoarseBlock, 0, s>>>(op.aptr, op.anodes, op.dv, r, w, op.cv, st);
   k_coarse_gemv<<<(op.nc + kCoarseWaves - 1) / kCoarseWaves, kCoarseBlock, 0, s>>>(op.nc, op.ldw, op.W, op.cv, op.yv,
                                                                                    st);
   const int g = coarse_vec_grid(n);
-  k_coarse_post<<<g, kCoarseBlock, 0, s>>>(n, op.agg, op.dv, ncnt, nagg, W, op.yv, z, w, az, r, q, st, part);
+  k_coarse_post<<<g, kCoarseBlock, 0, s>>>(n, op.agg, op.dv, sc, aggW, W, op.yv, z, w, az, r, q, st, part);
   return g;
 }
 

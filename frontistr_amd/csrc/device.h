@@ -273,16 +273,19 @@ struct CoarseOp {
 void coarse_assemble(const Sell<double>& A, const CoarseOp& op, int32_t npair, const int32_t* pa, const int32_t* pb,
                      const int32_t* pn_ptr, const int32_t* pn, double* Eblk, double* E, int32_t lda, hipStream_t s);
 void coarse_inverse_f32(const CoarseOp& op, const double* Ei, int32_t lda, hipStream_t s);
-// The symmetric two-level step (W = A Z per node and touched aggregate, node-major ncnt/nagg):
-// build W; pre: z = Z E^-1 Z^T r, r1 = r - A z; add: z += t (the SSOR output of r1);
+// The symmetric two-level step (W = A Z per node and touched aggregate):
+// build W from the symbolic node-major lists ncnt/nagg into the compacted
+// sc (per node: first slot, count) / aggW / W, dropping the entries that are
+// zero in exact arithmetic, with kept[g] entries per 256-node workgroup;
+// pre: z = Z E^-1 Z^T r, r1 = r - A z; add: z += t (the SSOR output of r1);
 // post: with w = A z, z += Z E^-1 Z^T (r - w), az = w + A Z E^-1 Z^T (r - w) = A z, partials
 // (r.z, q_prev.z); pq: p = z + beta p, q = az + beta q, partials (p.q, r.p).
 void coarse_build_W(const Sell<double>& A, const CoarseOp& op, int32_t n, const int32_t* ncnt, const int32_t* nagg,
-                    float* W, hipStream_t s);
-void coarse_sym_pre(const CoarseOp& op, int32_t n, const int32_t* ncnt, const int32_t* nagg, const float* W,
+                    int2* sc, int32_t* aggW, float* W, int32_t* kept, hipStream_t s);
+void coarse_sym_pre(const CoarseOp& op, int32_t n, const int2* sc, const int32_t* aggW, const float* W,
                     const float* r, float* z, float* r1, const KrylovState* st, hipStream_t s);
 void coarse_sym_add(int64_t n3, const float* t, float* z, const KrylovState* st, hipStream_t s);
-int coarse_sym_post(const CoarseOp& op, int32_t n, const int32_t* ncnt, const int32_t* nagg, const float* W,
+int coarse_sym_post(const CoarseOp& op, int32_t n, const int2* sc, const int32_t* aggW, const float* W,
                     const float* r, const float* w, float* z, float* az, const float* q, const KrylovState* st,
                     double* part, hipStream_t s);
 int coarse_fcg_pq(int64_t n3, const float* z, const float* az, float* p, float* q, const float* r,

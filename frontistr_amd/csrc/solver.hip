@@ -268,7 +268,7 @@ struct Context {
   bool coarse_w_valid = false;    // A Z (per node and touched aggregate) built for the current values
   int64_t coarse_nw = 0;          // node-aggregate entries of A Z
   DMem cg_agg, cg_aptr, cg_anodes, cg_dv, cg_pa, cg_pb, cg_pnptr, cg_pn, cg_Eblk, cg_E, cg_W, cg_cv, cg_yv, cg_info;
-  DMem cg_ncnt, cg_nagg, cg_AZ, vr1_32, vt32, vw32, vaz32;
+  DMem cg_ncnt, cg_nagg, cg_AZ, cg_sc, cg_aggW, cg_kept, vr1_32, vt32, vw32, vaz32;
   rocblas_handle blas = nullptr;
   // multipoint constraints (hecMESH%mpc) for the MPC-CG product T^T A T
   // (Iarray 83 set, as hecmw_solve_iterative sets it for MPC_METHOD = 2)
@@ -309,7 +309,7 @@ struct Context {
                     &vx0})
       m->release();
     for (DMem* m : {&cg_agg, &cg_aptr, &cg_anodes, &cg_dv, &cg_pa, &cg_pb, &cg_pnptr, &cg_pn, &cg_Eblk, &cg_E, &cg_W,
-                    &cg_cv, &cg_yv, &cg_info, &cg_ncnt, &cg_nagg, &cg_AZ, &vr1_32, &vt32, &vw32, &vaz32})
+                    &cg_cv, &cg_yv, &cg_info, &cg_ncnt, &cg_nagg, &cg_AZ, &cg_sc, &cg_aggW, &cg_kept, &vr1_32, &vt32, &vw32, &vaz32})
       m->release();
     if (blas) rocblas_destroy_handle(blas);
     for (DMem* m : {&mpc_slave, &mpc_mptr, &mpc_mdof, &mpc_mval, &mpc_tptr, &mpc_tslave, &mpc_tval, &mpc_w1, &mpc_w2})
@@ -1135,18 +1135,18 @@ static void enqueue_fcg_sym_iteration(Context& c, int32_t kind, bool spmv64, hip
   double* part = c.part.as<double>();
   float *r = c.vr32.as<float>(), *z = c.vz32.as<float>(), *p = c.vp32.as<float>(), *q = c.vq32.as<float>();
   float *r1 = c.vr1_32.as<float>(), *t = c.vt32.as<float>(), *w = c.vw32.as<float>(), *az = c.vaz32.as<float>();
-  const int32_t* ncnt = c.cg_ncnt.as<int32_t>();
-  const int32_t* nagg = c.cg_nagg.as<int32_t>();
+  const int2* sc = c.cg_sc.as<int2>();
+  const int32_t* aggW = c.cg_aggW.as<int32_t>();
   const float* AZ = c.cg_AZ.as<float>();
   if (ev) (void)hipEventRecord(ev[0], c.stream);
-  coarse_sym_pre(c.coarse, c.N, ncnt, nagg, AZ, r, z, r1, st, c.stream);
+  coarse_sym_pre(c.coarse, c.N, sc, aggW, AZ, r, z, r1, st, c.stream);
   precond_apply_f32(kind, c.ssor_op32(), c.alu_diag32.as<float>(), c.N, r1, t, st, c.stream);
   coarse_sym_add(n3, t, z, st, c.stream);
   if (ev) (void)hipEventRecord(ev[1], c.stream);
   if (ev) (void)hipEventRecord(ev[2], c.stream);
   fcg_spmv_store(c.A.view32, c.A.view, spmv64, z, w, st, c.stream);
   if (ev) (void)hipEventRecord(ev[3], c.stream);
-  fcg_beta(st, part, coarse_sym_post(c.coarse, c.N, ncnt, nagg, AZ, r, w, z, az, q, st, part, c.stream), c.stream);
+  fcg_beta(st, part, coarse_sym_post(c.coarse, c.N, sc, aggW, AZ, r, w, z, az, q, st, part, c.stream), c.stream);
   fcg_alpha(st, part, coarse_fcg_pq(n3, z, az, p, q, r, st, part, c.stream), c.stream);
   fcg_update(n3, p, q, c.vx.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
   fcg_reliable(c.A.view, c.vx.as<double>(), c.vb.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
@@ -1437,9 +1437,13 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
       for (DMem* m : {&c.vr1_32, &c.vt32, &c.vw32, &c.vaz32})
         if (m->ensure(vb32)) return HECMW33_ERROR_DEVICE;
       if (!c.coarse_w_valid) {
-        if (c.cg_AZ.ensure(sizeof(float) * 18 * (size_t)std::max<int64_t>(c.coarse_nw, 1))) return HECMW33_ERROR_DEVICE;
-        coarse_build_W(c.A.view, c.coarse, c.N, c.cg_ncnt.as<int32_t>(), c.cg_nagg.as<int32_t>(),
-                       c.cg_AZ.as<float>(), c.stream);
+        const size_t nw = (size_t)std::max<int64_t>(c.coarse_nw, 1);
+        const size_t nblk = (size_t)std::max<int32_t>((c.N + 255) / 256, 1);
+        if (c.cg_AZ.ensure(sizeof(float) * 18 * nw) || c.cg_aggW.ensure(sizeof(int32_t) * nw) ||
+            c.cg_sc.ensure(sizeof(int2) * (size_t)std::max<int32_t>(c.N, 1)) || c.cg_kept.ensure(sizeof(int32_t) * nblk))
+          return HECMW33_ERROR_DEVICE;
+        coarse_build_W(c.A.view, c.coarse, c.N, c.cg_ncnt.as<int32_t>(), c.cg_nagg.as<int32_t>(), c.cg_sc.as<int2>(),
+                       c.cg_aggW.as<int32_t>(), c.cg_AZ.as<float>(), c.cg_kept.as<int32_t>(), c.stream);
         HIPC(hipGetLastError());
         c.coarse_w_valid = true;
       }
@@ -2028,7 +2032,17 @@ int32_t hecmw33_ctx_history(hecmw33_ctx* ctx, double* hist, int32_t cap) {
 int32_t hecmw33_ctx_info(hecmw33_ctx* ctx, int64_t* info, int32_t n) {
   if (!ctx || !info) return HECMW33_ERROR_ARGS;
   Context& c = ctx->c;
-  const int64_t v[13] = {c.A.view.nslots,
+  // A Z entries of the symmetric two-level step: kept after the last build, and symbolic
+  int64_t az_kept = 0, az_sym = 0;
+  if (c.coarse_w_valid && c.cg_kept.p && c.N > 0) {
+    const int32_t nblk = (c.N + 255) / 256;
+    std::vector<int32_t> kept(nblk);
+    HIPC(hipMemcpyAsync(kept.data(), c.cg_kept.p, sizeof(int32_t) * nblk, hipMemcpyDeviceToHost, c.stream));
+    HIPC(hipStreamSynchronize(c.stream));
+    for (int32_t k : kept) az_kept += k;
+    az_sym = c.coarse_nw;
+  }
+  const int64_t v[15] = {c.A.view.nslots,
                         (int64_t)c.N,
                         (int64_t)c.NPL,
                         (int64_t)c.NPU,
@@ -2040,8 +2054,10 @@ int32_t hecmw33_ctx_info(hecmw33_ctx* ctx, int64_t* info, int32_t n) {
                         (int64_t)c.last_spmv64,
                         (int64_t)std::llround(c.last_probe_ratio * 1e6),
                         (int64_t)c.last_coarse,
-                        (int64_t)c.coarse_npair};
-  for (int i = 0; i < n && i < 13; ++i) info[i] = v[i];
+                        (int64_t)c.coarse_npair,
+                        az_kept,
+                        az_sym};
+  for (int i = 0; i < n && i < 15; ++i) info[i] = v[i];
   return 0;
 }
 

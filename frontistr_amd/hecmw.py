@@ -510,11 +510,12 @@ class DeviceContext:
         return h[:n]
 
     def info(self):
-        v = np.zeros(13, np.int64)
-        _check(lib().hecmw33_ctx_info(self._h, _ptr(v), 13))
+        v = np.zeros(15, np.int64)
+        _check(lib().hecmw33_ctx_info(self._h, _ptr(v), 15))
         return {"nslots": int(v[0]), "N": int(v[1]), "NPL": int(v[2]), "NPU": int(v[3]), "ncolor": int(v[4]),
                 "ssor_slots": int(v[5]), "nreliable": int(v[6]), "nfallback": int(v[7]), "learned_off": int(v[8]),
-                "spmv64": int(v[9]), "probe_ratio": v[10] / 1e6, "coarse": int(v[11]), "coarse_pairs": int(v[12])}
+                "spmv64": int(v[9]), "probe_ratio": v[10] / 1e6, "coarse": int(v[11]), "coarse_pairs": int(v[12]),
+                "az_entries": int(v[13]), "az_entries_symbolic": int(v[14])}
 
     def coarse(self):
         """The last two-level set-up: dict(n_agg, n_pair, agg [N], dv [N,3], pair_a, pair_b [n_pair],

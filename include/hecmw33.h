@@ -196,7 +196,9 @@ int32_t hecmw33_ctx_history(hecmw33_ctx* ctx, double* hist, int32_t cap);
  * CG used the FP64 matrix values in q = A p, [10] the last learned probe's error reduction with
  * / without the correction, in millionths, [11] aggregates of the last flexible CG's two-level
  * correction (0: none, -1: E was not positive definite and the solve ran without it), [12] coupled
- * aggregate pairs (blocks of E). */
+ * aggregate pairs (blocks of E), [13] A Z entries the symmetric two-level step streams (after
+ * dropping the rows whose A Z is zero in exact arithmetic), [14] A Z entries before that (node and
+ * touched aggregate); 0 and 0 when the symmetric step has not been set up. */
 int32_t hecmw33_ctx_info(hecmw33_ctx* ctx, int64_t* info, int32_t n);
 /* the multipoint constraints of a device context (as hecmw33_set_mpc) */
 int32_t hecmw33_ctx_set_mpc(hecmw33_ctx* ctx, int32_t N, int32_t n_mpc, const int32_t* mpc_index,
