@@ -337,7 +337,10 @@ __device__ __forceinline__ float quad_bcast(float v) {
 // SpMV rows, SELL-64, one row per lane (kRowsA).  FP64 in the reference's
 // order: the D block first, then L, then U in storage order;
 // y = ((y + a1*x1) + a2*x2) + a3*x3, no fused multiply-add.  A chunk of U
-// blocks is loaded before it is accumulated in order.  xo = x of the row itself.
+// blocks is loaded before it is accumulated in order; the D block is slot 0
+// of the first chunk, so no load waits for a block of its own.  y starts at
+// -0, for which -0 + a is a exactly (also for a = -0), so the D block's sum is
+// the reference's (a1*x1 + a2*x2) + a3*x3.  xo = x of the row itself.
 template <int U>
 __device__ __forceinline__ void sell_row(const Sell<double>& a, int s, int lane, const double* __restrict__ x,
                                          double (&y)[3], double (&xo)[3]) {
@@ -347,20 +350,9 @@ __device__ __forceinline__ void sell_row(const Sell<double>& a, int s, int lane,
   y[0] = y[1] = y[2] = 0.0;
   xo[0] = xo[1] = xo[2] = 0.0;
   if (len == 0) return;
+  y[0] = y[1] = y[2] = -0.0;
   const int32_t* __restrict__ colp = a.col + (int64_t)base * kRowsA + lane;
-  {
-    const int c = colp[0];
-    double v[9];
-    aval_load(a.val, base, lane, v);
-    const double x0 = x[3 * (int64_t)c], x1 = x[3 * (int64_t)c + 1], x2 = x[3 * (int64_t)c + 2];
-    xo[0] = x0;
-    xo[1] = x1;
-    xo[2] = x2;
-    y[0] = v[0] * x0 + v[1] * x1 + v[2] * x2;
-    y[1] = v[3] * x0 + v[4] * x1 + v[5] * x2;
-    y[2] = v[6] * x0 + v[7] * x1 + v[8] * x2;
-  }
-  for (int k0 = 1; k0 < len; k0 += U) {
+  for (int k0 = 0; k0 < len; k0 += U) {
     // branch-free chunk: slots past the row's end re-read the last block and
     // are masked out, so the chunk's loads are all in flight together
     int kk[U], c[U];
@@ -376,6 +368,11 @@ __device__ __forceinline__ void sell_row(const Sell<double>& a, int s, int lane,
       xv[j][0] = x[3 * (int64_t)c[j]];
       xv[j][1] = x[3 * (int64_t)c[j] + 1];
       xv[j][2] = x[3 * (int64_t)c[j] + 2];
+    }
+    if (k0 == 0) {
+      xo[0] = xv[0][0];
+      xo[1] = xv[0][1];
+      xo[2] = xv[0][2];
     }
 #pragma unroll
     for (int j = 0; j < U; ++j) {
