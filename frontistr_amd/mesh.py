@@ -1,0 +1,274 @@
+"""fistr1 input models in hecmw's text formats, for the linear static systems
+the solver receives.
+
+The subset read is what fistr1's own solver tests use
+(tests/solver/CG/*.msh, *.cnt in the reference):
+
+* mesh (``.msh``, hecmw's native format): ``!NODE``, ``!ELEMENT, TYPE=341``
+  (4-node tetrahedron), ``!MATERIAL`` with ``!ITEM=1, SUBITEM=2`` (Young's
+  modulus, Poisson's ratio), ``!NGROUP`` (list or ``GENERATE`` start, end,
+  step); other blocks are skipped;
+* control (``.cnt``): ``!BOUNDARY`` (node or group, first dof, last dof,
+  value), ``!CLOAD`` (node or group, dof, value) and ``!SOLVER`` (METHOD,
+  PRECOND and the two data lines: ITER, ITERPREMAX / RESID, SIGMA_DIAG,
+  SIGMA);
+* result (``.res``, ``*fstrresult 2.0``): the nodal DISPLACEMENT of every node.
+
+:func:`assemble` builds the static system fistr1 hands to ``hecmw_solve``:
+
+* element stiffness of type 341 as ``STF_C3`` computes it
+  (fistr1/src/lib/static_LIB_3d.f90:50-208): small strain, isotropic
+  elasticity, one quadrature point, so K_e = V B^T D B with the constant
+  strain-displacement matrix B of the linear tetrahedron;
+* the nodes are those the elements use, in the order of ``!NODE`` (the mesh
+  reader drops nodes no element references), the block profile is every
+  node pair sharing an element with items ascending per row
+  (``hecmw_mat_con``, hecmw1/src/solver/matrix/hecmw_mat_con.f90:21-107);
+* Dirichlet values by row and column elimination with a unit pivot
+  (``hecmw_mat_ass_bc``, hecmw1/src/solver/matrix/hecmw_mat_ass.f90:328-463),
+  the right-hand side moved by the prescribed values;
+* concentrated loads added to the right-hand side.
+"""
+import re
+
+import numpy as np
+
+from .hecmw import HecMatrix
+
+
+def _blocks(path):
+    """(header keyword line, [data lines]) in file order; '!!' and '#' lines are comments."""
+    out = []
+    with open(path) as f:
+        for raw in f:
+            line = raw.strip()
+            if not line or line.startswith("!!") or line.startswith("#"):
+                continue
+            if line.startswith("!"):
+                out.append((line, []))
+            elif out:
+                out[-1][1].append(line)
+    return out
+
+
+def _opts(header):
+    """'!KEY, A=1, B' -> ('KEY', {'A': '1', 'B': ''})"""
+    parts = [p.strip() for p in header[1:].split(",")]
+    d = {}
+    for p in parts[1:]:
+        k, _, v = p.partition("=")
+        d[k.strip().upper()] = v.strip()
+    return parts[0].upper(), d
+
+
+def _fields(line):
+    return [t for t in re.split(r"[,\s]+", line.strip()) if t]
+
+
+def read_msh(path):
+    """Nodes {id: (x, y, z)} in file order, elements [(id, type, node ids)],
+    material (E, nu) and node groups {name: [ids]}."""
+    nodes, elems, groups = {}, [], {}
+    material = None
+    for header, lines in _blocks(path):
+        key, opt = _opts(header)
+        if key == "END":
+            break
+        if key == "NODE":
+            for ln in lines:
+                f = _fields(ln)
+                nodes[int(f[0])] = tuple(float(v) for v in f[1:4])
+        elif key == "ELEMENT":
+            etype = int(opt.get("TYPE", "0"))
+            for ln in lines:
+                f = [int(v) for v in _fields(ln)]
+                elems.append((f[0], etype, f[1:]))
+        elif key == "NGROUP":
+            ids = []
+            for ln in lines:
+                f = [int(v) for v in _fields(ln)]
+                ids += list(range(f[0], f[1] + 1, f[2] if len(f) > 2 else 1)) if "GENERATE" in opt else f
+            groups.setdefault(opt["NGRP"].upper(), []).extend(ids)
+        elif key.replace(" ", "") == "ITEM=1":
+            # '!ITEM=1, SUBITEM=2': Young's modulus and Poisson's ratio
+            if opt.get("SUBITEM", "2") != "2":
+                raise ValueError("%s: only isotropic elasticity (ITEM=1, SUBITEM=2) is read" % path)
+            f = [float(v) for v in _fields(lines[0])]
+            material = (f[0], f[1])
+    return nodes, elems, material, groups
+
+
+def read_cnt(path):
+    """Boundary conditions [(node or group, first dof, last dof, value)], loads
+    [(node or group, dof, value)] and solver settings."""
+    bcs, loads, solver = [], [], {}
+    for header, lines in _blocks(path):
+        key, opt = _opts(header)
+        if key == "END":
+            break
+        if key == "BOUNDARY":
+            for ln in lines:
+                f = _fields(ln)
+                bcs.append((f[0], int(f[1]), int(f[2]), float(f[3]) if len(f) > 3 else 0.0))
+        elif key == "CLOAD":
+            for ln in lines:
+                f = _fields(ln)
+                loads.append((f[0], int(f[1]), float(f[2])))
+        elif key == "SOLVER":
+            solver["METHOD"] = opt.get("METHOD", "CG").upper()
+            solver["PRECOND"] = int(opt.get("PRECOND", "1"))
+            if len(lines) > 0:
+                f = _fields(lines[0])
+                solver["ITER"] = int(f[0])
+                if len(f) > 1:
+                    solver["ITERPREMAX"] = int(f[1])
+            if len(lines) > 1:
+                f = [float(v) for v in _fields(lines[1])]
+                solver["RESID"] = f[0]
+                if len(f) > 1:
+                    solver["SIGMA_DIAG"] = f[1]
+                if len(f) > 2:
+                    solver["SIGMA"] = f[2]
+    return bcs, loads, solver
+
+
+def read_res_displacement(path):
+    """{node id: (ux, uy, uz)} from an ``*fstrresult 2.0`` file whose first
+    nodal field is DISPLACEMENT (3 components)."""
+    with open(path) as f:
+        tok = f.read().split()
+    k = tok.index("*data")
+    nn, _ne = int(tok[k + 1]), int(tok[k + 2])
+    nfield = int(tok[k + 3])
+    p = k + 5
+    sizes = [int(tok[p + i]) for i in range(nfield)]
+    p += nfield
+    names = tok[p:p + nfield]
+    if names[0] != "DISPLACEMENT" or sizes[0] != 3:
+        raise ValueError("%s: the first nodal field is not a 3-component DISPLACEMENT" % path)
+    p += nfield
+    width = sum(sizes)
+    out = {}
+    for _ in range(nn):
+        nid = int(tok[p])
+        vals = [float(v) for v in tok[p + 1:p + 1 + width]]
+        out[nid] = tuple(vals[:3])
+        p += 1 + width
+    return out
+
+
+def _D_iso(E, nu):
+    """Isotropic elasticity in (xx, yy, zz, xy, yz, zx) with engineering shears."""
+    lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    mu = E / (2.0 * (1.0 + nu))
+    D = np.zeros((6, 6))
+    D[:3, :3] = lam
+    D[np.arange(3), np.arange(3)] = lam + 2.0 * mu
+    D[np.arange(3, 6), np.arange(3, 6)] = mu
+    return D
+
+
+def _tet4_stiffness(X, D):
+    """12x12 stiffness of a linear tetrahedron with corner coordinates X (4x3)."""
+    J = (X[1:] - X[0]).T  # columns: edges from node 1
+    det = np.linalg.det(J)
+    if det <= 0.0:
+        raise ValueError("tetrahedron with non-positive volume")
+    G = np.linalg.inv(J)  # rows: gradients of the barycentric coordinates of nodes 2-4
+    grads = np.vstack([-G.sum(axis=0), G])
+    B = np.zeros((6, 12))
+    for a in range(4):
+        gx, gy, gz = grads[a]
+        c = 3 * a
+        B[0, c] = gx
+        B[1, c + 1] = gy
+        B[2, c + 2] = gz
+        B[3, c], B[3, c + 1] = gy, gx
+        B[4, c + 1], B[4, c + 2] = gz, gy
+        B[5, c], B[5, c + 2] = gz, gx
+    return (det / 6.0) * (B.T @ D @ B)
+
+
+def assemble(msh_path, cnt_path):
+    """The static system of a fistr1 model as a :class:`HecMatrix` (solver
+    settings of the ``!SOLVER`` block applied) and the node ids of its rows."""
+    nodes, elems, material, groups = read_msh(msh_path)
+    bcs, loads, solver = read_cnt(cnt_path)
+    if material is None:
+        raise ValueError("%s: no !MATERIAL elasticity" % msh_path)
+    if any(t != 341 for _, t, _ in elems):
+        raise ValueError("only element type 341 is assembled")
+    used = set(n for _, _, ns in elems for n in ns)
+    ids = [n for n in nodes if n in used]
+    idx = {n: i for i, n in enumerate(ids)}
+    N = len(ids)
+    X = np.array([nodes[n] for n in ids])
+    D = _D_iso(*material)
+    K = np.zeros((3 * N, 3 * N))
+    pairs = set()
+    for _, _, ns in elems:
+        loc = [idx[n] for n in ns]
+        ke = _tet4_stiffness(X[loc], D)
+        dofs = np.array([[3 * i, 3 * i + 1, 3 * i + 2] for i in loc]).ravel()
+        K[np.ix_(dofs, dofs)] += ke
+        pairs.update((a, b) for a in loc for b in loc)
+    B = np.zeros(3 * N)
+
+    def members(target):
+        t = target.upper()
+        return [idx[n] for n in groups[t] if n in idx] if t in groups else [idx[int(target)]]
+
+    for target, dof, val in loads:
+        for i in members(target):
+            B[3 * i + dof - 1] += val
+    fixed = {}
+    for target, d0, d1, val in bcs:
+        for i in members(target):
+            for d in range(d0, d1 + 1):
+                fixed[3 * i + d - 1] = val
+    # elimination with a unit pivot: move the prescribed values to the right-hand side
+    for k, val in fixed.items():
+        if val != 0.0:
+            B -= K[:, k] * val
+    for k, val in fixed.items():
+        K[k, :] = 0.0
+        K[:, k] = 0.0
+        K[k, k] = 1.0
+        B[k] = val
+    # block CSR of the element profile, items ascending
+    nbr = [[] for _ in range(N)]
+    for a, b in pairs:
+        if a != b:
+            nbr[a].append(b)
+    for lst in nbr:
+        lst.sort()
+    indexL, indexU, itemL, itemU, AL, AU = [0], [0], [], [], [], []
+    Dd = np.zeros((N, 3, 3))
+    for i in range(N):
+        Dd[i] = K[3 * i:3 * i + 3, 3 * i:3 * i + 3]
+        for j in nbr[i]:
+            blk = K[3 * i:3 * i + 3, 3 * j:3 * j + 3]
+            if j < i:
+                itemL.append(j + 1)
+                AL.append(blk)
+            else:
+                itemU.append(j + 1)
+                AU.append(blk)
+        indexL.append(len(itemL))
+        indexU.append(len(itemU))
+    m = HecMatrix(N, Dd.ravel(), np.array(AL).ravel() if AL else np.zeros(0),
+                  np.array(AU).ravel() if AU else np.zeros(0), np.array(indexL, np.int32),
+                  np.array(itemL, np.int32), np.array(indexU, np.int32), np.array(itemU, np.int32), B=B)
+    m.set_method({"CG": 1, "BICGSTAB": 2}.get(solver.get("METHOD", "CG"), 1))
+    m.set_precond(solver.get("PRECOND", 1))
+    if "ITER" in solver:
+        m.set_iter(solver["ITER"])
+    if "ITERPREMAX" in solver:
+        m.set_iterpremax(solver["ITERPREMAX"])
+    if "RESID" in solver:
+        m.set_resid(solver["RESID"])
+    if "SIGMA_DIAG" in solver:
+        m.Rarray[m.R_SIGMA_DIAG - 1] = solver["SIGMA_DIAG"]
+    if "SIGMA" in solver:
+        m.Rarray[m.R_SIGMA - 1] = solver["SIGMA"]
+    return m, ids

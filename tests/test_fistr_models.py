@@ -1,0 +1,76 @@
+"""fistr1's own CG test model (tests/golden/fistr_cg, from the reference's
+tests/solver/CG) assembled by frontistr_amd.mesh, pinned on the CPU:
+
+* the displacement fistr1 reported (.res, SSOR and DIAG runs) solves the
+  assembled system to fistr1's RESID (1e-8): the element stiffness, material,
+  profile, Dirichlet elimination and load are fistr1's;
+* a dense direct solve of the assembled system equals that displacement to
+  the CG's accuracy;
+* the reference's own CG (oracle/_ref) on the assembled system reproduces it;
+* a planted mistake in the element (the tensor instead of the engineering
+  shear modulus) is caught by the residual check.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from frontistr_amd import mesh
+from oracle import ref
+from tests.systems import dense
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fistr_cg")
+
+
+def _model(prec):
+    m, ids = mesh.assemble(os.path.join(HERE, "SSOR.msh"), os.path.join(HERE, prec + ".cnt"))
+    u = mesh.read_res_displacement(os.path.join(HERE, prec + ".res.0.1"))
+    return m, ids, np.array([u[n] for n in ids]).ravel()
+
+
+def test_reader_sizes_and_solver_settings():
+    nodes, elems, material, groups = mesh.read_msh(os.path.join(HERE, "SSOR.msh"))
+    assert len(elems) == 240 and all(t == 341 and len(ns) == 4 for _, t, ns in elems)
+    assert material == (4000.0, 0.3)
+    assert groups["FIX"] == [1001, 1101, 1201, 3001, 3101, 3201, 5001, 5101, 5201]
+    for prec, code in (("SSOR", 1), ("DIAG", 3)):
+        m, ids, u = _model(prec)
+        assert m.N == len(ids) == 99 and u.size == 297
+        assert m.get_i(m.I_METHOD) == 1 and m.get_i(m.I_PRECOND) == code
+        assert m.get_i(m.I_ITER) == 10000 and m.Rarray[0] == 1e-8
+
+
+@pytest.mark.parametrize("prec", ["SSOR", "DIAG"])
+def test_fistr1_displacement_solves_the_assembled_system(prec):
+    m, ids, u = _model(prec)
+    A = dense(m)
+    assert np.array_equal(A, A.T)
+    rel = np.linalg.norm(A @ u - m.B) / np.linalg.norm(m.B)
+    assert rel <= 2e-8, rel
+    x = np.linalg.solve(A, m.B)
+    assert np.abs(x - u).max() <= 1e-8 * np.abs(u).max()
+
+
+def test_planted_shear_mistake_is_caught():
+    m, ids, u = _model("SSOR")
+    orig = mesh._D_iso
+    try:
+        def wrong(E, nu):
+            D = orig(E, nu)
+            D[3:, 3:] *= 2.0  # tensor instead of engineering shear strains
+            return D
+        mesh._D_iso = wrong
+        mw, _ = mesh.assemble(os.path.join(HERE, "SSOR.msh"), os.path.join(HERE, "SSOR.cnt"))
+    finally:
+        mesh._D_iso = orig
+    rel = np.linalg.norm(dense(mw) @ u - mw.B) / np.linalg.norm(mw.B)
+    assert rel > 1e-2, rel
+
+
+@pytest.mark.skipif(not ref.available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("prec", ["SSOR", "DIAG"])
+def test_reference_cg_reproduces_fistr1(prec):
+    m, ids, u = _model(prec)
+    x, info = ref.solve_krylov(m)
+    assert info["error"] == 0
+    assert np.abs(x - u).max() <= 1e-6 * np.abs(u).max()
