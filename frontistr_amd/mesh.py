@@ -6,12 +6,13 @@ The subset read is what fistr1's own solver tests use
 
 * mesh (``.msh``, hecmw's native format): ``!NODE``, ``!ELEMENT, TYPE=341``
   (4-node tetrahedron), ``!MATERIAL`` with ``!ITEM=1, SUBITEM=2`` (Young's
-  modulus, Poisson's ratio), ``!NGROUP`` (list or ``GENERATE`` start, end,
-  step); other blocks are skipped;
+  modulus, Poisson's ratio) and ``!ITEM=2`` (density), ``!NGROUP`` (list or
+  ``GENERATE`` start, end, step); other blocks are skipped;
 * control (``.cnt``): ``!BOUNDARY`` (node or group, first dof, last dof,
-  value), ``!CLOAD`` (node or group, dof, value) and ``!SOLVER`` (METHOD,
-  PRECOND and the two data lines: ITER, ITERPREMAX / RESID, SIGMA_DIAG,
-  SIGMA);
+  value), ``!CLOAD`` (node or group, dof, value), ``!DLOAD`` body forces on
+  all elements (``BX``/``BY``/``BZ`` force per volume, ``GRAV`` acceleration
+  and direction times the density) and ``!SOLVER`` (METHOD, PRECOND and the
+  two data lines: ITER, ITERPREMAX / RESID, SIGMA_DIAG, SIGMA);
 * result (``.res``, ``*fstrresult 2.0``): the nodal DISPLACEMENT of every node.
 
 :func:`assemble` builds the static system fistr1 hands to ``hecmw_solve``:
@@ -27,7 +28,9 @@ The subset read is what fistr1's own solver tests use
 * Dirichlet values by row and column elimination with a unit pivot
   (``hecmw_mat_ass_bc``, hecmw1/src/solver/matrix/hecmw_mat_ass.f90:328-463),
   the right-hand side moved by the prescribed values;
-* concentrated loads added to the right-hand side.
+* concentrated loads added to the right-hand side, and a constant body
+  force f over a linear tetrahedron as V f / 4 on each of its nodes (the
+  integral of f times a linear shape function).
 """
 import re
 
@@ -67,9 +70,10 @@ def _fields(line):
 
 def read_msh(path):
     """Nodes {id: (x, y, z)} in file order, elements [(id, type, node ids)],
-    material (E, nu) and node groups {name: [ids]}."""
+    material (E, nu, density or None) and node groups {name: [ids]}."""
     nodes, elems, groups = {}, [], {}
     material = None
+    density = None
     for header, lines in _blocks(path):
         key, opt = _opts(header)
         if key == "END":
@@ -95,13 +99,18 @@ def read_msh(path):
                 raise ValueError("%s: only isotropic elasticity (ITEM=1, SUBITEM=2) is read" % path)
             f = [float(v) for v in _fields(lines[0])]
             material = (f[0], f[1])
+        elif key.replace(" ", "") == "ITEM=2":
+            density = float(_fields(lines[0])[0])
+    if material is not None:
+        material = (material[0], material[1], density)
     return nodes, elems, material, groups
 
 
 def read_cnt(path):
     """Boundary conditions [(node or group, first dof, last dof, value)], loads
-    [(node or group, dof, value)] and solver settings."""
-    bcs, loads, solver = [], [], {}
+    [(node or group, dof, value)], body loads [(element group, kind, values)]
+    and solver settings."""
+    bcs, loads, dloads, solver = [], [], [], {}
     for header, lines in _blocks(path):
         key, opt = _opts(header)
         if key == "END":
@@ -114,6 +123,13 @@ def read_cnt(path):
             for ln in lines:
                 f = _fields(ln)
                 loads.append((f[0], int(f[1]), float(f[2])))
+        elif key == "DLOAD":
+            for ln in lines:
+                f = _fields(ln)
+                kind = f[1].upper()
+                if kind not in ("BX", "BY", "BZ", "GRAV"):
+                    raise ValueError("%s: !DLOAD %s is not read" % (path, kind))
+                dloads.append((f[0].upper(), kind, [float(v) for v in f[2:]]))
         elif key == "SOLVER":
             solver["METHOD"] = opt.get("METHOD", "CG").upper()
             solver["PRECOND"] = int(opt.get("PRECOND", "1"))
@@ -129,7 +145,7 @@ def read_cnt(path):
                     solver["SIGMA_DIAG"] = f[1]
                 if len(f) > 2:
                     solver["SIGMA"] = f[2]
-    return bcs, loads, solver
+    return bcs, loads, dloads, solver
 
 
 def read_res_displacement(path):
@@ -193,7 +209,7 @@ def assemble(msh_path, cnt_path):
     """The static system of a fistr1 model as a :class:`HecMatrix` (solver
     settings of the ``!SOLVER`` block applied) and the node ids of its rows."""
     nodes, elems, material, groups = read_msh(msh_path)
-    bcs, loads, solver = read_cnt(cnt_path)
+    bcs, loads, dloads, solver = read_cnt(cnt_path)
     if material is None:
         raise ValueError("%s: no !MATERIAL elasticity" % msh_path)
     if any(t != 341 for _, t, _ in elems):
@@ -203,7 +219,7 @@ def assemble(msh_path, cnt_path):
     idx = {n: i for i, n in enumerate(ids)}
     N = len(ids)
     X = np.array([nodes[n] for n in ids])
-    D = _D_iso(*material)
+    D = _D_iso(material[0], material[1])
     K = np.zeros((3 * N, 3 * N))
     pairs = set()
     for _, _, ns in elems:
@@ -221,6 +237,21 @@ def assemble(msh_path, cnt_path):
     for target, dof, val in loads:
         for i in members(target):
             B[3 * i + dof - 1] += val
+    for egrp, kind, vals in dloads:
+        if egrp != "ALL":
+            raise ValueError("!DLOAD on element group %s: only ALL is read" % egrp)
+        if kind == "GRAV":
+            if material[2] is None:
+                raise ValueError("%s: GRAV needs the density (!ITEM=2)" % msh_path)
+            f = material[2] * vals[0] * np.array(vals[1:4])
+        else:
+            f = np.zeros(3)
+            f["XYZ".index(kind[1])] = vals[0]
+        for _, _, ns in elems:
+            loc = [idx[n] for n in ns]
+            V = np.linalg.det((X[loc[1:]] - X[loc[0]]).T) / 6.0
+            for i in loc:
+                B[3 * i:3 * i + 3] += V * f / 4.0
     fixed = {}
     for target, d0, d1, val in bcs:
         for i in members(target):
@@ -259,7 +290,7 @@ def assemble(msh_path, cnt_path):
     m = HecMatrix(N, Dd.ravel(), np.array(AL).ravel() if AL else np.zeros(0),
                   np.array(AU).ravel() if AU else np.zeros(0), np.array(indexL, np.int32),
                   np.array(itemL, np.int32), np.array(indexU, np.int32), np.array(itemU, np.int32), B=B)
-    m.set_method({"CG": 1, "BICGSTAB": 2}.get(solver.get("METHOD", "CG"), 1))
+    m.set_method({"CG": 1, "1": 1, "BICGSTAB": 2, "2": 2}[solver.get("METHOD", "CG")])
     m.set_precond(solver.get("PRECOND", 1))
     if "ITER" in solver:
         m.set_iter(solver["ITER"])

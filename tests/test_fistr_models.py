@@ -1,5 +1,8 @@
-"""fistr1's own CG test model (tests/golden/fistr_cg, from the reference's
-tests/solver/CG) assembled by frontistr_amd.mesh, pinned on the CPU:
+"""fistr1's own test models assembled by frontistr_amd.mesh, pinned on the
+CPU: the CG solver test (tests/golden/fistr_cg, from the reference's
+tests/solver/CG) and the static examples A, C and D with TET4 elements
+(tests/golden/fistr_static, from tests/analysis/static: a point load, a body
+force per volume, gravity times the density):
 
 * the displacement fistr1 reported (.res, SSOR and DIAG runs) solves the
   assembled system to fistr1's RESID (1e-8): the element stiffness, material,
@@ -20,18 +23,36 @@ from oracle import ref
 from tests.systems import dense
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fistr_cg")
+STATIC = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fistr_static")
+# name -> (mesh, control, result)
+MODELS = {
+    "cg_ssor": (os.path.join(HERE, "SSOR.msh"), os.path.join(HERE, "SSOR.cnt"), os.path.join(HERE, "SSOR.res.0.1")),
+    "cg_diag": (os.path.join(HERE, "SSOR.msh"), os.path.join(HERE, "DIAG.cnt"), os.path.join(HERE, "DIAG.res.0.1")),
+    "static_A341": (os.path.join(HERE, "SSOR.msh"), os.path.join(STATIC, "A341.cnt"),
+                    os.path.join(STATIC, "A341.res.0.1")),
+    "static_C341": (os.path.join(STATIC, "C341.msh"), os.path.join(STATIC, "C341.cnt"),
+                    os.path.join(STATIC, "C341.res.0.1")),
+    "static_D341": (os.path.join(STATIC, "D341.msh"), os.path.join(STATIC, "D341.cnt"),
+                    os.path.join(STATIC, "D341.res.0.1")),
+}
+
+
+def model(name):
+    msh, cnt, res = MODELS[name]
+    m, ids = mesh.assemble(msh, cnt)
+    u = mesh.read_res_displacement(res)
+    return m, ids, np.array([u[n] for n in ids]).ravel()
 
 
 def _model(prec):
-    m, ids = mesh.assemble(os.path.join(HERE, "SSOR.msh"), os.path.join(HERE, prec + ".cnt"))
-    u = mesh.read_res_displacement(os.path.join(HERE, prec + ".res.0.1"))
-    return m, ids, np.array([u[n] for n in ids]).ravel()
+    return model("cg_" + prec.lower())
 
 
 def test_reader_sizes_and_solver_settings():
     nodes, elems, material, groups = mesh.read_msh(os.path.join(HERE, "SSOR.msh"))
     assert len(elems) == 240 and all(t == 341 and len(ns) == 4 for _, t, ns in elems)
-    assert material == (4000.0, 0.3)
+    assert material == (4000.0, 0.3, None)
+    assert mesh.read_msh(MODELS["static_D341"][0])[2] == (4000.0, 0.3, 8.0102e-10)
     assert groups["FIX"] == [1001, 1101, 1201, 3001, 3101, 3201, 5001, 5101, 5201]
     for prec, code in (("SSOR", 1), ("DIAG", 3)):
         m, ids, u = _model(prec)
@@ -40,9 +61,9 @@ def test_reader_sizes_and_solver_settings():
         assert m.get_i(m.I_ITER) == 10000 and m.Rarray[0] == 1e-8
 
 
-@pytest.mark.parametrize("prec", ["SSOR", "DIAG"])
-def test_fistr1_displacement_solves_the_assembled_system(prec):
-    m, ids, u = _model(prec)
+@pytest.mark.parametrize("name", sorted(MODELS))
+def test_fistr1_displacement_solves_the_assembled_system(name):
+    m, ids, u = model(name)
     A = dense(m)
     assert np.array_equal(A, A.T)
     rel = np.linalg.norm(A @ u - m.B) / np.linalg.norm(m.B)
@@ -68,9 +89,9 @@ def test_planted_shear_mistake_is_caught():
 
 
 @pytest.mark.skipif(not ref.available(), reason="oracle/_ref not built")
-@pytest.mark.parametrize("prec", ["SSOR", "DIAG"])
-def test_reference_cg_reproduces_fistr1(prec):
-    m, ids, u = _model(prec)
+@pytest.mark.parametrize("name", sorted(MODELS))
+def test_reference_cg_reproduces_fistr1(name):
+    m, ids, u = model(name)
     x, info = ref.solve_krylov(m)
     assert info["error"] == 0
     assert np.abs(x - u).max() <= 1e-6 * np.abs(u).max()

@@ -1,7 +1,8 @@
-"""fistr1's own CG test model (tests/golden/fistr_cg) solved on the GPU
-through hecmw_solve_iterative, against the displacement fistr1 reported:
+"""fistr1's own test models (tests/golden/fistr_cg and fistr_static) solved
+on the GPU through hecmw_solve_iterative, against the displacement fistr1
+reported:
 
-* CG with SSOR and with DIAG, as the model's !SOLVER blocks ask: converged
+* every model as its !SOLVER block asks (CG with SSOR or DIAG): converged
   to RESID, displacement within 1e-6 of fistr1's (relative to the largest
   component), iterations within +-2 of the reference's own CG on the same
   system (oracle/_ref, multithreaded: with one thread the reference's SSOR
@@ -18,6 +19,7 @@ import pytest
 import frontistr_amd as fa
 from frontistr_amd import mesh
 from oracle import ref
+from tests.test_fistr_models import MODELS, model
 
 pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not ref.available(), reason="oracle/_ref not built")]
 
@@ -25,9 +27,7 @@ HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fistr
 
 
 def _model(prec):
-    m, ids = mesh.assemble(os.path.join(HERE, "SSOR.msh"), os.path.join(HERE, prec + ".cnt"))
-    u = mesh.read_res_displacement(os.path.join(HERE, prec + ".res.0.1"))
-    return m, ids, np.array([u[n] for n in ids]).ravel()
+    return model("cg_" + prec.lower())
 
 
 def _check_u(x, u):
@@ -35,12 +35,12 @@ def _check_u(x, u):
     assert err <= 1e-6, err
 
 
-@pytest.mark.parametrize("prec", ["SSOR", "DIAG"])
-def test_cg_as_the_model_asks(prec):
-    m, ids, u = _model(prec)
+@pytest.mark.parametrize("name", sorted(MODELS))
+def test_cg_as_the_model_asks(name):
+    m, ids, u = model(name)
     xr, info = ref.solve_krylov(m.copy())
     res = fa.hecmw_solve_iterative(m)
-    print("%s: %d iterations (reference CG %d), resid %.2e" % (prec, res.iter, info["iter"], res.resid))
+    print("%s: %d iterations (reference CG %d), resid %.2e" % (name, res.iter, info["iter"], res.resid))
     assert res.error == 0 and res.resid <= 1e-8
     assert abs(res.iter - info["iter"]) <= 2
     _check_u(m.X, u)
