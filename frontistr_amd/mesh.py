@@ -5,7 +5,7 @@ The subset read is what fistr1's own solver tests use
 (tests/solver/CG/*.msh, *.cnt in the reference):
 
 * mesh (``.msh``, hecmw's native format): ``!NODE``, ``!ELEMENT, TYPE=341``
-  (4-node tetrahedron), ``!MATERIAL`` with ``!ITEM=1, SUBITEM=2`` (Young's
+  or ``342`` (4- and 10-node tetrahedra), ``!MATERIAL`` with ``!ITEM=1, SUBITEM=2`` (Young's
   modulus, Poisson's ratio) and ``!ITEM=2`` (density), ``!NGROUP`` (list or
   ``GENERATE`` start, end, step); other blocks are skipped;
 * control (``.cnt``): ``!BOUNDARY`` (node or group, first dof, last dof,
@@ -17,10 +17,17 @@ The subset read is what fistr1's own solver tests use
 
 :func:`assemble` builds the static system fistr1 hands to ``hecmw_solve``:
 
-* element stiffness of type 341 as ``STF_C3`` computes it
+* element stiffness as ``STF_C3`` computes it
   (fistr1/src/lib/static_LIB_3d.f90:50-208): small strain, isotropic
-  elasticity, one quadrature point, so K_e = V B^T D B with the constant
-  strain-displacement matrix B of the linear tetrahedron;
+  elasticity, K_e = sum over quadrature points of B^T D B det(J) w.  Type
+  341 has one point, so K_e = V B^T D B with the constant B of the linear
+  tetrahedron; type 342 uses the quadratic shape functions of
+  fistr1/src/lib/element/tet10n.f90 (vertices 1-4, then the midpoints of
+  edges 1-2, 2-3, 3-1, 1-4, 2-4, 3-4) and the 4-point rule (a = (5 - sqrt 5)
+  / 20, b = (5 + 3 sqrt 5) / 20, w = 1/24).  A ``.msh`` file lists a 342's
+  midpoints in hecmw's order, edges 2-3, 3-1, 1-2, 1-4, 2-4, 3-4 (the order
+  hecmw1/src/common/hecmw_conn_conv.c:18 converts Abaqus' C3D10 into), so
+  nodes 5-7 are rotated onto the shape functions' edges;
 * the nodes are those the elements use, in the order of ``!NODE`` (the mesh
   reader drops nodes no element references), the block profile is every
   node pair sharing an element with items ascending per row
@@ -29,8 +36,9 @@ The subset read is what fistr1's own solver tests use
   (``hecmw_mat_ass_bc``, hecmw1/src/solver/matrix/hecmw_mat_ass.f90:328-463),
   the right-hand side moved by the prescribed values;
 * concentrated loads added to the right-hand side, and a constant body
-  force f over a linear tetrahedron as V f / 4 on each of its nodes (the
-  integral of f times a linear shape function).
+  force f as the integral of f times each shape function: V f / 4 per node
+  of a linear tetrahedron, the 4-point rule (exact for these quadratics) for
+  type 342.
 """
 import re
 
@@ -184,17 +192,12 @@ def _D_iso(E, nu):
     return D
 
 
-def _tet4_stiffness(X, D):
-    """12x12 stiffness of a linear tetrahedron with corner coordinates X (4x3)."""
-    J = (X[1:] - X[0]).T  # columns: edges from node 1
-    det = np.linalg.det(J)
-    if det <= 0.0:
-        raise ValueError("tetrahedron with non-positive volume")
-    G = np.linalg.inv(J)  # rows: gradients of the barycentric coordinates of nodes 2-4
-    grads = np.vstack([-G.sum(axis=0), G])
-    B = np.zeros((6, 12))
-    for a in range(4):
-        gx, gy, gz = grads[a]
+def _B(dN):
+    """6 x 3n strain-displacement matrix from the shape-function gradients (n x 3)."""
+    n = dN.shape[0]
+    B = np.zeros((6, 3 * n))
+    for a in range(n):
+        gx, gy, gz = dN[a]
         c = 3 * a
         B[0, c] = gx
         B[1, c + 1] = gy
@@ -202,6 +205,61 @@ def _tet4_stiffness(X, D):
         B[3, c], B[3, c + 1] = gy, gx
         B[4, c + 1], B[4, c + 2] = gz, gy
         B[5, c], B[5, c + 2] = gz, gx
+    return B
+
+
+_HEC342_TO_SHAPE = (0, 1, 2, 3, 6, 4, 5, 7, 8, 9)
+_QA, _QB = (5.0 - np.sqrt(5.0)) / 20.0, (5.0 + 3.0 * np.sqrt(5.0)) / 20.0
+_TET10_POINTS = ((_QA, _QA, _QA), (_QB, _QA, _QA), (_QA, _QB, _QA), (_QA, _QA, _QB))
+
+
+def _tet10_shape(xi, et, ze):
+    a = 1.0 - xi - et - ze
+    N = np.array([a * (2 * a - 1), xi * (2 * xi - 1), et * (2 * et - 1), ze * (2 * ze - 1),
+                  4 * a * xi, 4 * xi * et, 4 * et * a, 4 * a * ze, 4 * xi * ze, 4 * et * ze])
+    g = np.array([[1 - 4 * a, 1 - 4 * a, 1 - 4 * a],
+                  [4 * xi - 1, 0, 0],
+                  [0, 4 * et - 1, 0],
+                  [0, 0, 4 * ze - 1],
+                  [4 * (a - xi), -4 * xi, -4 * xi],
+                  [4 * et, 4 * xi, 0],
+                  [-4 * et, 4 * (a - et), -4 * et],
+                  [-4 * ze, -4 * ze, 4 * (a - ze)],
+                  [4 * ze, 0, 4 * xi],
+                  [0, 4 * ze, 4 * et]])
+    return N, g
+
+
+def _tet10_points(X):
+    """(N, dN/dx, det J * w) at the 4 quadrature points of a 10-node tetrahedron."""
+    out = []
+    for xi, et, ze in _TET10_POINTS:
+        N, g = _tet10_shape(xi, et, ze)
+        J = g.T @ X  # J[r, d] = d x_d / d xi_r
+        det = np.linalg.det(J)
+        if det <= 0.0:
+            raise ValueError("tetrahedron with non-positive Jacobian")
+        out.append((N, g @ np.linalg.inv(J).T, det / 24.0))
+    return out
+
+
+def _tet10_stiffness(X, D):
+    """30x30 stiffness of a 10-node tetrahedron with node coordinates X (10x3)."""
+    K = np.zeros((30, 30))
+    for _, dN, w in _tet10_points(X):
+        B = _B(dN)
+        K += w * (B.T @ D @ B)
+    return K
+
+
+def _tet4_stiffness(X, D):
+    """12x12 stiffness of a linear tetrahedron with corner coordinates X (4x3)."""
+    J = (X[1:] - X[0]).T  # columns: edges from node 1
+    det = np.linalg.det(J)
+    if det <= 0.0:
+        raise ValueError("tetrahedron with non-positive volume")
+    G = np.linalg.inv(J)  # rows: gradients of the barycentric coordinates of nodes 2-4
+    B = _B(np.vstack([-G.sum(axis=0), G]))
     return (det / 6.0) * (B.T @ D @ B)
 
 
@@ -212,8 +270,10 @@ def assemble(msh_path, cnt_path):
     bcs, loads, dloads, solver = read_cnt(cnt_path)
     if material is None:
         raise ValueError("%s: no !MATERIAL elasticity" % msh_path)
-    if any(t != 341 for _, t, _ in elems):
-        raise ValueError("only element type 341 is assembled")
+    if any(t not in (341, 342) for _, t, _ in elems):
+        raise ValueError("only element types 341 and 342 are assembled")
+    # 342: hecmw's midpoint order (2-3, 3-1, 1-2, ...) -> the shape functions' (1-2, 2-3, 3-1, ...)
+    elems = [(e, t, [ns[i] for i in _HEC342_TO_SHAPE] if t == 342 else ns) for e, t, ns in elems]
     used = set(n for _, _, ns in elems for n in ns)
     ids = [n for n in nodes if n in used]
     idx = {n: i for i, n in enumerate(ids)}
@@ -224,7 +284,7 @@ def assemble(msh_path, cnt_path):
     pairs = set()
     for _, _, ns in elems:
         loc = [idx[n] for n in ns]
-        ke = _tet4_stiffness(X[loc], D)
+        ke = _tet4_stiffness(X[loc], D) if len(loc) == 4 else _tet10_stiffness(X[loc], D)
         dofs = np.array([[3 * i, 3 * i + 1, 3 * i + 2] for i in loc]).ravel()
         K[np.ix_(dofs, dofs)] += ke
         pairs.update((a, b) for a in loc for b in loc)
@@ -249,9 +309,14 @@ def assemble(msh_path, cnt_path):
             f["XYZ".index(kind[1])] = vals[0]
         for _, _, ns in elems:
             loc = [idx[n] for n in ns]
-            V = np.linalg.det((X[loc[1:]] - X[loc[0]]).T) / 6.0
-            for i in loc:
-                B[3 * i:3 * i + 3] += V * f / 4.0
+            if len(loc) == 4:
+                V = np.linalg.det((X[loc[1:]] - X[loc[0]]).T) / 6.0
+                for i in loc:
+                    B[3 * i:3 * i + 3] += V * f / 4.0
+            else:
+                for shp, _, w in _tet10_points(X[loc]):
+                    for a, i in enumerate(loc):
+                        B[3 * i:3 * i + 3] += (w * shp[a]) * f
     fixed = {}
     for target, d0, d1, val in bcs:
         for i in members(target):

@@ -1,8 +1,8 @@
 """fistr1's own test models assembled by frontistr_amd.mesh, pinned on the
 CPU: the CG solver test (tests/golden/fistr_cg, from the reference's
-tests/solver/CG) and the static examples A, C and D with TET4 elements
-(tests/golden/fistr_static, from tests/analysis/static: a point load, a body
-force per volume, gravity times the density):
+tests/solver/CG) and the static examples A, C and D with TET4 and TET10
+elements (tests/golden/fistr_static, from tests/analysis/static: a point
+load, a body force per volume, gravity times the density):
 
 * the displacement fistr1 reported (.res, SSOR and DIAG runs) solves the
   assembled system to fistr1's RESID (1e-8): the element stiffness, material,
@@ -10,8 +10,9 @@ force per volume, gravity times the density):
 * a dense direct solve of the assembled system equals that displacement to
   the CG's accuracy;
 * the reference's own CG (oracle/_ref) on the assembled system reproduces it;
-* a planted mistake in the element (the tensor instead of the engineering
-  shear modulus) is caught by the residual check.
+* planted mistakes are caught by the residual check: the tensor instead of
+  the engineering shear modulus, and a TET10's midpoints taken in the file's
+  order instead of rotated onto the shape functions' edges.
 """
 import os
 
@@ -35,6 +36,8 @@ MODELS = {
     "static_D341": (os.path.join(STATIC, "D341.msh"), os.path.join(STATIC, "D341.cnt"),
                     os.path.join(STATIC, "D341.res.0.1")),
 }
+for _x in "ACD":
+    MODELS["static_%s342" % _x] = tuple(os.path.join(STATIC, "%s342.%s" % (_x, ext)) for ext in ("msh", "cnt", "res.0.1"))
 
 
 def model(name):
@@ -65,11 +68,33 @@ def test_reader_sizes_and_solver_settings():
 def test_fistr1_displacement_solves_the_assembled_system(name):
     m, ids, u = model(name)
     A = dense(m)
-    assert np.array_equal(A, A.T)
+    # symmetric to rounding: B^T D B summed over quadrature points
+    assert np.abs(A - A.T).max() <= 1e-14 * np.abs(A).max()
     rel = np.linalg.norm(A @ u - m.B) / np.linalg.norm(m.B)
     assert rel <= 2e-8, rel
     x = np.linalg.solve(A, m.B)
     assert np.abs(x - u).max() <= 1e-8 * np.abs(u).max()
+
+
+def test_planted_tet10_midpoint_order_mistake_is_caught(monkeypatch):
+    """The .msh lists a 342's midpoints as edges 2-3, 3-1, 1-2: taken in the
+    shape functions' order instead, the mid-edge nodes land on the wrong edges."""
+    m, ids, u = model("static_A342")
+    monkeypatch.setattr(mesh, "_HEC342_TO_SHAPE", tuple(range(10)))
+    monkeypatch.setattr(mesh, "_tet10_points", _tet10_points_any_sign)
+    msh, cnt, _ = MODELS["static_A342"]
+    mw, _ = mesh.assemble(msh, cnt)
+    rel = np.linalg.norm(dense(mw) @ u - mw.B) / np.linalg.norm(mw.B)
+    assert rel > 1e-2, rel
+
+
+def _tet10_points_any_sign(X):
+    out = []
+    for xi, et, ze in mesh._TET10_POINTS:
+        shp, g = mesh._tet10_shape(xi, et, ze)
+        J = g.T @ X
+        out.append((shp, g @ np.linalg.inv(J).T, abs(np.linalg.det(J)) / 24.0))
+    return out
 
 
 def test_planted_shear_mistake_is_caught():
