@@ -9,9 +9,10 @@
 // cell corners, edge midpoints, face centres (the split's face diagonals) and
 // cell centres (the main diagonal).
 //
-// Element node order and shape functions are fistr1's 342
-// (fistr1/src/lib/element/tet10n.f90:11-75): vertices 1-4, then midpoints
-// 5=(1,2) 6=(2,3) 7=(3,1) 8=(1,4) 9=(2,4) 10=(3,4); stiffness B^T D B by the
+// Shape functions are fistr1's 342 (fistr1/src/lib/element/tet10n.f90:11-75),
+// in their own node order: vertices 1-4, then midpoints 5=(1,2) 6=(2,3)
+// 7=(3,1) 8=(1,4) 9=(2,4) 10=(3,4) (a mesh file lists 5-7 as edges 2-3, 3-1,
+// 1-2, hecmw's order; the element matrix is the same); stiffness B^T D B by the
 // 4-point rule fistr1 uses for 342 (element.f90 NumOfQuadPoints, gauss3d5,
 // weight 1/24; points a=(5-sqrt5)/20, b=(5+3 sqrt5)/20 computed here in closed
 // form).  Profile: hecmw_mat_con's insertion order (elements in order, local
