@@ -4,8 +4,9 @@ the solver receives.
 The subset read is what fistr1's own solver tests use
 (tests/solver/CG/*.msh, *.cnt in the reference):
 
-* mesh (``.msh``, hecmw's native format): ``!NODE``, ``!ELEMENT, TYPE=341``
-  or ``342`` (4- and 10-node tetrahedra), ``!MATERIAL`` with ``!ITEM=1, SUBITEM=2`` (Young's
+* mesh (``.msh``, hecmw's native format): ``!NODE``, ``!ELEMENT, TYPE=341``,
+  ``342`` or ``361`` (4- and 10-node tetrahedra, 8-node hexahedron),
+  ``!MATERIAL`` with ``!ITEM=1, SUBITEM=2`` (Young's
   modulus, Poisson's ratio) and ``!ITEM=2`` (density), ``!NGROUP`` (list or
   ``GENERATE`` start, end, step); other blocks are skipped;
 * control (``.cnt``): ``!BOUNDARY`` (node or group, first dof, last dof,
@@ -27,7 +28,14 @@ The subset read is what fistr1's own solver tests use
   / 20, b = (5 + 3 sqrt 5) / 20, w = 1/24).  A ``.msh`` file lists a 342's
   midpoints in hecmw's order, edges 2-3, 3-1, 1-2, 1-4, 2-4, 3-4 (the order
   hecmw1/src/common/hecmw_conn_conv.c:18 converts Abaqus' C3D10 into), so
-  nodes 5-7 are rotated onto the shape functions' edges;
+  nodes 5-7 are rotated onto the shape functions' edges.  Type 361 is the
+  formulation fistr1's static analysis picks by default (incompatible modes,
+  fistr1/src/common/fstr_setup.f90:415-424; ``STF_C3D8IC``,
+  fistr1/src/lib/static_LIB_3dIC.f90:21-215): trilinear shape functions
+  (fistr1/src/lib/element/hex8n.f90), 2x2x2 Gauss points, three bubble modes
+  1 - xi_i^2 whose gradients -2 xi_i grad(xi_i) use the Jacobian at the
+  element centre scaled by det(J_centre) / det(J), and the 9 bubble dofs
+  condensed out: K = K_uu - K_ua K_aa^-1 K_au;
 * the nodes are those the elements use, in the order of ``!NODE`` (the mesh
   reader drops nodes no element references), the block profile is every
   node pair sharing an element with items ascending per row
@@ -38,7 +46,7 @@ The subset read is what fistr1's own solver tests use
 * concentrated loads added to the right-hand side, and a constant body
   force f as the integral of f times each shape function: V f / 4 per node
   of a linear tetrahedron, the 4-point rule (exact for these quadratics) for
-  type 342.
+  type 342, the 2x2x2 rule for type 361.
 """
 import re
 
@@ -252,6 +260,50 @@ def _tet10_stiffness(X, D):
     return K
 
 
+_HEX8_SIGNS = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
+                        [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]], dtype=np.float64)
+_G2 = 1.0 / np.sqrt(3.0)
+_HEX8_POINTS = tuple((a, b, c) for c in (-_G2, _G2) for b in (-_G2, _G2) for a in (-_G2, _G2))
+
+
+def _hex8_shape(xi):
+    f = 1.0 + _HEX8_SIGNS * np.asarray(xi)[None, :]  # (8, 3)
+    N = 0.125 * f.prod(axis=1)
+    g = np.empty((8, 3))
+    for r in range(3):
+        o = [d for d in range(3) if d != r]
+        g[:, r] = 0.125 * _HEX8_SIGNS[:, r] * f[:, o[0]] * f[:, o[1]]
+    return N, g
+
+
+def _hex8_points(X):
+    """(N, dN/dx, det J * w, det J, xi) at the 8 Gauss points of a hexahedron."""
+    out = []
+    for xi in _HEX8_POINTS:
+        N, g = _hex8_shape(xi)
+        J = g.T @ X
+        det = np.linalg.det(J)
+        if det <= 0.0:
+            raise ValueError("hexahedron with non-positive Jacobian")
+        out.append((N, g @ np.linalg.inv(J).T, det, det, xi))
+    return out
+
+
+def _hex8ic_stiffness(X, D):
+    """24x24 stiffness of an incompatible-mode hexahedron (9 bubble dofs condensed)."""
+    _, g0 = _hex8_shape((0.0, 0.0, 0.0))
+    J0 = g0.T @ X
+    det0 = np.linalg.det(J0)
+    grad_xi = np.linalg.inv(J0)  # column i: the gradient of xi_i in x
+    K = np.zeros((33, 33))
+    for _, dN, w, det, xi in _hex8_points(X):
+        bub = np.array([-2.0 * xi[i] * grad_xi[:, i] * det0 / det for i in range(3)])
+        B = _B(np.vstack([dN, bub]))
+        K += w * (B.T @ D @ B)
+    Kaa = K[24:, 24:]
+    return K[:24, :24] - K[:24, 24:] @ np.linalg.solve(Kaa, K[24:, :24])
+
+
 def _tet4_stiffness(X, D):
     """12x12 stiffness of a linear tetrahedron with corner coordinates X (4x3)."""
     J = (X[1:] - X[0]).T  # columns: edges from node 1
@@ -270,8 +322,8 @@ def assemble(msh_path, cnt_path):
     bcs, loads, dloads, solver = read_cnt(cnt_path)
     if material is None:
         raise ValueError("%s: no !MATERIAL elasticity" % msh_path)
-    if any(t not in (341, 342) for _, t, _ in elems):
-        raise ValueError("only element types 341 and 342 are assembled")
+    if any(t not in (341, 342, 361) for _, t, _ in elems):
+        raise ValueError("only element types 341, 342 and 361 are assembled")
     # 342: hecmw's midpoint order (2-3, 3-1, 1-2, ...) -> the shape functions' (1-2, 2-3, 3-1, ...)
     elems = [(e, t, [ns[i] for i in _HEC342_TO_SHAPE] if t == 342 else ns) for e, t, ns in elems]
     used = set(n for _, _, ns in elems for n in ns)
@@ -284,7 +336,7 @@ def assemble(msh_path, cnt_path):
     pairs = set()
     for _, _, ns in elems:
         loc = [idx[n] for n in ns]
-        ke = _tet4_stiffness(X[loc], D) if len(loc) == 4 else _tet10_stiffness(X[loc], D)
+        ke = {4: _tet4_stiffness, 10: _tet10_stiffness, 8: _hex8ic_stiffness}[len(loc)](X[loc], D)
         dofs = np.array([[3 * i, 3 * i + 1, 3 * i + 2] for i in loc]).ravel()
         K[np.ix_(dofs, dofs)] += ke
         pairs.update((a, b) for a in loc for b in loc)
@@ -314,7 +366,8 @@ def assemble(msh_path, cnt_path):
                 for i in loc:
                     B[3 * i:3 * i + 3] += V * f / 4.0
             else:
-                for shp, _, w in _tet10_points(X[loc]):
+                pts = _tet10_points(X[loc]) if len(loc) == 10 else [p[:3] for p in _hex8_points(X[loc])]
+                for shp, _, w in pts:
                     for a, i in enumerate(loc):
                         B[3 * i:3 * i + 3] += (w * shp[a]) * f
     fixed = {}

@@ -1,8 +1,8 @@
 """fistr1's own test models assembled by frontistr_amd.mesh, pinned on the
 CPU: the CG solver test (tests/golden/fistr_cg, from the reference's
-tests/solver/CG) and the static examples A, C and D with TET4 and TET10
-elements (tests/golden/fistr_static, from tests/analysis/static: a point
-load, a body force per volume, gravity times the density):
+tests/solver/CG) and the static examples A, C and D with TET4, TET10 and
+HEX8 elements (tests/golden/fistr_static, from tests/analysis/static: a
+point load, a body force per volume, gravity times the density):
 
 * the displacement fistr1 reported (.res, SSOR and DIAG runs) solves the
   assembled system to fistr1's RESID (1e-8): the element stiffness, material,
@@ -11,8 +11,9 @@ load, a body force per volume, gravity times the density):
   the CG's accuracy;
 * the reference's own CG (oracle/_ref) on the assembled system reproduces it;
 * planted mistakes are caught by the residual check: the tensor instead of
-  the engineering shear modulus, and a TET10's midpoints taken in the file's
-  order instead of rotated onto the shape functions' edges.
+  the engineering shear modulus, a TET10's midpoints taken in the file's
+  order instead of rotated onto the shape functions' edges, and a HEX8 with
+  plain integration instead of fistr1's incompatible modes.
 """
 import os
 
@@ -37,7 +38,9 @@ MODELS = {
                     os.path.join(STATIC, "D341.res.0.1")),
 }
 for _x in "ACD":
-    MODELS["static_%s342" % _x] = tuple(os.path.join(STATIC, "%s342.%s" % (_x, ext)) for ext in ("msh", "cnt", "res.0.1"))
+    for _t in ("342", "361"):
+        MODELS["static_%s%s" % (_x, _t)] = tuple(os.path.join(STATIC, "%s%s.%s" % (_x, _t, ext))
+                                                  for ext in ("msh", "cnt", "res.0.1"))
 
 
 def model(name):
@@ -95,6 +98,24 @@ def _tet10_points_any_sign(X):
         J = g.T @ X
         out.append((shp, g @ np.linalg.inv(J).T, abs(np.linalg.det(J)) / 24.0))
     return out
+
+
+def test_planted_hex8_full_integration_is_caught(monkeypatch):
+    """fistr1's static 361 is the incompatible-mode element: plain 2x2x2
+    integration (no bubble modes) is stiffer in bending and is caught."""
+    m, ids, u = model("static_A361")
+
+    def full(X, D):
+        K = np.zeros((24, 24))
+        for _, dN, w, _, _ in mesh._hex8_points(X):
+            B = mesh._B(dN)
+            K += w * (B.T @ D @ B)
+        return K
+    monkeypatch.setattr(mesh, "_hex8ic_stiffness", full)
+    msh, cnt, _ = MODELS["static_A361"]
+    mw, _ = mesh.assemble(msh, cnt)
+    rel = np.linalg.norm(dense(mw) @ u - mw.B) / np.linalg.norm(mw.B)
+    assert rel > 1e-2, rel
 
 
 def test_planted_shear_mistake_is_caught():
