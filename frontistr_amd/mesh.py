@@ -12,7 +12,9 @@ The subset read is what fistr1's own solver tests use
 * control (``.cnt``): ``!BOUNDARY`` (node or group, first dof, last dof,
   value), ``!CLOAD`` (node or group, dof, value), ``!DLOAD`` body forces on
   all elements (``BX``/``BY``/``BZ`` force per volume, ``GRAV`` acceleration
-  and direction times the density) and ``!SOLVER`` (METHOD, PRECOND and the
+  times the density along the normalised direction, as ``DL_C3``,
+  fistr1/src/lib/static_LIB_3d.f90:213-382, applies them) and ``!SOLVER``
+  (METHOD, PRECOND and the
   two data lines: ITER, ITERPREMAX / RESID, SIGMA_DIAG, SIGMA);
 * result (``.res``, ``*fstrresult 2.0``): the nodal DISPLACEMENT of every node.
 
@@ -355,7 +357,8 @@ def assemble(msh_path, cnt_path):
         if kind == "GRAV":
             if material[2] is None:
                 raise ValueError("%s: GRAV needs the density (!ITEM=2)" % msh_path)
-            f = material[2] * vals[0] * np.array(vals[1:4])
+            d = np.array(vals[1:4])
+            f = material[2] * vals[0] * d / np.linalg.norm(d)  # the direction is normalised (DL_C3)
         else:
             f = np.zeros(3)
             f["XYZ".index(kind[1])] = vals[0]
