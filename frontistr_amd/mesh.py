@@ -12,7 +12,8 @@ The subset read is what fistr1's own solver tests use
 * control (``.cnt``): ``!BOUNDARY`` (node or group, first dof, last dof,
   value), ``!CLOAD`` (node or group, dof, value), ``!DLOAD`` body forces on
   all elements (``BX``/``BY``/``BZ`` force per volume, ``GRAV`` acceleration
-  times the density along the normalised direction, as ``DL_C3``,
+  times the density along the normalised direction, ``CENT`` angular speed,
+  axis point and axis direction, as ``DL_C3``,
   fistr1/src/lib/static_LIB_3d.f90:213-382, applies them) and ``!SOLVER``
   (METHOD, PRECOND and the
   two data lines: ITER, ITERPREMAX / RESID, SIGMA_DIAG, SIGMA);
@@ -45,10 +46,10 @@ The subset read is what fistr1's own solver tests use
 * Dirichlet values by row and column elimination with a unit pivot
   (``hecmw_mat_ass_bc``, hecmw1/src/solver/matrix/hecmw_mat_ass.f90:328-463),
   the right-hand side moved by the prescribed values;
-* concentrated loads added to the right-hand side, and a constant body
-  force f as the integral of f times each shape function: V f / 4 per node
-  of a linear tetrahedron, the 4-point rule (exact for these quadratics) for
-  type 342, the 2x2x2 rule for type 361.
+* concentrated loads added to the right-hand side, and body forces
+  integrated against each shape function with the element's own quadrature
+  as ``DL_C3`` does (1 point for type 341, 4 for 342, 8 for 361; exact for a
+  constant force, so V f / 4 per node of a linear tetrahedron).
 """
 import re
 
@@ -145,7 +146,7 @@ def read_cnt(path):
             for ln in lines:
                 f = _fields(ln)
                 kind = f[1].upper()
-                if kind not in ("BX", "BY", "BZ", "GRAV"):
+                if kind not in ("BX", "BY", "BZ", "GRAV", "CENT"):
                     raise ValueError("%s: !DLOAD %s is not read" % (path, kind))
                 dloads.append((f[0].upper(), kind, [float(v) for v in f[2:]]))
         elif key == "SOLVER":
@@ -306,6 +307,18 @@ def _hex8ic_stiffness(X, D):
     return K[:24, :24] - K[:24, 24:] @ np.linalg.solve(Kaa, K[24:, :24])
 
 
+def _quadrature(X):
+    """(shape values, det J * weight) at the quadrature points fistr1 uses for the
+    element with nodes X: 1 (the centroid) for 341, 4 for 342, 8 for 361."""
+    n = X.shape[0]
+    if n == 4:
+        V = np.linalg.det((X[1:] - X[0]).T) / 6.0
+        return [(np.full(4, 0.25), V)]
+    if n == 10:
+        return [(shp, w) for shp, _, w in _tet10_points(X)]
+    return [(shp, w) for shp, _, w, _, _ in _hex8_points(X)]
+
+
 def _tet4_stiffness(X, D):
     """12x12 stiffness of a linear tetrahedron with corner coordinates X (4x3)."""
     J = (X[1:] - X[0]).T  # columns: edges from node 1
@@ -354,25 +367,26 @@ def assemble(msh_path, cnt_path):
     for egrp, kind, vals in dloads:
         if egrp != "ALL":
             raise ValueError("!DLOAD on element group %s: only ALL is read" % egrp)
+        if kind in ("GRAV", "CENT") and material[2] is None:
+            raise ValueError("%s: %s needs the density (!ITEM=2)" % (msh_path, kind))
         if kind == "GRAV":
-            if material[2] is None:
-                raise ValueError("%s: GRAV needs the density (!ITEM=2)" % msh_path)
             d = np.array(vals[1:4])
             f = material[2] * vals[0] * d / np.linalg.norm(d)  # the direction is normalised (DL_C3)
+        elif kind == "CENT":
+            # rho w|w| (x - its projection on the axis through A along R), at each quadrature point
+            omega, A_, R_ = vals[0], np.array(vals[1:4]), np.array(vals[4:7])
+            coef = material[2] * omega * abs(omega)
         else:
             f = np.zeros(3)
             f["XYZ".index(kind[1])] = vals[0]
         for _, _, ns in elems:
             loc = [idx[n] for n in ns]
-            if len(loc) == 4:
-                V = np.linalg.det((X[loc[1:]] - X[loc[0]]).T) / 6.0
-                for i in loc:
-                    B[3 * i:3 * i + 3] += V * f / 4.0
-            else:
-                pts = _tet10_points(X[loc]) if len(loc) == 10 else [p[:3] for p in _hex8_points(X[loc])]
-                for shp, _, w in pts:
-                    for a, i in enumerate(loc):
-                        B[3 * i:3 * i + 3] += (w * shp[a]) * f
+            for shp, w in _quadrature(X[loc]):
+                if kind == "CENT":
+                    xg = shp @ X[loc]
+                    f = coef * (xg - (A_ + ((xg - A_) @ R_) / (R_ @ R_) * R_))
+                for a, i in enumerate(loc):
+                    B[3 * i:3 * i + 3] += (w * shp[a]) * f
     fixed = {}
     for target, d0, d1, val in bcs:
         for i in members(target):

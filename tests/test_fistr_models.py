@@ -1,8 +1,9 @@
 """fistr1's own test models assembled by frontistr_amd.mesh, pinned on the
 CPU: the CG solver test (tests/golden/fistr_cg, from the reference's
-tests/solver/CG) and the static examples A, C and D with TET4, TET10 and
+tests/solver/CG) and the static examples A, C, D and E with TET4, TET10 and
 HEX8 elements (tests/golden/fistr_static, from tests/analysis/static: a
-point load, a body force per volume, gravity times the density):
+point load, a body force per volume, gravity times the density, a
+centrifugal load):
 
 * the displacement fistr1 reported (.res, SSOR and DIAG runs) solves the
   assembled system to fistr1's RESID (1e-8): the element stiffness, material,
@@ -37,6 +38,8 @@ MODELS = {
     "static_D341": (os.path.join(STATIC, "D341.msh"), os.path.join(STATIC, "D341.cnt"),
                     os.path.join(STATIC, "D341.res.0.1")),
 }
+for _t in ("341", "342", "361"):
+    MODELS["static_E%s" % _t] = tuple(os.path.join(STATIC, "E%s.%s" % (_t, ext)) for ext in ("msh", "cnt", "res.0.1"))
 for _x in "ACD":
     for _t in ("342", "361"):
         MODELS["static_%s%s" % (_x, _t)] = tuple(os.path.join(STATIC, "%s%s.%s" % (_x, _t, ext))
