@@ -13,7 +13,8 @@ The subset read is what fistr1's own solver tests use
   value), ``!CLOAD`` (node or group, dof, value), ``!DLOAD`` body forces on
   all elements (``BX``/``BY``/``BZ`` force per volume, ``GRAV`` acceleration
   times the density along the normalised direction, ``CENT`` angular speed,
-  axis point and axis direction, as ``DL_C3``,
+  axis point and axis direction) and pressure ``P1``..``P6`` on a face of the
+  elements of an ``!EGROUP`` (types 341 and 361), as ``DL_C3``,
   fistr1/src/lib/static_LIB_3d.f90:213-382, applies them) and ``!SOLVER``
   (METHOD, PRECOND and the
   two data lines: ITER, ITERPREMAX / RESID, SIGMA_DIAG, SIGMA);
@@ -89,8 +90,9 @@ def _fields(line):
 
 def read_msh(path):
     """Nodes {id: (x, y, z)} in file order, elements [(id, type, node ids)],
-    material (E, nu, density or None) and node groups {name: [ids]}."""
-    nodes, elems, groups = {}, [], {}
+    material (E, nu, density or None), node groups and element groups
+    ({name: [ids]} each)."""
+    nodes, elems, groups, egroups = {}, [], {}, {}
     material = None
     density = None
     for header, lines in _blocks(path):
@@ -106,12 +108,13 @@ def read_msh(path):
             for ln in lines:
                 f = [int(v) for v in _fields(ln)]
                 elems.append((f[0], etype, f[1:]))
-        elif key == "NGROUP":
+        elif key in ("NGROUP", "EGROUP"):
             ids = []
             for ln in lines:
                 f = [int(v) for v in _fields(ln)]
                 ids += list(range(f[0], f[1] + 1, f[2] if len(f) > 2 else 1)) if "GENERATE" in opt else f
-            groups.setdefault(opt["NGRP"].upper(), []).extend(ids)
+            name = opt["NGRP" if key == "NGROUP" else "EGRP"].upper()
+            (groups if key == "NGROUP" else egroups).setdefault(name, []).extend(ids)
         elif key.replace(" ", "") == "ITEM=1":
             # '!ITEM=1, SUBITEM=2': Young's modulus and Poisson's ratio
             if opt.get("SUBITEM", "2") != "2":
@@ -122,7 +125,7 @@ def read_msh(path):
             density = float(_fields(lines[0])[0])
     if material is not None:
         material = (material[0], material[1], density)
-    return nodes, elems, material, groups
+    return nodes, elems, material, groups, egroups
 
 
 def read_cnt(path):
@@ -146,7 +149,7 @@ def read_cnt(path):
             for ln in lines:
                 f = _fields(ln)
                 kind = f[1].upper()
-                if kind not in ("BX", "BY", "BZ", "GRAV", "CENT"):
+                if kind not in ("BX", "BY", "BZ", "GRAV", "CENT") and kind not in ("P1", "P2", "P3", "P4", "P5", "P6"):
                     raise ValueError("%s: !DLOAD %s is not read" % (path, kind))
                 dloads.append((f[0].upper(), kind, [float(v) for v in f[2:]]))
         elif key == "SOLVER":
@@ -307,6 +310,32 @@ def _hex8ic_stiffness(X, D):
     return K[:24, :24] - K[:24, 24:] @ np.linalg.solve(Kaa, K[24:, :24])
 
 
+# face k of an element: its nodes in the order whose right-hand rule gives the
+# normal (fistr1/src/lib/element/element.f90 getSubFace)
+_FACES = {
+    341: ((1, 2, 3), (4, 2, 1), (4, 3, 2), (4, 1, 3)),
+    361: ((1, 2, 3, 4), (8, 7, 6, 5), (5, 6, 2, 1), (6, 7, 3, 2), (7, 8, 4, 3), (8, 5, 1, 4)),
+}
+
+
+def _face_points(X):
+    """(shape values, dX/dxi x dX/deta * weight) over a 3- or 4-node face (SurfaceNormal
+    in element.f90: the cross product of the two tangents, not normalised)."""
+    if X.shape[0] == 3:
+        # linear triangle: constant tangents, one point, weight 1/2
+        nrm = np.cross(X[1] - X[0], X[2] - X[0])
+        return [(np.full(3, 1.0 / 3.0), 0.5 * nrm)]
+    out = []
+    s = np.array([[-1, -1], [1, -1], [1, 1], [-1, 1]], dtype=np.float64)
+    for b in (-_G2, _G2):
+        for a in (-_G2, _G2):
+            shp = 0.25 * (1 + s[:, 0] * a) * (1 + s[:, 1] * b)
+            dxi = 0.25 * s[:, 0] * (1 + s[:, 1] * b)
+            deta = 0.25 * s[:, 1] * (1 + s[:, 0] * a)
+            out.append((shp, np.cross(dxi @ X, deta @ X)))
+    return out
+
+
 def _quadrature(X):
     """(shape values, det J * weight) at the quadrature points fistr1 uses for the
     element with nodes X: 1 (the centroid) for 341, 4 for 342, 8 for 361."""
@@ -333,7 +362,7 @@ def _tet4_stiffness(X, D):
 def assemble(msh_path, cnt_path):
     """The static system of a fistr1 model as a :class:`HecMatrix` (solver
     settings of the ``!SOLVER`` block applied) and the node ids of its rows."""
-    nodes, elems, material, groups = read_msh(msh_path)
+    nodes, elems, material, groups, egroups = read_msh(msh_path)
     bcs, loads, dloads, solver = read_cnt(cnt_path)
     if material is None:
         raise ValueError("%s: no !MATERIAL elasticity" % msh_path)
@@ -365,6 +394,21 @@ def assemble(msh_path, cnt_path):
         for i in members(target):
             B[3 * i + dof - 1] += val
     for egrp, kind, vals in dloads:
+        if kind.startswith("P"):
+            face = int(kind[1:])
+            members_e = set(egroups[egrp]) if egrp in egroups else None
+            if members_e is None and egrp != "ALL":
+                raise ValueError("!DLOAD on unknown element group %s" % egrp)
+            for eid, t, ns in elems:
+                if members_e is not None and eid not in members_e:
+                    continue
+                if t not in _FACES:
+                    raise ValueError("pressure on element type %d is not read" % t)
+                loc = [idx[ns[k - 1]] for k in _FACES[t][face - 1]]
+                for shp, nrm in _face_points(X[loc]):
+                    for a, i in enumerate(loc):
+                        B[3 * i:3 * i + 3] += vals[0] * shp[a] * nrm
+            continue
         if egrp != "ALL":
             raise ValueError("!DLOAD on element group %s: only ALL is read" % egrp)
         if kind in ("GRAV", "CENT") and material[2] is None:

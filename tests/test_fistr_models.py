@@ -1,9 +1,9 @@
 """fistr1's own test models assembled by frontistr_amd.mesh, pinned on the
 CPU: the CG solver test (tests/golden/fistr_cg, from the reference's
-tests/solver/CG) and the static examples A, C, D and E with TET4, TET10 and
-HEX8 elements (tests/golden/fistr_static, from tests/analysis/static: a
-point load, a body force per volume, gravity times the density, a
-centrifugal load):
+tests/solver/CG) and the static examples A-E with TET4, TET10 and HEX8
+elements (tests/golden/fistr_static, from tests/analysis/static: a point
+load, a face pressure (B, TET4 and HEX8), a body force per volume, gravity
+times the density, a centrifugal load):
 
 * the displacement fistr1 reported (.res, SSOR and DIAG runs) solves the
   assembled system to fistr1's RESID (1e-8): the element stiffness, material,
@@ -38,6 +38,8 @@ MODELS = {
     "static_D341": (os.path.join(STATIC, "D341.msh"), os.path.join(STATIC, "D341.cnt"),
                     os.path.join(STATIC, "D341.res.0.1")),
 }
+for _t in ("341", "361"):
+    MODELS["static_B%s" % _t] = tuple(os.path.join(STATIC, "B%s.%s" % (_t, ext)) for ext in ("msh", "cnt", "res.0.1"))
 for _t in ("341", "342", "361"):
     MODELS["static_E%s" % _t] = tuple(os.path.join(STATIC, "E%s.%s" % (_t, ext)) for ext in ("msh", "cnt", "res.0.1"))
 for _x in "ACD":
@@ -58,7 +60,9 @@ def _model(prec):
 
 
 def test_reader_sizes_and_solver_settings():
-    nodes, elems, material, groups = mesh.read_msh(os.path.join(HERE, "SSOR.msh"))
+    nodes, elems, material, groups, egroups = mesh.read_msh(os.path.join(HERE, "SSOR.msh"))
+    assert egroups == {}
+    assert mesh.read_msh(MODELS["static_B341"][0])[4]["B340P3"] == list(range(125, 240, 6)) + list(range(126, 241, 6))
     assert len(elems) == 240 and all(t == 341 and len(ns) == 4 for _, t, ns in elems)
     assert material == (4000.0, 0.3, None)
     assert mesh.read_msh(MODELS["static_D341"][0])[2] == (4000.0, 0.3, 8.0102e-10)
@@ -116,6 +120,18 @@ def test_planted_hex8_full_integration_is_caught(monkeypatch):
         return K
     monkeypatch.setattr(mesh, "_hex8ic_stiffness", full)
     msh, cnt, _ = MODELS["static_A361"]
+    mw, _ = mesh.assemble(msh, cnt)
+    rel = np.linalg.norm(dense(mw) @ u - mw.B) / np.linalg.norm(mw.B)
+    assert rel > 1e-2, rel
+
+
+def test_planted_pressure_sign_mistake_is_caught(monkeypatch):
+    """The pressure's normal follows the face's node order (right-hand rule);
+    the opposite orientation is caught."""
+    m, ids, u = model("static_B361")
+    orig = mesh._face_points
+    monkeypatch.setattr(mesh, "_face_points", lambda X: [(s_, -n_) for s_, n_ in orig(X)])
+    msh, cnt, _ = MODELS["static_B361"]
     mw, _ = mesh.assemble(msh, cnt)
     rel = np.linalg.norm(dense(mw) @ u - mw.B) / np.linalg.norm(mw.B)
     assert rel > 1e-2, rel
