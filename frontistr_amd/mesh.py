@@ -43,7 +43,9 @@ The subset read is what fistr1's own solver tests use
 * the nodes are those the elements use, in the order of ``!NODE`` (the mesh
   reader drops nodes no element references), the block profile is every
   node pair sharing an element with items ascending per row
-  (``hecmw_mat_con``, hecmw1/src/solver/matrix/hecmw_mat_con.f90:21-107);
+  (``hecmw_mat_con``, hecmw1/src/solver/matrix/hecmw_mat_con.f90:21-107),
+  assembled sparse (element matrices summed in a scipy COO, then 3x3 blocks)
+  so models of any size load;
 * Dirichlet values by row and column elimination with a unit pivot
   (``hecmw_mat_ass_bc``, hecmw1/src/solver/matrix/hecmw_mat_ass.f90:328-463),
   the right-hand side moved by the prescribed values;
@@ -55,6 +57,7 @@ The subset read is what fistr1's own solver tests use
 import re
 
 import numpy as np
+import scipy.sparse as sp
 
 from .hecmw import HecMatrix
 
@@ -376,14 +379,20 @@ def assemble(msh_path, cnt_path):
     N = len(ids)
     X = np.array([nodes[n] for n in ids])
     D = _D_iso(material[0], material[1])
-    K = np.zeros((3 * N, 3 * N))
-    pairs = set()
+    # element matrices summed into a sparse K; every entry of an element's
+    # node pairs stays stored (the profile of hecmw_mat_con), zero or not
+    rows, cols, vals = [], [], []
     for _, _, ns in elems:
         loc = [idx[n] for n in ns]
         ke = {4: _tet4_stiffness, 10: _tet10_stiffness, 8: _hex8ic_stiffness}[len(loc)](X[loc], D)
         dofs = np.array([[3 * i, 3 * i + 1, 3 * i + 2] for i in loc]).ravel()
-        K[np.ix_(dofs, dofs)] += ke
-        pairs.update((a, b) for a in loc for b in loc)
+        rows.append(np.repeat(dofs, dofs.size))
+        cols.append(np.tile(dofs, dofs.size))
+        vals.append(ke.ravel())
+    K = sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(3 * N, 3 * N)).tocsr()
+    K.sum_duplicates()
+    K.sort_indices()
     B = np.zeros(3 * N)
 
     def members(target):
@@ -431,39 +440,36 @@ def assemble(msh_path, cnt_path):
                     f = coef * (xg - (A_ + ((xg - A_) @ R_) / (R_ @ R_) * R_))
                 for a, i in enumerate(loc):
                     B[3 * i:3 * i + 3] += (w * shp[a]) * f
-    fixed = {}
+    fixed = np.zeros(3 * N, dtype=bool)
+    xfix = np.zeros(3 * N)
     for target, d0, d1, val in bcs:
         for i in members(target):
             for d in range(d0, d1 + 1):
-                fixed[3 * i + d - 1] = val
-    # elimination with a unit pivot: move the prescribed values to the right-hand side
-    for k, val in fixed.items():
-        if val != 0.0:
-            B -= K[:, k] * val
-    for k, val in fixed.items():
-        K[k, :] = 0.0
-        K[:, k] = 0.0
-        K[k, k] = 1.0
-        B[k] = val
-    # block CSR of the element profile, items ascending
-    nbr = [[] for _ in range(N)]
-    for a, b in pairs:
-        if a != b:
-            nbr[a].append(b)
-    for lst in nbr:
-        lst.sort()
+                fixed[3 * i + d - 1] = True
+                xfix[3 * i + d - 1] = val
+    # elimination with a unit pivot: move the prescribed values to the right-hand
+    # side, then zero the rows and columns in place (the entries stay stored)
+    B -= K @ xfix
+    r_of = np.repeat(np.arange(3 * N), np.diff(K.indptr))
+    K.data[fixed[r_of] | fixed[K.indices]] = 0.0
+    K.data[fixed[r_of] & (r_of == K.indices)] = 1.0
+    B[fixed] = xfix[fixed]
+    # 3x3 blocks: row i's items ascending, split at the diagonal
+    Kb = K.tobsr(blocksize=(3, 3))
+    Kb.sort_indices()
     indexL, indexU, itemL, itemU, AL, AU = [0], [0], [], [], [], []
     Dd = np.zeros((N, 3, 3))
     for i in range(N):
-        Dd[i] = K[3 * i:3 * i + 3, 3 * i:3 * i + 3]
-        for j in nbr[i]:
-            blk = K[3 * i:3 * i + 3, 3 * j:3 * j + 3]
+        for q in range(Kb.indptr[i], Kb.indptr[i + 1]):
+            j = Kb.indices[q]
             if j < i:
                 itemL.append(j + 1)
-                AL.append(blk)
-            else:
+                AL.append(Kb.data[q])
+            elif j > i:
                 itemU.append(j + 1)
-                AU.append(blk)
+                AU.append(Kb.data[q])
+            else:
+                Dd[i] = Kb.data[q]
         indexL.append(len(itemL))
         indexU.append(len(itemU))
     m = HecMatrix(N, Dd.ravel(), np.array(AL).ravel() if AL else np.zeros(0),
