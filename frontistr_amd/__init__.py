@@ -35,6 +35,7 @@ from .hecmw import (  # noqa: F401
     hecmw_rel_resid_L2,
     hecmw_ordering,
     gen_hex8,
+    gen_hex8_structure,
     gen_tet10,
     hex8_coords,
     tet10_coords,

@@ -119,6 +119,7 @@ extern "C" int32_t hecmw33_gen_hex8_count(int32_t n, int32_t* N, int32_t* NPL, i
 // Loads: unit pressure on z=1 (consistent nodal forces, -z).
 // BCs: every dof of every z=0 node fixed to 0 (hecmw_mat_ass_bc).
 // Arrays use hecmw1 conventions: index*(0:N) cumulative, items 1-based.
+// D == nullptr fills only the profile (indexL/itemL/indexU/itemU).
 extern "C" int32_t hecmw33_gen_hex8(int32_t n, int32_t mat_kind, const double* p, uint64_t seed,
                                     double* D, double* AL, double* AU, int32_t* indexL,
                                     int32_t* itemL, int32_t* indexU, int32_t* itemU, double* B,
@@ -189,6 +190,9 @@ extern "C" int32_t hecmw33_gen_hex8(int32_t n, int32_t mat_kind, const double* p
       }
     }
   }
+
+  // structure only (D == nullptr): the profile is all the orderings need
+  if (!D) return 0;
 
   // ---- material field ------------------------------------------------------
   std::vector<double> lam(ne), mu(ne);

@@ -255,6 +255,7 @@ struct Context {
   // two-level correction: node coordinates (hecMESH%node), aggregation, coarse inverse
   std::vector<double> h_coord;   // 3*NP
   int64_t coord_gen = 0;         // bumped whenever coordinates are set
+  bool coord_fresh = false;      // set since the last structure load (kept by the next one)
   bool coarse_sym = false;       // aggregation built for (structure, coordinates, target)
   bool coarse_num = false;       // E^-1 built for the current values
   bool coarse_ok = false;        // E was positive definite
@@ -391,12 +392,24 @@ static int32_t check_sys(const hecmw33_sys* A) {
   return 0;
 }
 
-// Structure of A (rebuilt when sizes, array identities or the symbolic flag change).
+// Structure of A: rebuilt when the symbolic flag asks for it or when the
+// sizes or the contents of index/item differ from the kept host copies (a
+// new system may sit at the addresses of a freed one, so addresses alone
+// do not identify a structure).
 static int32_t load_structure(Context& c, const hecmw33_sys* A, bool force) {
-  const bool same = c.have_struct && c.N == A->N && c.NP == A->NP && c.NPL == A->NPL && c.NPU == A->NPU &&
-                    c.key[0] == A->indexL && c.key[1] == A->indexU && c.key[2] == A->itemL &&
-                    c.key[3] == A->itemU;
+  bool same = c.have_struct && c.N == A->N && c.NP == A->NP && c.NPL == A->NPL && c.NPU == A->NPU;
+  if (same) {
+    const size_t np1 = (size_t)A->NP + 1;
+    same = std::equal(A->indexL, A->indexL + np1, c.h_indexL.data()) &&
+           std::equal(A->indexU, A->indexU + np1, c.h_indexU.data()) &&
+           (A->NPL == 0 || std::equal(A->itemL, A->itemL + A->NPL, c.h_itemL.data())) &&
+           (A->NPU == 0 || std::equal(A->itemU, A->itemU + A->NPU, c.h_itemU.data()));
+  }
   if (same && !force) return 0;
+  // node coordinates given before a different structure was loaded belong
+  // to an earlier system; those given since the last load are for this one
+  if (!same && !c.coord_fresh) c.h_coord.clear();
+  c.coord_fresh = false;
   const int32_t N = A->N, NP = A->NP;
   c.h_indexL.assign(A->indexL, A->indexL + NP + 1);
   c.h_indexU.assign(A->indexU, A->indexU + NP + 1);
@@ -1089,8 +1102,10 @@ static int32_t coarse_aim(const Context& c, int32_t opt3) {
 }
 
 static int32_t coarse_setup(Context& c, int32_t target) {
-  if ((int64_t)c.h_coord.size() < 3 * (int64_t)c.NP) {
-    set_error("SOLVER_OPT(3) (two-level correction) needs the node coordinates (hecmw33_set_coords)");
+  if ((int64_t)c.h_coord.size() != 3 * (int64_t)c.NP) {
+    set_error(c.h_coord.empty()
+                  ? "SOLVER_OPT(3) (two-level correction) needs the node coordinates of this system (hecmw33_set_coords)"
+                  : "SOLVER_OPT(3): the node coordinates given do not have 3*NP entries for this system");
     return HECMW33_ERROR_ARGS;
   }
   target = coarse_aim(c, target);
@@ -1926,7 +1941,7 @@ int32_t hecmw33_ctx_load(hecmw33_ctx* ctx, const hecmw33_sys* A, const int32_t i
   // aggregation (symbolic) and the first coarse inverse, which also brings
   // up the dense-solver library once
   const int32_t o3 = IA(iarray, I_SOLVER_OPT3);
-  if (o3 > 0 && (int64_t)ctx->c.h_coord.size() >= 3 * (int64_t)ctx->c.NP)
+  if (o3 > 0 && (int64_t)ctx->c.h_coord.size() == 3 * (int64_t)ctx->c.NP)
     if (int32_t e = coarse_setup(ctx->c, o3)) return e;
   return 0;
 }
@@ -2314,6 +2329,7 @@ static int32_t set_coords(Context& c, int32_t NP, const double* xyz) {
   if (NP < 0 || (NP > 0 && !xyz)) return HECMW33_ERROR_ARGS;
   c.h_coord.assign(xyz, xyz + 3 * (size_t)NP);
   ++c.coord_gen;
+  c.coord_fresh = true;
   return 0;
 }
 

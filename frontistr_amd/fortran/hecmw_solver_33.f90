@@ -12,6 +12,12 @@
 ! hecmw_solve_iterative handles them: hecmw_solve_error (which stops on
 ! 1001/2001) and the ITERLOG/TIMELOG lines.
 !
+! The side effects of hecmw_solve_iterative around the Krylov call are kept:
+! hecmw_mat_dump before the solve (DUMPTYPE, Iarray 31; DUMPEXIT stops there),
+! hecmw_mat_dump_solution after it (hecmw_solver_Iterative.f90:90,160), and the
+! TIMELOG=VERBOSE matvec bandwidth/FLOPs block of hecmw_output_flops (:185,
+! :189-254) from the GPU's SpMV time.
+!
 ! hecmw33_available() tells a caller whether the GPU path can take a system
 ! (library loaded, a device visible, METHOD 1/2, PRECOND 1/2/3, one domain,
 ! multipoint constraints the MPC-CG product supports; SCALING is applied on
@@ -106,6 +112,7 @@ contains
 
   subroutine hecmw_solve_33(hecMESH, hecMAT)
     use m_hecmw_solve_error
+    use hecmw_matrix_dump
     type(hecmwST_local_mesh) :: hecMESH
     type(hecmwST_matrix), target :: hecMAT
     type(hecmw33_sys) :: A
@@ -153,6 +160,10 @@ contains
       endif
     endif
 
+    ! the system as handed over (DUMPTYPE; the dump numbering is the reference's
+    ! own counter, shared with every other hecmw_mat_dump call of the process)
+    call hecmw_mat_dump(hecMAT, hecMESH)
+
     rc = c_hecmw33_solve(A, c_loc(hecMAT%B(1)), c_loc(hecMAT%X(1)), ia, ra, out)
     hecMAT%Iarray(:) = ia(:)
     hecMAT%Rarray(:) = ra(:)
@@ -181,6 +192,7 @@ contains
     if (hecMESH%my_rank == 0 .and. (hecMAT%Iarray(21) == 1 .or. hecMAT%Iarray(22) >= 1)) then
       write(*,"(a,1pe12.5)") '### Relative residual =', out%resid_true
     endif
+    call hecmw_mat_dump_solution(hecMAT)
     if (hecMESH%my_rank == 0 .and. hecMAT%Iarray(22) >= 1) then
       TR = (out%Tsol - out%Tcomm) / (out%Tsol + 1.d-24) * 100.d0
       write(*,'(/a)')          '### summary of linear solver'
@@ -193,6 +205,59 @@ contains
       if (out%iter > 0) write(*,'(a, 1pe16.6 )') '    solver/1 iter    : ', out%Tsol / out%iter
       write(*,'(a, 1pe16.6/)') '    work ratio (%)   : ', TR
     endif
+    if (hecMAT%Iarray(22) == 2) call output_flops(hecMESH, hecMAT, out%iter, out%time_Ax)
   end subroutine hecmw_solve_33
+
+  ! TIMELOG=VERBOSE: the matrix's memory amount, the matvec bandwidth and FLOP
+  ! rate over count_Ax products taking time_Ax seconds, with their statistics
+  ! over the ranks, in hecmw_output_flops' lines and units
+  ! (hecmw_solver_Iterative.f90:189-254: bytes 8*9*NZ + 4*(NPU+NPL) + 8*(N+1)
+  ! + 24*N + 24*NP, 18*NZ flops, NZ = N + NPU + NPL, SI prefixes in 1000s).
+  subroutine output_flops(hecMESH, hecMAT, count_Ax, time_Ax)
+    use m_hecmw_comm_f
+    use hecmw_solver_misc
+    type(hecmwST_local_mesh) :: hecMESH
+    type(hecmwST_matrix) :: hecMAT
+    integer(c_int32_t), intent(in) :: count_Ax
+    real(c_double), intent(in) :: time_Ax
+    character(2), parameter :: prefix(0:6) = ['  ', ' K', ' M', ' G', ' T', ' P', ' E']
+    character(len=29), parameter :: label(2) = ['matvec memory band width     ', 'matvec FLOPs                 ']
+    character(len=5), parameter :: unit(2) = ['B/s  ', 'FLOPs']
+    integer(kind=kint) :: NZ, k, e
+    real(kind=kreal) :: bytes, flops, rate, scaled, t_max, t_min, t_avg, t_sd
+
+    NZ = hecMAT%N + hecMAT%indexU(hecMAT%N) + hecMAT%indexL(hecMAT%N)
+    bytes = dble(kreal) * NZ * 9 + dble(kint) * (hecMAT%indexU(hecMAT%N) + hecMAT%indexL(hecMAT%N)) &
+          + dble(kint) * 2 * (hecMAT%N + 1) + dble(kreal) * 3 * (hecMAT%N + hecMAT%NP)
+    flops = 2.0d0 * NZ * 9
+    call hecmw_allreduce_R1(hecMESH, bytes, hecmw_sum)
+    call hecmw_allreduce_R1(hecMESH, flops, hecmw_sum)
+    call si_scale(bytes, scaled, e)
+    if (hecMESH%my_rank == 0) write(*, "(a,f11.3,a,a)") 'memory amount of coef. matrix: ', scaled, prefix(e), 'B'
+    do k = 1, 2
+      if (k == 1) then
+        rate = count_Ax * bytes / time_Ax
+      else
+        rate = count_Ax * flops / time_Ax
+      endif
+      call si_scale(rate, scaled, e)
+      call hecmw_time_statistics(hecMESH, scaled, t_max, t_min, t_avg, t_sd)
+      if (hecMESH%my_rank == 0) then
+        write(*, "(a,f11.3,a,a)") label(k) // ': ', scaled, prefix(e), trim(unit(k))
+        write(*, "(a,f11.3)") '  Max     :', t_max
+        write(*, "(a,f11.3)") '  Min     :', t_min
+        write(*, "(a,f11.3)") '  Avg     :', t_avg
+        write(*, "(a,f11.3)") '  Std Dev :', t_sd
+      endif
+    enddo
+  contains
+    subroutine si_scale(v, scaled, e)
+      real(kind=kreal), intent(in) :: v
+      real(kind=kreal), intent(out) :: scaled
+      integer(kind=kint), intent(out) :: e
+      e = int(log(v) / log(1000.d0))
+      scaled = v / 1000.d0**e
+    end subroutine si_scale
+  end subroutine output_flops
 
 end module hecmw_solver_33

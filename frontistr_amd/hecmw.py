@@ -412,6 +412,26 @@ def gen_hex8(n, kind="uniform", E=210000.0, nu=0.3, seed=0, emin=1e3, emax=1e6, 
     return m
 
 
+def gen_hex8_structure(n):
+    """The profile alone of gen_hex8(n) (no values, loads or boundary
+    conditions): an object with N, NP, NPL, NPU, indexL, itemL, indexU, itemU,
+    for the orderings of large cubes."""
+    from types import SimpleNamespace
+    L = lib()
+    N, NPL, NPU = C.c_int32(), C.c_int32(), C.c_int32()
+    _check(L.hecmw33_gen_hex8_count(n, C.byref(N), C.byref(NPL), C.byref(NPU)))
+    N, NPL, NPU = N.value, NPL.value, NPU.value
+    iL = np.empty(N + 1, np.int32)
+    iU = np.empty(N + 1, np.int32)
+    tL = np.empty(NPL, np.int32)
+    tU = np.empty(NPU, np.int32)
+    pa = np.array([210000.0, 0.3] + [0.0] * 6)
+    rc = L.hecmw33_gen_hex8(n, 0, _ptr(pa), 0, None, None, None, _ptr(iL), _ptr(tL), _ptr(iU), _ptr(tU), None, None)
+    if rc != 0:
+        raise HecmwError("hecmw33_gen_hex8 (structure) failed (%d)" % rc)
+    return SimpleNamespace(N=N, NP=N, NPL=NPL, NPU=NPU, indexL=iL, itemL=tL, indexU=iU, itemU=tU)
+
+
 def hex8_coords(n):
     """Node coordinates of gen_hex8(n) (node i + (n+1)(j + (n+1)k) at (i, j, k)/n), 3*(n+1)^3."""
     nn = n + 1

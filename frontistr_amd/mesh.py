@@ -98,10 +98,23 @@ def read_msh(path):
     nodes, elems, groups, egroups = {}, [], {}, {}
     material = None
     density = None
+    n_material = n_section = 0
     for header, lines in _blocks(path):
         key, opt = _opts(header)
         if key == "END":
             break
+        # one material on every element: a second !MATERIAL or !SECTION would
+        # give some elements other properties, which this reader does not map
+        if key == "MATERIAL":
+            n_material += 1
+            if n_material > 1:
+                raise ValueError("%s: more than one !MATERIAL; only single-material models are read" % path)
+        elif key == "SECTION":
+            n_section += 1
+            if n_section > 1:
+                raise ValueError("%s: more than one !SECTION; only single-section models are read" % path)
+            if opt.get("TYPE", "SOLID").upper() != "SOLID":
+                raise ValueError("%s: !SECTION TYPE=%s is not read" % (path, opt.get("TYPE")))
         if key == "NODE":
             for ln in lines:
                 f = _fields(ln)
@@ -156,13 +169,24 @@ def read_cnt(path):
                     raise ValueError("%s: !DLOAD %s is not read" % (path, kind))
                 dloads.append((f[0].upper(), kind, [float(v) for v in f[2:]]))
         elif key == "SOLVER":
+            # header parameters and data lines as fstr_ctrl_get_SOLVER reads them
+            # (fistr1/src/common/fstr_ctrl_common.f90:138-201)
             solver["METHOD"] = opt.get("METHOD", "CG").upper()
+            if "METHOD2" in opt:
+                solver["METHOD2"] = opt["METHOD2"].upper()
             solver["PRECOND"] = int(opt.get("PRECOND", "1"))
+            for name in ("SCALING", "ITERLOG", "TIMELOG"):
+                if name in opt:
+                    words = ("NO", "YES", "VERBOSE") if name == "TIMELOG" else ("NO", "YES")
+                    v = opt[name].upper()
+                    if v not in words:
+                        raise ValueError("%s: !SOLVER %s=%s" % (path, name, opt[name]))
+                    solver[name] = words.index(v)
             if len(lines) > 0:
-                f = _fields(lines[0])
-                solver["ITER"] = int(f[0])
-                if len(f) > 1:
-                    solver["ITERPREMAX"] = int(f[1])
+                # nier, iterpremax, nrest, ncolor_in, recyclepre (the 'iiiiii' line)
+                f = [int(v) for v in _fields(lines[0])]
+                for name, v in zip(("ITER", "ITERPREMAX", "NREST", "NCOLOR_IN", "RECYCLEPRE"), f):
+                    solver[name] = v
             if len(lines) > 1:
                 f = [float(v) for v in _fields(lines[1])]
                 solver["RESID"] = f[0]
@@ -362,6 +386,22 @@ def _tet4_stiffness(X, D):
     return (det / 6.0) * (B.T @ D @ B)
 
 
+# !SOLVER METHOD names and numbers (fstr_ctrl_common.f90:119-171: names after
+# the five numbers, CR and the GMRES variants counted among the iterative ones)
+_METHODS = {"1": 1, "2": 2, "3": 3, "4": 4, "5": 5, "CG": 1, "BICGSTAB": 2, "GMRES": 3, "GPBICG": 4,
+            "GMRESR": 5, "GMRESREN": 6, "CR": 7}
+
+
+def _method_code(name, path, library_only=True):
+    code = _METHODS.get(name.upper())
+    if code is None:
+        raise ValueError("%s: !SOLVER METHOD=%s is not an iterative method this reader knows" % (path, name))
+    if library_only and code not in (1, 2):
+        raise ValueError("%s: !SOLVER METHOD=%s: the GPU library provides CG (1) and BiCGSTAB (2) only"
+                         % (path, name))
+    return code
+
+
 def assemble(msh_path, cnt_path):
     """The static system of a fistr1 model as a :class:`HecMatrix` (solver
     settings of the ``!SOLVER`` block applied) and the node ids of its rows."""
@@ -475,12 +515,26 @@ def assemble(msh_path, cnt_path):
     m = HecMatrix(N, Dd.ravel(), np.array(AL).ravel() if AL else np.zeros(0),
                   np.array(AU).ravel() if AU else np.zeros(0), np.array(indexL, np.int32),
                   np.array(itemL, np.int32), np.array(indexU, np.int32), np.array(itemU, np.int32), B=B)
-    m.set_method({"CG": 1, "1": 1, "BICGSTAB": 2, "2": 2}[solver.get("METHOD", "CG")])
+    m.set_method(_method_code(solver.get("METHOD", "CG"), cnt_path))
+    if "METHOD2" in solver:
+        m.set_method2(_method_code(solver["METHOD2"], cnt_path, library_only=False))
     m.set_precond(solver.get("PRECOND", 1))
     if "ITER" in solver:
         m.set_iter(solver["ITER"])
     if "ITERPREMAX" in solver:
         m.set_iterpremax(solver["ITERPREMAX"])
+    if "NREST" in solver:
+        m.set_i(6, solver["NREST"])
+    if "NCOLOR_IN" in solver:
+        m.set_ncolor_in(solver["NCOLOR_IN"])
+    if "RECYCLEPRE" in solver:
+        m.set_i(35, solver["RECYCLEPRE"])
+    if "SCALING" in solver:
+        m.set_i(7, solver["SCALING"])
+    if "ITERLOG" in solver:
+        m.set_i(21, solver["ITERLOG"])
+    if "TIMELOG" in solver:
+        m.set_i(22, solver["TIMELOG"])
     if "RESID" in solver:
         m.set_resid(solver["RESID"])
     if "SIGMA_DIAG" in solver:

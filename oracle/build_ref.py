@@ -14,7 +14,7 @@ What is compiled
     Fortran objects call (timer, comm/control init, message table, ML wrapper
     compiled without ML).  Nothing is stubbed: every symbol resolves to
     reference code.
-  flags: flang -cpp -DHECMW_SERIAL -O2 -fopenmp (the reference's serial+OpenMP
+  flags: flang -cpp -DHECMW_SERIAL -O3 -fopenmp (the reference's serial+OpenMP
   configuration; OpenMP with >=2 threads selects the RCM+multicolour SSOR
   ordering, hecmw_precond_SSOR_33.F90:106-127).
 
@@ -40,7 +40,7 @@ C_FILES = [
     "common/hecmw_error.c", "common/hecmw_lib_fc.c", "common/hecmw_msg.c", "common/hecmw_msg_table.c",
     "solver/precond/ml/hecmw_ML_wrapper.c",
 ]
-FFLAGS = ["-cpp", "-DHECMW_SERIAL", "-O2", "-fopenmp", "-fPIC"]
+FFLAGS = ["-cpp", "-DHECMW_SERIAL", "-O3", "-fopenmp", "-fPIC"]
 
 
 def fortran_sources(src):

@@ -81,6 +81,7 @@ def _capture_stdout(fn):
     import sys
     import tempfile
     sys.stdout.flush()
+    lib().ref_flush()  # what earlier calls left in Fortran's buffer goes to the real stdout
     saved = os.dup(1)
     with tempfile.TemporaryFile() as tf:
         os.dup2(tf.fileno(), 1)
@@ -130,16 +131,34 @@ def solve_krylov(m, x0=None, nthreads=NTHREADS, iarray=None, rarray=None, histor
     return x, info
 
 
-def solve_iterative(m, x0=None, nthreads=NTHREADS):
-    """hecmw_solve_iterative (full dispatcher).  Returns (x, iarray_out, rarray_out)."""
+def solve_iterative(m, x0=None, nthreads=NTHREADS, iarray=None, rarray=None):
+    """hecmw_solve_iterative (full dispatcher).  Returns (x, iarray_out, rarray_out).
+    iarray/rarray (default m's) are passed as they are, so a sequence of calls
+    can carry the recycle counters (Iarray 96-98) the way fistr1 does; the
+    reference keeps its preconditioner between calls unless they ask for a
+    new one."""
     x = (np.zeros(3 * m.NP) if x0 is None else np.array(x0, dtype=np.float64)).copy()
     b = m.B.copy()
-    ia = np.array(m.Iarray, dtype=np.int32).copy()
-    ra = np.array(m.Rarray, dtype=np.float64).copy()
+    ia = np.array(m.Iarray if iarray is None else iarray, dtype=np.int32).copy()
+    ra = np.array(m.Rarray if rarray is None else rarray, dtype=np.float64).copy()
     iout = np.zeros(100, np.int32)
     rout = np.zeros(100)
     lib().ref_solve_iterative(*_mat_args(m), _p(b), _p(x), _p(ia), _p(ra), nthreads, _p(iout), _p(rout))
     return x, iout, rout
+
+
+def solve_iterative_iter(m, x0=None, nthreads=NTHREADS, iarray=None, rarray=None):
+    """solve_iterative with TIMELOG=YES, returning (x, iarray_out, rarray_out,
+    iterations) where the iterations are read from the summary the reference
+    prints ('ITER iterations RESID', hecmw_solver_Iterative.f90:173)."""
+    ia = np.array(m.Iarray if iarray is None else iarray, dtype=np.int32).copy()
+    ia[21] = 1  # Iarray(22): TIMELOG=YES
+    (x, iout, rout), text = _capture_stdout(lambda: solve_iterative(m, x0, nthreads, ia, rarray))
+    iters = [int(l.split()[0]) for l in text.splitlines() if len(l.split()) == 3 and l.split()[1] == "iterations"]
+    if not iters:
+        raise RuntimeError("no iteration summary in the reference's output:\n" + text[-2000:])
+    iout[21] = 0 if iarray is None else np.asarray(iarray)[21]
+    return x, iout, rout, iters[-1]
 
 
 def ordering(m, ncolor_in=10):

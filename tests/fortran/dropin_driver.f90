@@ -4,12 +4,19 @@
 ! sources) and once by hecmw_solve_33 (GPU, frontistr_amd/fortran), both
 ! called from Fortran exactly as fistr1 would call them.
 !
-!   dropin_test <n> <method> <precond> [scaling] [fcg] [mpc]    (OMP_NUM_THREADS >= 2)
+!   dropin_test <n> <method> <precond> [scaling] [fcg] [mpc] [extra]    (OMP_NUM_THREADS >= 2)
 ! mpc = 1: the z displacement of every top-face node (but one corner) is tied
 ! to the top-face centre node's (hecMESH%mpc, MPC_METHOD = 2), B replaced by
 ! T^T B as fistr1 hands it over; both sides then solve with T^T A T products
 ! fcg = 1: the GPU side runs the flexible CG with FP32 operators and the
 ! two-level correction (Iarray 41 = 1, 43 = 1), with hecMESH%node set
+! extra = 1: both sides with DUMPTYPE=BSR and TIMELOG=VERBOSE, so each writes
+! dump_matrix_<call>_0.{bsr,rhs,sol} into the working directory (call 1 the
+! reference's, 2 the GPU's) and prints its summary and matvec bandwidth lines
+! extra = 2: a Newton-like sequence of six solves of the same profile with new
+! values each time (stiffer, plus a diagonal term), Iarray(97) = 2 on the first
+! call and 1 afterwards (fstr_solve_NonLinear.f90:127-131), MAXRECYCLE = 3;
+! one line per call:  DROPIN_SEQ k nrecycle numfact symbfact (reference, GPU) relerr
 ! prints one line:  DROPIN n method precond conv_ref conv_gpu div_ref div_gpu relerr
 !-------------------------------------------------------------------------------
 program dropin_test
@@ -42,7 +49,9 @@ program dropin_test
   type(hecmwST_matrix), target :: mref, mgpu
   type(hecmwST_local_mesh) :: mesh
   integer(c_int32_t) :: n, N_, NPL, NPU, rc
-  integer :: method, precond, scaling, fcg, mpc, i, j, k, nn, c, s, t, ks, km
+  integer :: method, precond, scaling, fcg, mpc, extra, i, j, k, nn, c, s, t, ks, km, call_k
+  real(kind=kreal), allocatable :: D0(:), AL0(:), AU0(:), B0(:)
+  real(kind=kreal) :: f, dmean
   character(len=32) :: arg
   real(c_double) :: p(8)
   real(kind=kreal) :: relerr
@@ -61,6 +70,10 @@ program dropin_test
   mpc = 0
   if (command_argument_count() >= 6) then
     call get_command_argument(6, arg); read(arg, *) mpc
+  endif
+  extra = 0
+  if (command_argument_count() >= 7) then
+    call get_command_argument(7, arg); read(arg, *) extra
   endif
 
   rc = gen_count(n, N_, NPL, NPU)
@@ -137,6 +150,64 @@ program dropin_test
     call hecmw_mat_set_mpc_method(mgpu, 2)
   endif
 
+  if (extra == 1) then
+    mref%Iarray(31) = 3
+    mgpu%Iarray(31) = 3
+    call hecmw_mat_set_timelog(mref, 2)
+    call hecmw_mat_set_timelog(mgpu, 2)
+  endif
+
+  if (extra == 2) then
+    allocate(D0(size(mref%D)), AL0(size(mref%AL)), AU0(size(mref%AU)), B0(size(mref%B)))
+    D0 = mref%D
+    AL0 = mref%AL
+    AU0 = mref%AU
+    B0 = mref%B
+    dmean = sum(D0(1::9)) / N_
+    relerr = 0.d0
+    do call_k = 0, 5
+      f = 1.d0 + 0.2d0 * call_k
+      mref%AL = AL0 * f
+      mref%AU = AU0 * f
+      mref%D = D0
+      do i = 1, N_
+        if (D0(9*i-8) /= 1.d0) then   ! a free node (fixed ones keep their unit rows)
+          mref%D(9*i-8:9*i) = D0(9*i-8:9*i) * f
+          mref%D(9*i-8) = mref%D(9*i-8) + 0.05d0 * call_k * dmean
+          mref%D(9*i-4) = mref%D(9*i-4) + 0.05d0 * call_k * dmean
+          mref%D(9*i) = mref%D(9*i) + 0.05d0 * call_k * dmean
+        endif
+      enddo
+      mref%B = B0 * (1.d0 + 0.1d0 * call_k)
+      mgpu%D = mref%D
+      mgpu%AL = mref%AL
+      mgpu%AU = mref%AU
+      mgpu%B = mref%B
+      do k = 1, 2
+        if (k == 1) then
+          call newton_flags(mref, call_k)
+        else
+          call newton_flags(mgpu, call_k)
+        endif
+      enddo
+      mref%X = 0.d0
+      mgpu%X = 0.d0
+      call hecmw_solve_iterative(mesh, mref)
+      if (.not. hecmw33_available(mesh, mgpu)) then
+        write(*,'(a)') 'DROPIN unavailable'
+        stop 1
+      endif
+      call hecmw_solve_33(mesh, mgpu)
+      write(*,'(a,1x,i0,6(1x,i0),1x,es12.4)') 'DROPIN_SEQ', call_k, mref%Iarray(96:98), mgpu%Iarray(96:98), &
+        sqrt(sum((mgpu%X - mref%X)**2) / sum(mref%X**2))
+      relerr = max(relerr, sqrt(sum((mgpu%X - mref%X)**2) / sum(mref%X**2)))
+    enddo
+    write(*,'(a,3(1x,i0),4(1x,i0),1x,es12.4)') 'DROPIN', n, method, precond, &
+      hecmw_mat_get_flag_converged(mref), hecmw_mat_get_flag_converged(mgpu), &
+      hecmw_mat_get_flag_diverged(mref), hecmw_mat_get_flag_diverged(mgpu), relerr
+    stop
+  endif
+
   call hecmw_solve_iterative(mesh, mref)
   if (.not. hecmw33_available(mesh, mgpu)) then
     write(*,'(a)') 'DROPIN unavailable'
@@ -149,6 +220,19 @@ program dropin_test
     hecmw_mat_get_flag_diverged(mref), hecmw_mat_get_flag_diverged(mgpu), relerr
 
 contains
+
+  ! the recycle flags fistr1's Newton loop sets before each solve
+  subroutine newton_flags(m, it)
+    type(hecmwST_matrix) :: m
+    integer :: it
+    if (it == 0) then
+      m%Iarray(98) = 1
+      m%Iarray(97) = 2
+    else
+      m%Iarray(97) = 1
+    endif
+    m%Iarray(35) = 3
+  end subroutine newton_flags
 
   subroutine setup(m)
     type(hecmwST_matrix), target :: m

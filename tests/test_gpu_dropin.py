@@ -32,3 +32,50 @@ def test_fortran_dropin_matches_reference(n, method, precond, scaling, fcg, mpc)
     assert conv_ref == conv_gpu == 1
     assert div_ref == div_gpu == 0
     assert relerr <= 1e-6
+
+
+def _run(args, cwd=None):
+    env = dict(os.environ, OMP_NUM_THREADS="2")
+    p = subprocess.run([EXE] + [str(a) for a in args], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       env=env, timeout=120, cwd=cwd)
+    assert p.returncode == 0, p.stdout
+    return p.stdout
+
+
+@pytest.mark.parametrize("method,precond", [(1, 1), (2, 3)])
+def test_fortran_dropin_dump_and_verbose_timelog(tmp_path, method, precond):
+    """DUMPTYPE=BSR and TIMELOG=VERBOSE on both sides: hecmw_solve_33 writes the
+    reference's dump files (hecmw_mat_dump before the solve, hecmw_mat_dump_solution
+    after it; hecmw_solver_Iterative.f90:90,160), byte-identical for the matrix
+    and right-hand side, and prints hecmw_output_flops' lines (:189-254)."""
+    import numpy as np
+    out = _run([6, method, precond, 0, 0, 0, 1], cwd=str(tmp_path))
+    f = [l for l in out.splitlines() if l.startswith("DROPIN")][-1].split()
+    assert int(f[4]) == int(f[5]) == 1 and float(f[8]) <= 1e-6
+    for ext in ("bsr", "rhs"):
+        a = (tmp_path / ("dump_matrix_1_0." + ext)).read_bytes()
+        b = (tmp_path / ("dump_matrix_2_0." + ext)).read_bytes()
+        assert a == b and len(a) > 0, ext
+    xa = np.loadtxt(tmp_path / "dump_matrix_1_0.sol")
+    xb = np.loadtxt(tmp_path / "dump_matrix_2_0.sol")
+    assert np.linalg.norm(xa - xb) <= 1e-6 * np.linalg.norm(xa)
+    lines = out.splitlines()
+    mem = [l for l in lines if l.startswith("memory amount of coef. matrix")]
+    assert len(mem) == 2 and mem[0] == mem[1], mem
+    for key in ("matvec memory band width", "matvec FLOPs"):
+        assert sum(1 for l in lines if l.startswith(key)) == 2, key
+    assert sum(1 for l in lines if "### summary of linear solver" in l) == 2
+
+
+@pytest.mark.parametrize("method,precond", [(1, 1), (1, 3)])
+def test_fortran_dropin_newton_sequence(method, precond):
+    """Six solves with new values each time and fistr1's recycle flags: after
+    every call the recycle counters (Iarray 96-98) agree with the reference's,
+    and the solutions are within 1e-6."""
+    out = _run([7, method, precond, 0, 0, 0, 2])
+    seq = [l.split() for l in out.splitlines() if l.startswith("DROPIN_SEQ")]
+    assert len(seq) == 6
+    for f in seq:
+        assert f[2:5] == f[5:8], f
+        assert float(f[8]) <= 1e-6, f
+    assert [int(f[2]) for f in seq] == [0, 1, 2, 3, 0, 1]

@@ -57,3 +57,20 @@ def test_coloring_is_race_free(name, m, nc):
         nb = list(m.itemL[m.indexL[i - 1]:m.indexL[i]]) + [j for j in m.itemU[m.indexU[i - 1]:m.indexU[i]] if j <= N]
         for j in nb:
             assert color[i] != color[j], (i, j)
+
+
+@needs_ref
+@pytest.mark.parametrize("n", [100, 160])
+def test_ordering_bit_exact_at_benchmark_sizes(n):
+    """The 100^3 and 160^3 cubes (1,030,301 and 4,173,281 rows, 17 colours):
+    long BFS levels where the unstable middle-pivot sort's tie-breaks and the
+    capped greedy colouring decide most positions
+    (hecmw_matrix_ordering_CM.f90:207-233, hecmw_matrix_ordering_MC.f90:15-72)."""
+    m = fa.gen_hex8_structure(n)
+    pr, p, ip, ci = fa.hecmw_ordering(m, 10)
+    rpr, _, rp, rip, rci = ref.ordering(m, 10)
+    assert np.array_equal(pr, rpr), "RCM permutation differs"
+    assert np.array_equal(p, rp), "multicolour permutation differs"
+    assert np.array_equal(ip, rip)
+    assert np.array_equal(ci, rci), "COLORindex differs"
+    assert len(ci) - 1 == 17
