@@ -1,29 +1,32 @@
 #!/usr/bin/env python3
-"""Benchmark: time to 1e-8 relative residual of hecmw1's 3x3-block CG on one MI355X.
+"""Benchmark: time to 1e-8 relative residual on one MI355X -- mixed-precision +
+learned flexible CG against the plain FP64 multicolour-SSOR CG; BCSR SpMV GB/s.
 
-Workload (default): 100^3 HEX8 linear-elastic unit cube, 3,090,903 DOF,
-27,270,901 3x3 blocks; E=210000, nu=0.3, bottom face fixed, unit pressure on
-top; assembled like fistr1 (frontistr_amd/csrc/gen_hex8.cpp).  One step = one
-complete FP64 solve x0=0 -> 1e-8 relative residual with the multicolour-SSOR
-preconditioned CG (METHOD=1, PRECOND=1, NCOLOR_IN=10), matrix and vectors
-resident in HBM, preconditioner set up inside the step (numeric refresh;
-the symbolic RCM+multicolour ordering is built once at load).
+Headline workload (the `value`): the 160^3 HEX8 cube with per-element E
+log-uniform in [1e3, 1e6] and nu uniform in [0.2, 0.4] (seed 0), 12,519,843
+DOF, 111,284,641 3x3 blocks, assembled like fistr1 (gen_hex8.cpp).  One step
+= one complete solve x0 = 0 -> FP64 true relative residual 1e-8 with the
+flexible CG (FP32 SpMV/SSOR/vectors, FP64 inner products, x and residual
+replacement) whose symmetric two-level step runs on the aggregation the
+learned network refines (Iarray 41 = 2, 43 = 1), matrix and vectors resident
+in HBM, numeric set-up (SSOR factors, coarse matrix, the network's mask)
+inside the step.  The network is trained first, untimed, on errors the solver
+harvests from SSOR-CG runs on seeded variants of the two-phase workload.
 
-Also reported: DIAG-CG on the same system, the FP64/FP32 SpMV bandwidth, and
-the reference's own CPU solver (oracle/_ref, built from the FrontISTR sources)
-timed on the host on a bounded sample of the same solve; the 160^3
-heterogeneous cube ("mixed": FP64 SSOR-CG vs the flexible CG with FP32
-operators, alone, with the two-level rigid-body-mode correction, and with the
-learned correction on top, same GPU); the batched workload ("batch": 48 TET10
-systems x 16 right-hand sides); the 120^3 two-phase inclusion cube
-("learned": the same modes plus the learned correction alone, with the
-network trained on the GPU on solver-harvested residuals first).
+The same JSON line carries, on the same GPU: the plain FP64 multicolour-SSOR
+CG on the same system, the speed-up and |x - x_FP64|/|x_FP64|; the other
+flexible-CG modes; the SpMV medians of 100 calls (100^3 FP64, 160^3 FP32);
+the 100^3 cube's FP64 SSOR-CG / DIAG-CG (the plain method on GPU); the
+batched workload (48 TET10 boxes x 16 right-hand sides); the 120^3 two-phase
+inclusion cube with and without the learned aggregation; and the reference's
+own CPU solver (oracle/_ref, built from the FrontISTR sources) on the host.
 
-    python bench.py [--steps K] [--warmup W] [--n 100] [--gpus 1] [--c3-n 160] [--c4-nsys 48]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--only c2|c3|c4|c5]
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -38,276 +41,216 @@ HBM_PEAK_GBS = 8000.0
 # 1 GiB copy of the same run gives); profiles/spmv_hex8_100_traffic.md
 SPMV_TRAFFIC_100 = 2199539866
 
+# (name, Iarray 41, Iarray 43, Iarray 44): FP64 CG; flexible CG with FP32
+# operators; with the symmetric two-level step on the cells; on the learned
+# aggregation
+FCG_MODES = (("fp64_ssor_cg", 0, 0, 0), ("mixed_fcg", 1, 0, 0), ("mixed_coarse_fcg", 1, 1, 0),
+             ("learned_coarse_fcg", 2, 1, 0))
 
-def spmv_bytes(m, w):
+
+def spmv_bytes(N, NP, nnzb, w):
     """Algorithmic bytes of one y = A x: the reference's matvec bandwidth
     formula (hecmw_solver_Iterative.f90:212-217) with merged indices:
     9w per block + 4 B column index per block + row pointers + x read + y write."""
-    nnzb = m.N + m.NPL + m.NPU
-    return 9 * w * nnzb + 4 * nnzb + 4 * (m.N + 1) + 3 * w * m.NP + 3 * w * m.N
+    return 9 * w * nnzb + 4 * nnzb + 4 * (N + 1) + 3 * w * NP + 3 * w * N
+
+
+def ssor_bytes(N, nnz_lu, w):
+    """Algorithmic bytes of one multicolour SSOR application: L'/U' values and
+    column indices, the 3x3 LU factors, r read and z written, and the sweep
+    vector read and written by each of the two sweeps (gathers counted once)."""
+    return (9 * w + 4) * nnz_lu + 9 * w * N + 6 * 3 * w * N
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=100, help="elements per cube edge")
-    ap.add_argument("--precond", type=int, default=1, help="1 SSOR, 3 DIAG for the headline solve")
-    ap.add_argument("--cpu-sample-s", type=float, default=20.0, help="budget of the CPU reference sample")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--spmv-reps", type=int, default=100)
-    ap.add_argument("--c3-n", type=int, default=160, help="mixed-precision comparison cube (0: skip)")
-    ap.add_argument("--c4-nsys", type=int, default=48, help="batched TET10 systems (0: skip)")
-    ap.add_argument("--c5-n", type=int, default=120, help="learned-correction cube (0: skip)")
-    ap.add_argument("--c5-variants", type=int, default=32, help="training variants for the learned correction")
-    ap.add_argument("--only", choices=("c3", "c4", "c5"), default=None,
-                    help="run one workload section alone (for profiling); prints that section's JSON")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py: one GPU only (--gpus %d requested); sharding over GPUs (one rank per GPU, a halo "
-              "exchange per product over RCCL) is not implemented" % args.gpus, file=sys.stderr)
-        sys.exit(2)
+def median_ms(fn, reps):
+    return float(np.median([fn() for _ in range(reps)]))
 
+
+def lscpu():
+    try:
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, text=True, timeout=10).stdout
+    except Exception as e:  # pragma: no cover
+        return {"error": str(e)}
+    keep = {"Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)", "NUMA node(s)"}
+    d = {}
+    for line in out.splitlines():
+        k, _, v = line.partition(":")
+        if k.strip() in keep:
+            d[k.strip()] = v.strip()
+    d["affinity_cpus"] = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
+    return d
+
+
+def host_threads():
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 2)
+    return max(2, min(n, 16))  # the box's CPU share for one GPU is 16 threads
+
+
+# ---- the network ----------------------------------------------------------------------------
+def inclusions(fa, n, seed):
+    """the two-phase workload's generator: matrix E 1e3, spheres of E 1e7, radii 2-6 elements, 15%"""
+    m = fa.gen_hex8(n, kind="inclusions", seed=seed, E=1000.0, nu=0.3, contrast=1e4, rmin=2, rmax=6, vfrac=0.15)
+    m.set_iter(40000)
+    m.set_precond(1)
+    m.set_i(m.I_TIMELOG, 1)
+    return m
+
+
+def train_net(args, dev):
+    """The learned aggregation's network, trained on the GPU on labels built
+    from errors the solver harvests itself: per seeded variant of the two-phase
+    workload (args.train_n^3, the workload's inclusion statistics) and load case
+    (the pressure load, then random loads on the loaded face), the FP64
+    SSOR-CG solution x* and the errors x* - x_k of its iterates at 32
+    checkpoints (every 16 iterations from 32 on)."""
     import torch
     import frontistr_amd as fa
+    t0 = time.time()
+    net = fa.LearnedNet(seed=0)
+    keep = []
+    rng = np.random.default_rng(123)
+    n = args.train_n
+    checkpoints = tuple(32 + 16 * k for k in range(32))
+    for v in range(args.train_variants):
+        m = inclusions(fa, n, 1000 + v)
+        ctx = fa.DeviceContext()
+        ctx.set_coords(fa.hex8_coords(n))
+        ctx.load(m)
+        top = np.nonzero(m.B)[0]
+        c = 0
+        for lc in range(2):
+            bvec = m.B.copy()
+            if lc > 0:
+                bvec[:] = 0.0
+                bvec[top] = rng.standard_normal(top.size) * np.abs(m.B[top]).mean()
+            b = torch.from_numpy(bvec).to(dev)
+            c = max(c, fa.harvest_errors(net, ctx, m, b, iters=checkpoints))
+        keep.append((ctx, c))
+    t_harvest = time.time() - t0
+    total = sum(k[1] for k in keep)
+    t0 = time.time()
+    loss = net.train(epochs=args.train_epochs, lr=3e-3, seed=0)
+    t_train = time.time() - t0
+    _, acc = net.score()
+    log("trained: %d error samples from %d variants (%d^3) in %.1fs; training %.1fs, logistic loss %.4f -> %.4f, "
+        "accuracy %.4f" % (total, args.train_variants, n, t_harvest, t_train, loss[0], loss[-1], acc))
+    net.clear_samples()
+    for c, _ in keep:
+        c.close()
+    return net, {"error_samples": int(total), "variants": args.train_variants, "variant_n": n,
+                 "harvest_s": round(t_harvest, 2), "train_s": round(t_train, 2), "epochs": args.train_epochs,
+                 "loss_before": round(float(loss[0]), 5), "loss_after": round(float(loss[-1]), 5),
+                 "train_accuracy": round(float(acc), 5)}
 
-    if not torch.cuda.is_available():
-        print("bench.py: no GPU visible", file=sys.stderr)
-        sys.exit(1)
-    dev = torch.device("cuda:0")
 
-    if args.only:
-        net, train = train_net(args, dev) if args.only != "c4" and args.c5_variants > 0 else (None, None)
-        if args.only == "c3":
-            sec = {"mixed": bench_mixed(args, dev, net)}
-        elif args.only == "c4":
-            sec = {"batch": bench_batch(args)}
-        else:
-            sec = {"learned": bench_learned(args, dev, net, train)}
+# ---- a resident system and its modes ------------------------------------------------------
+class Resident:
+    def __init__(self, fa, m, coords, net, dev):
+        import torch
+        self.fa, self.m = fa, m
+        self.ctx = fa.DeviceContext()
+        if coords is not None:
+            self.ctx.set_coords(coords)
         if net is not None:
-            net.close()
-        print(json.dumps(dict(only=args.only, **sec)), flush=True)
-        return
+            self.ctx.set_net(net)
+        self.net = net
+        t0 = time.time()
+        self.ctx.load(m)
+        self.load_s = time.time() - t0
+        self.b = torch.from_numpy(m.B).to(dev)
+        self.x = torch.zeros(3 * m.NP, dtype=torch.float64, device=dev)
+        self.y = torch.zeros_like(self.x)
+        self.bn = float(torch.linalg.vector_norm(self.b[:3 * m.N]))
 
-    t0 = time.time()
-    m = fa.gen_hex8(args.n)
-    log("generated %d^3 HEX8: N=%d nnzb=%d in %.2fs" % (args.n, m.N, m.N + m.NPL + m.NPU, time.time() - t0))
-    m.set_iter(20000)
-    m.set_precond(args.precond)
-    m.set_i(m.I_TIMELOG, 1)
+    def iarray(self, opt, coarse, additive):
+        ia = self.m.Iarray.copy()
+        ia[96], ia[97] = 1, 0  # numeric refresh inside the step, ordering and aggregation cells kept
+        ia[40], ia[42], ia[43] = opt, coarse, additive
+        return ia
 
-    ctx = fa.DeviceContext()
-    t0 = time.time()
-    ctx.load(m)
-    t_load = time.time() - t0
-    info = ctx.info()
-    log("loaded in %.2fs (%s)" % (t_load, info))
-
-    b = torch.from_numpy(m.B).to(dev)
-    x = torch.zeros(3 * m.NP, dtype=torch.float64, device=dev)
-    torch.cuda.synchronize()
-
-    ia = m.Iarray.copy()
-    ra = m.Rarray.copy()
-
-    def step(ia_step):
-        x.zero_()
+    def solve(self, ia):
+        import torch
+        self.x.zero_()
         torch.cuda.synchronize()
-        return ctx.solve_device(b.data_ptr(), x.data_ptr(), ia_step, ra)
+        t0 = time.perf_counter()
+        r = self.ctx.solve_device(self.b.data_ptr(), self.x.data_ptr(), ia, self.m.Rarray)
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - t0
 
-    # numeric preconditioner refresh every step (Iarray(97)=1), ordering reused
-    ia_step = ia.copy()
-    ia_step[96] = 1
-    ia_step[97] = 0
-    for i in range(args.warmup):
-        r = step(ia_step)
-        log("warmup %d: iter=%d resid=%.3e err=%d Tset=%.3f Tsol=%.3f" % (i, r.iter, r.resid, r.error, r.Tset, r.Tsol))
-    torch.cuda.synchronize()
-    results = []
-    t_start = time.perf_counter()
-    for i in range(args.steps):
-        results.append(step(ia_step))
-    torch.cuda.synchronize()
-    t_total = time.perf_counter() - t_start
-    for i, r in enumerate(results):
-        log("step %d: iter=%d resid=%.3e err=%d Tset=%.3f Tsol=%.3f time_Ax=%.3f time_precond=%.3f" % (
-            i, r.iter, r.resid, r.error, r.Tset, r.Tsol, r.time_Ax, r.time_precond))
-    r0 = results[-1]
-    xs = x.cpu().numpy()
-    m.X[:] = xs
-    true_resid = fa.hecmw_rel_resid_L2(m)
-    log("true relative residual %.3e" % true_resid)
-    sec_per_solve = t_total / args.steps
-    iters = r0.iter
+    def true_resid(self):
+        """FP64 |b - A x| / |b| with the FP64 SpMV (hecmw_matvec_33's sums) on the device"""
+        import torch
+        self.ctx.matvec_device(self.x.data_ptr(), self.y.data_ptr())
+        n3 = 3 * self.m.N
+        return float(torch.linalg.vector_norm(self.b[:n3] - self.y[:n3])) / self.bn
 
-    # dominant kernel: the CG SpMV (q = A p with p.q), timed by events inside the solves
-    ax_ms = sum(r.time_Ax for r in results) / sum(r.iter for r in results) * 1e3
-    bytes64 = spmv_bytes(m, 8)
-    achieved = bytes64 / (ax_ms * 1e-3) / 1e9
-
-    # standalone SpMV bandwidth, FP64 and FP32
-    y = torch.zeros(3 * m.NP, dtype=torch.float64, device=dev)
-    xr = torch.rand(3 * m.NP, dtype=torch.float64, device=dev)
-    torch.cuda.synchronize()
-    ms64 = ctx.matvec_device(xr.data_ptr(), y.data_ptr(), args.spmv_reps)
-    x32 = xr.float()
-    y32 = torch.zeros(3 * m.NP, dtype=torch.float32, device=dev)
-    torch.cuda.synchronize()
-    ms32 = ctx.matvec_device(x32.data_ptr(), y32.data_ptr(), args.spmv_reps, fp32=True)
-    gbs64 = bytes64 / (ms64 * 1e-3) / 1e9
-    gbs32 = spmv_bytes(m, 4) / (ms32 * 1e-3) / 1e9
-    # a plain copy of known size (1 GiB read + 1 GiB written): the reference
-    # point that scales the L2 traffic counters of a profiled run
-    cpy_src = torch.empty(1 << 27, dtype=torch.float64, device=dev).uniform_()
-    cpy_dst = torch.empty_like(cpy_src)
-    for _ in range(3):
-        cpy_dst.copy_(cpy_src)
-    torch.cuda.synchronize()
-    # what a plain streaming read reaches on this GPU: the practical ceiling
-    # under the published 8 TB/s
-    part = torch.empty(65536, dtype=torch.float64, device=dev)
-    stream_gbs = fa.stream_read(cpy_src.data_ptr(), cpy_src.numel(), part.data_ptr(), 10)
-    del cpy_src, cpy_dst, part
-    log("SpMV FP64 %.3f ms %.0f GB/s; FP32 %.3f ms %.0f GB/s; in-solve SpMV %.3f ms %.0f GB/s; stream read %.0f GB/s" % (
-        ms64, gbs64, ms32, gbs32, ax_ms, achieved, stream_gbs))
-
-    # DIAG-CG on the same system
-    ia_d = ia.copy()
-    ia_d[2] = 3
-    ia_d[96] = 1
-    ia_d[97] = 1
-    x.zero_()
-    torch.cuda.synchronize()
-    rd = ctx.solve_device(b.data_ptr(), x.data_ptr(), ia_d, ra)
-    x.zero_()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    rd = ctx.solve_device(b.data_ptr(), x.data_ptr(), ia_d, ra)
-    t_diag = time.perf_counter() - t0
-    log("DIAG-CG: iter=%d resid=%.3e time %.3fs" % (rd.iter, rd.resid, t_diag))
-
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(m, iters, args.cpu_sample_s)
-    del ctx, b, x, y, xr, x32, y32
-    torch.cuda.empty_cache()
-    net, train = train_net(args, dev) if (args.c5_n > 0 or args.c3_n > 0) and args.c5_variants > 0 else (None, None)
-    torch.cuda.empty_cache()
-    mixed = bench_mixed(args, dev, net) if args.c3_n > 0 else None
-    torch.cuda.empty_cache()
-    batch = bench_batch(args) if args.c4_nsys > 0 else None
-    torch.cuda.empty_cache()
-    learned = bench_learned(args, dev, net, train) if args.c5_n > 0 else None
-    if net is not None:
-        net.close()
-
-    out = {
-        "metric": "time to 1e-8 rel. residual: mixed+learned FCG vs plain CG; BCSR SpMV GB/s",
-        "value": round(sec_per_solve, 5),
-        "unit": "s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(sec_per_solve * 1e3, 3),
-        "higher_is_better": False,
-        "scaling": "strong",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {
-            "workload": "%d^3 HEX8 elastic cube, %d DOF, %d blocks; plain FP64 multicolour-SSOR CG to 1e-8"
-                        % (args.n, 3 * m.N, m.N + m.NPL + m.NPU),
-            "method": "CG", "precond": {1: "SSOR", 3: "DIAG"}.get(args.precond, args.precond),
-            "ncolor": r0.ncolor, "iterations": iters, "rel_resid_true": true_resid,
-        },
-        "roofline": {
-            "bound": "hbm", "kernel": "k_spmv<kEpiDotX> (q = A p with p.q)",
-            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": SPMV_TRAFFIC_100 if args.n == 100 else None,
-            "bytes_per_launch": bytes64, "avg_launch_ms": round(ax_ms, 4),
-            "stream_read_GBs": round(stream_gbs, 1), "frac_of_stream_read": round(achieved / stream_gbs, 4),
-        },
-        "spmv": {"fp64_ms": round(ms64, 4), "fp64_GBs": round(gbs64, 1), "fp32_ms": round(ms32, 4),
-                 "fp32_GBs": round(gbs32, 1)},
-        "diag_cg": {"seconds": round(t_diag, 4), "iterations": rd.iter},
-        "solve_parts": {"Tset": round(r0.Tset, 4), "Tsol": round(r0.Tsol, 4),
-                        "time_Ax": round(r0.time_Ax, 4), "time_precond": round(r0.time_precond, 4),
-                        "load_s": round(t_load, 3)},
-        "cpu_baseline": cpu,
-        "mixed": mixed,
-        "batch": batch,
-        "learned": learned,
-    }
-    print(json.dumps(out), flush=True)
+    def close(self):
+        if self.net is not None:
+            self.ctx.set_net(None)
+        self.ctx.close()
 
 
-# flexible CG variants: (name, Iarray 41, Iarray 43, Iarray 44); the two-level
-# correction symmetric (coarse, SSOR, coarse; 44 = 0) or additive (44 = 1); the
-# learned solve adds it additively
-FCG_MODES = (("fp64_ssor_cg", 0, 0, 0), ("mixed_fcg", 1, 0, 0), ("mixed_coarse_fcg", 1, 1, 0),
-             ("mixed_coarse_additive_fcg", 1, 1, 1), ("learned_coarse_fcg", 2, 1, 1))
-
-
-def _run_modes(tag, fa, ctx, m, b, x, net, modes):
-    """Each mode from x0 = 0 to 1e-8 with the system resident and a numeric
-    preconditioner refresh (SSOR factors and, with Iarray 43, the coarse
-    inverse) inside the timed solve."""
-    import torch
+def run_modes(tag, R, modes):
+    """Each mode once from x0 = 0 to 1e-8, numeric set-up inside."""
     res, xs = {}, {}
     for mode, opt, coarse, additive in modes:
-        if opt == 2 and net is None:
+        if opt == 2 and R.net is None:
             continue
-        ia = m.Iarray.copy()
-        ia[96] = 1
-        ia[97] = 0
-        ia[40] = opt
-        ia[42] = coarse
-        ia[43] = additive
-        x.zero_()
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        r = ctx.solve_device(b.data_ptr(), x.data_ptr(), ia, m.Rarray)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t1
-        xs[mode] = x.cpu().numpy()
-        m.X[:] = xs[mode]
-        inf = ctx.info()
+        r, wall = R.solve(R.iarray(opt, coarse, additive))
+        inf = R.ctx.info()
+        xs[mode] = R.x.cpu().numpy()
         res[mode] = {"seconds": round(wall, 4), "iterations": r.iter, "error": r.error, "Tset": round(r.Tset, 4),
                      "Tsol": round(r.Tsol, 4), "ms_per_iter": round(1e3 * r.Tsol / max(1, r.iter), 4),
                      "time_Ax": round(r.time_Ax, 4), "time_precond": round(r.time_precond, 4),
-                     "rel_resid_true": fa.hecmw_rel_resid_L2(m)}
+                     "rel_resid_true": R.true_resid()}
         if opt > 0:
             res[mode]["fp64_residual_updates"] = inf["nreliable"]
             res[mode]["spmv_fp64_values"] = bool(inf["spmv64"])
         if coarse:
             res[mode]["coarse_aggregates"] = inf["coarse"]
-            res[mode]["coarse_form"] = "additive" if additive or opt == 2 else "symmetric"
         if opt == 2:
-            res[mode]["fallback_iterations"] = inf["nfallback"]
-            res[mode]["correction_dropped"] = bool(inf["learned_off"])
-            res[mode]["probe_ratio"] = inf["probe_ratio"]
+            res[mode]["pieces_from_network"] = inf["pieces"]
+            res[mode]["learned_step_dropped_at"] = inf["fallback_iter"]
+            res[mode]["learned_step_kept"] = not bool(inf["learned_off"])
         log("%s %s: %s" % (tag, mode, res[mode]))
-    ref_x = xs["fp64_ssor_cg"]
-    base = res["fp64_ssor_cg"]["seconds"]
-    for mode in res:
-        res[mode]["rel_err_vs_fp64"] = float(np.linalg.norm(xs[mode] - ref_x) / np.linalg.norm(ref_x))
-        if mode != "fp64_ssor_cg":
-            res[mode]["speedup_vs_fp64"] = round(base / res[mode]["seconds"], 3)
-    return res
+    if "fp64_ssor_cg" in xs:
+        ref_x = xs["fp64_ssor_cg"]
+        base = res["fp64_ssor_cg"]["seconds"]
+        for mode in res:
+            res[mode]["rel_err_vs_fp64"] = float(np.linalg.norm(xs[mode] - ref_x) / np.linalg.norm(ref_x))
+            if mode != "fp64_ssor_cg":
+                res[mode]["speedup_vs_fp64"] = round(base / res[mode]["seconds"], 3)
+    return res, xs
 
 
-def bench_mixed(args, dev, net):
-    """Heterogeneous cube (per-element E log-uniform in [1e3, 1e6], nu uniform
-    in [0.2, 0.4], seed 0): FP64 multicolour-SSOR CG against the flexible CG
-    with FP32 SpMV/SSOR/vectors and FP64 residual replacement, alone, with the
-    two-level (rigid-body-mode coarse space) correction, and with the learned
-    correction on top."""
+def spmv_median(R, fp32, reps):
+    """median kernel time (ms) of `reps` single SpMV calls (events on the context's stream)"""
+    import torch
+    xr = torch.rand(3 * R.m.NP, dtype=torch.float64, device=R.x.device)
+    if fp32:
+        xin, yout = xr.float(), torch.zeros(3 * R.m.NP, dtype=torch.float32, device=R.x.device)
+    else:
+        xin, yout = xr, torch.zeros_like(xr)
+    torch.cuda.synchronize()
+    R.ctx.matvec_device(xin.data_ptr(), yout.data_ptr(), 3, fp32=fp32)  # warm
+    return median_ms(lambda: R.ctx.matvec_device(xin.data_ptr(), yout.data_ptr(), 1, fp32=fp32), reps)
+
+
+def ssor_ms(R, fp32, reps=20):
+    import torch
+    r = torch.rand(3 * R.m.NP, dtype=torch.float32 if fp32 else torch.float64, device=R.x.device)
+    z = torch.zeros_like(r)
+    torch.cuda.synchronize()
+    R.ctx.ssor_device(r.data_ptr(), z.data_ptr(), 2, fp32=fp32)  # warm
+    return R.ctx.ssor_device(r.data_ptr(), z.data_ptr(), reps, fp32=fp32)
+
+
+# ---- C3: the headline -------------------------------------------------------------------
+def bench_c3(args, dev, net):
     import torch
     import frontistr_amd as fa
     n = args.c3_n
@@ -317,98 +260,122 @@ def bench_mixed(args, dev, net):
     m.set_iter(20000)
     m.set_precond(1)
     m.set_i(m.I_TIMELOG, 1)
-    m.set_coarse(1)  # the aggregation is symbolic set-up, built at load like the colouring
-    ctx = fa.DeviceContext()
-    ctx.set_coords(fa.hex8_coords(n))
-    ctx.set_net(net)
-    t0 = time.time()
-    ctx.load(m)
-    t_load = time.time() - t0
-    b = torch.from_numpy(m.B).to(dev)
-    x = torch.zeros(3 * m.NP, dtype=torch.float64, device=dev)
-    res = _run_modes("C3", fa, ctx, m, b, x, net, FCG_MODES)
-    ctx.set_net(None)
-    ctx.close()
-    out = {"workload": "%d^3 HEX8, lognormal E in [1e3,1e6], nu in [0.2,0.4], seed 0, %d DOF, %d blocks"
-                       % (n, 3 * m.N, m.N + m.NPL + m.NPU), "load_s": round(t_load, 2)}
-    out.update(res)
-    return out
+    m.set_coarse(1)  # the aggregation cells are symbolic set-up, built at load like the colouring
+    R = Resident(fa, m, fa.hex8_coords(n), net, dev)
+    modes, xs = run_modes("C3", R, FCG_MODES)
+    head = "learned_coarse_fcg" if net is not None else "mixed_coarse_fcg"
+    opt = 2 if net is not None else 1
+    ia = R.iarray(opt, 1, 0)
+    ia_log = ia.copy()
+    ia[21] = 0  # no TIMELOG events inside the timed steps; the warm-up carries them
+    r_log = None
+    for i in range(args.warmup):
+        r_log, wall = R.solve(ia_log)
+        log("C3 warmup %d: %s %d iterations %.4fs (time_Ax %.4f, time_precond %.4f)" % (
+            i, head, r_log.iter, wall, r_log.time_Ax, r_log.time_precond))
+    steps = []
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        steps.append(R.solve(ia))
+    t_total = time.perf_counter() - t_start
+    for i, (r, wall) in enumerate(steps):
+        log("C3 step %d: %d iterations, resid %.3e, %.4fs (Tset %.4f, Tsol %.4f)" % (i, r.iter, r.resid, wall, r.Tset,
+                                                                                      r.Tsol))
+    r_last = steps[-1][0]
+    x_last = R.x.cpu().numpy()
+    resid = R.true_resid()
+    nnzb = m.N + m.NPL + m.NPU
+    # the FP32 SpMV and SSOR application of the learned step's operators, and the FP64 ones
+    ms32 = spmv_median(R, True, args.spmv_reps)
+    ssor32 = ssor_ms(R, True)
+    ssor64 = ssor_ms(R, False)
+    sb32 = ssor_bytes(m.N, m.NPL + m.NPU, 4)
+    sb64 = ssor_bytes(m.N, m.NPL + m.NPU, 8)
+    out = {
+        "workload": "%d^3 HEX8, lognormal E in [1e3,1e6], nu in [0.2,0.4], seed 0, %d DOF, %d blocks"
+                    % (n, 3 * m.N, nnzb),
+        "load_s": round(R.load_s, 2), "modes": modes, "headline_mode": head,
+        "seconds_per_step": t_total / args.steps, "iterations": r_last.iter, "rel_resid_true": resid,
+        "rel_err_vs_fp64": float(np.linalg.norm(x_last - xs["fp64_ssor_cg"]) / np.linalg.norm(xs["fp64_ssor_cg"])),
+        "step_Tset": round(r_last.Tset, 4), "step_Tsol": round(r_last.Tsol, 4),
+        "warmup_time_Ax": round(r_log.time_Ax, 4) if r_log else None,
+        "warmup_time_precond": round(r_log.time_precond, 4) if r_log else None,
+        "spmv_fp32_ms_median100": round(ms32, 4),
+        "spmv_fp32_GBs": round(spmv_bytes(m.N, m.NP, nnzb, 4) / (ms32 * 1e-3) / 1e9, 1),
+        "spmv_fp32_bytes": spmv_bytes(m.N, m.NP, nnzb, 4),
+        "ssor_fp32_ms": round(ssor32, 4), "ssor_fp32_bytes": sb32,
+        "ssor_fp32_GBs": round(sb32 / (ssor32 * 1e-3) / 1e9, 1),
+        "ssor_fp64_ms": round(ssor64, 4), "ssor_fp64_GBs": round(sb64 / (ssor64 * 1e-3) / 1e9, 1),
+        "ncolor": R.ctx.info()["ncolor"],
+    }
+    log("C3 %s: %.4f s/step, %d iterations; SpMV FP32 %.4f ms (%.0f GB/s); SSOR FP32 %.4f ms (%.0f GB/s), "
+        "FP64 %.4f ms" % (head, out["seconds_per_step"], r_last.iter, ms32, out["spmv_fp32_GBs"], ssor32,
+                          out["ssor_fp32_GBs"], ssor64))
+    R.close()
+    return out, m, modes["fp64_ssor_cg"]["iterations"]
 
 
-def _inclusions(fa, n, seed):
-    m = fa.gen_hex8(n, kind="inclusions", seed=seed, E=1000.0, nu=0.3, contrast=1e4, rmin=2, rmax=6, vfrac=0.15)
-    m.set_iter(30000)
+# ---- C2: the plain method on GPU ------------------------------------------------------------
+def bench_c2(args, dev):
+    import torch
+    import frontistr_amd as fa
+    n = args.c2_n
+    m = fa.gen_hex8(n)
+    m.set_iter(20000)
     m.set_precond(1)
     m.set_i(m.I_TIMELOG, 1)
-    return m
+    R = Resident(fa, m, None, None, dev)
+    nnzb = m.N + m.NPL + m.NPU
+    res = {}
+    for name, pc in (("fp64_ssor_cg", 1), ("fp64_diag_cg", 3)):
+        ia = R.iarray(0, 0, 0)
+        ia[2] = pc
+        ia[97] = 1 if pc == 3 else 0
+        R.solve(ia)  # set-up and warm
+        r, wall = R.solve(ia)
+        res[name] = {"seconds": round(wall, 4), "iterations": r.iter, "rel_resid_true": R.true_resid(),
+                     "time_Ax": round(r.time_Ax, 4), "time_precond": round(r.time_precond, 4),
+                     "spmv_in_solve_ms": round(1e3 * r.time_Ax / max(1, r.iter), 4)}
+        log("C2 %s: %s" % (name, res[name]))
+    ms64 = spmv_median(R, False, args.spmv_reps)
+    ia = R.iarray(0, 0, 0)
+    R.solve(ia)  # SSOR set-up back in place
+    ssor64 = ssor_ms(R, False)
+    sb64 = ssor_bytes(m.N, m.NPL + m.NPU, 8)
+    out = {"workload": "%d^3 HEX8 elastic cube, %d DOF, %d blocks, uniform E" % (n, 3 * m.N, nnzb),
+           "solves": res, "ncolor": R.ctx.info()["ncolor"],
+           "spmv_fp64_ms_median100": round(ms64, 4), "spmv_fp64_bytes": spmv_bytes(m.N, m.NP, nnzb, 8),
+           "spmv_fp64_GBs": round(spmv_bytes(m.N, m.NP, nnzb, 8) / (ms64 * 1e-3) / 1e9, 1),
+           "ssor_fp64_ms": round(ssor64, 4), "ssor_fp64_bytes": sb64,
+           "ssor_fp64_GBs": round(sb64 / (ssor64 * 1e-3) / 1e9, 1)}
+    log("C2: SpMV FP64 %.4f ms (%.0f GB/s), SSOR FP64 %.4f ms (%.0f GB/s)" % (
+        ms64, out["spmv_fp64_GBs"], ssor64, out["ssor_fp64_GBs"]))
+    R.close()
+    return out, m
 
 
-def train_net(args, dev):
-    """The learned correction's network, trained on residuals the solver
-    harvests from FP64 SSOR-CG runs on seeded variants of the two-phase cube
-    (24^3, the inclusion statistics of the learned-correction workload, 7 load
-    cases x 10 iteration counts each), 4 epochs on the GPU."""
-    import torch
+# ---- C5: the two-phase cube, with and without the learned aggregation ---------------------
+def bench_c5(args, dev, net, train):
     import frontistr_amd as fa
-    t0 = time.time()
-    net = fa.LearnedNet(seed=0)
-    keep = []
-    rng = np.random.default_rng(123)
-    nv = args.c5_variants
-    count = 0
-    for v in range(nv):
-        m = _inclusions(fa, 24, 1000 + v)
-        ctx = fa.DeviceContext()
-        ctx.load(m)
-        top = np.nonzero(m.B)[0]
-        for lc in range(7):
-            # the pressure load, then random nodal loads on the loaded face
-            bvec = m.B.copy()
-            if lc > 0:
-                bvec[:] = 0.0
-                bvec[top] = rng.standard_normal(top.size) * np.abs(m.B[top]).mean()
-            b = torch.from_numpy(bvec).to(dev)
-            count = fa.harvest_residuals(net, ctx, m, b)
-        keep.append(ctx)
-    t_harvest = time.time() - t0
-    t0 = time.time()
-    loss = net.train(epochs=4, lr=2e-3, seed=0)
-    t_train = time.time() - t0
-    log("trained: %d residuals from %d variants in %.1fs; training %.1fs, energy %s" % (
-        count, nv, t_harvest, t_train, np.round(loss, 4)))
-    net.clear_samples()
-    for c in keep:
-        c.close()
-    return net, {"samples": count, "variants": nv, "harvest_s": round(t_harvest, 2), "train_s": round(t_train, 2),
-                 "energy_plain_ssor": loss[0], "energy_per_epoch": [float(v) for v in loss[1:]]}
-
-
-def bench_learned(args, dev, net, train):
-    """Two-phase inclusion cube (E contrast 1e4, 15% volume, radii 2-6
-    elements): FP64 SSOR-CG, the flexible CG with FP32 operators, with the
-    learned correction, and both with the two-level correction."""
-    import torch
-    import frontistr_amd as fa
-    m = _inclusions(fa, args.c5_n, 7)
+    m = inclusions(fa, args.c5_n, 7)
     m.set_coarse(1)
-    ctx = fa.DeviceContext()
-    ctx.set_coords(fa.hex8_coords(args.c5_n))
-    ctx.load(m)
-    b = torch.from_numpy(m.B).to(dev)
-    ctx.set_net(net)
-    x = torch.zeros(3 * m.NP, dtype=torch.float64, device=dev)
-    modes = FCG_MODES[:2] + (("learned_fcg", 2, 0, 0),) + FCG_MODES[2:]
-    res = _run_modes("C5", fa, ctx, m, b, x, net, modes)
-    ctx.set_net(None)
-    ctx.close()
+    R = Resident(fa, m, fa.hex8_coords(args.c5_n), net, dev)
+    modes = (FCG_MODES[0],) + FCG_MODES[2:]
+    res, _ = run_modes("C5", R, modes)
+    R.close()
     out = {"workload": "%d^3 HEX8, two-phase inclusions (E 1e3 / 1e7, 15%% volume, radii 2-6 elements, seed 7), "
-                       "%d DOF; network trained on %s residuals harvested from SSOR-CG on %s seeded 24^3 variants"
-                       % (args.c5_n, 3 * m.N, train["samples"] if train else 0, train["variants"] if train else 0),
-           "train": train}
+                       "%d DOF" % (args.c5_n, 3 * m.N), "train": train}
     out.update(res)
+    if "learned_coarse_fcg" in res:
+        out["learned_vs_cells"] = {
+            "iterations": [res["mixed_coarse_fcg"]["iterations"], res["learned_coarse_fcg"]["iterations"]],
+            "seconds": [res["mixed_coarse_fcg"]["seconds"], res["learned_coarse_fcg"]["seconds"]],
+            "speedup": round(res["mixed_coarse_fcg"]["seconds"] / res["learned_coarse_fcg"]["seconds"], 3)}
     return out
 
 
+# ---- C4: batched ----------------------------------------------------------------------------
 def bench_batch(args):
     """48 TET10 boxes of 5k-600k DOF (log-uniform, seeded) x 16 right-hand
     sides, DIAG flexible CG with FP32 SpMM and FP64 reductions, one
@@ -432,85 +399,186 @@ def bench_batch(args):
     blocks = info["blocks"]
     spmm_bytes = 9 * 4 * blocks + 4 * blocks + 4 * (info["rows"] + 1) + 16 * 3 * 4 * 2 * info["rows"]
     spmm_ms = info["variant_us"][info["variant"]] / 1e3
-    log("C4: %.3fs, iterations %d..%d, max resid %.2e, errors %d, SpMM %s %.3f ms, FP64 residual SpMM %s %.3f ms" % (
-        wall, it.min(), it.max(), rs.max(), int((er != 0).sum()), info["variant"], spmm_ms, info["variant64"],
-        info["variant64_us"][info["variant64"]] / 1e3))
+    log("C4: %.3fs, iterations %d..%d, max resid %.2e, errors %d, SpMM %s %.3f ms" % (
+        wall, it.min(), it.max(), rs.max(), int((er != 0).sum()), info["variant"], spmm_ms))
     bs.close()
-    # the same instances one at a time through the single-system path (first
-    # right-hand side of every system, x 16): FP64 DIAG-CG (the reference's
-    # method) and the single-system mixed DIAG flexible CG
-    import torch
-    seq = {}
-    for name, opt in (("fp64_diag_cg", 0), ("mixed_diag_fcg", 1)):
-        total, iters = 0.0, 0
-        for s_, mat in enumerate(mats):
-            mm = mat.copy()
-            mm.set_iter(20000)
-            mm.set_precond(3)
-            mm.set_i(41, opt)
-            ctx = fa.DeviceContext()
-            ctx.load(mm)
-            b = torch.from_numpy(np.ascontiguousarray(Bs[s_][0])).to("cuda:0")
-            xv = torch.zeros_like(b)
-            torch.cuda.synchronize()
-            t1 = time.perf_counter()
-            r = ctx.solve_device(b.data_ptr(), xv.data_ptr(), mm.Iarray, mm.Rarray)
-            torch.cuda.synchronize()
-            total += time.perf_counter() - t1
-            iters += r.iter
-            ctx.close()
-        seq[name] = {"seconds_first_rhs": round(total, 3), "iterations_first_rhs": iters,
-                     "seconds_all_rhs_est": round(16 * total, 2)}
-        log("C4 one at a time, %s: %.2fs for the 48 first right-hand sides (x16: %.1fs)" % (name, total, 16 * total))
     return {"workload": "%d TET10 boxes (5k-600k DOF log-uniform, aspect 0.5-2, seed 0) x 16 RHS, %d DOF, %d blocks"
                         % (len(mats), dof, blocks),
             "seconds": round(wall, 4), "instances": int(it.size), "converged": int((er == 0).sum()),
             "iterations_min": int(it.min()), "iterations_max": int(it.max()), "max_rel_resid": float(rs.max()),
-            "one_at_a_time": seq,
-            "speedup_vs_one_at_a_time_fp64": round(seq["fp64_diag_cg"]["seconds_all_rhs_est"] / wall, 2),
             "spmm_variant": info["variant"], "spmm_ms": spmm_ms,
             "spmm64_variant": info["variant64"], "spmm64_res_ms": info["variant64_us"][info["variant64"]] / 1e3,
             "spmm_GBs": round(spmm_bytes / (spmm_ms * 1e-3) / 1e9, 1), "load_s": round(t_load, 2)}
 
 
-def cpu_baseline(m, gpu_iters, budget_s):
-    """The reference's CPU solver (oracle/_ref) on the same system, all host
-    cores, on a bounded sample: a solve capped at a number of iterations that
-    fits the budget; time to 1e-8 = setup + per-iteration time x the number of
-    iterations the full solve needs (the GPU's count: same ordering)."""
+# ---- the reference on the host ------------------------------------------------------------
+def cpu_baseline(m3, iters3, m2, budget_s):
+    """The reference's own solver (oracle/_ref: hecmw_solve_CG with
+    hecmw_precond_SSOR_33, flang -O3 -fopenmp, HECMW_SERIAL) on the same arrays:
+    * value: the 160^3 headline system, FP64 multicolour SSOR-CG, time to 1e-8
+      estimated from a bounded sample = set-up + s/iteration x the iterations of
+      the full solve (the GPU's FP64 count: same ordering, same method);
+    * the 100^3 cube solved to 1e-8 in full;
+    * one thread (natural order, one colour, hecmw_precond_SSOR_33.F90:106-114):
+      the 20^3 cube in full and a bounded sample of the 100^3 cube."""
     try:
         from oracle import ref
+        import frontistr_amd as fa
     except Exception as e:  # pragma: no cover
         log("cpu baseline unavailable: %s" % e)
         return None
     if not ref.available():
         log("cpu baseline unavailable: oracle/_ref not built")
         return None
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    cores = max(2, min(cores, 16))  # the box's CPU share for one GPU is 16 threads
-    # probe: 5 iterations to size the sample
-    ia = m.Iarray.copy()
-    ia[0] = 5
-    ia[21] = 0
+    cores = host_threads()
+    out = {"kind": "reference", "cores": cores, "unit": "s", "lscpu": lscpu()}
+
+    def sample(m, nthreads, iters_full, budget):
+        ia = m.Iarray.copy()
+        ia[0] = 3
+        ia[21] = 0
+        _, info = ref.solve_krylov(m, nthreads=nthreads, iarray=ia)
+        per_it = max(info["Tsol"] / max(1, min(info["iter"], 3)), 1e-6)
+        n_it = int(max(5, min(iters_full, (budget - info["Tset"]) / per_it)))
+        ia[0] = n_it
+        t0 = time.time()
+        _, info = ref.solve_krylov(m, nthreads=nthreads, iarray=ia)
+        wall = time.time() - t0
+        ran = min(info["iter"], n_it)
+        per_it = info["Tsol"] / ran
+        return info["Tset"] + per_it * iters_full, ran, wall, info["Tset"], per_it
+
+    est, ran, wall, tset, per_it = sample(m3, cores, iters3, budget_s)
+    out["value"] = round(est, 2)
+    out["sample"] = ("reference FP64 multicolour SSOR-CG on the same 160^3 system, %d threads: %d of %d iterations "
+                     "run (%.1fs wall incl. set-up %.2fs); value = set-up + %.4f s/iteration x %d" % (
+                         cores, ran, iters3, wall, tset, per_it, iters3))
+    log("CPU reference C3: " + out["sample"])
+    m = m2.copy()
+    m.set_i(m.I_TIMELOG, 0)
     t0 = time.time()
-    _, info = ref.solve_krylov(m, nthreads=cores, iarray=ia)
-    probe = time.time() - t0
-    per_it = max(info["Tsol"] / max(1, min(info["iter"], 5)), 1e-6)
-    n_it = int(max(10, min(gpu_iters, (budget_s - info["Tset"]) / per_it)))
-    ia[0] = n_it
+    _, info = ref.solve_krylov(m, nthreads=cores)
+    out["c2_full"] = {"seconds": round(time.time() - t0, 3), "iterations": info["iter"], "error": info["error"],
+                      "resid": info["resid"], "Tset": round(info["Tset"], 3), "Tsol": round(info["Tsol"], 3),
+                      "threads": cores}
+    log("CPU reference C2 full: %s" % out["c2_full"])
+    m1 = fa.gen_hex8(20)
+    m1.set_iter(20000)
     t0 = time.time()
-    _, info = ref.solve_krylov(m, nthreads=cores, iarray=ia)
-    wall = time.time() - t0
-    ran = min(info["iter"], n_it)
-    per_it = info["Tsol"] / ran
-    est = info["Tset"] + per_it * gpu_iters
-    log("CPU reference: %d threads, %d iterations in %.2fs (Tset %.2fs, %.4fs/iter) -> est %.2fs to 1e-8" % (
-        cores, ran, wall, info["Tset"], per_it, est))
-    return {"value": round(est, 3), "unit": "s", "cores": cores, "kind": "reference",
-            "sample": "reference hecmw_solve_CG (flang -O2 -fopenmp, HECMW_SERIAL) on the same 100^3 system, "
-                      "%d of %d iterations run (%.1fs wall incl. setup %.2fs); value = Tset + %.4f s/iter x %d"
-                      % (ran, gpu_iters, wall, info["Tset"], per_it, gpu_iters),
-            "spmv_s_per_call": round(info["time_Ax"] / max(1, ran + 1), 5)}
+    _, info = ref.solve_krylov(m1, nthreads=1)
+    out["one_thread_c1_full"] = {"seconds": round(time.time() - t0, 3), "iterations": info["iter"],
+                                 "note": "natural order, one colour"}
+    est1, ran1, wall1, tset1, per1 = sample(m2, 1, info["iter"], min(15.0, budget_s))
+    out["one_thread_c2_sample"] = {"s_per_iteration": round(per1, 4), "Tset": round(tset1, 3),
+                                   "iterations_run": ran1, "wall": round(wall1, 2)}
+    log("CPU reference 1 thread: C1 %s, C2 %s" % (out["one_thread_c1_full"], out["one_thread_c2_sample"]))
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--c2-n", type=int, default=100, help="the plain-method cube (0: skip)")
+    ap.add_argument("--c3-n", type=int, default=160, help="the headline cube")
+    ap.add_argument("--c4-nsys", type=int, default=48, help="batched TET10 systems (0: skip)")
+    ap.add_argument("--c5-n", type=int, default=120, help="the two-phase cube (0: skip)")
+    ap.add_argument("--train-variants", type=int, default=32, help="seeded variants the network trains on (0: none)")
+    ap.add_argument("--train-n", type=int, default=24)
+    ap.add_argument("--train-epochs", type=int, default=40)
+    ap.add_argument("--cpu-sample-s", type=float, default=20.0, help="budget of the CPU reference sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--spmv-reps", type=int, default=100)
+    ap.add_argument("--only", choices=("c2", "c3", "c4", "c5"), default=None,
+                    help="run one workload section alone (for profiling); prints that section's JSON")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: one GPU only (--gpus %d requested); sharding over GPUs (one rank per GPU, a halo "
+              "exchange per product over RCCL) is not implemented" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    dev = torch.device("cuda:0")
+
+    net, train = (None, None)
+    if args.train_variants > 0 and args.only in (None, "c3", "c5"):
+        net, train = train_net(args, dev)
+    if args.only:
+        if args.only == "c3":
+            sec = {"c3": bench_c3(args, dev, net)[0]}
+        elif args.only == "c2":
+            sec = {"c2": bench_c2(args, dev)[0]}
+        elif args.only == "c4":
+            sec = {"batch": bench_batch(args)}
+        else:
+            sec = {"c5": bench_c5(args, dev, net, train)}
+        print(json.dumps(dict(only=args.only, **sec)), flush=True)
+        return
+
+    c3, m3, iters3 = bench_c3(args, dev, net)
+    torch.cuda.empty_cache()
+    c2, m2 = bench_c2(args, dev) if args.c2_n > 0 else (None, None)
+    torch.cuda.empty_cache()
+    cpu = None
+    if not args.no_cpu and m2 is not None:
+        cpu = cpu_baseline(m3, iters3, m2, args.cpu_sample_s)
+    del m3
+    batch = bench_batch(args) if args.c4_nsys > 0 else None
+    torch.cuda.empty_cache()
+    c5 = bench_c5(args, dev, net, train) if args.c5_n > 0 else None
+    if net is not None:
+        net.close()
+
+    fp64 = c3["modes"]["fp64_ssor_cg"]
+    value = c3["seconds_per_step"]
+    n = args.c3_n
+    out = {
+        "metric": "time to 1e-8 rel. residual: mixed+learned FCG vs plain CG; BCSR SpMV GB/s",
+        "value": round(value, 5),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(value * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32 operators, f64 reductions/x",
+        "data": "synthetic",
+        "config": {
+            "workload": "%d^3 HEX8 heterogeneous cube (lognormal E in [1e3,1e6], nu in [0.2,0.4], seed 0), "
+                        "12,519,843 DOF: mixed-precision flexible CG with the learned two-level step to an FP64 "
+                        "true residual of 1e-8" % n,
+            "method": c3["headline_mode"], "iterations": c3["iterations"], "rel_resid_true": c3["rel_resid_true"],
+            "rel_err_vs_fp64_ssor_cg": c3["rel_err_vs_fp64"],
+        },
+        "plain_fp64_ssor_cg": {"seconds": fp64["seconds"], "iterations": fp64["iterations"],
+                               "rel_resid_true": fp64["rel_resid_true"]},
+        "speedup_vs_plain_fp64": round(fp64["seconds"] / value, 3),
+        "roofline": {
+            "bound": "hbm", "kernel": "multicolour SSOR application, FP32 (k_ssor_stage, %d x k_ssor64_fwd/bwd "
+                                      "colour launches, k_ssor_unstage)" % c3["ncolor"],
+            "achieved": c3["ssor_fp32_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(c3["ssor_fp32_GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_application": c3["ssor_fp32_bytes"], "avg_application_ms": c3["ssor_fp32_ms"],
+            "spmv_fp32_GBs": c3["spmv_fp32_GBs"], "spmv_fp32_frac": round(c3["spmv_fp32_GBs"] / HBM_PEAK_GBS, 4),
+        },
+        "spmv": {"c2_fp64_ms_median100": c2["spmv_fp64_ms_median100"] if c2 else None,
+                 "c2_fp64_GBs": c2["spmv_fp64_GBs"] if c2 else None,
+                 "c3_fp32_ms_median100": c3["spmv_fp32_ms_median100"], "c3_fp32_GBs": c3["spmv_fp32_GBs"]},
+        "cpu_baseline": cpu,
+        "c3": c3,
+        "c2": c2,
+        "batch": batch,
+        "c5": c5,
+        "train": train,
+    }
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -45,7 +45,8 @@ from .hecmw import (  # noqa: F401
     gen_tet10_batch,
     LearnedNet,
     set_net,
-    harvest_residuals,
+    harvest_errors,
+    rigid_labels,
     DeviceContext,
 )
 from .dump import read_dump, write_dump  # noqa: F401

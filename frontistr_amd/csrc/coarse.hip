@@ -20,6 +20,9 @@
 //
 // Z_i (3x6) = [ I | R(d_i) ] with R's columns the rotations about x, y, z:
 // (0, -dz, dy), (dz, 0, -dx), (-dy, dx, 0).
+//
+// When a learned solve drops its two-level step (KrylovState::coarse_off)
+// the per-iteration kernels leave z to the SSOR sweeps: no coarse term, A z = w.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -151,7 +154,7 @@ __global__ __launch_bounds__(kCoarseBlock) void k_coarse_restrict(const int32_t*
                                                                    const float* __restrict__ dv,
                                                                    const float* __restrict__ r, float* __restrict__ c,
                                                                    const KrylovState* st) {
-  if (st->done) return;
+  if (st->done || st->coarse_off) return;
   const int a = blockIdx.x;
   double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   const int t1 = aptr[a + 1];
@@ -175,7 +178,7 @@ __global__ __launch_bounds__(kCoarseBlock) void k_coarse_restrict(const int32_t*
 __global__ __launch_bounds__(kCoarseBlock) void k_coarse_gemv(int32_t nc, int32_t ldw, const float* __restrict__ W,
                                                                const float* __restrict__ c, float* __restrict__ y,
                                                                const KrylovState* st) {
-  if (st->done) return;
+  if (st->done || st->coarse_off) return;
   const int row = blockIdx.x * kCoarseWaves + threadIdx.x / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   if (row >= nc) return;
@@ -201,15 +204,19 @@ __global__ __launch_bounds__(kCoarseBlock) void k_coarse_prolong(int32_t n, cons
                                                                   const float* __restrict__ q, const KrylovState* st,
                                                                   double* part) {
   if (st->done) return;
+  const bool off = st->coarse_off;
   double d[2] = {0.0, 0.0};
   for (int64_t i = (int64_t)blockIdx.x * kCoarseBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kCoarseBlock) {
-    const int a = agg[i];
-    const float dx = dv[3 * i], dy = dv[3 * i + 1], dz = dv[3 * i + 2];
-    const float* ya = y + 6 * a;
-    const float t0 = ya[0], t1 = ya[1], t2 = ya[2], w0 = ya[3], w1 = ya[4], w2 = ya[5];
-    const float z0 = z[3 * i] + (t0 + (w1 * dz - w2 * dy));
-    const float z1 = z[3 * i + 1] + (t1 + (w2 * dx - w0 * dz));
-    const float z2 = z[3 * i + 2] + (t2 + (w0 * dy - w1 * dx));
+    float z0 = z[3 * i], z1 = z[3 * i + 1], z2 = z[3 * i + 2];
+    if (!off) {
+      const int a = agg[i];
+      const float dx = dv[3 * i], dy = dv[3 * i + 1], dz = dv[3 * i + 2];
+      const float* ya = y + 6 * a;
+      const float t0 = ya[0], t1 = ya[1], t2 = ya[2], w0 = ya[3], w1 = ya[4], w2 = ya[5];
+      z0 = z0 + (t0 + (w1 * dz - w2 * dy));
+      z1 = z1 + (t1 + (w2 * dx - w0 * dz));
+      z2 = z2 + (t2 + (w0 * dy - w1 * dx));
+    }
     z[3 * i] = z0;
     z[3 * i + 1] = z1;
     z[3 * i + 2] = z2;
@@ -368,10 +375,13 @@ __global__ __launch_bounds__(kCoarseBlock) void k_coarse_pre(int32_t n, const in
                                                               const float* __restrict__ r, float* __restrict__ z,
                                                               float* __restrict__ r1, const KrylovState* st) {
   if (st->done) return;
+  const bool off = st->coarse_off;
   for (int64_t i = (int64_t)blockIdx.x * kCoarseBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kCoarseBlock) {
-    float zi[3], ai[3];
-    zy(dv, y + 6 * agg[i], i, zi);
-    wy(sc, aggW, W, y, i, ai);
+    float zi[3] = {0.f, 0.f, 0.f}, ai[3] = {0.f, 0.f, 0.f};
+    if (!off) {
+      zy(dv, y + 6 * agg[i], i, zi);
+      wy(sc, aggW, W, y, i, ai);
+    }
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       z[3 * i + d] = zi[d];
@@ -395,7 +405,7 @@ __global__ __launch_bounds__(kCoarseBlock) void k_coarse_restrict2(const int32_t
                                                                     const float* __restrict__ r,
                                                                     const float* __restrict__ w,
                                                                     float* __restrict__ c, const KrylovState* st) {
-  if (st->done) return;
+  if (st->done || st->coarse_off) return;
   const int a = blockIdx.x;
   double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   const int t1 = aptr[a + 1];
@@ -428,11 +438,14 @@ __global__ __launch_bounds__(kCoarseBlock) void k_coarse_post(int32_t n, const i
                                                                const float* __restrict__ q, const KrylovState* st,
                                                                double* part) {
   if (st->done) return;
+  const bool off = st->coarse_off;
   double d[2] = {0.0, 0.0};
   for (int64_t i = (int64_t)blockIdx.x * kCoarseBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kCoarseBlock) {
-    float zi[3], ai[3];
-    zy(dv, y + 6 * agg[i], i, zi);
-    wy(sc, aggW, W, y, i, ai);
+    float zi[3] = {0.f, 0.f, 0.f}, ai[3] = {0.f, 0.f, 0.f};
+    if (!off) {
+      zy(dv, y + 6 * agg[i], i, zi);
+      wy(sc, aggW, W, y, i, ai);
+    }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const float zn = z[3 * i + k] + zi[k];

@@ -87,6 +87,9 @@ struct SsorOp {
   const int32_t* color_slice;  // host: first slice of every colour, ncolor+1 entries
   int32_t ncolor;
   T* zs;                       // sweep vector in slot order, 3 entries per slot
+  const int32_t* node_slot;    // slot of every internal node (the inverse of row_old)
+  int32_t n;                   // internal nodes
+  int32_t last_bwd_empty;      // 1: the last colour's rows have no U' entries (their backward step is z -= 0)
 };
 void ssor_apply(const SsorOp<double>& op, const double* r, double* z, const KrylovState* st, hipStream_t s);
 void ssor_apply_f32(const SsorOp<float>& op, const float* r, float* z, const KrylovState* st, hipStream_t s);
@@ -109,8 +112,10 @@ struct KrylovState {
   double delta2;       // update when |r|^2 <= delta2 * rr_ref
   int32_t reliable;    // recompute r = b - A x in FP64 in this iteration
   int32_t nreliable;   // FP64 true-residual updates so far
-  // learned correction: off for good / used in this iteration / fallbacks to plain SSOR so far
-  int32_t learned_off, learned_now, nfallback, pad2;
+  // learned two-level step: learned_off = 1 when the solve is not a learned one or has dropped
+  // the step; coarse_off = 1: the coarse kernels leave z to the SSOR sweeps (plain SSOR FCG);
+  // fallback_iter: the iteration at which the learned step was dropped (0: kept)
+  int32_t learned_off, coarse_off, fallback_iter, pad2;
   double edrop;  // sum of alpha (r.p): the decrease of |e|_A^2 over the iterations so far
   // true / recurrence residual seen at the last replacement: near convergence
   // the FP32 recurrence runs ahead, so it claims convergence at resid * gap <= tol
@@ -247,7 +252,6 @@ void launch_scale_vec(bool mul, int64_t n, double* v, const double* sc, hipStrea
 
 // learned flexible CG: beta from the four dots of net_apply (r.z, z.q, r.z0, z0.q), falling
 // back to the plain SSOR output z0 when the learned z is not a descent direction
-void fcg_beta_learned(KrylovState* st, const double* part, int nparts, hipStream_t s);
 // the flexible CG's beta tail over nparts partials of (r.z, z.q_prev), and its
 // alpha tail over (p.q, r.p)
 void fcg_beta(KrylovState* st, const double* part, int nparts, hipStream_t s);
@@ -294,16 +298,10 @@ int coarse_fcg_pq(int64_t n3, const float* z, const float* az, float* p, float* 
 // Returns the number of partials.
 int coarse_apply(const CoarseOp& op, int32_t n, const float* r, float* z, const float* q, const KrylovState* st,
                  double* part, bool dots, hipStream_t s);
-// p = (learned z or z0) + beta p
-void fcg_p_sel(int64_t n3, const float* z, const float* z0, float* p, const KrylovState* st, hipStream_t s);
 
-// ---- learned correction (learned.hip) -------------------------------------------------
+// ---- learned coarse space (learned.hip) -------------------------------------------------
 int net_weight_count();
 void net_static_features(const Sell<double>& A, int64_t n, const float* mat, __bf16* X, hipStream_t s);
-void learned_combine(int64_t n, const float* coef, const float* r, const float* z0, const float* ajac, const float* q,
-                     float* z, const KrylovState* st, double* part, int* nparts, hipStream_t s);
-void learned_combine_train(int64_t n, const float* Y, const float* z0, const float* jr, float* z, hipStream_t s);
-void learned_coef_grad(int64_t n, const float* Y, const float* gz, const float* jr, float* dY, hipStream_t s);
 void net_forward(int64_t n, const __bf16* Wb, const __bf16* X, __bf16* H1, __bf16* H2, const float* z0, float* z,
                  hipStream_t s);
 int net_bwd_parts(int64_t n);
@@ -312,9 +310,8 @@ void net_backward(int64_t n, const __bf16* Wb, const __bf16* X, const __bf16* H1
 void net_adam(float* Wm, float* M, float* V, const float* dW, __bf16* Wb, float lr, float b1, float b2, float eps,
               int step, float scale, hipStream_t s);
 void net_to_bf16(const float* Wm, __bf16* Wb, hipStream_t s);
-void net_energy_dots(int64_t n3, const float* z, const float* q, const float* r, const float* z0, double* part,
-                     double* dots, hipStream_t s);
-void net_energy_grad(int64_t n3, const float* q, const float* r, const double* dots, float* dY, hipStream_t s);
+void net_bce(int64_t n, const float* Y, const float* t, float* dY, double* part, double* out, hipStream_t s);
+void net_mask(int64_t n, const float* Y, uint8_t* mask, hipStream_t s);
 void net_material(int64_t n, const double* D, float* m, double* part, double* sum, hipStream_t s);
 
 // r32 = (float) r64 over n entries

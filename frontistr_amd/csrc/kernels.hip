@@ -117,7 +117,6 @@ enum Tail : int {
   kTailFcgBeta = 8,    // flexible CG: rho = r.z, beta = -(z.q_prev)/(p.q)_prev
   kTailFcgAlpha = 9,   // alpha = (r.p)/(p.q)
   kTailFcgRes = 10,    // recurrence |r|^2, reliable-update decision
-  kTailFcgBetaL = 11,  // learned: pick z or z0 from r.z, z.q, r.z0, z0.q, then as kTailFcgBeta
 };
 
 template <int TAIL, int K>
@@ -170,24 +169,12 @@ __device__ void run_tail(KrylovState* st, const double (&tot)[K], double* out, d
     st->ts = tot[0];
     st->tt = tot[K > 1 ? 1 : 0];
     st->omega = st->ts / st->tt;
-  } else if (TAIL == kTailFcgBeta || TAIL == kTailFcgBetaL) {
+  } else if (TAIL == kTailFcgBeta) {
     // Notay's FCG(1): beta keeps p A-orthogonal to the previous direction
     // whatever the (FP32) preconditioner did; r.z only checks positivity
     // with CG's rules.
-    double rho = tot[0];
+    const double rho = tot[0];
     st->zq = tot[K > 1 ? 1 : 0];
-    if (TAIL == kTailFcgBetaL) {
-      // the learned z must be a descent direction (r.z > 0, finite), else the
-      // plain SSOR output z0 takes its place in this iteration
-      const bool use = !st->learned_off && tot[0] > 0.0 && tot[0] == tot[0] && st->zq == st->zq &&
-                       tot[0] < 1e300 && fabs(st->zq) < 1e300;
-      if (!use && !st->learned_off) st->nfallback += 1;
-      st->learned_now = use;
-      if (!use) {
-        rho = tot[K > 2 ? 2 : 0];
-        st->zq = tot[K > 3 ? 3 : 0];
-      }
-    }
     st->rho = rho;
     if (rho == 0.0) {
       st->done = 1;
@@ -281,9 +268,14 @@ __global__ __launch_bounds__(kTailThreads) void k_tail_fcg_rel(KrylovState* st, 
     double tot[1];
     sum_partials<1>(part, nparts, tot);
     if (threadIdx.x == 0) {
-      // the learned correction is dropped for the rest of the solve once the
-      // FP64 residual grows (4x in |r|^2) between two replacements
-      if (!st->learned_off && tot[0] > 4.0 * st->rr_ref) st->learned_off = 1;
+      // the learned two-level step is dropped for the rest of the solve once
+      // the FP64 residual grows (4x in |r|^2) between two replacements: the
+      // iterations go on as the plain SSOR flexible CG
+      if (!st->learned_off && tot[0] > 4.0 * st->rr_ref) {
+        st->learned_off = 1;
+        st->coarse_off = 1;
+        st->fallback_iter = st->iter;
+      }
       const double rec = st->resid;
       st->dnrm2 = tot[0];
       st->rr_ref = tot[0];
@@ -734,15 +726,19 @@ __device__ __forceinline__ void ssor_chunk(const int32_t* __restrict__ colp, con
 // The sweeps work on zs, the preconditioned vector in slot order (3 entries
 // per slot; L'/U' columns hold slot ids), so the rows of a slice, which are
 // consecutive in colour order, gather neighbours that are consecutive too.
-// r comes in and z goes out in the caller's (natural) node order.
+// r is staged into zs once before the forward sweep (ZP = R,
+// hecmw_precond.f90:114-117) and z leaves in the caller's (natural) node
+// order once after the backward sweep, so the colour launches read and write
+// slot-order memory only: a colour's rows are spread over the whole mesh, and
+// per-colour accesses in node order would touch a separate cache line for
+// every row.
 //
 // Forward, rows of one colour independent:
-//   zs[i] = LU^-1 (r[old(i)] - sum_{j in L'(i), ascending} L'_ij zs[j])
+//   zs[i] = LU^-1 (zs[i] - sum_{j in L'(i), ascending} L'_ij zs[j])
 template <typename T, int CH>
 __global__ __launch_bounds__(kBlock) void k_ssor_fwd(Sell<T> L, const int32_t* __restrict__ row_old,
                                                        const T* __restrict__ alu, int s0, int ns,
-                                                       const T* __restrict__ rin, T* __restrict__ zs,
-                                                       const KrylovState* st) {
+                                                       T* __restrict__ zs, const KrylovState* st) {
 #pragma clang fp contract(off)
   if (st && st->done) return;
   const int lane = lane_id(), r = lane >> 2, d = lane & 3, dd = d < 3 ? d : 2;
@@ -758,7 +754,7 @@ __global__ __launch_bounds__(kBlock) void k_ssor_fwd(Sell<T> L, const int32_t* _
     if (iold < 0) continue;
     const int32_t* __restrict__ colp = L.col + (int64_t)base * kRowsS + r;
     const T* __restrict__ vp = L.val + ((int64_t)base * 9 + 3 * dd) * kRowsS + r;
-    T sw = rin[3 * (int64_t)iold + dd];
+    T sw = zs[3 * slot + dd];
     for (int k0 = 0; k0 < len; k0 += CH) ssor_chunk<T, CH, false, -1>(colp, vp, zs, k0, len, dd, sw);
     T x0 = quad_bcast<0>(sw), x1 = quad_bcast<1>(sw), x2 = quad_bcast<2>(sw);
     lu3_solve<T>(a, x0, x1, x2);
@@ -767,21 +763,15 @@ __global__ __launch_bounds__(kBlock) void k_ssor_fwd(Sell<T> L, const int32_t* _
 }
 
 // Backward, colours in reverse:
-//   zs[i] -= LU^-1 sum_{j in U'(i), descending} U'_ij zs[j];  z[old(i)] = zs[i]
-// ND fused dots over this colour's rows (final now): w0.z (and w1.z) go to
-// part[k*kMaxPartials + pbase + block], accumulated in FP64.
-template <typename T, int CH, int ND>
+//   zs[i] -= LU^-1 sum_{j in U'(i), descending} U'_ij zs[j]
+template <typename T, int CH>
 __global__ __launch_bounds__(kBlock) void k_ssor_bwd(Sell<T> U, const int32_t* __restrict__ row_old,
                                                        const T* __restrict__ alu, int s0, int ns,
-                                                       T* __restrict__ zs, T* __restrict__ zout,
-                                                       const T* __restrict__ w0, const T* __restrict__ w1,
-                                                       const KrylovState* st, double* part, int pbase) {
+                                                       T* __restrict__ zs, const KrylovState* st) {
 #pragma clang fp contract(off)
   if (st && st->done) return;
   const int lane = lane_id(), r = lane >> 2, d = lane & 3, dd = d < 3 ? d : 2;
   const SliceRange sr = wave_slices(ns);
-  constexpr int KD = ND > 0 ? ND : 1;
-  double acc[KD] = {};
   for (int si = sr.begin; si < sr.end; ++si) {
     const int s = s0 + si;
     const int64_t slot = (int64_t)s * kRowsS + r;
@@ -799,15 +789,8 @@ __global__ __launch_bounds__(kBlock) void k_ssor_bwd(Sell<T> U, const int32_t* _
     T x0 = quad_bcast<0>(sw), x1 = quad_bcast<1>(sw), x2 = quad_bcast<2>(sw);
     lu3_solve<T>(a, x0, x1, x2);
     const T zn = zo - pick3(dd, x0, x1, x2);
-    if (d < 3) {
-      zs[3 * slot + d] = zn;
-      const int64_t o = 3 * (int64_t)iold + d;
-      zout[o] = zn;
-      if constexpr (ND > 0) acc[0] += (double)w0[o] * (double)zn;
-      if constexpr (ND > 1) acc[KD - 1] += (double)w1[o] * (double)zn;
-    }
+    if (d < 3) zs[3 * slot + d] = zn;
   }
-  if constexpr (ND > 0) block_partials<KD>(acc, part, pbase + blockIdx.x);
 }
 
 // ---- SSOR sweeps, SELL-64 layout: one lane per row ------------------------------
@@ -858,8 +841,7 @@ __device__ __forceinline__ void load_alu64(const T* __restrict__ alu, int s, int
 template <typename T, int CH>
 __global__ __launch_bounds__(kBlock) void k_ssor64_fwd(Sell<T> L, const int32_t* __restrict__ row_old,
                                                          const T* __restrict__ alu, int s0, int ns,
-                                                         const T* __restrict__ rin, T* __restrict__ zs,
-                                                         const KrylovState* st) {
+                                                         T* __restrict__ zs, const KrylovState* st) {
 #pragma clang fp contract(off)
   if (st && st->done) return;
   const int lane = lane_id();
@@ -875,7 +857,7 @@ __global__ __launch_bounds__(kBlock) void k_ssor64_fwd(Sell<T> L, const int32_t*
     if (iold < 0) continue;
     const int32_t* __restrict__ colp = L.col + (int64_t)base * kRowsA + lane;
     const T* __restrict__ vp = L.val + (int64_t)base * 9 * kRowsA + lane;
-    T sw[3] = {rin[3 * (int64_t)iold], rin[3 * (int64_t)iold + 1], rin[3 * (int64_t)iold + 2]};
+    T sw[3] = {zs[3 * slot], zs[3 * slot + 1], zs[3 * slot + 2]};
     for (int k0 = 0; k0 < len; k0 += CH) ssor64_chunk<T, CH, false, -1>(colp, vp, zs, k0, len, sw);
     lu3_solve<T>(a, sw[0], sw[1], sw[2]);
 #pragma unroll
@@ -883,18 +865,14 @@ __global__ __launch_bounds__(kBlock) void k_ssor64_fwd(Sell<T> L, const int32_t*
   }
 }
 
-template <typename T, int CH, int ND>
+template <typename T, int CH>
 __global__ __launch_bounds__(kBlock) void k_ssor64_bwd(Sell<T> U, const int32_t* __restrict__ row_old,
                                                          const T* __restrict__ alu, int s0, int ns,
-                                                         T* __restrict__ zs, T* __restrict__ zout,
-                                                         const T* __restrict__ w0, const T* __restrict__ w1,
-                                                         const KrylovState* st, double* part, int pbase) {
+                                                         T* __restrict__ zs, const KrylovState* st) {
 #pragma clang fp contract(off)
   if (st && st->done) return;
   const int lane = lane_id();
   const SliceRange sr = wave_slices(ns);
-  constexpr int KD = ND > 0 ? ND : 1;
-  double acc[KD] = {};
   for (int si = sr.begin; si < sr.end; ++si) {
     const int s = s0 + si;
     const int64_t slot = (int64_t)s * kRowsA + lane;
@@ -910,16 +888,46 @@ __global__ __launch_bounds__(kBlock) void k_ssor64_bwd(Sell<T> U, const int32_t*
     for (int top = len - 1; top >= 0; top -= CH) ssor64_chunk<T, CH, true, 1>(colp, vp, zs, top, len, sw);
     lu3_solve<T>(a, sw[0], sw[1], sw[2]);
 #pragma unroll
+    for (int d = 0; d < 3; ++d) zs[3 * slot + d] = zs[3 * slot + d] - sw[d];
+  }
+}
+
+// r into slot order: zs[slot(i)] = r[i] (node order in, so the reads are
+// contiguous; a line of zs is completed by waves of nearby nodes)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_ssor_stage(int32_t n, const int32_t* __restrict__ node_slot,
+                                                         const T* __restrict__ r, T* __restrict__ zs,
+                                                         const KrylovState* st) {
+  if (st && st->done) return;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const int64_t t = node_slot[i];
+    const T a = r[3 * i], b = r[3 * i + 1], c = r[3 * i + 2];
+    zs[3 * t] = a;
+    zs[3 * t + 1] = b;
+    zs[3 * t + 2] = c;
+  }
+}
+
+// z = zs back in node order; ND fused dots w0.z (and w1.z), FP64 accumulation
+template <typename T, int ND>
+__global__ __launch_bounds__(kBlock) void k_ssor_unstage(int32_t n, const int32_t* __restrict__ node_slot,
+                                                           const T* __restrict__ zs, T* __restrict__ z,
+                                                           const T* __restrict__ w0, const T* __restrict__ w1,
+                                                           const KrylovState* st, double* part) {
+  if (st && st->done) return;
+  constexpr int KD = ND > 0 ? ND : 1;
+  double acc[KD] = {};
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const int64_t t = node_slot[i];
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const T zn = zs[3 * slot + d] - sw[d];
-      zs[3 * slot + d] = zn;
-      const int64_t o = 3 * (int64_t)iold + d;
-      zout[o] = zn;
-      if constexpr (ND > 0) acc[0] += (double)w0[o] * (double)zn;
-      if constexpr (ND > 1) acc[KD - 1] += (double)w1[o] * (double)zn;
+      const T v = zs[3 * t + d];
+      z[3 * i + d] = v;
+      if constexpr (ND > 0) acc[0] += (double)w0[3 * i + d] * (double)v;
+      if constexpr (ND > 1) acc[KD - 1] += (double)w1[3 * i + d] * (double)v;
     }
   }
-  if constexpr (ND > 0) block_partials<KD>(acc, part, pbase + blockIdx.x);
+  if constexpr (ND > 0) block_partials<KD>(acc, part, blockIdx.x);
 }
 
 // ---- DIAG (block Jacobi), hecmw_precond_DIAG_33.F90:106-142 --------------------
@@ -1092,19 +1100,6 @@ __global__ __launch_bounds__(kBlock) void k_copy_dots_f32(int64_t n, const float
   if (ndots > 0) block_partials<2>(d, part, blockIdx.x);
 }
 
-// p = (z or z0, as the learned tail chose) + beta p
-__global__ __launch_bounds__(kBlock) void k_fcg_p_sel(int64_t n, const float* __restrict__ z,
-                                                        const float* __restrict__ z0, float* __restrict__ p,
-                                                        const KrylovState* st) {
-  if (st->done) return;
-  const bool first = st->iter == 1;
-  const double beta = st->beta;
-  const float* __restrict__ zz = st->learned_now ? z : z0;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
-    p[i] = first ? zz[i] : (float)((double)zz[i] + beta * (double)p[i]);
-}
-
-// p = z (first iteration) or p = z + beta p
 __global__ __launch_bounds__(kBlock) void k_fcg_p(int64_t n, const float* __restrict__ z, float* __restrict__ p,
                                                     const KrylovState* st) {
   if (st->done) return;
@@ -1339,68 +1334,63 @@ static int ssor_chunk_sel() {
 }
 
 template <typename T>
-static void ssor_fwd_any(const SsorOp<T>& op, int c, const T* rin, const KrylovState* st, hipStream_t s) {
+static void ssor_fwd_any(const SsorOp<T>& op, int c, const KrylovState* st, hipStream_t s) {
   const int32_t s0 = op.color_slice[c], ns = op.color_slice[c + 1] - s0;
   if (ns <= 0) return;
   const int g = spmv_grid(ns);
   if (op.L.rows == kRowsA) {
-    k_ssor64_fwd<T, 4><<<g, kBlock, 0, s>>>(op.L, op.row_old, op.alu, s0, ns, rin, op.zs, st);
+    k_ssor64_fwd<T, 4><<<g, kBlock, 0, s>>>(op.L, op.row_old, op.alu, s0, ns, op.zs, st);
     return;
   }
   switch (ssor_chunk_sel()) {
-    case 4: k_ssor_fwd<T, 4><<<g, kBlock, 0, s>>>(op.L, op.row_old, op.alu, s0, ns, rin, op.zs, st); break;
-    case 16: k_ssor_fwd<T, 16><<<g, kBlock, 0, s>>>(op.L, op.row_old, op.alu, s0, ns, rin, op.zs, st); break;
-    default: k_ssor_fwd<T, 8><<<g, kBlock, 0, s>>>(op.L, op.row_old, op.alu, s0, ns, rin, op.zs, st); break;
+    case 4: k_ssor_fwd<T, 4><<<g, kBlock, 0, s>>>(op.L, op.row_old, op.alu, s0, ns, op.zs, st); break;
+    case 16: k_ssor_fwd<T, 16><<<g, kBlock, 0, s>>>(op.L, op.row_old, op.alu, s0, ns, op.zs, st); break;
+    default: k_ssor_fwd<T, 8><<<g, kBlock, 0, s>>>(op.L, op.row_old, op.alu, s0, ns, op.zs, st); break;
   }
 }
 
-template <typename T, int ND>
-static void ssor_bwd_any(const SsorOp<T>& op, int c, T* z, const T* w0, const T* w1, const KrylovState* st,
-                         double* part, int pbase, int g, hipStream_t s) {
+template <typename T>
+static void ssor_bwd_any(const SsorOp<T>& op, int c, const KrylovState* st, hipStream_t s) {
   const int32_t s0 = op.color_slice[c], ns = op.color_slice[c + 1] - s0;
   if (ns <= 0) return;
+  const int g = spmv_grid(ns);
   if (op.U.rows == kRowsA) {
-    k_ssor64_bwd<T, 4, ND><<<g, kBlock, 0, s>>>(op.U, op.row_old, op.alu, s0, ns, op.zs, z, w0, w1, st, part, pbase);
+    k_ssor64_bwd<T, 4><<<g, kBlock, 0, s>>>(op.U, op.row_old, op.alu, s0, ns, op.zs, st);
     return;
   }
   switch (ssor_chunk_sel()) {
-    case 4:
-      k_ssor_bwd<T, 4, ND><<<g, kBlock, 0, s>>>(op.U, op.row_old, op.alu, s0, ns, op.zs, z, w0, w1, st, part, pbase);
-      break;
-    case 16:
-      k_ssor_bwd<T, 16, ND><<<g, kBlock, 0, s>>>(op.U, op.row_old, op.alu, s0, ns, op.zs, z, w0, w1, st, part, pbase);
-      break;
-    default:
-      k_ssor_bwd<T, 8, ND><<<g, kBlock, 0, s>>>(op.U, op.row_old, op.alu, s0, ns, op.zs, z, w0, w1, st, part, pbase);
-      break;
+    case 4: k_ssor_bwd<T, 4><<<g, kBlock, 0, s>>>(op.U, op.row_old, op.alu, s0, ns, op.zs, st); break;
+    case 16: k_ssor_bwd<T, 16><<<g, kBlock, 0, s>>>(op.U, op.row_old, op.alu, s0, ns, op.zs, st); break;
+    default: k_ssor_bwd<T, 8><<<g, kBlock, 0, s>>>(op.U, op.row_old, op.alu, s0, ns, op.zs, st); break;
   }
 }
 
-// z = M^-1 r, every colour forward then backward (z in node order)
+// stage r, every colour forward, then backward in reverse (the last colour's
+// backward subtracts zero when it has no U' entries and is left out), unstage
+// z with ND fused dots w0.z (and w1.z); returns the partials written
+template <typename T, int ND>
+static int ssor_apply_run(const SsorOp<T>& op, const T* r, T* z, const T* w1, const KrylovState* st, double* part,
+                          hipStream_t s) {
+  const int gv = vec_grid(op.n);
+  k_ssor_stage<T><<<gv, kBlock, 0, s>>>(op.n, op.node_slot, r, op.zs, st);
+  for (int c = 0; c < op.ncolor; ++c) ssor_fwd_any<T>(op, c, st, s);
+  for (int c = op.ncolor - 1; c >= 0; --c) {
+    if (c == op.ncolor - 1 && op.last_bwd_empty) continue;
+    ssor_bwd_any<T>(op, c, st, s);
+  }
+  k_ssor_unstage<T, ND><<<gv, kBlock, 0, s>>>(op.n, op.node_slot, op.zs, z, r, w1, st, part);
+  return ND > 0 ? gv : 0;
+}
+
 template <typename T>
 static void ssor_apply_any(const SsorOp<T>& op, const T* r, T* z, const KrylovState* st, hipStream_t s) {
-  for (int c = 0; c < op.ncolor; ++c) ssor_fwd_any<T>(op, c, r, st, s);
-  for (int c = op.ncolor - 1; c >= 0; --c) {
-    const int ns = op.color_slice[c + 1] - op.color_slice[c];
-    ssor_bwd_any<T, 0>(op, c, z, nullptr, nullptr, st, nullptr, 0, spmv_grid(ns), s);
-  }
+  ssor_apply_run<T, 0>(op, r, z, nullptr, st, nullptr, s);
 }
 
-// fused dots: colours' partials in launch order, grids capped to fit the buffer
 template <typename T, int ND>
 static int ssor_apply_dots(const SsorOp<T>& op, const T* r, T* z, const T* w1, const KrylovState* st, double* part,
                            hipStream_t s) {
-  for (int c = 0; c < op.ncolor; ++c) ssor_fwd_any<T>(op, c, r, st, s);
-  const int gcap = std::max(1, kMaxPartials / std::max(1, op.ncolor));
-  int pbase = 0;
-  for (int c = op.ncolor - 1; c >= 0; --c) {
-    const int ns = op.color_slice[c + 1] - op.color_slice[c];
-    if (ns <= 0) continue;
-    const int g = std::min(gcap, spmv_grid(ns));
-    ssor_bwd_any<T, ND>(op, c, z, r, w1, st, part, pbase, g, s);
-    pbase += g;
-  }
-  return pbase;
+  return ssor_apply_run<T, ND>(op, r, z, w1, st, part, s);
 }
 
 void ssor_apply(const SsorOp<double>& op, const double* r, double* z, const KrylovState* st, hipStream_t s) {
@@ -1575,12 +1565,6 @@ void fcg_spmv_store(const Sell<float>& a32, const Sell<double>& a64, bool fp64_v
 }
 void fcg_beta(KrylovState* st, const double* part, int nparts, hipStream_t s) {
   k_tail<kTailFcgBeta, 2><<<1, kTailThreads, 0, s>>>(st, part, nparts, nullptr, nullptr);
-}
-void fcg_beta_learned(KrylovState* st, const double* part, int nparts, hipStream_t s) {
-  k_tail<kTailFcgBetaL, 4><<<1, kTailThreads, 0, s>>>(st, part, nparts, nullptr, nullptr);
-}
-void fcg_p_sel(int64_t n3, const float* z, const float* z0, float* p, const KrylovState* st, hipStream_t s) {
-  k_fcg_p_sel<<<vec_grid(n3), kBlock, 0, s>>>(n3, z, z0, p, st);
 }
 void fcg_p(int64_t n3, const float* z, float* p, const KrylovState* st, hipStream_t s) {
   k_fcg_p<<<vec_grid(n3), kBlock, 0, s>>>(n3, z, p, st);

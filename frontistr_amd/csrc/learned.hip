@@ -1,27 +1,35 @@
-// Learned correction for the flexible CG: a per-node MLP evaluated and
+// Learned coarse space for the flexible CG: a per-node MLP evaluated and
 // trained with BF16 matrix-core (MFMA) kernels accumulating in FP32.
 //
 // No counterpart in the reference (its preconditioners are SSOR, DIAG and
 // the ones hecmw_precond_setup dispatches to, hecmw_precond.f90:30-58).
 //
-// The preconditioned residual is z = M r + B J r, i.e. z_i = z0_i + b_i (J r)_i:
-//   z0 = M r, the FP32 multicolour-SSOR output (M symmetric positive definite),
-//   J r = the block-Jacobi output (LU of D_ii, as DIAG),
-//   b_i = softplus(y_i - 4) > 0 from the network's output y_i = N(g_i)
-// for static node features g_i.  M + B J is symmetric positive definite for
-// every b >= 0 (J is block diagonal and B a scalar per block), and the
-// features do not depend on r, so the flexible CG runs as a preconditioned CG
-// on a fixed SPD operator.  (A correction that depends on r, or that scales
-// z0 node by node, is not symmetric: the flexible CG then lost its short
-// recurrence and needed several times the iterations.)  Features g_i (16):
-// m_i = tanh((ln tr D_ii - mean)/2), the mean, maximum and minimum of m over
-// the node's stencil neighbours, m_i^2, the local contrast max - min, 1 (a
-// bias), the stencil size / 27, zeros.
+// The two-level step of the flexible CG takes its coarse space from
+// aggregates of nodes (six rigid-body modes each).  Geometric cells alone
+// miss what dominates the slow error of high-contrast bodies: a stiff
+// inclusion moves as one rigid piece inside softer material that deforms
+// around it.  The network predicts, per node, the logit y_i of "the slow
+// error moves this node rigidly with a stiff piece"; the nodes with y_i > 0
+// of every cell are split off into one aggregate per connected piece, whose
+// rigid-body modes then enter the coarse space.  The preconditioner stays the
+// symmetric positive definite two-level step; only its aggregation is
+// learned.
 //
-// Network: 16 -> 64 -> ReLU -> 64 -> ReLU -> 3 (output 1 gives b; 0 and 2 are
-// unused), no biases beyond the constant feature.  Layouts (row-major [k][n],
-// BF16): W1 [16][64], W2 [64][64], W3 [64][16] (columns 3..15 zero).  A
-// v_mfma_f32_16x16x32_bf16 tile takes, in lane l, A[i = l%16][k = 8(l/16)+e]
+// Training targets come from the solver itself: errors e = x* - x_k = A^-1 r_k
+// of multicolour-SSOR CG iterates on sample systems.  An edge (i, j) of the
+// matrix graph whose axial strain |(e_j - e_i).(x_j - x_i)| / |x_j - x_i|^2
+// stays below tau times the sample's median in every sample lies inside a
+// rigidly moving region, and its nodes are labelled 1 (solver.hip,
+// rigid_labels).  The loss is the logistic loss of y against the labels.
+//
+// Features g_i (16): m_i = tanh((ln tr D_ii - mean)/2), the mean, maximum and
+// minimum of m over the node's stencil neighbours, m_i^2, the local contrast
+// max - min, 1 (a bias), the stencil size / 27, zeros.
+//
+// Network: 16 -> 64 -> ReLU -> 64 -> ReLU -> 3 (output 0 is the logit; 1 and 2
+// are unused), no biases beyond the constant feature.  Layouts (row-major
+// [k][n], BF16): W1 [16][64], W2 [64][64], W3 [64][16] (columns 3..15 zero).
+// A v_mfma_f32_16x16x32_bf16 tile takes, in lane l, A[i = l%16][k = 8(l/16)+e]
 // and B[k = 8(l/16)+e][j = l%16], e = 0..7, and returns D[i = 4(l/16)+v][j =
 // l%16], v = 0..3.
 #include <hip/hip_runtime.h>
@@ -171,83 +179,30 @@ __global__ __launch_bounds__(kNT) void k_static_features(Sell<double> A, int64_t
   }
 }
 
-// J r for node i with the DIAG factors (64-row planes)
-__device__ __forceinline__ void jacobi3(int64_t i, const float* __restrict__ ajac, const float* __restrict__ r,
-                                        float (&x)[3]) {
-#pragma clang fp contract(off)
-  const float* fa = ajac + (i / 64) * 9 * 64 + (i % 64);
-  float a[9];
-#pragma unroll
-  for (int e = 0; e < 9; ++e) a[e] = fa[e * 64];
-  float x0 = r[3 * i], x1 = r[3 * i + 1], x2 = r[3 * i + 2];
-  x1 = x1 - a[3] * x0;
-  x2 = x2 - a[6] * x0 - a[7] * x1;
-  x2 = a[8] * x2;
-  x1 = a[4] * (x1 - a[5] * x2);
-  x0 = a[0] * (x0 - a[2] * x2 - a[1] * x1);
-  x[0] = x0;
-  x[1] = x1;
-  x[2] = x2;
-}
-
-__device__ __forceinline__ float coef_b(float y) {
-  const float t = y - 4.0f;
-  return t > 20.0f ? t : log1pf(expf(t));
-}
-__device__ __forceinline__ float coef_db(float y) { return 1.0f / (1.0f + expf(4.0f - y)); }
-
-// Inside the flexible CG: z = z0 + b J r (z = z0 once the learned step is
-// switched off), with the FP64 partials of r.z, z.q_prev, r.z0, z0.q_prev.
-__global__ __launch_bounds__(kNT) void k_learned_combine(int64_t n, const float* __restrict__ coef,
-                                                         const float* __restrict__ r, const float* __restrict__ z0,
-                                                         const float* __restrict__ ajac,
-                                                         const float* __restrict__ q, float* __restrict__ z,
-                                                         const KrylovState* st, double* part) {
-  if (st && st->done) return;
-  const bool use_net = !(st && st->learned_off);
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+// logistic loss of output 0 against labels t: l_i = softplus(y_i) - t_i y_i;
+// dY_i = ((sigmoid(y_i) - t_i) / n, 0, 0); FP64 partials of the loss and of
+// the nodes classified as their label (y_i > 0 exactly when t_i >= 1/2)
+__global__ __launch_bounds__(kNT) void k_net_bce(int64_t n, const float* __restrict__ Y,
+                                                 const float* __restrict__ t, float* __restrict__ dY, double* part) {
+  double acc[2] = {0.0, 0.0};
+  const float inv = 1.0f / (float)n;
   for (int64_t i = (int64_t)blockIdx.x * kNT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kNT) {
-    float jr[3] = {0.f, 0.f, 0.f};
-    float cb = 0.f;
-    if (use_net) {
-      jacobi3(i, ajac, r, jr);
-      cb = coef_b(coef[3 * i + 1]);
-    }
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const int64_t o = 3 * i + d;
-      const float zz0 = z0[o];
-      const float zz = use_net ? zz0 + cb * jr[d] : zz0;
-      z[o] = zz;
-      const double rv = r[o], qv = q[o];
-      acc[0] += rv * zz;
-      acc[1] += (double)zz * qv;
-      acc[2] += rv * zz0;
-      acc[3] += (double)zz0 * qv;
-    }
-  }
-  block_sum<4>(acc, part, blockIdx.x);
-}
-
-// training: z = z0 + b J r from the network output Y, and dL/dY from dL/dz
-__global__ void k_combine_train(int64_t n, const float* __restrict__ Y, const float* __restrict__ z0,
-                                const float* __restrict__ jr, float* __restrict__ z) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const float b = coef_b(Y[3 * i + 1]);
-#pragma unroll
-    for (int d = 0; d < 3; ++d) z[3 * i + d] = z0[3 * i + d] + b * jr[3 * i + d];
-  }
-}
-__global__ void k_coef_grad(int64_t n, const float* __restrict__ Y, const float* __restrict__ gz,
-                            const float* __restrict__ jr, float* __restrict__ dY) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    float gb = 0.f;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) gb += gz[3 * i + d] * jr[3 * i + d];
-    dY[3 * i] = 0.0f;
-    dY[3 * i + 1] = gb * coef_db(Y[3 * i + 1]);
+    const float y = Y[3 * i], ti = t[i];
+    const float sp = y > 20.0f ? y : log1pf(expf(y));
+    const float sg = 1.0f / (1.0f + expf(-y));
+    acc[0] += (double)(sp - ti * y);
+    acc[1] += ((y > 0.0f) == (ti >= 0.5f)) ? 1.0 : 0.0;
+    dY[3 * i] = (sg - ti) * inv;
+    dY[3 * i + 1] = 0.0f;
     dY[3 * i + 2] = 0.0f;
   }
+  block_sum<2>(acc, part, blockIdx.x);
+}
+
+// the nodes the network splits off: mask_i = (y_i > 0)
+__global__ void k_net_mask(int64_t n, const float* __restrict__ Y, uint8_t* __restrict__ mask) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    mask[i] = Y[3 * i] > 0.0f ? 1 : 0;
 }
 
 // ---- training -------------------------------------------------------------------
@@ -456,18 +411,6 @@ __global__ void k_net_to_bf16(int n, const float* __restrict__ a, __bf16* __rest
   if (w < n) b[w] = (__bf16)a[w];
 }
 
-// energy functional parts: z.Az, z.r, r.z0 (FP64 partials)
-__global__ __launch_bounds__(kNT) void k_energy_dots(int64_t n3, const float* __restrict__ z,
-                                                     const float* __restrict__ q, const float* __restrict__ r,
-                                                     const float* __restrict__ z0, double* part) {
-  double acc[3] = {0.0, 0.0, 0.0};
-  for (int64_t i = (int64_t)blockIdx.x * kNT + threadIdx.x; i < n3; i += (int64_t)gridDim.x * kNT) {
-    acc[0] += (double)z[i] * (double)q[i];
-    acc[1] += (double)z[i] * (double)r[i];
-    acc[2] += (double)r[i] * (double)z0[i];
-  }
-  block_sum<3>(acc, part, blockIdx.x);
-}
 __global__ void k_sum_parts(int nparts, int K, const double* __restrict__ part, double* __restrict__ out) {
   if ((int)threadIdx.x < K) {
     double s = 0.0;
@@ -475,14 +418,6 @@ __global__ void k_sum_parts(int nparts, int K, const double* __restrict__ part, 
     out[threadIdx.x] = s;
   }
 }
-// dL/dz for L = (z.Az - 2 z.r) / (r.z0): 2 (Az - r) / (r.z0)
-__global__ void k_energy_grad(int64_t n3, const float* __restrict__ q, const float* __restrict__ r,
-                              const double* __restrict__ dots, float* __restrict__ dY) {
-  const double s = 2.0 / dots[2];
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (int64_t)gridDim.x * blockDim.x)
-    dY[i] = (float)(s * ((double)q[i] - (double)r[i]));
-}
-
 // m_i = tanh((ln tr D_i - mean)/2): pass 1 ln tr D and partial sums, pass 2 normalise
 __global__ __launch_bounds__(kNT) void k_mat_log(int64_t n, const double* __restrict__ D, float* __restrict__ m,
                                                  double* part) {
@@ -515,21 +450,6 @@ void net_static_features(const Sell<double>& A, int64_t n, const float* mat, __b
   k_static_features<<<grid_for(n, kNT), kNT, 0, s>>>(A, n, mat, X);
 }
 
-void learned_combine(int64_t n, const float* coef, const float* r, const float* z0, const float* ajac, const float* q,
-                     float* z, const KrylovState* st, double* part, int* nparts, hipStream_t s) {
-  const int g = grid_for(n, kNT);
-  k_learned_combine<<<g, kNT, 0, s>>>(n, coef, r, z0, ajac, q, z, st, part);
-  *nparts = g;
-}
-
-void learned_combine_train(int64_t n, const float* Y, const float* z0, const float* jr, float* z, hipStream_t s) {
-  k_combine_train<<<grid_for(n, 256), 256, 0, s>>>(n, Y, z0, jr, z);
-}
-
-void learned_coef_grad(int64_t n, const float* Y, const float* gz, const float* jr, float* dY, hipStream_t s) {
-  k_coef_grad<<<grid_for(n, 256), 256, 0, s>>>(n, Y, gz, jr, dY);
-}
-
 void net_forward(int64_t n, const __bf16* Wb, const __bf16* X, __bf16* H1, __bf16* H2, const float* z0, float* z,
                  hipStream_t s) {
   const NetW W{Wb, Wb + kFt * kHid, Wb + kFt * kHid + kHid * kHid};
@@ -557,15 +477,15 @@ void net_to_bf16(const float* Wm, __bf16* Wb, hipStream_t s) {
   k_net_to_bf16<<<(kWsz + 255) / 256, 256, 0, s>>>(kWsz, Wm, Wb);
 }
 
-// dots[0..2] = z.Az, z.r, r.z0 (q = A z given)
-void net_energy_dots(int64_t n3, const float* z, const float* q, const float* r, const float* z0, double* part,
-                     double* dots, hipStream_t s) {
-  const int g = grid_for(n3, kNT);
-  k_energy_dots<<<g, kNT, 0, s>>>(n3, z, q, r, z0, part);
-  k_sum_parts<<<1, 64, 0, s>>>(g, 3, part, dots);
+// logistic loss against labels t (n), gradient into dY (3n); out[0] = summed
+// loss, out[1] = correctly classified nodes
+void net_bce(int64_t n, const float* Y, const float* t, float* dY, double* part, double* out, hipStream_t s) {
+  const int g = grid_for(n, kNT);
+  k_net_bce<<<g, kNT, 0, s>>>(n, Y, t, dY, part);
+  k_sum_parts<<<1, 64, 0, s>>>(g, 2, part, out);
 }
-void net_energy_grad(int64_t n3, const float* q, const float* r, const double* dots, float* dY, hipStream_t s) {
-  k_energy_grad<<<grid_for(n3, 256), 256, 0, s>>>(n3, q, r, dots, dY);
+void net_mask(int64_t n, const float* Y, uint8_t* mask, hipStream_t s) {
+  k_net_mask<<<grid_for(n, 256), 256, 0, s>>>(n, Y, mask);
 }
 
 void net_material(int64_t n, const double* D, float* m, double* part, double* sum, hipStream_t s) {

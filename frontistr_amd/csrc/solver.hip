@@ -26,6 +26,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <limits>
 #include <mutex>
 #include <string>
 #include <type_traits>
@@ -68,13 +69,16 @@ enum {
 // PRECOND 1-3) = 1 selects the flexible CG with FP32 operators;
 // Rarray(41) in (0,1) overrides its reliable-update factor.
 constexpr int32_t kOptMixedFcg = 1;
-// = 2: the same flexible CG with the learned correction of an attached
-// network added to the FP32 SSOR output (hecmw33_ctx_set_net / hecmw33_set_net)
+// = 2: the same flexible CG with the two-level step (Iarray 43, 1 when not
+// set) on the aggregation an attached network refines: the nodes it marks as
+// moving rigidly with a stiff piece leave their cell, one aggregate per
+// connected piece (hecmw33_ctx_set_net / hecmw33_set_net)
 constexpr int32_t kOptLearnedFcg = 2;
 constexpr double kFcgDelta = 1.0e-2;
-// iterations of each probe run that decides whether a learned solve keeps
-// its correction
-constexpr int kLearnProbe = 25;
+// a piece the network marks becomes an aggregate from this many nodes on
+// (smaller ones stay with their cell); the largest pieces are kept first
+// within kCoarseAggMax
+constexpr int32_t kPieceMinNodes = 32;
 // SOLVER_OPT(2) (Iarray 42) for the flexible CG's q = A p: 1 = FP32-rounded
 // values, 2 = FP64 values with FP32 vectors (FP64 accumulation), 0 = FP32
 // values with the two-level correction, else chosen from the stiffness
@@ -91,8 +95,7 @@ constexpr int32_t kCoarseAggDefault = 1000;
 // (default) the symmetric step z1 = Q r, z2 = z1 + M^-1 (r - A z1),
 // z = z2 + Q (r - A z2) with Q = Z E^-1 Z^T (A z1 and the coarse part of A z
 // from A Z kept per node; q = A p carried as A z + beta q, so an iteration
-// still has one SpMV); 1 the additive z = M^-1 r + Q r.  The learned solve
-// always adds the correction additively.
+// still has one SpMV); 1 the additive z = M^-1 r + Q r.
 constexpr int I_SOLVER_OPT4 = 44;
 // the dense coarse inverse is n_c^2 entries (n_c = 6 per aggregate): at most
 // 2048 aggregates (n_c = 12288, 0.6 GB in FP32)
@@ -226,8 +229,9 @@ struct Context {
   int32_t ssor_R = kRowsS;      // rows per SSOR slice (16 or 64)
   int32_t ncolor = 0;
   std::vector<int32_t> perm, iperm, colorindex, color_slice;
+  int32_t ssor_last_bwd_empty = 0;  // the last colour's U' rows are all empty
   SellDev L, U;
-  DMem row_old, alu_ssor, vzs, vzs32;
+  DMem row_old, node_slot, alu_ssor, vzs, vzs32;
   // DIAG
   bool diag_init = false;
   DMem row_diag, alu_diag;
@@ -237,21 +241,23 @@ struct Context {
   // FP32 copies for the flexible CG (rounded from the FP64 storage)
   bool a32_valid = false, ssor32_valid = false, diag32_valid = false;
   DMem alu_ssor32, alu_diag32, vr32, vz32, vp32, vq32;
-  // learned correction: attached network, plain SSOR output z0, node stiffness feature
+  // learned aggregation: attached network, node stiffness feature, the network's outputs and
+  // the nodes it splits off (mask, with a hash of the mask the aggregation was built with)
   hecmw33_net* net = nullptr;
   bool mat_valid = false;
-  DMem vz0_32, vmat, scal, vfeat, vcoef;  // vfeat: BF16 static node features, vcoef: network outputs per node
+  DMem vmat, scal, vfeat, vnety, vmask;  // vfeat: BF16 static node features, vnety: network outputs (3 per node)
+  std::vector<uint8_t> coarse_mask;      // empty: cells only
+  uint64_t coarse_mask_hash = 0, coarse_sym_mask_hash = 0;
   // Krylov vectors and state
   DMem vb, vx, vr, vz, vp, vq, vrt, vs, vt, vv, vpt, vst, vzp, vzt;
   DMem st, part, hist;
   KrylovState* h_st = nullptr;  // pinned, 2 slots for polling
   int32_t last_iter = 0;
   int32_t last_nreliable = 0;  // FP64 residual replacements of the last flexible CG solve
-  int32_t last_nfallback = 0;  // iterations of the last learned solve that used the plain SSOR output
-  int32_t last_learned_off = 1;  // the last learned solve switched the correction off
+  int32_t last_fallback_iter = 0;  // iteration at which the last learned solve dropped its step (0: kept)
+  int32_t last_learned_off = 1;    // the last solve ran without the learned step (not learned, or dropped)
+  int32_t coarse_pieces = 0;       // aggregates split off by the network in the current aggregation
   bool last_spmv64 = false;      // the last flexible CG used the FP64 matrix values in q = A p
-  double last_probe_ratio = 0.0; // error reduction with / without the learned correction in the last probe
-  DMem vx0;                      // x0 kept across the learned probe runs
   // two-level correction: node coordinates (hecMESH%node), aggregation, coarse inverse
   std::vector<double> h_coord;   // 3*NP
   int64_t coord_gen = 0;         // bumped whenever coordinates are set
@@ -296,18 +302,18 @@ struct Context {
   }
 
   SsorOp<double> ssor_op() {
-    return SsorOp<double>{L.view, U.view, row_old.as<int32_t>(), alu_ssor.as<double>(), color_slice.data(), ncolor,
-                          vzs.as<double>()};
+    return SsorOp<double>{L.view,  U.view, row_old.as<int32_t>(), alu_ssor.as<double>(), color_slice.data(), ncolor,
+                          vzs.as<double>(), node_slot.as<int32_t>(), N, ssor_last_bwd_empty};
   }
   SsorOp<float> ssor_op32() {
     return SsorOp<float>{L.view32, U.view32, row_old.as<int32_t>(), alu_ssor32.as<float>(), color_slice.data(),
-                         ncolor, vzs32.as<float>()};
+                         ncolor, vzs32.as<float>(), node_slot.as<int32_t>(), N, ssor_last_bwd_empty};
   }
 
   ~Context() {
     if (stream) (void)hipStreamSynchronize(stream);
-    for (DMem* m : {&dIndexL, &dIndexU, &dItemL, &dItemU, &vscale, &vzs, &vzs32, &vz0_32, &vmat, &scal, &vfeat, &vcoef,
-                    &vx0})
+    for (DMem* m : {&dIndexL, &dIndexU, &dItemL, &dItemU, &vscale, &vzs, &vzs32, &vmat, &scal, &vfeat, &vnety,
+                    &vmask})
       m->release();
     for (DMem* m : {&cg_agg, &cg_aptr, &cg_anodes, &cg_dv, &cg_pa, &cg_pb, &cg_pnptr, &cg_pn, &cg_Eblk, &cg_E, &cg_W,
                     &cg_cv, &cg_yv, &cg_info, &cg_ncnt, &cg_nagg, &cg_AZ, &cg_sc, &cg_aggW, &cg_kept, &vr1_32, &vt32, &vw32, &vaz32})
@@ -315,7 +321,7 @@ struct Context {
     if (blas) rocblas_destroy_handle(blas);
     for (DMem* m : {&mpc_slave, &mpc_mptr, &mpc_mdof, &mpc_mval, &mpc_tptr, &mpc_tslave, &mpc_tval, &mpc_w1, &mpc_w2})
       m->release();
-    for (DMem* m : {&dD, &dAL, &dAU, &row_old, &alu_ssor, &row_diag, &alu_diag, &alu_ssor32, &alu_diag32, &vr32,
+    for (DMem* m : {&dD, &dAL, &dAU, &row_old, &node_slot, &alu_ssor, &row_diag, &alu_diag, &alu_ssor32, &alu_diag32, &vr32,
                     &vz32, &vp32, &vq32, &vb, &vx, &vr, &vz, &vp, &vq, &vrt, &vs, &vt, &vv, &vpt, &vst, &vzp, &vzt, &st,
                     &part, &hist})
       m->release();
@@ -329,25 +335,26 @@ struct Context {
 
 }  // namespace hecmw33
 
-// A learned correction: FP32 master weights, BF16 copies for the MFMA kernels,
-// Adam moments, and the training samples (residual, SSOR output, block-Jacobi
-// output) harvested on device contexts, which must outlive the samples.
+// A learned aggregation network: FP32 master weights, BF16 copies for the
+// MFMA kernels, Adam moments, and per context the training labels built from
+// the error samples added on it (contexts must outlive their samples).
 struct hecmw33_net {
   struct Sample {
     hecmw33::Context* c = nullptr;
-    hecmw33::DMem r32, z0, jr;
-    int64_t n = 0;
-    double base = 0.0;  // energy functional of the plain SSOR output
+    std::vector<float> worst;  // per stored edge: max over the samples of its axial strain / the sample median
+    int32_t nsamp = 0;
+    double tau = 0.1;
+    hecmw33::DMem t;           // labels on the device (built at training)
+    bool t_valid = false;
   };
-  hecmw33::DMem Wm, Wb, M, V, dW, wpart, H1, H2, y, z, q, dY, dYc, dots, dpart, xb;
+  hecmw33::DMem Wm, Wb, M, V, dW, wpart, H1, H2, y, z, dY, dots, dpart, xb;
   hipStream_t stream = nullptr;
   int32_t step = 0;
+  uint64_t gen = 1;  // bumped whenever the weights change
   std::vector<Sample*> samples;
   void clear_samples() {
     for (Sample* p : samples) {
-      p->r32.release();
-      p->z0.release();
-      p->jr.release();
+      p->t.release();
       delete p;
     }
     samples.clear();
@@ -355,8 +362,7 @@ struct hecmw33_net {
   ~hecmw33_net() {
     if (stream) (void)hipStreamSynchronize(stream);
     clear_samples();
-    for (hecmw33::DMem* m : {&Wm, &Wb, &M, &V, &dW, &wpart, &H1, &H2, &y, &z, &q, &dY, &dYc, &dots, &dpart, &xb})
-      m->release();
+    for (hecmw33::DMem* m : {&Wm, &Wb, &M, &V, &dW, &wpart, &H1, &H2, &y, &z, &dY, &dots, &dpart, &xb}) m->release();
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -612,10 +618,21 @@ static int32_t setup_ssor_symbolic(Context& c, int32_t ncolor_in) {
     build_lu(std::integral_constant<int, kRowsS>{});
   if (int32_t e = upload_sell(hl, c.L, c.stream)) return e;
   if (int32_t e = upload_sell(hu, c.U, c.stream)) return e;
-  if (c.row_old.ensure(sizeof(int32_t) * row_old.size()) || c.alu_ssor.ensure(sizeof(double) * 9 * row_old.size()))
+  // the slot of every node, for staging r into slot order and z back out
+  std::vector<int32_t> node_slot(N, -1);
+  for (int64_t t = 0; t < (int64_t)row_old.size(); ++t)
+    if (row_old[t] >= 0) node_slot[row_old[t]] = (int32_t)t;
+  // the last colour is an independent set with the largest new indices, so
+  // its rows have no U' entries and its backward step subtracts LU^-1 0 = 0
+  c.ssor_last_bwd_empty = 1;
+  for (int32_t inew = c.colorindex[c.ncolor - 1]; inew < N; ++inew)
+    if (iUp[inew + 1] != iUp[inew]) c.ssor_last_bwd_empty = 0;
+  if (c.row_old.ensure(sizeof(int32_t) * row_old.size()) || c.alu_ssor.ensure(sizeof(double) * 9 * row_old.size()) ||
+      c.node_slot.ensure(sizeof(int32_t) * (size_t)N))
     return HECMW33_ERROR_DEVICE;
   HIPC(hipMemcpyAsync(c.row_old.p, row_old.data(), sizeof(int32_t) * row_old.size(), hipMemcpyHostToDevice,
                       c.stream));
+  HIPC(hipMemcpyAsync(c.node_slot.p, node_slot.data(), sizeof(int32_t) * (size_t)N, hipMemcpyHostToDevice, c.stream));
   if (c.vzs.ensure(sizeof(double) * 3 * row_old.size())) return HECMW33_ERROR_DEVICE;
   HIPC(hipMemsetAsync(c.vzs.p, 0, sizeof(double) * 3 * row_old.size(), c.stream));
   HIPC(hipStreamSynchronize(c.stream));
@@ -702,29 +719,37 @@ static int32_t setup_f32(Context& c, int32_t kind) {
 
 // hecmw_precond_setup (hecmw_precond.f90:30-58) with the SSOR/DIAG
 // reuse rule (hecmw_precond_SSOR_33.F90:74-82, hecmw_precond_DIAG_33.F90:38-41).
-// Learned flexible CG on top of the SSOR set-up: FP32 copies of A, of the
-// SSOR operator and of the block-Jacobi factors (a feature), the z0 buffer and
-// the node stiffness feature.
-static int32_t learned_setup(Context& c, double sigma) {
-  if (!c.ssor_init) {
-    set_error("the learned correction needs the SSOR preconditioner (PRECOND 1 or 2) set up");
-    return HECMW33_ERROR_UNSUPPORTED;
-  }
-  if (!c.diag_init)
-    if (int32_t e = setup_diag(c, sigma > 0.0 ? sigma : 1.0)) return e;
-  if (int32_t e = setup_f32(c, 1)) return e;
-  if (int32_t e = setup_f32(c, 3)) return e;
-  if (c.vz0_32.ensure(sizeof(float) * 3 * (size_t)c.NP) || c.vmat.ensure(sizeof(float) * (size_t)c.N) ||
-      c.scal.ensure(sizeof(double) * 8) || c.vfeat.ensure(2 * 16 * (size_t)c.N) ||
-      c.vcoef.ensure(sizeof(float) * 3 * (size_t)c.N))
+// Static node features of the learned aggregation (node stiffness and its
+// stencil statistics), refreshed with the values.
+static int32_t features_setup(Context& c) {
+  if (c.vmat.ensure(sizeof(float) * (size_t)c.N) || c.scal.ensure(sizeof(double) * 8) ||
+      c.vfeat.ensure(2 * 16 * (size_t)c.N) || c.vnety.ensure(sizeof(float) * 3 * (size_t)c.N) ||
+      c.vmask.ensure((size_t)c.N))
     return HECMW33_ERROR_DEVICE;
   if (!c.mat_valid) {
-    // node stiffness and the static features built on it (refreshed with the values)
     net_material(c.N, c.dD.as<double>(), c.vmat.as<float>(), c.part.as<double>(), c.scal.as<double>(), c.stream);
     net_static_features(c.A.view, c.N, c.vmat.as<float>(), c.vfeat.as<__bf16>(), c.stream);
     c.mat_valid = true;
   }
   HIPC(hipGetLastError());
+  return 0;
+}
+
+// The network's mask for this system: which nodes leave their cell.  The
+// aggregation is rebuilt only when the mask differs from the one it was built
+// with (a hash over the mask bytes).
+static int32_t learned_mask(Context& c) {
+  if (int32_t e = features_setup(c)) return e;
+  net_forward(c.N, c.net->Wb.as<__bf16>(), c.vfeat.as<__bf16>(), nullptr, nullptr, nullptr, c.vnety.as<float>(),
+              c.stream);
+  net_mask(c.N, c.vnety.as<float>(), c.vmask.as<uint8_t>(), c.stream);
+  c.coarse_mask.resize(c.N);
+  HIPC(hipMemcpyAsync(c.coarse_mask.data(), c.vmask.p, (size_t)c.N, hipMemcpyDeviceToHost, c.stream));
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(c.stream));
+  uint64_t h = 1469598103934665603ull;  // FNV-1a
+  for (uint8_t v : c.coarse_mask) h = (h ^ v) * 1099511628211ull;
+  c.coarse_mask_hash = h ? h : 1;
   return 0;
 }
 
@@ -904,6 +929,55 @@ static int32_t coarse_symbolic(Context& c, int32_t target) {
   if (na == 0) {
     set_error("coarse correction: no aggregation within the size limit");
     return HECMW33_ERROR_ARGS;
+  }
+  // learned refinement: the nodes the network marks leave their cell, one
+  // aggregate per connected piece (matrix-graph neighbours that are marked and
+  // in the same cell); pieces of at least kPieceMinNodes nodes, largest first
+  // within kCoarseAggMax; cells left empty are dropped
+  c.coarse_pieces = 0;
+  if ((int32_t)c.coarse_mask.size() == N) {
+    const uint8_t* mk = c.coarse_mask.data();
+    std::vector<int32_t> up(N);
+    for (int32_t i = 0; i < N; ++i) up[i] = i;
+    auto find = [&](int32_t x) {
+      while (up[x] != x) x = up[x] = up[up[x]];
+      return x;
+    };
+    for (int32_t i = 0; i < N; ++i) {
+      if (!mk[i]) continue;
+      for (int part = 0; part < 2; ++part) {
+        const int32_t* ix = part ? iU : iL;
+        const int32_t* it = part ? tU : tL;
+        for (int32_t q = ix[i]; q < ix[i + 1]; ++q) {
+          const int32_t j = it[q] - 1;
+          if (j >= N || !mk[j] || agg[j] != agg[i]) continue;
+          const int32_t a = find(i), b = find(j);
+          if (a != b) up[std::max(a, b)] = std::min(a, b);
+        }
+      }
+    }
+    std::vector<int32_t> size(N, 0);
+    for (int32_t i = 0; i < N; ++i)
+      if (mk[i]) ++size[find(i)];
+    std::vector<int32_t> roots;
+    for (int32_t i = 0; i < N; ++i)
+      if (mk[i] && up[i] == i && size[i] >= kPieceMinNodes) roots.push_back(i);
+    std::stable_sort(roots.begin(), roots.end(), [&](int32_t a, int32_t b) { return size[a] > size[b]; });
+    if ((int64_t)roots.size() > kCoarseAggMax - na) roots.resize(std::max(0, kCoarseAggMax - na));
+    std::vector<int32_t> piece(N, -1);
+    for (size_t k = 0; k < roots.size(); ++k) piece[roots[k]] = na + (int32_t)k;
+    for (int32_t i = 0; i < N; ++i)
+      if (mk[i] && piece[find(i)] >= 0) agg[i] = piece[find(i)];
+    const int32_t na_all = na + (int32_t)roots.size();
+    std::vector<int32_t> used(na_all, -1);
+    int32_t nn = 0;
+    for (int32_t i = 0; i < N; ++i)
+      if (used[agg[i]] < 0) used[agg[i]] = 0;
+    for (int32_t a = 0; a < na_all; ++a)
+      if (used[a] == 0) used[a] = nn++;
+    for (int32_t i = 0; i < N; ++i) agg[i] = used[agg[i]];
+    c.coarse_pieces = (int32_t)roots.size();
+    na = nn;
   }
   std::vector<int32_t> cnt(na + 1, 0);
   for (int32_t i = 0; i < N; ++i) ++cnt[agg[i] + 1];
@@ -1109,8 +1183,12 @@ static int32_t coarse_setup(Context& c, int32_t target) {
     return HECMW33_ERROR_ARGS;
   }
   target = coarse_aim(c, target);
-  if (!c.coarse_sym || c.coarse_target != target || c.coarse_coord_gen != c.coord_gen)
+  const uint64_t mh = (int32_t)c.coarse_mask.size() == c.N ? c.coarse_mask_hash : 0;
+  if (!c.coarse_sym || c.coarse_target != target || c.coarse_coord_gen != c.coord_gen ||
+      c.coarse_sym_mask_hash != mh) {
     if (int32_t e = coarse_symbolic(c, target)) return e;
+    c.coarse_sym_mask_hash = mh;
+  }
   if (!c.coarse_num)
     if (int32_t e = coarse_numeric(c)) return e;
   return 0;
@@ -1163,32 +1241,6 @@ static void enqueue_fcg_sym_iteration(Context& c, int32_t kind, bool spmv64, hip
   if (ev) (void)hipEventRecord(ev[3], c.stream);
   fcg_beta(st, part, coarse_sym_post(c.coarse, c.N, sc, aggW, AZ, r, w, z, az, q, st, part, c.stream), c.stream);
   fcg_alpha(st, part, coarse_fcg_pq(n3, z, az, p, q, r, st, part, c.stream), c.stream);
-  fcg_update(n3, p, q, c.vx.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
-  fcg_reliable(c.A.view, c.vx.as<double>(), c.vb.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
-}
-
-// One learned flexible CG iteration: z0 = M32^-1 r (FP32 SSOR), z = z0 + b J r
-// with the four dots, the tail keeps z or falls back to z0, then as the
-// flexible CG.
-static void enqueue_fcg_learned_iteration(Context& c, bool spmv64, hipEvent_t* ev) {
-  const int64_t n3 = 3 * (int64_t)c.N;
-  KrylovState* st = c.st.as<KrylovState>();
-  double* part = c.part.as<double>();
-  float *r = c.vr32.as<float>(), *z = c.vz32.as<float>(), *z0 = c.vz0_32.as<float>();
-  float *p = c.vp32.as<float>(), *q = c.vq32.as<float>();
-  if (ev) (void)hipEventRecord(ev[0], c.stream);
-  ssor_apply_f32(c.ssor_op32(), r, z0, st, c.stream);
-  if (c.coarse_use) coarse_apply(c.coarse, c.N, r, z0, nullptr, st, nullptr, false, c.stream);
-  int np = 0;
-  learned_combine(c.N, c.vcoef.as<float>(), r, z0, c.alu_diag32.as<float>(), q, z, st, part, &np, c.stream);
-  fcg_beta_learned(st, part, np, c.stream);
-  if (ev) (void)hipEventRecord(ev[1], c.stream);
-  fcg_p_sel(n3, z, z0, p, st, c.stream);
-  if (ev) (void)hipEventRecord(ev[2], c.stream);
-  if (spmv64)
-    fcg_spmv_mixed(c.A.view, p, r, q, st, part, c.stream, ev ? ev[3] : nullptr);
-  else
-    fcg_spmv(c.A.view32, p, r, q, st, part, c.stream, ev ? ev[3] : nullptr);
   fcg_update(n3, p, q, c.vx.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
   fcg_reliable(c.A.view, c.vx.as<double>(), c.vb.as<double>(), r, st, part, c.hist.as<double>(), c.stream);
 }
@@ -1416,8 +1468,8 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
     set_error("SOLVER_OPT(1)=1/2 (flexible CG, FP32 operators) needs METHOD=1 and ITERPREMAX<=1");
     return HECMW33_ERROR_UNSUPPORTED;
   }
-  if (learned && (!c.net || (pc != 1 && pc != 2) || iterpremax < 1)) {
-    set_error("SOLVER_OPT(1)=2 (learned correction) needs an attached network and PRECOND 1 or 2");
+  if (learned && (!c.net || iterpremax < 1)) {
+    set_error("SOLVER_OPT(1)=2 (learned aggregation) needs an attached network and ITERPREMAX >= 1");
     return HECMW33_ERROR_UNSUPPORTED;
   }
   if (int32_t e = precond_setup(c, iarray, rarray)) return e;
@@ -1425,11 +1477,12 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
   if (mixed)
     if (int32_t e = setup_f32(c, kind)) return e;
   bool spmv64 = false;
+  const int32_t o3 = learned && IA(iarray, I_SOLVER_OPT3) <= 0 ? 1 : IA(iarray, I_SOLVER_OPT3);
   if (mixed) {
     const int32_t o2 = IA(iarray, I_SOLVER_OPT2);
     if (o2 == 2) {
       spmv64 = true;
-    } else if (o2 != 1 && IA(iarray, I_SOLVER_OPT3) <= 0) {
+    } else if (o2 != 1 && o3 <= 0) {
       // without the two-level correction the FCG runs long enough on
       // high-contrast systems for FP32-rounded values to lose conjugacy; with
       // it FP32 values win (120^3 inclusions, contrast 1e4: 2409 iterations,
@@ -1442,12 +1495,18 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
   c.coarse_use = false;
   c.last_coarse = 0;
   c.coarse_sym_use = false;
-  if (mixed && IA(iarray, I_SOLVER_OPT3) > 0) {
-    const int32_t o3 = IA(iarray, I_SOLVER_OPT3);
+  if (mixed && o3 > 0) {
+    // the learned solve refines the cells with the network's mask; the
+    // others use the cells alone
+    if (learned) {
+      if (int32_t e = learned_mask(c)) return e;
+    } else {
+      c.coarse_mask.clear();
+    }
     if (int32_t e = coarse_setup(c, o3)) return e;
     c.coarse_use = c.coarse_ok;
     c.last_coarse = c.coarse_ok ? c.coarse.n_agg : -1;
-    if (c.coarse_use && IA(iarray, I_SOLVER_OPT4) != 1 && !learned) {
+    if (c.coarse_use && IA(iarray, I_SOLVER_OPT4) != 1) {
       const size_t vb32 = sizeof(float) * 3 * (size_t)c.NP;
       for (DMem* m : {&c.vr1_32, &c.vt32, &c.vw32, &c.vaz32})
         if (m->ensure(vb32)) return HECMW33_ERROR_DEVICE;
@@ -1464,13 +1523,6 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
       }
       c.coarse_sym_use = true;
     }
-  }
-  if (learned) {
-    if (int32_t e = learned_setup(c, RA(rarray, R_SIGMA_DIAG))) return e;
-    // the network's coefficients for this system, fixed for the solve
-    net_forward(c.N, c.net->Wb.as<__bf16>(), c.vfeat.as<__bf16>(), nullptr, nullptr, nullptr, c.vcoef.as<float>(),
-                c.stream);
-    HIPC(hipGetLastError());
   }
   const int64_t n3 = 3 * (int64_t)c.N;
   const size_t vbytes = sizeof(double) * 3 * (size_t)c.NP;
@@ -1535,59 +1587,16 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
   h.delta2 = delta * delta;
   h.reliable = 0;
   h.nreliable = 0;
-  h.learned_off = learned ? 0 : 1;
-  h.learned_now = 0;
-  h.nfallback = 0;
+  // a learned solve whose two-level step was built keeps it unless its FP64
+  // residual grows between replacements (k_tail_fcg_rel)
+  h.learned_off = learned && c.coarse_use ? 0 : 1;
+  h.coarse_off = 0;
+  h.fallback_iter = 0;
   h.edrop = 0.0;
   h.gap = 1.0;
   c.h_st[1] = h;
   HIPC(hipMemcpyAsync(st, &c.h_st[1], sizeof(KrylovState), hipMemcpyHostToDevice, c.stream));
   HIPC(hipStreamSynchronize(c.stream));
-  if (learned) {
-    // "fails to reduce": kLearnProbe iterations from x0 with and without the
-    // correction; in (flexible) CG |e|_A^2 drops by alpha (r.p) per step, so
-    // the larger sum is the larger error reduction.  The solve then runs from
-    // x0 with the winner; plain SSOR stays the fallback.
-    if (c.vx0.ensure(vbytes)) return HECMW33_ERROR_DEVICE;
-    HIPC(hipMemcpyAsync(c.vx0.p, c.vx.p, vbytes, hipMemcpyDeviceToDevice, c.stream));
-    double drop[2] = {0.0, 0.0};
-    for (int pass = 0; pass < 2; ++pass) {
-      HIPC(hipMemcpyAsync(c.vx.p, c.vx0.p, vbytes, hipMemcpyDeviceToDevice, c.stream));
-      fcg_init_resid(c.A.view, c.vx.as<double>(), c.vb.as<double>(), c.vr32.as<float>(), &st->rr_ref, st, part,
-                     c.stream);
-      for (DMem* m : {&c.vz32, &c.vp32, &c.vq32}) HIPC(hipMemsetAsync(m->p, 0, vbytes / 2, c.stream));
-      HIPC(hipMemcpyAsync(&c.h_st[0], st, sizeof(KrylovState), hipMemcpyDeviceToHost, c.stream));
-      HIPC(hipStreamSynchronize(c.stream));
-      KrylovState hp = h;
-      hp.rr_ref = c.h_st[0].rr_ref;
-      hp.tol = 0.0;
-      hp.maxit = kLearnProbe + 1;
-      hp.learned_off = pass == 0 ? 0 : 1;
-      hp.edrop = 0.0;
-      hp.gap = 1.0;
-      c.h_st[1] = hp;
-      HIPC(hipMemcpyAsync(st, &c.h_st[1], sizeof(KrylovState), hipMemcpyHostToDevice, c.stream));
-      for (int i = 0; i < kLearnProbe; ++i) enqueue_fcg_learned_iteration(c, spmv64, nullptr);
-      HIPC(hipGetLastError());
-      HIPC(hipMemcpyAsync(&c.h_st[0], st, sizeof(KrylovState), hipMemcpyDeviceToHost, c.stream));
-      HIPC(hipStreamSynchronize(c.stream));
-      drop[pass] = c.h_st[0].error ? -1.0 : c.h_st[0].edrop;
-    }
-    h.learned_off = drop[0] > drop[1] ? 0 : 1;
-    c.last_probe_ratio = drop[1] > 0.0 ? drop[0] / drop[1] : 0.0;
-    // back to x0 and the initial residual for the solve itself
-    HIPC(hipMemcpyAsync(c.vx.p, c.vx0.p, vbytes, hipMemcpyDeviceToDevice, c.stream));
-    fcg_init_resid(c.A.view, c.vx.as<double>(), c.vb.as<double>(), c.vr32.as<float>(), &st->rr_ref, st, part,
-                   c.stream);
-    for (DMem* m : {&c.vz32, &c.vp32, &c.vq32}) HIPC(hipMemsetAsync(m->p, 0, vbytes / 2, c.stream));
-    HIPC(hipMemcpyAsync(&c.h_st[0], st, sizeof(KrylovState), hipMemcpyDeviceToHost, c.stream));
-    HIPC(hipStreamSynchronize(c.stream));
-    h.rr_ref = c.h_st[0].rr_ref;
-    h.edrop = 0.0;
-    c.h_st[1] = h;
-    HIPC(hipMemcpyAsync(st, &c.h_st[1], sizeof(KrylovState), hipMemcpyHostToDevice, c.stream));
-    HIPC(hipStreamSynchronize(c.stream));
-  }
   const double t1 = now_s();
   out->Tset = t1 - t0;
   // iterations: enqueue in chunks and poll the previous chunk's state while
@@ -1611,9 +1620,7 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
         for (int j = 0; j < nev; ++j) itev[j] = c.event((size_t)nev * nchunk + j);
         e = itev.data();
       }
-      if (learned)
-        enqueue_fcg_learned_iteration(c, spmv64, e);
-      else if (c.coarse_sym_use)
+      if (c.coarse_sym_use)
         enqueue_fcg_sym_iteration(c, kind, spmv64, e);
       else if (mixed)
         enqueue_fcg_iteration(c, kind, spmv64, e);
@@ -1663,7 +1670,7 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
   out->resid = h.resid;
   c.last_iter = std::min(h.iter, maxit);
   c.last_nreliable = mixed ? h.nreliable : 0;
-  c.last_nfallback = learned ? h.nfallback : 0;
+  c.last_fallback_iter = learned ? h.fallback_iter : 0;
   c.last_learned_off = learned ? h.learned_off : 1;
   return 0;
 }
@@ -2033,6 +2040,41 @@ int32_t hecmw33_ctx_matvec_device_f32(hecmw33_ctx* ctx, const float* x_dev, floa
   return 0;
 }
 
+// `count` back-to-back SSOR applications z = M^-1 r on device vectors with the
+// context's multicolour SSOR set-up (FP64, or its FP32 image when fp32 != 0:
+// float vectors); the mean time per application in *ms (HIP events on the
+// context's stream)
+int32_t hecmw33_ctx_ssor_device(hecmw33_ctx* ctx, const void* r_dev, void* z_dev, int32_t fp32, int32_t count,
+                                double* ms) {
+  if (!ctx || !ctx->c.have_struct || !r_dev || !z_dev || count < 1) return HECMW33_ERROR_ARGS;
+  Context& c = ctx->c;
+  if (!c.ssor_init) {
+    set_error("hecmw33_ctx_ssor_device: the context has no SSOR set-up (load with PRECOND 1 or 2)");
+    return HECMW33_ERROR_ARGS;
+  }
+  if (fp32)
+    if (int32_t e = setup_f32(c, 1)) return e;
+  hipEvent_t e0, e1;
+  HIPC(hipEventCreate(&e0));
+  HIPC(hipEventCreate(&e1));
+  HIPC(hipEventRecord(e0, c.stream));
+  for (int i = 0; i < count; ++i) {
+    if (fp32)
+      ssor_apply_f32(c.ssor_op32(), static_cast<const float*>(r_dev), static_cast<float*>(z_dev), nullptr, c.stream);
+    else
+      ssor_apply(c.ssor_op(), static_cast<const double*>(r_dev), static_cast<double*>(z_dev), nullptr, c.stream);
+  }
+  HIPC(hipEventRecord(e1, c.stream));
+  HIPC(hipEventSynchronize(e1));
+  float t = 0.f;
+  HIPC(hipEventElapsedTime(&t, e0, e1));
+  HIPC(hipGetLastError());
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ms) *ms = (double)t / count;
+  return 0;
+}
+
 int32_t hecmw33_ctx_history(hecmw33_ctx* ctx, double* hist, int32_t cap) {
   if (!ctx || !hist) return HECMW33_ERROR_ARGS;
   Context& c = ctx->c;
@@ -2064,10 +2106,10 @@ int32_t hecmw33_ctx_info(hecmw33_ctx* ctx, int64_t* info, int32_t n) {
                         (int64_t)c.ncolor,
                         c.L.view.nslots + c.U.view.nslots,
                         (int64_t)c.last_nreliable,
-                        (int64_t)c.last_nfallback,
+                        (int64_t)c.last_fallback_iter,
                         (int64_t)c.last_learned_off,
                         (int64_t)c.last_spmv64,
-                        (int64_t)std::llround(c.last_probe_ratio * 1e6),
+                        (int64_t)c.coarse_pieces,
                         (int64_t)c.last_coarse,
                         (int64_t)c.coarse_npair,
                         az_kept,
@@ -2078,7 +2120,86 @@ int32_t hecmw33_ctx_info(hecmw33_ctx* ctx, int64_t* info, int32_t n) {
 
 }  // extern "C"
 
-// ---- learned correction ---------------------------------------------------------------------
+// ---- learned aggregation ---------------------------------------------------------------------
+// Training labels from the solver's own errors e = x* - x_k = A^-1 r_k.  Per
+// stored edge k = (i, j) of the profile (the L items of every row, then the U
+// items; both nodes internal) the axial strain of e along the edge,
+// |(e_j - e_i).(x_j - x_i)| / |x_j - x_i|^2, divided by its median over the
+// edges of the sample; worst[k] keeps the largest ratio over the samples.
+// An edge whose e is zero at both ends (fixed degrees of freedom) or that
+// leaves the internal nodes says nothing: +inf.  In a body whose stiffness
+// varies by orders of magnitude the slow error strains the soft material and
+// moves the stiff pieces rigidly, so their edges stay orders of magnitude
+// below the median in every sample.
+static void rigid_edge_ratios(int32_t N, const int32_t* iL, const int32_t* tL, const int32_t* iU, const int32_t* tU,
+                              const double* X, const double* e, std::vector<float>& worst) {
+  const int64_t nl = iL[N], ne = (int64_t)iL[N] + iU[N];
+  const float inf = std::numeric_limits<float>::infinity();
+  if ((int64_t)worst.size() != ne) worst.assign(ne, 0.0f);
+  std::vector<double> eps(ne, -1.0);
+#pragma omp parallel for schedule(static)
+  for (int32_t i = 0; i < N; ++i)
+    for (int part = 0; part < 2; ++part) {
+      const int32_t* ix = part ? iU : iL;
+      const int32_t* it = part ? tU : tL;
+      for (int32_t q = ix[i]; q < ix[i + 1]; ++q) {
+        const int32_t j = it[q] - 1;
+        const int64_t k = part ? nl + q : q;
+        if (j >= N) continue;
+        double de[3], dx[3], num = 0.0, den = 0.0, mag = 0.0;
+        for (int d = 0; d < 3; ++d) {
+          de[d] = e[3 * (int64_t)j + d] - e[3 * (int64_t)i + d];
+          dx[d] = X[3 * (int64_t)j + d] - X[3 * (int64_t)i + d];
+          num += de[d] * dx[d];
+          den += dx[d] * dx[d];
+          mag += std::fabs(e[3 * (int64_t)j + d]) + std::fabs(e[3 * (int64_t)i + d]);
+        }
+        if (mag > 0.0 && den > 0.0) eps[k] = std::fabs(num) / den;
+      }
+    }
+  std::vector<double> live;
+  live.reserve(ne);
+  for (double v : eps)
+    if (v >= 0.0) live.push_back(v);
+  double med = 0.0;
+  if (!live.empty()) {
+    std::nth_element(live.begin(), live.begin() + live.size() / 2, live.end());
+    med = live[live.size() / 2];
+  }
+  for (int64_t k = 0; k < ne; ++k) {
+    float r;
+    if (eps[k] < 0.0)
+      r = inf;
+    else if (med > 0.0)
+      r = (float)(eps[k] / med);
+    else
+      r = eps[k] > 0.0 ? inf : 0.0f;
+    worst[k] = std::max(worst[k], r);
+  }
+}
+
+// label 1 for both nodes of every edge whose worst ratio is below tau
+static int32_t rigid_labels_from_ratios(int32_t N, const int32_t* iL, const int32_t* tL, const int32_t* iU,
+                                        const int32_t* tU, const std::vector<float>& worst, double tau, float* t) {
+  const int64_t nl = iL[N];
+  std::fill(t, t + N, 0.0f);
+  for (int32_t i = 0; i < N; ++i)
+    for (int part = 0; part < 2; ++part) {
+      const int32_t* ix = part ? iU : iL;
+      const int32_t* it = part ? tU : tL;
+      for (int32_t q = ix[i]; q < ix[i + 1]; ++q) {
+        const int64_t k = part ? nl + q : q;
+        if (k < (int64_t)worst.size() && worst[k] < tau) {
+          t[i] = 1.0f;
+          t[it[q] - 1] = 1.0f;
+        }
+      }
+    }
+  int32_t cnt = 0;
+  for (int32_t i = 0; i < N; ++i) cnt += t[i] > 0.5f;
+  return cnt;
+}
+
 static uint16_t to_bf16_bits(float f) {
   // round to nearest even (what the device's float -> __bf16 conversion does)
   uint32_t u;
@@ -2122,8 +2243,7 @@ static int32_t net_upload(hecmw33_net& N, const float* w) {
 static int32_t net_work(hecmw33_net& N, int64_t n) {
   const size_t n3 = 3 * (size_t)n;
   if (N.H1.ensure(2 * 64 * (size_t)n) || N.H2.ensure(2 * 64 * (size_t)n) || N.z.ensure(sizeof(float) * n3) ||
-      N.y.ensure(sizeof(float) * n3) || N.dYc.ensure(sizeof(float) * n3) ||
-      N.q.ensure(sizeof(float) * n3) || N.dY.ensure(sizeof(float) * n3) ||
+      N.y.ensure(sizeof(float) * n3) || N.dY.ensure(sizeof(float) * n3) ||
       N.wpart.ensure(sizeof(float) * (size_t)net_bwd_parts(n) * net_weight_count()))
     return HECMW33_ERROR_DEVICE;
   return 0;
@@ -2201,39 +2321,35 @@ int32_t hecmw33_net_eval(hecmw33_net* net, int64_t n, const float* X, const floa
   return 0;
 }
 
-// One training sample from a device context holding an SSOR set-up: the
-// residual r (device, FP64, 3*NP), its FP32 SSOR and block-Jacobi outputs.
-// Returns the sample count.
-int32_t hecmw33_net_add_sample(hecmw33_net* net, hecmw33_ctx* ctx, const double* r_dev, double sigma_diag) {
-  if (!net || !ctx || !r_dev || !ctx->c.have_struct) return HECMW33_ERROR_ARGS;
+// Error samples e = A^-1 r (device, FP64, 3*NP) on a context with node
+// coordinates: per stored edge of the profile, the largest ratio over the
+// samples of its axial strain to the sample's median (rigid_edge_ratios);
+// the labels follow at training.  Returns the context's sample count.
+int32_t hecmw33_net_add_sample(hecmw33_net* net, hecmw33_ctx* ctx, const double* e_dev, double tau) {
+  if (!net || !ctx || !e_dev || !ctx->c.have_struct || !(tau > 0.0)) return HECMW33_ERROR_ARGS;
   Context& c = ctx->c;
-  if (int32_t e = learned_setup(c, sigma_diag)) return e;
-  hecmw33_net::Sample* S = new (std::nothrow) hecmw33_net::Sample();
-  if (!S) return HECMW33_ERROR_DEVICE;
-  S->c = &c;
-  S->n = c.N;
-  const int64_t n3 = 3 * (int64_t)c.N;
-  if (S->r32.ensure(sizeof(float) * 3 * (size_t)c.NP) || S->z0.ensure(sizeof(float) * 3 * (size_t)c.NP) ||
-      S->jr.ensure(sizeof(float) * 3 * (size_t)c.NP) || net_work(*net, c.N)) {
-    delete S;
-    return HECMW33_ERROR_DEVICE;
+  if ((int64_t)c.h_coord.size() != 3 * (int64_t)c.NP) {
+    set_error("learned aggregation samples need the node coordinates of the system (hecmw33_ctx_set_coords)");
+    return HECMW33_ERROR_ARGS;
   }
-  HIPC(hipMemsetAsync(S->r32.p, 0, sizeof(float) * 3 * (size_t)c.NP, c.stream));
-  launch_to_f32(n3, r_dev, S->r32.as<float>(), c.stream);
-  ssor_apply_f32(c.ssor_op32(), S->r32.as<float>(), S->z0.as<float>(), nullptr, c.stream);
-  precond_apply_f32(3, c.ssor_op32(), c.alu_diag32.as<float>(), c.N, S->r32.as<float>(), S->jr.as<float>(), nullptr,
-                    c.stream);
-  // energy functional of the plain SSOR output: (z0.A z0 - 2 z0.r) / (r.z0)
-  launch_spmv_f32(c.A.view32, S->z0.as<float>(), net->q.as<float>(), c.stream);
-  net_energy_dots(n3, S->z0.as<float>(), net->q.as<float>(), S->r32.as<float>(), S->z0.as<float>(),
-                  net->dpart.as<double>(), net->dots.as<double>(), c.stream);
-  double d[3];
-  HIPC(hipMemcpyAsync(d, net->dots.p, sizeof(d), hipMemcpyDeviceToHost, c.stream));
-  HIPC(hipGetLastError());
+  hecmw33_net::Sample* S = nullptr;
+  for (auto* p : net->samples)
+    if (p->c == &c) S = p;
+  if (!S) {
+    S = new (std::nothrow) hecmw33_net::Sample();
+    if (!S) return HECMW33_ERROR_DEVICE;
+    S->c = &c;
+    net->samples.push_back(S);
+  }
+  std::vector<double> e(3 * (size_t)c.N);
+  HIPC(hipMemcpyAsync(e.data(), e_dev, sizeof(double) * e.size(), hipMemcpyDeviceToHost, c.stream));
   HIPC(hipStreamSynchronize(c.stream));
-  S->base = (d[0] - 2.0 * d[1]) / d[2];
-  net->samples.push_back(S);
-  return (int32_t)net->samples.size();
+  rigid_edge_ratios(c.N, c.h_indexL.data(), c.h_itemL.data(), c.h_indexU.data(), c.h_itemU.data(), c.h_coord.data(),
+                    e.data(), S->worst);
+  S->nsamp += 1;
+  S->tau = tau;
+  S->t_valid = false;
+  return S->nsamp;
 }
 
 int32_t hecmw33_net_clear_samples(hecmw33_net* net) {
@@ -2242,25 +2358,57 @@ int32_t hecmw33_net_clear_samples(hecmw33_net* net) {
   return 0;
 }
 
-// Adam on the energy functional L(z) = (z.Az - 2 z.r) / (r.z0),
-// z = z0 + b J r with b = softplus(N(g) - 4) on the context's static features,
-// one sample per step, samples in a seeded shuffled order every epoch.
-// loss[0] = mean L of the plain SSOR output, loss[1+e] = mean L over epoch e
-// (before each step's update).
+// labels of a sample's context on the device (once per set of samples)
+static int32_t sample_labels(hecmw33_net::Sample& S) {
+  if (S.t_valid) return 0;
+  Context& c = *S.c;
+  std::vector<float> t(c.N);
+  rigid_labels_from_ratios(c.N, c.h_indexL.data(), c.h_itemL.data(), c.h_indexU.data(), c.h_itemU.data(), S.worst,
+                           S.tau, t.data());
+  if (S.t.ensure(sizeof(float) * (size_t)c.N)) return HECMW33_ERROR_DEVICE;
+  HIPC(hipMemcpy(S.t.p, t.data(), sizeof(float) * t.size(), hipMemcpyHostToDevice));
+  S.t_valid = true;
+  return 0;
+}
+
+// loss and accuracy of the current weights on one sample (forward on the context's features)
+static int32_t sample_bce(hecmw33_net& N, hecmw33_net::Sample& S, bool keep, double* out) {
+  Context& c = *S.c;
+  if (int32_t e = features_setup(c)) return e;
+  HIPC(hipStreamSynchronize(c.stream));
+  if (int32_t e = net_work(N, c.N)) return e;
+  net_forward(c.N, N.Wb.as<__bf16>(), c.vfeat.as<__bf16>(), keep ? N.H1.as<__bf16>() : nullptr,
+              keep ? N.H2.as<__bf16>() : nullptr, nullptr, N.y.as<float>(), N.stream);
+  net_bce(c.N, N.y.as<float>(), S.t.as<float>(), N.dY.as<float>(), N.dpart.as<double>(), N.dots.as<double>(),
+          N.stream);
+  HIPC(hipMemcpyAsync(out, N.dots.p, 2 * sizeof(double), hipMemcpyDeviceToHost, N.stream));
+  return 0;
+}
+
+// Adam on the mean logistic loss of the network's output 0 against the labels,
+// one sample (context) per step, in a seeded shuffled order every epoch.
+// loss[0] = mean loss of the weights before training, loss[1+e] = mean loss
+// over epoch e (before each step's update).
 int32_t hecmw33_net_train(hecmw33_net* net, int32_t epochs, double lr, uint64_t seed, double* loss) {
   if (!net || epochs < 0 || net->samples.empty()) return HECMW33_ERROR_ARGS;
   hecmw33_net& N = *net;
   const size_t ns = N.samples.size();
+  int64_t nmax = 0;
+  for (auto* S : N.samples) {
+    if (int32_t e = sample_labels(*S)) return e;
+    nmax = std::max<int64_t>(nmax, S->c->N);
+  }
+  if (int32_t e = net_work(N, nmax)) return e;  // no reallocation inside the loop
+  std::vector<double> d(2 * ns);
   double base = 0.0;
-  for (auto* S : N.samples) base += S->base;
+  for (size_t k = 0; k < ns; ++k)
+    if (int32_t e = sample_bce(N, *N.samples[k], false, &d[2 * k])) return e;
+  HIPC(hipStreamSynchronize(N.stream));
+  for (size_t k = 0; k < ns; ++k) base += d[2 * k] / (double)N.samples[k]->c->N;
   if (loss) loss[0] = base / (double)ns;
   std::vector<size_t> order(ns);
   for (size_t i = 0; i < ns; ++i) order[i] = i;
   uint64_t st = seed + 0x1234567ull;
-  std::vector<double> dots(3 * ns);
-  int64_t nmax = 0;
-  for (auto* S : N.samples) nmax = std::max(nmax, S->n);
-  if (int32_t e = net_work(N, nmax)) return e;  // no reallocation inside the loop
   for (int ep = 0; ep < epochs; ++ep) {
     for (size_t i = ns; i > 1; --i) {  // Fisher-Yates, splitmix64
       uint64_t z = (st += 0x9E3779B97F4A7C15ull);
@@ -2272,56 +2420,38 @@ int32_t hecmw33_net_train(hecmw33_net* net, int32_t epochs, double lr, uint64_t 
     for (size_t k = 0; k < ns; ++k) {
       hecmw33_net::Sample& S = *N.samples[order[k]];
       Context& c = *S.c;
-      const int64_t n = S.n, n3 = 3 * n;
-      if (int32_t e = net_work(N, n)) return e;
-      net_forward(n, N.Wb.as<__bf16>(), c.vfeat.as<__bf16>(), N.H1.as<__bf16>(), N.H2.as<__bf16>(), nullptr,
-                  N.y.as<float>(), N.stream);
-      learned_combine_train(n, N.y.as<float>(), S.z0.as<float>(), S.jr.as<float>(), N.z.as<float>(), N.stream);
-      launch_spmv_f32(c.A.view32, N.z.as<float>(), N.q.as<float>(), N.stream);
-      net_energy_dots(n3, N.z.as<float>(), N.q.as<float>(), S.r32.as<float>(), S.z0.as<float>(),
-                      N.dpart.as<double>(), N.dots.as<double>(), N.stream);
-      HIPC(hipMemcpyAsync(&dots[3 * k], N.dots.p, 3 * sizeof(double), hipMemcpyDeviceToHost, N.stream));
-      net_energy_grad(n3, N.q.as<float>(), S.r32.as<float>(), N.dots.as<double>(), N.dY.as<float>(), N.stream);
-      learned_coef_grad(n, N.y.as<float>(), N.dY.as<float>(), S.jr.as<float>(), N.dYc.as<float>(), N.stream);
-      net_backward(n, N.Wb.as<__bf16>(), c.vfeat.as<__bf16>(), N.H1.as<__bf16>(), N.H2.as<__bf16>(),
-                   N.dYc.as<float>(), N.wpart.as<float>(), N.dW.as<float>(), N.stream);
+      if (int32_t e = sample_bce(N, S, true, &d[2 * k])) return e;
+      net_backward(c.N, N.Wb.as<__bf16>(), c.vfeat.as<__bf16>(), N.H1.as<__bf16>(), N.H2.as<__bf16>(),
+                   N.dY.as<float>(), N.wpart.as<float>(), N.dW.as<float>(), N.stream);
       net_adam(N.Wm.as<float>(), N.M.as<float>(), N.V.as<float>(), N.dW.as<float>(), N.Wb.as<__bf16>(), (float)lr,
                0.9f, 0.999f, 1e-8f, ++N.step, 1.0f, N.stream);
     }
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(N.stream));
     double acc = 0.0;
-    for (size_t k = 0; k < ns; ++k) acc += (dots[3 * k] - 2.0 * dots[3 * k + 1]) / dots[3 * k + 2];
+    for (size_t k = 0; k < ns; ++k) acc += d[2 * k] / (double)N.samples[order[k]]->c->N;
     if (loss) loss[1 + ep] = acc / (double)ns;
   }
+  N.gen += 1;
   return 0;
 }
 
-// Mean energy functional of the network's correction over the current
-// samples without training (for held-out checks); base (optional): the plain
-// SSOR output's.
-int32_t hecmw33_net_score(hecmw33_net* net, double* mean, double* base) {
+// Mean logistic loss (*mean) and the fraction of nodes classified as their
+// label (*accuracy, optional) of the current weights over the samples.
+int32_t hecmw33_net_score(hecmw33_net* net, double* mean, double* accuracy) {
   if (!net || !mean || net->samples.empty()) return HECMW33_ERROR_ARGS;
   hecmw33_net& N = *net;
-  double acc = 0.0, b = 0.0;
+  double l = 0.0, a = 0.0;
   for (auto* Sp : N.samples) {
-    hecmw33_net::Sample& S = *Sp;
-    Context& c = *S.c;
-    const int64_t n = S.n, n3 = 3 * n;
-    if (int32_t e = net_work(N, n)) return e;
-    net_forward(n, N.Wb.as<__bf16>(), c.vfeat.as<__bf16>(), nullptr, nullptr, nullptr, N.y.as<float>(), N.stream);
-    learned_combine_train(n, N.y.as<float>(), S.z0.as<float>(), S.jr.as<float>(), N.z.as<float>(), N.stream);
-    launch_spmv_f32(c.A.view32, N.z.as<float>(), N.q.as<float>(), N.stream);
-    net_energy_dots(n3, N.z.as<float>(), N.q.as<float>(), S.r32.as<float>(), S.z0.as<float>(), N.dpart.as<double>(),
-                    N.dots.as<double>(), N.stream);
-    double d[3];
-    HIPC(hipMemcpyAsync(d, N.dots.p, sizeof(d), hipMemcpyDeviceToHost, N.stream));
+    if (int32_t e = sample_labels(*Sp)) return e;
+    double d[2];
+    if (int32_t e = sample_bce(N, *Sp, false, d)) return e;
     HIPC(hipStreamSynchronize(N.stream));
-    acc += (d[0] - 2.0 * d[1]) / d[2];
-    b += S.base;
+    l += d[0] / (double)Sp->c->N;
+    a += d[1] / (double)Sp->c->N;
   }
-  *mean = acc / (double)N.samples.size();
-  if (base) *base = b / (double)N.samples.size();
+  *mean = l / (double)N.samples.size();
+  if (accuracy) *accuracy = a / (double)N.samples.size();
   return 0;
 }
 
@@ -2374,6 +2504,18 @@ int32_t hecmw33_ctx_coarse_get(hecmw33_ctx* ctx, int32_t* n_agg, int32_t* n_pair
     HIPC(hipMemcpy(Eblk, c.cg_Eblk.p, sizeof(double) * 36 * (size_t)np, hipMemcpyDeviceToHost));
   }
   return 0;
+}
+
+int32_t hecmw33_rigid_labels(int32_t N, const int32_t* indexL, const int32_t* itemL, const int32_t* indexU,
+                             const int32_t* itemU, const double* xyz, int32_t nsamp, const double* e, double tau,
+                             float* labels) {
+  if (N < 1 || !indexL || !indexU || !xyz || nsamp < 1 || !e || !labels || !(tau > 0.0) ||
+      (indexL[N] > 0 && !itemL) || (indexU[N] > 0 && !itemU))
+    return HECMW33_ERROR_ARGS;
+  std::vector<float> worst;
+  for (int32_t k = 0; k < nsamp; ++k)
+    rigid_edge_ratios(N, indexL, itemL, indexU, itemU, xyz, e + 3 * (int64_t)N * k, worst);
+  return rigid_labels_from_ratios(N, indexL, itemL, indexU, itemU, worst, tau, labels);
 }
 
 int32_t hecmw33_ctx_set_net(hecmw33_ctx* ctx, hecmw33_net* net) {

@@ -76,6 +76,7 @@ def lib():
     L.hecmw33_ctx_matvec_device.argtypes = [P, P, P, I, C.POINTER(C.c_double)]
     L.hecmw33_ctx_matvec_device_f32.argtypes = [P, P, P, I, C.POINTER(C.c_double)]
     L.hecmw33_stream_read.argtypes = [P, C.c_int64, P, I, C.POINTER(C.c_double)]
+    L.hecmw33_ctx_ssor_device.argtypes = [P, P, P, I, I, C.POINTER(C.c_double)]
     L.hecmw33_ctx_history.argtypes = [P, P, I]
     L.hecmw33_ctx_info.argtypes = [P, P, I]
     L.hecmw33_gen_hex8_count.argtypes = [I, C.POINTER(I), C.POINTER(I), C.POINTER(I)]
@@ -106,6 +107,7 @@ def lib():
     L.hecmw33_net_clear_samples.argtypes = [P]
     L.hecmw33_net_train.argtypes = [P, I, C.c_double, C.c_uint64, P]
     L.hecmw33_net_score.argtypes = [P, P, P]
+    L.hecmw33_rigid_labels.argtypes = [I, P, P, P, P, P, I, P, C.c_double, P]
     L.hecmw33_ctx_set_net.argtypes = [P, P]
     L.hecmw33_set_net.argtypes = [P]
     L.hecmw33_set_coords.argtypes = [I, P]
@@ -125,7 +127,7 @@ def lib():
               "hecmw33_net_get_weights", "hecmw33_net_set_weights", "hecmw33_net_eval", "hecmw33_net_add_sample",
               "hecmw33_net_clear_samples", "hecmw33_net_train", "hecmw33_net_score", "hecmw33_ctx_set_net",
               "hecmw33_set_net", "hecmw33_set_coords", "hecmw33_ctx_set_coords", "hecmw33_ctx_coarse_get", "hecmw33_set_mpc",
-              "hecmw33_ctx_set_mpc", "hecmw33_matvec_mpc"):
+              "hecmw33_ctx_set_mpc", "hecmw33_matvec_mpc", "hecmw33_rigid_labels", "hecmw33_ctx_ssor_device"):
         getattr(L, f).restype = C.c_int32
     _lib = L
     return L
@@ -524,6 +526,13 @@ class DeviceContext:
         _check(f(self._h, C.c_void_p(x_ptr), C.c_void_p(y_ptr), count, C.byref(ms)))
         return ms.value
 
+    def ssor_device(self, r_ptr, z_ptr, count=1, fp32=False):
+        """count SSOR applications on device vectors; mean ms per application"""
+        ms = C.c_double(0.0)
+        _check(lib().hecmw33_ctx_ssor_device(self._h, C.c_void_p(r_ptr), C.c_void_p(z_ptr), 1 if fp32 else 0, count,
+                                             C.byref(ms)))
+        return ms.value
+
     def history(self, cap):
         h = np.zeros(cap)
         n = _check(lib().hecmw33_ctx_history(self._h, _ptr(h), cap))
@@ -533,8 +542,8 @@ class DeviceContext:
         v = np.zeros(15, np.int64)
         _check(lib().hecmw33_ctx_info(self._h, _ptr(v), 15))
         return {"nslots": int(v[0]), "N": int(v[1]), "NPL": int(v[2]), "NPU": int(v[3]), "ncolor": int(v[4]),
-                "ssor_slots": int(v[5]), "nreliable": int(v[6]), "nfallback": int(v[7]), "learned_off": int(v[8]),
-                "spmv64": int(v[9]), "probe_ratio": v[10] / 1e6, "coarse": int(v[11]), "coarse_pairs": int(v[12]),
+                "ssor_slots": int(v[5]), "nreliable": int(v[6]), "fallback_iter": int(v[7]), "learned_off": int(v[8]),
+                "spmv64": int(v[9]), "pieces": int(v[10]), "coarse": int(v[11]), "coarse_pairs": int(v[12]),
                 "az_entries": int(v[13]), "az_entries_symbolic": int(v[14])}
 
     def coarse(self):
@@ -697,9 +706,12 @@ def gen_tet10_batch(nsys=48, nrhs=16, dof_min=5000, dof_max=600000, seed=0, E=21
 
 
 class LearnedNet:
-    """The learned correction (hecmw33_net_*): a per-node MLP 16 -> 64 -> 64 -> 3
-    (ReLU; BF16 MFMA kernels) mapping static node features to the coefficient
-    b = softplus(y - 4) of z = z0 + b J r inside the flexible CG; trained on the GPU."""
+    """The learned aggregation (hecmw33_net_*): a per-node MLP 16 -> 64 -> 64 -> 3
+    (ReLU; BF16 MFMA kernels) whose first output is the logit of "the slow
+    error moves this node rigidly with a stiff piece"; the flexible CG's
+    two-level step (Iarray 41 = 2) splits the marked nodes of every cell into
+    their own aggregates.  Trained on the GPU on labels built from the
+    solver's own errors (add_sample)."""
 
     def __init__(self, seed=0):
         self._h = lib().hecmw33_net_create(seed)
@@ -741,8 +753,10 @@ class LearnedNet:
             return Y
         return Y, (dW[:1024].reshape(16, 64), dW[1024:5120].reshape(64, 64), dW[5120:].reshape(64, 16))
 
-    def add_sample(self, ctx, r_ptr, sigma_diag=1.0):
-        return _check(lib().hecmw33_net_add_sample(self._h, ctx._h, C.c_void_p(r_ptr), sigma_diag))
+    def add_sample(self, ctx, e_ptr, tau=0.1):
+        """An error sample e = A^-1 r (device FP64, 3*NP) on a loaded context
+        with node coordinates; returns that context's sample count."""
+        return _check(lib().hecmw33_net_add_sample(self._h, ctx._h, C.c_void_p(e_ptr), tau))
 
     def clear_samples(self):
         _check(lib().hecmw33_net_clear_samples(self._h))
@@ -753,10 +767,11 @@ class LearnedNet:
         return loss
 
     def score(self):
+        """(mean logistic loss, fraction of nodes classified as their label) over the samples."""
         m = C.c_double(0.0)
-        b = C.c_double(0.0)
-        _check(lib().hecmw33_net_score(self._h, C.byref(m), C.byref(b)))
-        return m.value, b.value
+        a = C.c_double(0.0)
+        _check(lib().hecmw33_net_score(self._h, C.byref(m), C.byref(a)))
+        return m.value, a.value
 
 
 def set_net(net):
@@ -764,34 +779,52 @@ def set_net(net):
     _check(lib().hecmw33_set_net(net._h if net is not None else None))
 
 
-def harvest_residuals(net, ctx, mat, b_dev, iters=(1, 2, 3, 5, 8, 13, 21, 34, 55, 89)):
-    """Training residuals from the solver itself: r_k = b - A x_k after k
-    FP64 SSOR-CG iterations from x0 = 0 (the context keeps its loaded
-    preconditioner), normalised, added to the network's samples.  b_dev: a
-    torch float64 device tensor of 3*NP entries.  Returns the sample count."""
+def harvest_errors(net, ctx, mat, b_dev, iters=(10, 40), tau=0.1):
+    """Training samples from the solver itself: the FP64 multicolour-SSOR CG
+    solution x* of the loaded system, then the errors e_k = x* - x_k = A^-1 r_k
+    of its iterates at the cumulative iteration counts `iters` (the solve
+    continues from the previous checkpoint's x), added to the network's samples
+    (labels: hecmw33_rigid_labels).  Iterates that have converged (|e| below
+    1e-10 |x*|) give no sample.  The context needs the node coordinates.
+    b_dev: a torch float64 device tensor of 3*NP entries.  Returns the
+    context's sample count (0 if none was added)."""
     import torch
     ia = mat.Iarray.copy()
     ia[mat.I_FLAG_NUMFACT - 1] = 0
     ia[mat.I_FLAG_SYMBFACT - 1] = 0
     ia[mat.I_SOLVER_OPT - 1] = 0
+    ia[mat.I_PRECOND - 1] = 1
+    xs = torch.zeros_like(b_dev)
+    res = ctx.solve_device(b_dev.data_ptr(), xs.data_ptr(), ia, mat.Rarray)
+    if res.error != 0:
+        raise HecmwError("harvest_errors: the reference solve failed (%d)" % res.error)
+    nx = float(torch.linalg.vector_norm(xs))
     x = torch.zeros_like(b_dev)
-    y = torch.zeros_like(b_dev)
-    n3 = 3 * mat.N
-    count = 0
-    for k in iters:
-        ia[mat.I_ITER - 1] = k
-        x.zero_()
+    count, done = 0, 0
+    for k in sorted(iters):
+        if k <= done:
+            continue
+        ia[mat.I_ITER - 1] = k - done
         torch.cuda.synchronize()
         res = ctx.solve_device(b_dev.data_ptr(), x.data_ptr(), ia, mat.Rarray)
+        done = k
         if res.error not in (0, 3001):
-            continue
-        ctx.matvec_device(x.data_ptr(), y.data_ptr())
-        r = b_dev - y
-        r[n3:] = 0.0
-        nr = float(torch.linalg.vector_norm(r))
-        if not np.isfinite(nr) or nr == 0.0:
-            continue
-        r /= nr
+            break
+        e = xs - x
+        if float(torch.linalg.vector_norm(e)) <= 1e-10 * nx:
+            break
         torch.cuda.synchronize()
-        count = net.add_sample(ctx, r.data_ptr(), float(mat.Rarray[mat.R_SIGMA_DIAG - 1]))
+        count = net.add_sample(ctx, e.data_ptr(), tau)
     return count
+
+
+def rigid_labels(mat, xyz, errors, tau=0.1):
+    """hecmw33_rigid_labels (host): labels (N) of the nodes of every edge whose
+    axial strain stays below tau times the sample median in every error
+    sample; errors: (nsamp, 3N)."""
+    e = np.ascontiguousarray(errors, dtype=np.float64).reshape(-1, 3 * mat.N)
+    X = np.ascontiguousarray(xyz, dtype=np.float64).ravel()
+    t = np.zeros(mat.N, np.float32)
+    _check(lib().hecmw33_rigid_labels(mat.N, _ptr(mat.indexL), _ptr(mat.itemL), _ptr(mat.indexU), _ptr(mat.itemU),
+                                      _ptr(X), e.shape[0], _ptr(e), float(tau), _ptr(t)))
+    return t

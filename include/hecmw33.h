@@ -112,7 +112,7 @@ int32_t hecmw33_solve_krylov(const hecmw33_sys* A, const double* B, double* X, i
  *   z1 = Q r, z2 = z1 + M^-1 (r - A z1), z = z2 + Q (r - A z2) with Q = Z E^-1 Z^T
  *   (A Z is kept per node and neighbouring aggregate, and q = A p is carried as
  *   A z + beta q, so an iteration still has one SpMV); 1 the additive
- *   z = M^-1 r + Q r, which the learned solve (Iarray 41 = 2) always uses.
+ *   z = M^-1 r + Q r.
  *   Accepted by hecmw33_solve, hecmw33_solve_krylov and hecmw33_ctx_solve_device. */
 
 /* Node coordinates for the two-level correction: xyz interleaved, 3*NP doubles
@@ -181,6 +181,12 @@ int32_t hecmw33_ctx_solve_device(hecmw33_ctx* ctx, const double* B_dev, double* 
  * kernel time in *ms (HIP events on the context's stream). */
 int32_t hecmw33_ctx_matvec_device(hecmw33_ctx* ctx, const double* x_dev, double* y_dev, int32_t count, double* ms);
 int32_t hecmw33_ctx_matvec_device_f32(hecmw33_ctx* ctx, const float* x_dev, float* y_dev, int32_t count, double* ms);
+/* `count` back-to-back multicolour SSOR applications z = M^-1 r (the context's SSOR set-up,
+ * hecmw_precond_SSOR_33_apply's sweeps) on device vectors of 3*NP entries: FP64, or with
+ * fp32 != 0 the FP32 image the flexible CG uses (float vectors); mean time per application in
+ * *ms (HIP events on the context's stream). */
+int32_t hecmw33_ctx_ssor_device(hecmw33_ctx* ctx, const void* r_dev, void* z_dev, int32_t fp32, int32_t count,
+                                double* ms);
 /* No counterpart in the reference (a measurement aid for the benchmark): the
  * HBM read bandwidth a plain streaming kernel reaches on this GPU, in GB/s,
  * best of a few launch shapes, each timed over `reps` reads of the n doubles
@@ -191,10 +197,10 @@ int32_t hecmw33_stream_read(const double* buf_dev, int64_t n, double* part_dev, 
 int32_t hecmw33_ctx_history(hecmw33_ctx* ctx, double* hist, int32_t cap);
 /* Context figures: [0] SELL-64 slots of A (incl. padding), [1] N, [2] NPL, [3] NPU, [4] colours,
  * [5] SSOR slots of L' + U', [6] FP64 residual replacements of the last flexible CG solve,
- * [7] iterations of the last learned solve that fell back to the SSOR output, [8] 1 if that
- * solve dropped the learned correction (or was not a learned solve), [9] 1 if the last flexible
- * CG used the FP64 matrix values in q = A p, [10] the last learned probe's error reduction with
- * / without the correction, in millionths, [11] aggregates of the last flexible CG's two-level
+ * [7] the iteration at which the last learned solve dropped its two-level step (0: kept), [8] 1 if
+ * that solve ran without the learned step (or was not a learned solve), [9] 1 if the last flexible
+ * CG used the FP64 matrix values in q = A p, [10] aggregates the network split off in the current
+ * aggregation, [11] aggregates of the last flexible CG's two-level
  * correction (0: none, -1: E was not positive definite and the solve ran without it), [12] coupled
  * aggregate pairs (blocks of E), [13] A Z entries the symmetric two-level step streams (after
  * dropping the rows whose A Z is zero in exact arithmetic), [14] A Z entries before that (node and
@@ -248,19 +254,21 @@ int32_t hecmw33_solve_batch(int32_t nsys, const hecmw33_sys* sys, int32_t nrhs, 
                             double* const* X, const int32_t iarray[100], const double rarray[100], int32_t* iter,
                             int32_t* error, double* resid);
 
-/* ---- learned correction (no counterpart in the reference) -----------------------------------
- * Inside the flexible CG (iarray[40] = Iarray 41 = 2, PRECOND 1/2, with an attached network:
+/* ---- learned aggregation (no counterpart in the reference) ---------------------------------
+ * Inside the flexible CG (iarray[40] = Iarray 41 = 2, with an attached network:
  * hecmw33_ctx_set_net for a device context, hecmw33_set_net for hecmw33_solve /
- * hecmw33_solve_krylov) the preconditioned residual is z_i = z0_i + b_i (J r)_i: z0 the FP32
- * SSOR output, J r the block-Jacobi one, b_i = softplus(y_i - 4) > 0 with y_i from a per-node MLP
- * (16 static node features -> 64 -> 64 -> 3, ReLU; BF16 MFMA kernels accumulating in FP32)
- * evaluated once per solve, so the preconditioner is linear in r, symmetric and positive
- * definite.  Plain SSOR stays the fallback: each solve first runs 25 iterations from x0 with and
- * without the correction and keeps it only if its A-norm error reduction (sum of alpha r.p) is
- * the larger; during the solve the plain SSOR output replaces z when z is not a descent
- * direction, and the correction is dropped when the FP64 residual grows 4x between two residual
- * replacements.  Trained on the GPU on the energy functional (z.Az - 2 z.r)/(r.z0) of residuals
- * harvested on device contexts.
+ * hecmw33_solve_krylov) the two-level step of Iarray 43 (1 when not set) and Iarray 44 takes its
+ * coarse space from an aggregation the network refines: a per-node MLP (16 static node features
+ * -> 64 -> 64 -> 3, ReLU; BF16 MFMA kernels accumulating in FP32) gives each node the logit of
+ * "the slow error moves this node rigidly with a stiff piece"; the nodes with a positive logit
+ * leave their cell, one aggregate (six rigid-body modes) per connected piece of at least 32
+ * nodes.  The preconditioner stays symmetric positive definite.  Plain SSOR stays the fallback:
+ * the solve runs the plain SSOR flexible CG from the iteration at which its FP64 residual grows 4x
+ * between two residual replacements (hecmw33_ctx_info [7], [8]).
+ * Training: error samples e = x* - x_k = A^-1 r_k of the solver's own iterates, on device
+ * contexts with node coordinates; an edge of the matrix graph whose axial strain stays below tau
+ * times the sample's median in every sample lies in a rigidly moving piece, and its nodes are
+ * labelled 1; Adam on the logistic loss of the network's logit against the labels.
  * A network must outlive the contexts it is attached to; contexts must outlive its samples. */
 typedef struct hecmw33_net hecmw33_net;
 hecmw33_net* hecmw33_net_create(uint64_t seed);
@@ -271,14 +279,23 @@ int32_t hecmw33_net_get_weights(hecmw33_net* net, float* w, int32_t n);
 int32_t hecmw33_net_set_weights(hecmw33_net* net, const float* w, int32_t n);
 /* Y = N(X) for n x 16 features (host); with dY (n x 3) also dW = dL/dW for L = sum dY.Y */
 int32_t hecmw33_net_eval(hecmw33_net* net, int64_t n, const float* X, const float* dY, float* Y, float* dW);
-/* residual r (device, FP64, 3*NP) on a context loaded with an SSOR set-up; returns the sample count */
-int32_t hecmw33_net_add_sample(hecmw33_net* net, hecmw33_ctx* ctx, const double* r_dev, double sigma_diag);
+/* an error sample e = A^-1 r (device, FP64, 3*NP) on a loaded context with node coordinates;
+ * returns that context's sample count */
+int32_t hecmw33_net_add_sample(hecmw33_net* net, hecmw33_ctx* ctx, const double* e_dev, double tau);
 int32_t hecmw33_net_clear_samples(hecmw33_net* net);
-/* loss[0] = mean energy of the plain SSOR output, loss[1..epochs] = mean per epoch */
+/* loss[0] = mean logistic loss before training, loss[1..epochs] = mean per epoch */
 int32_t hecmw33_net_train(hecmw33_net* net, int32_t epochs, double lr, uint64_t seed, double* loss);
-int32_t hecmw33_net_score(hecmw33_net* net, double* mean, double* base);
+/* mean logistic loss and fraction of nodes classified as their label, over the current samples */
+int32_t hecmw33_net_score(hecmw33_net* net, double* mean, double* accuracy);
 int32_t hecmw33_ctx_set_net(hecmw33_ctx* ctx, hecmw33_net* net);
 int32_t hecmw33_set_net(hecmw33_net* net);
+/* The training labels alone, host only: nsamp error samples e (nsamp x 3N, xyz interleaved) of an
+ * N-node system with node coordinates xyz (3N); labels (N) receive 1 for the nodes of every edge
+ * whose axial strain stays below tau times its sample's median in every sample, else 0.  Returns
+ * the number of nodes labelled 1. */
+int32_t hecmw33_rigid_labels(int32_t N, const int32_t* indexL, const int32_t* itemL, const int32_t* indexU,
+                             const int32_t* itemU, const double* xyz, int32_t nsamp, const double* e, double tau,
+                             float* labels);
 
 /* ---- synthetic systems (fistr1's assembly for a structured HEX8 cube) ------- */
 int32_t hecmw33_gen_hex8_count(int32_t n, int32_t* N, int32_t* NPL, int32_t* NPU);

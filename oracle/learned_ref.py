@@ -1,5 +1,6 @@
-"""CPU restatement of the learned correction's network (test infrastructure
-only: tests and the benchmark's checks import it, the product never does).
+"""CPU restatement of the learned aggregation's network and its training
+labels (test infrastructure only: tests and the benchmark's checks import it,
+the product never does).
 
 The network of frontistr_amd/csrc/learned.hip: per node, features x (16) ->
 H1 = relu(x W1) (64) -> H2 = relu(H1 W2) (64) -> y = H2 W3[:, :3]; no biases.
@@ -11,8 +12,10 @@ so they differ from the GPU only by the FP32 accumulation order (and the rare
 BF16 rounding flip that order can cause).
 
 `forward64`/`backward64` are the same formulas without rounding, pinned by
-finite differences in tests/test_learned_oracle.py.  There is no reference
-counterpart: FrontISTR has no learned preconditioner.
+finite differences in tests/test_learned_oracle.py.  `logistic` is the
+training loss of output 0 against labels, `rigid_labels` the labels
+(frontistr_amd/csrc/solver.hip, rigid_edge_ratios / rigid_labels_from_ratios).
+There is no reference counterpart: FrontISTR has no learned preconditioner.
 """
 import numpy as np
 
@@ -73,6 +76,48 @@ def backward64(dY, cache):
     return X.T @ dH1, H1.T @ dH2, H2.T @ dYp
 
 
-def energy(z, Az, r, z0):
-    """The training functional (z.Az - 2 z.r) / (r.z0)."""
-    return (z @ Az - 2.0 * (z @ r)) / (r @ z0)
+def logistic(Y, t):
+    """Mean logistic loss of Y[:, 0] against labels t and its gradient dL/dY (n x 3)."""
+    y = np.asarray(Y, np.float64)[:, 0]
+    t = np.asarray(t, np.float64)
+    n = y.size
+    loss = np.mean(np.logaddexp(0.0, y) - t * y)
+    dY = np.zeros((n, 3))
+    dY[:, 0] = (1.0 / (1.0 + np.exp(-y)) - t) / n
+    return loss, dY
+
+
+def rigid_labels(N, indexL, itemL, indexU, itemU, X, errors, tau):
+    """Labels of the nodes of every stored edge (i, j), both internal, whose
+    axial strain |(e_j - e_i).(x_j - x_i)| / |x_j - x_i|^2, divided by the
+    median over the sample's edges, stays below tau in every error sample;
+    edges whose e is zero at both ends are left out (median included)."""
+    X = np.asarray(X, np.float64).reshape(-1, 3)
+    rows, cols = [], []
+    for idx, item in ((indexL, itemL), (indexU, itemU)):
+        cnt = np.diff(np.asarray(idx[:N + 1]))
+        rows.append(np.repeat(np.arange(N), cnt))
+        cols.append(np.asarray(item[:idx[N]]) - 1)
+    rows, cols = np.concatenate(rows), np.concatenate(cols)
+    internal = cols < N
+    worst = np.zeros(rows.size)
+    for e in np.asarray(errors, np.float64).reshape(-1, 3 * N):
+        e = e.reshape(-1, 3)
+        ratio = np.full(rows.size, np.inf)
+        ii, jj = rows[internal], cols[internal]
+        d = X[jj] - X[ii]
+        de = e[jj] - e[ii]
+        mag = np.abs(e[jj]).sum(1) + np.abs(e[ii]).sum(1)
+        den = (d * d).sum(1)
+        live = (mag > 0) & (den > 0)
+        eps = np.abs((de * d).sum(1)) / np.where(den > 0, den, 1.0)
+        med = np.sort(eps[live])[live.sum() // 2] if live.any() else 0.0
+        r = np.where(eps > 0, np.inf, 0.0) if med == 0 else eps / med
+        sub = np.where(live, r, np.inf)
+        ratio[np.nonzero(internal)[0]] = sub
+        worst = np.maximum(worst, ratio)
+    lab = np.zeros(N, np.float32)
+    small = worst < tau
+    lab[rows[small]] = 1
+    lab[cols[small]] = 1
+    return lab

@@ -1,16 +1,16 @@
-"""The learned correction on the GPU.
+"""The learned aggregation on the GPU.
 
 * forward and backward BF16 MFMA kernels against the CPU restatement
-  (oracle/learned_ref.py: same BF16 rounding points, FP64 accumulation), on a
-  node count that is not a multiple of the 16/32-node tiles;
-* training on residuals the solver harvests from SSOR-CG runs lowers the
-  energy functional of z = z0 + b J r below the plain SSOR output's,
-  also on a held-out system;
-* the learned flexible CG reaches an FP64 true residual of 1e-8 with the
-  solution within 1e-6 of the reference's FP64 SSOR-CG solution; the test
-  reports its iterations beside the plain flexible CG's and checks the
-  fallback: a correction that loses the start-of-solve probe is dropped, so
-  the learned solve is never much worse than the plain one.
+  (oracle/learned_ref.py: same BF16 rounding points, FP64 accumulation), on
+  node counts that are not multiples of the 16/32-node tiles; the differences
+  are those of the FP32 accumulation order (and the BF16 roundings it flips);
+* training on labels built from the solver's own errors (SSOR-CG iterates on
+  seeded two-phase cubes) lowers the loss, and on a held-out cube the network
+  classifies the nodes as their labels;
+* on a held-out cube the learned two-level flexible CG takes fewer iterations
+  than the same solve on the cells alone, keeps its learned step, reaches an
+  FP64 true residual of 1e-8 and the reference's FP64 SSOR-CG solution within
+  1e-6.
 """
 import numpy as np
 import pytest
@@ -46,67 +46,121 @@ def test_net_forward_backward_match_oracle(n):
     Y, dW = net.eval(X, dY)
     Yo, cache = L.forward(X, *W)
     gW = L.backward(dY, cache)
-    assert np.linalg.norm(Y - Yo) <= 1e-2 * np.linalg.norm(Yo) + 1e-6, np.linalg.norm(Y - Yo) / np.linalg.norm(Yo)
+    # FP32 accumulation in another order than the oracle's FP64: most entries
+    # agree to ~1e-6; an intermediate whose BF16 rounding flips moves its
+    # entries by up to 2^-8 relative, which bounds the norms
+    rel = np.abs(Y - Yo) / (np.abs(Yo) + 1e-3 * np.abs(Yo).max())
+    assert np.median(rel) <= 1e-5, np.median(rel)
+    assert np.linalg.norm(Y - Yo) <= 2e-3 * np.linalg.norm(Yo), np.linalg.norm(Y - Yo) / np.linalg.norm(Yo)
     for name, g, go in zip(("W1", "W2", "W3"), dW, gW):
         err = np.linalg.norm(g - go) / (np.linalg.norm(go) + 1e-12)
-        assert err <= 2e-2, (name, err)
+        assert err <= 5e-3, (name, err)
     assert np.all(dW[2][:, 3:] == 0)
     net.close()
 
 
-def _variant(n, seed):
-    m = fa.gen_hex8(n, kind="inclusions", seed=seed, E=1000.0, contrast=1e4, rmin=1, rmax=3, vfrac=0.15)
-    m.set_iter(5000)
+def _cube(n, seed, cells=8):
+    """two-phase cube with the 120^3 workload's inclusion statistics (E contrast
+    1e4, 15% volume, radii 2-6 elements); cells of 12 elements (Iarray 43)"""
+    m = fa.gen_hex8(n, kind="inclusions", seed=seed, E=1000.0, contrast=1e4, rmin=2, rmax=6, vfrac=0.15)
+    m.set_iter(20000)
     m.set_precond(1)
+    m.set_coarse(cells)
     return m
 
 
-def test_training_lowers_energy_and_learned_fcg_converges():
+def _context(m, n, torch):
+    ctx = fa.DeviceContext()
+    ctx.set_coords(fa.hex8_coords(n))
+    ctx.load(m)
+    return ctx, torch.tensor(m.B, device="cuda:0")
+
+
+def test_training_learns_the_labels():
     torch = pytest.importorskip("torch")
-    dev = torch.device("cuda:0")
     net = fa.LearnedNet(seed=3)
     keep = []
-    for seed in (1, 2, 3):
-        m = _variant(12, seed)
-        ctx = fa.DeviceContext()
-        ctx.load(m)
-        b = torch.tensor(m.B, device=dev)
-        fa.harvest_residuals(net, ctx, m, b)
+    for seed in (11, 12, 13):
+        m = _cube(20, seed)
+        ctx, b = _context(m, 20, torch)
+        assert fa.harvest_errors(net, ctx, m, b) == 2
         keep.append((ctx, m, b))
-    loss = net.train(epochs=6, lr=2e-3, seed=0)
-    print("energy: plain SSOR %.4f, per epoch %s" % (loss[0], np.round(loss[1:], 4)))
-    assert loss[-1] < loss[0]
-    # held-out variant
-    m = _variant(12, 9)
-    ctx = fa.DeviceContext()
-    ctx.load(m)
-    b = torch.tensor(m.B, device=dev)
+    loss = net.train(epochs=60, lr=3e-3, seed=0)
+    print("logistic loss: before %.4f, after %.4f" % (loss[0], loss[-1]))
+    assert loss[-1] < 0.5 * loss[0]
+    # held out
     net.clear_samples()
-    fa.harvest_residuals(net, ctx, m, b)
-    score, base = net.score()
-    print("held-out energy: plain SSOR %.4f, learned %.4f" % (base, score))
-    assert score < base
-    # learned flexible CG on the held-out system vs the reference FP64 SSOR-CG
+    m = _cube(20, 17)
+    ctx, b = _context(m, 20, torch)
+    fa.harvest_errors(net, ctx, m, b)
+    l, acc = net.score()
+    print("held-out: loss %.4f, accuracy %.4f" % (l, acc))
+    assert acc >= 0.95
+    net.close()
+
+
+def test_learned_aggregation_gains_on_a_held_out_cube():
+    torch = pytest.importorskip("torch")
+    net = fa.LearnedNet(seed=5)
+    keep = []
+    for seed in (21, 22, 23):
+        m = _cube(24, seed)
+        ctx, b = _context(m, 24, torch)
+        fa.harvest_errors(net, ctx, m, b)
+        keep.append(ctx)
+    net.train(epochs=60, lr=3e-3, seed=1)
+    net.clear_samples()
+    m = _cube(24, 7)
+    ctx, b = _context(m, 24, torch)
     ctx.set_net(net)
-    its = {}
-    for opt in (1, 2):
+    x_ref, info = ref.solve_krylov(m, nthreads=2)
+    res = {}
+    for name, opt in (("cells", 1), ("learned", 2)):
         mm = m.copy()
         mm.set_i(mm.I_SOLVER_OPT, opt)
         x = torch.zeros_like(b)
         torch.cuda.synchronize()
-        res = ctx.solve_device(b.data_ptr(), x.data_ptr(), mm.Iarray, mm.Rarray)
-        assert res.error == 0 and res.resid <= 1e-8
+        r = ctx.solve_device(b.data_ptr(), x.data_ptr(), mm.Iarray, mm.Rarray)
         xs = x.cpu().numpy()
+        inf = ctx.info()
+        res[name] = (r.iter, inf)
+        assert r.error == 0
         assert ref.rel_resid(m, xs) <= 1e-8 * 1.0001
-        xr, info = ref.solve_krylov(m, nthreads=2)
-        assert np.linalg.norm(xs - xr) / np.linalg.norm(xr) <= 1e-6
-        its[opt] = res.iter
-        if opt == 2:
-            inf = ctx.info()
-            print("learned FCG: %d iterations (plain flexible CG %d, reference SSOR-CG %d), fallbacks %d, off %d, "
-                  "probe error reduction with/without %.3f" % (res.iter, its[1], info["iter"], inf["nfallback"],
-                                                                inf["learned_off"], inf["probe_ratio"]))
-            # plain SSOR stays the fallback: the learned solve is never much worse than the plain one
-            assert res.iter <= 1.1 * its[1] + 10
+        assert np.linalg.norm(xs - x_ref) <= 1e-6 * np.linalg.norm(x_ref)
+    it_c, inf_c = res["cells"]
+    it_l, inf_l = res["learned"]
+    print("held-out 24^3: cells %d iterations (%d aggregates), learned %d iterations (%d aggregates, %d pieces); "
+          "reference SSOR-CG %d" % (it_c, inf_c["coarse"], it_l, inf_l["coarse"], inf_l["pieces"], info["iter"]))
+    assert inf_l["pieces"] > 0 and inf_l["learned_off"] == 0 and inf_l["fallback_iter"] == 0
+    assert it_l < 0.8 * it_c
     ctx.set_net(None)
+    net.close()
+
+
+def test_learned_solve_needs_a_network():
+    torch = pytest.importorskip("torch")
+    m = _cube(8, 1, cells=1)
+    ctx, b = _context(m, 8, torch)
+    mm = m.copy()
+    mm.set_i(mm.I_SOLVER_OPT, 2)
+    x = torch.zeros_like(b)
+    with pytest.raises(fa.HecmwError):
+        ctx.solve_device(b.data_ptr(), x.data_ptr(), mm.Iarray, mm.Rarray)
+    ctx.close()
+
+
+def test_score_is_the_loss_training_starts_from():
+    """hecmw33_net_score and the first entry of hecmw33_net_train both evaluate
+    the mean logistic loss of the current weights on the samples; with a zero
+    learning rate the weights stay, so the two agree."""
+    torch = pytest.importorskip("torch")
+    net = fa.LearnedNet(seed=9)
+    m = _cube(12, 3, cells=1)
+    ctx, b = _context(m, 12, torch)
+    fa.harvest_errors(net, ctx, m, b)
+    l0, acc0 = net.score()
+    assert np.isfinite(l0) and l0 > 0.0 and 0.0 <= acc0 <= 1.0
+    loss = net.train(epochs=1, lr=0.0, seed=0)
+    assert abs(loss[0] - l0) <= 1e-9 * l0
+    assert abs(net.score()[0] - l0) <= 1e-9 * l0
     net.close()

@@ -3,6 +3,7 @@ against known bit patterns, the gradient formulas against central finite
 differences (a dropped ReLU mask, a transposed weight or a missing layer
 fails), and a planted mistake that the check detects."""
 import numpy as np
+import pytest
 
 from oracle import learned_ref as L
 
@@ -74,3 +75,113 @@ def test_rounded_version_tracks_exact():
     for g, g64 in zip(L.backward(dY, c), L.backward64(dY, c64)):
         # BF16 rounding of the pre-activations flips a few ReLU masks
         assert np.linalg.norm(g - g64) <= 0.1 * np.linalg.norm(g64)
+
+
+# ---- training labels and loss of the learned aggregation ------------------------------
+def _region_sample(m, X, centre, radius, omega, stretch):
+    """e = omega x (x - centre) inside the ball (a rigid rotation), a uniform
+    stretch diag(stretch) x outside: edges with both ends inside have zero
+    axial strain, the others a strain of the order of the stretch."""
+    inside = np.linalg.norm(X - centre, axis=1) < radius
+    e = X * np.asarray(stretch)
+    e[inside] = np.cross(omega, X[inside] - centre)
+    return e.ravel(), inside
+
+
+def _expected(m, inside_sets):
+    """(must, may): the nodes of the edges whose ends both lie inside the region
+    of every sample, which must be labelled; the nodes with a neighbour (or
+    themselves) inside, the only ones that may be (an edge that crosses the
+    region's boundary has an arbitrary strain)."""
+    must = np.zeros(m.N, bool)
+    near = np.zeros(m.N, bool)
+    both = np.logical_and.reduce(inside_sets)
+    for idx, item in ((m.indexL, m.itemL), (m.indexU, m.itemU)):
+        for i in range(m.N):
+            for q in range(idx[i], idx[i + 1]):
+                j = item[q] - 1
+                if j >= m.N:
+                    continue
+                if both[i] and both[j]:
+                    must[i] = must[j] = True
+                if both[i] or both[j]:
+                    near[i] = near[j] = True
+    return must, near
+
+
+def _check(lab, must, near):
+    lab = lab > 0.5
+    assert np.all(lab[must]), "an edge inside the rigid region is not labelled"
+    assert not np.any(lab[~near]), "a node away from the rigid region is labelled"
+
+
+def test_rigid_labels_mark_the_rigid_region():
+    import frontistr_amd as fa
+    m = fa.gen_hex8_structure(8)
+    X = fa.hex8_coords(8).reshape(-1, 3)
+    e, inside = _region_sample(m, X, np.array([0.45, 0.5, 0.55]), 0.3, [0.3, -0.2, 0.5], [1.0, 2.0, 3.0])
+    must, near = _expected(m, [inside])
+    assert must.sum() > 20 and (~near).sum() > 300
+    got_ref = L.rigid_labels(m.N, m.indexL, m.itemL, m.indexU, m.itemU, X, e, 0.1)
+    got_lib = fa.rigid_labels(m, X, e[None, :], 0.1)
+    assert np.array_equal(got_lib, got_ref)
+    _check(got_lib, must, near)
+
+
+def test_rigid_labels_need_every_sample_and_skip_fixed_nodes():
+    import frontistr_amd as fa
+    m = fa.gen_hex8_structure(8)
+    X = fa.hex8_coords(8).reshape(-1, 3)
+    e1, in1 = _region_sample(m, X, np.array([0.4, 0.5, 0.5]), 0.32, [0.3, -0.2, 0.5], [1.0, 2.0, 3.0])
+    e2, in2 = _region_sample(m, X, np.array([0.6, 0.5, 0.5]), 0.32, [-0.1, 0.4, 0.2], [3.0, 1.0, 2.0])
+    # a fixed bottom face: e = 0 there in both samples, so its edges say nothing
+    bottom = X[:, 2] == 0.0
+    for e in (e1, e2):
+        e.reshape(-1, 3)[bottom] = 0.0
+    must, near = _expected(m, [in1, in2])
+    got_lib = fa.rigid_labels(m, X, np.stack([e1, e2]), 0.1)
+    got_ref = L.rigid_labels(m.N, m.indexL, m.itemL, m.indexU, m.itemU, X, np.stack([e1, e2]), 0.1)
+    assert np.array_equal(got_lib, got_ref)
+    _check(got_lib, must, near)
+    # the second sample removes what only the first region holds
+    only1 = _expected(m, [in1])[0] & ~_expected(m, [in1, in2])[1]
+    assert only1.sum() > 0 and not np.any(got_lib[only1] > 0.5)
+    assert not np.any(got_lib[bottom & ~near] > 0.5)
+
+
+def test_rigid_labels_planted_mistake_is_caught():
+    """Taking the whole relative motion |e_j - e_i| instead of its component
+    along the edge marks nothing inside a rotating piece: the region check fails."""
+    import frontistr_amd as fa
+    m = fa.gen_hex8_structure(8)
+    X = fa.hex8_coords(8).reshape(-1, 3)
+    e, inside = _region_sample(m, X, np.array([0.45, 0.5, 0.55]), 0.3, [0.3, -0.2, 0.5], [1.0, 2.0, 3.0])
+    must, near = _expected(m, [inside])
+    E = e.reshape(-1, 3)
+    rows = np.repeat(np.arange(m.N), np.diff(m.indexU))
+    cols = m.itemU - 1
+    d = X[cols] - X[rows]
+    wrong = np.linalg.norm(E[cols] - E[rows], axis=1) / np.linalg.norm(d, axis=1)
+    lab = np.zeros(m.N, np.float32)
+    small = wrong / np.median(wrong) < 0.1
+    lab[rows[small]] = 1
+    lab[cols[small]] = 1
+    with pytest.raises(AssertionError):
+        _check(lab, must, near)
+
+
+def test_logistic_gradient_matches_finite_differences():
+    rng = np.random.default_rng(3)
+    Y = rng.standard_normal((9, 3))
+    t = (rng.random(9) > 0.5).astype(np.float64)
+    loss, dY = L.logistic(Y, t)
+    h = 1e-6
+    for i in range(9):
+        Yp, Ym = Y.copy(), Y.copy()
+        Yp[i, 0] += h
+        Ym[i, 0] -= h
+        fd = (L.logistic(Yp, t)[0] - L.logistic(Ym, t)[0]) / (2 * h)
+        assert abs(fd - dY[i, 0]) <= 1e-7
+    assert np.all(dY[:, 1:] == 0)
+    # closed form at y = 0: log 2 whatever the labels
+    assert abs(L.logistic(np.zeros((4, 3)), np.array([0, 1, 0, 1]))[0] - np.log(2.0)) <= 1e-15
