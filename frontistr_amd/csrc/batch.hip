@@ -28,11 +28,18 @@
 //               matrix (autotuner)
 #include <hip/hip_runtime.h>
 
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -71,9 +78,85 @@ struct Tiles {
   int32_t ntiles;
 };
 
-// Tiles keep the hardware's round-robin deal over the XCDs: once small
-// systems finish, the live tiles stay spread over every XCD (a contiguous
-// per-XCD remap measured 27% slower on the 48-system workload for that reason).
+// ---- persistent kernels over live-tile work queues -----------------------------------
+// The per-tile kernels of an iteration run persistently: a grid of resident
+// workgroups walks a list of the live tiles (those of the systems still
+// running, or of those whose residuals are being replaced), rebuilt on the
+// device after every tail, instead of one workgroup per tile of the whole
+// batch, so finished systems cost nothing.  The list is dealt without atomics
+// (a ticket counter per XCD measured 13% slower on the 48-system workload: its
+// atomics serialise across the XCDs): runs of kQRun consecutive list entries
+// go round-robin to the XCDs (workgroup b runs on XCD b % kNumXcd, the
+// hardware's deal), so an XCD's workgroups share neighbouring rows whose x
+// stays in its L2 while every system's tiles spread over all XCDs, and the
+// workgroups of one XCD take that XCD's entries in turn.  Which workgroup
+// computes a tile does not matter: every tile writes its own partials in a
+// fixed order.
+enum QList : int { kqActive = 0, kqRel = 1, kqLists = 2 };
+constexpr int kQRun = 16;
+
+struct Queue {
+  int32_t* tile;  // [kqLists][ntiles] live tile ids, system by system
+  int32_t* n;     // [kqLists] live tiles per list
+};
+
+// what one launch walks (tile == nullptr: one tile per workgroup over every
+// tile, the layout of the set-up launches)
+struct QView {
+  const int32_t* tile;
+  const int32_t* n;
+};
+
+__device__ __forceinline__ int spmm_tile();
+
+template <class F>
+__device__ __forceinline__ void walk_tiles(const QView& q, F&& body) {
+  if (!q.tile) {
+    body(spmm_tile());
+    return;
+  }
+  const int n = *q.n, G = gridDim.x, b = blockIdx.x;
+  if (G < kNumXcd) {
+    for (int i = b; i < n; i += G) body(q.tile[i]);
+    return;
+  }
+  // XCD x holds list entries ((j / kQRun) * kNumXcd + x) * kQRun + j % kQRun,
+  // j = 0, 1, ...; its Gx workgroups b = x, x + 8, ... take j = b / 8 + k Gx
+  const int x = b % kNumXcd, gx = (G - x + kNumXcd - 1) / kNumXcd;
+  for (int j = b / kNumXcd;; j += gx) {
+    // entries only grow along a workgroup's walk: the first one past the list ends it
+    const int64_t i = ((int64_t)(j / kQRun) * kNumXcd + x) * kQRun + j % kQRun;
+    if (i >= n) break;
+    body(q.tile[i]);
+  }
+}
+
+// Rebuild list LIST from the systems' flags (active: sys_active != 0; rel:
+// sys_rel != 0) and, for the active list, publish the live instance count the
+// host polls.
+template <int LIST>
+__global__ __launch_bounds__(1024) void kb_queue(Tiles T, int32_t nsys, BState S, Queue Q) {
+  int32_t* out = Q.tile + (int64_t)LIST * T.ntiles;
+  int32_t off = 0, live_inst = 0;
+  for (int s = 0; s < nsys; ++s) {
+    const int32_t a = S.sys_active[s];
+    const bool sel = LIST == kqActive ? a != 0 : S.sys_rel[s] != 0;
+    live_inst += a;
+    if (!sel) continue;
+    const int32_t t0 = T.sys_tile0[s], nt = T.sys_ntile[s];
+    for (int i = threadIdx.x; i < nt; i += blockDim.x) out[off + i] = t0 + i;
+    off += nt;
+  }
+  if (threadIdx.x == 0) {
+    Q.n[LIST] = off;
+    if (LIST == kqActive) *S.active_total = live_inst;
+  }
+}
+
+// The set-up launches (one workgroup per tile) keep the hardware's round-robin
+// deal over the XCDs: a contiguous per-XCD remap of every tile measured 27%
+// slower on the 48-system workload (load imbalance across systems of very
+// different sizes).
 
 // Partials of ND inner products per right-hand side: thread tid holds the
 // values of rhs k = tid % 16 (rows tid / 16, ...); fixed-order sum over the
@@ -113,94 +196,98 @@ __device__ __forceinline__ void kb_jacobi(const float* __restrict__ alu, int64_t
 
 // guard 0: every active system; 2: only systems whose residuals were just
 // replaced (their z is recomputed from the new r)
-__global__ __launch_bounds__(kBT) void kb_diag(Tiles T, const float* __restrict__ alu, const float* __restrict__ r,
-                                               float* __restrict__ z, const float* __restrict__ q, BState S,
-                                               double* part, int guard) {
+__global__ __launch_bounds__(kBT) void kb_diag(Tiles T, QView Qv, const float* __restrict__ alu,
+                                               const float* __restrict__ r, float* __restrict__ z,
+                                               const float* __restrict__ q, BState S, double* part, int guard) {
+  walk_tiles(Qv, [&](int t) {
 #pragma clang fp contract(off)
-  const int t = blockIdx.x;
-  const int s = T.sys[t];
-  if (S.sys_active[s] == 0) return;
-  if (guard == 2 && S.sys_rel[s] == 0) return;
-  const int k = threadIdx.x % KS, rl = threadIdx.x / KS;
-  const bool live = !S.done[s * KS + k];
-  double acc[2] = {0.0, 0.0};
-  const int r0 = T.row0[t], nr = T.nrows[t];
-  for (int i = rl; i < nr && live; i += kBT / KS) {
-    const int64_t g = r0 + i;
-    const int64_t o = (g * KS + k) * 3;
-    const float r0v = r[o], r1v = r[o + 1], r2v = r[o + 2];
-    float x0, x1, x2;
-    kb_jacobi(alu, g, r0v, r1v, r2v, x0, x1, x2);
-    z[o] = x0;
-    z[o + 1] = x1;
-    z[o + 2] = x2;
-    acc[0] += (double)r0v * x0 + (double)r1v * x1 + (double)r2v * x2;
-    acc[1] += (double)q[o] * x0 + (double)q[o + 1] * x1 + (double)q[o + 2] * x2;
-  }
-  tile_partials_rowlanes<2>(acc, part, T.ntiles, t);
+    const int s = T.sys[t];
+    if (S.sys_active[s] == 0) return;
+    if (guard == 2 && S.sys_rel[s] == 0) return;
+    const int k = threadIdx.x % KS, rl = threadIdx.x / KS;
+    const bool live = !S.done[s * KS + k];
+    double acc[2] = {0.0, 0.0};
+    const int r0 = T.row0[t], nr = T.nrows[t];
+    for (int i = rl; i < nr && live; i += kBT / KS) {
+      const int64_t g = r0 + i;
+      const int64_t o = (g * KS + k) * 3;
+      const float r0v = r[o], r1v = r[o + 1], r2v = r[o + 2];
+      float x0, x1, x2;
+      kb_jacobi(alu, g, r0v, r1v, r2v, x0, x1, x2);
+      z[o] = x0;
+      z[o + 1] = x1;
+      z[o + 2] = x2;
+      acc[0] += (double)r0v * x0 + (double)r1v * x1 + (double)r2v * x2;
+      acc[1] += (double)q[o] * x0 + (double)q[o + 1] * x1 + (double)q[o + 2] * x2;
+    }
+    tile_partials_rowlanes<2>(acc, part, T.ntiles, t);
+  });
 }
 
 // ---- p = z (first iteration of an instance) or z + beta p --------------------------
-__global__ __launch_bounds__(kBT) void kb_p(Tiles T, const float* __restrict__ z, float* __restrict__ p, BState S) {
-  const int t = blockIdx.x;
-  const int s = T.sys[t];
-  if (S.sys_active[s] == 0) return;
-  const int k = threadIdx.x % KS, rl = threadIdx.x / KS;
-  const int id = s * KS + k;
-  if (S.done[id]) return;
-  const bool first = S.iter[id] == 1;
-  const double beta = S.beta[id];
-  const int r0 = T.row0[t], nr = T.nrows[t];
-  for (int i = rl; i < nr; i += kBT / KS) {
-    const int64_t o = ((int64_t)(r0 + i) * KS + k) * 3;
+__global__ __launch_bounds__(kBT) void kb_p(Tiles T, QView Qv, const float* __restrict__ z, float* __restrict__ p,
+                                            BState S) {
+  walk_tiles(Qv, [&](int t) {
+    const int s = T.sys[t];
+    if (S.sys_active[s] == 0) return;
+    const int k = threadIdx.x % KS, rl = threadIdx.x / KS;
+    const int id = s * KS + k;
+    if (S.done[id]) return;
+    const bool first = S.iter[id] == 1;
+    const double beta = S.beta[id];
+    const int r0 = T.row0[t], nr = T.nrows[t];
+    for (int i = rl; i < nr; i += kBT / KS) {
+      const int64_t o = ((int64_t)(r0 + i) * KS + k) * 3;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) p[o + d] = first ? z[o + d] : (float)((double)z[o + d] + beta * (double)p[o + d]);
-  }
+      for (int d = 0; d < 3; ++d) p[o + d] = first ? z[o + d] : (float)((double)z[o + d] + beta * (double)p[o + d]);
+    }
+  });
 }
 
 // ---- x += alpha p (FP64), r = r - alpha q (FP32), r.r; then the next
 //      iteration's preconditioner on the new r while it is in registers:
 //      z = D^-1 r with r.z and z.q (q of this iteration, the next beta's
 //      q_prev), the values and partials kb_diag would give
-__global__ __launch_bounds__(kBT) void kb_update_diag(Tiles T, const float* __restrict__ alu,
+__global__ __launch_bounds__(kBT) void kb_update_diag(Tiles T, QView Qv, const float* __restrict__ alu,
                                                       const float* __restrict__ p, const float* __restrict__ q,
                                                       double* __restrict__ x, float* __restrict__ r,
                                                       float* __restrict__ z, BState S, double* part_rr,
                                                       double* part_rz) {
-  const int t = blockIdx.x;
-  const int s = T.sys[t];
-  if (S.sys_active[s] == 0) return;
-  const int k = threadIdx.x % KS, rl = threadIdx.x / KS;
-  const int id = s * KS + k;
-  const bool live = !S.done[id];
-  const double alpha = S.alpha[id];
-  double acc[1] = {0.0}, acc2[2] = {0.0, 0.0};
-  const int r0 = T.row0[t], nr = T.nrows[t];
-  for (int i = rl; i < nr && live; i += kBT / KS) {
-    const int64_t g = r0 + i;
-    const int64_t o = (g * KS + k) * 3;
-    float v[3], qv[3];
+  walk_tiles(Qv, [&](int t) {
+    const int s = T.sys[t];
+    if (S.sys_active[s] == 0) return;
+    const int k = threadIdx.x % KS, rl = threadIdx.x / KS;
+    const int id = s * KS + k;
+    const bool live = !S.done[id];
+    const double alpha = S.alpha[id];
+    double acc[1] = {0.0}, acc2[2] = {0.0, 0.0};
+    const int r0 = T.row0[t], nr = T.nrows[t];
+    for (int i = rl; i < nr && live; i += kBT / KS) {
+      const int64_t g = r0 + i;
+      const int64_t o = (g * KS + k) * 3;
+      float v[3], qv[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      qv[d] = q[o + d];
-      x[o + d] = x[o + d] + alpha * (double)p[o + d];
-      v[d] = (float)((double)r[o + d] - alpha * (double)qv[d]);
-      r[o + d] = v[d];
-      acc[0] += (double)v[d] * (double)v[d];
-    }
-    float x0, x1, x2;
-    kb_jacobi(alu, g, v[0], v[1], v[2], x0, x1, x2);
-    z[o] = x0;
-    z[o + 1] = x1;
-    z[o + 2] = x2;
-    {
+      for (int d = 0; d < 3; ++d) {
+        qv[d] = q[o + d];
+        x[o + d] = x[o + d] + alpha * (double)p[o + d];
+        v[d] = (float)((double)r[o + d] - alpha * (double)qv[d]);
+        r[o + d] = v[d];
+        acc[0] += (double)v[d] * (double)v[d];
+      }
+      float x0, x1, x2;
+      kb_jacobi(alu, g, v[0], v[1], v[2], x0, x1, x2);
+      z[o] = x0;
+      z[o + 1] = x1;
+      z[o + 2] = x2;
+      {
 #pragma clang fp contract(off)
-      acc2[0] += (double)v[0] * x0 + (double)v[1] * x1 + (double)v[2] * x2;
-      acc2[1] += (double)qv[0] * x0 + (double)qv[1] * x1 + (double)qv[2] * x2;
+        acc2[0] += (double)v[0] * x0 + (double)v[1] * x1 + (double)v[2] * x2;
+        acc2[1] += (double)qv[0] * x0 + (double)qv[1] * x1 + (double)qv[2] * x2;
+      }
     }
-  }
-  tile_partials_rowlanes<1>(acc, part_rr, T.ntiles, t);
-  tile_partials_rowlanes<2>(acc2, part_rz, T.ntiles, t);
+    tile_partials_rowlanes<1>(acc, part_rr, T.ntiles, t);
+    tile_partials_rowlanes<2>(acc2, part_rz, T.ntiles, t);
+  });
 }
 
 // ---- SpMM ----------------------------------------------------------------------------
@@ -229,14 +316,13 @@ enum BEpi : int {
 };
 
 template <typename T, int LPR, int RPL, int U, int EPI>
-__global__ __launch_bounds__(kBT) void kb_spmm(Tiles TL, const int32_t* __restrict__ rowptr,
-                                               const int32_t* __restrict__ col, const T* __restrict__ val,
-                                               const T* __restrict__ x, const void* __restrict__ w,
-                                               void* __restrict__ y, BState S, double* part, int guard) {
+__device__ __forceinline__ void spmm_one_tile(int t, const Tiles& TL, const int32_t* __restrict__ rowptr,
+                                              const int32_t* __restrict__ col, const T* __restrict__ val,
+                                              const T* __restrict__ x, const void* __restrict__ w,
+                                              void* __restrict__ y, const BState& S, double* part, int guard) {
 #pragma clang fp contract(off)
   constexpr int G = KS / RPL, NPH = LPR / G, RPW = 64 / LPR;
   constexpr int ND = EPI == kbEpiQ ? 2 : (EPI == kbEpiRes ? 1 : 0);
-  const int t = spmm_tile();
   const int s = TL.sys[t];
   if (guard == 1 && S.sys_active[s] == 0) return;
   if (guard == 2 && S.sys_rel[s] == 0) return;
@@ -354,6 +440,14 @@ __global__ __launch_bounds__(kBT) void kb_spmm(Tiles TL, const int32_t* __restri
       part[(int64_t)(d * KS + k) * TL.ntiles + t] = sum;
     }
   }
+}
+
+template <typename T, int LPR, int RPL, int U, int EPI>
+__global__ __launch_bounds__(kBT) void kb_spmm(Tiles TL, QView Qv, const int32_t* __restrict__ rowptr,
+                                               const int32_t* __restrict__ col, const T* __restrict__ val,
+                                               const T* __restrict__ x, const void* __restrict__ w,
+                                               void* __restrict__ y, BState S, double* part, int guard) {
+  walk_tiles(Qv, [&](int t) { spmm_one_tile<T, LPR, RPL, U, EPI>(t, TL, rowptr, col, val, x, w, y, S, part, guard); });
 }
 
 // ---- per-system tails: sum a system's tile partials, then the CG scalar logic ---------
@@ -498,15 +592,7 @@ __global__ __launch_bounds__(kBT) void kb_tail(Tiles T, BState S, const double* 
   }
 }
 
-__global__ void kb_count(int32_t nsys, BState S) {
-  if (threadIdx.x == 0) {
-    int32_t n = 0;
-    for (int s = 0; s < nsys; ++s) n += S.sys_active[s];
-    *S.active_total = n;
-  }
-}
-
-// b.b partials per tile (initial phase)
+// b.b partials per tile (initial phase, one workgroup per tile)
 __global__ __launch_bounds__(kBT) void kb_bb(Tiles T, const double* __restrict__ b, double* part) {
   const int t = blockIdx.x;
   const int k = threadIdx.x % KS, rl = threadIdx.x / KS;
@@ -529,25 +615,39 @@ __global__ void kb_to_f32(int64_t n, const double* __restrict__ a, float* __rest
 }
 
 // ---- SpMM variant table ------------------------------------------------------------------
-using SpmmFn = void (*)(int grid, hipStream_t, Tiles, const int32_t*, const int32_t*, const void*, const void*,
-                        const void*, void*, BState, double*, int);
+using SpmmFn = void (*)(int grid, hipStream_t, Tiles, QView, const int32_t*, const int32_t*, const void*,
+                        const void*, const void*, void*, BState, double*, int);
+using OccFn = int (*)();
 
 template <typename T, int LPR, int RPL, int U, int EPI>
-void spmm_launch(int grid, hipStream_t st, Tiles tl, const int32_t* rp, const int32_t* col, const void* val,
+void spmm_launch(int grid, hipStream_t st, Tiles tl, QView qv, const int32_t* rp, const int32_t* col, const void* val,
                  const void* x, const void* w, void* y, BState S, double* part, int guard) {
-  kb_spmm<T, LPR, RPL, U, EPI><<<grid, kBT, 0, st>>>(tl, rp, col, static_cast<const T*>(val),
+  kb_spmm<T, LPR, RPL, U, EPI><<<grid, kBT, 0, st>>>(tl, qv, rp, col, static_cast<const T*>(val),
                                                      static_cast<const T*>(x), w, y, S, part, guard);
+}
+
+// resident workgroups of a kernel per CU (the persistent grid is that times the CU count)
+template <class K>
+int occupancy_of(K kernel) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, kBT, 0) != hipSuccess) nb = 1;
+  return nb < 1 ? 1 : nb;
+}
+template <typename T, int LPR, int RPL, int U, int EPI>
+int spmm_occupancy() {
+  return occupancy_of(kb_spmm<T, LPR, RPL, U, EPI>);
 }
 
 struct Variant {
   const char* name;
   int lpr, rpl, u;
   SpmmFn q32, st32;
+  OccFn occ;
 };
 #define HECMW33_VARIANT(L, R, U)                                                                        \
   Variant {                                                                                             \
     "lpr" #L "_rpl" #R "_u" #U, L, R, U, &spmm_launch<float, L, R, U, kbEpiQ>,                          \
-        &spmm_launch<float, L, R, U, kbEpiStore>                                                        \
+        &spmm_launch<float, L, R, U, kbEpiStore>, &spmm_occupancy<float, L, R, U, kbEpiQ>               \
   }
 const Variant kVariants[] = {
     HECMW33_VARIANT(16, 1, 1), HECMW33_VARIANT(16, 1, 2), HECMW33_VARIANT(8, 2, 1), HECMW33_VARIANT(8, 2, 2),
@@ -565,11 +665,12 @@ constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 struct Variant64 {
   const char* name;
   SpmmFn res, st;
+  OccFn occ;
 };
 #define HECMW33_VARIANT64(L, R, U)                                                                   \
   Variant64 {                                                                                        \
     "lpr" #L "_rpl" #R "_u" #U, &spmm_launch<double, L, R, U, kbEpiRes>,                             \
-        &spmm_launch<double, L, R, U, kbEpiStore>                                                    \
+        &spmm_launch<double, L, R, U, kbEpiStore>, &spmm_occupancy<double, L, R, U, kbEpiRes>        \
   }
 const Variant64 kVariants64[] = {
     HECMW33_VARIANT64(16, 1, 2), HECMW33_VARIANT64(16, 1, 1), HECMW33_VARIANT64(8, 2, 1), HECMW33_VARIANT64(8, 2, 2),
@@ -642,12 +743,17 @@ struct hecmw33_batch {
   hipGraphExec_t graph = nullptr;
   int32_t* h_active = nullptr;  // pinned, 2 slots
   double spmm_ms = 0.0;
+  // live-tile work queues and the persistent grids that walk them
+  BMem qtile, qn;
+  int ncu = 0;
+  int grid_p = 0, grid_q = 0, grid_upd = 0, grid_res = 0, grid_diag = 0;
+  int tune_cached = 0;  // 1: the SpMM shapes came from the autotuner's persisted winners
 
   ~hecmw33_batch() {
     if (stream) (void)hipStreamSynchronize(stream);
     if (graph) (void)hipGraphExecDestroy(graph);
     for (BMem* m : {&rowptr, &col, &val64, &val32, &alu32, &d64, &tile_row0, &tile_nrows, &tile_sys, &sys_tile0,
-                    &sys_ntile, &b, &x, &r, &z, &p, &q, &part, &state})
+                    &sys_ntile, &b, &x, &r, &z, &p, &q, &part, &state, &qtile, &qn})
       m->release();
     if (h_active) (void)hipHostFree(h_active);
     if (stream) (void)hipStreamDestroy(stream);
@@ -678,10 +784,21 @@ static int32_t batch_state_layout(hecmw33_batch& B) {
 
 static int grid_of(int64_t n) { return (int)std::min<int64_t>(std::max<int64_t>((n + kBT - 1) / kBT, 1), 8192); }
 
+static Queue queue_of(const hecmw33_batch& B) {
+  return Queue{B.qtile.as<int32_t>(), B.qn.as<int32_t>()};
+}
+
+// a launch that walks list `list`
+static QView qview(const hecmw33_batch& B, int list) {
+  return QView{B.qtile.as<int32_t>() + (int64_t)list * B.ntiles, B.qn.as<int32_t>() + list};
+}
+
+// one workgroup per tile of the whole batch (set-up launches)
+static QView qall() { return QView{nullptr, nullptr}; }
+
 // one FCG iteration for every live instance
 static void batch_enqueue_iteration(hecmw33_batch& B) {
   hipStream_t st = B.stream;
-  const int nt = B.ntiles;
   double* part = B.part.as<double>();
   float *r = B.r.as<float>(), *z = B.z.as<float>(), *p = B.p.as<float>(), *q = B.q.as<float>();
   // z = D^-1 r and its dots (r.z, z.q) come from the previous iteration's
@@ -689,37 +806,173 @@ static void batch_enqueue_iteration(hecmw33_batch& B) {
   // first iteration), in the second half of the partials
   double* part_rz = part + (size_t)2 * KS * B.ntiles;
   kb_tail<kbBeta, 2><<<B.nsys, kBT, 0, st>>>(B.T, B.S, part_rz);
-  kb_p<<<nt, kBT, 0, st>>>(B.T, z, p, B.S);
-  kVariants[B.variant].q32(nt, st, B.T, B.rowptr.as<int32_t>(), B.col.as<int32_t>(), B.val32.p, p, r, q, B.S, part, 1);
+  kb_p<<<B.grid_p, kBT, 0, st>>>(B.T, qview(B, kqActive), z, p, B.S);
+  kVariants[B.variant].q32(B.grid_q, st, B.T, qview(B, kqActive), B.rowptr.as<int32_t>(), B.col.as<int32_t>(),
+                           B.val32.p, p, r, q, B.S, part, 1);
   kb_tail<kbAlpha, 2><<<B.nsys, kBT, 0, st>>>(B.T, B.S, part);
-  kb_update_diag<<<nt, kBT, 0, st>>>(B.T, B.alu32.as<float>(), p, q, B.x.as<double>(), r, z, B.S, part, part_rz);
+  kb_update_diag<<<B.grid_upd, kBT, 0, st>>>(B.T, qview(B, kqActive), B.alu32.as<float>(), p, q,
+                                             B.x.as<double>(), r, z, B.S, part, part_rz);
   kb_tail<kbRes, 1><<<B.nsys, kBT, 0, st>>>(B.T, B.S, part);
-  kVariants64[B.variant64].res(nt, st, B.T, B.rowptr.as<int32_t>(), B.col.as<int32_t>(), B.val64.p, B.x.p, B.b.p, r,
-                               B.S, part, 2);
+  // the systems whose residuals are replaced now
+  kb_queue<kqRel><<<1, 1024, 0, st>>>(B.T, B.nsys, B.S, queue_of(B));
+  kVariants64[B.variant64].res(B.grid_res, st, B.T, qview(B, kqRel), B.rowptr.as<int32_t>(),
+                               B.col.as<int32_t>(), B.val64.p, B.x.p, B.b.p, r, B.S, part, 2);
   // systems whose residuals were just replaced: z from the new r
-  kb_diag<<<nt, kBT, 0, st>>>(B.T, B.alu32.as<float>(), r, z, q, B.S, part_rz, 2);
+  kb_diag<<<B.grid_diag, kBT, 0, st>>>(B.T, qview(B, kqRel), B.alu32.as<float>(), r, z, q, B.S, part_rz, 2);
   kb_tail<kbRel, 1><<<B.nsys, kBT, 0, st>>>(B.T, B.S, part);
+  // the systems still running after this iteration, and their live count for the host
+  kb_queue<kqActive><<<1, 1024, 0, st>>>(B.T, B.nsys, B.S, queue_of(B));
+}
+
+// ---- the autotuner's winners, persisted per (blocks per row, rows) bucket ------------------
+// Key: the device (architecture and CU count), the mean blocks per row rounded, and floor(log2(rows)).
+// Value: the names of the FP32 and FP64 SpMM shapes that won on a batch of that bucket.  Kept in a
+// text file, one "<arch> <cus> <blocks/row> <log2 rows> <fp32 shape> <fp64 shape>" line per bucket,
+// and in this process for that file: $HECMW33_TUNE_CACHE names the file ("off": nothing is kept and
+// every load times the shapes), else $XDG_CACHE_HOME/hecmw33/batch_tune.txt, else
+// $HOME/.cache/hecmw33/batch_tune.txt.  A line whose shape names this library does not have is
+// ignored (and re-timed).
+struct TuneKey {
+  std::string arch;
+  int cus, bpr, lrows;
+  std::string str() const {
+    return arch + " " + std::to_string(cus) + " " + std::to_string(bpr) + " " + std::to_string(lrows);
+  }
+};
+
+static std::mutex g_tune_mu;
+static std::map<std::string, std::pair<int, int>> g_tune_mem;
+
+static std::string tune_path() {
+  const char* e = getenv("HECMW33_TUNE_CACHE");
+  if (e && *e) return std::string(e) == "off" ? std::string() : std::string(e);
+  const char* x = getenv("XDG_CACHE_HOME");
+  if (x && *x) return std::string(x) + "/hecmw33/batch_tune.txt";
+  const char* h = getenv("HOME");
+  if (h && *h) return std::string(h) + "/.cache/hecmw33/batch_tune.txt";
+  return std::string();
+}
+
+static int variant_by_name(const std::string& n) {
+  for (int v = 0; v < kNumVariants; ++v)
+    if (n == kVariants[v].name) return v;
+  return -1;
+}
+static int variant64_by_name(const std::string& n) {
+  for (int v = 0; v < kNumVariants64; ++v)
+    if (n == kVariants64[v].name) return v;
+  return -1;
+}
+
+// every valid line of the file, key -> (fp32, fp64)
+static std::map<std::string, std::pair<int, int>> tune_read(const std::string& path) {
+  std::map<std::string, std::pair<int, int>> m;
+  if (path.empty()) return m;
+  std::ifstream f(path);
+  std::string line;
+  while (std::getline(f, line)) {
+    std::istringstream is(line);
+    std::string arch, n32, n64;
+    int cus, bpr, lrows;
+    if (!(is >> arch >> cus >> bpr >> lrows >> n32 >> n64)) continue;
+    const int v32 = variant_by_name(n32), v64 = variant64_by_name(n64);
+    if (v32 < 0 || v64 < 0) continue;
+    m[TuneKey{arch, cus, bpr, lrows}.str()] = {v32, v64};
+  }
+  return m;
+}
+
+static void tune_write(const std::string& path, const std::map<std::string, std::pair<int, int>>& m) {
+  if (path.empty()) return;
+  // create the missing directories, then replace the file through a rename
+  for (size_t i = 1; i < path.size(); ++i)
+    if (path[i] == '/') (void)::mkdir(path.substr(0, i).c_str(), 0755);
+  const std::string tmp = path + ".tmp" + std::to_string((long)::getpid());
+  {
+    std::ofstream f(tmp, std::ios::trunc);
+    if (!f) return;
+    for (const auto& kv : m)
+      f << kv.first << " " << kVariants[kv.second.first].name << " " << kVariants64[kv.second.second].name << "\n";
+    if (!f) return;
+  }
+  (void)::rename(tmp.c_str(), path.c_str());
+}
+
+static TuneKey tune_key(const hecmw33_batch& B) {
+  int dev = 0;
+  hipDeviceProp_t prop{};
+  std::string arch = "unknown";
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+    arch = prop.gcnArchName;
+    const size_t c = arch.find(':');
+    if (c != std::string::npos) arch = arch.substr(0, c);
+  }
+  const int bpr = (int)std::lround((double)B.nnzb / (double)std::max<int64_t>(1, B.nrows));
+  int lrows = 0;
+  while ((int64_t(2) << lrows) <= B.nrows) ++lrows;
+  return TuneKey{arch, B.ncu, bpr, lrows};
+}
+
+// persistent grids of the iteration's per-tile launches for the chosen shapes
+static void batch_grids(hecmw33_batch& B) {
+  auto grid = [&](int per_cu) {
+    const int64_t g = (int64_t)per_cu * std::max(1, B.ncu);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(g, B.ntiles));
+  };
+  B.grid_p = grid(occupancy_of(kb_p));
+  B.grid_upd = grid(occupancy_of(kb_update_diag));
+  B.grid_diag = grid(occupancy_of(kb_diag));
+  B.grid_q = grid(kVariants[B.variant].occ());
+  B.grid_res = grid(kVariants64[B.variant64].occ());
 }
 
 static int32_t batch_autotune(hecmw33_batch& B) {
-  // time every variant's FP32 SpMM with its fused dots on the loaded matrix
+  const TuneKey key = tune_key(B);
+  const std::string ks = key.str(), path = tune_path(), mk = path + "|" + ks;
+  if (!path.empty()) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_mem.find(mk);
+    if (it == g_tune_mem.end()) {
+      const auto disk = tune_read(path);
+      auto jt = disk.find(ks);
+      if (jt != disk.end()) it = g_tune_mem.emplace(mk, jt->second).first;
+    }
+    if (it != g_tune_mem.end()) {
+      B.variant = it->second.first;
+      B.variant64 = it->second.second;
+      B.tune_cached = 1;
+      std::fill(std::begin(B.variant_ms), std::end(B.variant_ms), 0.f);
+      std::fill(std::begin(B.variant64_ms), std::end(B.variant64_ms), 0.f);
+      B.spmm_ms = 0.0;
+      batch_grids(B);
+      return 0;
+    }
+  }
+  B.tune_cached = 0;
+  // time every shape's FP32 SpMM with its fused dots on the loaded matrix, as
+  // the iteration runs it: persistent, over the live-tile list of every system
+  // (the FP64 shapes over the list of residual replacements, also every system)
   hipEvent_t e0, e1;
   BHIPC(hipEventCreate(&e0));
   BHIPC(hipEventCreate(&e1));
   float best = 1e30f;
   B.variant = 0;
-  // all systems live for the timing
   std::vector<int32_t> ones(B.nsys, 1);
   BHIPC(hipMemcpyAsync(B.S.sys_active, ones.data(), sizeof(int32_t) * B.nsys, hipMemcpyHostToDevice, B.stream));
+  BHIPC(hipMemcpyAsync(B.S.sys_rel, ones.data(), sizeof(int32_t) * B.nsys, hipMemcpyHostToDevice, B.stream));
+  // both live-tile lists: every tile
+  kb_queue<kqActive><<<1, 1024, 0, B.stream>>>(B.T, B.nsys, B.S, queue_of(B));
+  kb_queue<kqRel><<<1, 1024, 0, B.stream>>>(B.T, B.nsys, B.S, queue_of(B));
   for (int v = 0; v < kNumVariants; ++v) {
-    for (int rep = 0; rep < 2; ++rep)
-      kVariants[v].q32(B.ntiles, B.stream, B.T, B.rowptr.as<int32_t>(), B.col.as<int32_t>(), B.val32.p, B.p.p,
-                       B.r.p, B.q.p, B.S, B.part.as<double>(), 0);
+    const int g = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)kVariants[v].occ() * B.ncu, B.ntiles));
+    auto one = [&]() {
+      kVariants[v].q32(g, B.stream, B.T, qview(B, kqActive), B.rowptr.as<int32_t>(), B.col.as<int32_t>(),
+                       B.val32.p, B.p.p, B.r.p, B.q.p, B.S, B.part.as<double>(), 1);
+    };
+    for (int rep = 0; rep < 2; ++rep) one();
     const int reps = 5;
     BHIPC(hipEventRecord(e0, B.stream));
-    for (int rep = 0; rep < reps; ++rep)
-      kVariants[v].q32(B.ntiles, B.stream, B.T, B.rowptr.as<int32_t>(), B.col.as<int32_t>(), B.val32.p, B.p.p,
-                       B.r.p, B.q.p, B.S, B.part.as<double>(), 0);
+    for (int rep = 0; rep < reps; ++rep) one();
     BHIPC(hipEventRecord(e1, B.stream));
     BHIPC(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -741,13 +994,15 @@ static int32_t batch_autotune(hecmw33_batch& B) {
   best = 1e30f;
   B.variant64 = 0;
   for (int v = 0; v < kNumVariants64; ++v) {
-    kVariants64[v].res(B.ntiles, B.stream, B.T, B.rowptr.as<int32_t>(), B.col.as<int32_t>(), B.val64.p, B.x.p,
-                       B.b.p, B.r.p, B.S, B.part.as<double>(), 0);
+    const int g = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)kVariants64[v].occ() * B.ncu, B.ntiles));
+    auto one = [&]() {
+      kVariants64[v].res(g, B.stream, B.T, qview(B, kqRel), B.rowptr.as<int32_t>(), B.col.as<int32_t>(),
+                         B.val64.p, B.x.p, B.b.p, B.r.p, B.S, B.part.as<double>(), 2);
+    };
+    one();
     const int reps = 3;
     BHIPC(hipEventRecord(e0, B.stream));
-    for (int rep = 0; rep < reps; ++rep)
-      kVariants64[v].res(B.ntiles, B.stream, B.T, B.rowptr.as<int32_t>(), B.col.as<int32_t>(), B.val64.p, B.x.p,
-                         B.b.p, B.r.p, B.S, B.part.as<double>(), 0);
+    for (int rep = 0; rep < reps; ++rep) one();
     BHIPC(hipEventRecord(e1, B.stream));
     BHIPC(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -762,6 +1017,14 @@ static int32_t batch_autotune(hecmw33_batch& B) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   BHIPC(hipGetLastError());
+  batch_grids(B);
+  if (!path.empty()) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_mem[mk] = {B.variant, B.variant64};
+    auto disk = tune_read(path);
+    disk[ks] = {B.variant, B.variant64};
+    tune_write(path, disk);
+  }
   return 0;
 }
 
@@ -877,8 +1140,14 @@ static int32_t batch_load(hecmw33_batch& B, int32_t nsys, const hecmw33_sys* sys
       B.sys_ntile.ensure(sizeof(int32_t) * nsys) || B.b.ensure(sizeof(double) * nv) ||
       B.x.ensure(sizeof(double) * nv) || B.r.ensure(sizeof(float) * nv) || B.z.ensure(sizeof(float) * nv) ||
       B.p.ensure(sizeof(float) * nv) || B.q.ensure(sizeof(float) * nv) ||
-      B.part.ensure(sizeof(double) * 4 * KS * (size_t)B.ntiles) || batch_state_layout(B))
+      B.part.ensure(sizeof(double) * 4 * KS * (size_t)B.ntiles) || batch_state_layout(B) ||
+      B.qtile.ensure(sizeof(int32_t) * kqLists * (size_t)B.ntiles) || B.qn.ensure(sizeof(int32_t) * kqLists))
     return HECMW33_ERROR_DEVICE;
+  {
+    int dev = 0;
+    BHIPC(hipGetDevice(&dev));
+    BHIPC(hipDeviceGetAttribute(&B.ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  }
   hipStream_t st = B.stream;
   BHIPC(hipMemcpyAsync(B.rowptr.p, rp.data(), sizeof(int32_t) * rp.size(), hipMemcpyHostToDevice, st));
   BHIPC(hipMemcpyAsync(B.col.p, cl.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice, st));
@@ -960,12 +1229,14 @@ static int32_t batch_solve(hecmw33_batch& B, const int32_t* iarray, const double
   BHIPC(hipMemcpyAsync(B.S.rel, ones.data(), sizeof(int32_t) * ones.size(), hipMemcpyHostToDevice, st));
   BHIPC(hipMemcpyAsync(B.S.sys_rel, sys1.data(), sizeof(int32_t) * B.nsys, hipMemcpyHostToDevice, st));
   // |b - A x0|^2 into the second dot slot
-  kVariants64[B.variant64].res(B.ntiles, st, B.T, B.rowptr.as<int32_t>(), B.col.as<int32_t>(), B.val64.p, B.x.p,
-                               B.b.p, B.r.p, B.S, part + (size_t)KS * B.ntiles, 0);
+  kVariants64[B.variant64].res(B.ntiles, st, B.T, qall(), B.rowptr.as<int32_t>(), B.col.as<int32_t>(), B.val64.p,
+                               B.x.p, B.b.p, B.r.p, B.S, part + (size_t)KS * B.ntiles, 0);
   kb_tail<kbInit, 2><<<B.nsys, kBT, 0, st>>>(B.T, B.S, part);
   // the first iteration's z = D^-1 r0 and its dots
-  kb_diag<<<B.ntiles, kBT, 0, st>>>(B.T, B.alu32.as<float>(), B.r.as<float>(), B.z.as<float>(), B.q.as<float>(), B.S,
-                                    part + (size_t)2 * KS * B.ntiles, 0);
+  kb_diag<<<B.ntiles, kBT, 0, st>>>(B.T, qall(), B.alu32.as<float>(), B.r.as<float>(), B.z.as<float>(),
+                                    B.q.as<float>(), B.S, part + (size_t)2 * KS * B.ntiles, 0);
+  // the systems the first iteration runs
+  kb_queue<kqActive><<<1, 1024, 0, st>>>(B.T, B.nsys, B.S, queue_of(B));
   BHIPC(hipGetLastError());
   BHIPC(hipStreamSynchronize(st));
   const auto t1 = std::chrono::steady_clock::now();
@@ -974,7 +1245,6 @@ static int32_t batch_solve(hecmw33_batch& B, const int32_t* iarray, const double
     hipGraph_t g;
     BHIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     for (int i = 0; i < kChunk; ++i) batch_enqueue_iteration(B);
-    kb_count<<<1, 64, 0, st>>>(B.nsys, B.S);
     BHIPC(hipStreamEndCapture(st, &g));
     BHIPC(hipGraphInstantiate(&B.graph, g, nullptr, nullptr, 0));
     (void)hipGraphDestroy(g);
@@ -1077,14 +1347,14 @@ int32_t hecmw33_batch_spmm(hecmw33_batch* b, int32_t fp64, const double* const* 
   BHIPC(hipMemcpy(xin.p, h.data(), sizeof(double) * nv, hipMemcpyHostToDevice));
   if (fp64) {
     const int v = fp64 >= 2 ? fp64 - 2 : b->variant64;
-    kVariants64[v].st(b->ntiles, st, b->T, b->rowptr.as<int32_t>(), b->col.as<int32_t>(), b->val64.p, xin.p, nullptr,
-                      yout.p, b->S, nullptr, 0);
+    kVariants64[v].st(b->ntiles, st, b->T, qall(), b->rowptr.as<int32_t>(), b->col.as<int32_t>(), b->val64.p, xin.p,
+                      nullptr, yout.p, b->S, nullptr, 0);
   } else {
     float* x32 = b->z.as<float>();  // scratch: the solve re-initialises it
     float* y32 = b->q.as<float>();
     kb_to_f32<<<grid_of(nv), kBT, 0, st>>>(nv, xin.as<double>(), x32);
-    kVariants[b->variant].st32(b->ntiles, st, b->T, b->rowptr.as<int32_t>(), b->col.as<int32_t>(), b->val32.p, x32,
-                               nullptr, y32, b->S, nullptr, 0);
+    kVariants[b->variant].st32(b->ntiles, st, b->T, qall(), b->rowptr.as<int32_t>(), b->col.as<int32_t>(),
+                               b->val32.p, x32, nullptr, y32, b->S, nullptr, 0);
     BHIPC(hipStreamSynchronize(st));
     std::vector<float> hf(nv);
     BHIPC(hipMemcpy(hf.data(), y32, sizeof(float) * nv, hipMemcpyDeviceToHost));
@@ -1102,7 +1372,10 @@ int32_t hecmw33_batch_spmm(hecmw33_batch* b, int32_t fp64, const double* const* 
   return 0;
 }
 
-// [0] rows, [1] blocks, [2] tiles, [3] tuned variant, [4..4+n) variant times in us (FP32 SpMM + dots)
+// [0] rows, [1] blocks, [2] tiles, [3] FP32 shape, [4] nv32, [5..5+nv32) FP32 shape times in us (SpMM +
+// dots; 0 when the shapes came from the persisted winners), FP64 shape, nv64, FP64 shape times in us,
+// then: shapes from the persisted winners (0/1), CU count, the persistent grids of the p, q = A p,
+// update, residual and DIAG launches
 int32_t hecmw33_batch_info(hecmw33_batch* b, int64_t* info, int32_t n) {
   if (!b || !info) return HECMW33_ERROR_ARGS;
   std::vector<int64_t> v = {b->nrows, b->nnzb, b->ntiles, b->variant, kNumVariants};
@@ -1110,6 +1383,9 @@ int32_t hecmw33_batch_info(hecmw33_batch* b, int64_t* info, int32_t n) {
   v.push_back(b->variant64);
   v.push_back(kNumVariants64);
   for (int i = 0; i < kNumVariants64; ++i) v.push_back((int64_t)std::llround(b->variant64_ms[i] * 1000.0));
+  for (int64_t e : {(int64_t)b->tune_cached, (int64_t)b->ncu, (int64_t)b->grid_p, (int64_t)b->grid_q,
+                    (int64_t)b->grid_upd, (int64_t)b->grid_res, (int64_t)b->grid_diag})
+    v.push_back(e);
   for (int i = 0; i < n && i < (int)v.size(); ++i) info[i] = v[i];
   return (int32_t)v.size();
 }

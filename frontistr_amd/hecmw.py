@@ -660,10 +660,13 @@ class BatchSolver:
         o = 5 + n32
         n64 = int(v[o + 1])
         names64 = [lib().hecmw33_batch_variant64_name(i).decode() for i in range(n64)]
+        e = o + 2 + n64
         return {"rows": int(v[0]), "blocks": int(v[1]), "tiles": int(v[2]), "variant": names[int(v[3])],
                 "variant_us": dict(zip(names, [int(t) for t in v[5:5 + n32]])),
                 "variant64": names64[int(v[o])],
-                "variant64_us": dict(zip(names64, [int(t) for t in v[o + 2:o + 2 + n64]]))}
+                "variant64_us": dict(zip(names64, [int(t) for t in v[o + 2:o + 2 + n64]])),
+                "tune_cached": bool(v[e]), "cus": int(v[e + 1]),
+                "grids": dict(zip(("p", "q", "update", "res", "diag"), [int(t) for t in v[e + 2:e + 7]]))}
 
 
 def solve_batch(mats, Bs, iarray, rarray):

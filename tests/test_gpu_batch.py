@@ -123,3 +123,65 @@ def test_solve_batch_one_shot():
             mk = m.copy()
             mk.B[:] = Bs[s][k]
             assert ref.rel_resid(mk, Xs[s][k]) <= 1e-8 * 1.0001
+
+
+_TUNE_CHILD = r"""
+import sys
+sys.path.insert(0, sys.argv[1])
+import numpy as np
+import frontistr_amd as fa
+mats, Bs = [], []
+for i, (n, box) in enumerate([((2, 2, 2), (1, 1, 1)), ((4, 2, 3), (2.0, 0.8, 1.1))]):
+    m, B = fa.gen_tet10(*n, box=box, nrhs=2, seed=i)
+    mats.append(m)
+    Bs.append(B)
+bs = fa.BatchSolver(mats, Bs)
+info = bs.info()
+print("TUNE", int(info["tune_cached"]), info["variant"], info["variant64"], min(info["grids"].values()))
+bs.close()
+"""
+
+
+def test_autotuner_winners_persist_per_bucket(tmp_path, monkeypatch):
+    """The first load of a (blocks/row, rows) bucket times every shape and
+    writes the winners; a later load of that bucket, in this process or in a
+    new one, takes them without timing; a line naming a shape this library
+    does not have is ignored."""
+    import os
+    import subprocess
+    import sys
+    path = tmp_path / "tune" / "batch_tune.txt"
+    monkeypatch.setenv("HECMW33_TUNE_CACHE", str(path))
+    mats, Bs = _batch(nrhs=2)
+    bs = fa.BatchSolver(mats, Bs)
+    i1 = bs.info()
+    assert not i1["tune_cached"] and min(i1["variant_us"].values()) > 0
+    bs.close()
+    lines = path.read_text().split()
+    assert i1["variant"] in lines and i1["variant64"] in lines
+    bs = fa.BatchSolver(mats, Bs)
+    i2 = bs.info()
+    assert i2["tune_cached"] and (i2["variant"], i2["variant64"]) == (i1["variant"], i1["variant64"])
+    assert all(v == 0 for v in i2["variant_us"].values())
+    assert all(g >= 1 for g in i2["grids"].values())
+    # the persisted shapes solve as the timed ones do
+    ia, ra = _params()
+    it, er, rs, _ = bs.solve(ia, ra)
+    assert np.all(er == 0) and np.all(rs <= 1e-8)
+    bs.close()
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, HECMW33_TUNE_CACHE=str(path))
+    run = lambda: subprocess.run([sys.executable, "-c", _TUNE_CHILD, root], env=env, stdout=subprocess.PIPE,
+                                 text=True, timeout=300, check=True).stdout
+    out = [l for l in run().splitlines() if l.startswith("TUNE")][0].split()
+    first = out[1]
+    # another bucket (two smaller systems): timed in the new process, then written beside the first
+    assert out[1] == "0" and int(out[4]) >= 1
+    assert len(path.read_text().strip().splitlines()) == 2
+    out = [l for l in run().splitlines() if l.startswith("TUNE")][0].split()
+    assert out[1] == "1", (first, out)
+    # a line with an unknown shape name is dropped and that bucket re-timed
+    text = path.read_text().strip().splitlines()
+    path.write_text("\n".join(l.rsplit(" ", 2)[0] + " no_such_shape lpr16_rpl1_u1" for l in text) + "\n")
+    out = [l for l in run().splitlines() if l.startswith("TUNE")][0].split()
+    assert out[1] == "0"

@@ -94,9 +94,32 @@ def test_training_learns_the_labels():
     ctx, b = _context(m, 20, torch)
     fa.harvest_errors(net, ctx, m, b)
     l, acc = net.score()
-    print("held-out: loss %.4f, accuracy %.4f" % (l, acc))
-    assert acc >= 0.95
+    # the same labels on the host (hecmw33_rigid_labels over the same two errors),
+    # for the share of the majority class
+    errs = _errors(ctx, m, b, torch, (10, 40))
+    t = fa.rigid_labels(m, fa.hex8_coords(20), errs)
+    major = max(t.mean(), 1.0 - t.mean())
+    print("held-out: loss %.4f, accuracy %.4f, labelled rigid %.4f" % (l, acc, t.mean()))
+    assert 0.0 < t.mean() < 1.0
+    # well above what one class alone scores
+    assert acc >= 0.9 and (1.0 - acc) <= 0.6 * (1.0 - major)
     net.close()
+
+
+def _errors(ctx, m, b, torch, iters):
+    """e_k = x* - x_k of the FP64 SSOR-CG iterates at `iters` (as harvest_errors takes them), on the host"""
+    ia = m.Iarray.copy()
+    ia[m.I_FLAG_NUMFACT - 1] = ia[m.I_FLAG_SYMBFACT - 1] = ia[m.I_SOLVER_OPT - 1] = 0
+    xs = torch.zeros_like(b)
+    assert ctx.solve_device(b.data_ptr(), xs.data_ptr(), ia, m.Rarray).error == 0
+    x = torch.zeros_like(b)
+    out, done = [], 0
+    for k in iters:
+        ia[m.I_ITER - 1] = k - done
+        ctx.solve_device(b.data_ptr(), x.data_ptr(), ia, m.Rarray)
+        done = k
+        out.append((xs - x).cpu().numpy()[:3 * m.N])
+    return np.array(out)
 
 
 def test_learned_aggregation_gains_on_a_held_out_cube():
