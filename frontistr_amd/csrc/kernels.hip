@@ -1333,13 +1333,27 @@ static int ssor_chunk_sel() {
   return v;
 }
 
+// Blocks in flight per lane of the one-lane-per-row sweeps (HECMW33_SSOR64_CHUNK
+// = 4 or 8; 4 by default).
+static int ssor64_chunk_sel() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = getenv("HECMW33_SSOR64_CHUNK");
+    v = (e && atoi(e) == 8) ? 8 : 4;
+  }
+  return v;
+}
+
 template <typename T>
 static void ssor_fwd_any(const SsorOp<T>& op, int c, const KrylovState* st, hipStream_t s) {
   const int32_t s0 = op.color_slice[c], ns = op.color_slice[c + 1] - s0;
   if (ns <= 0) return;
   const int g = spmv_grid(ns);
   if (op.L.rows == kRowsA) {
-    k_ssor64_fwd<T, 4><<<g, kBlock, 0, s>>>(op.L, op.row_old, op.alu, s0, ns, op.zs, st);
+    if (ssor64_chunk_sel() == 8)
+      k_ssor64_fwd<T, 8><<<g, kBlock, 0, s>>>(op.L, op.row_old, op.alu, s0, ns, op.zs, st);
+    else
+      k_ssor64_fwd<T, 4><<<g, kBlock, 0, s>>>(op.L, op.row_old, op.alu, s0, ns, op.zs, st);
     return;
   }
   switch (ssor_chunk_sel()) {
@@ -1355,7 +1369,10 @@ static void ssor_bwd_any(const SsorOp<T>& op, int c, const KrylovState* st, hipS
   if (ns <= 0) return;
   const int g = spmv_grid(ns);
   if (op.U.rows == kRowsA) {
-    k_ssor64_bwd<T, 4><<<g, kBlock, 0, s>>>(op.U, op.row_old, op.alu, s0, ns, op.zs, st);
+    if (ssor64_chunk_sel() == 8)
+      k_ssor64_bwd<T, 8><<<g, kBlock, 0, s>>>(op.U, op.row_old, op.alu, s0, ns, op.zs, st);
+    else
+      k_ssor64_bwd<T, 4><<<g, kBlock, 0, s>>>(op.U, op.row_old, op.alu, s0, ns, op.zs, st);
     return;
   }
   switch (ssor_chunk_sel()) {
