@@ -32,6 +32,7 @@ SOURCES = [
     ("batch.hip", True),
     ("learned.hip", True),
     ("coarse.hip", True),
+    ("noalloc.hip", True),
 ]
 
 

@@ -2,8 +2,12 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <string>
 
 namespace hecmw33 {
+
+// the message hecmw33_last_error() returns on this thread
+void set_last_error(const std::string& s);
 
 constexpr int kWave = 64;       // CDNA wavefront
 constexpr int kWavesPerBlock = 4;

@@ -43,6 +43,7 @@ namespace hecmw33 {
 static thread_local std::string g_last_error;
 
 static void set_error(const std::string& s) { g_last_error = s; }
+void set_last_error(const std::string& s) { g_last_error = s; }
 
 #define HIPC(expr)                                                                          \
   do {                                                                                      \

@@ -68,6 +68,7 @@ def lib():
     L.hecmw33_matvec.argtypes = [C.POINTER(_Sys), P, P]
     L.hecmw33_precond_apply.argtypes = [C.POINTER(_Sys), P, P, P, P]
     L.hecmw33_rel_resid_L2.argtypes = [C.POINTER(_Sys), P, P, C.POINTER(C.c_double)]
+    L.hecmw33_rel_resid_L2_device_noalloc.argtypes = [C.POINTER(_Sys), P, P, P, P]
     L.hecmw33_ordering.argtypes = [I, P, P, P, P, I, P, P, P, C.POINTER(I), P]
     L.hecmw33_ctx_create.restype = P
     L.hecmw33_ctx_destroy.argtypes = [P]

@@ -151,6 +151,19 @@ int32_t hecmw33_precond_apply(const hecmw33_sys* A, const int32_t iarray[100], c
 /* sqrt(|b - A x|^2 / |b|^2) (hecmw_rel_resid_L2_33, hecmw_solver_las_33.F90:406-435). */
 int32_t hecmw33_rel_resid_L2(const hecmw33_sys* A, const double* B, const double* X, double* resid);
 
+/* hecmw33_rel_resid_L2 for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc); no
+ * counterpart in the reference.  Every array of A, B, X (3*NP each) and resid
+ * (one double, written) is in device memory; the work is enqueued on `stream`
+ * (a hipStream_t; NULL: the null stream) and nothing is allocated: the partial
+ * sums live in a fixed device array of HECMW33_NOALLOC_MAX_PARTS pairs, one per
+ * 256 rows, so N above 256 * HECMW33_NOALLOC_MAX_PARTS rows is refused
+ * (HECMW33_ERROR_ARGS).  Each GPU holds one such array (1 MiB) until the process
+ * ends; calls that overlap on one GPU share it and race. */
+#define HECMW33_NOALLOC_MAX_PARTS 65536
+int32_t hecmw33_rel_resid_L2_device_noalloc(const hecmw33_sys* A, const double* B, const double* X, double* resid,
+                                           void* stream);
+
 /* Per-iteration relative residuals of the last hecmw33_solve / hecmw33_solve_krylov
  * call, the values the reference prints with ITERLOG (hecmw_solver_CG.f90:253);
  * returns the number written (at most cap). */
