@@ -7,19 +7,21 @@ log-uniform in [1e3, 1e6] and nu uniform in [0.2, 0.4] (seed 0), 12,519,843
 DOF, 111,284,641 3x3 blocks, assembled like fistr1 (gen_hex8.cpp).  One step
 = one complete solve x0 = 0 -> FP64 true relative residual 1e-8 with the
 flexible CG (FP32 SpMV/SSOR/vectors, FP64 inner products, x and residual
-replacement) whose symmetric two-level step runs on the aggregation the
-learned network refines (Iarray 41 = 2, 43 = 1), matrix and vectors resident
-in HBM, numeric set-up (SSOR factors, coarse matrix, the network's mask)
-inside the step.  The network is trained first, untimed, on errors the solver
-harvests from SSOR-CG runs on seeded variants of the two-phase workload.
+replacement) whose symmetric two-level step carries, beside each aggregate's
+six rigid-body modes, six learned modes (Iarray 41 = 2, 43 = 1): the decoders
+of per-aggregate linear autoencoders fitted to slow errors the solver
+generates on the system (8 seeded samples x 20 FCG iterations), matrix and
+vectors resident in HBM, the numeric set-up (SSOR factors, the harvest and fit
+of the learned modes, the coarse matrix) inside the step.
 
 The same JSON line carries, on the same GPU: the plain FP64 multicolour-SSOR
 CG on the same system, the speed-up and |x - x_FP64|/|x_FP64|; the other
 flexible-CG modes; the SpMV medians of 100 calls (100^3 FP64, 160^3 FP32);
 the 100^3 cube's FP64 SSOR-CG / DIAG-CG (the plain method on GPU); the
 batched workload (48 TET10 boxes x 16 right-hand sides); the 120^3 two-phase
-inclusion cube with and without the learned aggregation; and the reference's
-own CPU solver (oracle/_ref, built from the FrontISTR sources) on the host.
+inclusion cube with the rigid-body and the learned coarse modes; and the
+reference's own CPU solver (oracle/_ref, built from the FrontISTR sources) on
+the host.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--only c2|c3|c4|c5]
 """
@@ -42,8 +44,8 @@ HBM_PEAK_GBS = 8000.0
 SPMV_TRAFFIC_100 = 2199539866
 
 # (name, Iarray 41, Iarray 43, Iarray 44): FP64 CG; flexible CG with FP32
-# operators; with the symmetric two-level step on the cells; on the learned
-# aggregation
+# operators; with the symmetric two-level step (rigid-body modes); with the
+# learned modes beside them
 FCG_MODES = (("fp64_ssor_cg", 0, 0, 0), ("mixed_fcg", 1, 0, 0), ("mixed_coarse_fcg", 1, 1, 0),
              ("learned_coarse_fcg", 2, 1, 0))
 
@@ -90,7 +92,7 @@ def host_threads():
     return max(2, min(n, 16))  # the box's CPU share for one GPU is 16 threads
 
 
-# ---- the network ----------------------------------------------------------------------------
+# ---- workloads ------------------------------------------------------------------------------
 def inclusions(fa, n, seed):
     """the two-phase workload's generator: matrix E 1e3, spheres of E 1e7, radii 2-6 elements, 15%"""
     m = fa.gen_hex8(n, kind="inclusions", seed=seed, E=1000.0, nu=0.3, contrast=1e4, rmin=2, rmax=6, vfrac=0.15)
@@ -100,64 +102,14 @@ def inclusions(fa, n, seed):
     return m
 
 
-def train_net(args, dev):
-    """The learned aggregation's network, trained on the GPU on labels built
-    from errors the solver harvests itself: per seeded variant of the two-phase
-    workload (args.train_n^3, the workload's inclusion statistics) and load case
-    (the pressure load, then random loads on the loaded face), the FP64
-    SSOR-CG solution x* and the errors x* - x_k of its iterates at 32
-    checkpoints (every 16 iterations from 32 on)."""
-    import torch
-    import frontistr_amd as fa
-    t0 = time.time()
-    net = fa.LearnedNet(seed=0)
-    keep = []
-    rng = np.random.default_rng(123)
-    n = args.train_n
-    checkpoints = tuple(32 + 16 * k for k in range(32))
-    for v in range(args.train_variants):
-        m = inclusions(fa, n, 1000 + v)
-        ctx = fa.DeviceContext()
-        ctx.set_coords(fa.hex8_coords(n))
-        ctx.load(m)
-        top = np.nonzero(m.B)[0]
-        c = 0
-        for lc in range(2):
-            bvec = m.B.copy()
-            if lc > 0:
-                bvec[:] = 0.0
-                bvec[top] = rng.standard_normal(top.size) * np.abs(m.B[top]).mean()
-            b = torch.from_numpy(bvec).to(dev)
-            c = max(c, fa.harvest_errors(net, ctx, m, b, iters=checkpoints))
-        keep.append((ctx, c))
-    t_harvest = time.time() - t0
-    total = sum(k[1] for k in keep)
-    t0 = time.time()
-    loss = net.train(epochs=args.train_epochs, lr=3e-3, seed=0)
-    t_train = time.time() - t0
-    _, acc = net.score()
-    log("trained: %d error samples from %d variants (%d^3) in %.1fs; training %.1fs, logistic loss %.4f -> %.4f, "
-        "accuracy %.4f" % (total, args.train_variants, n, t_harvest, t_train, loss[0], loss[-1], acc))
-    net.clear_samples()
-    for c, _ in keep:
-        c.close()
-    return net, {"error_samples": int(total), "variants": args.train_variants, "variant_n": n,
-                 "harvest_s": round(t_harvest, 2), "train_s": round(t_train, 2), "epochs": args.train_epochs,
-                 "loss_before": round(float(loss[0]), 5), "loss_after": round(float(loss[-1]), 5),
-                 "train_accuracy": round(float(acc), 5)}
-
-
 # ---- a resident system and its modes ------------------------------------------------------
 class Resident:
-    def __init__(self, fa, m, coords, net, dev):
+    def __init__(self, fa, m, coords, dev):
         import torch
         self.fa, self.m = fa, m
         self.ctx = fa.DeviceContext()
         if coords is not None:
             self.ctx.set_coords(coords)
-        if net is not None:
-            self.ctx.set_net(net)
-        self.net = net
         t0 = time.time()
         self.ctx.load(m)
         self.load_s = time.time() - t0
@@ -189,8 +141,6 @@ class Resident:
         return float(torch.linalg.vector_norm(self.b[:n3] - self.y[:n3])) / self.bn
 
     def close(self):
-        if self.net is not None:
-            self.ctx.set_net(None)
         self.ctx.close()
 
 
@@ -198,8 +148,6 @@ def run_modes(tag, R, modes):
     """Each mode once from x0 = 0 to 1e-8, numeric set-up inside."""
     res, xs = {}, {}
     for mode, opt, coarse, additive in modes:
-        if opt == 2 and R.net is None:
-            continue
         r, wall = R.solve(R.iarray(opt, coarse, additive))
         inf = R.ctx.info()
         xs[mode] = R.x.cpu().numpy()
@@ -212,8 +160,14 @@ def run_modes(tag, R, modes):
             res[mode]["spmv_fp64_values"] = bool(inf["spmv64"])
         if coarse:
             res[mode]["coarse_aggregates"] = inf["coarse"]
+            res[mode]["modes_per_aggregate"] = inf["modes_per_aggregate"]
+            res[mode]["coarse_unknowns"] = inf["coarse_unknowns"]
         if opt == 2:
-            res[mode]["pieces_from_network"] = inf["pieces"]
+            res[mode]["learned_modes"] = inf["learned_modes"]
+            res[mode]["error_samples"] = inf["mode_samples"]
+            res[mode]["fcg_iterations_per_sample"] = inf["mode_sample_iters"]
+            res[mode]["harvest_s"] = round(inf["mode_harvest_s"], 4)
+            res[mode]["fit_s"] = round(inf["mode_fit_s"], 4)
             res[mode]["learned_step_dropped_at"] = inf["fallback_iter"]
             res[mode]["learned_step_kept"] = not bool(inf["learned_off"])
         log("%s %s: %s" % (tag, mode, res[mode]))
@@ -250,7 +204,7 @@ def ssor_ms(R, fp32, reps=20):
 
 
 # ---- C3: the headline -------------------------------------------------------------------
-def bench_c3(args, dev, net):
+def bench_c3(args, dev):
     import torch
     import frontistr_amd as fa
     n = args.c3_n
@@ -261,10 +215,10 @@ def bench_c3(args, dev, net):
     m.set_precond(1)
     m.set_i(m.I_TIMELOG, 1)
     m.set_coarse(1)  # the aggregation cells are symbolic set-up, built at load like the colouring
-    R = Resident(fa, m, fa.hex8_coords(n), net, dev)
+    R = Resident(fa, m, fa.hex8_coords(n), dev)
     modes, xs = run_modes("C3", R, FCG_MODES)
-    head = "learned_coarse_fcg" if net is not None else "mixed_coarse_fcg"
-    opt = 2 if net is not None else 1
+    head = "learned_coarse_fcg"
+    opt = 2
     ia = R.iarray(opt, 1, 0)
     ia_log = ia.copy()
     ia[21] = 0  # no TIMELOG events inside the timed steps; the warm-up carries them
@@ -308,12 +262,45 @@ def bench_c3(args, dev, net):
         "ssor_fp32_GBs": round(sb32 / (ssor32 * 1e-3) / 1e9, 1),
         "ssor_fp64_ms": round(ssor64, 4), "ssor_fp64_GBs": round(sb64 / (ssor64 * 1e-3) / 1e9, 1),
         "ncolor": R.ctx.info()["ncolor"],
+        "learned_vs_rigid_body_modes": {
+            "iterations": [modes["mixed_coarse_fcg"]["iterations"], modes["learned_coarse_fcg"]["iterations"]],
+            "seconds": [modes["mixed_coarse_fcg"]["seconds"], modes["learned_coarse_fcg"]["seconds"]]},
     }
     log("C3 %s: %.4f s/step, %d iterations; SpMV FP32 %.4f ms (%.0f GB/s); SSOR FP32 %.4f ms (%.0f GB/s), "
         "FP64 %.4f ms" % (head, out["seconds_per_step"], r_last.iter, ms32, out["spmv_fp32_GBs"], ssor32,
                           out["ssor_fp32_GBs"], ssor64))
     R.close()
     return out, m, modes["fp64_ssor_cg"]["iterations"]
+
+
+# ---- the SSOR application and SpMV alone, for hardware-counter runs --------------------------
+def bench_ssor_probe(args, dev):
+    """The headline system's FP32 multicolour SSOR application and FP32 SpMV,
+    10 calls each, after a 1 GiB device-to-device copy that calibrates the HBM
+    byte counters (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE per dispatch)."""
+    import torch
+    import frontistr_amd as fa
+    n = args.c3_n
+    m = fa.gen_hex8(n, kind="lognormal", seed=0)
+    m.set_iter(2)
+    m.set_precond(1)
+    R = Resident(fa, m, None, dev)
+    R.solve(R.iarray(1, 0, 0))  # SSOR set-up and the FP32 copies
+    a = torch.empty(1 << 27, dtype=torch.float64, device=dev)
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    ssor = ssor_ms(R, True, reps=10)
+    spmv = spmv_median(R, True, 10)
+    nnzb = m.N + m.NPL + m.NPU
+    out = {"workload": "%d^3 HEX8 lognormal, %d DOF" % (n, 3 * m.N), "copy_bytes": 2 * a.numel() * 8,
+           "ssor_fp32_ms": round(ssor, 4), "ssor_fp32_bytes": ssor_bytes(m.N, m.NPL + m.NPU, 4),
+           "spmv_fp32_ms": round(spmv, 4), "spmv_fp32_bytes": spmv_bytes(m.N, m.NP, nnzb, 4),
+           "ncolor": R.ctx.info()["ncolor"]}
+    R.close()
+    return out
 
 
 # ---- C2: the plain method on GPU ------------------------------------------------------------
@@ -325,7 +312,7 @@ def bench_c2(args, dev):
     m.set_iter(20000)
     m.set_precond(1)
     m.set_i(m.I_TIMELOG, 1)
-    R = Resident(fa, m, None, None, dev)
+    R = Resident(fa, m, None, dev)
     nnzb = m.N + m.NPL + m.NPU
     res = {}
     for name, pc in (("fp64_ssor_cg", 1), ("fp64_diag_cg", 3)):
@@ -355,20 +342,20 @@ def bench_c2(args, dev):
     return out, m
 
 
-# ---- C5: the two-phase cube, with and without the learned aggregation ---------------------
-def bench_c5(args, dev, net, train):
+# ---- C5: the two-phase cube, rigid-body and learned coarse modes -----------------------------
+def bench_c5(args, dev):
     import frontistr_amd as fa
     m = inclusions(fa, args.c5_n, 7)
     m.set_coarse(1)
-    R = Resident(fa, m, fa.hex8_coords(args.c5_n), net, dev)
+    R = Resident(fa, m, fa.hex8_coords(args.c5_n), dev)
     modes = (FCG_MODES[0],) + FCG_MODES[2:]
     res, _ = run_modes("C5", R, modes)
     R.close()
     out = {"workload": "%d^3 HEX8, two-phase inclusions (E 1e3 / 1e7, 15%% volume, radii 2-6 elements, seed 7), "
-                       "%d DOF" % (args.c5_n, 3 * m.N), "train": train}
+                       "%d DOF" % (args.c5_n, 3 * m.N)}
     out.update(res)
     if "learned_coarse_fcg" in res:
-        out["learned_vs_cells"] = {
+        out["learned_vs_rigid_body_modes"] = {
             "iterations": [res["mixed_coarse_fcg"]["iterations"], res["learned_coarse_fcg"]["iterations"]],
             "seconds": [res["mixed_coarse_fcg"]["seconds"], res["learned_coarse_fcg"]["seconds"]],
             "speedup": round(res["mixed_coarse_fcg"]["seconds"] / res["learned_coarse_fcg"]["seconds"], 3)}
@@ -484,14 +471,12 @@ def main():
     ap.add_argument("--c3-n", type=int, default=160, help="the headline cube")
     ap.add_argument("--c4-nsys", type=int, default=48, help="batched TET10 systems (0: skip)")
     ap.add_argument("--c5-n", type=int, default=120, help="the two-phase cube (0: skip)")
-    ap.add_argument("--train-variants", type=int, default=32, help="seeded variants the network trains on (0: none)")
-    ap.add_argument("--train-n", type=int, default=24)
-    ap.add_argument("--train-epochs", type=int, default=40)
     ap.add_argument("--cpu-sample-s", type=float, default=20.0, help="budget of the CPU reference sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=100)
-    ap.add_argument("--only", choices=("c2", "c3", "c4", "c5"), default=None,
-                    help="run one workload section alone (for profiling); prints that section's JSON")
+    ap.add_argument("--only", choices=("c2", "c3", "c4", "c5", "ssor"), default=None,
+                    help="run one workload section alone (for profiling); prints that section's JSON; ssor: the "
+                         "headline system's FP32 SSOR application and SpMV after a 1 GiB calibration copy")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: one GPU only (--gpus %d requested); sharding over GPUs (one rank per GPU, a halo "
@@ -505,22 +490,21 @@ def main():
         sys.exit(1)
     dev = torch.device("cuda:0")
 
-    net, train = (None, None)
-    if args.train_variants > 0 and args.only in (None, "c3", "c5"):
-        net, train = train_net(args, dev)
     if args.only:
         if args.only == "c3":
-            sec = {"c3": bench_c3(args, dev, net)[0]}
+            sec = {"c3": bench_c3(args, dev)[0]}
         elif args.only == "c2":
             sec = {"c2": bench_c2(args, dev)[0]}
         elif args.only == "c4":
             sec = {"batch": bench_batch(args)}
+        elif args.only == "ssor":
+            sec = {"ssor": bench_ssor_probe(args, dev)}
         else:
-            sec = {"c5": bench_c5(args, dev, net, train)}
+            sec = {"c5": bench_c5(args, dev)}
         print(json.dumps(dict(only=args.only, **sec)), flush=True)
         return
 
-    c3, m3, iters3 = bench_c3(args, dev, net)
+    c3, m3, iters3 = bench_c3(args, dev)
     torch.cuda.empty_cache()
     c2, m2 = bench_c2(args, dev) if args.c2_n > 0 else (None, None)
     torch.cuda.empty_cache()
@@ -530,9 +514,7 @@ def main():
     del m3
     batch = bench_batch(args) if args.c4_nsys > 0 else None
     torch.cuda.empty_cache()
-    c5 = bench_c5(args, dev, net, train) if args.c5_n > 0 else None
-    if net is not None:
-        net.close()
+    c5 = bench_c5(args, dev) if args.c5_n > 0 else None
 
     fp64 = c3["modes"]["fp64_ssor_cg"]
     value = c3["seconds_per_step"]
@@ -576,7 +558,6 @@ def main():
         "c2": c2,
         "batch": batch,
         "c5": c5,
-        "train": train,
     }
     print(json.dumps(out), flush=True)
 

@@ -148,6 +148,41 @@ __global__ void k_coarse_inv32(int32_t nc, const double* __restrict__ Ei, int32_
   W[t] = v;
 }
 
+// W = E^-1 in FP64, full symmetric (the twelve-mode step: with the learned modes
+// E is too ill-conditioned for an FP32 inverse)
+__global__ void k_coarse_inv64(int32_t nc, const double* __restrict__ Ei, int32_t lda, double* __restrict__ W,
+                               int32_t ldw) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)nc * ldw) return;
+  const int row = (int)(t / ldw), col = (int)(t % ldw);
+  double v = 0.0;
+  if (col < nc) {
+    const int lo = row > col ? row : col, hi = row > col ? col : row;
+    v = Ei[(int64_t)hi * lda + lo];
+  }
+  W[t] = v;
+}
+
+// y = W c in FP64, one wavefront per row, double2 loads (ldw a multiple of 64)
+__global__ __launch_bounds__(kCoarseBlock) void k_coarse_gemv64(int32_t nc, int32_t ldw, const double* __restrict__ W,
+                                                                 const double* __restrict__ c, double* __restrict__ y,
+                                                                 const KrylovState* st) {
+  if (st->done || st->coarse_off) return;
+  const int row = blockIdx.x * kCoarseWaves + threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (row >= nc) return;
+  const double2* w = reinterpret_cast<const double2*>(W + (int64_t)row * ldw);
+  const double2* c2 = reinterpret_cast<const double2*>(c);
+  double acc = 0.0;
+  for (int k = lane; k < ldw / 2; k += kWave) {
+    const double2 a = w[k], b = c2[k];
+    acc += a.x * b.x + a.y * b.y;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kWave);
+  if (lane == 0) y[row] = acc;
+}
+
 // c = Z^T r per aggregate (FP64 sums, fixed order)
 __global__ __launch_bounds__(kCoarseBlock) void k_coarse_restrict(const int32_t* __restrict__ aptr,
                                                                    const int32_t* __restrict__ anodes,
@@ -482,13 +517,276 @@ __global__ __launch_bounds__(kCoarseBlock) void k_fcg_pq(int64_t n3, const float
   if (threadIdx.x < 2) part[threadIdx.x * kMaxPartials + blockIdx.x] = tot[threadIdx.x];
 }
 
+// ---- twelve modes per aggregate: the six rigid-body modes and six learned ones -------------
+// Z_i (3x12) = [ I | R(d_i) | X_i ], X_i (3x6) = the learned modes' values at node i,
+// xm[18 i + 3 j + d] (mode j, component d).  E, A Z and the per-iteration kernels
+// follow the six-mode ones; E_ab is assembled as four 6x6 tiles.
+
+// the 3x6 block t of Z_i: t = 0 the rigid-body modes, t = 1 the learned ones
+__device__ __forceinline__ void zblock(const float* __restrict__ dv, const float* __restrict__ xm, int64_t i, int t,
+                                       double (&z)[3][6]) {
+  if (t == 0) {
+    const double dx = dv[3 * i], dy = dv[3 * i + 1], dz = dv[3 * i + 2];
+    z[0][0] = 1.0, z[0][1] = 0.0, z[0][2] = 0.0, z[0][3] = 0.0, z[0][4] = dz, z[0][5] = -dy;
+    z[1][0] = 0.0, z[1][1] = 1.0, z[1][2] = 0.0, z[1][3] = -dz, z[1][4] = 0.0, z[1][5] = dx;
+    z[2][0] = 0.0, z[2][1] = 0.0, z[2][2] = 1.0, z[2][3] = dy, z[2][4] = -dx, z[2][5] = 0.0;
+  } else {
+    const float* x = xm + 18 * i;
+#pragma unroll
+    for (int j = 0; j < 6; ++j)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) z[d][j] = x[3 * j + d];
+  }
+}
+
+// tile (tr, tc) = blockIdx.y of E_ab: rows 6 tr.., columns 6 tc.. of Z_a^T A Z_b
+__global__ __launch_bounds__(kCoarseBlock) void k_coarse_E12(Sell<double> A, const int32_t* __restrict__ agg,
+                                                              const int32_t* __restrict__ pb,
+                                                              const int32_t* __restrict__ pn_ptr,
+                                                              const int32_t* __restrict__ pn,
+                                                              const float* __restrict__ dv,
+                                                              const float* __restrict__ xm, double* __restrict__ Eblk) {
+  const int p = blockIdx.x, tr = blockIdx.y >> 1, tc = blockIdx.y & 1;
+  const int b = pb[p];
+  double acc[36];
+#pragma unroll
+  for (int k = 0; k < 36; ++k) acc[k] = 0.0;
+  const int t1 = pn_ptr[p + 1];
+  for (int t = pn_ptr[p] + threadIdx.x; t < t1; t += kCoarseBlock) {
+    const int i = pn[t];
+    double zi[3][6];
+    zblock(dv, xm, i, tr, zi);
+    const int sl = i / A.rows, l = i % A.rows;
+    const int64_t base = (int64_t)A.ptr[sl] * A.rows;
+    const int len = A.len[i];
+    for (int k = 0; k < len; ++k) {
+      const int j = A.col[base + (int64_t)k * A.rows + l];
+      if (agg[j] != b) continue;
+      double v[9], zj[3][6];
+      aval_load(A.val, (int64_t)A.ptr[sl] + k, l, v);
+      zblock(dv, xm, j, tc, zj);
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+          // (A_ij Z_j)[r][q], then Z_i^T of it
+          const double m = v[3 * r] * zj[0][q] + v[3 * r + 1] * zj[1][q] + v[3 * r + 2] * zj[2][q];
+#pragma unroll
+          for (int u = 0; u < 6; ++u) acc[6 * u + q] += zi[r][u] * m;
+        }
+    }
+  }
+  block_reduce<36>(acc, Eblk + ((int64_t)p * 4 + blockIdx.y) * 36);
+}
+
+// dense E (column-major): E(12a + 6tr + r, 12b + 6tc + q) = tile (tr, tc) of pair p
+__global__ void k_coarse_scatter12(int32_t npair, const int32_t* __restrict__ pa, const int32_t* __restrict__ pb,
+                                   const double* __restrict__ Eblk, double* __restrict__ E, int32_t lda) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)npair * 144) return;
+  const int p = (int)(t / 144), e = (int)(t % 144);
+  const int tile = e / 36, tr = tile >> 1, tc = tile & 1, r = (e % 36) / 6, q = e % 6;
+  const int a = pa[p], b = pb[p];
+  E[(int64_t)(12 * b + 6 * tc + q) * lda + 12 * a + 6 * tr + r] = Eblk[t];
+}
+
+// A Z per node and touched aggregate, 3x12 FP32 per entry, every entry kept
+// (the learned modes do not vanish under A inside their aggregate); the
+// entries of node i are its symbolic slots ncnt[i].. (sc[i] = (ncnt[i], count))
+__global__ __launch_bounds__(kCoarseBlock) void k_coarse_W12(Sell<double> A, int32_t n,
+                                                              const int32_t* __restrict__ agg,
+                                                              const int32_t* __restrict__ ncnt,
+                                                              const int32_t* __restrict__ nagg,
+                                                              const float* __restrict__ dv,
+                                                              const float* __restrict__ xm, int2* __restrict__ sc,
+                                                              int32_t* __restrict__ aggW, float* __restrict__ W,
+                                                              int32_t* __restrict__ kept) {
+  const int64_t i = (int64_t)blockIdx.x * kCoarseBlock + threadIdx.x;
+  if (threadIdx.x == 0) {
+    const int64_t i0 = (int64_t)blockIdx.x * kCoarseBlock, i1 = i0 + kCoarseBlock < n ? i0 + kCoarseBlock : n;
+    kept[blockIdx.x] = ncnt[i1] - ncnt[i0];
+  }
+  if (i >= n) return;
+  const int e0 = ncnt[i], ne = ncnt[i + 1] - e0;
+  sc[i] = make_int2(e0, ne);
+  const int sl = (int)(i / A.rows), l = (int)(i % A.rows);
+  const int64_t base = (int64_t)A.ptr[sl] * A.rows;
+  const int len = A.len[i];
+  for (int t = 0; t < ne; ++t) {
+    const int b = nagg[e0 + t];
+    double m[3][12];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int q = 0; q < 12; ++q) m[r][q] = 0.0;
+    for (int k = 0; k < len; ++k) {
+      const int j = A.col[base + (int64_t)k * A.rows + l];
+      if (agg[j] != b) continue;
+      double v[9];
+      aval_load(A.val, (int64_t)A.ptr[sl] + k, l, v);
+#pragma unroll
+      for (int tb = 0; tb < 2; ++tb) {
+        double zj[3][6];
+        zblock(dv, xm, j, tb, zj);
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int q = 0; q < 6; ++q)
+            m[r][6 * tb + q] += v[3 * r] * zj[0][q] + v[3 * r + 1] * zj[1][q] + v[3 * r + 2] * zj[2][q];
+      }
+    }
+    aggW[e0 + t] = b;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int q = 0; q < 12; ++q) W[36 * (int64_t)(e0 + t) + 12 * r + q] = (float)m[r][q];
+  }
+}
+
+// c = Z^T (r - w) per aggregate, 12 sums (w == nullptr: Z^T r)
+__global__ __launch_bounds__(kCoarseBlock) void k_coarse_restrict12(const int32_t* __restrict__ aptr,
+                                                                     const int32_t* __restrict__ anodes,
+                                                                     const float* __restrict__ dv,
+                                                                     const float* __restrict__ xm,
+                                                                     const float* __restrict__ r,
+                                                                     const float* __restrict__ w,
+                                                                     double* __restrict__ c, const KrylovState* st) {
+  if (st->done || st->coarse_off) return;
+  const int a = blockIdx.x;
+  double acc[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) acc[k] = 0.0;
+  const int t1 = aptr[a + 1];
+  for (int t = aptr[a] + threadIdx.x; t < t1; t += kCoarseBlock) {
+    const int64_t i = anodes[t];
+    double rv[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) rv[d] = w ? (double)r[3 * i + d] - (double)w[3 * i + d] : (double)r[3 * i + d];
+    const double dx = dv[3 * i], dy = dv[3 * i + 1], dz = dv[3 * i + 2];
+    acc[0] += rv[0];
+    acc[1] += rv[1];
+    acc[2] += rv[2];
+    acc[3] += -dz * rv[1] + dy * rv[2];
+    acc[4] += dz * rv[0] - dx * rv[2];
+    acc[5] += -dy * rv[0] + dx * rv[1];
+    const float* x = xm + 18 * i;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) acc[6 + j] += (double)x[3 * j] * rv[0] + (double)x[3 * j + 1] * rv[1] +
+                                              (double)x[3 * j + 2] * rv[2];
+  }
+  __shared__ double tot[12];
+  block_reduce<12>(acc, tot);
+  if (threadIdx.x < 12) c[12 * a + threadIdx.x] = tot[threadIdx.x];
+}
+
+// Z_i y_a and (A Z)_i y with FP64 coarse values, accumulated in FP64
+__device__ __forceinline__ void zy12(const float* __restrict__ dv, const float* __restrict__ xm,
+                                     const double* __restrict__ ya, int64_t i, float (&o)[3]) {
+  const double dx = dv[3 * i], dy = dv[3 * i + 1], dz = dv[3 * i + 2];
+  double u0 = ya[0] + (ya[4] * dz - ya[5] * dy);
+  double u1 = ya[1] + (ya[5] * dx - ya[3] * dz);
+  double u2 = ya[2] + (ya[3] * dy - ya[4] * dx);
+  const float* x = xm + 18 * i;
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    u0 += (double)x[3 * j] * ya[6 + j];
+    u1 += (double)x[3 * j + 1] * ya[6 + j];
+    u2 += (double)x[3 * j + 2] * ya[6 + j];
+  }
+  o[0] = (float)u0;
+  o[1] = (float)u1;
+  o[2] = (float)u2;
+}
+
+__device__ __forceinline__ void wy12(const int2* __restrict__ sc, const int32_t* __restrict__ aggW,
+                                     const float* __restrict__ W, const double* __restrict__ y, int64_t i,
+                                     float (&o)[3]) {
+  double u[3] = {0.0, 0.0, 0.0};
+  const int2 r = sc[i];
+  for (int e = r.x; e < r.x + r.y; ++e) {
+    const float* w = W + 36 * (int64_t)e;
+    const double* yb = y + 12 * aggW[e];
+#pragma unroll
+    for (int rr = 0; rr < 3; ++rr)
+#pragma unroll
+      for (int q = 0; q < 12; ++q) u[rr] += (double)w[12 * rr + q] * yb[q];
+  }
+  o[0] = (float)u[0];
+  o[1] = (float)u[1];
+  o[2] = (float)u[2];
+}
+
+// z = Z y, r1 = r - (A Z) y
+__global__ __launch_bounds__(kCoarseBlock) void k_coarse_pre12(int32_t n, const int32_t* __restrict__ agg,
+                                                                const float* __restrict__ dv,
+                                                                const float* __restrict__ xm,
+                                                                const int2* __restrict__ sc,
+                                                                const int32_t* __restrict__ aggW,
+                                                                const float* __restrict__ W,
+                                                                const double* __restrict__ y,
+                                                                const float* __restrict__ r, float* __restrict__ z,
+                                                                float* __restrict__ r1, const KrylovState* st) {
+  if (st->done) return;
+  const bool off = st->coarse_off;
+  for (int64_t i = (int64_t)blockIdx.x * kCoarseBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kCoarseBlock) {
+    float zi[3] = {0.f, 0.f, 0.f}, ai[3] = {0.f, 0.f, 0.f};
+    if (!off) {
+      zy12(dv, xm, y + 12 * agg[i], i, zi);
+      wy12(sc, aggW, W, y, i, ai);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      z[3 * i + d] = zi[d];
+      r1[3 * i + d] = r[3 * i + d] - ai[d];
+    }
+  }
+}
+
+// z = z + Z y, az = w + (A Z) y, partials r.z and q_prev.z
+__global__ __launch_bounds__(kCoarseBlock) void k_coarse_post12(int32_t n, const int32_t* __restrict__ agg,
+                                                                 const float* __restrict__ dv,
+                                                                 const float* __restrict__ xm,
+                                                                 const int2* __restrict__ sc,
+                                                                 const int32_t* __restrict__ aggW,
+                                                                 const float* __restrict__ W,
+                                                                 const double* __restrict__ y, float* __restrict__ z,
+                                                                 const float* __restrict__ w, float* __restrict__ az,
+                                                                 const float* __restrict__ r,
+                                                                 const float* __restrict__ q, const KrylovState* st,
+                                                                 double* part) {
+  if (st->done) return;
+  const bool off = st->coarse_off;
+  double d[2] = {0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kCoarseBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kCoarseBlock) {
+    float zi[3] = {0.f, 0.f, 0.f}, ai[3] = {0.f, 0.f, 0.f};
+    if (!off) {
+      zy12(dv, xm, y + 12 * agg[i], i, zi);
+      wy12(sc, aggW, W, y, i, ai);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float zn = z[3 * i + k] + zi[k];
+      z[3 * i + k] = zn;
+      az[3 * i + k] = w[3 * i + k] + ai[k];
+      d[0] += (double)zn * (double)r[3 * i + k];
+      d[1] += (double)zn * (double)q[3 * i + k];
+    }
+  }
+  __shared__ double tot[2];
+  block_reduce<2>(d, tot);
+  if (threadIdx.x < 2) part[threadIdx.x * kMaxPartials + blockIdx.x] = tot[threadIdx.x];
+}
+
 }  // namespace
 
 void coarse_build_W(const Sell<double>& A, const CoarseOp& op, int32_t n, const int32_t* ncnt, const int32_t* nagg,
                     int2* sc, int32_t* aggW, float* W, int32_t* kept, hipStream_t s) {
-  if (n > 0)
-    k_coarse_W<<<(n + kCoarseBlock - 1) / kCoarseBlock, kCoarseBlock, 0, s>>>(A, n, op.agg, ncnt, nagg, op.dv, sc, aggW,
-                                                                              W, kept);
+  if (n <= 0) return;
+  const int g = (n + kCoarseBlock - 1) / kCoarseBlock;
+  if (op.nm == 12)
+    k_coarse_W12<<<g, kCoarseBlock, 0, s>>>(A, n, op.agg, ncnt, nagg, op.dv, op.xm, sc, aggW, W, kept);
+  else
+    k_coarse_W<<<g, kCoarseBlock, 0, s>>>(A, n, op.agg, ncnt, nagg, op.dv, sc, aggW, W, kept);
 }
 
 static int coarse_vec_grid(int64_t n) {
@@ -497,10 +795,17 @@ static int coarse_vec_grid(int64_t n) {
 
 void coarse_sym_pre(const CoarseOp& op, int32_t n, const int2* sc, const int32_t* aggW, const float* W,
                     const float* r, float* z, float* r1, const KrylovState* st, hipStream_t s) {
-  k_coarse_restrict<<<op.n_agg, kCoarseBlock, 0, s>>>(op.aptr, op.anodes, op.dv, r, op.cv, st);
-  k_coarse_gemv<<<(op.nc + kCoarseWaves - 1) / kCoarseWaves, kCoarseBlock, 0, s>>>(op.nc, op.ldw, op.W, op.cv, op.yv,
-                                                                                   st);
-  k_coarse_pre<<<coarse_vec_grid(n), kCoarseBlock, 0, s>>>(n, op.agg, op.dv, sc, aggW, W, op.yv, r, z, r1, st);
+  const int gr = (op.nc + kCoarseWaves - 1) / kCoarseWaves;
+  if (op.nm == 12) {
+    k_coarse_restrict12<<<op.n_agg, kCoarseBlock, 0, s>>>(op.aptr, op.anodes, op.dv, op.xm, r, nullptr, op.cv64, st);
+    k_coarse_gemv64<<<gr, kCoarseBlock, 0, s>>>(op.nc, op.ldw, op.W64, op.cv64, op.yv64, st);
+    k_coarse_pre12<<<coarse_vec_grid(n), kCoarseBlock, 0, s>>>(n, op.agg, op.dv, op.xm, sc, aggW, W, op.yv64, r, z,
+                                                               r1, st);
+  } else {
+    k_coarse_restrict<<<op.n_agg, kCoarseBlock, 0, s>>>(op.aptr, op.anodes, op.dv, r, op.cv, st);
+    k_coarse_gemv<<<gr, kCoarseBlock, 0, s>>>(op.nc, op.ldw, op.W, op.cv, op.yv, st);
+    k_coarse_pre<<<coarse_vec_grid(n), kCoarseBlock, 0, s>>>(n, op.agg, op.dv, sc, aggW, W, op.yv, r, z, r1, st);
+  }
 }
 
 void coarse_sym_add(int64_t n3, const float* t, float* z, const KrylovState* st, hipStream_t s) {
@@ -510,11 +815,18 @@ void coarse_sym_add(int64_t n3, const float* t, float* z, const KrylovState* st,
 int coarse_sym_post(const CoarseOp& op, int32_t n, const int2* sc, const int32_t* aggW, const float* W,
                     const float* r, const float* w, float* z, float* az, const float* q, const KrylovState* st,
                     double* part, hipStream_t s) {
-  k_coarse_restrict2<<<op.n_agg, kCoarseBlock, 0, s>>>(op.aptr, op.anodes, op.dv, r, w, op.cv, st);
-  k_coarse_gemv<<<(op.nc + kCoarseWaves - 1) / kCoarseWaves, kCoarseBlock, 0, s>>>(op.nc, op.ldw, op.W, op.cv, op.yv,
-                                                                                   st);
+  const int gr = (op.nc + kCoarseWaves - 1) / kCoarseWaves;
   const int g = coarse_vec_grid(n);
-  k_coarse_post<<<g, kCoarseBlock, 0, s>>>(n, op.agg, op.dv, sc, aggW, W, op.yv, z, w, az, r, q, st, part);
+  if (op.nm == 12) {
+    k_coarse_restrict12<<<op.n_agg, kCoarseBlock, 0, s>>>(op.aptr, op.anodes, op.dv, op.xm, r, w, op.cv64, st);
+    k_coarse_gemv64<<<gr, kCoarseBlock, 0, s>>>(op.nc, op.ldw, op.W64, op.cv64, op.yv64, st);
+    k_coarse_post12<<<g, kCoarseBlock, 0, s>>>(n, op.agg, op.dv, op.xm, sc, aggW, W, op.yv64, z, w, az, r, q, st,
+                                               part);
+  } else {
+    k_coarse_restrict2<<<op.n_agg, kCoarseBlock, 0, s>>>(op.aptr, op.anodes, op.dv, r, w, op.cv, st);
+    k_coarse_gemv<<<gr, kCoarseBlock, 0, s>>>(op.nc, op.ldw, op.W, op.cv, op.yv, st);
+    k_coarse_post<<<g, kCoarseBlock, 0, s>>>(n, op.agg, op.dv, sc, aggW, W, op.yv, z, w, az, r, q, st, part);
+  }
   return g;
 }
 
@@ -528,10 +840,21 @@ int coarse_fcg_pq(int64_t n3, const float* z, const float* az, float* p, float* 
 void coarse_assemble(const Sell<double>& A, const CoarseOp& op, int32_t npair, const int32_t* pa, const int32_t* pb,
                      const int32_t* pn_ptr, const int32_t* pn, double* Eblk, double* E, int32_t lda, hipStream_t s) {
   if (npair <= 0) return;
-  k_coarse_E<<<(unsigned)npair, kCoarseBlock, 0, s>>>(A, op.agg, pb, pn_ptr, pn, op.dv, Eblk);
-  const int64_t ne = (int64_t)npair * 36;
-  k_coarse_scatter<<<(unsigned)((ne + 255) / 256), 256, 0, s>>>(npair, pa, pb, Eblk, E, lda);
+  if (op.nm == 12) {
+    k_coarse_E12<<<dim3((unsigned)npair, 4), kCoarseBlock, 0, s>>>(A, op.agg, pb, pn_ptr, pn, op.dv, op.xm, Eblk);
+    const int64_t ne = (int64_t)npair * 144;
+    k_coarse_scatter12<<<(unsigned)((ne + 255) / 256), 256, 0, s>>>(npair, pa, pb, Eblk, E, lda);
+  } else {
+    k_coarse_E<<<(unsigned)npair, kCoarseBlock, 0, s>>>(A, op.agg, pb, pn_ptr, pn, op.dv, Eblk);
+    const int64_t ne = (int64_t)npair * 36;
+    k_coarse_scatter<<<(unsigned)((ne + 255) / 256), 256, 0, s>>>(npair, pa, pb, Eblk, E, lda);
+  }
   k_coarse_unit_pivots<<<(op.nc + 255) / 256, 256, 0, s>>>(op.nc, E, lda);
+}
+
+void coarse_inverse_f64(const CoarseOp& op, const double* Ei, int32_t lda, hipStream_t s) {
+  const int64_t ne = (int64_t)op.nc * op.ldw;
+  k_coarse_inv64<<<(unsigned)((ne + 255) / 256), 256, 0, s>>>(op.nc, Ei, lda, op.W64, op.ldw);
 }
 
 void coarse_inverse_f32(const CoarseOp& op, const double* Ei, int32_t lda, hipStream_t s) {

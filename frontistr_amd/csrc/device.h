@@ -124,6 +124,9 @@ struct KrylovState {
   // true / recurrence residual seen at the last replacement: near convergence
   // the FP32 recurrence runs ahead, so it claims convergence at resid * gap <= tol
   double gap;
+  // the learned step is dropped when |r|^2 at a replacement exceeds drop2 times the smallest
+  // FP64 |r|^2 seen so far (rr_min)
+  double drop2, rr_min;
 };
 
 constexpr int kMaxPartials = 65536;  // per reduction component
@@ -267,20 +270,27 @@ void fcg_spmv_store(const Sell<float>& a32, const Sell<double>& a64, bool fp64_v
 // Two-level correction (coarse.hip): piecewise rigid-body modes on geometric
 // aggregates, z += Z E^-1 Z^T r.
 struct CoarseOp {
-  int32_t n_agg = 0, nc = 0, ldw = 0;  // aggregates, coarse unknowns (6 per aggregate), row stride of W
+  int32_t n_agg = 0, nc = 0, ldw = 0;  // aggregates, coarse unknowns (nm per aggregate), row stride of W
+  int32_t nm = 6;                      // modes per aggregate: 6 rigid-body, or 12 with the learned ones
   const int32_t* agg = nullptr;        // [NP] aggregate of each node, -1 for external nodes
   const int32_t* aptr = nullptr;       // [n_agg+1] into anodes
   const int32_t* anodes = nullptr;     // nodes of each aggregate, ascending
   const float* dv = nullptr;           // [3N] scaled offsets from the aggregate centroid
-  float* W = nullptr;                  // E^-1, FP32, nc x ldw row-major
+  const float* xm = nullptr;           // nm = 12: [N][6][3] the learned modes' values at each node
+  float* W = nullptr;                  // E^-1, FP32, nc x ldw row-major (six modes)
   float* cv = nullptr;                 // [ldw] Z^T r (entries nc..ldw-1 stay 0)
   float* yv = nullptr;                 // [nc] E^-1 Z^T r
+  double* W64 = nullptr;               // twelve modes: E^-1 in FP64 (nc x ldw), and c, y in FP64
+  double* cv64 = nullptr;
+  double* yv64 = nullptr;
 };
-// E = Z^T A Z: the 6x6 block of every coupled aggregate pair (pa[p], pb[p]), summed
+// E = Z^T A Z: the nm x nm block of every coupled aggregate pair (pa[p], pb[p]), summed
 // over the pair's nodes pn[pn_ptr[p]..pn_ptr[p+1]), then scattered into the dense E
+// (Eblk: nm*nm doubles per pair)
 void coarse_assemble(const Sell<double>& A, const CoarseOp& op, int32_t npair, const int32_t* pa, const int32_t* pb,
                      const int32_t* pn_ptr, const int32_t* pn, double* Eblk, double* E, int32_t lda, hipStream_t s);
 void coarse_inverse_f32(const CoarseOp& op, const double* Ei, int32_t lda, hipStream_t s);
+void coarse_inverse_f64(const CoarseOp& op, const double* Ei, int32_t lda, hipStream_t s);
 // The symmetric two-level step (W = A Z per node and touched aggregate):
 // build W from the symbolic node-major lists ncnt/nagg into the compacted
 // sc (per node: first slot, count) / aggW / W, dropping the entries that are
@@ -317,6 +327,19 @@ void net_to_bf16(const float* Wm, __bf16* Wb, hipStream_t s);
 void net_bce(int64_t n, const float* Y, const float* t, float* dY, double* part, double* out, hipStream_t s);
 void net_mask(int64_t n, const float* Y, uint8_t* mask, hipStream_t s);
 void net_material(int64_t n, const double* D, float* m, double* part, double* sum, hipStream_t s);
+
+// learned coarse modes (per-aggregate linear autoencoders on the solver's slow
+// errors): samples S[(3i+d)*16+s] = v - x; per (aggregate, sample) the rigid-body
+// restriction Z^T e (rhs, 6 per pair) and per aggregate Z^T Z (ztz, 36); the part
+// beyond the rigid-body modes R = S - Z c; the Gram matrices G_a = R_a^T R_a (16x16,
+// BF16 MFMA, FP32 accumulation); the modes xm[18i+3j+d] = R V[agg(i)] (V: 16x6 per aggregate)
+void modes_random(int64_t n3, uint64_t seed, double* v, hipStream_t st);
+void modes_sample(int64_t n3, const double* v, const double* x, float* S, int s, hipStream_t st);
+void modes_rigid(int32_t n_agg, int ns, const int32_t* aptr, const int32_t* anodes, const float* dv, const float* S,
+                 double* rhs, double* ztz, hipStream_t st);
+void modes_resid(int32_t n, const int32_t* agg, const float* dv, const double* c, float* S, hipStream_t st);
+void modes_gram(int32_t n_agg, const int32_t* aptr, const int32_t* anodes, const float* S, float* G, hipStream_t st);
+void modes_make(int32_t n, const int32_t* agg, const float* S, const float* V, float* xm, hipStream_t st);
 
 // r32 = (float) r64 over n entries
 void launch_to_f32(int64_t n, const double* x, float* y, hipStream_t s);

@@ -269,13 +269,16 @@ __global__ __launch_bounds__(kTailThreads) void k_tail_fcg_rel(KrylovState* st, 
     sum_partials<1>(part, nparts, tot);
     if (threadIdx.x == 0) {
       // the learned two-level step is dropped for the rest of the solve once
-      // the FP64 residual grows (4x in |r|^2) between two replacements: the
-      // iterations go on as the plain SSOR flexible CG
-      if (!st->learned_off && tot[0] > 4.0 * st->rr_ref) {
+      // the FP64 residual grows to drop2 (in |r|^2) times the smallest one seen
+      // at a replacement (the residual of CG is not monotone, so a transient
+      // rise is no failure; a divergence is): the iterations go on as the
+      // plain SSOR flexible CG
+      if (!st->learned_off && tot[0] > st->drop2 * st->rr_min) {
         st->learned_off = 1;
         st->coarse_off = 1;
         st->fallback_iter = st->iter;
       }
+      st->rr_min = fmin(st->rr_min, tot[0]);
       const double rec = st->resid;
       st->dnrm2 = tot[0];
       st->rr_ref = tot[0];

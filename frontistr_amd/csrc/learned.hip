@@ -441,6 +441,164 @@ int grid_for(int64_t n, int per) {
   return (int)(g < 1 ? 1 : (g > 4096 ? 4096 : g));
 }
 
+// ---- learned coarse modes: per-aggregate linear autoencoders -----------------------------
+// Samples: the solver's own slow errors e_s = A^-1 r_s, stored node-major with
+// the sample index inner, S[(3 i + d) * 16 + s] (FP32, up to 16 samples; unused
+// slots zero).  Per aggregate a linear autoencoder x -> W^T x -> W W^T x with
+// k columns in W is fitted to the samples' part beyond the rigid-body modes;
+// its optimum is the span of the top-k eigenvectors of the aggregate's Gram
+// matrix G = R^T R (16 x 16), which a BF16 MFMA pass accumulates in FP32.  The
+// decoder columns W = R V diag(lambda)^-1/2 become the aggregate's extra
+// coarse modes.
+
+// v = a seeded pseudo-random vector, uniform in [-1, 1) (splitmix64 of seed and index)
+__global__ void k_modes_random(int64_t n3, uint64_t seed, double* __restrict__ v) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n3; k += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t z = seed * 0x9E3779B97F4A7C15ull + (uint64_t)k * 0xD1B54A32D192ED03ull + 0x632BE59BD9B4E019ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    v[k] = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+  }
+}
+
+// S[:, s] = v - x (FP32)
+__global__ void k_modes_sample(int64_t n3, const double* __restrict__ v, const double* __restrict__ x,
+                               float* __restrict__ S, int s) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n3; k += (int64_t)gridDim.x * blockDim.x)
+    S[16 * k + s] = (float)(v[k] - x[k]);
+}
+
+// per (aggregate a = blockIdx.x, sample s = blockIdx.y): Z_a^T e_s over the six
+// rigid-body modes (FP64, fixed order); s == 0 also writes Z_a^T Z_a (36)
+__global__ __launch_bounds__(kNT) void k_modes_rigid(const int32_t* __restrict__ aptr,
+                                                      const int32_t* __restrict__ anodes,
+                                                      const float* __restrict__ dv, const float* __restrict__ S,
+                                                      double* __restrict__ rhs, double* __restrict__ ztz) {
+  const int a = blockIdx.x, s = blockIdx.y;
+  double acc[6 + 21];
+#pragma unroll
+  for (int k = 0; k < 27; ++k) acc[k] = 0.0;
+  for (int t = aptr[a] + threadIdx.x; t < aptr[a + 1]; t += kNT) {
+    const int64_t i = anodes[t];
+    const double ex = S[16 * (3 * i) + s], ey = S[16 * (3 * i + 1) + s], ez = S[16 * (3 * i + 2) + s];
+    const double dx = dv[3 * i], dy = dv[3 * i + 1], dz = dv[3 * i + 2];
+    // Z_i columns: e_x, e_y, e_z, (0,-dz,dy), (dz,0,-dx), (-dy,dx,0)
+    const double z[6][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, -dz, dy}, {dz, 0, -dx}, {-dy, dx, 0}};
+#pragma unroll
+    for (int q = 0; q < 6; ++q) acc[q] += z[q][0] * ex + z[q][1] * ey + z[q][2] * ez;
+    if (s == 0) {
+      int k = 6;
+#pragma unroll
+      for (int q = 0; q < 6; ++q)
+#pragma unroll
+        for (int u = q; u < 6; ++u) acc[k++] += z[q][0] * z[u][0] + z[q][1] * z[u][1] + z[q][2] * z[u][2];
+    }
+  }
+  __shared__ double red[27][kNT / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 27; ++k) {
+    double v = acc[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) red[k][wave] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 27) {
+    double v = 0.0;
+#pragma unroll
+    for (int w = 0; w < kNT / 64; ++w) v += red[threadIdx.x][w];
+    if (threadIdx.x < 6) {
+      rhs[((int64_t)a * 16 + s) * 6 + threadIdx.x] = v;
+    } else if (s == 0) {
+      // unpack the upper triangle into the full 6 x 6
+      int q = 0, u = 0, k = 6;
+      while (k < (int)threadIdx.x) {
+        if (++u == 6) u = ++q;
+        ++k;
+      }
+      ztz[(int64_t)a * 36 + 6 * q + u] = v;
+      ztz[(int64_t)a * 36 + 6 * u + q] = v;
+    }
+  }
+}
+
+// R = S - Z_rigid c: the samples' part beyond the rigid-body modes of each
+// node's aggregate (c[(a * 16 + s) * 6 + q], zero for unused samples)
+__global__ void k_modes_resid(int32_t n, const int32_t* __restrict__ agg, const float* __restrict__ dv,
+                              const double* __restrict__ c, float* __restrict__ S) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int a = agg[i];
+    const double dx = dv[3 * i], dy = dv[3 * i + 1], dz = dv[3 * i + 2];
+    for (int s = 0; s < 16; ++s) {
+      const double* cs = c + ((int64_t)a * 16 + s) * 6;
+      const double u0 = cs[0] + (cs[4] * dz - cs[5] * dy);
+      const double u1 = cs[1] + (cs[5] * dx - cs[3] * dz);
+      const double u2 = cs[2] + (cs[3] * dy - cs[4] * dx);
+      S[16 * (3 * i) + s] = (float)((double)S[16 * (3 * i) + s] - u0);
+      S[16 * (3 * i + 1) + s] = (float)((double)S[16 * (3 * i + 1) + s] - u1);
+      S[16 * (3 * i + 2) + s] = (float)((double)S[16 * (3 * i + 2) + s] - u2);
+    }
+  }
+}
+
+// G_a = R_a^T R_a (16 x 16) with v_mfma_f32_16x16x32_bf16: the A operand of a
+// 32-dof chunk, A[i = l%16][k = 8(l/16)+e] = R[k][i], equals its B operand
+// B[k][j = l%16] = R[k][j] lane for lane, so one load feeds both.  Four waves
+// take every fourth chunk; their tiles are summed in a fixed order.
+__global__ __launch_bounds__(kNT) void k_modes_gram(const int32_t* __restrict__ aptr,
+                                                     const int32_t* __restrict__ anodes, const float* __restrict__ S,
+                                                     float* __restrict__ G) {
+  const int a = blockIdx.x, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int t0 = aptr[a], nd = 3 * (aptr[a + 1] - t0);
+  const int g = lane >> 4, col = lane & 15;
+  floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+  for (int kb = wave; kb * 32 < nd; kb += kNT / 64) {
+    bf16x8 v;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int k = kb * 32 + 8 * g + e;
+      float x = 0.f;
+      if (k < nd) {
+        const int64_t i = anodes[t0 + k / 3];
+        x = S[16 * (3 * i + k % 3) + col];
+      }
+      v[e] = (__bf16)x;
+    }
+    acc = mfma(v, v, acc);
+  }
+  __shared__ float red[kNT / 64][16][16];
+#pragma unroll
+  for (int vv = 0; vv < 4; ++vv) red[wave][4 * g + vv][col] = acc[vv];
+  __syncthreads();
+  if (threadIdx.x < 256) {
+    const int i = threadIdx.x / 16, j = threadIdx.x % 16;
+    float sum = 0.f;
+#pragma unroll
+    for (int w = 0; w < kNT / 64; ++w) sum += red[w][i][j];
+    G[(int64_t)a * 256 + 16 * i + j] = sum;
+  }
+}
+
+// xm[18 i + 3 j + d] = sum_s R[i][d][s] V[a][s][j]  (the fitted decoder, mode j of node i)
+__global__ void k_modes_make(int32_t n, const int32_t* __restrict__ agg, const float* __restrict__ S,
+                             const float* __restrict__ V, float* __restrict__ xm) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float* va = V + (int64_t)agg[i] * 96;
+    for (int d = 0; d < 3; ++d) {
+      float o[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      for (int s = 0; s < 16; ++s) {
+        const float r = S[16 * (3 * i + d) + s];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) o[j] += r * va[6 * s + j];
+      }
+#pragma unroll
+      for (int j = 0; j < 6; ++j) xm[18 * i + 3 * j + d] = o[j];
+    }
+  }
+}
+
 }  // namespace
 
 // ---- launchers ---------------------------------------------------------------------
@@ -486,6 +644,26 @@ void net_bce(int64_t n, const float* Y, const float* t, float* dY, double* part,
 }
 void net_mask(int64_t n, const float* Y, uint8_t* mask, hipStream_t s) {
   k_net_mask<<<grid_for(n, 256), 256, 0, s>>>(n, Y, mask);
+}
+
+void modes_random(int64_t n3, uint64_t seed, double* v, hipStream_t st) {
+  k_modes_random<<<grid_for(n3, 256), 256, 0, st>>>(n3, seed, v);
+}
+void modes_sample(int64_t n3, const double* v, const double* x, float* S, int s, hipStream_t st) {
+  k_modes_sample<<<grid_for(n3, 256), 256, 0, st>>>(n3, v, x, S, s);
+}
+void modes_rigid(int32_t n_agg, int ns, const int32_t* aptr, const int32_t* anodes, const float* dv, const float* S,
+                 double* rhs, double* ztz, hipStream_t st) {
+  k_modes_rigid<<<dim3((unsigned)n_agg, (unsigned)ns), kNT, 0, st>>>(aptr, anodes, dv, S, rhs, ztz);
+}
+void modes_resid(int32_t n, const int32_t* agg, const float* dv, const double* c, float* S, hipStream_t st) {
+  k_modes_resid<<<grid_for(n, 256), 256, 0, st>>>(n, agg, dv, c, S);
+}
+void modes_gram(int32_t n_agg, const int32_t* aptr, const int32_t* anodes, const float* S, float* G, hipStream_t st) {
+  k_modes_gram<<<(unsigned)n_agg, kNT, 0, st>>>(aptr, anodes, S, G);
+}
+void modes_make(int32_t n, const int32_t* agg, const float* S, const float* V, float* xm, hipStream_t st) {
+  k_modes_make<<<grid_for(n, 256), 256, 0, st>>>(n, agg, S, V, xm);
 }
 
 void net_material(int64_t n, const double* D, float* m, double* part, double* sum, hipStream_t s) {

@@ -64,7 +64,7 @@ enum {
   I_ITERLOG = 21, I_TIMELOG = 22, I_NCOLOR_IN = 34, I_MAXRECYCLE = 35, I_SOLVER_OPT = 41,
   I_MPC_METHOD = 13, I_FLAG_CONVERGED = 81, I_FLAG_DIVERGED = 82, I_FLAG_MPCMATVEC = 83, I_NRECYCLE = 96,
   I_FLAG_NUMFACT = 97, I_FLAG_SYMBFACT = 98,
-  R_RESID = 1, R_SIGMA_DIAG = 2, R_SOLVER_OPT = 41,
+  R_RESID = 1, R_SIGMA_DIAG = 2, R_SOLVER_OPT = 41, R_SOLVER_OPT2 = 42,
 };
 // SOLVER_OPT(1) (Iarray(41), unused by the reference's CG/BiCGSTAB with
 // PRECOND 1-3) = 1 selects the flexible CG with FP32 operators;
@@ -242,13 +242,18 @@ struct Context {
   // FP32 copies for the flexible CG (rounded from the FP64 storage)
   bool a32_valid = false, ssor32_valid = false, diag32_valid = false;
   DMem alu_ssor32, alu_diag32, vr32, vz32, vp32, vq32;
-  // learned aggregation: attached network, node stiffness feature, the network's outputs and
-  // the nodes it splits off (mask, with a hash of the mask the aggregation was built with)
+  // the node classifier network (hecmw33_net_*): attached network and its static node features
   hecmw33_net* net = nullptr;
   bool mat_valid = false;
   DMem vmat, scal, vfeat, vnety, vmask;  // vfeat: BF16 static node features, vnety: network outputs (3 per node)
-  std::vector<uint8_t> coarse_mask;      // empty: cells only
-  uint64_t coarse_mask_hash = 0, coarse_sym_mask_hash = 0;
+  // learned coarse modes (Iarray 41 = 2): per-aggregate linear autoencoders fitted to the
+  // solver's own slow errors; cg_xm holds the fitted modes ([N][6][3] FP32), valid for the
+  // current values and aggregation
+  bool modes_valid = false;
+  int32_t modes_kept = 0;          // learned modes with a usable eigenvalue, over all aggregates
+  int32_t modes_samples = 0, modes_iters = 0;  // error samples and FCG iterations per sample of the last fit
+  double modes_harvest_s = 0.0, modes_fit_s = 0.0;
+  DMem cg_xm, ml_S, ml_v, ml_bsave, ml_xsave, ml_rhs, ml_ztz, ml_c, ml_G, ml_V;
   // Krylov vectors and state
   DMem vb, vx, vr, vz, vp, vq, vrt, vs, vt, vv, vpt, vst, vzp, vzt;
   DMem st, part, hist;
@@ -257,7 +262,6 @@ struct Context {
   int32_t last_nreliable = 0;  // FP64 residual replacements of the last flexible CG solve
   int32_t last_fallback_iter = 0;  // iteration at which the last learned solve dropped its step (0: kept)
   int32_t last_learned_off = 1;    // the last solve ran without the learned step (not learned, or dropped)
-  int32_t coarse_pieces = 0;       // aggregates split off by the network in the current aggregation
   bool last_spmv64 = false;      // the last flexible CG used the FP64 matrix values in q = A p
   // two-level correction: node coordinates (hecMESH%node), aggregation, coarse inverse
   std::vector<double> h_coord;   // 3*NP
@@ -274,6 +278,9 @@ struct Context {
   int32_t coarse_npair = 0;       // coupled aggregate pairs (blocks of E)
   bool coarse_sym_use = false;    // this solve takes the symmetric two-level step
   bool coarse_w_valid = false;    // A Z (per node and touched aggregate) built for the current values
+  int32_t last_modes_nm = 0;      // modes per aggregate of the last two-level solve
+  int32_t coarse_nm_num = 0;      // modes per aggregate of the E^-1 built (6 or 12)
+  int32_t coarse_w_nm = 0;        // modes per aggregate of the A Z built
   int64_t coarse_nw = 0;          // node-aggregate entries of A Z
   DMem cg_agg, cg_aptr, cg_anodes, cg_dv, cg_pa, cg_pb, cg_pnptr, cg_pn, cg_Eblk, cg_E, cg_W, cg_cv, cg_yv, cg_info;
   DMem cg_ncnt, cg_nagg, cg_AZ, cg_sc, cg_aggW, cg_kept, vr1_32, vt32, vw32, vaz32;
@@ -736,24 +743,6 @@ static int32_t features_setup(Context& c) {
   return 0;
 }
 
-// The network's mask for this system: which nodes leave their cell.  The
-// aggregation is rebuilt only when the mask differs from the one it was built
-// with (a hash over the mask bytes).
-static int32_t learned_mask(Context& c) {
-  if (int32_t e = features_setup(c)) return e;
-  net_forward(c.N, c.net->Wb.as<__bf16>(), c.vfeat.as<__bf16>(), nullptr, nullptr, nullptr, c.vnety.as<float>(),
-              c.stream);
-  net_mask(c.N, c.vnety.as<float>(), c.vmask.as<uint8_t>(), c.stream);
-  c.coarse_mask.resize(c.N);
-  HIPC(hipMemcpyAsync(c.coarse_mask.data(), c.vmask.p, (size_t)c.N, hipMemcpyDeviceToHost, c.stream));
-  HIPC(hipGetLastError());
-  HIPC(hipStreamSynchronize(c.stream));
-  uint64_t h = 1469598103934665603ull;  // FNV-1a
-  for (uint8_t v : c.coarse_mask) h = (h ^ v) * 1099511628211ull;
-  c.coarse_mask_hash = h ? h : 1;
-  return 0;
-}
-
 static int32_t precond_setup(Context& c, int32_t* iarray, const double* rarray) {
   if (IA(iarray, I_ITERPREMAX) <= 0) return 0;
   const int32_t pc = IA(iarray, I_PRECOND);
@@ -769,8 +758,9 @@ static int32_t precond_setup(Context& c, int32_t* iarray, const double* rarray) 
     set_error("PRECOND not provided by the GPU library (1, 2: SSOR; 3: DIAG)");
     return HECMW33_ERROR_INCONS_PC;
   }
-  // the two-level correction's coarse inverse is part of the numeric set-up
+  // the two-level correction's coarse inverse and its learned modes are part of the numeric set-up
   c.coarse_num = false;
+  c.modes_valid = false;
   IA(iarray, I_FLAG_SYMBFACT) = 0;
   IA(iarray, I_FLAG_NUMFACT) = 0;
   return 0;
@@ -931,55 +921,6 @@ static int32_t coarse_symbolic(Context& c, int32_t target) {
     set_error("coarse correction: no aggregation within the size limit");
     return HECMW33_ERROR_ARGS;
   }
-  // learned refinement: the nodes the network marks leave their cell, one
-  // aggregate per connected piece (matrix-graph neighbours that are marked and
-  // in the same cell); pieces of at least kPieceMinNodes nodes, largest first
-  // within kCoarseAggMax; cells left empty are dropped
-  c.coarse_pieces = 0;
-  if ((int32_t)c.coarse_mask.size() == N) {
-    const uint8_t* mk = c.coarse_mask.data();
-    std::vector<int32_t> up(N);
-    for (int32_t i = 0; i < N; ++i) up[i] = i;
-    auto find = [&](int32_t x) {
-      while (up[x] != x) x = up[x] = up[up[x]];
-      return x;
-    };
-    for (int32_t i = 0; i < N; ++i) {
-      if (!mk[i]) continue;
-      for (int part = 0; part < 2; ++part) {
-        const int32_t* ix = part ? iU : iL;
-        const int32_t* it = part ? tU : tL;
-        for (int32_t q = ix[i]; q < ix[i + 1]; ++q) {
-          const int32_t j = it[q] - 1;
-          if (j >= N || !mk[j] || agg[j] != agg[i]) continue;
-          const int32_t a = find(i), b = find(j);
-          if (a != b) up[std::max(a, b)] = std::min(a, b);
-        }
-      }
-    }
-    std::vector<int32_t> size(N, 0);
-    for (int32_t i = 0; i < N; ++i)
-      if (mk[i]) ++size[find(i)];
-    std::vector<int32_t> roots;
-    for (int32_t i = 0; i < N; ++i)
-      if (mk[i] && up[i] == i && size[i] >= kPieceMinNodes) roots.push_back(i);
-    std::stable_sort(roots.begin(), roots.end(), [&](int32_t a, int32_t b) { return size[a] > size[b]; });
-    if ((int64_t)roots.size() > kCoarseAggMax - na) roots.resize(std::max(0, kCoarseAggMax - na));
-    std::vector<int32_t> piece(N, -1);
-    for (size_t k = 0; k < roots.size(); ++k) piece[roots[k]] = na + (int32_t)k;
-    for (int32_t i = 0; i < N; ++i)
-      if (mk[i] && piece[find(i)] >= 0) agg[i] = piece[find(i)];
-    const int32_t na_all = na + (int32_t)roots.size();
-    std::vector<int32_t> used(na_all, -1);
-    int32_t nn = 0;
-    for (int32_t i = 0; i < N; ++i)
-      if (used[agg[i]] < 0) used[agg[i]] = 0;
-    for (int32_t a = 0; a < na_all; ++a)
-      if (used[a] == 0) used[a] = nn++;
-    for (int32_t i = 0; i < N; ++i) agg[i] = used[agg[i]];
-    c.coarse_pieces = (int32_t)roots.size();
-    na = nn;
-  }
   std::vector<int32_t> cnt(na + 1, 0);
   for (int32_t i = 0; i < N; ++i) ++cnt[agg[i] + 1];
   for (int32_t a = 0; a < na; ++a) cnt[a + 1] += cnt[a];
@@ -1092,8 +1033,8 @@ static int32_t coarse_symbolic(Context& c, int32_t target) {
 
   CoarseOp& op = c.coarse;
   op.n_agg = na;
-  op.nc = 6 * na;
-  op.ldw = (op.nc + 63) / 64 * 64;
+  op.nc = 0;
+  op.ldw = 0;
   c.coarse_npair = npair;
   c.coarse_nw = ncnt[N];
   if (c.cg_ncnt.ensure(sizeof(int32_t) * (N + 1)) || c.cg_nagg.ensure(sizeof(int32_t) * std::max<int64_t>(ncnt[N], 1)))
@@ -1106,9 +1047,6 @@ static int32_t coarse_symbolic(Context& c, int32_t target) {
       c.cg_anodes.ensure(sizeof(int32_t) * N) || c.cg_dv.ensure(sizeof(float) * 3 * (size_t)N) ||
       c.cg_pa.ensure(sizeof(int32_t) * npair) || c.cg_pb.ensure(sizeof(int32_t) * npair) ||
       c.cg_pnptr.ensure(sizeof(int32_t) * (npair + 1)) || c.cg_pn.ensure(sizeof(int32_t) * std::max<int64_t>(npn, 1)) ||
-      c.cg_Eblk.ensure(sizeof(double) * 36 * (size_t)npair) ||
-      c.cg_E.ensure(sizeof(double) * (size_t)op.nc * op.nc) || c.cg_W.ensure(sizeof(float) * (size_t)op.nc * op.ldw) ||
-      c.cg_cv.ensure(sizeof(float) * op.ldw) || c.cg_yv.ensure(sizeof(float) * op.ldw) ||
       c.cg_info.ensure(sizeof(int32_t) * 2))
     return HECMW33_ERROR_DEVICE;
   HIPC(hipMemcpyAsync(c.cg_agg.p, agg.data(), sizeof(int32_t) * NP, hipMemcpyHostToDevice, c.stream));
@@ -1119,27 +1057,42 @@ static int32_t coarse_symbolic(Context& c, int32_t target) {
   HIPC(hipMemcpyAsync(c.cg_pb.p, pb.data(), sizeof(int32_t) * npair, hipMemcpyHostToDevice, c.stream));
   HIPC(hipMemcpyAsync(c.cg_pnptr.p, pn_ptr.data(), sizeof(int32_t) * (npair + 1), hipMemcpyHostToDevice, c.stream));
   if (npn) HIPC(hipMemcpyAsync(c.cg_pn.p, pn.data(), sizeof(int32_t) * npn, hipMemcpyHostToDevice, c.stream));
-  HIPC(hipMemsetAsync(c.cg_cv.p, 0, sizeof(float) * op.ldw, c.stream));
   op.agg = c.cg_agg.as<int32_t>();
   op.aptr = c.cg_aptr.as<int32_t>();
   op.anodes = c.cg_anodes.as<int32_t>();
   op.dv = c.cg_dv.as<float>();
-  op.W = c.cg_W.as<float>();
-  op.cv = c.cg_cv.as<float>();
-  op.yv = c.cg_yv.as<float>();
   HIPC(hipStreamSynchronize(c.stream));  // the host vectors above go out of scope
   c.coarse_sym = true;
   c.coarse_num = false;
+  c.modes_valid = false;
   c.coarse_target = target;
   c.coarse_coord_gen = c.coord_gen;
   return 0;
 }
 
-// E = Z^T A Z on the device, Cholesky factor and inverse by rocSOLVER (FP64),
+// E = Z^T A Z on the device with nm modes per aggregate (6 rigid-body; 12 with
+// the learned ones in cg_xm), Cholesky factor and inverse by rocSOLVER (FP64),
 // then the FP32 copy the iterations read
-static int32_t coarse_numeric(Context& c) {
+static int32_t coarse_numeric(Context& c, int32_t nm) {
   CoarseOp& op = c.coarse;
+  op.nm = nm;
+  op.nc = nm * op.n_agg;
+  op.ldw = (op.nc + 63) / 64 * 64;
+  op.xm = nm == 12 ? c.cg_xm.as<float>() : nullptr;
   const int32_t nc = op.nc;
+  // six modes: E^-1, c and y in FP32; twelve: in FP64
+  const size_t we = nm == 12 ? sizeof(double) : sizeof(float);
+  if (c.cg_Eblk.ensure(sizeof(double) * nm * nm * (size_t)std::max(c.coarse_npair, 1)) ||
+      c.cg_E.ensure(sizeof(double) * (size_t)nc * nc) || c.cg_W.ensure(we * (size_t)nc * op.ldw) ||
+      c.cg_cv.ensure(we * op.ldw) || c.cg_yv.ensure(we * op.ldw))
+    return HECMW33_ERROR_DEVICE;
+  op.W = nm == 12 ? nullptr : c.cg_W.as<float>();
+  op.cv = nm == 12 ? nullptr : c.cg_cv.as<float>();
+  op.yv = nm == 12 ? nullptr : c.cg_yv.as<float>();
+  op.W64 = nm == 12 ? c.cg_W.as<double>() : nullptr;
+  op.cv64 = nm == 12 ? c.cg_cv.as<double>() : nullptr;
+  op.yv64 = nm == 12 ? c.cg_yv.as<double>() : nullptr;
+  HIPC(hipMemsetAsync(c.cg_cv.p, 0, we * op.ldw, c.stream));
   HIPC(hipMemsetAsync(c.cg_E.p, 0, sizeof(double) * (size_t)nc * nc, c.stream));
   coarse_assemble(c.A.view, op, c.coarse_npair, c.cg_pa.as<int32_t>(), c.cg_pb.as<int32_t>(),
                   c.cg_pnptr.as<int32_t>(), c.cg_pn.as<int32_t>(), c.cg_Eblk.as<double>(), c.cg_E.as<double>(), nc,
@@ -1165,9 +1118,15 @@ static int32_t coarse_numeric(Context& c) {
   HIPC(hipStreamSynchronize(c.stream));
   c.coarse_ok = hinfo[0] == 0 && hinfo[1] == 0;
   c.coarse_w_valid = false;
-  if (c.coarse_ok) coarse_inverse_f32(op, c.cg_E.as<double>(), nc, c.stream);
+  if (c.coarse_ok) {
+    if (nm == 12)
+      coarse_inverse_f64(op, c.cg_E.as<double>(), nc, c.stream);
+    else
+      coarse_inverse_f32(op, c.cg_E.as<double>(), nc, c.stream);
+  }
   HIPC(hipGetLastError());
   c.coarse_num = true;
+  c.coarse_nm_num = nm;
   return 0;
 }
 
@@ -1176,7 +1135,7 @@ static int32_t coarse_aim(const Context& c, int32_t opt3) {
   return std::max(1, std::min(std::min(t, kCoarseAggMax), c.N / kCoarseMinNodes));
 }
 
-static int32_t coarse_setup(Context& c, int32_t target) {
+static int32_t coarse_setup(Context& c, int32_t target, int32_t nm) {
   if ((int64_t)c.h_coord.size() != 3 * (int64_t)c.NP) {
     set_error(c.h_coord.empty()
                   ? "SOLVER_OPT(3) (two-level correction) needs the node coordinates of this system (hecmw33_set_coords)"
@@ -1184,14 +1143,14 @@ static int32_t coarse_setup(Context& c, int32_t target) {
     return HECMW33_ERROR_ARGS;
   }
   target = coarse_aim(c, target);
-  const uint64_t mh = (int32_t)c.coarse_mask.size() == c.N ? c.coarse_mask_hash : 0;
-  if (!c.coarse_sym || c.coarse_target != target || c.coarse_coord_gen != c.coord_gen ||
-      c.coarse_sym_mask_hash != mh) {
+  if (!c.coarse_sym || c.coarse_target != target || c.coarse_coord_gen != c.coord_gen)
     if (int32_t e = coarse_symbolic(c, target)) return e;
-    c.coarse_sym_mask_hash = mh;
+  if (nm == 12 && !c.modes_valid) {
+    set_error("the learned coarse modes are not fitted for these values");
+    return HECMW33_ERROR_ARGS;
   }
-  if (!c.coarse_num)
-    if (int32_t e = coarse_numeric(c)) return e;
+  if (!c.coarse_num || c.coarse_nm_num != nm)
+    if (int32_t e = coarse_numeric(c, nm)) return e;
   return 0;
 }
 
@@ -1439,6 +1398,186 @@ static int32_t set_mpc(Context& c, int32_t N, int32_t n, const int32_t* index, c
   return 0;
 }
 
+// Eigen-decomposition of a symmetric n x n matrix (cyclic Jacobi, FP64):
+// a is destroyed, lam[k] and the columns v[.][k] are its eigenpairs, unsorted.
+static void jacobi_eigen(int n, double* a, double* lam, double* v) {
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) v[i * n + j] = i == j ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0.0, diag = 0.0;
+    for (int i = 0; i < n; ++i) {
+      diag += a[i * n + i] * a[i * n + i];
+      for (int j = i + 1; j < n; ++j) off += a[i * n + j] * a[i * n + j];
+    }
+    if (off <= 1e-30 * diag || off == 0.0) break;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = a[p * n + q];
+        if (apq == 0.0) continue;
+        const double th = (a[q * n + q] - a[p * n + p]) / (2.0 * apq);
+        const double t = (th >= 0.0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+        const double cs = 1.0 / std::sqrt(t * t + 1.0), sn = t * cs;
+        for (int k = 0; k < n; ++k) {
+          const double akp = a[k * n + p], akq = a[k * n + q];
+          a[k * n + p] = cs * akp - sn * akq;
+          a[k * n + q] = sn * akp + cs * akq;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double apk = a[p * n + k], aqk = a[q * n + k];
+          a[p * n + k] = cs * apk - sn * aqk;
+          a[q * n + k] = sn * apk + cs * aqk;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double vkp = v[k * n + p], vkq = v[k * n + q];
+          v[k * n + p] = cs * vkp - sn * vkq;
+          v[k * n + q] = sn * vkp + cs * vkq;
+        }
+      }
+  }
+  for (int i = 0; i < n; ++i) lam[i] = a[i * n + i];
+}
+
+// Default sample count and FCG iterations per sample of the learned coarse
+// modes (Iarray 45 and 46 override them).
+constexpr int32_t I_SOLVER_OPT5 = 45, I_SOLVER_OPT6 = 46;
+constexpr int32_t kModeSamples = 8, kModeIters = 20;
+// default growth of the FP64 |r| over its smallest value at a replacement that drops the learned step
+constexpr double kLearnedDropGrowth = 1000.0;
+constexpr int32_t kModesPerAgg = 6;
+
+// Learned coarse modes: fit, per aggregate, a linear autoencoder with six
+// decoder columns to slow errors the solver generates on this system, and
+// keep the decoders as six extra coarse modes beside the rigid-body ones.
+//  * Samples (the solver's own (residual, A^-1 residual) pairs): for seeded
+//    random v_s, b_s = A v_s, and the flexible CG with the rigid-body
+//    two-level step run for a few iterations from x = 0; e_s = v_s - x_s =
+//    A^-1 r_s is what that solver has not yet removed.
+//  * Per aggregate: the part R of the samples beyond the rigid-body modes (a
+//    6 x 6 least-squares fit per sample), its Gram matrix R^T R (BF16 MFMA,
+//    FP32 accumulation), whose top eigenvectors V give the autoencoder's
+//    optimum; the decoder columns R V lambda^-1/2, scaled to the norm of a
+//    translation mode, become the modes.  Eigenvalues below 1e-6 of the
+//    largest leave their mode at zero (a unit pivot of E).
+// The caller's b and x in vb/vx are kept; harvest and fit times go to
+// modes_harvest_s / modes_fit_s.
+static int32_t learned_modes(Context& c, const int32_t* iarray, const double* rarray, int32_t o3) {
+  const int32_t N = c.N, na = c.coarse.n_agg;
+  const int ns = std::max(1, std::min(16, IA(iarray, I_SOLVER_OPT5) > 0 ? IA(iarray, I_SOLVER_OPT5) : kModeSamples));
+  const int nu = std::max(1, IA(iarray, I_SOLVER_OPT6) > 0 ? IA(iarray, I_SOLVER_OPT6) : kModeIters);
+  const int64_t n3 = 3 * (int64_t)N;
+  const size_t vbytes = sizeof(double) * 3 * (size_t)c.NP;
+  const double t0 = now_s();
+  if (c.ml_S.ensure(sizeof(float) * 16 * (size_t)n3) || c.ml_v.ensure(vbytes) || c.ml_bsave.ensure(vbytes) ||
+      c.ml_xsave.ensure(vbytes) || c.ml_rhs.ensure(sizeof(double) * 96 * (size_t)na) ||
+      c.ml_ztz.ensure(sizeof(double) * 36 * (size_t)na) || c.ml_c.ensure(sizeof(double) * 96 * (size_t)na) ||
+      c.ml_G.ensure(sizeof(float) * 256 * (size_t)na) || c.ml_V.ensure(sizeof(float) * 96 * (size_t)na) ||
+      c.cg_xm.ensure(sizeof(float) * 18 * (size_t)std::max(N, 1)))
+    return HECMW33_ERROR_DEVICE;
+  HIPC(hipMemcpyAsync(c.ml_bsave.p, c.vb.p, vbytes, hipMemcpyDeviceToDevice, c.stream));
+  HIPC(hipMemcpyAsync(c.ml_xsave.p, c.vx.p, vbytes, hipMemcpyDeviceToDevice, c.stream));
+  HIPC(hipMemsetAsync(c.ml_S.p, 0, sizeof(float) * 16 * (size_t)n3, c.stream));
+  HIPC(hipMemsetAsync(c.ml_v.p, 0, vbytes, c.stream));
+  std::vector<int32_t> ia(iarray, iarray + 100);
+  std::vector<double> ra(rarray, rarray + 100);
+  ia[I_ITER - 1] = nu;
+  ia[I_SOLVER_OPT - 1] = kOptMixedFcg;
+  ia[I_SOLVER_OPT3 - 1] = o3;
+  ia[I_SOLVER_OPT4 - 1] = 0;
+  ia[I_FLAG_NUMFACT - 1] = 0;
+  ia[I_FLAG_SYMBFACT - 1] = 0;
+  ia[I_ITERLOG - 1] = 0;
+  ia[I_TIMELOG - 1] = 0;
+  int32_t e = 0, got = 0;
+  for (int s = 0; s < ns; ++s) {
+    // v_s (external entries stay zero), b_s = A v_s, x = 0
+    modes_random(n3, 0x5EED0000ull + (uint64_t)s, c.ml_v.as<double>(), c.stream);
+    launch_spmv(c.A.view, c.ml_v.as<double>(), c.vb.as<double>(), c.stream);
+    HIPC(hipMemsetAsync(c.vx.p, 0, vbytes, c.stream));
+    hecmw33_out o{};
+    std::vector<int32_t> ias(ia);
+    e = krylov_core(c, ias.data(), ra.data(), &o);
+    // a failed sample solve leaves the modes unfitted (the caller keeps the rigid-body modes)
+    if (e || (o.error != 0 && o.error != HECMW33_ERROR_NOCONV_MAXIT)) break;
+    modes_sample(n3, c.ml_v.as<double>(), c.vx.as<double>(), c.ml_S.as<float>(), s, c.stream);
+    ++got;
+  }
+  HIPC(hipMemcpyAsync(c.vb.p, c.ml_bsave.p, vbytes, hipMemcpyDeviceToDevice, c.stream));
+  HIPC(hipMemcpyAsync(c.vx.p, c.ml_xsave.p, vbytes, hipMemcpyDeviceToDevice, c.stream));
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(c.stream));
+  if (e) return e;
+  c.modes_harvest_s = now_s() - t0;
+  c.modes_samples = got;
+  c.modes_iters = nu;
+  if (got != ns) return 0;
+  const double t1 = now_s();
+  const CoarseOp& op = c.coarse;
+  // the rigid-body part of every sample, aggregate by aggregate
+  modes_rigid(na, ns, op.aptr, op.anodes, op.dv, c.ml_S.as<float>(), c.ml_rhs.as<double>(), c.ml_ztz.as<double>(),
+              c.stream);
+  std::vector<double> rhs(96 * (size_t)na), ztz(36 * (size_t)na), coef(96 * (size_t)na, 0.0);
+  HIPC(hipMemcpyAsync(rhs.data(), c.ml_rhs.p, sizeof(double) * 96 * na, hipMemcpyDeviceToHost, c.stream));
+  HIPC(hipMemcpyAsync(ztz.data(), c.ml_ztz.p, sizeof(double) * 36 * na, hipMemcpyDeviceToHost, c.stream));
+  HIPC(hipStreamSynchronize(c.stream));
+#pragma omp parallel for schedule(static)
+  for (int32_t a = 0; a < na; ++a) {
+    // least squares on the rigid-body modes through the eigenpairs of Z^T Z (an
+    // aggregate whose rotations vanish has zero eigenvalues there: those
+    // directions get no coefficient)
+    double m[36], lam[6], v[36];
+    std::copy(&ztz[36 * (size_t)a], &ztz[36 * (size_t)a] + 36, m);
+    jacobi_eigen(6, m, lam, v);
+    double lmax = 0.0;
+    for (int k = 0; k < 6; ++k) lmax = std::max(lmax, lam[k]);
+    for (int s = 0; s < ns; ++s) {
+      const double* b = &rhs[((size_t)a * 16 + s) * 6];
+      double* x = &coef[((size_t)a * 16 + s) * 6];
+      for (int k = 0; k < 6; ++k) {
+        if (lam[k] <= 1e-12 * lmax) continue;
+        double pk = 0.0;
+        for (int q = 0; q < 6; ++q) pk += v[q * 6 + k] * b[q];
+        pk /= lam[k];
+        for (int q = 0; q < 6; ++q) x[q] += v[q * 6 + k] * pk;
+      }
+    }
+  }
+  HIPC(hipMemcpyAsync(c.ml_c.p, coef.data(), sizeof(double) * 96 * na, hipMemcpyHostToDevice, c.stream));
+  modes_resid(N, op.agg, op.dv, c.ml_c.as<double>(), c.ml_S.as<float>(), c.stream);
+  modes_gram(na, op.aptr, op.anodes, c.ml_S.as<float>(), c.ml_G.as<float>(), c.stream);
+  std::vector<float> G(256 * (size_t)na), V(96 * (size_t)na, 0.f);
+  std::vector<int32_t> acnt(na + 1);
+  HIPC(hipMemcpyAsync(G.data(), c.ml_G.p, sizeof(float) * 256 * na, hipMemcpyDeviceToHost, c.stream));
+  HIPC(hipMemcpyAsync(acnt.data(), op.aptr, sizeof(int32_t) * (na + 1), hipMemcpyDeviceToHost, c.stream));
+  HIPC(hipStreamSynchronize(c.stream));
+  int32_t kept = 0;
+#pragma omp parallel for schedule(static) reduction(+ : kept)
+  for (int32_t a = 0; a < na; ++a) {
+    double m[256], lam[16], v[256];
+    for (int k = 0; k < 256; ++k) m[k] = G[256 * (size_t)a + k];
+    jacobi_eigen(16, m, lam, v);
+    int order[16];
+    for (int k = 0; k < 16; ++k) order[k] = k;
+    std::sort(order, order + 16, [&](int x, int y) { return lam[x] > lam[y]; });
+    const double lmax = lam[order[0]];
+    const double scale = std::sqrt((double)std::max(1, acnt[a + 1] - acnt[a]));
+    for (int j = 0; j < kModesPerAgg; ++j) {
+      const int k = order[j];
+      if (!(lmax > 0.0) || lam[k] <= 1e-6 * lmax) continue;
+      ++kept;
+      const double f = scale / std::sqrt(lam[k]);
+      for (int s = 0; s < 16; ++s) V[96 * (size_t)a + 6 * s + j] = (float)(v[s * 16 + k] * f);
+    }
+  }
+  HIPC(hipMemcpyAsync(c.ml_V.p, V.data(), sizeof(float) * 96 * na, hipMemcpyHostToDevice, c.stream));
+  modes_make(N, op.agg, c.ml_S.as<float>(), c.ml_V.as<float>(), c.cg_xm.as<float>(), c.stream);
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(c.stream));
+  c.modes_kept = kept;
+  c.modes_fit_s = now_s() - t1;
+  c.modes_valid = true;
+  return 0;
+}
+
 static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_out* out) {
   const double t0 = now_s();
   const int32_t method = IA(iarray, I_METHOD);
@@ -1469,8 +1608,9 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
     set_error("SOLVER_OPT(1)=1/2 (flexible CG, FP32 operators) needs METHOD=1 and ITERPREMAX<=1");
     return HECMW33_ERROR_UNSUPPORTED;
   }
-  if (learned && (!c.net || iterpremax < 1)) {
-    set_error("SOLVER_OPT(1)=2 (learned aggregation) needs an attached network and ITERPREMAX >= 1");
+  if (learned && (iterpremax < 1 || (IA(iarray, I_SOLVER_OPT3) > 0 && IA(iarray, I_SOLVER_OPT4) == 1))) {
+    set_error("SOLVER_OPT(1)=2 (learned coarse modes) needs ITERPREMAX >= 1 and the symmetric two-level step "
+              "(SOLVER_OPT(4) = 0)");
     return HECMW33_ERROR_UNSUPPORTED;
   }
   if (int32_t e = precond_setup(c, iarray, rarray)) return e;
@@ -1495,35 +1635,45 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
   c.last_spmv64 = spmv64;
   c.coarse_use = false;
   c.last_coarse = 0;
+  c.last_modes_nm = 0;
   c.coarse_sym_use = false;
   if (mixed && o3 > 0) {
-    // the learned solve refines the cells with the network's mask; the
-    // others use the cells alone
+    // the learned solve first fits its modes with solves on the rigid-body
+    // two-level step (learned_modes), then takes the twelve-mode space; if
+    // the fit or its E fails, the rigid-body modes stay
+    int32_t nm = 6;
     if (learned) {
-      if (int32_t e = learned_mask(c)) return e;
-    } else {
-      c.coarse_mask.clear();
+      if (int32_t e = coarse_setup(c, o3, 6)) return e;
+      if (c.coarse_ok && !c.modes_valid)
+        if (int32_t e = learned_modes(c, iarray, rarray, o3)) return e;
+      if (c.modes_valid) nm = 12;
     }
-    if (int32_t e = coarse_setup(c, o3)) return e;
+    if (int32_t e = coarse_setup(c, o3, nm)) return e;
+    if (nm == 12 && !c.coarse_ok) {
+      nm = 6;
+      if (int32_t e = coarse_setup(c, o3, 6)) return e;
+    }
     c.coarse_use = c.coarse_ok;
     c.last_coarse = c.coarse_ok ? c.coarse.n_agg : -1;
     if (c.coarse_use && IA(iarray, I_SOLVER_OPT4) != 1) {
       const size_t vb32 = sizeof(float) * 3 * (size_t)c.NP;
       for (DMem* m : {&c.vr1_32, &c.vt32, &c.vw32, &c.vaz32})
         if (m->ensure(vb32)) return HECMW33_ERROR_DEVICE;
-      if (!c.coarse_w_valid) {
+      if (!c.coarse_w_valid || c.coarse_w_nm != nm) {
         const size_t nw = (size_t)std::max<int64_t>(c.coarse_nw, 1);
         const size_t nblk = (size_t)std::max<int32_t>((c.N + 255) / 256, 1);
-        if (c.cg_AZ.ensure(sizeof(float) * 18 * nw) || c.cg_aggW.ensure(sizeof(int32_t) * nw) ||
+        if (c.cg_AZ.ensure(sizeof(float) * 3 * nm * nw) || c.cg_aggW.ensure(sizeof(int32_t) * nw) ||
             c.cg_sc.ensure(sizeof(int2) * (size_t)std::max<int32_t>(c.N, 1)) || c.cg_kept.ensure(sizeof(int32_t) * nblk))
           return HECMW33_ERROR_DEVICE;
         coarse_build_W(c.A.view, c.coarse, c.N, c.cg_ncnt.as<int32_t>(), c.cg_nagg.as<int32_t>(), c.cg_sc.as<int2>(),
                        c.cg_aggW.as<int32_t>(), c.cg_AZ.as<float>(), c.cg_kept.as<int32_t>(), c.stream);
         HIPC(hipGetLastError());
         c.coarse_w_valid = true;
+        c.coarse_w_nm = nm;
       }
       c.coarse_sym_use = true;
     }
+    c.last_modes_nm = nm;
   }
   const int64_t n3 = 3 * (int64_t)c.N;
   const size_t vbytes = sizeof(double) * 3 * (size_t)c.NP;
@@ -1586,11 +1736,16 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
   const double delta = RA(rarray, R_SOLVER_OPT) > 0.0 && RA(rarray, R_SOLVER_OPT) < 1.0 ? RA(rarray, R_SOLVER_OPT)
                                                                                          : kFcgDelta;
   h.delta2 = delta * delta;
+  // Rarray 42: the growth of the FP64 |r| over its smallest value at a replacement that drops
+  // the learned step
+  const double grow = RA(rarray, R_SOLVER_OPT2) >= 1.0 ? RA(rarray, R_SOLVER_OPT2) : kLearnedDropGrowth;
+  h.drop2 = grow * grow;
+  h.rr_min = h.rr_ref;
   h.reliable = 0;
   h.nreliable = 0;
   // a learned solve whose two-level step was built keeps it unless its FP64
   // residual grows between replacements (k_tail_fcg_rel)
-  h.learned_off = learned && c.coarse_use ? 0 : 1;
+  h.learned_off = learned && c.coarse_use && c.last_modes_nm == 12 ? 0 : 1;
   h.coarse_off = 0;
   h.fallback_iter = 0;
   h.edrop = 0.0;
@@ -1950,7 +2105,7 @@ int32_t hecmw33_ctx_load(hecmw33_ctx* ctx, const hecmw33_sys* A, const int32_t i
   // up the dense-solver library once
   const int32_t o3 = IA(iarray, I_SOLVER_OPT3);
   if (o3 > 0 && (int64_t)ctx->c.h_coord.size() == 3 * (int64_t)ctx->c.NP)
-    if (int32_t e = coarse_setup(ctx->c, o3)) return e;
+    if (int32_t e = coarse_setup(ctx->c, o3, 6)) return e;
   return 0;
 }
 
@@ -2100,7 +2255,7 @@ int32_t hecmw33_ctx_info(hecmw33_ctx* ctx, int64_t* info, int32_t n) {
     for (int32_t k : kept) az_kept += k;
     az_sym = c.coarse_nw;
   }
-  const int64_t v[15] = {c.A.view.nslots,
+  const int64_t v[21] = {c.A.view.nslots,
                         (int64_t)c.N,
                         (int64_t)c.NPL,
                         (int64_t)c.NPU,
@@ -2110,12 +2265,18 @@ int32_t hecmw33_ctx_info(hecmw33_ctx* ctx, int64_t* info, int32_t n) {
                         (int64_t)c.last_fallback_iter,
                         (int64_t)c.last_learned_off,
                         (int64_t)c.last_spmv64,
-                        (int64_t)c.coarse_pieces,
+                        (int64_t)(c.modes_valid ? c.modes_kept : 0),
                         (int64_t)c.last_coarse,
                         (int64_t)c.coarse_npair,
                         az_kept,
-                        az_sym};
-  for (int i = 0; i < n && i < 15; ++i) info[i] = v[i];
+                        az_sym,
+                        (int64_t)c.last_modes_nm,
+                        (int64_t)c.modes_samples,
+                        (int64_t)c.modes_iters,
+                        (int64_t)std::llround(c.modes_harvest_s * 1e6),
+                        (int64_t)std::llround(c.modes_fit_s * 1e6),
+                        (int64_t)c.coarse.nc};
+  for (int i = 0; i < n && i < 21; ++i) info[i] = v[i];
   return 0;
 }
 
@@ -2498,11 +2659,40 @@ int32_t hecmw33_ctx_coarse_get(hecmw33_ctx* ctx, int32_t* n_agg, int32_t* n_pair
   if (pair_a) HIPC(hipMemcpy(pair_a, c.cg_pa.p, sizeof(int32_t) * np, hipMemcpyDeviceToHost));
   if (pair_b) HIPC(hipMemcpy(pair_b, c.cg_pb.p, sizeof(int32_t) * np, hipMemcpyDeviceToHost));
   if (Eblk) {
-    if (!c.coarse_num) {
-      set_error("coarse matrix not assembled for the current values");
+    if (!c.coarse_num || c.coarse_nm_num != 6) {
+      set_error("coarse matrix of the rigid-body modes not assembled for the current values");
       return HECMW33_ERROR_ARGS;
     }
     HIPC(hipMemcpy(Eblk, c.cg_Eblk.p, sizeof(double) * 36 * (size_t)np, hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+int32_t hecmw33_ctx_modes_get(hecmw33_ctx* ctx, float* xm, double* Eblk, float* R) {
+  if (!ctx) return HECMW33_ERROR_ARGS;
+  Context& c = ctx->c;
+  if (!c.coarse_sym || !c.modes_valid) {
+    set_error("no learned coarse modes fitted for the current values");
+    return HECMW33_ERROR_ARGS;
+  }
+  HIPC(hipStreamSynchronize(c.stream));
+  if (xm) HIPC(hipMemcpy(xm, c.cg_xm.p, sizeof(float) * 18 * (size_t)c.N, hipMemcpyDeviceToHost));
+  if (R) HIPC(hipMemcpy(R, c.ml_S.p, sizeof(float) * 48 * (size_t)c.N, hipMemcpyDeviceToHost));
+  if (Eblk) {
+    if (!c.coarse_num || c.coarse_nm_num != 12) {
+      set_error("coarse matrix of the twelve modes not assembled for the current values");
+      return HECMW33_ERROR_ARGS;
+    }
+    // the four 6x6 tiles of every pair into row-major 12x12 blocks
+    const int32_t np = c.coarse_npair;
+    std::vector<double> t(144 * (size_t)np);
+    HIPC(hipMemcpy(t.data(), c.cg_Eblk.p, sizeof(double) * 144 * (size_t)np, hipMemcpyDeviceToHost));
+    for (int32_t p = 0; p < np; ++p)
+      for (int tile = 0; tile < 4; ++tile)
+        for (int r = 0; r < 6; ++r)
+          for (int q = 0; q < 6; ++q)
+            Eblk[144 * (size_t)p + 12 * (6 * (tile >> 1) + r) + 6 * (tile & 1) + q] =
+                t[(4 * (size_t)p + tile) * 36 + 6 * r + q];
   }
   return 0;
 }

@@ -117,6 +117,7 @@ def lib():
     L.hecmw33_matvec_mpc.argtypes = [C.POINTER(_Sys), P, P]
     L.hecmw33_ctx_set_coords.argtypes = [P, I, P]
     L.hecmw33_ctx_coarse_get.argtypes = [P, P, P, P, P, P, P, P]
+    L.hecmw33_ctx_modes_get.argtypes = [P, P, P, P]
     L.hecmw33_last_error.restype = C.c_char_p
     for f in ("hecmw33_solve", "hecmw33_solve_krylov", "hecmw33_matvec", "hecmw33_precond_apply",
               "hecmw33_rel_resid_L2", "hecmw33_ordering", "hecmw33_ctx_load", "hecmw33_ctx_solve_device",
@@ -540,16 +541,18 @@ class DeviceContext:
         return h[:n]
 
     def info(self):
-        v = np.zeros(15, np.int64)
-        _check(lib().hecmw33_ctx_info(self._h, _ptr(v), 15))
+        v = np.zeros(21, np.int64)
+        _check(lib().hecmw33_ctx_info(self._h, _ptr(v), 21))
         return {"nslots": int(v[0]), "N": int(v[1]), "NPL": int(v[2]), "NPU": int(v[3]), "ncolor": int(v[4]),
                 "ssor_slots": int(v[5]), "nreliable": int(v[6]), "fallback_iter": int(v[7]), "learned_off": int(v[8]),
-                "spmv64": int(v[9]), "pieces": int(v[10]), "coarse": int(v[11]), "coarse_pairs": int(v[12]),
-                "az_entries": int(v[13]), "az_entries_symbolic": int(v[14])}
+                "spmv64": int(v[9]), "learned_modes": int(v[10]), "coarse": int(v[11]), "coarse_pairs": int(v[12]),
+                "az_entries": int(v[13]), "az_entries_symbolic": int(v[14]), "modes_per_aggregate": int(v[15]),
+                "mode_samples": int(v[16]), "mode_sample_iters": int(v[17]), "mode_harvest_s": v[18] * 1e-6,
+                "mode_fit_s": v[19] * 1e-6, "coarse_unknowns": int(v[20])}
 
-    def coarse(self):
+    def coarse(self, with_E=True):
         """The last two-level set-up: dict(n_agg, n_pair, agg [N], dv [N,3], pair_a, pair_b [n_pair],
-        Eblk [n_pair,6,6])."""
+        Eblk [n_pair,6,6] of the rigid-body modes, or None without with_E)."""
         na, npair = C.c_int32(0), C.c_int32(0)
         _check(lib().hecmw33_ctx_coarse_get(self._h, C.byref(na), C.byref(npair), None, None, None, None, None))
         na, npair = na.value, npair.value
@@ -560,12 +563,27 @@ class DeviceContext:
         dv = np.zeros(3 * N, np.float32)
         pa = np.zeros(npair, np.int32)
         pb = np.zeros(npair, np.int32)
-        E = np.zeros(36 * npair)
+        E = np.zeros(36 * npair) if with_E else None
         n2, p2 = C.c_int32(0), C.c_int32(0)
         _check(lib().hecmw33_ctx_coarse_get(self._h, C.byref(n2), C.byref(p2), _ptr(agg), _ptr(dv), _ptr(pa),
-                                            _ptr(pb), _ptr(E)))
+                                            _ptr(pb), _ptr(E) if with_E else None))
         return {"n_agg": na, "n_pair": npair, "agg": agg, "dv": dv.reshape(N, 3), "pair_a": pa, "pair_b": pb,
-                "Eblk": E.reshape(npair, 6, 6)}
+                "Eblk": E.reshape(npair, 6, 6) if with_E else None}
+
+    def modes(self, with_E=True):
+        """The learned coarse modes of the last fit: dict(xm [N,6,3], Eblk [n_pair,12,12] or None,
+        R [N,3,16]: the error samples beyond the rigid-body modes that the fit used)."""
+        N = self.info()["N"]
+        xm = np.zeros(18 * N, np.float32)
+        R = np.zeros(48 * N, np.float32)
+        E = None
+        if with_E:
+            na, npair = C.c_int32(0), C.c_int32(0)
+            _check(lib().hecmw33_ctx_coarse_get(self._h, C.byref(na), C.byref(npair), None, None, None, None, None))
+            E = np.zeros(144 * npair.value)
+        _check(lib().hecmw33_ctx_modes_get(self._h, _ptr(xm), _ptr(E) if E is not None else None, _ptr(R)))
+        return {"xm": xm.reshape(N, 6, 3), "Eblk": E.reshape(-1, 12, 12) if E is not None else None,
+                "R": R.reshape(N, 3, 16)}
 
     def set_mpc(self, N, mpc):
         """Multipoint constraints (as frontistr_amd.set_mpc) for this context's solves."""

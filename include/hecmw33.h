@@ -212,12 +212,15 @@ int32_t hecmw33_ctx_history(hecmw33_ctx* ctx, double* hist, int32_t cap);
  * [5] SSOR slots of L' + U', [6] FP64 residual replacements of the last flexible CG solve,
  * [7] the iteration at which the last learned solve dropped its two-level step (0: kept), [8] 1 if
  * that solve ran without the learned step (or was not a learned solve), [9] 1 if the last flexible
- * CG used the FP64 matrix values in q = A p, [10] aggregates the network split off in the current
- * aggregation, [11] aggregates of the last flexible CG's two-level
+ * CG used the FP64 matrix values in q = A p, [10] learned coarse modes of the current fit (over all
+ * aggregates; 0: none fitted), [11] aggregates of the last flexible CG's two-level
  * correction (0: none, -1: E was not positive definite and the solve ran without it), [12] coupled
  * aggregate pairs (blocks of E), [13] A Z entries the symmetric two-level step streams (after
  * dropping the rows whose A Z is zero in exact arithmetic), [14] A Z entries before that (node and
- * touched aggregate); 0 and 0 when the symmetric step has not been set up. */
+ * touched aggregate); 0 and 0 when the symmetric step has not been set up; [15] modes per
+ * aggregate of the last two-level solve (6, or 12 with the learned ones), [16] error samples and
+ * [17] FCG iterations per sample of the last learned fit, [18] its harvest and [19] its fit time
+ * (microseconds), [20] coarse unknowns (modes per aggregate x aggregates). */
 int32_t hecmw33_ctx_info(hecmw33_ctx* ctx, int64_t* info, int32_t n);
 /* the multipoint constraints of a device context (as hecmw33_set_mpc) */
 int32_t hecmw33_ctx_set_mpc(hecmw33_ctx* ctx, int32_t N, int32_t n_mpc, const int32_t* mpc_index,
@@ -267,22 +270,36 @@ int32_t hecmw33_solve_batch(int32_t nsys, const hecmw33_sys* sys, int32_t nrhs, 
                             double* const* X, const int32_t iarray[100], const double rarray[100], int32_t* iter,
                             int32_t* error, double* resid);
 
-/* ---- learned aggregation (no counterpart in the reference) ---------------------------------
- * Inside the flexible CG (iarray[40] = Iarray 41 = 2, with an attached network:
- * hecmw33_ctx_set_net for a device context, hecmw33_set_net for hecmw33_solve /
- * hecmw33_solve_krylov) the two-level step of Iarray 43 (1 when not set) and Iarray 44 takes its
- * coarse space from an aggregation the network refines: a per-node MLP (16 static node features
- * -> 64 -> 64 -> 3, ReLU; BF16 MFMA kernels accumulating in FP32) gives each node the logit of
- * "the slow error moves this node rigidly with a stiff piece"; the nodes with a positive logit
- * leave their cell, one aggregate (six rigid-body modes) per connected piece of at least 32
- * nodes.  The preconditioner stays symmetric positive definite.  Plain SSOR stays the fallback:
- * the solve runs the plain SSOR flexible CG from the iteration at which its FP64 residual grows 4x
- * between two residual replacements (hecmw33_ctx_info [7], [8]).
- * Training: error samples e = x* - x_k = A^-1 r_k of the solver's own iterates, on device
- * contexts with node coordinates; an edge of the matrix graph whose axial strain stays below tau
- * times the sample's median in every sample lies in a rigidly moving piece, and its nodes are
- * labelled 1; Adam on the logistic loss of the network's logit against the labels.
- * A network must outlive the contexts it is attached to; contexts must outlive its samples. */
+/* ---- learned coarse modes (no counterpart in the reference) ---------------------------------
+ * Inside the flexible CG (iarray[40] = Iarray 41 = 2, ITERPREMAX >= 1, the symmetric two-level
+ * step: Iarray 43 (1 when not set) > 0, Iarray 44 = 0) each aggregate gets six learned coarse
+ * modes beside its six rigid-body modes.  They are the decoders of per-aggregate linear
+ * autoencoders fitted to slow errors the solver generates on the system itself: for seeded random
+ * v_s (Iarray 45 samples, default 8, at most 16), b_s = A v_s and Iarray 46 iterations (default 20)
+ * of the flexible CG with the rigid-body two-level step from x = 0, e_s = v_s - x_s = A^-1 r_s;
+ * per aggregate the part of the e_s beyond the rigid-body modes, its Gram matrix (BF16 MFMA, FP32
+ * accumulation) and that matrix's top eigenvectors give the autoencoder's optimum.  The fit is
+ * part of the numeric set-up (redone when Iarray 97 asks for it).  E = Z^T A Z keeps the
+ * preconditioner symmetric positive definite; if the fit or E fails the rigid-body modes stay.
+ * Plain SSOR stays the fallback: the solve runs the plain SSOR flexible CG from the iteration at
+ * which its FP64 residual at a replacement exceeds rarray[41] (Rarray 42, default 1000) times the
+ * smallest one seen at a replacement (hecmw33_ctx_info [7], [8]).
+ *
+ * A per-node classifier network (hecmw33_net_*): a 16 -> 64 -> 64 -> 3 MLP (ReLU; BF16 MFMA
+ * forward and backward kernels accumulating in FP32; Adam) over static node features, trained
+ * on labels built from the solver's errors: an edge of the matrix graph whose axial strain stays
+ * below tau times the sample's median in every error sample lies in a rigidly moving piece, and
+ * its nodes are labelled 1 (the logistic loss of the network's logit against the labels).  The
+ * solve does not use it.  A network must outlive the contexts it is attached to; contexts must
+ * outlive its samples. */
+/* The learned coarse modes of a device context's last fit: xm (18*N floats: node i, mode j,
+ * component d at 18 i + 3 j + d); when the twelve-mode E is assembled, Eblk (144 per coupled
+ * aggregate pair, in hecmw33_ctx_coarse_get's pair order: E_(a,b) row-major 12x12, rows and columns
+ * 0-5 the rigid-body modes, 6-11 the learned ones); R (48*N floats: the error samples' part beyond
+ * their aggregate's rigid-body modes that the fit used, entry 16 (3 i + d) + s for node i,
+ * component d, sample s; unused samples zero).  Any may be NULL. */
+int32_t hecmw33_ctx_modes_get(hecmw33_ctx* ctx, float* xm, double* Eblk, float* R);
+
 typedef struct hecmw33_net hecmw33_net;
 hecmw33_net* hecmw33_net_create(uint64_t seed);
 void hecmw33_net_destroy(hecmw33_net* net);

@@ -121,3 +121,62 @@ def rigid_labels(N, indexL, itemL, indexU, itemU, X, errors, tau):
     lab[rows[small]] = 1
     lab[cols[small]] = 1
     return lab
+
+
+# ---- learned coarse modes -------------------------------------------------------------------
+# The fit of frontistr_amd/csrc/solver.hip (learned_modes) and learned.hip
+# (k_modes_rigid / k_modes_resid / k_modes_gram / k_modes_make), in FP64:
+# per aggregate, the samples' least-squares rigid-body part (through the
+# eigenpairs of Z^T Z, directions below 1e-12 of its largest eigenvalue
+# dropped) is removed, the Gram matrix of the rest R is formed (the GPU rounds
+# R to BF16 for it), and its eigenvectors of the six largest eigenvalues above
+# 1e-6 of the largest give the modes R v / sqrt(lambda) * sqrt(n_a).
+
+def rigid_modes(dv):
+    """Z_i (3 x 6) per node from the scaled offsets: translations, then
+    rotations about x, y, z, as the coarse kernels write them."""
+    n = dv.shape[0]
+    Z = np.zeros((n, 3, 6))
+    for k in range(3):
+        Z[:, k, k] = 1.0
+    dx, dy, dz = dv[:, 0], dv[:, 1], dv[:, 2]
+    Z[:, 1, 3], Z[:, 2, 3] = -dz, dy
+    Z[:, 0, 4], Z[:, 2, 4] = dz, -dx
+    Z[:, 0, 5], Z[:, 1, 5] = -dy, dx
+    return Z
+
+
+def remove_rigid(S, agg, dv):
+    """S (ns, N, 3) -> R: each sample minus its least-squares rigid-body part
+    on every aggregate."""
+    Z = rigid_modes(np.asarray(dv, np.float64))
+    R = np.array(S, dtype=np.float64, copy=True)
+    for a in np.unique(agg):
+        nodes = np.nonzero(agg == a)[0]
+        Za = Z[nodes].reshape(-1, 6)
+        lam, V = np.linalg.eigh(Za.T @ Za)
+        keep = lam > 1e-12 * lam.max()
+        P = V[:, keep] @ np.diag(1.0 / lam[keep]) @ V[:, keep].T
+        Sa = R[:, nodes, :].reshape(R.shape[0], -1).T  # (3 n_a, ns)
+        c = P @ (Za.T @ Sa)
+        R[:, nodes, :] = (Sa - Za @ c).T.reshape(R.shape[0], nodes.size, 3)
+    return R
+
+
+def fit_modes(R, agg, k=6, gram_bf16=False):
+    """R (ns, N, 3) (the samples beyond the rigid-body modes) -> xm (N, k, 3)."""
+    ns, N, _ = R.shape
+    xm = np.zeros((N, k, 3))
+    for a in np.unique(agg):
+        nodes = np.nonzero(agg == a)[0]
+        Ra = R[:, nodes, :].reshape(ns, -1).T  # (3 n_a, ns)
+        Rg = bf16(Ra).astype(np.float64) if gram_bf16 else Ra
+        lam, V = np.linalg.eigh(Rg.T @ Rg)
+        order = np.argsort(lam)[::-1]
+        lmax = lam[order[0]]
+        for j in range(min(k, ns)):
+            q = order[j]
+            if not lmax > 0.0 or lam[q] <= 1e-6 * lmax:
+                continue
+            xm[nodes, j, :] = (Ra @ V[:, q] * (np.sqrt(nodes.size) / np.sqrt(lam[q]))).reshape(nodes.size, 3)
+    return xm

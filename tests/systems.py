@@ -94,3 +94,16 @@ def shuffle_items(m, seed=0):
             blk = A[9 * a:9 * b].reshape(-1, 9)[p].reshape(-1)
             A[9 * a:9 * b] = blk
     return m2
+
+
+def to_csr(m):
+    """A (3N x 3NP) as scipy CSR, the rows of the internal nodes."""
+    import scipy.sparse as sp
+    D, AL, AU = m.D.reshape(-1, 3, 3), m.AL.reshape(-1, 3, 3), m.AU.reshape(-1, 3, 3)
+    idx, blk, ptr = [], [], [0]
+    for i in range(m.N):
+        l0, l1, u0, u1 = m.indexL[i], m.indexL[i + 1], m.indexU[i], m.indexU[i + 1]
+        idx += list(m.itemL[l0:l1] - 1) + [i] + list(m.itemU[u0:u1] - 1)
+        blk += [AL[l0:l1], D[i:i + 1], AU[u0:u1]]
+        ptr.append(len(idx))
+    return sp.bsr_matrix((np.concatenate(blk), np.array(idx), np.array(ptr)), shape=(3 * m.N, 3 * m.NP)).tocsr()

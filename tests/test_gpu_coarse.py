@@ -177,12 +177,11 @@ def test_coarse_needs_coordinates():
 
 
 def test_learned_with_coarse_converges():
+    """the learned coarse modes on a small two-phase cube: twelve modes per aggregate, converged"""
     import torch
     m = fa.gen_hex8(12, kind="inclusions", seed=2, E=1000.0, contrast=1e4, rmin=2, rmax=3, vfrac=0.15)
-    net = fa.LearnedNet(seed=1)
     ctx = fa.DeviceContext()
     ctx.set_coords(fa.hex8_coords(12))
-    ctx.set_net(net)
     mm = m.copy()
     mm.set_iter(20000)
     mm.set_precond(1)
@@ -192,7 +191,8 @@ def test_learned_with_coarse_converges():
     b = torch.tensor(mm.B, device="cuda:0")
     x = torch.zeros_like(b)
     r = ctx.solve_device(b.data_ptr(), x.data_ptr(), mm.Iarray, mm.Rarray)
-    assert r.error == 0 and ctx.info()["coarse"] > 1
+    inf = ctx.info()
+    assert r.error == 0 and inf["coarse"] > 1 and inf["modes_per_aggregate"] == 12
     assert ref.rel_resid(m, x.cpu().numpy()) <= 1e-8 * 1.0001
     ctx.close()
 

@@ -98,9 +98,9 @@ def test_c2_cg_against_full_reference_solve(c2, pc):
 
 # ---- C3 and C5: every mode against the GPU's FP64 SSOR-CG ----------------------------------
 # (name, Iarray 41, Iarray 43, Iarray 44): FP64 CG; flexible CG with FP32 operators; with the
-# symmetric two-level step; with the additive one; the learned flexible CG with the additive step
+# symmetric two-level step; with the additive one; with the learned coarse modes (symmetric step)
 MODES = (("fp64_ssor_cg", 0, 0, 0), ("mixed_fcg", 1, 0, 0), ("mixed_coarse_fcg", 1, 1, 0),
-         ("mixed_coarse_additive_fcg", 1, 1, 1), ("learned_coarse_fcg", 2, 1, 1))
+         ("mixed_coarse_additive_fcg", 1, 1, 1), ("learned_coarse_fcg", 2, 1, 0))
 
 
 class _Big:
@@ -112,8 +112,6 @@ class _Big:
         m.set_coarse(1)
         self.ctx = fa.DeviceContext()
         self.ctx.set_coords(coords)
-        self.net = fa.LearnedNet(seed=0)
-        self.ctx.set_net(self.net)
         self.ctx.load(m)
         self.b = torch.from_numpy(m.B).to("cuda:0")
         self.x = torch.zeros(3 * m.NP, dtype=torch.float64, device="cuda:0")
@@ -130,9 +128,7 @@ class _Big:
         return r, self.x.cpu().numpy()
 
     def close(self):
-        self.ctx.set_net(None)
         self.ctx.close()
-        self.net.close()
 
 
 def _check_mode(big, name, opt, coarse, additive):

@@ -185,3 +185,86 @@ def test_logistic_gradient_matches_finite_differences():
     assert np.all(dY[:, 1:] == 0)
     # closed form at y = 0: log 2 whatever the labels
     assert abs(L.logistic(np.zeros((4, 3)), np.array([0, 1, 0, 1]))[0] - np.log(2.0)) <= 1e-15
+
+
+# ---- learned coarse modes ---------------------------------------------------------------------
+def _aggregates(rng, n=400, na=5):
+    """na aggregates of random points with their scaled offsets from the centroid"""
+    X = rng.standard_normal((n, 3))
+    agg = rng.integers(0, na, n)
+    dv = np.zeros_like(X)
+    for a in range(na):
+        m = agg == a
+        dv[m] = X[m] - X[m].mean(0)
+    return agg, dv / np.abs(dv).max()
+
+
+def test_mode_fit_recovers_planted_fields_beyond_the_rigid_modes():
+    """Samples = rigid-body motions + combinations of three fields per
+    aggregate: the fit removes the rigid part exactly, and its modes span the
+    planted fields' part beyond the rigid-body modes, with the norm of a
+    translation mode; modes past the planted rank stay zero."""
+    rng = np.random.default_rng(3)
+    agg, dv = _aggregates(rng)
+    N, ns = agg.size, 10
+    Z = L.rigid_modes(dv)
+    fields = rng.standard_normal((3, N, 3))
+    S = np.zeros((ns, N, 3))
+    for s in range(ns):
+        for a in np.unique(agg):
+            m = agg == a
+            S[s, m] = (Z[m] @ rng.standard_normal(6)) + np.einsum("f,fnd->nd", rng.standard_normal(3), fields[:, m])
+    R = L.remove_rigid(S, agg, dv)
+    xm = L.fit_modes(R, agg)
+    for a in np.unique(agg):
+        m = np.nonzero(agg == a)[0]
+        Za = Z[m].reshape(-1, 6)
+        # the rigid part is gone from the samples and absent from the modes
+        assert np.abs(Za.T @ R[:, m].reshape(ns, -1).T).max() <= 1e-9 * np.abs(R).max() * m.size
+        X = xm[m].transpose(1, 0, 2).reshape(6, -1).T  # (3 n_a, 6)
+        live = np.linalg.norm(X, axis=0) > 0
+        assert live.sum() == 3, live
+        assert np.allclose(np.linalg.norm(X[:, live], axis=0), np.sqrt(m.size))
+        # the planted fields beyond the rigid modes lie in the modes' span
+        F = fields[:, m].reshape(3, -1).T
+        F = F - Za @ np.linalg.lstsq(Za, F, rcond=None)[0]
+        Q, _ = np.linalg.qr(X[:, live])
+        assert np.linalg.norm(F - Q @ (Q.T @ F)) <= 1e-9 * np.linalg.norm(F)
+
+
+def test_mode_fit_catches_a_kept_rigid_part():
+    """A fit that skips the rigid-body removal gives modes that are not
+    orthogonal to the rigid-body modes (what the GPU test checks)."""
+    rng = np.random.default_rng(4)
+    agg, dv = _aggregates(rng, na=3)
+    Z = L.rigid_modes(dv)
+    S = np.stack([Z @ rng.standard_normal(6) + 0.1 * rng.standard_normal((agg.size, 3)) for _ in range(8)])
+    bad = L.fit_modes(S, agg)
+    good = L.fit_modes(L.remove_rigid(S, agg, dv), agg)
+    for xm, ok in ((good, True), (bad, False)):
+        worst = 0.0
+        for a in np.unique(agg):
+            m = np.nonzero(agg == a)[0]
+            X = xm[m].transpose(1, 0, 2).reshape(6, -1).T
+            worst = max(worst, np.abs(Z[m].reshape(-1, 6).T @ X).max() / np.sqrt(m.size))
+        assert (worst <= 1e-8) == ok, worst
+
+
+def test_mode_fit_bf16_gram_keeps_the_span():
+    """The GPU forms the Gram matrix from BF16-rounded samples: the fitted
+    span moves by about the BF16 rounding, not more."""
+    rng = np.random.default_rng(5)
+    agg, dv = _aggregates(rng, na=2)
+    S = rng.standard_normal((12, agg.size, 3)) * np.linspace(1.0, 0.05, 12)[:, None, None]
+    R = L.remove_rigid(S, agg, dv)
+    x64 = L.fit_modes(R, agg)
+    x16 = L.fit_modes(R, agg, gram_bf16=True)
+    for a in np.unique(agg):
+        m = np.nonzero(agg == a)[0]
+        A = x64[m].transpose(1, 0, 2).reshape(6, -1).T
+        B = x16[m].transpose(1, 0, 2).reshape(6, -1).T
+        Qa, _ = np.linalg.qr(A)
+        Qb, _ = np.linalg.qr(B)
+        # principal angles between the two 6-dimensional spans
+        s = np.linalg.svd(Qa.T @ Qb, compute_uv=False)
+        assert s.min() >= 0.98, s
