@@ -7,12 +7,13 @@ log-uniform in [1e3, 1e6] and nu uniform in [0.2, 0.4] (seed 0), 12,519,843
 DOF, 111,284,641 3x3 blocks, assembled like fistr1 (gen_hex8.cpp).  One step
 = one complete solve x0 = 0 -> FP64 true relative residual 1e-8 with the
 flexible CG (FP32 SpMV/SSOR/vectors, FP64 inner products, x and residual
-replacement) whose symmetric two-level step carries, beside each aggregate's
-six rigid-body modes, six learned modes (Iarray 41 = 2, 43 = 1): the decoders
-of per-aggregate linear autoencoders fitted to slow errors the solver
-generates on the system (8 seeded samples x 20 FCG iterations), matrix and
-vectors resident in HBM, the numeric set-up (SSOR factors, the harvest and fit
-of the learned modes, the coarse matrix) inside the step.
+replacement) whose two-level step (deflation form, Iarray 44 = 2: y = M^-1 r,
+z = y + Q (r - A y)) carries, beside each aggregate's six rigid-body modes, six
+learned modes (Iarray 41 = 2, 43 = 1): the decoders of per-aggregate linear
+autoencoders fitted to slow errors the solver generates on the system (8
+seeded samples x 20 FCG iterations), matrix and vectors resident in HBM, the
+numeric set-up (SSOR factors, the harvest and fit of the learned modes, the
+coarse matrix) inside the step.
 
 The same JSON line carries, on the same GPU: the plain FP64 multicolour-SSOR
 CG on the same system, the speed-up and |x - x_FP64|/|x_FP64|; the other
@@ -23,7 +24,7 @@ inclusion cube with the rigid-body and the learned coarse modes; and the
 reference's own CPU solver (oracle/_ref, built from the FrontISTR sources) on
 the host.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--only c2|c3|c4|c5]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--only c2|c3|c4|c5|ssor]
 """
 import argparse
 import json
@@ -44,10 +45,12 @@ HBM_PEAK_GBS = 8000.0
 SPMV_TRAFFIC_100 = 2199539866
 
 # (name, Iarray 41, Iarray 43, Iarray 44): FP64 CG; flexible CG with FP32
-# operators; with the symmetric two-level step (rigid-body modes); with the
-# learned modes beside them
+# operators; with the symmetric two-level step (rigid-body modes), and in its
+# deflation form (one coarse solve per iteration); with the learned modes
+# beside the rigid-body ones, both forms
 FCG_MODES = (("fp64_ssor_cg", 0, 0, 0), ("mixed_fcg", 1, 0, 0), ("mixed_coarse_fcg", 1, 1, 0),
-             ("learned_coarse_fcg", 2, 1, 0))
+             ("mixed_coarse_deflation_fcg", 1, 1, 2), ("learned_coarse_fcg", 2, 1, 0),
+             ("learned_coarse_deflation_fcg", 2, 1, 2))
 
 
 def spmv_bytes(N, NP, nnzb, w):
@@ -181,6 +184,19 @@ def run_modes(tag, R, modes):
     return res, xs
 
 
+def learned_vs_rigid(res):
+    """[rigid-body modes, learned modes] per form of the two-level step, and the
+    time ratio rigid / learned (above 1: the learned modes pay)."""
+    out = {}
+    for form, rigid, learned in (("symmetric", "mixed_coarse_fcg", "learned_coarse_fcg"),
+                                 ("deflation", "mixed_coarse_deflation_fcg", "learned_coarse_deflation_fcg")):
+        if rigid in res and learned in res:
+            a, b = res[rigid], res[learned]
+            out[form] = {"iterations": [a["iterations"], b["iterations"]], "seconds": [a["seconds"], b["seconds"]],
+                         "rigid_over_learned_time": round(a["seconds"] / b["seconds"], 3)}
+    return out
+
+
 def spmv_median(R, fp32, reps):
     """median kernel time (ms) of `reps` single SpMV calls (events on the context's stream)"""
     import torch
@@ -217,9 +233,8 @@ def bench_c3(args, dev):
     m.set_coarse(1)  # the aggregation cells are symbolic set-up, built at load like the colouring
     R = Resident(fa, m, fa.hex8_coords(n), dev)
     modes, xs = run_modes("C3", R, FCG_MODES)
-    head = "learned_coarse_fcg"
-    opt = 2
-    ia = R.iarray(opt, 1, 0)
+    head = "learned_coarse_deflation_fcg"
+    ia = R.iarray(2, 1, 2)
     ia_log = ia.copy()
     ia[21] = 0  # no TIMELOG events inside the timed steps; the warm-up carries them
     r_log = None
@@ -262,9 +277,7 @@ def bench_c3(args, dev):
         "ssor_fp32_GBs": round(sb32 / (ssor32 * 1e-3) / 1e9, 1),
         "ssor_fp64_ms": round(ssor64, 4), "ssor_fp64_GBs": round(sb64 / (ssor64 * 1e-3) / 1e9, 1),
         "ncolor": R.ctx.info()["ncolor"],
-        "learned_vs_rigid_body_modes": {
-            "iterations": [modes["mixed_coarse_fcg"]["iterations"], modes["learned_coarse_fcg"]["iterations"]],
-            "seconds": [modes["mixed_coarse_fcg"]["seconds"], modes["learned_coarse_fcg"]["seconds"]]},
+        "learned_vs_rigid_body_modes": learned_vs_rigid(modes),
     }
     log("C3 %s: %.4f s/step, %d iterations; SpMV FP32 %.4f ms (%.0f GB/s); SSOR FP32 %.4f ms (%.0f GB/s), "
         "FP64 %.4f ms" % (head, out["seconds_per_step"], r_last.iter, ms32, out["spmv_fp32_GBs"], ssor32,
@@ -282,10 +295,8 @@ def bench_ssor_probe(args, dev):
     import frontistr_amd as fa
     n = args.c3_n
     m = fa.gen_hex8(n, kind="lognormal", seed=0)
-    m.set_iter(2)
     m.set_precond(1)
-    R = Resident(fa, m, None, dev)
-    R.solve(R.iarray(1, 0, 0))  # SSOR set-up and the FP32 copies
+    R = Resident(fa, m, None, dev)  # the load builds the colouring and the SSOR factors
     a = torch.empty(1 << 27, dtype=torch.float64, device=dev)
     b = torch.empty_like(a)
     a.fill_(1.0)
@@ -354,11 +365,7 @@ def bench_c5(args, dev):
     out = {"workload": "%d^3 HEX8, two-phase inclusions (E 1e3 / 1e7, 15%% volume, radii 2-6 elements, seed 7), "
                        "%d DOF" % (args.c5_n, 3 * m.N)}
     out.update(res)
-    if "learned_coarse_fcg" in res:
-        out["learned_vs_rigid_body_modes"] = {
-            "iterations": [res["mixed_coarse_fcg"]["iterations"], res["learned_coarse_fcg"]["iterations"]],
-            "seconds": [res["mixed_coarse_fcg"]["seconds"], res["learned_coarse_fcg"]["seconds"]],
-            "speedup": round(res["mixed_coarse_fcg"]["seconds"] / res["learned_coarse_fcg"]["seconds"], 3)}
+    out["learned_vs_rigid_body_modes"] = learned_vs_rigid(res)
     return out
 
 
@@ -534,8 +541,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "%d^3 HEX8 heterogeneous cube (lognormal E in [1e3,1e6], nu in [0.2,0.4], seed 0), "
-                        "12,519,843 DOF: mixed-precision flexible CG with the learned two-level step to an FP64 "
-                        "true residual of 1e-8" % n,
+                        "12,519,843 DOF: mixed-precision flexible CG with the learned two-level step (deflation "
+                        "form) to an FP64 true residual of 1e-8" % n,
             "method": c3["headline_mode"], "iterations": c3["iterations"], "rel_resid_true": c3["rel_resid_true"],
             "rel_err_vs_fp64_ssor_cg": c3["rel_err_vs_fp64"],
         },

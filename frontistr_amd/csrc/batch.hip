@@ -41,6 +41,7 @@
 #include <mutex>
 #include <sstream>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/hecmw33.h"
@@ -343,14 +344,19 @@ __device__ __forceinline__ void spmm_one_tile(int t, const Tiles& TL, const int3
     T yv[RPL][3];
 #pragma unroll
     for (int j = 0; j < RPL; ++j) yv[j][0] = yv[j][1] = yv[j][2] = (T)0;
-    for (int bb = b0 + ph; bb < b1; bb += NPH * U) {
+    // a chunk of U blocks from bb; FULL (all U live) keeps every load and sum
+    // out of conditions, so the compiler cannot move a block's loads into a
+    // branch of their own; the row's last, partial chunk re-reads its last
+    // block in the dead slots and masks them out
+    auto chunk = [&](int bb, auto full_tag) {
+      constexpr bool FULL = decltype(full_tag)::value;
       int c[U];
       T a[U][9];
       T xv[U][RPL][3];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int b = bb + u * NPH;
-        const int bc = b < b1 ? b : b1 - 1;
+        const int bc = (FULL || b < b1) ? b : b1 - 1;
         c[u] = col[bc];
 #pragma unroll
         for (int e = 0; e < 9; ++e) a[u][e] = val[9 * (int64_t)bc + e];
@@ -361,7 +367,7 @@ __device__ __forceinline__ void spmm_one_tile(int t, const Tiles& TL, const int3
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const bool live = bb + u * NPH < b1;
+        const bool live = FULL || bb + u * NPH < b1;
 #pragma unroll
         for (int j = 0; j < RPL; ++j) {
           T n0, n1, n2;
@@ -380,7 +386,10 @@ __device__ __forceinline__ void spmm_one_tile(int t, const Tiles& TL, const int3
           yv[j][2] = live ? n2 : yv[j][2];
         }
       }
-    }
+    };
+    int bb = b0 + ph;
+    for (; bb + (U - 1) * NPH < b1; bb += NPH * U) chunk(bb, std::true_type{});
+    if (bb < b1) chunk(bb, std::false_type{});
     if constexpr (NPH > 1) {
 #pragma unroll
       for (int off = G; off < LPR; off <<= 1)

@@ -336,6 +336,54 @@ __device__ __forceinline__ float quad_bcast(float v) {
 // of the first chunk, so no load waits for a block of its own.  y starts at
 // -0, for which -0 + a is a exactly (also for a = -0), so the D block's sum is
 // the reference's (a1*x1 + a2*x2) + a3*x3.  xo = x of the row itself.
+// One chunk of U blocks from k0.  FULL: every block is live, so no load or sum
+// of the chunk sits under a condition and all its loads are in flight
+// together (a per-block condition lets the compiler move a block's loads into
+// a branch of their own, one more dependent round trip per chunk).  Not FULL
+// (the row's last chunk): slots past the end re-read the last block and are
+// masked out of the sum.
+template <int U, bool FULL>
+__device__ __forceinline__ void sell_chunk(const Sell<double>& a, int base, int lane, const int32_t* __restrict__ colp,
+                                           const double* __restrict__ x, int k0, int len, double (&y)[3],
+                                           double (&xo)[3]) {
+#pragma clang fp contract(off)
+  int kk[U], c[U];
+#pragma unroll
+  for (int j = 0; j < U; ++j) {
+    kk[j] = (FULL || k0 + j < len) ? k0 + j : len - 1;
+    c[j] = colp[(int64_t)kk[j] * kRowsA];
+  }
+  double v[U][9], xv[U][3];
+#pragma unroll
+  for (int j = 0; j < U; ++j) {
+    aval_load(a.val, (int64_t)base + kk[j], lane, v[j]);
+    xv[j][0] = x[3 * (int64_t)c[j]];
+    xv[j][1] = x[3 * (int64_t)c[j] + 1];
+    xv[j][2] = x[3 * (int64_t)c[j] + 2];
+  }
+  if (k0 == 0) {
+    xo[0] = xv[0][0];
+    xo[1] = xv[0][1];
+    xo[2] = xv[0][2];
+  }
+#pragma unroll
+  for (int j = 0; j < U; ++j) {
+    const double t0 = y[0] + v[j][0] * xv[j][0] + v[j][1] * xv[j][1] + v[j][2] * xv[j][2];
+    const double t1 = y[1] + v[j][3] * xv[j][0] + v[j][4] * xv[j][1] + v[j][5] * xv[j][2];
+    const double t2 = y[2] + v[j][6] * xv[j][0] + v[j][7] * xv[j][1] + v[j][8] * xv[j][2];
+    if (FULL) {
+      y[0] = t0;
+      y[1] = t1;
+      y[2] = t2;
+    } else {
+      const bool live = k0 + j < len;
+      y[0] = live ? t0 : y[0];
+      y[1] = live ? t1 : y[1];
+      y[2] = live ? t2 : y[2];
+    }
+  }
+}
+
 template <int U>
 __device__ __forceinline__ void sell_row(const Sell<double>& a, int s, int lane, const double* __restrict__ x,
                                          double (&y)[3], double (&xo)[3]) {
@@ -347,43 +395,43 @@ __device__ __forceinline__ void sell_row(const Sell<double>& a, int s, int lane,
   if (len == 0) return;
   y[0] = y[1] = y[2] = -0.0;
   const int32_t* __restrict__ colp = a.col + (int64_t)base * kRowsA + lane;
-  for (int k0 = 0; k0 < len; k0 += U) {
-    // branch-free chunk: slots past the row's end re-read the last block and
-    // are masked out, so the chunk's loads are all in flight together
-    int kk[U], c[U];
-#pragma unroll
-    for (int j = 0; j < U; ++j) {
-      kk[j] = (k0 + j < len) ? k0 + j : len - 1;
-      c[j] = colp[(int64_t)kk[j] * kRowsA];
-    }
-    double v[U][9], xv[U][3];
-#pragma unroll
-    for (int j = 0; j < U; ++j) {
-      aval_load(a.val, (int64_t)base + kk[j], lane, v[j]);
-      xv[j][0] = x[3 * (int64_t)c[j]];
-      xv[j][1] = x[3 * (int64_t)c[j] + 1];
-      xv[j][2] = x[3 * (int64_t)c[j] + 2];
-    }
-    if (k0 == 0) {
-      xo[0] = xv[0][0];
-      xo[1] = xv[0][1];
-      xo[2] = xv[0][2];
-    }
-#pragma unroll
-    for (int j = 0; j < U; ++j) {
-      const bool live = k0 + j < len;
-      const double t0 = y[0] + v[j][0] * xv[j][0] + v[j][1] * xv[j][1] + v[j][2] * xv[j][2];
-      const double t1 = y[1] + v[j][3] * xv[j][0] + v[j][4] * xv[j][1] + v[j][5] * xv[j][2];
-      const double t2 = y[2] + v[j][6] * xv[j][0] + v[j][7] * xv[j][1] + v[j][8] * xv[j][2];
-      y[0] = live ? t0 : y[0];
-      y[1] = live ? t1 : y[1];
-      y[2] = live ? t2 : y[2];
-    }
-  }
+  int k0 = 0;
+  for (; k0 + U <= len; k0 += U) sell_chunk<U, true>(a, base, lane, colp, x, k0, len, y, xo);
+  if (k0 < len) sell_chunk<U, false>(a, base, lane, colp, x, k0, len, y, xo);
 }
 
 // FP32 values and vectors, FP32 accumulation with FMA (mixed-precision inner
 // solves only).
+template <int U, bool FULL>
+__device__ __forceinline__ void sell_chunk_f32(const Sell<float>& a, int base, int lane,
+                                               const int32_t* __restrict__ colp, const float* __restrict__ x, int k0,
+                                               int len, float (&y)[3]) {
+  int kk[U], c[U];
+#pragma unroll
+  for (int j = 0; j < U; ++j) {
+    kk[j] = (FULL || k0 + j < len) ? k0 + j : len - 1;
+    c[j] = colp[(int64_t)kk[j] * kRowsA];
+  }
+  float v[U][9], xv[U][3];
+#pragma unroll
+  for (int j = 0; j < U; ++j) {
+    aval_load(a.val, (int64_t)base + kk[j], lane, v[j]);
+    xv[j][0] = x[3 * (int64_t)c[j]];
+    xv[j][1] = x[3 * (int64_t)c[j] + 1];
+    xv[j][2] = x[3 * (int64_t)c[j] + 2];
+  }
+#pragma unroll
+  for (int j = 0; j < U; ++j) {
+    const float t0 = fmaf(v[j][2], xv[j][2], fmaf(v[j][1], xv[j][1], fmaf(v[j][0], xv[j][0], y[0])));
+    const float t1 = fmaf(v[j][5], xv[j][2], fmaf(v[j][4], xv[j][1], fmaf(v[j][3], xv[j][0], y[1])));
+    const float t2 = fmaf(v[j][8], xv[j][2], fmaf(v[j][7], xv[j][1], fmaf(v[j][6], xv[j][0], y[2])));
+    const bool live = FULL || k0 + j < len;
+    y[0] = live ? t0 : y[0];
+    y[1] = live ? t1 : y[1];
+    y[2] = live ? t2 : y[2];
+  }
+}
+
 template <int U>
 __device__ __forceinline__ void sell_row_f32(const Sell<float>& a, int s, int lane, const float* __restrict__ x,
                                              float (&y)[3]) {
@@ -391,33 +439,9 @@ __device__ __forceinline__ void sell_row_f32(const Sell<float>& a, int s, int la
   const int len = a.len[s * kRowsA + lane];
   y[0] = y[1] = y[2] = 0.0f;
   const int32_t* __restrict__ colp = a.col + (int64_t)base * kRowsA + lane;
-  for (int k0 = 0; k0 < len; k0 += U) {
-    // branch-free chunk, as in sell_row
-    int kk[U], c[U];
-#pragma unroll
-    for (int j = 0; j < U; ++j) {
-      kk[j] = (k0 + j < len) ? k0 + j : len - 1;
-      c[j] = colp[(int64_t)kk[j] * kRowsA];
-    }
-    float v[U][9], xv[U][3];
-#pragma unroll
-    for (int j = 0; j < U; ++j) {
-      aval_load(a.val, (int64_t)base + kk[j], lane, v[j]);
-      xv[j][0] = x[3 * (int64_t)c[j]];
-      xv[j][1] = x[3 * (int64_t)c[j] + 1];
-      xv[j][2] = x[3 * (int64_t)c[j] + 2];
-    }
-#pragma unroll
-    for (int j = 0; j < U; ++j) {
-      const bool live = k0 + j < len;
-      const float t0 = fmaf(v[j][2], xv[j][2], fmaf(v[j][1], xv[j][1], fmaf(v[j][0], xv[j][0], y[0])));
-      const float t1 = fmaf(v[j][5], xv[j][2], fmaf(v[j][4], xv[j][1], fmaf(v[j][3], xv[j][0], y[1])));
-      const float t2 = fmaf(v[j][8], xv[j][2], fmaf(v[j][7], xv[j][1], fmaf(v[j][6], xv[j][0], y[2])));
-      y[0] = live ? t0 : y[0];
-      y[1] = live ? t1 : y[1];
-      y[2] = live ? t2 : y[2];
-    }
-  }
+  int k0 = 0;
+  for (; k0 + U <= len; k0 += U) sell_chunk_f32<U, true>(a, base, lane, colp, x, k0, len, y);
+  if (k0 < len) sell_chunk_f32<U, false>(a, base, lane, colp, x, k0, len, y);
 }
 
 // Slices handled by this wavefront: a contiguous chunk after XCD remapping.
@@ -582,6 +606,35 @@ __global__ __launch_bounds__(kBlock) void k_spmv_f32(Sell<float> a, const float*
   if (DOTS) block_partials<2>(acc, part, blockIdx.x);
 }
 
+// A chunk of the FP64-value, FP32-vector row product (FULL as in sell_chunk)
+template <int U, bool FULL>
+__device__ __forceinline__ void mixed_chunk(const Sell<double>& a, int base, int lane,
+                                            const int32_t* __restrict__ colp, const float* __restrict__ x, int k0,
+                                            int len, double (&yv)[3]) {
+  int kk[U], c[U];
+#pragma unroll
+  for (int j = 0; j < U; ++j) {
+    kk[j] = (FULL || k0 + j < len) ? k0 + j : len - 1;
+    c[j] = colp[(int64_t)kk[j] * kRowsA];
+  }
+  double v[U][9], xv[U][3];
+#pragma unroll
+  for (int j = 0; j < U; ++j) {
+    aval_load(a.val, (int64_t)base + kk[j], lane, v[j]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) xv[j][d] = (double)x[3 * (int64_t)c[j] + d];
+  }
+#pragma unroll
+  for (int j = 0; j < U; ++j) {
+    const bool live = FULL || k0 + j < len;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double t = fma(v[j][3 * d + 2], xv[j][2], fma(v[j][3 * d + 1], xv[j][1], fma(v[j][3 * d], xv[j][0], yv[d])));
+      yv[d] = live ? t : yv[d];
+    }
+  }
+}
+
 // FP64 values, FP32 vectors, FP64 accumulation: q = A p for the flexible CG
 // when the FP32-rounded matrix is too inexact (high material contrast), with
 // the FP64 partials of p.q and r.p.
@@ -598,30 +651,9 @@ __global__ __launch_bounds__(kBlock) void k_spmv_mixed(Sell<double> a, const flo
     const int len = a.len[s * kRowsA + lane];
     const int32_t* __restrict__ colp = a.col + (int64_t)base * kRowsA + lane;
     double yv[3] = {0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < len; k0 += kSpmvChunk) {
-      int kk[kSpmvChunk], c[kSpmvChunk];
-#pragma unroll
-      for (int j = 0; j < kSpmvChunk; ++j) {
-        kk[j] = (k0 + j < len) ? k0 + j : len - 1;
-        c[j] = colp[(int64_t)kk[j] * kRowsA];
-      }
-      double v[kSpmvChunk][9], xv[kSpmvChunk][3];
-#pragma unroll
-      for (int j = 0; j < kSpmvChunk; ++j) {
-        aval_load(a.val, (int64_t)base + kk[j], lane, v[j]);
-#pragma unroll
-        for (int d = 0; d < 3; ++d) xv[j][d] = (double)x[3 * (int64_t)c[j] + d];
-      }
-#pragma unroll
-      for (int j = 0; j < kSpmvChunk; ++j) {
-        const bool live = k0 + j < len;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          const double t = fma(v[j][3 * d + 2], xv[j][2], fma(v[j][3 * d + 1], xv[j][1], fma(v[j][3 * d], xv[j][0], yv[d])));
-          yv[d] = live ? t : yv[d];
-        }
-      }
-    }
+    int k0 = 0;
+    for (; k0 + kSpmvChunk <= len; k0 += kSpmvChunk) mixed_chunk<kSpmvChunk, true>(a, base, lane, colp, x, k0, len, yv);
+    if (k0 < len) mixed_chunk<kSpmvChunk, false>(a, base, lane, colp, x, k0, len, yv);
     const int64_t row = (int64_t)s * kRowsA + lane;
     if (row < a.nrows) {
 #pragma unroll
@@ -693,17 +725,18 @@ __device__ __forceinline__ T pick3(int d, T a, T b, T c) {
 // lane's component sum; all CH column indices, gathers and values are
 // requested before the first one is used.  SIGN -1: sw - a*x (forward),
 // +1: sw + a*x (backward), each block as the reference writes it.
-template <typename T, int CH, bool DESC, int SIGN>
+template <typename T, int CH, bool DESC, int SIGN, bool FULL>
 __device__ __forceinline__ void ssor_chunk(const int32_t* __restrict__ colp, const T* __restrict__ vp,
                                            const T* __restrict__ z, int k0, int len, int dd, T& sw) {
 #pragma clang fp contract(off)
-  // branch-free: slots past the row's end re-read a valid block and are
-  // masked out of the sum, so every load of the chunk can be in flight at once
+  // FULL (every block live): no load or sum under a condition.  The row's
+  // last chunk re-reads a valid block in the slots past the end and masks
+  // them out of the sum.
   int kk[CH], c[CH];
 #pragma unroll
   for (int j = 0; j < CH; ++j) {
     const int k = DESC ? k0 - j : k0 + j;
-    kk[j] = DESC ? (k < 0 ? 0 : k) : (k < len ? k : len - 1);
+    kk[j] = FULL ? k : (DESC ? (k < 0 ? 0 : k) : (k < len ? k : len - 1));
     c[j] = colp[(int64_t)kk[j] * kRowsS];
   }
   T xd[CH], v[CH][3];
@@ -718,7 +751,7 @@ __device__ __forceinline__ void ssor_chunk(const int32_t* __restrict__ colp, con
 #pragma unroll
   for (int j = 0; j < CH; ++j) {
     const int k = DESC ? k0 - j : k0 + j;
-    const bool live = DESC ? k >= 0 : k < len;
+    const bool live = FULL || (DESC ? k >= 0 : k < len);
     const T x0 = quad_bcast<0>(xd[j]), x1 = quad_bcast<1>(xd[j]), x2 = quad_bcast<2>(xd[j]);
     const T t = SIGN < 0 ? sw - v[j][0] * x0 - v[j][1] * x1 - v[j][2] * x2
                          : sw + v[j][0] * x0 + v[j][1] * x1 + v[j][2] * x2;
@@ -758,7 +791,11 @@ __global__ __launch_bounds__(kBlock) void k_ssor_fwd(Sell<T> L, const int32_t* _
     const int32_t* __restrict__ colp = L.col + (int64_t)base * kRowsS + r;
     const T* __restrict__ vp = L.val + ((int64_t)base * 9 + 3 * dd) * kRowsS + r;
     T sw = zs[3 * slot + dd];
-    for (int k0 = 0; k0 < len; k0 += CH) ssor_chunk<T, CH, false, -1>(colp, vp, zs, k0, len, dd, sw);
+    {
+      int k0 = 0;
+      for (; k0 + CH <= len; k0 += CH) ssor_chunk<T, CH, false, -1, true>(colp, vp, zs, k0, len, dd, sw);
+      if (k0 < len) ssor_chunk<T, CH, false, -1, false>(colp, vp, zs, k0, len, dd, sw);
+    }
     T x0 = quad_bcast<0>(sw), x1 = quad_bcast<1>(sw), x2 = quad_bcast<2>(sw);
     lu3_solve<T>(a, x0, x1, x2);
     if (d < 3) zs[3 * slot + d] = pick3(d, x0, x1, x2);
@@ -788,7 +825,11 @@ __global__ __launch_bounds__(kBlock) void k_ssor_bwd(Sell<T> U, const int32_t* _
     const T* __restrict__ vp = U.val + ((int64_t)base * 9 + 3 * dd) * kRowsS + r;
     const T zo = zs[3 * slot + dd];
     T sw = 0;
-    for (int top = len - 1; top >= 0; top -= CH) ssor_chunk<T, CH, true, 1>(colp, vp, zs, top, len, dd, sw);
+    {
+      int top = len - 1;
+      for (; top - CH + 1 >= 0; top -= CH) ssor_chunk<T, CH, true, 1, true>(colp, vp, zs, top, len, dd, sw);
+      if (top >= 0) ssor_chunk<T, CH, true, 1, false>(colp, vp, zs, top, len, dd, sw);
+    }
     T x0 = quad_bcast<0>(sw), x1 = quad_bcast<1>(sw), x2 = quad_bcast<2>(sw);
     lu3_solve<T>(a, x0, x1, x2);
     const T zn = zo - pick3(dd, x0, x1, x2);
@@ -801,7 +842,7 @@ __global__ __launch_bounds__(kBlock) void k_ssor_bwd(Sell<T> U, const int32_t* _
 // needs a quarter of the waves of the four-lane layout for the same dependent
 // load chains.  Same arithmetic per row and component as ssor_chunk + the
 // quad layout: the reference's statements in the reference's order.
-template <typename T, int CH, bool DESC, int SIGN>
+template <typename T, int CH, bool DESC, int SIGN, bool FULL>
 __device__ __forceinline__ void ssor64_chunk(const int32_t* __restrict__ colp, const T* __restrict__ vp,
                                              const T* __restrict__ z, int k0, int len, T (&sw)[3]) {
 #pragma clang fp contract(off)
@@ -809,7 +850,7 @@ __device__ __forceinline__ void ssor64_chunk(const int32_t* __restrict__ colp, c
 #pragma unroll
   for (int j = 0; j < CH; ++j) {
     const int k = DESC ? k0 - j : k0 + j;
-    kk[j] = DESC ? (k < 0 ? 0 : k) : (k < len ? k : len - 1);
+    kk[j] = FULL ? k : (DESC ? (k < 0 ? 0 : k) : (k < len ? k : len - 1));
     c[j] = colp[(int64_t)kk[j] * kRowsA];
   }
   T v[CH][9], x[CH][3];
@@ -824,7 +865,7 @@ __device__ __forceinline__ void ssor64_chunk(const int32_t* __restrict__ colp, c
 #pragma unroll
   for (int j = 0; j < CH; ++j) {
     const int k = DESC ? k0 - j : k0 + j;
-    const bool live = DESC ? k >= 0 : k < len;
+    const bool live = FULL || (DESC ? k >= 0 : k < len);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const T t = SIGN < 0 ? sw[d] - v[j][3 * d] * x[j][0] - v[j][3 * d + 1] * x[j][1] - v[j][3 * d + 2] * x[j][2]
@@ -861,7 +902,11 @@ __global__ __launch_bounds__(kBlock) void k_ssor64_fwd(Sell<T> L, const int32_t*
     const int32_t* __restrict__ colp = L.col + (int64_t)base * kRowsA + lane;
     const T* __restrict__ vp = L.val + (int64_t)base * 9 * kRowsA + lane;
     T sw[3] = {zs[3 * slot], zs[3 * slot + 1], zs[3 * slot + 2]};
-    for (int k0 = 0; k0 < len; k0 += CH) ssor64_chunk<T, CH, false, -1>(colp, vp, zs, k0, len, sw);
+    {
+      int k0 = 0;
+      for (; k0 + CH <= len; k0 += CH) ssor64_chunk<T, CH, false, -1, true>(colp, vp, zs, k0, len, sw);
+      if (k0 < len) ssor64_chunk<T, CH, false, -1, false>(colp, vp, zs, k0, len, sw);
+    }
     lu3_solve<T>(a, sw[0], sw[1], sw[2]);
 #pragma unroll
     for (int d = 0; d < 3; ++d) zs[3 * slot + d] = sw[d];
@@ -888,7 +933,11 @@ __global__ __launch_bounds__(kBlock) void k_ssor64_bwd(Sell<T> U, const int32_t*
     const int32_t* __restrict__ colp = U.col + (int64_t)base * kRowsA + lane;
     const T* __restrict__ vp = U.val + (int64_t)base * 9 * kRowsA + lane;
     T sw[3] = {0, 0, 0};
-    for (int top = len - 1; top >= 0; top -= CH) ssor64_chunk<T, CH, true, 1>(colp, vp, zs, top, len, sw);
+    {
+      int top = len - 1;
+      for (; top - CH + 1 >= 0; top -= CH) ssor64_chunk<T, CH, true, 1, true>(colp, vp, zs, top, len, sw);
+      if (top >= 0) ssor64_chunk<T, CH, true, 1, false>(colp, vp, zs, top, len, sw);
+    }
     lu3_solve<T>(a, sw[0], sw[1], sw[2]);
 #pragma unroll
     for (int d = 0; d < 3; ++d) zs[3 * slot + d] = zs[3 * slot + d] - sw[d];
@@ -1324,14 +1373,15 @@ void launch_block_lu_f32(int32_t nslots, const int32_t* row_src, const double* D
                          hipStream_t s) {
   k_block_lu<float><<<2048, 256, 0, s>>>(nslots, row_src, D, sigma, alu, rows);
 }
-// Blocks whose loads a sweep issues together (HECMW33_SSOR_CHUNK = 4, 8 or
-// 16; 8 measured fastest on the 100^3 cube).
+// Blocks whose loads the four-lane sweeps issue together (HECMW33_SSOR_CHUNK =
+// 4, 8 or 16; 4 by default: with the row's last chunk alone masked, 4 runs
+// the 100^3 FP64 application in 1.000 ms against 1.094 ms for 8).
 static int ssor_chunk_sel() {
   static int v = -1;
   if (v < 0) {
     const char* e = getenv("HECMW33_SSOR_CHUNK");
-    const int c = e ? atoi(e) : 8;
-    v = (c == 4 || c == 16) ? c : 8;
+    const int c = e ? atoi(e) : 4;
+    v = (c == 8 || c == 16) ? c : 4;
   }
   return v;
 }
@@ -1347,23 +1397,44 @@ static int ssor64_chunk_sel() {
   return v;
 }
 
+template <typename T, bool FWD, int CH>
+static void ssor_launch4(const SsorOp<T>& op, int g, int s0, int ns, const KrylovState* st, hipStream_t s) {
+  if constexpr (FWD)
+    k_ssor_fwd<T, CH><<<g, kBlock, 0, s>>>(op.L, op.row_old, op.alu, s0, ns, op.zs, st);
+  else
+    k_ssor_bwd<T, CH><<<g, kBlock, 0, s>>>(op.U, op.row_old, op.alu, s0, ns, op.zs, st);
+}
+
+template <typename T, bool FWD, int CH>
+static void ssor_launch64(const SsorOp<T>& op, int g, int s0, int ns, const KrylovState* st, hipStream_t s) {
+  if constexpr (FWD)
+    k_ssor64_fwd<T, CH><<<g, kBlock, 0, s>>>(op.L, op.row_old, op.alu, s0, ns, op.zs, st);
+  else
+    k_ssor64_bwd<T, CH><<<g, kBlock, 0, s>>>(op.U, op.row_old, op.alu, s0, ns, op.zs, st);
+}
+
+template <typename T, bool FWD>
+static void ssor_colour(const SsorOp<T>& op, int g, int s0, int ns, const KrylovState* st, hipStream_t s) {
+  if ((FWD ? op.L.rows : op.U.rows) == kRowsA) {
+    if (ssor64_chunk_sel() == 8)
+      ssor_launch64<T, FWD, 8>(op, g, s0, ns, st, s);
+    else
+      ssor_launch64<T, FWD, 4>(op, g, s0, ns, st, s);
+    return;
+  }
+  switch (ssor_chunk_sel()) {
+    case 4: ssor_launch4<T, FWD, 4>(op, g, s0, ns, st, s); break;
+    case 16: ssor_launch4<T, FWD, 16>(op, g, s0, ns, st, s); break;
+    default: ssor_launch4<T, FWD, 8>(op, g, s0, ns, st, s); break;
+  }
+}
+
 template <typename T>
 static void ssor_fwd_any(const SsorOp<T>& op, int c, const KrylovState* st, hipStream_t s) {
   const int32_t s0 = op.color_slice[c], ns = op.color_slice[c + 1] - s0;
   if (ns <= 0) return;
   const int g = spmv_grid(ns);
-  if (op.L.rows == kRowsA) {
-    if (ssor64_chunk_sel() == 8)
-      k_ssor64_fwd<T, 8><<<g, kBlock, 0, s>>>(op.L, op.row_old, op.alu, s0, ns, op.zs, st);
-    else
-      k_ssor64_fwd<T, 4><<<g, kBlock, 0, s>>>(op.L, op.row_old, op.alu, s0, ns, op.zs, st);
-    return;
-  }
-  switch (ssor_chunk_sel()) {
-    case 4: k_ssor_fwd<T, 4><<<g, kBlock, 0, s>>>(op.L, op.row_old, op.alu, s0, ns, op.zs, st); break;
-    case 16: k_ssor_fwd<T, 16><<<g, kBlock, 0, s>>>(op.L, op.row_old, op.alu, s0, ns, op.zs, st); break;
-    default: k_ssor_fwd<T, 8><<<g, kBlock, 0, s>>>(op.L, op.row_old, op.alu, s0, ns, op.zs, st); break;
-  }
+  ssor_colour<T, true>(op, g, s0, ns, st, s);
 }
 
 template <typename T>
@@ -1371,18 +1442,7 @@ static void ssor_bwd_any(const SsorOp<T>& op, int c, const KrylovState* st, hipS
   const int32_t s0 = op.color_slice[c], ns = op.color_slice[c + 1] - s0;
   if (ns <= 0) return;
   const int g = spmv_grid(ns);
-  if (op.U.rows == kRowsA) {
-    if (ssor64_chunk_sel() == 8)
-      k_ssor64_bwd<T, 8><<<g, kBlock, 0, s>>>(op.U, op.row_old, op.alu, s0, ns, op.zs, st);
-    else
-      k_ssor64_bwd<T, 4><<<g, kBlock, 0, s>>>(op.U, op.row_old, op.alu, s0, ns, op.zs, st);
-    return;
-  }
-  switch (ssor_chunk_sel()) {
-    case 4: k_ssor_bwd<T, 4><<<g, kBlock, 0, s>>>(op.U, op.row_old, op.alu, s0, ns, op.zs, st); break;
-    case 16: k_ssor_bwd<T, 16><<<g, kBlock, 0, s>>>(op.U, op.row_old, op.alu, s0, ns, op.zs, st); break;
-    default: k_ssor_bwd<T, 8><<<g, kBlock, 0, s>>>(op.U, op.row_old, op.alu, s0, ns, op.zs, st); break;
-  }
+  ssor_colour<T, false>(op, g, s0, ns, st, s);
 }
 
 // stage r, every colour forward, then backward in reverse (the last colour's

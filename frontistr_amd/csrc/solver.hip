@@ -96,7 +96,11 @@ constexpr int32_t kCoarseAggDefault = 1000;
 // (default) the symmetric step z1 = Q r, z2 = z1 + M^-1 (r - A z1),
 // z = z2 + Q (r - A z2) with Q = Z E^-1 Z^T (A z1 and the coarse part of A z
 // from A Z kept per node; q = A p carried as A z + beta q, so an iteration
-// still has one SpMV); 1 the additive z = M^-1 r + Q r.
+// still has one SpMV); 1 the additive z = M^-1 r + Q r; 2 the deflation form
+// y = M^-1 r, z = y + Q (r - A y): one coarse solve per iteration instead of
+// two (the A-DEF2 preconditioner of Tang, Nabben, Vuik and Erlangga, J. Sci.
+// Comput. 39 (2009) 340-370, whose spectrum is the symmetric step's; it is
+// not symmetric, which the flexible CG's explicit beta allows).
 constexpr int I_SOLVER_OPT4 = 44;
 // the dense coarse inverse is n_c^2 entries (n_c = 6 per aggregate): at most
 // 2048 aggregates (n_c = 12288, 0.6 GB in FP32)
@@ -277,6 +281,7 @@ struct Context {
   CoarseOp coarse;
   int32_t coarse_npair = 0;       // coupled aggregate pairs (blocks of E)
   bool coarse_sym_use = false;    // this solve takes the symmetric two-level step
+  bool coarse_defl = false;       // ... in its deflation form (no pre-smoothing coarse solve)
   bool coarse_w_valid = false;    // A Z (per node and touched aggregate) built for the current values
   int32_t last_modes_nm = 0;      // modes per aggregate of the last two-level solve
   int32_t coarse_nm_num = 0;      // modes per aggregate of the E^-1 built (6 or 12)
@@ -1180,6 +1185,7 @@ static void enqueue_fcg_iteration(Context& c, int32_t kind, bool spmv64, hipEven
 // One flexible CG iteration with the symmetric two-level step (FP32 vectors):
 //   z1 = Q r, r1 = r - A z1 (from A Z), z2 = z1 + M32^-1 r1, w = A z2 (the one SpMV),
 //   z = z2 + Q (r - w), A z = w + A Z E^-1 Z^T (r - w);
+// in the deflation form z2 = M32^-1 r (no z1);
 // then beta from (r.z, z.q_prev), p = z + beta p and q = A z + beta q (so q = A p
 // without a second SpMV), alpha from (p.q, r.p), and the update as the flexible CG.
 static void enqueue_fcg_sym_iteration(Context& c, int32_t kind, bool spmv64, hipEvent_t* ev) {
@@ -1192,9 +1198,13 @@ static void enqueue_fcg_sym_iteration(Context& c, int32_t kind, bool spmv64, hip
   const int32_t* aggW = c.cg_aggW.as<int32_t>();
   const float* AZ = c.cg_AZ.as<float>();
   if (ev) (void)hipEventRecord(ev[0], c.stream);
-  coarse_sym_pre(c.coarse, c.N, sc, aggW, AZ, r, z, r1, st, c.stream);
-  precond_apply_f32(kind, c.ssor_op32(), c.alu_diag32.as<float>(), c.N, r1, t, st, c.stream);
-  coarse_sym_add(n3, t, z, st, c.stream);
+  if (c.coarse_defl) {
+    precond_apply_f32(kind, c.ssor_op32(), c.alu_diag32.as<float>(), c.N, r, z, st, c.stream);
+  } else {
+    coarse_sym_pre(c.coarse, c.N, sc, aggW, AZ, r, z, r1, st, c.stream);
+    precond_apply_f32(kind, c.ssor_op32(), c.alu_diag32.as<float>(), c.N, r1, t, st, c.stream);
+    coarse_sym_add(n3, t, z, st, c.stream);
+  }
   if (ev) (void)hipEventRecord(ev[1], c.stream);
   if (ev) (void)hipEventRecord(ev[2], c.stream);
   fcg_spmv_store(c.A.view32, c.A.view, spmv64, z, w, st, c.stream);
@@ -1482,7 +1492,7 @@ static int32_t learned_modes(Context& c, const int32_t* iarray, const double* ra
   ia[I_ITER - 1] = nu;
   ia[I_SOLVER_OPT - 1] = kOptMixedFcg;
   ia[I_SOLVER_OPT3 - 1] = o3;
-  ia[I_SOLVER_OPT4 - 1] = 0;
+  ia[I_SOLVER_OPT4 - 1] = IA(iarray, I_SOLVER_OPT4) == 2 ? 2 : 0;  // the solve's own form of the step
   ia[I_FLAG_NUMFACT - 1] = 0;
   ia[I_FLAG_SYMBFACT - 1] = 0;
   ia[I_ITERLOG - 1] = 0;
@@ -1610,7 +1620,7 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
   }
   if (learned && (iterpremax < 1 || (IA(iarray, I_SOLVER_OPT3) > 0 && IA(iarray, I_SOLVER_OPT4) == 1))) {
     set_error("SOLVER_OPT(1)=2 (learned coarse modes) needs ITERPREMAX >= 1 and the symmetric two-level step "
-              "(SOLVER_OPT(4) = 0)");
+              "(SOLVER_OPT(4) = 0 or 2)");
     return HECMW33_ERROR_UNSUPPORTED;
   }
   if (int32_t e = precond_setup(c, iarray, rarray)) return e;
@@ -1637,6 +1647,7 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
   c.last_coarse = 0;
   c.last_modes_nm = 0;
   c.coarse_sym_use = false;
+  c.coarse_defl = false;
   if (mixed && o3 > 0) {
     // the learned solve first fits its modes with solves on the rigid-body
     // two-level step (learned_modes), then takes the twelve-mode space; if
@@ -1672,6 +1683,7 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
         c.coarse_w_nm = nm;
       }
       c.coarse_sym_use = true;
+      c.coarse_defl = IA(iarray, I_SOLVER_OPT4) == 2;
     }
     c.last_modes_nm = nm;
   }

@@ -224,13 +224,14 @@ class HecMatrix:
         self.set_i(self.I_SOLVER_OPT + 1, spmv)
         self.Rarray[self.R_SOLVER_OPT - 1] = 0.0 if delta is None else float(delta)
 
-    def set_coarse(self, n=1, additive=False):
+    def set_coarse(self, n=1, additive=False, deflation=False):
         """Two-level correction of the flexible CG (Iarray 43): 0 off, 1 about
         1000 aggregates, larger values that many (needs node coordinates:
         set_coords / DeviceContext.set_coords).  Iarray 44: the symmetric
-        coarse-SSOR-coarse step (default) or, with additive=True, M^-1 r + Q r."""
+        coarse-SSOR-coarse step (default), with additive=True M^-1 r + Q r,
+        with deflation=True y = M^-1 r, z = y + Q (r - A y)."""
         self.set_i(43, int(n))
-        self.set_i(44, 1 if additive else 0)
+        self.set_i(44, 1 if additive else (2 if deflation else 0))
 
     def set_sigma_diag(self, v):
         # hecmw_mat_set_sigma_diag clamps to [1, 2], negative = automatic (-1)

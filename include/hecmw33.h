@@ -112,7 +112,8 @@ int32_t hecmw33_solve_krylov(const hecmw33_sys* A, const double* B, double* X, i
  *   z1 = Q r, z2 = z1 + M^-1 (r - A z1), z = z2 + Q (r - A z2) with Q = Z E^-1 Z^T
  *   (A Z is kept per node and neighbouring aggregate, and q = A p is carried as
  *   A z + beta q, so an iteration still has one SpMV); 1 the additive
- *   z = M^-1 r + Q r.
+ *   z = M^-1 r + Q r; 2 the deflation form y = M^-1 r, z = y + Q (r - A y)
+ *   (A-DEF2: one coarse solve per iteration, the symmetric step's spectrum).
  *   Accepted by hecmw33_solve, hecmw33_solve_krylov and hecmw33_ctx_solve_device. */
 
 /* Node coordinates for the two-level correction: xyz interleaved, 3*NP doubles
@@ -272,7 +273,7 @@ int32_t hecmw33_solve_batch(int32_t nsys, const hecmw33_sys* sys, int32_t nrhs, 
 
 /* ---- learned coarse modes (no counterpart in the reference) ---------------------------------
  * Inside the flexible CG (iarray[40] = Iarray 41 = 2, ITERPREMAX >= 1, the symmetric two-level
- * step: Iarray 43 (1 when not set) > 0, Iarray 44 = 0) each aggregate gets six learned coarse
+ * step: Iarray 43 (1 when not set) > 0, Iarray 44 = 0 or 2) each aggregate gets six learned coarse
  * modes beside its six rigid-body modes.  They are the decoders of per-aggregate linear
  * autoencoders fitted to slow errors the solver generates on the system itself: for seeded random
  * v_s (Iarray 45 samples, default 8, at most 16), b_s = A v_s and Iarray 46 iterations (default 20)

@@ -27,7 +27,7 @@ from tests.systems import dense
 pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not ref.available(), reason="oracle/_ref not built")]
 
 
-def _ctx_solve(m, coords, coarse, mixed=True, maxit=5000, additive=False):
+def _ctx_solve(m, coords, coarse, mixed=True, maxit=5000, additive=False, deflation=False):
     import torch
     ctx = fa.DeviceContext()
     ctx.set_coords(coords)
@@ -35,7 +35,7 @@ def _ctx_solve(m, coords, coarse, mixed=True, maxit=5000, additive=False):
     mm.set_iter(maxit)
     mm.set_precond(1)
     mm.set_mixed(mixed)
-    mm.set_coarse(coarse, additive=additive)
+    mm.set_coarse(coarse, additive=additive, deflation=deflation)
     ctx.load(mm)
     b = torch.tensor(mm.B, device="cuda:0")
     x = torch.zeros_like(b)
@@ -105,10 +105,10 @@ def test_coarse_matrix_is_ZtAZ(case):
     assert inf["coarse_pairs"] == len(pa)
 
 
-@pytest.mark.parametrize("additive", [False, True])
+@pytest.mark.parametrize("form", ["symmetric", "additive", "deflation"])
 @pytest.mark.parametrize("case", ["hex8_lognormal", "hex8_inclusions", "tet10_box"])
-def test_mixed_coarse_matches_reference(case, additive):
-    """The symmetric (default) and the additive two-level forms."""
+def test_mixed_coarse_matches_reference(case, form):
+    """The symmetric (default), the additive and the deflation two-level forms."""
     if case == "hex8_lognormal":
         m, xyz = fa.gen_hex8(16, kind="lognormal", seed=5), fa.hex8_coords(16)
     elif case == "hex8_inclusions":
@@ -123,7 +123,7 @@ def test_mixed_coarse_matches_reference(case, additive):
     mr.set_precond(1)
     xr, info = ref.solve_krylov(mr, nthreads=2)
     assert info["error"] == 0
-    ctx, r, x = _ctx_solve(m, xyz, 1, maxit=20000, additive=additive)
+    ctx, r, x = _ctx_solve(m, xyz, 1, maxit=20000, additive=form == "additive", deflation=form == "deflation")
     used = ctx.info()["coarse"]
     ctx.close()
     assert r.error == 0 and used > 1

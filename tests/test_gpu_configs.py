@@ -98,9 +98,11 @@ def test_c2_cg_against_full_reference_solve(c2, pc):
 
 # ---- C3 and C5: every mode against the GPU's FP64 SSOR-CG ----------------------------------
 # (name, Iarray 41, Iarray 43, Iarray 44): FP64 CG; flexible CG with FP32 operators; with the
-# symmetric two-level step; with the additive one; with the learned coarse modes (symmetric step)
+# symmetric two-level step; with the additive one; with the deflation one; with the learned
+# coarse modes (symmetric and deflation steps)
 MODES = (("fp64_ssor_cg", 0, 0, 0), ("mixed_fcg", 1, 0, 0), ("mixed_coarse_fcg", 1, 1, 0),
-         ("mixed_coarse_additive_fcg", 1, 1, 1), ("learned_coarse_fcg", 2, 1, 0))
+         ("mixed_coarse_additive_fcg", 1, 1, 1), ("mixed_coarse_deflation_fcg", 1, 1, 2),
+         ("learned_coarse_fcg", 2, 1, 0), ("learned_coarse_deflation_fcg", 2, 1, 2))
 
 
 class _Big:
