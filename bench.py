@@ -43,6 +43,12 @@ HBM_PEAK_GBS = 8000.0
 # two rocprofv3 passes (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000, the factors a
 # 1 GiB copy of the same run gives); profiles/spmv_hex8_100_traffic.md
 SPMV_TRAFFIC_100 = 2199539866
+# HBM bytes per FP32 multicolour SSOR application on the 160^3 lognormal cube
+# (stage, 17 + 16 colour launches, unstage), from two rocprofv3 passes over
+# `bench.py --only ssor` (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000, the factors of
+# the run's 1 GiB copy; every read request of the sweeps is 128 B, as the
+# copy's); profiles/ssor_fp32_hex8_160_lognormal_traffic.md
+SSOR_TRAFFIC_160_FP32 = 8867202616
 
 # (name, Iarray 41, Iarray 43, Iarray 44): FP64 CG; flexible CG with FP32
 # operators; with the symmetric two-level step (rigid-body modes), and in its
@@ -553,12 +559,14 @@ def main():
             "bound": "hbm", "kernel": "multicolour SSOR application, FP32 (k_ssor_stage, %d x k_ssor64_fwd/bwd "
                                       "colour launches, k_ssor_unstage)" % c3["ncolor"],
             "achieved": c3["ssor_fp32_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(c3["ssor_fp32_GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(c3["ssor_fp32_GBs"] / HBM_PEAK_GBS, 4),
+            "traffic": SSOR_TRAFFIC_160_FP32 if n == 160 else None,
             "bytes_per_application": c3["ssor_fp32_bytes"], "avg_application_ms": c3["ssor_fp32_ms"],
             "spmv_fp32_GBs": c3["spmv_fp32_GBs"], "spmv_fp32_frac": round(c3["spmv_fp32_GBs"] / HBM_PEAK_GBS, 4),
         },
         "spmv": {"c2_fp64_ms_median100": c2["spmv_fp64_ms_median100"] if c2 else None,
                  "c2_fp64_GBs": c2["spmv_fp64_GBs"] if c2 else None,
+                 "c2_fp64_traffic": SPMV_TRAFFIC_100 if c2 and args.c2_n == 100 else None,
                  "c3_fp32_ms_median100": c3["spmv_fp32_ms_median100"], "c3_fp32_GBs": c3["spmv_fp32_GBs"]},
         "cpu_baseline": cpu,
         "c3": c3,
