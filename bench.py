@@ -48,7 +48,7 @@ SPMV_TRAFFIC_100 = 2199539866
 # `bench.py --only ssor` (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000, the factors of
 # the run's 1 GiB copy; every read request of the sweeps is 128 B, as the
 # copy's); profiles/ssor_fp32_hex8_160_lognormal_traffic.md
-SSOR_TRAFFIC_160_FP32 = 8867202616
+SSOR_TRAFFIC_160_FP32 = 6736586215
 
 # (name, Iarray 41, Iarray 43, Iarray 44): FP64 CG; flexible CG with FP32
 # operators; with the symmetric two-level step (rigid-body modes), and in its
