@@ -245,13 +245,14 @@ __global__ __launch_bounds__(kBT) void kb_p(Tiles T, QView Qv, const float* __re
   });
 }
 
-// ---- x += alpha p (FP64), r = r - alpha q (FP32), r.r; then the next
-//      iteration's preconditioner on the new r while it is in registers:
+// ---- dx += alpha p (FP32 increments of the FP64 x, folded into x before each
+//      residual replacement by kb_fold), r = r - alpha q (FP32), r.r; then the
+//      next iteration's preconditioner on the new r while it is in registers:
 //      z = D^-1 r with r.z and z.q (q of this iteration, the next beta's
 //      q_prev), the values and partials kb_diag would give
 __global__ __launch_bounds__(kBT) void kb_update_diag(Tiles T, QView Qv, const float* __restrict__ alu,
                                                       const float* __restrict__ p, const float* __restrict__ q,
-                                                      double* __restrict__ x, float* __restrict__ r,
+                                                      float* __restrict__ dx, float* __restrict__ r,
                                                       float* __restrict__ z, BState S, double* part_rr,
                                                       double* part_rz) {
   walk_tiles(Qv, [&](int t) {
@@ -270,7 +271,7 @@ __global__ __launch_bounds__(kBT) void kb_update_diag(Tiles T, QView Qv, const f
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         qv[d] = q[o + d];
-        x[o + d] = x[o + d] + alpha * (double)p[o + d];
+        dx[o + d] = (float)((double)dx[o + d] + alpha * (double)p[o + d]);
         v[d] = (float)((double)r[o + d] - alpha * (double)qv[d]);
         r[o + d] = v[d];
         acc[0] += (double)v[d] * (double)v[d];
@@ -288,6 +289,25 @@ __global__ __launch_bounds__(kBT) void kb_update_diag(Tiles T, QView Qv, const f
     }
     tile_partials_rowlanes<1>(acc, part_rr, T.ntiles, t);
     tile_partials_rowlanes<2>(acc2, part_rz, T.ntiles, t);
+  });
+}
+
+// ---- x += dx, dx = 0 for every row and right-hand side of the listed tiles'
+//      systems: before a residual replacement (guard 2: the systems being
+//      replaced) and at the end of the solve (guard 0: all)
+__global__ __launch_bounds__(kBT) void kb_fold(Tiles T, QView Qv, float* __restrict__ dx, double* __restrict__ x,
+                                               BState S, int guard) {
+  walk_tiles(Qv, [&](int t) {
+    const int s = T.sys[t];
+    if (guard == 2 && S.sys_rel[s] == 0) return;
+    const int64_t o0 = (int64_t)T.row0[t] * KS * 3, n = (int64_t)T.nrows[t] * KS * 3;
+    for (int64_t i = threadIdx.x; i < n; i += kBT) {
+      const float d = dx[o0 + i];
+      if (d != 0.0f) {
+        x[o0 + i] += (double)d;
+        dx[o0 + i] = 0.0f;
+      }
+    }
   });
 }
 
@@ -742,7 +762,7 @@ struct hecmw33_batch {
   int32_t ntiles = 0;
   BMem rowptr, col, val64, val32, alu32, d64;
   BMem tile_row0, tile_nrows, tile_sys, sys_tile0, sys_ntile;
-  BMem b, x, r, z, p, q, part, state;
+  BMem b, x, dx, r, z, p, q, part, state;
   BState S{};
   Tiles T{};
   int variant = -1;
@@ -762,7 +782,7 @@ struct hecmw33_batch {
     if (stream) (void)hipStreamSynchronize(stream);
     if (graph) (void)hipGraphExecDestroy(graph);
     for (BMem* m : {&rowptr, &col, &val64, &val32, &alu32, &d64, &tile_row0, &tile_nrows, &tile_sys, &sys_tile0,
-                    &sys_ntile, &b, &x, &r, &z, &p, &q, &part, &state, &qtile, &qn})
+                    &sys_ntile, &b, &x, &dx, &r, &z, &p, &q, &part, &state, &qtile, &qn})
       m->release();
     if (h_active) (void)hipHostFree(h_active);
     if (stream) (void)hipStreamDestroy(stream);
@@ -820,10 +840,11 @@ static void batch_enqueue_iteration(hecmw33_batch& B) {
                            B.val32.p, p, r, q, B.S, part, 1);
   kb_tail<kbAlpha, 2><<<B.nsys, kBT, 0, st>>>(B.T, B.S, part);
   kb_update_diag<<<B.grid_upd, kBT, 0, st>>>(B.T, qview(B, kqActive), B.alu32.as<float>(), p, q,
-                                             B.x.as<double>(), r, z, B.S, part, part_rz);
+                                             B.dx.as<float>(), r, z, B.S, part, part_rz);
   kb_tail<kbRes, 1><<<B.nsys, kBT, 0, st>>>(B.T, B.S, part);
-  // the systems whose residuals are replaced now
+  // the systems whose residuals are replaced now, their x brought up to date
   kb_queue<kqRel><<<1, 1024, 0, st>>>(B.T, B.nsys, B.S, queue_of(B));
+  kb_fold<<<B.grid_res, kBT, 0, st>>>(B.T, qview(B, kqRel), B.dx.as<float>(), B.x.as<double>(), B.S, 2);
   kVariants64[B.variant64].res(B.grid_res, st, B.T, qview(B, kqRel), B.rowptr.as<int32_t>(),
                                B.col.as<int32_t>(), B.val64.p, B.x.p, B.b.p, r, B.S, part, 2);
   // systems whose residuals were just replaced: z from the new r
@@ -1147,7 +1168,8 @@ static int32_t batch_load(hecmw33_batch& B, int32_t nsys, const hecmw33_sys* sys
       B.tile_row0.ensure(sizeof(int32_t) * B.ntiles) || B.tile_nrows.ensure(sizeof(int32_t) * B.ntiles) ||
       B.tile_sys.ensure(sizeof(int32_t) * B.ntiles) || B.sys_tile0.ensure(sizeof(int32_t) * nsys) ||
       B.sys_ntile.ensure(sizeof(int32_t) * nsys) || B.b.ensure(sizeof(double) * nv) ||
-      B.x.ensure(sizeof(double) * nv) || B.r.ensure(sizeof(float) * nv) || B.z.ensure(sizeof(float) * nv) ||
+      B.x.ensure(sizeof(double) * nv) || B.dx.ensure(sizeof(float) * nv) || B.r.ensure(sizeof(float) * nv) ||
+      B.z.ensure(sizeof(float) * nv) ||
       B.p.ensure(sizeof(float) * nv) || B.q.ensure(sizeof(float) * nv) ||
       B.part.ensure(sizeof(double) * 4 * KS * (size_t)B.ntiles) || batch_state_layout(B) ||
       B.qtile.ensure(sizeof(int32_t) * kqLists * (size_t)B.ntiles) || B.qn.ensure(sizeof(int32_t) * kqLists))
@@ -1232,7 +1254,7 @@ static int32_t batch_solve(hecmw33_batch& B, const int32_t* iarray, const double
   double* part = B.part.as<double>();
   // x0 = 0; r = b - A x0 in FP64 -> r32; b.b; instance state
   BHIPC(hipMemsetAsync(B.x.p, 0, sizeof(double) * nv, st));
-  for (BMem* m : {&B.z, &B.p, &B.q}) kb_fill0<<<grid_of(nv), kBT, 0, st>>>(nv, m->as<float>());
+  for (BMem* m : {&B.dx, &B.z, &B.p, &B.q}) kb_fill0<<<grid_of(nv), kBT, 0, st>>>(nv, m->as<float>());
   kb_bb<<<B.ntiles, kBT, 0, st>>>(B.T, B.b.as<double>(), part);
   std::vector<int32_t> ones((size_t)B.nsys * KS, 1), sys1(B.nsys, 1);
   BHIPC(hipMemcpyAsync(B.S.rel, ones.data(), sizeof(int32_t) * ones.size(), hipMemcpyHostToDevice, st));
@@ -1275,6 +1297,9 @@ static int32_t batch_solve(hecmw33_batch& B, const int32_t* iarray, const double
       if (B.h_active[slot ^ 1] == 0) finished = true;
     }
   }
+  // the increments of instances that stopped without a final replacement
+  kb_fold<<<B.ntiles, kBT, 0, st>>>(B.T, qall(), B.dx.as<float>(), B.x.as<double>(), B.S, 0);
+  BHIPC(hipGetLastError());
   BHIPC(hipStreamSynchronize(st));
   (void)hipEventDestroy(ev[0]);
   (void)hipEventDestroy(ev[1]);
