@@ -148,8 +148,9 @@ __global__ void k_coarse_inv32(int32_t nc, const double* __restrict__ Ei, int32_
   W[t] = v;
 }
 
-// W = E^-1 in FP64, full symmetric (the twelve-mode step: with the learned modes
-// E is too ill-conditioned for an FP32 inverse)
+// W = the lower triangle of Ei mirrored into a full FP64 array: L^-1 below and
+// on the diagonal, L^-T above (the twelve-mode step: with the learned modes E
+// is too ill-conditioned for FP32)
 __global__ void k_coarse_inv64(int32_t nc, const double* __restrict__ Ei, int32_t lda, double* __restrict__ W,
                                int32_t ldw) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -163,7 +164,11 @@ __global__ void k_coarse_inv64(int32_t nc, const double* __restrict__ Ei, int32_
   W[t] = v;
 }
 
-// y = W c in FP64, one wavefront per row, double2 loads (ldw a multiple of 64)
+// y = W c in FP64, one wavefront per row, double2 loads (ldw a multiple of 64).
+// W holds L^-1 in its lower triangle and L^-T in its upper one (the mirror):
+// PART 1 takes the columns j <= row (y = L^-1 c), PART 2 the columns j >= row
+// (y = L^-T c), each half of the row's bytes
+template <int PART>
 __global__ __launch_bounds__(kCoarseBlock) void k_coarse_gemv64(int32_t nc, int32_t ldw, const double* __restrict__ W,
                                                                  const double* __restrict__ c, double* __restrict__ y,
                                                                  const KrylovState* st) {
@@ -173,9 +178,18 @@ __global__ __launch_bounds__(kCoarseBlock) void k_coarse_gemv64(int32_t nc, int3
   if (row >= nc) return;
   const double2* w = reinterpret_cast<const double2*>(W + (int64_t)row * ldw);
   const double2* c2 = reinterpret_cast<const double2*>(c);
+  const int k0 = PART == 2 ? row / 2 : 0, k1 = PART == 1 ? row / 2 + 1 : ldw / 2;
   double acc = 0.0;
-  for (int k = lane; k < ldw / 2; k += kWave) {
-    const double2 a = w[k], b = c2[k];
+  for (int k = k0 + lane; k < k1; k += kWave) {
+    double2 a = w[k];
+    const double2 b = c2[k];
+    if (PART == 1) {
+      if (2 * k > row) a.x = 0.0;
+      if (2 * k + 1 > row) a.y = 0.0;
+    } else {
+      if (2 * k < row) a.x = 0.0;
+      if (2 * k + 1 < row) a.y = 0.0;
+    }
     acc += a.x * b.x + a.y * b.y;
   }
 #pragma unroll
@@ -798,7 +812,8 @@ void coarse_sym_pre(const CoarseOp& op, int32_t n, const int2* sc, const int32_t
   const int gr = (op.nc + kCoarseWaves - 1) / kCoarseWaves;
   if (op.nm == 12) {
     k_coarse_restrict12<<<op.n_agg, kCoarseBlock, 0, s>>>(op.aptr, op.anodes, op.dv, op.xm, r, nullptr, op.cv64, st);
-    k_coarse_gemv64<<<gr, kCoarseBlock, 0, s>>>(op.nc, op.ldw, op.W64, op.cv64, op.yv64, st);
+    k_coarse_gemv64<1><<<gr, kCoarseBlock, 0, s>>>(op.nc, op.ldw, op.W64, op.cv64, op.tv64, st);
+    k_coarse_gemv64<2><<<gr, kCoarseBlock, 0, s>>>(op.nc, op.ldw, op.W64, op.tv64, op.yv64, st);
     k_coarse_pre12<<<coarse_vec_grid(n), kCoarseBlock, 0, s>>>(n, op.agg, op.dv, op.xm, sc, aggW, W, op.yv64, r, z,
                                                                r1, st);
   } else {
@@ -819,7 +834,8 @@ int coarse_sym_post(const CoarseOp& op, int32_t n, const int2* sc, const int32_t
   const int g = coarse_vec_grid(n);
   if (op.nm == 12) {
     k_coarse_restrict12<<<op.n_agg, kCoarseBlock, 0, s>>>(op.aptr, op.anodes, op.dv, op.xm, r, w, op.cv64, st);
-    k_coarse_gemv64<<<gr, kCoarseBlock, 0, s>>>(op.nc, op.ldw, op.W64, op.cv64, op.yv64, st);
+    k_coarse_gemv64<1><<<gr, kCoarseBlock, 0, s>>>(op.nc, op.ldw, op.W64, op.cv64, op.tv64, st);
+    k_coarse_gemv64<2><<<gr, kCoarseBlock, 0, s>>>(op.nc, op.ldw, op.W64, op.tv64, op.yv64, st);
     k_coarse_post12<<<g, kCoarseBlock, 0, s>>>(n, op.agg, op.dv, op.xm, sc, aggW, W, op.yv64, z, w, az, r, q, st,
                                                part);
   } else {

@@ -280,8 +280,9 @@ struct CoarseOp {
   float* W = nullptr;                  // E^-1, FP32, nc x ldw row-major (six modes)
   float* cv = nullptr;                 // [ldw] Z^T r (entries nc..ldw-1 stay 0)
   float* yv = nullptr;                 // [nc] E^-1 Z^T r
-  double* W64 = nullptr;               // twelve modes: E^-1 in FP64 (nc x ldw), and c, y in FP64
-  double* cv64 = nullptr;
+  double* W64 = nullptr;               // twelve modes: L^-1 (E = L L^T) in FP64, mirrored into a full
+  double* cv64 = nullptr;              // nc x ldw array, and c, L^-1 c and y in FP64
+  double* tv64 = nullptr;
   double* yv64 = nullptr;
 };
 // E = Z^T A Z: the nm x nm block of every coupled aggregate pair (pa[p], pb[p]), summed
@@ -289,6 +290,8 @@ struct CoarseOp {
 // (Eblk: nm*nm doubles per pair)
 void coarse_assemble(const Sell<double>& A, const CoarseOp& op, int32_t npair, const int32_t* pa, const int32_t* pb,
                      const int32_t* pn_ptr, const int32_t* pn, double* Eblk, double* E, int32_t lda, hipStream_t s);
+// the iterations' copies: E^-1 (potri's lower triangle) mirrored into FP32 W,
+// or L^-1 (trtri's lower triangle) mirrored into FP64 W64
 void coarse_inverse_f32(const CoarseOp& op, const double* Ei, int32_t lda, hipStream_t s);
 void coarse_inverse_f64(const CoarseOp& op, const double* Ei, int32_t lda, hipStream_t s);
 // The symmetric two-level step (W = A Z per node and touched aggregate):

@@ -287,7 +287,7 @@ struct Context {
   int32_t coarse_nm_num = 0;      // modes per aggregate of the E^-1 built (6 or 12)
   int32_t coarse_w_nm = 0;        // modes per aggregate of the A Z built
   int64_t coarse_nw = 0;          // node-aggregate entries of A Z
-  DMem cg_agg, cg_aptr, cg_anodes, cg_dv, cg_pa, cg_pb, cg_pnptr, cg_pn, cg_Eblk, cg_E, cg_W, cg_cv, cg_yv, cg_info;
+  DMem cg_agg, cg_aptr, cg_anodes, cg_dv, cg_pa, cg_pb, cg_pnptr, cg_pn, cg_Eblk, cg_E, cg_W, cg_cv, cg_tv, cg_yv, cg_info;
   DMem cg_ncnt, cg_nagg, cg_AZ, cg_sc, cg_aggW, cg_kept, vr1_32, vt32, vw32, vaz32;
   rocblas_handle blas = nullptr;
   // multipoint constraints (hecMESH%mpc) for the MPC-CG product T^T A T
@@ -329,7 +329,7 @@ struct Context {
                     &vmask})
       m->release();
     for (DMem* m : {&cg_agg, &cg_aptr, &cg_anodes, &cg_dv, &cg_pa, &cg_pb, &cg_pnptr, &cg_pn, &cg_Eblk, &cg_E, &cg_W,
-                    &cg_cv, &cg_yv, &cg_info, &cg_ncnt, &cg_nagg, &cg_AZ, &cg_sc, &cg_aggW, &cg_kept, &vr1_32, &vt32, &vw32, &vaz32})
+                    &cg_cv, &cg_tv, &cg_yv, &cg_info, &cg_ncnt, &cg_nagg, &cg_AZ, &cg_sc, &cg_aggW, &cg_kept, &vr1_32, &vt32, &vw32, &vaz32})
       m->release();
     if (blas) rocblas_destroy_handle(blas);
     for (DMem* m : {&mpc_slave, &mpc_mptr, &mpc_mdof, &mpc_mval, &mpc_tptr, &mpc_tslave, &mpc_tval, &mpc_w1, &mpc_w2})
@@ -1088,8 +1088,9 @@ static int32_t coarse_symbolic(Context& c, int32_t target) {
 }
 
 // E = Z^T A Z on the device with nm modes per aggregate (6 rigid-body; 12 with
-// the learned ones in cg_xm), Cholesky factor and inverse by rocSOLVER (FP64),
-// then the FP32 copy the iterations read
+// the learned ones in cg_xm), Cholesky factor and inverse (six modes) or
+// inverse factor (twelve) by rocSOLVER in FP64, then the copy the iterations
+// read
 static int32_t coarse_numeric(Context& c, int32_t nm) {
   CoarseOp& op = c.coarse;
   op.nm = nm;
@@ -1101,7 +1102,7 @@ static int32_t coarse_numeric(Context& c, int32_t nm) {
   const size_t we = nm == 12 ? sizeof(double) : sizeof(float);
   if (c.cg_Eblk.ensure(sizeof(double) * nm * nm * (size_t)std::max(c.coarse_npair, 1)) ||
       c.cg_E.ensure(sizeof(double) * (size_t)nc * nc) || c.cg_W.ensure(we * (size_t)nc * op.ldw) ||
-      c.cg_cv.ensure(we * op.ldw) || c.cg_yv.ensure(we * op.ldw))
+      c.cg_cv.ensure(we * op.ldw) || c.cg_yv.ensure(we * op.ldw) || c.cg_tv.ensure(we * op.ldw))
     return HECMW33_ERROR_DEVICE;
   op.W = nm == 12 ? nullptr : c.cg_W.as<float>();
   op.cv = nm == 12 ? nullptr : c.cg_cv.as<float>();
@@ -1109,7 +1110,9 @@ static int32_t coarse_numeric(Context& c, int32_t nm) {
   op.W64 = nm == 12 ? c.cg_W.as<double>() : nullptr;
   op.cv64 = nm == 12 ? c.cg_cv.as<double>() : nullptr;
   op.yv64 = nm == 12 ? c.cg_yv.as<double>() : nullptr;
+  op.tv64 = nm == 12 ? c.cg_tv.as<double>() : nullptr;
   HIPC(hipMemsetAsync(c.cg_cv.p, 0, we * op.ldw, c.stream));
+  HIPC(hipMemsetAsync(c.cg_tv.p, 0, we * op.ldw, c.stream));
   HIPC(hipMemsetAsync(c.cg_E.p, 0, sizeof(double) * (size_t)nc * nc, c.stream));
   coarse_assemble(c.A.view, op, c.coarse_npair, c.cg_pa.as<int32_t>(), c.cg_pb.as<int32_t>(),
                   c.cg_pnptr.as<int32_t>(), c.cg_pn.as<int32_t>(), c.cg_Eblk.as<double>(), c.cg_E.as<double>(), nc,
@@ -1124,10 +1127,17 @@ static int32_t coarse_numeric(Context& c, int32_t nm) {
     set_error("rocblas_set_stream failed");
     return HECMW33_ERROR_DEVICE;
   }
+  // six modes: E^-1 (potri) for one FP32 product per coarse solve; twelve:
+  // L^-1 (trtri) for two FP64 triangular products, y = L^-T (L^-1 c), the
+  // same bytes read without forming L^-T L^-1 (at 12,000 coarse unknowns
+  // potri takes 57 ms, trtri 22 ms)
   int32_t* info = c.cg_info.as<int32_t>();
   if (rocsolver_dpotrf(c.blas, rocblas_fill_lower, nc, c.cg_E.as<double>(), nc, info) != rocblas_status_success ||
-      rocsolver_dpotri(c.blas, rocblas_fill_lower, nc, c.cg_E.as<double>(), nc, info + 1) != rocblas_status_success) {
-    set_error("rocsolver potrf/potri failed");
+      (nm == 12 ? rocsolver_dtrtri(c.blas, rocblas_fill_lower, rocblas_diagonal_non_unit, nc, c.cg_E.as<double>(), nc,
+                                   info + 1)
+                : rocsolver_dpotri(c.blas, rocblas_fill_lower, nc, c.cg_E.as<double>(), nc, info + 1)) !=
+          rocblas_status_success) {
+    set_error("rocsolver potrf/potri/trtri failed");
     return HECMW33_ERROR_DEVICE;
   }
   int32_t hinfo[2] = {0, 0};
