@@ -43,12 +43,12 @@ HBM_PEAK_GBS = 8000.0
 # two rocprofv3 passes (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000, the factors a
 # 1 GiB copy of the same run gives); profiles/spmv_hex8_100_traffic.md
 SPMV_TRAFFIC_100 = 2199539866
-# HBM bytes per FP32 multicolour SSOR application on the 160^3 lognormal cube
-# (stage, 17 + 16 colour launches, unstage), from two rocprofv3 passes over
-# `bench.py --only ssor` (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000, the factors of
-# the run's 1 GiB copy; every read request of the sweeps is 128 B, as the
-# copy's); profiles/ssor_fp32_hex8_160_lognormal_traffic.md
-SSOR_TRAFFIC_160_FP32 = 6736586215
+# HBM bytes per FP32 multicolour SSOR application (BF16 L'/U' values) on the
+# 160^3 lognormal cube (stage, 17 + 16 colour launches, unstage), from two
+# rocprofv3 passes over `bench.py --only ssor` (FETCH_SIZE x 2.000, WRITE_SIZE
+# x 1.000, the factors of the run's 1 GiB copy; every read request of the
+# sweeps is 128 B, as the copy's); profiles/ssor_fp32_hex8_160_lognormal_traffic.md
+SSOR_TRAFFIC_160 = 4482793267
 
 # (name, Iarray 41, Iarray 43, Iarray 44): FP64 CG; flexible CG with FP32
 # operators; with the symmetric two-level step (rigid-body modes), and in its
@@ -66,11 +66,13 @@ def spmv_bytes(N, NP, nnzb, w):
     return 9 * w * nnzb + 4 * nnzb + 4 * (N + 1) + 3 * w * NP + 3 * w * N
 
 
-def ssor_bytes(N, nnz_lu, w):
-    """Algorithmic bytes of one multicolour SSOR application: L'/U' values and
-    column indices, the 3x3 LU factors, r read and z written, and the sweep
-    vector read and written by each of the two sweeps (gathers counted once)."""
-    return (9 * w + 4) * nnz_lu + 9 * w * N + 6 * 3 * w * N
+def ssor_bytes(N, nnz_lu, w, wv=None):
+    """Algorithmic bytes of one multicolour SSOR application: L'/U' values
+    (wv bytes each, w when not given) and column indices, the 3x3 LU factors,
+    r read and z written, and the sweep vector read and written by each of the
+    two sweeps (gathers counted once); w is the vectors' and factors' width."""
+    wv = w if wv is None else wv
+    return (9 * wv + 4) * nnz_lu + 9 * w * N + 6 * 3 * w * N
 
 
 def log(*a):
@@ -216,13 +218,15 @@ def spmv_median(R, fp32, reps):
     return median_ms(lambda: R.ctx.matvec_device(xin.data_ptr(), yout.data_ptr(), 1, fp32=fp32), reps)
 
 
-def ssor_ms(R, fp32, reps=20):
+def ssor_ms(R, fp32, reps=20, bf16=True):
+    """mean time (ms) of `reps` SSOR applications; FP32: with BF16 (the flexible
+    CG's default) or FP32 L'/U' values"""
     import torch
     r = torch.rand(3 * R.m.NP, dtype=torch.float32 if fp32 else torch.float64, device=R.x.device)
     z = torch.zeros_like(r)
     torch.cuda.synchronize()
-    R.ctx.ssor_device(r.data_ptr(), z.data_ptr(), 2, fp32=fp32)  # warm
-    return R.ctx.ssor_device(r.data_ptr(), z.data_ptr(), reps, fp32=fp32)
+    R.ctx.ssor_device(r.data_ptr(), z.data_ptr(), 2, fp32=fp32, bf16=bf16)  # warm
+    return R.ctx.ssor_device(r.data_ptr(), z.data_ptr(), reps, fp32=fp32, bf16=bf16)
 
 
 # ---- C3: the headline -------------------------------------------------------------------
@@ -264,8 +268,10 @@ def bench_c3(args, dev):
     # the FP32 SpMV and SSOR application of the learned step's operators, and the FP64 ones
     ms32 = spmv_median(R, True, args.spmv_reps)
     ssor32 = ssor_ms(R, True)
+    ssor32v = ssor_ms(R, True, bf16=False)
     ssor64 = ssor_ms(R, False)
-    sb32 = ssor_bytes(m.N, m.NPL + m.NPU, 4)
+    sb32 = ssor_bytes(m.N, m.NPL + m.NPU, 4, 2)
+    sb32v = ssor_bytes(m.N, m.NPL + m.NPU, 4)
     sb64 = ssor_bytes(m.N, m.NPL + m.NPU, 8)
     out = {
         "workload": "%d^3 HEX8, lognormal E in [1e3,1e6], nu in [0.2,0.4], seed 0, %d DOF, %d blocks"
@@ -281,13 +287,16 @@ def bench_c3(args, dev):
         "spmv_fp32_bytes": spmv_bytes(m.N, m.NP, nnzb, 4),
         "ssor_fp32_ms": round(ssor32, 4), "ssor_fp32_bytes": sb32,
         "ssor_fp32_GBs": round(sb32 / (ssor32 * 1e-3) / 1e9, 1),
+        "ssor_fp32_values_ms": round(ssor32v, 4), "ssor_fp32_values_bytes": sb32v,
+        "ssor_fp32_values_GBs": round(sb32v / (ssor32v * 1e-3) / 1e9, 1),
         "ssor_fp64_ms": round(ssor64, 4), "ssor_fp64_GBs": round(sb64 / (ssor64 * 1e-3) / 1e9, 1),
         "ncolor": R.ctx.info()["ncolor"],
         "learned_vs_rigid_body_modes": learned_vs_rigid(modes),
     }
-    log("C3 %s: %.4f s/step, %d iterations; SpMV FP32 %.4f ms (%.0f GB/s); SSOR FP32 %.4f ms (%.0f GB/s), "
-        "FP64 %.4f ms" % (head, out["seconds_per_step"], r_last.iter, ms32, out["spmv_fp32_GBs"], ssor32,
-                          out["ssor_fp32_GBs"], ssor64))
+    log("C3 %s: %.4f s/step, %d iterations; SpMV FP32 %.4f ms (%.0f GB/s); SSOR FP32 %.4f ms (%.0f GB/s; "
+        "FP32 L'/U' values %.4f ms), FP64 %.4f ms" % (head, out["seconds_per_step"], r_last.iter, ms32,
+                                                       out["spmv_fp32_GBs"], ssor32, out["ssor_fp32_GBs"], ssor32v,
+                                                       ssor64))
     R.close()
     return out, m, modes["fp64_ssor_cg"]["iterations"]
 
@@ -313,7 +322,7 @@ def bench_ssor_probe(args, dev):
     spmv = spmv_median(R, True, 10)
     nnzb = m.N + m.NPL + m.NPU
     out = {"workload": "%d^3 HEX8 lognormal, %d DOF" % (n, 3 * m.N), "copy_bytes": 2 * a.numel() * 8,
-           "ssor_fp32_ms": round(ssor, 4), "ssor_fp32_bytes": ssor_bytes(m.N, m.NPL + m.NPU, 4),
+           "ssor_fp32_ms": round(ssor, 4), "ssor_fp32_bytes": ssor_bytes(m.N, m.NPL + m.NPU, 4, 2),
            "spmv_fp32_ms": round(spmv, 4), "spmv_fp32_bytes": spmv_bytes(m.N, m.NP, nnzb, 4),
            "ncolor": R.ctx.info()["ncolor"]}
     R.close()
@@ -556,11 +565,11 @@ def main():
                                "rel_resid_true": fp64["rel_resid_true"]},
         "speedup_vs_plain_fp64": round(fp64["seconds"] / value, 3),
         "roofline": {
-            "bound": "hbm", "kernel": "multicolour SSOR application, FP32 (k_ssor_stage, %d x k_ssor64_fwd/bwd "
-                                      "colour launches, k_ssor_unstage)" % c3["ncolor"],
+            "bound": "hbm", "kernel": "multicolour SSOR application, FP32 with BF16 L'/U' values (k_ssor_stage, "
+                                      "%d x k_ssor64_fwd/bwd colour launches, k_ssor_unstage)" % c3["ncolor"],
             "achieved": c3["ssor_fp32_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(c3["ssor_fp32_GBs"] / HBM_PEAK_GBS, 4),
-            "traffic": SSOR_TRAFFIC_160_FP32 if n == 160 else None,
+            "traffic": SSOR_TRAFFIC_160 if n == 160 else None,
             "bytes_per_application": c3["ssor_fp32_bytes"], "avg_application_ms": c3["ssor_fp32_ms"],
             "spmv_fp32_GBs": c3["spmv_fp32_GBs"], "spmv_fp32_frac": round(c3["spmv_fp32_GBs"] / HBM_PEAK_GBS, 4),
         },

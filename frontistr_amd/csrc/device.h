@@ -94,7 +94,10 @@ struct SsorOp {
   const int32_t* node_slot;    // slot of every internal node (the inverse of row_old)
   int32_t n;                   // internal nodes
   int32_t last_bwd_empty;      // 1: the last colour's rows have no U' entries (their backward step is z -= 0)
+  const uint16_t* Lv16 = nullptr;  // FP32 sweeps: L'/U' values in BF16 instead of L.val/U.val (null: T)
+  const uint16_t* Uv16 = nullptr;
 };
+void launch_to_bf16(int64_t n, const float* a, uint16_t* b, hipStream_t s);
 void ssor_apply(const SsorOp<double>& op, const double* r, double* z, const KrylovState* st, hipStream_t s);
 void ssor_apply_f32(const SsorOp<float>& op, const float* r, float* z, const KrylovState* st, hipStream_t s);
 

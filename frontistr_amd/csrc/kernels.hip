@@ -716,6 +716,16 @@ __device__ __forceinline__ void load_alu(const T* __restrict__ alu, int s, int r
   for (int e = 0; e < 9; ++e) a[e] = f[e * kRowsS];
 }
 
+// an L'/U' value as the sweep's arithmetic type: stored as T, or as BF16 (the
+// upper 16 bits of an FP32) for the FP32 sweeps
+template <typename T, typename S>
+__device__ __forceinline__ T ldv(const S* __restrict__ p) {
+  if constexpr (sizeof(S) == 2)
+    return __uint_as_float((uint32_t)*p << 16);
+  else
+    return *p;
+}
+
 template <typename T>
 __device__ __forceinline__ T pick3(int d, T a, T b, T c) {
   return d == 0 ? a : (d == 1 ? b : c);
@@ -725,8 +735,8 @@ __device__ __forceinline__ T pick3(int d, T a, T b, T c) {
 // lane's component sum; all CH column indices, gathers and values are
 // requested before the first one is used.  SIGN -1: sw - a*x (forward),
 // +1: sw + a*x (backward), each block as the reference writes it.
-template <typename T, int CH, bool DESC, int SIGN, bool FULL>
-__device__ __forceinline__ void ssor_chunk(const int32_t* __restrict__ colp, const T* __restrict__ vp,
+template <typename T, typename S, int CH, bool DESC, int SIGN, bool FULL>
+__device__ __forceinline__ void ssor_chunk(const int32_t* __restrict__ colp, const S* __restrict__ vp,
                                            const T* __restrict__ z, int k0, int len, int dd, T& sw) {
 #pragma clang fp contract(off)
   // FULL (every block live): no load or sum under a condition.  The row's
@@ -742,11 +752,11 @@ __device__ __forceinline__ void ssor_chunk(const int32_t* __restrict__ colp, con
   T xd[CH], v[CH][3];
 #pragma unroll
   for (int j = 0; j < CH; ++j) {
-    const T* __restrict__ vk = vp + (int64_t)kk[j] * 9 * kRowsS;
+    const S* __restrict__ vk = vp + (int64_t)kk[j] * 9 * kRowsS;
     xd[j] = z[3 * (int64_t)c[j] + dd];
-    v[j][0] = vk[0];
-    v[j][1] = vk[kRowsS];
-    v[j][2] = vk[2 * kRowsS];
+    v[j][0] = ldv<T>(vk);
+    v[j][1] = ldv<T>(vk + kRowsS);
+    v[j][2] = ldv<T>(vk + 2 * kRowsS);
   }
 #pragma unroll
   for (int j = 0; j < CH; ++j) {
@@ -771,8 +781,8 @@ __device__ __forceinline__ void ssor_chunk(const int32_t* __restrict__ colp, con
 //
 // Forward, rows of one colour independent:
 //   zs[i] = LU^-1 (zs[i] - sum_{j in L'(i), ascending} L'_ij zs[j])
-template <typename T, int CH>
-__global__ __launch_bounds__(kBlock) void k_ssor_fwd(Sell<T> L, const int32_t* __restrict__ row_old,
+template <typename T, typename S, int CH>
+__global__ __launch_bounds__(kBlock) void k_ssor_fwd(Sell<T> L, const S* __restrict__ vals, const int32_t* __restrict__ row_old,
                                                        const T* __restrict__ alu, int s0, int ns,
                                                        T* __restrict__ zs, const KrylovState* st) {
 #pragma clang fp contract(off)
@@ -789,12 +799,12 @@ __global__ __launch_bounds__(kBlock) void k_ssor_fwd(Sell<T> L, const int32_t* _
     load_alu<T>(alu, s, r, a);
     if (iold < 0) continue;
     const int32_t* __restrict__ colp = L.col + (int64_t)base * kRowsS + r;
-    const T* __restrict__ vp = L.val + ((int64_t)base * 9 + 3 * dd) * kRowsS + r;
+    const S* __restrict__ vp = vals + ((int64_t)base * 9 + 3 * dd) * kRowsS + r;
     T sw = zs[3 * slot + dd];
     {
       int k0 = 0;
-      for (; k0 + CH <= len; k0 += CH) ssor_chunk<T, CH, false, -1, true>(colp, vp, zs, k0, len, dd, sw);
-      if (k0 < len) ssor_chunk<T, CH, false, -1, false>(colp, vp, zs, k0, len, dd, sw);
+      for (; k0 + CH <= len; k0 += CH) ssor_chunk<T, S, CH, false, -1, true>(colp, vp, zs, k0, len, dd, sw);
+      if (k0 < len) ssor_chunk<T, S, CH, false, -1, false>(colp, vp, zs, k0, len, dd, sw);
     }
     T x0 = quad_bcast<0>(sw), x1 = quad_bcast<1>(sw), x2 = quad_bcast<2>(sw);
     lu3_solve<T>(a, x0, x1, x2);
@@ -804,8 +814,8 @@ __global__ __launch_bounds__(kBlock) void k_ssor_fwd(Sell<T> L, const int32_t* _
 
 // Backward, colours in reverse:
 //   zs[i] -= LU^-1 sum_{j in U'(i), descending} U'_ij zs[j]
-template <typename T, int CH>
-__global__ __launch_bounds__(kBlock) void k_ssor_bwd(Sell<T> U, const int32_t* __restrict__ row_old,
+template <typename T, typename S, int CH>
+__global__ __launch_bounds__(kBlock) void k_ssor_bwd(Sell<T> U, const S* __restrict__ vals, const int32_t* __restrict__ row_old,
                                                        const T* __restrict__ alu, int s0, int ns,
                                                        T* __restrict__ zs, const KrylovState* st) {
 #pragma clang fp contract(off)
@@ -822,13 +832,13 @@ __global__ __launch_bounds__(kBlock) void k_ssor_bwd(Sell<T> U, const int32_t* _
     load_alu<T>(alu, s, r, a);
     if (iold < 0) continue;
     const int32_t* __restrict__ colp = U.col + (int64_t)base * kRowsS + r;
-    const T* __restrict__ vp = U.val + ((int64_t)base * 9 + 3 * dd) * kRowsS + r;
+    const S* __restrict__ vp = vals + ((int64_t)base * 9 + 3 * dd) * kRowsS + r;
     const T zo = zs[3 * slot + dd];
     T sw = 0;
     {
       int top = len - 1;
-      for (; top - CH + 1 >= 0; top -= CH) ssor_chunk<T, CH, true, 1, true>(colp, vp, zs, top, len, dd, sw);
-      if (top >= 0) ssor_chunk<T, CH, true, 1, false>(colp, vp, zs, top, len, dd, sw);
+      for (; top - CH + 1 >= 0; top -= CH) ssor_chunk<T, S, CH, true, 1, true>(colp, vp, zs, top, len, dd, sw);
+      if (top >= 0) ssor_chunk<T, S, CH, true, 1, false>(colp, vp, zs, top, len, dd, sw);
     }
     T x0 = quad_bcast<0>(sw), x1 = quad_bcast<1>(sw), x2 = quad_bcast<2>(sw);
     lu3_solve<T>(a, x0, x1, x2);
@@ -837,13 +847,21 @@ __global__ __launch_bounds__(kBlock) void k_ssor_bwd(Sell<T> U, const int32_t* _
   }
 }
 
+// FP32 -> BF16, round to nearest even (finite values)
+__global__ __launch_bounds__(kBlock) void k_to_bf16(int64_t n, const float* __restrict__ a, uint16_t* __restrict__ b) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const uint32_t u = __float_as_uint(a[i]);
+    b[i] = (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+  }
+}
+
 // ---- SSOR sweeps, SELL-64 layout: one lane per row ------------------------------
 // For large colours (many more slices than resident waves) a lane per row
 // needs a quarter of the waves of the four-lane layout for the same dependent
 // load chains.  Same arithmetic per row and component as ssor_chunk + the
 // quad layout: the reference's statements in the reference's order.
-template <typename T, int CH, bool DESC, int SIGN, bool FULL>
-__device__ __forceinline__ void ssor64_chunk(const int32_t* __restrict__ colp, const T* __restrict__ vp,
+template <typename T, typename S, int CH, bool DESC, int SIGN, bool FULL>
+__device__ __forceinline__ void ssor64_chunk(const int32_t* __restrict__ colp, const S* __restrict__ vp,
                                              const T* __restrict__ z, int k0, int len, T (&sw)[3]) {
 #pragma clang fp contract(off)
   int kk[CH], c[CH];
@@ -856,9 +874,9 @@ __device__ __forceinline__ void ssor64_chunk(const int32_t* __restrict__ colp, c
   T v[CH][9], x[CH][3];
 #pragma unroll
   for (int j = 0; j < CH; ++j) {
-    const T* __restrict__ vk = vp + (int64_t)kk[j] * 9 * kRowsA;
+    const S* __restrict__ vk = vp + (int64_t)kk[j] * 9 * kRowsA;
 #pragma unroll
-    for (int e = 0; e < 9; ++e) v[j][e] = vk[e * kRowsA];
+    for (int e = 0; e < 9; ++e) v[j][e] = ldv<T>(vk + e * kRowsA);
 #pragma unroll
     for (int d = 0; d < 3; ++d) x[j][d] = z[3 * (int64_t)c[j] + d];
   }
@@ -882,8 +900,8 @@ __device__ __forceinline__ void load_alu64(const T* __restrict__ alu, int s, int
   for (int e = 0; e < 9; ++e) a[e] = f[e * kRowsA];
 }
 
-template <typename T, int CH>
-__global__ __launch_bounds__(kBlock) void k_ssor64_fwd(Sell<T> L, const int32_t* __restrict__ row_old,
+template <typename T, typename S, int CH>
+__global__ __launch_bounds__(kBlock) void k_ssor64_fwd(Sell<T> L, const S* __restrict__ vals, const int32_t* __restrict__ row_old,
                                                          const T* __restrict__ alu, int s0, int ns,
                                                          T* __restrict__ zs, const KrylovState* st) {
 #pragma clang fp contract(off)
@@ -900,12 +918,12 @@ __global__ __launch_bounds__(kBlock) void k_ssor64_fwd(Sell<T> L, const int32_t*
     load_alu64<T>(alu, s, lane, a);
     if (iold < 0) continue;
     const int32_t* __restrict__ colp = L.col + (int64_t)base * kRowsA + lane;
-    const T* __restrict__ vp = L.val + (int64_t)base * 9 * kRowsA + lane;
+    const S* __restrict__ vp = vals + (int64_t)base * 9 * kRowsA + lane;
     T sw[3] = {zs[3 * slot], zs[3 * slot + 1], zs[3 * slot + 2]};
     {
       int k0 = 0;
-      for (; k0 + CH <= len; k0 += CH) ssor64_chunk<T, CH, false, -1, true>(colp, vp, zs, k0, len, sw);
-      if (k0 < len) ssor64_chunk<T, CH, false, -1, false>(colp, vp, zs, k0, len, sw);
+      for (; k0 + CH <= len; k0 += CH) ssor64_chunk<T, S, CH, false, -1, true>(colp, vp, zs, k0, len, sw);
+      if (k0 < len) ssor64_chunk<T, S, CH, false, -1, false>(colp, vp, zs, k0, len, sw);
     }
     lu3_solve<T>(a, sw[0], sw[1], sw[2]);
 #pragma unroll
@@ -913,8 +931,8 @@ __global__ __launch_bounds__(kBlock) void k_ssor64_fwd(Sell<T> L, const int32_t*
   }
 }
 
-template <typename T, int CH>
-__global__ __launch_bounds__(kBlock) void k_ssor64_bwd(Sell<T> U, const int32_t* __restrict__ row_old,
+template <typename T, typename S, int CH>
+__global__ __launch_bounds__(kBlock) void k_ssor64_bwd(Sell<T> U, const S* __restrict__ vals, const int32_t* __restrict__ row_old,
                                                          const T* __restrict__ alu, int s0, int ns,
                                                          T* __restrict__ zs, const KrylovState* st) {
 #pragma clang fp contract(off)
@@ -931,12 +949,12 @@ __global__ __launch_bounds__(kBlock) void k_ssor64_bwd(Sell<T> U, const int32_t*
     load_alu64<T>(alu, s, lane, a);
     if (iold < 0) continue;
     const int32_t* __restrict__ colp = U.col + (int64_t)base * kRowsA + lane;
-    const T* __restrict__ vp = U.val + (int64_t)base * 9 * kRowsA + lane;
+    const S* __restrict__ vp = vals + (int64_t)base * 9 * kRowsA + lane;
     T sw[3] = {0, 0, 0};
     {
       int top = len - 1;
-      for (; top - CH + 1 >= 0; top -= CH) ssor64_chunk<T, CH, true, 1, true>(colp, vp, zs, top, len, sw);
-      if (top >= 0) ssor64_chunk<T, CH, true, 1, false>(colp, vp, zs, top, len, sw);
+      for (; top - CH + 1 >= 0; top -= CH) ssor64_chunk<T, S, CH, true, 1, true>(colp, vp, zs, top, len, sw);
+      if (top >= 0) ssor64_chunk<T, S, CH, true, 1, false>(colp, vp, zs, top, len, sw);
     }
     lu3_solve<T>(a, sw[0], sw[1], sw[2]);
 #pragma unroll
@@ -1399,18 +1417,38 @@ static int ssor64_chunk_sel() {
 
 template <typename T, bool FWD, int CH>
 static void ssor_launch4(const SsorOp<T>& op, int g, int s0, int ns, const KrylovState* st, hipStream_t s) {
+  const uint16_t* v16 = FWD ? op.Lv16 : op.Uv16;
+  if constexpr (std::is_same<T, float>::value) {
+    if (v16) {
+      if constexpr (FWD)
+        k_ssor_fwd<T, uint16_t, CH><<<g, kBlock, 0, s>>>(op.L, v16, op.row_old, op.alu, s0, ns, op.zs, st);
+      else
+        k_ssor_bwd<T, uint16_t, CH><<<g, kBlock, 0, s>>>(op.U, v16, op.row_old, op.alu, s0, ns, op.zs, st);
+      return;
+    }
+  }
   if constexpr (FWD)
-    k_ssor_fwd<T, CH><<<g, kBlock, 0, s>>>(op.L, op.row_old, op.alu, s0, ns, op.zs, st);
+    k_ssor_fwd<T, T, CH><<<g, kBlock, 0, s>>>(op.L, op.L.val, op.row_old, op.alu, s0, ns, op.zs, st);
   else
-    k_ssor_bwd<T, CH><<<g, kBlock, 0, s>>>(op.U, op.row_old, op.alu, s0, ns, op.zs, st);
+    k_ssor_bwd<T, T, CH><<<g, kBlock, 0, s>>>(op.U, op.U.val, op.row_old, op.alu, s0, ns, op.zs, st);
 }
 
 template <typename T, bool FWD, int CH>
 static void ssor_launch64(const SsorOp<T>& op, int g, int s0, int ns, const KrylovState* st, hipStream_t s) {
+  const uint16_t* v16 = FWD ? op.Lv16 : op.Uv16;
+  if constexpr (std::is_same<T, float>::value) {
+    if (v16) {
+      if constexpr (FWD)
+        k_ssor64_fwd<T, uint16_t, CH><<<g, kBlock, 0, s>>>(op.L, v16, op.row_old, op.alu, s0, ns, op.zs, st);
+      else
+        k_ssor64_bwd<T, uint16_t, CH><<<g, kBlock, 0, s>>>(op.U, v16, op.row_old, op.alu, s0, ns, op.zs, st);
+      return;
+    }
+  }
   if constexpr (FWD)
-    k_ssor64_fwd<T, CH><<<g, kBlock, 0, s>>>(op.L, op.row_old, op.alu, s0, ns, op.zs, st);
+    k_ssor64_fwd<T, T, CH><<<g, kBlock, 0, s>>>(op.L, op.L.val, op.row_old, op.alu, s0, ns, op.zs, st);
   else
-    k_ssor64_bwd<T, CH><<<g, kBlock, 0, s>>>(op.U, op.row_old, op.alu, s0, ns, op.zs, st);
+    k_ssor64_bwd<T, T, CH><<<g, kBlock, 0, s>>>(op.U, op.U.val, op.row_old, op.alu, s0, ns, op.zs, st);
 }
 
 template <typename T, bool FWD>
@@ -1683,4 +1721,10 @@ void fcg_init_resid(const Sell<double>& a, const double* x, const double* b, flo
   k_tail<kTailStore, 1><<<1, kTailThreads, 0, s>>>(st, part, g, rr, nullptr);
 }
 
+}  // namespace hecmw33
+
+namespace hecmw33 {
+void launch_to_bf16(int64_t n, const float* a, uint16_t* b, hipStream_t s) {
+  k_to_bf16<<<vec_grid(n), kBlock, 0, s>>>(n, a, b);
+}
 }  // namespace hecmw33

@@ -102,6 +102,11 @@ constexpr int32_t kCoarseAggDefault = 1000;
 // Comput. 39 (2009) 340-370, whose spectrum is the symmetric step's; it is
 // not symmetric, which the flexible CG's explicit beta allows).
 constexpr int I_SOLVER_OPT4 = 44;
+// SOLVER_OPT(7) (Iarray 47): the L'/U' values the flexible CG's FP32 SSOR
+// sweeps read: 0 (default) BF16 copies (22 B per block instead of 40, the
+// sweeps' dominant stream; D^-1 stays FP32, and since the rounding is
+// entrywise M stays symmetric positive definite), 1 FP32
+constexpr int I_SOLVER_OPT7 = 47;
 // the dense coarse inverse is n_c^2 entries (n_c = 6 per aggregate): at most
 // 2048 aggregates (n_c = 12288, 0.6 GB in FP32)
 constexpr int32_t kCoarseAggMax = 2048;
@@ -142,7 +147,7 @@ struct SellHost {
 };
 
 struct SellDev {
-  DMem ptr, len, col, src, val, val32;
+  DMem ptr, len, col, src, val, val32, val16;
   Sell<double> view;
   Sell<float> view32;
 };
@@ -237,6 +242,8 @@ struct Context {
   int32_t ssor_last_bwd_empty = 0;  // the last colour's U' rows are all empty
   SellDev L, U;
   DMem row_old, node_slot, alu_ssor, vzs, vzs32;
+  bool ssor16 = true;          // the flexible CG's FP32 sweeps read BF16 copies of the L'/U' values
+  bool ssor16_valid = false;   // L.val16 / U.val16 hold the current values
   // DIAG
   bool diag_init = false;
   DMem row_diag, alu_diag;
@@ -318,9 +325,14 @@ struct Context {
     return SsorOp<double>{L.view,  U.view, row_old.as<int32_t>(), alu_ssor.as<double>(), color_slice.data(), ncolor,
                           vzs.as<double>(), node_slot.as<int32_t>(), N, ssor_last_bwd_empty};
   }
-  SsorOp<float> ssor_op32() {
-    return SsorOp<float>{L.view32, U.view32, row_old.as<int32_t>(), alu_ssor32.as<float>(), color_slice.data(),
-                         ncolor, vzs32.as<float>(), node_slot.as<int32_t>(), N, ssor_last_bwd_empty};
+  SsorOp<float> ssor_op32(bool v16) {
+    SsorOp<float> op{L.view32, U.view32, row_old.as<int32_t>(), alu_ssor32.as<float>(), color_slice.data(),
+                     ncolor, vzs32.as<float>(), node_slot.as<int32_t>(), N, ssor_last_bwd_empty};
+    if (v16) {
+      op.Lv16 = L.val16.as<uint16_t>();
+      op.Uv16 = U.val16.as<uint16_t>();
+    }
+    return op;
   }
 
   ~Context() {
@@ -339,7 +351,7 @@ struct Context {
                     &part, &hist})
       m->release();
     for (SellDev* s : {&A, &L, &U})
-      for (DMem* m : {&s->ptr, &s->len, &s->col, &s->src, &s->val, &s->val32}) m->release();
+      for (DMem* m : {&s->ptr, &s->len, &s->col, &s->src, &s->val, &s->val32, &s->val16}) m->release();
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
     if (h_st) (void)hipHostFree(h_st);
     if (stream) (void)hipStreamDestroy(stream);
@@ -686,6 +698,7 @@ static int32_t setup_ssor(Context& c, double sigma, int32_t ncolor_in, bool symb
   HIPC(hipStreamSynchronize(c.stream));
   c.ssor_init = true;
   c.ssor32_valid = false;
+  c.ssor16_valid = false;
   return 0;
 }
 
@@ -714,13 +727,16 @@ static int32_t round_sell(SellDev& d, hipStream_t s) {
   launch_to_f32(9 * d.view.nslots, d.view.val, d.view32.val, s);
   return 0;
 }
-static int32_t setup_f32(Context& c, int32_t kind) {
+// v16: the SSOR sweeps will read BF16 copies of the L'/U' values (rounded to
+// nearest even from the FP32 image)
+static int32_t setup_f32(Context& c, int32_t kind, bool v16) {
   if (!c.a32_valid) {
     if (int32_t e = round_sell(c.A, c.stream)) return e;
     c.a32_valid = true;
   }
   if (kind == 1 && !c.ssor32_valid) {
     if (round_sell(c.L, c.stream) || round_sell(c.U, c.stream)) return HECMW33_ERROR_DEVICE;
+    c.ssor16_valid = false;
     const int64_t nslot = c.ssor_rows;
     if (c.alu_ssor32.ensure(sizeof(float) * 9 * std::max<int64_t>(nslot, 1)) ||
         c.vzs32.ensure(sizeof(float) * 3 * std::max<int64_t>(nslot, 1)))
@@ -728,6 +744,14 @@ static int32_t setup_f32(Context& c, int32_t kind) {
     HIPC(hipMemsetAsync(c.vzs32.p, 0, sizeof(float) * 3 * std::max<int64_t>(nslot, 1), c.stream));
     launch_to_f32(9 * nslot, c.alu_ssor.as<double>(), c.alu_ssor32.as<float>(), c.stream);
     c.ssor32_valid = true;
+  }
+  if (kind == 1 && v16 && !c.ssor16_valid) {
+    for (SellDev* d : {&c.L, &c.U}) {
+      const int64_t n = 9 * std::max<int64_t>(d->view.nslots, 1);
+      if (d->val16.ensure(sizeof(uint16_t) * n)) return HECMW33_ERROR_DEVICE;
+      launch_to_bf16(9 * d->view.nslots, d->view32.val, d->val16.as<uint16_t>(), c.stream);
+    }
+    c.ssor16_valid = true;
   }
   if (kind == 3 && !c.diag32_valid) {
     const int64_t nslot = (int64_t)((c.N + kWave - 1) / kWave) * kWave;
@@ -1188,10 +1212,10 @@ static void enqueue_fcg_iteration(Context& c, int32_t kind, bool spmv64, hipEven
   float *r = c.vr32.as<float>(), *z = c.vz32.as<float>(), *p = c.vp32.as<float>(), *q = c.vq32.as<float>();
   if (ev) (void)hipEventRecord(ev[0], c.stream);
   if (c.coarse_use) {
-    precond_apply_f32(kind, c.ssor_op32(), c.alu_diag32.as<float>(), c.N, r, z, st, c.stream);
+    precond_apply_f32(kind, c.ssor_op32(c.ssor16), c.alu_diag32.as<float>(), c.N, r, z, st, c.stream);
     fcg_beta(st, part, coarse_apply(c.coarse, c.N, r, z, q, st, part, true, c.stream), c.stream);
   } else {
-    fcg_precond_dots(kind, c.ssor_op32(), c.alu_diag32.as<float>(), c.N, r, z, q, st, part, c.stream);
+    fcg_precond_dots(kind, c.ssor_op32(c.ssor16), c.alu_diag32.as<float>(), c.N, r, z, q, st, part, c.stream);
   }
   if (ev) (void)hipEventRecord(ev[1], c.stream);
   fcg_p(n3, z, p, st, c.stream);
@@ -1221,10 +1245,10 @@ static void enqueue_fcg_sym_iteration(Context& c, int32_t kind, bool spmv64, hip
   const float* AZ = c.cg_AZ.as<float>();
   if (ev) (void)hipEventRecord(ev[0], c.stream);
   if (c.coarse_defl) {
-    precond_apply_f32(kind, c.ssor_op32(), c.alu_diag32.as<float>(), c.N, r, z, st, c.stream);
+    precond_apply_f32(kind, c.ssor_op32(c.ssor16), c.alu_diag32.as<float>(), c.N, r, z, st, c.stream);
   } else {
     coarse_sym_pre(c.coarse, c.N, sc, aggW, AZ, r, z, r1, st, c.stream);
-    precond_apply_f32(kind, c.ssor_op32(), c.alu_diag32.as<float>(), c.N, r1, t, st, c.stream);
+    precond_apply_f32(kind, c.ssor_op32(c.ssor16), c.alu_diag32.as<float>(), c.N, r1, t, st, c.stream);
     coarse_sym_add(n3, t, z, st, c.stream);
   }
   if (ev) (void)hipEventRecord(ev[1], c.stream);
@@ -1651,8 +1675,9 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
   }
   if (int32_t e = precond_setup(c, iarray, rarray)) return e;
   const int32_t kind = pc_kind(pc, iterpremax);
+  c.ssor16 = IA(iarray, I_SOLVER_OPT7) != 1;
   if (mixed)
-    if (int32_t e = setup_f32(c, kind)) return e;
+    if (int32_t e = setup_f32(c, kind, c.ssor16)) return e;
   bool spmv64 = false;
   const int32_t o3 = learned && IA(iarray, I_SOLVER_OPT3) <= 0 ? 1 : IA(iarray, I_SOLVER_OPT3);
   if (mixed) {
@@ -2235,26 +2260,27 @@ int32_t hecmw33_ctx_matvec_device_f32(hecmw33_ctx* ctx, const float* x_dev, floa
 }
 
 // `count` back-to-back SSOR applications z = M^-1 r on device vectors with the
-// context's multicolour SSOR set-up (FP64, or its FP32 image when fp32 != 0:
-// float vectors); the mean time per application in *ms (HIP events on the
-// context's stream)
+// context's multicolour SSOR set-up: FP64 (fp32 = 0), or its FP32 image on
+// float vectors with the L'/U' values in BF16 (fp32 = 1, the flexible CG's
+// default) or FP32 (fp32 = 2); the mean time per application in *ms (HIP
+// events on the context's stream)
 int32_t hecmw33_ctx_ssor_device(hecmw33_ctx* ctx, const void* r_dev, void* z_dev, int32_t fp32, int32_t count,
                                 double* ms) {
-  if (!ctx || !ctx->c.have_struct || !r_dev || !z_dev || count < 1) return HECMW33_ERROR_ARGS;
+  if (!ctx || !ctx->c.have_struct || !r_dev || !z_dev || count < 1 || fp32 < 0 || fp32 > 2) return HECMW33_ERROR_ARGS;
   Context& c = ctx->c;
   if (!c.ssor_init) {
     set_error("hecmw33_ctx_ssor_device: the context has no SSOR set-up (load with PRECOND 1 or 2)");
     return HECMW33_ERROR_ARGS;
   }
   if (fp32)
-    if (int32_t e = setup_f32(c, 1)) return e;
+    if (int32_t e = setup_f32(c, 1, fp32 == 1)) return e;
   hipEvent_t e0, e1;
   HIPC(hipEventCreate(&e0));
   HIPC(hipEventCreate(&e1));
   HIPC(hipEventRecord(e0, c.stream));
   for (int i = 0; i < count; ++i) {
     if (fp32)
-      ssor_apply_f32(c.ssor_op32(), static_cast<const float*>(r_dev), static_cast<float*>(z_dev), nullptr, c.stream);
+      ssor_apply_f32(c.ssor_op32(fp32 == 1), static_cast<const float*>(r_dev), static_cast<float*>(z_dev), nullptr, c.stream);
     else
       ssor_apply(c.ssor_op(), static_cast<const double*>(r_dev), static_cast<double*>(z_dev), nullptr, c.stream);
   }

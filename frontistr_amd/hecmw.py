@@ -215,13 +215,16 @@ class HecMatrix:
     def set_ncolor_in(self, v): self.set_i(self.I_NCOLOR_IN, v)
     def set_resid(self, v): self.Rarray[self.R_RESID - 1] = v
 
-    def set_mixed(self, on=True, delta=None, spmv=0):
+    def set_mixed(self, on=True, delta=None, spmv=0, ssor_bf16=True):
         """Flexible CG with FP32 operators and FP64 residual replacement
         (Iarray 41 = 1, METHOD 1 only); delta: reliable-update factor on |r|
         (Rarray 41, default 1e-2); spmv (Iarray 42): 1 FP32 values in q = A p,
-        2 FP64 values with FP32 vectors, 0 chosen from the stiffness contrast."""
+        2 FP64 values with FP32 vectors, 0 chosen from the stiffness contrast;
+        ssor_bf16 (Iarray 47 = 0, else 1): the FP32 SSOR sweeps read BF16
+        copies of the L'/U' values."""
         self.set_i(self.I_SOLVER_OPT, 1 if on else 0)
         self.set_i(self.I_SOLVER_OPT + 1, spmv)
+        self.set_i(self.I_SOLVER_OPT + 6, 0 if ssor_bf16 else 1)
         self.Rarray[self.R_SOLVER_OPT - 1] = 0.0 if delta is None else float(delta)
 
     def set_coarse(self, n=1, additive=False, deflation=False):
@@ -529,10 +532,12 @@ class DeviceContext:
         _check(f(self._h, C.c_void_p(x_ptr), C.c_void_p(y_ptr), count, C.byref(ms)))
         return ms.value
 
-    def ssor_device(self, r_ptr, z_ptr, count=1, fp32=False):
-        """count SSOR applications on device vectors; mean ms per application"""
+    def ssor_device(self, r_ptr, z_ptr, count=1, fp32=False, bf16=True):
+        """count SSOR applications on device vectors; mean ms per application.
+        fp32: float vectors and the FP32 image, with BF16 (bf16) or FP32 L'/U' values"""
         ms = C.c_double(0.0)
-        _check(lib().hecmw33_ctx_ssor_device(self._h, C.c_void_p(r_ptr), C.c_void_p(z_ptr), 1 if fp32 else 0, count,
+        mode = (1 if bf16 else 2) if fp32 else 0
+        _check(lib().hecmw33_ctx_ssor_device(self._h, C.c_void_p(r_ptr), C.c_void_p(z_ptr), mode, count,
                                              C.byref(ms)))
         return ms.value
 

@@ -114,6 +114,10 @@ int32_t hecmw33_solve_krylov(const hecmw33_sys* A, const double* B, double* X, i
  *   A z + beta q, so an iteration still has one SpMV); 1 the additive
  *   z = M^-1 r + Q r; 2 the deflation form y = M^-1 r, z = y + Q (r - A y)
  *   (A-DEF2: one coarse solve per iteration, the symmetric step's spectrum).
+ *   iarray[46] (Iarray 47) picks the L'/U' values the FP32 SSOR sweeps read: 0
+ *   (default) BF16 copies rounded to nearest even from the FP32 image (22 B per
+ *   off-diagonal block instead of 40; D^-1 stays FP32, and the rounding is
+ *   entrywise, so M stays symmetric positive definite), 1 FP32.
  *   Accepted by hecmw33_solve, hecmw33_solve_krylov and hecmw33_ctx_solve_device. */
 
 /* Node coordinates for the two-level correction: xyz interleaved, 3*NP doubles
@@ -196,8 +200,9 @@ int32_t hecmw33_ctx_solve_device(hecmw33_ctx* ctx, const double* B_dev, double* 
 int32_t hecmw33_ctx_matvec_device(hecmw33_ctx* ctx, const double* x_dev, double* y_dev, int32_t count, double* ms);
 int32_t hecmw33_ctx_matvec_device_f32(hecmw33_ctx* ctx, const float* x_dev, float* y_dev, int32_t count, double* ms);
 /* `count` back-to-back multicolour SSOR applications z = M^-1 r (the context's SSOR set-up,
- * hecmw_precond_SSOR_33_apply's sweeps) on device vectors of 3*NP entries: FP64, or with
- * fp32 != 0 the FP32 image the flexible CG uses (float vectors); mean time per application in
+ * hecmw_precond_SSOR_33_apply's sweeps) on device vectors of 3*NP entries: FP64 (fp32 = 0),
+ * or the FP32 image the flexible CG uses on float vectors, with its L'/U' values in BF16
+ * (fp32 = 1, Iarray 47 = 0) or FP32 (fp32 = 2, Iarray 47 = 1); mean time per application in
  * *ms (HIP events on the context's stream). */
 int32_t hecmw33_ctx_ssor_device(hecmw33_ctx* ctx, const void* r_dev, void* z_dev, int32_t fp32, int32_t count,
                                 double* ms);

@@ -36,3 +36,6 @@ def test_ssor_bytes_counts_every_off_diagonal_block_once():
     assert bench.ssor_bytes(m.N, nnz_lu, 4) == 4735168748
     # each block of L' and U' once (values + column), plus per-row terms that do not depend on the blocks
     assert bench.ssor_bytes(m.N, nnz_lu + 1, 4) - bench.ssor_bytes(m.N, nnz_lu, 4) == 9 * 4 + 4
+    # BF16 L'/U' values: 22 B per block, the per-row terms unchanged
+    assert bench.ssor_bytes(m.N, nnz_lu, 4, 2) == 4735168748 - 18 * nnz_lu == 2807164268
+    assert bench.ssor_bytes(m.N, nnz_lu + 1, 4, 2) - bench.ssor_bytes(m.N, nnz_lu, 4, 2) == 9 * 2 + 4

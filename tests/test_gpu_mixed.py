@@ -67,6 +67,74 @@ def test_mixed_fcg_matches_reference_solution(name, precond):
     assert res.iter <= 1.25 * info["iter"] + 10, (res.iter, info["iter"])
 
 
+def _bf16(a):
+    # FP64 -> FP32 -> BF16, each rounded to nearest even (torch's host conversions)
+    import torch
+    return torch.from_numpy(np.asarray(a, np.float64)).float().to(torch.bfloat16).double().numpy()
+
+
+@pytest.mark.parametrize("name", ["hex8_10", "hex8_8_lognormal", "random_700"])
+def test_fp32_ssor_application_against_the_reference_sweeps(name):
+    """The flexible CG's FP32 SSOR application (hecmw33_ctx_ssor_device), with
+    its L'/U' values in FP32 and in BF16, against the reference's FP64 sweeps
+    (hecmw_precond_SSOR_33_apply) on the same values: within FP32 rounding of
+    the arithmetic.  BF16 storage must equal the reference's sweeps on
+    BF16-rounded off-diagonal blocks (D unchanged), and the difference the
+    device shows between its two value widths must be the one the reference
+    shows between the rounded and the unrounded blocks."""
+    torch = pytest.importorskip("torch")
+    dev = torch.device("cuda:0")
+    m = SYSTEMS[name].copy()
+    m.set_precond(1)
+    m.set_iter(2)
+    ctx = fa.DeviceContext()
+    ctx.load(m)
+    b = torch.tensor(m.B, device=dev)
+    x = torch.zeros_like(b)
+    ctx.solve_device(b.data_ptr(), x.data_ptr(), m.Iarray, m.Rarray)  # the SSOR set-up
+    r = np.random.default_rng(7).standard_normal(3 * m.NP).astype(np.float32)
+    rd = torch.from_numpy(r).to(dev)
+    z = torch.zeros_like(rd)
+    n3 = 3 * m.N
+    out = {}
+    for bf16 in (False, True):
+        z.zero_()
+        torch.cuda.synchronize()
+        ctx.ssor_device(rd.data_ptr(), z.data_ptr(), 1, fp32=True, bf16=bf16)
+        out[bf16] = z.cpu().numpy()[:n3].astype(np.float64)
+    ctx.close()
+    r64 = r.astype(np.float64)
+    zr = ref.precond(m, r64, nthreads=2)
+    m16 = m.copy()
+    m16.AL[:] = _bf16(m.AL)
+    m16.AU[:] = _bf16(m.AU)
+    z16r = ref.precond(m16, r64, nthreads=2)
+    nrm = np.linalg.norm
+    tol = 1e-5  # FP32 sweeps (FP32 D^-1, FP32 vectors) against FP64 ones
+    assert nrm(out[False] - zr) <= tol * nrm(zr), nrm(out[False] - zr) / nrm(zr)
+    assert nrm(out[True] - z16r) <= tol * nrm(z16r), nrm(out[True] - z16r) / nrm(z16r)
+    gap = nrm(z16r - zr)
+    if gap > 10 * tol * nrm(zr):  # random_700: the rounding of the couplings shows through
+        assert nrm((out[True] - out[False]) - (z16r - zr)) <= 0.1 * gap
+
+
+@pytest.mark.parametrize("name", ["hex8_8_lognormal", "random_700"])
+def test_mixed_fcg_with_fp32_ssor_values(name):
+    # Iarray 47 = 1: the FP32 sweeps read FP32 L'/U' values; same bar as the default
+    m = _mixed(SYSTEMS[name], 1)
+    m.set_mixed(True, ssor_bf16=False)
+    assert m.Iarray[46] == 1
+    res = fa.hecmw_solve_krylov(m)
+    r64 = m.copy()
+    r64.set_mixed(False)
+    xr, info = ref.solve_krylov(r64, nthreads=2)
+    assert res.error == 0 and res.resid <= 1e-8
+    assert ref.rel_resid(m, m.X) <= 1e-8 * 1.0001
+    n3 = 3 * m.N
+    assert np.linalg.norm(m.X[:n3] - xr[:n3]) <= 1e-6 * np.linalg.norm(xr[:n3])
+    assert res.iter <= 1.25 * info["iter"] + 10, (res.iter, info["iter"])
+
+
 def test_mixed_history_ends_on_true_residual():
     m = _mixed(fa.gen_hex8(10), 1)
     m.set_i(m.I_ITERLOG, 1)
