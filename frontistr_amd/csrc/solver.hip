@@ -287,6 +287,7 @@ struct Context {
   int32_t last_coarse = 0;       // aggregates used by the last solve (-1: E not positive definite)
   CoarseOp coarse;
   int32_t coarse_npair = 0;       // coupled aggregate pairs (blocks of E)
+  int32_t coarse_band_agg = 0;    // max |a - b| over the coupled pairs: E's bandwidth in aggregates
   bool coarse_sym_use = false;    // this solve takes the symmetric two-level step
   bool coarse_defl = false;       // ... in its deflation form (no pre-smoothing coarse solve)
   bool coarse_w_valid = false;    // A Z (per node and touched aggregate) built for the current values
@@ -1077,6 +1078,8 @@ static int32_t coarse_symbolic(Context& c, int32_t target) {
   op.nc = 0;
   op.ldw = 0;
   c.coarse_npair = npair;
+  c.coarse_band_agg = 0;
+  for (int32_t p = 0; p < npair; ++p) c.coarse_band_agg = std::max(c.coarse_band_agg, std::abs(pa[p] - pb[p]));
   c.coarse_nw = ncnt[N];
   if (c.cg_ncnt.ensure(sizeof(int32_t) * (N + 1)) || c.cg_nagg.ensure(sizeof(int32_t) * std::max<int64_t>(ncnt[N], 1)))
     return HECMW33_ERROR_DEVICE;
@@ -1087,8 +1090,7 @@ static int32_t coarse_symbolic(Context& c, int32_t target) {
   if (c.cg_agg.ensure(sizeof(int32_t) * NP) || c.cg_aptr.ensure(sizeof(int32_t) * (na + 1)) ||
       c.cg_anodes.ensure(sizeof(int32_t) * N) || c.cg_dv.ensure(sizeof(float) * 3 * (size_t)N) ||
       c.cg_pa.ensure(sizeof(int32_t) * npair) || c.cg_pb.ensure(sizeof(int32_t) * npair) ||
-      c.cg_pnptr.ensure(sizeof(int32_t) * (npair + 1)) || c.cg_pn.ensure(sizeof(int32_t) * std::max<int64_t>(npn, 1)) ||
-      c.cg_info.ensure(sizeof(int32_t) * 2))
+      c.cg_pnptr.ensure(sizeof(int32_t) * (npair + 1)) || c.cg_pn.ensure(sizeof(int32_t) * std::max<int64_t>(npn, 1)))
     return HECMW33_ERROR_DEVICE;
   HIPC(hipMemcpyAsync(c.cg_agg.p, agg.data(), sizeof(int32_t) * NP, hipMemcpyHostToDevice, c.stream));
   HIPC(hipMemcpyAsync(c.cg_aptr.p, cnt.data(), sizeof(int32_t) * (na + 1), hipMemcpyHostToDevice, c.stream));
@@ -1113,8 +1115,37 @@ static int32_t coarse_symbolic(Context& c, int32_t target) {
 
 // E = Z^T A Z on the device with nm modes per aggregate (6 rigid-body; 12 with
 // the learned ones in cg_xm), Cholesky factor and inverse (six modes) or
-// inverse factor (twelve) by rocSOLVER in FP64, then the copy the iterations
+// inverse factor (twelve) in FP64 (chol_band, then rocSOLVER), then the copy the iterations
 // read
+// Cholesky E = L L^T of the coarse matrix (lower, column-major, n x n), which
+// is banded: aggregates couple only when their cells touch, and the cells are
+// numbered in order of first appearance over the nodes, so on a mesh numbered
+// along its axes every nonzero lies within `band` rows below the diagonal (at
+// 160^3 with 1000 aggregates: 1332 of 12,000 rows), and so does every nonzero
+// of L.  Right-looking over panels of kCholPanel columns: rocSOLVER factors the
+// panel's diagonal block, a triangular solve gives the at most `band` rows
+// below it, and a rank-kb product updates the trailing band; the zeros outside
+// the band are never touched.  info[p] != 0: panel p's block is not positive
+// definite.  Same operations as the dense right-looking factorization, minus
+// the ones on zeros.
+constexpr int32_t kCholPanel = 256;
+static int32_t chol_band(rocblas_handle h, int32_t n, double* A, int32_t band, int32_t* info) {
+  const double one = 1.0, mone = -1.0;
+  for (int32_t k = 0, p = 0; k < n; k += kCholPanel, ++p) {
+    const int32_t kb = std::min(kCholPanel, n - k);
+    double* Akk = A + (size_t)k * n + k;
+    if (rocsolver_dpotrf(h, rocblas_fill_lower, kb, Akk, n, info + p) != rocblas_status_success) return 1;
+    const int32_t m = std::min(n - k - kb, band);
+    if (m <= 0) continue;
+    if (rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                      rocblas_diagonal_non_unit, m, kb, &one, Akk, n, Akk + kb, n) != rocblas_status_success ||
+        rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, m, kb, &mone, Akk + kb, n, &one,
+                      A + (size_t)(k + kb) * n + k + kb, n) != rocblas_status_success)
+      return 1;
+  }
+  return 0;
+}
+
 static int32_t coarse_numeric(Context& c, int32_t nm) {
   CoarseOp& op = c.coarse;
   op.nm = nm;
@@ -1124,7 +1155,9 @@ static int32_t coarse_numeric(Context& c, int32_t nm) {
   const int32_t nc = op.nc;
   // six modes: E^-1, c and y in FP32; twelve: in FP64
   const size_t we = nm == 12 ? sizeof(double) : sizeof(float);
-  if (c.cg_Eblk.ensure(sizeof(double) * nm * nm * (size_t)std::max(c.coarse_npair, 1)) ||
+  const int32_t npanel = (nc + kCholPanel - 1) / kCholPanel;
+  if (c.cg_info.ensure(sizeof(int32_t) * (npanel + 1)) ||
+      c.cg_Eblk.ensure(sizeof(double) * nm * nm * (size_t)std::max(c.coarse_npair, 1)) ||
       c.cg_E.ensure(sizeof(double) * (size_t)nc * nc) || c.cg_W.ensure(we * (size_t)nc * op.ldw) ||
       c.cg_cv.ensure(we * op.ldw) || c.cg_yv.ensure(we * op.ldw) || c.cg_tv.ensure(we * op.ldw))
     return HECMW33_ERROR_DEVICE;
@@ -1156,18 +1189,19 @@ static int32_t coarse_numeric(Context& c, int32_t nm) {
   // same bytes read without forming L^-T L^-1 (at 12,000 coarse unknowns
   // potri takes 57 ms, trtri 22 ms)
   int32_t* info = c.cg_info.as<int32_t>();
-  if (rocsolver_dpotrf(c.blas, rocblas_fill_lower, nc, c.cg_E.as<double>(), nc, info) != rocblas_status_success ||
+  const int32_t band = std::max(0, std::min(nc - 1, nm * (c.coarse_band_agg + 1) - 1));
+  if (chol_band(c.blas, nc, c.cg_E.as<double>(), band, info) ||
       (nm == 12 ? rocsolver_dtrtri(c.blas, rocblas_fill_lower, rocblas_diagonal_non_unit, nc, c.cg_E.as<double>(), nc,
-                                   info + 1)
-                : rocsolver_dpotri(c.blas, rocblas_fill_lower, nc, c.cg_E.as<double>(), nc, info + 1)) !=
+                                   info + npanel)
+                : rocsolver_dpotri(c.blas, rocblas_fill_lower, nc, c.cg_E.as<double>(), nc, info + npanel)) !=
           rocblas_status_success) {
-    set_error("rocsolver potrf/potri/trtri failed");
+    set_error("coarse factorization (rocSOLVER potrf/potri/trtri, rocBLAS trsm/syrk) failed");
     return HECMW33_ERROR_DEVICE;
   }
-  int32_t hinfo[2] = {0, 0};
-  HIPC(hipMemcpyAsync(hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, c.stream));
+  std::vector<int32_t> hinfo(npanel + 1, 0);
+  HIPC(hipMemcpyAsync(hinfo.data(), info, sizeof(int32_t) * (npanel + 1), hipMemcpyDeviceToHost, c.stream));
   HIPC(hipStreamSynchronize(c.stream));
-  c.coarse_ok = hinfo[0] == 0 && hinfo[1] == 0;
+  c.coarse_ok = std::all_of(hinfo.begin(), hinfo.end(), [](int32_t v) { return v == 0; });
   c.coarse_w_valid = false;
   if (c.coarse_ok) {
     if (nm == 12)
