@@ -2766,6 +2766,25 @@ int32_t hecmw33_ctx_coarse_get(hecmw33_ctx* ctx, int32_t* n_agg, int32_t* n_pair
   return 0;
 }
 
+int32_t hecmw33_ctx_coarse_inverse_get(hecmw33_ctx* ctx, int32_t* nc, int32_t* ldw, int32_t* fp64, void* W) {
+  if (!ctx || !nc || !ldw || !fp64) return HECMW33_ERROR_ARGS;
+  Context& c = ctx->c;
+  if (!c.coarse_sym || !c.coarse_num || !c.coarse_ok) {
+    set_error("no coarse operator set up for the current values");
+    return HECMW33_ERROR_ARGS;
+  }
+  const CoarseOp& op = c.coarse;
+  *nc = op.nc;
+  *ldw = op.ldw;
+  *fp64 = op.nm == 12 ? 1 : 0;
+  if (W) {
+    HIPC(hipStreamSynchronize(c.stream));
+    const size_t bytes = (op.nm == 12 ? sizeof(double) : sizeof(float)) * (size_t)op.nc * op.ldw;
+    HIPC(hipMemcpy(W, op.nm == 12 ? (const void*)op.W64 : (const void*)op.W, bytes, hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
 int32_t hecmw33_ctx_modes_get(hecmw33_ctx* ctx, float* xm, double* Eblk, float* R) {
   if (!ctx) return HECMW33_ERROR_ARGS;
   Context& c = ctx->c;

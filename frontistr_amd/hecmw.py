@@ -118,6 +118,7 @@ def lib():
     L.hecmw33_ctx_set_coords.argtypes = [P, I, P]
     L.hecmw33_ctx_coarse_get.argtypes = [P, P, P, P, P, P, P, P]
     L.hecmw33_ctx_modes_get.argtypes = [P, P, P, P]
+    L.hecmw33_ctx_coarse_inverse_get.argtypes = [P, P, P, P, P]
     L.hecmw33_last_error.restype = C.c_char_p
     for f in ("hecmw33_solve", "hecmw33_solve_krylov", "hecmw33_matvec", "hecmw33_precond_apply",
               "hecmw33_rel_resid_L2", "hecmw33_ordering", "hecmw33_ctx_load", "hecmw33_ctx_solve_device",
@@ -128,8 +129,9 @@ def lib():
               "hecmw33_batch_spmm", "hecmw33_batch_info", "hecmw33_solve_batch", "hecmw33_net_weight_count",
               "hecmw33_net_get_weights", "hecmw33_net_set_weights", "hecmw33_net_eval", "hecmw33_net_add_sample",
               "hecmw33_net_clear_samples", "hecmw33_net_train", "hecmw33_net_score", "hecmw33_ctx_set_net",
-              "hecmw33_set_net", "hecmw33_set_coords", "hecmw33_ctx_set_coords", "hecmw33_ctx_coarse_get", "hecmw33_set_mpc",
-              "hecmw33_ctx_set_mpc", "hecmw33_matvec_mpc", "hecmw33_rigid_labels", "hecmw33_ctx_ssor_device"):
+              "hecmw33_set_net", "hecmw33_set_coords", "hecmw33_ctx_set_coords", "hecmw33_ctx_coarse_get",
+              "hecmw33_ctx_coarse_inverse_get", "hecmw33_set_mpc", "hecmw33_ctx_set_mpc", "hecmw33_matvec_mpc",
+              "hecmw33_rigid_labels", "hecmw33_ctx_ssor_device"):
         getattr(L, f).restype = C.c_int32
     _lib = L
     return L
@@ -575,6 +577,16 @@ class DeviceContext:
                                             _ptr(pb), _ptr(E) if with_E else None))
         return {"n_agg": na, "n_pair": npair, "agg": agg, "dv": dv.reshape(N, 3), "pair_a": pa, "pair_b": pb,
                 "Eblk": E.reshape(npair, 6, 6) if with_E else None}
+
+    def coarse_inverse(self):
+        """The coarse operator the iterations read: (W [nc, nc], fp64).  fp64 False
+        (six modes per aggregate): W = E^-1 in FP32; True (twelve): W holds L^-1 on
+        and below the diagonal and L^-T above it, E = L L^T."""
+        nc, ldw, f64 = C.c_int32(0), C.c_int32(0), C.c_int32(0)
+        _check(lib().hecmw33_ctx_coarse_inverse_get(self._h, C.byref(nc), C.byref(ldw), C.byref(f64), None))
+        W = np.zeros(nc.value * ldw.value, np.float64 if f64.value else np.float32)
+        _check(lib().hecmw33_ctx_coarse_inverse_get(self._h, C.byref(nc), C.byref(ldw), C.byref(f64), _ptr(W)))
+        return W.reshape(nc.value, ldw.value)[:, :nc.value], bool(f64.value)
 
     def modes(self, with_E=True):
         """The learned coarse modes of the last fit: dict(xm [N,6,3], Eblk [n_pair,12,12] or None,

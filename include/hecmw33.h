@@ -305,6 +305,12 @@ int32_t hecmw33_solve_batch(int32_t nsys, const hecmw33_sys* sys, int32_t nrhs, 
  * their aggregate's rigid-body modes that the fit used, entry 16 (3 i + d) + s for node i,
  * component d, sample s; unused samples zero).  Any may be NULL. */
 int32_t hecmw33_ctx_modes_get(hecmw33_ctx* ctx, float* xm, double* Eblk, float* R);
+/* The coarse solve operator the iterations of a device context's last two-level set-up read (no
+ * counterpart in the reference; for tests): *nc coarse unknowns, *ldw the row stride of W,
+ * *fp64 = 0 with six modes per aggregate (W: E^-1 in FP32, nc rows of ldw floats), 1 with twelve
+ * (W: FP64, L^-1 on and below the diagonal and L^-T above it, E = L L^T); columns nc..ldw-1 zero.
+ * W may be NULL to query the sizes. */
+int32_t hecmw33_ctx_coarse_inverse_get(hecmw33_ctx* ctx, int32_t* nc, int32_t* ldw, int32_t* fp64, void* W);
 
 typedef struct hecmw33_net hecmw33_net;
 hecmw33_net* hecmw33_net_create(uint64_t seed);

@@ -105,6 +105,34 @@ def test_coarse_matrix_is_ZtAZ(case):
     assert inf["coarse_pairs"] == len(pa)
 
 
+def test_coarse_inverse_is_the_dense_inverse_of_E():
+    """The banded panel Cholesky of E and the inverse built from it (what the
+    iterations read) against numpy's dense inverse of the same E, on a cube
+    whose E spans several 256-column panels with a band narrower than E (125
+    aggregates in a 5x5x5 grid of cells: 750 unknowns, bandwidth 186): a
+    dropped trailing update or a band cut too short leaves entries of E^-1
+    wrong by far more than FP32 rounding."""
+    m = fa.gen_hex8(12, kind="lognormal", seed=4)
+    xyz = fa.hex8_coords(12)
+    ctx, r, _ = _ctx_solve(m, xyz, 125)
+    assert r.error == 0
+    cs = ctx.coarse()
+    W, f64 = ctx.coarse_inverse()
+    ctx.close()
+    na, pa, pb, Eb = cs["n_agg"], cs["pair_a"], cs["pair_b"], cs["Eblk"]
+    nc = 6 * na
+    assert not f64 and W.shape == (nc, nc) and nc > 2 * 256
+    band = 6 * (np.abs(pa - pb).max() + 1) - 1
+    assert band < nc - 256
+    E = np.zeros((nc, nc))
+    for p in range(len(pa)):
+        E[6 * pa[p]:6 * pa[p] + 6, 6 * pb[p]:6 * pb[p] + 6] = Eb[p]
+    Ei = np.linalg.inv(E)
+    err = np.abs(W.astype(np.float64) - Ei).max() / np.abs(Ei).max()
+    assert err <= 1e-6, err
+    assert np.array_equal(W, W.T)
+
+
 @pytest.mark.parametrize("form", ["symmetric", "additive", "deflation"])
 @pytest.mark.parametrize("case", ["hex8_lognormal", "hex8_inclusions", "tet10_box"])
 def test_mixed_coarse_matches_reference(case, form):

@@ -238,7 +238,21 @@ def test_learned_modes_and_their_coarse_matrix():
         Eab = E[12 * a:12 * a + 12, 12 * bb:12 * bb + 12]
         scale = np.abs(E[12 * a:12 * a + 12, 12 * a:12 * a + 12]).max()
         assert np.abs(md["Eblk"][p] - Eab).max() <= 1e-9 * scale, (p, a, bb)
+    # the coarse solve operator: L^-1 of the GPU's own E = L L^T, mirrored (L^-T above the
+    # diagonal); y = L^-T L^-1 c solves E y = c to a backward error of FP64 rounding
+    W, f64 = ctx.coarse_inverse()
     ctx.close()
+    nc = 12 * co["n_agg"]
+    assert f64 and W.shape == (nc, nc)
+    assert np.array_equal(np.triu(W, 1), np.tril(W, -1).T)
+    Eg = np.zeros((nc, nc))
+    for p, (a, bb) in enumerate(zip(co["pair_a"], co["pair_b"])):
+        Eg[12 * a:12 * a + 12, 12 * bb:12 * bb + 12] = md["Eblk"][p]
+    Li = np.tril(W)
+    c = np.random.default_rng(3).standard_normal(nc)
+    y = Li.T @ (Li @ c)
+    berr = np.linalg.norm(Eg @ y - c) / (np.linalg.norm(Eg, 2) * np.linalg.norm(y))
+    assert berr <= 1e-12, berr
 
 
 def test_learned_solve_refuses_the_additive_step_and_missing_coordinates():
