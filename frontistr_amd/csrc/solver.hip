@@ -147,6 +147,8 @@ struct SellHost {
 };
 
 struct SellDev {
+  // val: FP64 values; val32: their FP32 image; val16: BF16 copies of the FP32
+  // image (the L'/U' values the flexible CG's SSOR sweeps read by default)
   DMem ptr, len, col, src, val, val32, val16;
   Sell<double> view;
   Sell<float> view32;
