@@ -263,6 +263,7 @@ struct Context {
   // solver's own slow errors; cg_xm holds the fitted modes ([N][6][3] FP32), valid for the
   // current values and aggregation
   bool modes_valid = false;
+  bool x_zero = false;  // the next krylov_core starts from x = 0 (its r0 = b needs no product)
   int32_t modes_kept = 0;          // learned modes with a usable eigenvalue, over all aggregates
   int32_t modes_samples = 0, modes_iters = 0;  // error samples and FCG iterations per sample of the last fit
   double modes_harvest_s = 0.0, modes_fit_s = 0.0;
@@ -1591,7 +1592,9 @@ static int32_t learned_modes(Context& c, const int32_t* iarray, const double* ra
     HIPC(hipMemsetAsync(c.vx.p, 0, vbytes, c.stream));
     hecmw33_out o{};
     std::vector<int32_t> ias(ia);
+    c.x_zero = true;
     e = krylov_core(c, ias.data(), ra.data(), &o);
+    c.x_zero = false;
     // a failed sample solve leaves the modes unfitted (the caller keeps the rigid-body modes)
     if (e || (o.error != 0 && o.error != HECMW33_ERROR_NOCONV_MAXIT)) break;
     modes_sample(n3, c.ml_v.as<double>(), c.vx.as<double>(), c.ml_S.as<float>(), s, c.stream);
@@ -1785,7 +1788,11 @@ static int32_t krylov_core(Context& c, int32_t* iarray, double* rarray, hecmw33_
       if (m->ensure(vbytes)) return HECMW33_ERROR_DEVICE;
   }
   // r0 = b - A x0; bnrm2 = b.b  (hecmw_solver_CG.f90:131-141)
-  if (mixed) {
+  if (mixed && c.x_zero) {
+    // x0 = 0 (the learned modes' sample solves): r0 = b exactly, without the FP64 product
+    launch_to_f32(n3, c.vb.as<double>(), c.vr32.as<float>(), c.stream);
+    launch_dot_store(n3, c.vb.as<double>(), c.vb.as<double>(), &st->rr_ref, st, part, c.stream);
+  } else if (mixed) {
     fcg_init_resid(c.A.view, c.vx.as<double>(), c.vb.as<double>(), c.vr32.as<float>(), &st->rr_ref, st, part,
                    c.stream);
   } else {
