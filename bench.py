@@ -6,8 +6,8 @@ Headline workload (the `value`): the 160^3 HEX8 cube with per-element E
 log-uniform in [1e3, 1e6] and nu uniform in [0.2, 0.4] (seed 0), 12,519,843
 DOF, 111,284,641 3x3 blocks, assembled like fistr1 (gen_hex8.cpp).  One step
 = one complete solve x0 = 0 -> FP64 true relative residual 1e-8 with the
-flexible CG (FP32 SpMV/SSOR/vectors, FP64 inner products, x and residual
-replacement) whose two-level step (deflation form, Iarray 44 = 2: y = M^-1 r,
+flexible CG (FP32 SpMV/SSOR/vectors, the SSOR's L'/U' values stored in BF16,
+FP64 inner products, x and residual replacement) whose two-level step (deflation form, Iarray 44 = 2: y = M^-1 r,
 z = y + Q (r - A y)) carries, beside each aggregate's six rigid-body modes, six
 learned modes (Iarray 41 = 2, 43 = 1): the decoders of per-aggregate linear
 autoencoders fitted to slow errors the solver generates on the system (8
