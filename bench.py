@@ -11,7 +11,7 @@ FP64 inner products, x and residual replacement) whose two-level step (deflation
 z = y + Q (r - A y)) carries, beside each aggregate's six rigid-body modes, six
 learned modes (Iarray 41 = 2, 43 = 1): the decoders of per-aggregate linear
 autoencoders fitted to slow errors the solver generates on the system (8
-seeded samples x 10 FCG iterations), matrix and vectors resident in HBM, the
+seeded samples x 8 FCG iterations), matrix and vectors resident in HBM, the
 numeric set-up (SSOR factors, the harvest and fit of the learned modes, the
 coarse matrix) inside the step.
 

@@ -1531,13 +1531,15 @@ static void jacobi_eigen(int n, double* a, double* lam, double* v) {
 }
 
 // Default sample count and FCG iterations per sample of the learned coarse
-// modes (Iarray 45 and 46 override them).  8 x 10: on the 160^3 lognormal
-// cube 107 iterations in 0.81 s against 100 in 1.01 s with 8 x 20; on the
-// 120^3 inclusion cube 2081 iterations in 3.89 s against 2022 in 3.90 s;
-// fewer samples lose more iterations than the harvest saves (6 x 10 on the
-// inclusion cube: 2201, 4 x 20: 2453).
+// modes (Iarray 45 and 46 override them), deflation form.  8 x 8: on the
+// 160^3 lognormal cube 108 iterations in 0.64 s against 107 in 0.68 s with
+// 8 x 10 and 100 in 1.01 s with 8 x 20; on the 120^3 inclusion cube 2111
+// iterations in 3.69 s against 2085 in 3.67 s with 8 x 10; on the small
+// two-phase cubes the gain holds (24^3: 62 against 57, 32^3: 369 against
+// 364), while 8 x 6 loses it (105, 452) and fewer samples lose more
+// iterations than the harvest saves (6 x 10 on the inclusion cube: 2201).
 constexpr int32_t I_SOLVER_OPT5 = 45, I_SOLVER_OPT6 = 46;
-constexpr int32_t kModeSamples = 8, kModeIters = 10;
+constexpr int32_t kModeSamples = 8, kModeIters = 8;
 // default growth of the FP64 |r| over its smallest value at a replacement that drops the learned step
 constexpr double kLearnedDropGrowth = 1000.0;
 constexpr int32_t kModesPerAgg = 6;

@@ -281,7 +281,7 @@ int32_t hecmw33_solve_batch(int32_t nsys, const hecmw33_sys* sys, int32_t nrhs, 
  * step: Iarray 43 (1 when not set) > 0, Iarray 44 = 0 or 2) each aggregate gets six learned coarse
  * modes beside its six rigid-body modes.  They are the decoders of per-aggregate linear
  * autoencoders fitted to slow errors the solver generates on the system itself: for seeded random
- * v_s (Iarray 45 samples, default 8, at most 16), b_s = A v_s and Iarray 46 iterations (default 10)
+ * v_s (Iarray 45 samples, default 8, at most 16), b_s = A v_s and Iarray 46 iterations (default 8)
  * of the flexible CG with the rigid-body two-level step from x = 0, e_s = v_s - x_s = A^-1 r_s;
  * per aggregate the part of the e_s beyond the rigid-body modes, its Gram matrix (BF16 MFMA, FP32
  * accumulation) and that matrix's top eigenvectors give the autoencoder's optimum.  The fit is
